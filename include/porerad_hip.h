@@ -1,0 +1,151 @@
+/*
+ * porerad_hip.h -- C ABI of the MI355X (gfx950) radiation hot path of porerad.
+ *
+ * Plain C types only (pointers, sizes, status codes); no C++ or torch types
+ * cross this boundary.  Every entry point returns PRH_OK (0) or a status code
+ * and leaves a message for prh_last_error().  Host pointers are caller-owned
+ * and never retained; device memory is owned by the library behind the opaque
+ * handles and freed by the matching *_destroy call.  Calls are synchronous
+ * with respect to the host unless stated otherwise.
+ *
+ * Each entry point replaces one function of the reference Python package
+ * (paths relative to the reference repository root):
+ *
+ *   prh_pcg64_seed            numpy PCG64(seed) seeding used by
+ *                             generate_rays, porerad/view_factors.py:149-150
+ *   prh_geometry_create       build_surface_patches, porerad/view_factors.py:70-132
+ *   prh_geometry_patch_table  the SurfacePatches arrays, porerad/view_factors.py:43-67
+ *   prh_generate_rays         generate_rays, porerad/view_factors.py:135-182
+ *   prh_trace_rays            trace_first_hit / _trace_kernel,
+ *                             porerad/view_factors.py:185-270
+ *   prh_trace_counts          first_hit_counts (+ pool_counts over several
+ *                             seeds), porerad/view_factors.py:273-300
+ *   prh_operator_create       ExchangeOperator.__init__ assembly,
+ *                             porerad/exchange.py:47-76
+ *   prh_operator_apply        raw_matvec / raw_rmatvec / symmetric_matvec /
+ *                             matvec / rmatvec, porerad/exchange.py:82-122
+ *   prh_radiosity             reflection_series, porerad/exchange.py:217-245
+ */
+#ifndef PORERAD_HIP_H
+#define PORERAD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define PRH_OK 0
+#define PRH_ERR_ARG 1        /* invalid argument or size */
+#define PRH_ERR_HIP 2        /* HIP runtime failure */
+#define PRH_ERR_UNRESOLVED 3 /* rays did not resolve a receiving patch
+                                (RuntimeError, porerad/view_factors.py:268-269) */
+#define PRH_ERR_COUNTS 4     /* first-hit counts do not account for every ray
+                                (ValueError, porerad/exchange.py:70-71) */
+#define PRH_ERR_SERIES 5     /* reflection series did not converge
+                                (RuntimeError, porerad/exchange.py:243-244) */
+#define PRH_ERR_CAPACITY 6   /* request exceeds a documented capacity */
+
+/* ---- operator application modes (prh_operator_apply) ----------------- */
+#define PRH_APPLY_SYMMETRIC 0 /* S x              (symmetric_matvec) */
+#define PRH_APPLY_MATVEC 1    /* F_p x            (matvec)           */
+#define PRH_APPLY_RMATVEC 2   /* F_p^T x          (rmatvec)          */
+#define PRH_APPLY_RAW 3       /* F x              (raw_matvec)       */
+#define PRH_APPLY_RAW_T 4     /* F^T x            (raw_rmatvec)      */
+
+/* ---- radiosity precisions (prh_radiosity) ---------------------------- */
+#define PRH_FP64 0 /* plain FP64 reflection series */
+
+typedef struct prh_geometry prh_geometry;
+typedef struct prh_edges prh_edges;
+typedef struct prh_operator prh_operator;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *prh_last_error(void);
+
+/* Library version string. */
+const char *prh_version(void);
+
+/* Kernel durations (ms, HIP events on the library stream) of the last
+ * prh_trace_counts call: [0] fused generate+trace kernel, [1] per-patch
+ * count kernel, [2] edge compaction.  n = number of slots to fill. */
+int prh_last_timings(double *ms, int32_t n);
+
+/* numpy.random.PCG64(seed) seeding: SeedSequence over the little-endian
+ * uint32 words of the seed.  state_inc = {state_hi, state_lo, inc_hi, inc_lo}.
+ * Replaces np.random.PCG64(int(seed)), porerad/view_factors.py:149. */
+int prh_pcg64_seed(const uint32_t *words, int32_t n_words, uint64_t state_inc[4]);
+
+/* Upload an x-oriented solid/pore grid (C order, nonzero = solid, i.e. the
+ * phases == 1 mask) and build the patch table on the device: wall patches
+ * first in (j, k, hot/cold) order, then exposed solid faces in (i, j, k, face)
+ * order with periodic y/z.  Replaces build_surface_patches,
+ * porerad/view_factors.py:70-132. */
+int prh_geometry_create(const uint8_t *solid_x, int32_t nx, int32_t ny, int32_t nz, double H_m,
+                        prh_geometry **out);
+int prh_geometry_sizes(const prh_geometry *g, int64_t *n_patches, int64_t *n_wall);
+/* Download the patch table (arrays of n_patches; origin is n_patches x 3).
+ * Any pointer may be NULL.  Replaces the SurfacePatches fields,
+ * porerad/view_factors.py:43-67. */
+int prh_geometry_patch_table(const prh_geometry *g, int8_t *patch_class, int32_t *patch_cell,
+                             int8_t *patch_face, double *patch_origin);
+void prh_geometry_destroy(prh_geometry *g);
+
+/* Origins and directions ((p1-p0)*rays_per_patch x 3, FP64) of the seeded
+ * Lambertian rays of patches [p0, p1).  Replaces generate_rays,
+ * porerad/view_factors.py:135-182. */
+int prh_generate_rays(const prh_geometry *g, int64_t p0, int64_t p1, int32_t rays_per_patch,
+                      const uint64_t state_inc[4], double *origins, double *directions);
+
+/* First patch hit by each given ray (receiver = -1 where unresolved).
+ * Returns PRH_ERR_UNRESOLVED with *n_failed > 0 if any ray failed.
+ * Replaces trace_first_hit, porerad/view_factors.py:261-270. */
+int prh_trace_rays(const prh_geometry *g, const double *origins, const double *directions, int64_t n_rays,
+                   int32_t *receiver, int64_t *n_failed);
+
+/* Fused ray generation + FP64 DDA + per-patch counting for patches
+ * [p0, p1) and n_seeds seeded ray blocks (state_inc: n_seeds x 4 words), all
+ * pooled into one device-resident edge table sorted by (origin, receiver).
+ * rays_per_patch * n_seeds <= 16384.  *occupancy_reads (may be NULL) receives
+ * the number of occupancy reads of the traversal (the exact count of
+ * solid[i, j, k] loads the reference kernel makes).
+ * Replaces first_hit_counts + pool_counts, porerad/view_factors.py:273-300. */
+int prh_trace_counts(const prh_geometry *g, int64_t p0, int64_t p1, int32_t rays_per_patch, int32_t n_seeds,
+                     const uint64_t *state_inc, prh_edges **out, int64_t *nnz, int64_t *occupancy_reads);
+/* Copy a device edge table to host arrays of nnz entries. */
+int prh_edges_copy(const prh_edges *e, int32_t *origin, int32_t *receiver, int64_t *count);
+void prh_edges_destroy(prh_edges *e);
+
+/* Assemble the symmetric exchange matrix
+ *     S = 0.5 (diag(A) F + F^T diag(A)) + delta diag(A),  F_ij = n_ij / R_i
+ * on the device from an edge table sorted by (origin, receiver) with unique
+ * pairs whose rows account for every ray.  Replaces the assembly in
+ * ExchangeOperator.__init__, porerad/exchange.py:59-76. */
+int prh_operator_create(int64_t n, const int32_t *origin, const int32_t *receiver, const int64_t *count,
+                        int64_t nnz, const int64_t *rays_per_origin, const double *area,
+                        double support_diagonal_fraction, prh_operator **out);
+/* Diagonal balancing factor d (n), used by modes MATVEC/RMATVEC and by
+ * prh_radiosity.  Defaults to ones. */
+int prh_operator_set_scale(prh_operator *op, const double *scale);
+int prh_operator_nnz(const prh_operator *op, int64_t *nnz_symmetric, int64_t *nnz_raw);
+/* y = op(x) for one of the PRH_APPLY_* modes; x, y host arrays of n. */
+int prh_operator_apply(prh_operator *op, int32_t mode, const double *x, double *y);
+
+/* Reflection series J = E + (1 - eps) F_p^T J for n_rhs emissions.
+ * emitted: n_rhs x n (row-major, one emission per row); emissivity:
+ * n_rhs x n if emissivity_per_rhs else n.  Outputs absorbed (n_rhs x n),
+ * residual[n_rhs] (power still in flight), iterations[n_rhs].  Stops each
+ * emission when ||flight||_1 <= relative_tolerance * ||E||_1.  Returns
+ * PRH_ERR_SERIES if an emission needs more than max_reflections.
+ * Replaces reflection_series, porerad/exchange.py:217-245. */
+int prh_radiosity(prh_operator *op, int32_t n_rhs, const double *emitted, const double *emissivity,
+                  int32_t emissivity_per_rhs, double relative_tolerance, int32_t max_reflections,
+                  int32_t precision, double *absorbed, double *residual, int32_t *iterations);
+void prh_operator_destroy(prh_operator *op);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PORERAD_HIP_H */

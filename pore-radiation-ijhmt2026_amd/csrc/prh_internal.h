@@ -1,0 +1,109 @@
+// Internal definitions shared by the porerad HIP library sources.
+// Target: gfx950 (MI355X), wave64.  All FP64 arithmetic on the tracer path is
+// compiled with -ffp-contract=off so every product and sum rounds on its own,
+// exactly as NumPy and Numba evaluate porerad/view_factors.py.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "porerad_hip.h"
+
+namespace prh {
+
+// ------------------------------------------------------------------ errors
+void set_error(const char *fmt, ...);
+
+#define PRH_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            ::prh::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),        \
+                             __FILE__, __LINE__);                                          \
+            return PRH_ERR_HIP;                                                            \
+        }                                                                                  \
+    } while (0)
+
+#define PRH_CHECK_ARG(cond, ...)                                                           \
+    do {                                                                                   \
+        if (!(cond)) {                                                                     \
+            ::prh::set_error(__VA_ARGS__);                                                 \
+            return PRH_ERR_ARG;                                                            \
+        }                                                                                  \
+    } while (0)
+
+// ------------------------------------------------------------------ 128-bit LCG
+// PCG64 (XSL-RR 128/64) state arithmetic on two 64-bit halves.
+struct u128 {
+    uint64_t lo, hi;
+};
+
+__host__ __device__ inline u128 make_u128(uint64_t hi, uint64_t lo) { return u128{lo, hi}; }
+
+__device__ inline u128 mul128(u128 a, u128 b)
+{
+    u128 r;
+    r.lo = a.lo * b.lo;
+    r.hi = __umul64hi(a.lo, b.lo) + a.lo * b.hi + a.hi * b.lo;
+    return r;
+}
+
+__device__ inline u128 add128(u128 a, u128 b)
+{
+    u128 r;
+    r.lo = a.lo + b.lo;
+    r.hi = a.hi + b.hi + (r.lo < a.lo ? 1u : 0u);
+    return r;
+}
+
+// Multiplier of the 128-bit LCG under PCG64.
+constexpr uint64_t PCG_MULT_HI = 0x2360ED051FC65DA4ULL;
+constexpr uint64_t PCG_MULT_LO = 0x4385DF649FCCF645ULL;
+
+// Jump tables: advancing the LCG by 2^b steps maps s -> A[b] s + G[b] inc.
+struct JumpTables {
+    u128 A[64];
+    u128 G[64];
+};
+
+// Host-side 128-bit helpers (built with the host compiler's __int128).
+void host_jump_tables(JumpTables *t);
+void host_seed_words(const uint32_t *words, int n_words, uint64_t out[4]);
+
+// ------------------------------------------------------------------ geometry
+// Device-resident patch table of one x-oriented microstructure.
+//   occ        bit-packed occupancy, word ((i*ny + j)*nzw + (k >> 5)), bit k & 31
+//   cell_code  per cell: (first patch id of the cell << 6) | exposed-face mask
+//   patch_*    per patch: origin (x,y,z) FP64, face 0..5 (walls: 0 hot, 1 cold)
+//   hot_map / cold_map  wall patch id per (j, k), -1 where the boundary cell is solid
+struct Geometry {
+    int32_t nx, ny, nz, nzw;
+    double H_m;
+    int64_t n_patches, n_wall;
+    int device;
+    uint32_t *occ;
+    uint32_t *cell_code;
+    double *patch_origin;  // n_patches * 3
+    int8_t *patch_face;    // n_patches
+    int32_t *patch_cell;   // n_patches (-1 for walls)
+    int8_t *patch_class;   // n_patches
+    int32_t *hot_map, *cold_map;
+};
+
+// Device exclusive scan helpers (prh_scan.hip).
+int exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *total, hipStream_t stream);
+int exclusive_scan_i32_to_i64(const int32_t *in, int64_t *out, int64_t n, int64_t *total, hipStream_t stream);
+
+// Persistent workspace helpers.
+int ensure_capacity(void **buf, size_t *cap, size_t bytes);
+
+int current_stream(hipStream_t *s);
+
+}  // namespace prh
+
+struct prh_geometry {
+    prh::Geometry g;
+};

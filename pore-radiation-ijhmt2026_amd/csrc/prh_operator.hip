@@ -1,0 +1,570 @@
+// Exchange operator and FP64 reflection series.
+//
+// Reference: ExchangeOperator (porerad/exchange.py:37-122) and
+// reflection_series (porerad/exchange.py:217-245).
+//
+// The symmetric matrix S = 0.5 (diag(A) F + F^T diag(A)) + delta diag(A) is
+// assembled once per operator as an FP64 CSR over the union of the sparsity
+// of C and C^T plus the diagonal; F_p is never stored: F_p x = (d/A) S(d x)
+// and F_p^T x = d S(d x / A), as in the reference.  The raw count matrix is
+// kept as two integer CSRs (C and C^T) for raw_matvec / raw_rmatvec.
+//
+// Every row is reduced by one wavefront in a fixed lane/tree order, so the
+// results do not depend on scheduling or on which other right-hand sides are
+// in flight: the same inputs always give the same bits.
+#include <algorithm>
+#include <cmath>
+#include <thread>
+#include <vector>
+
+#include "prh_internal.h"
+
+namespace prh {
+namespace {
+
+constexpr int ROW_THREADS = 256;  // 4 rows (one per wavefront) per workgroup
+constexpr int ROWS_PER_BLOCK = ROW_THREADS / 64;
+constexpr int RED_THREADS = 256;
+
+__device__ inline double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// y = S x (one wavefront per row).
+__global__ __launch_bounds__(ROW_THREADS) void k_spmv(const int64_t *rowptr, const int32_t *col, const double *val,
+                                                      int64_t n, const double *x, double *y)
+{
+    const int64_t row = (int64_t)blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double acc = 0.0;
+    for (int64_t e = rowptr[row] + lane; e < rowptr[row + 1]; e += 64) acc += val[e] * x[col[e]];
+    acc = wave_sum(acc);
+    if (lane == 0) y[row] = acc;
+}
+
+// Raw count rows: y_i = sum_j (c_ij / R_{who}) x_j, with who = i for F and
+// who = j for F^T (row of C^T holds origins).
+__global__ __launch_bounds__(ROW_THREADS) void k_spmv_counts(const int64_t *rowptr, const int32_t *col,
+                                                             const int32_t *cnt, const double *rays, int transpose,
+                                                             int64_t n, const double *x, double *y)
+{
+    const int64_t row = (int64_t)blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double acc = 0.0;
+    for (int64_t e = rowptr[row] + lane; e < rowptr[row + 1]; e += 64) {
+        const int32_t j = col[e];
+        const double w = (double)cnt[e] / rays[transpose ? j : row];
+        acc += w * x[j];
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) y[row] = acc;
+}
+
+// Elementwise pre/post scalings of matvec / rmatvec.
+__global__ void k_scale_in(const double *x, const double *d, const double *area, int mode, int64_t n, double *out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = mode == PRH_APPLY_MATVEC ? d[i] * x[i] : d[i] * (x[i] / area[i]);
+}
+
+__global__ void k_scale_out(const double *sx, const double *d, const double *area, int mode, int64_t n, double *y)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    y[i] = mode == PRH_APPLY_MATVEC ? (d[i] / area[i]) * sx[i] : d[i] * sx[i];
+}
+
+struct SeriesState {
+    double initial;   // max(||E||_1, tiny)
+    double l1;        // ||flight||_1 of the last reflection
+    double residual;  // sum(flight) at convergence
+    int32_t done;
+    int32_t iterations;
+};
+
+// flight = E, absorbed = 0, w = d (E / A), per-block sums of |E|.
+__global__ __launch_bounds__(RED_THREADS) void k_series_init(const double *E, const double *d, const double *area,
+                                                             int64_t n, double *flight, double *absorbed, double *w,
+                                                             double *part_abs)
+{
+    __shared__ double red[RED_THREADS];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
+        const double e = E[i];
+        flight[i] = e;
+        absorbed[i] = 0.0;
+        w[i] = d[i] * (e / area[i]);
+        s += fabs(e);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = RED_THREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part_abs[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_series_start(const double *part_abs, int n_part, SeriesState *S)
+{
+    __shared__ double red[RED_THREADS];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < n_part; b += RED_THREADS) s += part_abs[b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = RED_THREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double tiny = 2.2250738585072014e-308;
+        S->initial = red[0] > tiny ? red[0] : tiny;
+        S->l1 = red[0];
+        S->residual = 0.0;
+        S->done = 0;
+        S->iterations = 0;
+    }
+}
+
+// One reflection: incident = d * (S w); absorbed += eps * incident;
+// flight = (1 - eps) * incident; w_next = d * (flight / A).  Per-block
+// partial sums of |flight| and flight for the stopping rule and residual.
+__global__ __launch_bounds__(ROW_THREADS) void k_series_step(const int64_t *rowptr, const int32_t *col,
+                                                             const double *val, int64_t n, const double *d,
+                                                             const double *area, const double *eps,
+                                                             const double *w, double *w_next, double *absorbed,
+                                                             double *flight, double *part_abs, double *part_sum,
+                                                             const SeriesState *S)
+{
+    if (S->done) return;
+    __shared__ double sa[ROWS_PER_BLOCK], ss[ROWS_PER_BLOCK];
+    const int wave = threadIdx.x >> 6;
+    const int64_t row = (int64_t)blockIdx.x * ROWS_PER_BLOCK + wave;
+    const int lane = threadIdx.x & 63;
+    double fl = 0.0;
+    if (row < n) {
+        double acc = 0.0;
+        for (int64_t e = rowptr[row] + lane; e < rowptr[row + 1]; e += 64) acc += val[e] * w[col[e]];
+        acc = wave_sum(acc);
+        const double incident = d[row] * acc;
+        const double ep = eps[row];
+        fl = (1.0 - ep) * incident;
+        if (lane == 0) {
+            absorbed[row] = absorbed[row] + ep * incident;
+            flight[row] = fl;
+            w_next[row] = d[row] * (fl / area[row]);
+        }
+    }
+    if (lane == 0) {
+        sa[wave] = fabs(fl);
+        ss[wave] = fl;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, s = 0.0;
+        for (int k = 0; k < ROWS_PER_BLOCK; ++k) {
+            a += sa[k];
+            s += ss[k];
+        }
+        part_abs[blockIdx.x] = a;
+        part_sum[blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_series_check(const double *part_abs, const double *part_sum,
+                                                              int64_t n_part, double rtol, SeriesState *S)
+{
+    if (S->done) return;
+    __shared__ double ra[RED_THREADS], rs[RED_THREADS];
+    double a = 0.0, s = 0.0;
+    for (int64_t b = threadIdx.x; b < n_part; b += RED_THREADS) {
+        a += part_abs[b];
+        s += part_sum[b];
+    }
+    ra[threadIdx.x] = a;
+    rs[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = RED_THREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            ra[threadIdx.x] += ra[threadIdx.x + off];
+            rs[threadIdx.x] += rs[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        S->iterations += 1;
+        S->l1 = ra[0];
+        if (ra[0] <= rtol * S->initial) {
+            S->done = 1;
+            S->residual = rs[0];
+        }
+    }
+}
+
+unsigned blocks_for(int64_t n, int per_block)
+{
+    int64_t b = (n + per_block - 1) / per_block;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+}  // namespace
+}  // namespace prh
+
+struct prh_operator {
+    int device = 0;
+    int64_t n = 0;
+    // symmetric S (FP64 CSR)
+    int64_t nnz_s = 0;
+    int64_t *rowptr = nullptr;
+    int32_t *col = nullptr;
+    double *val = nullptr;
+    // raw counts: C by origin and C^T by receiver
+    int64_t nnz_c = 0;
+    int64_t *c_rowptr = nullptr, *t_rowptr = nullptr;
+    int32_t *c_col = nullptr, *c_cnt = nullptr, *t_col = nullptr, *t_cnt = nullptr;
+    double *rays = nullptr, *area = nullptr, *scale = nullptr;
+    double delta = 0.0;
+    // work vectors (n each)
+    double *vx = nullptr, *vy = nullptr, *vt = nullptr;
+    // series work
+    double *flight = nullptr, *absorbed = nullptr, *w0 = nullptr, *w1 = nullptr, *eps = nullptr, *emit = nullptr;
+    double *part_abs = nullptr, *part_sum = nullptr;
+    prh::SeriesState *state = nullptr;
+};
+
+using namespace prh;
+
+namespace {
+
+void free_operator(prh_operator *op)
+{
+    void *ptrs[] = {op->rowptr, op->col, op->val, op->c_rowptr, op->t_rowptr, op->c_col, op->c_cnt,
+                    op->t_col, op->t_cnt, op->rays, op->area, op->scale, op->vx, op->vy, op->vt,
+                    op->flight, op->absorbed, op->w0, op->w1, op->eps, op->emit, op->part_abs,
+                    op->part_sum, op->state};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+}
+
+// Host assembly of S over union(C, C^T, diagonal), rows merged in parallel.
+struct HostCSR {
+    std::vector<int64_t> c_rowptr, t_rowptr, s_rowptr;
+    std::vector<int32_t> t_col, t_cnt, s_col;
+    std::vector<double> s_val;
+};
+
+void assemble(int64_t n, const int32_t *origin, const int32_t *receiver, const int64_t *count, int64_t nnz,
+              const int64_t *rays, const double *area, double delta, HostCSR &H)
+{
+    H.c_rowptr.assign(n + 1, 0);
+    H.t_rowptr.assign(n + 1, 0);
+    for (int64_t e = 0; e < nnz; ++e) {
+        H.c_rowptr[origin[e] + 1]++;
+        H.t_rowptr[receiver[e] + 1]++;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        H.c_rowptr[i + 1] += H.c_rowptr[i];
+        H.t_rowptr[i + 1] += H.t_rowptr[i];
+    }
+    // Stable counting sort by receiver: rows of C^T come out sorted by origin.
+    H.t_col.resize(nnz);
+    H.t_cnt.resize(nnz);
+    {
+        std::vector<int64_t> cursor(H.t_rowptr.begin(), H.t_rowptr.end() - 1);
+        for (int64_t e = 0; e < nnz; ++e) {
+            const int64_t at = cursor[receiver[e]]++;
+            H.t_col[at] = origin[e];
+            H.t_cnt[at] = (int32_t)count[e];
+        }
+    }
+    // Row lengths of the union, then values.
+    const int n_threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<int64_t> len(n, 0);
+    auto row_len = [&](int64_t i) {
+        int64_t a = H.c_rowptr[i], ae = H.c_rowptr[i + 1], b = H.t_rowptr[i], be = H.t_rowptr[i + 1];
+        int64_t m = 0;
+        bool diag = false;
+        while (a < ae || b < be) {
+            int32_t ja = a < ae ? receiver[a] : INT32_MAX, jb = b < be ? H.t_col[b] : INT32_MAX;
+            int32_t j = std::min(ja, jb);
+            if (ja == j) ++a;
+            if (jb == j) ++b;
+            diag |= j == i;
+            ++m;
+        }
+        return m + (diag ? 0 : 1);
+    };
+    auto par = [&](auto fn) {
+        std::vector<std::thread> th;
+        for (int t = 0; t < n_threads; ++t)
+            th.emplace_back([&, t]() {
+                for (int64_t i = (n * t) / n_threads; i < (n * (t + 1)) / n_threads; ++i) fn(i);
+            });
+        for (auto &x : th) x.join();
+    };
+    par([&](int64_t i) { len[i] = row_len(i); });
+    H.s_rowptr.assign(n + 1, 0);
+    for (int64_t i = 0; i < n; ++i) H.s_rowptr[i + 1] = H.s_rowptr[i] + len[i];
+    const int64_t nnz_s = H.s_rowptr[n];
+    H.s_col.resize(nnz_s);
+    H.s_val.resize(nnz_s);
+    par([&](int64_t i) {
+        int64_t a = H.c_rowptr[i], ae = H.c_rowptr[i + 1], b = H.t_rowptr[i], be = H.t_rowptr[i + 1];
+        int64_t out = H.s_rowptr[i];
+        bool diag_done = false;
+        const double Ai = area[i], Ri = (double)rays[i];
+        auto emit_entry = [&](int32_t j, int64_t cij, int64_t cji) {
+            // 0.5 (A_i F_ij + A_j F_ji) + delta A_i [i == j]
+            const double fij = (double)cij / Ri;
+            const double fji = (double)cji / (double)rays[j];
+            double v = 0.5 * (Ai * fij + area[j] * fji);
+            if (j == i) v += delta * Ai;
+            H.s_col[out] = j;
+            H.s_val[out] = v;
+            ++out;
+        };
+        while (a < ae || b < be) {
+            int32_t ja = a < ae ? receiver[a] : INT32_MAX, jb = b < be ? H.t_col[b] : INT32_MAX;
+            int32_t j = std::min(ja, jb);
+            if (!diag_done && i < j) {
+                emit_entry((int32_t)i, 0, 0);
+                diag_done = true;
+            }
+            int64_t cij = 0, cji = 0;
+            if (ja == j) cij = count[a++];
+            if (jb == j) cji = H.t_cnt[b++];
+            if (j == i) diag_done = true;
+            emit_entry(j, cij, cji);
+        }
+        if (!diag_done) emit_entry((int32_t)i, 0, 0);
+    });
+}
+
+template <typename T>
+int upload(T **dst, const T *src, int64_t count, hipStream_t st)
+{
+    const int64_t c = count > 0 ? count : 1;
+    PRH_HIP(hipMalloc((void **)dst, sizeof(T) * c));
+    if (count > 0) PRH_HIP(hipMemcpyAsync(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice, st));
+    return PRH_OK;
+}
+
+int apply_device(prh_operator *op, int mode, const double *dx, double *dy, hipStream_t st)
+{
+    const int64_t n = op->n;
+    const unsigned rb = blocks_for(n, ROWS_PER_BLOCK), eb = blocks_for(n, 256);
+    switch (mode) {
+    case PRH_APPLY_SYMMETRIC:
+        k_spmv<<<rb, ROW_THREADS, 0, st>>>(op->rowptr, op->col, op->val, n, dx, dy);
+        break;
+    case PRH_APPLY_MATVEC:
+    case PRH_APPLY_RMATVEC:
+        k_scale_in<<<eb, 256, 0, st>>>(dx, op->scale, op->area, mode, n, op->vt);
+        k_spmv<<<rb, ROW_THREADS, 0, st>>>(op->rowptr, op->col, op->val, n, op->vt, dy);
+        k_scale_out<<<eb, 256, 0, st>>>(dy, op->scale, op->area, mode, n, dy);
+        break;
+    case PRH_APPLY_RAW:
+        k_spmv_counts<<<rb, ROW_THREADS, 0, st>>>(op->c_rowptr, op->c_col, op->c_cnt, op->rays, 0, n, dx, dy);
+        break;
+    case PRH_APPLY_RAW_T:
+        k_spmv_counts<<<rb, ROW_THREADS, 0, st>>>(op->t_rowptr, op->t_col, op->t_cnt, op->rays, 1, n, dx, dy);
+        break;
+    default:
+        set_error("prh_operator_apply: unknown mode %d", mode);
+        return PRH_ERR_ARG;
+    }
+    PRH_HIP(hipGetLastError());
+    return PRH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int prh_operator_create(int64_t n, const int32_t *origin, const int32_t *receiver, const int64_t *count,
+                        int64_t nnz, const int64_t *rays_per_origin, const double *area,
+                        double support_diagonal_fraction, prh_operator **out)
+{
+    PRH_CHECK_ARG(out && rays_per_origin && area && (nnz == 0 || (origin && receiver && count)),
+                  "prh_operator_create: null pointer");
+    PRH_CHECK_ARG(n > 0 && n < INT32_MAX && nnz >= 0, "prh_operator_create: bad sizes");
+    *out = nullptr;
+    // Validate order, ranges and row totals (exchange.py:59-71).
+    {
+        std::vector<int64_t> observed(n, 0);
+        for (int64_t e = 0; e < nnz; ++e) {
+            const int32_t o = origin[e], r = receiver[e];
+            PRH_CHECK_ARG(o >= 0 && o < n && r >= 0 && r < n && count[e] >= 0,
+                          "prh_operator_create: edge %lld out of range", (long long)e);
+            if (e > 0)
+                PRH_CHECK_ARG(origin[e - 1] < o || (origin[e - 1] == o && receiver[e - 1] < r),
+                              "prh_operator_create: edges must be sorted by (origin, receiver) and unique");
+            observed[o] += count[e];
+        }
+        for (int64_t i = 0; i < n; ++i) {
+            if (observed[i] != rays_per_origin[i]) {
+                set_error("first-hit counts do not account for every emitted ray");
+                return PRH_ERR_COUNTS;
+            }
+            PRH_CHECK_ARG(rays_per_origin[i] > 0, "prh_operator_create: rays_per_origin must be positive");
+        }
+    }
+    HostCSR H;
+    assemble(n, origin, receiver, count, nnz, rays_per_origin, area, support_diagonal_fraction, H);
+
+    prh_operator *op = new prh_operator();
+    PRH_HIP(hipGetDevice(&op->device));
+    hipStream_t st;
+    int rc = current_stream(&st);
+    if (rc) {
+        delete op;
+        return rc;
+    }
+    op->n = n;
+    op->nnz_s = H.s_rowptr[n];
+    op->nnz_c = nnz;
+    op->delta = support_diagonal_fraction;
+    std::vector<int32_t> c_cnt(nnz);
+    for (int64_t e = 0; e < nnz; ++e) c_cnt[e] = (int32_t)count[e];
+    std::vector<double> rays_d(n), ones(n, 1.0);
+    for (int64_t i = 0; i < n; ++i) rays_d[i] = (double)rays_per_origin[i];
+    auto fail = [&](int code) {
+        (void)hipStreamSynchronize(st);
+        free_operator(op);
+        delete op;
+        return code;
+    };
+    if ((rc = upload(&op->rowptr, H.s_rowptr.data(), n + 1, st))) return fail(rc);
+    if ((rc = upload(&op->col, H.s_col.data(), op->nnz_s, st))) return fail(rc);
+    if ((rc = upload(&op->val, H.s_val.data(), op->nnz_s, st))) return fail(rc);
+    if ((rc = upload(&op->c_rowptr, H.c_rowptr.data(), n + 1, st))) return fail(rc);
+    if ((rc = upload(&op->c_col, receiver, nnz, st))) return fail(rc);
+    if ((rc = upload(&op->c_cnt, c_cnt.data(), nnz, st))) return fail(rc);
+    if ((rc = upload(&op->t_rowptr, H.t_rowptr.data(), n + 1, st))) return fail(rc);
+    if ((rc = upload(&op->t_col, H.t_col.data(), nnz, st))) return fail(rc);
+    if ((rc = upload(&op->t_cnt, H.t_cnt.data(), nnz, st))) return fail(rc);
+    if ((rc = upload(&op->rays, rays_d.data(), n, st))) return fail(rc);
+    if ((rc = upload(&op->area, area, n, st))) return fail(rc);
+    if ((rc = upload(&op->scale, ones.data(), n, st))) return fail(rc);
+    double **work[] = {&op->vx, &op->vy, &op->vt, &op->flight, &op->absorbed, &op->w0, &op->w1, &op->eps, &op->emit};
+    for (double **p : work)
+        if (hipMalloc((void **)p, sizeof(double) * n) != hipSuccess) {
+            set_error("prh_operator_create: out of device memory");
+            return fail(PRH_ERR_HIP);
+        }
+    const int64_t n_part = blocks_for(n, ROWS_PER_BLOCK);
+    if (hipMalloc((void **)&op->part_abs, sizeof(double) * n_part) != hipSuccess ||
+        hipMalloc((void **)&op->part_sum, sizeof(double) * n_part) != hipSuccess ||
+        hipMalloc((void **)&op->state, sizeof(SeriesState)) != hipSuccess) {
+        set_error("prh_operator_create: out of device memory");
+        return fail(PRH_ERR_HIP);
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) {
+        set_error("prh_operator_create: upload failed");
+        return fail(PRH_ERR_HIP);
+    }
+    *out = op;
+    return PRH_OK;
+}
+
+int prh_operator_nnz(const prh_operator *op, int64_t *nnz_symmetric, int64_t *nnz_raw)
+{
+    PRH_CHECK_ARG(op, "prh_operator_nnz: null operator");
+    if (nnz_symmetric) *nnz_symmetric = op->nnz_s;
+    if (nnz_raw) *nnz_raw = op->nnz_c;
+    return PRH_OK;
+}
+
+int prh_operator_set_scale(prh_operator *op, const double *scale)
+{
+    PRH_CHECK_ARG(op && scale, "prh_operator_set_scale: null pointer");
+    PRH_HIP(hipSetDevice(op->device));
+    PRH_HIP(hipMemcpy(op->scale, scale, sizeof(double) * op->n, hipMemcpyHostToDevice));
+    return PRH_OK;
+}
+
+int prh_operator_apply(prh_operator *op, int32_t mode, const double *x, double *y)
+{
+    PRH_CHECK_ARG(op && x && y, "prh_operator_apply: null pointer");
+    PRH_HIP(hipSetDevice(op->device));
+    hipStream_t st;
+    int rc = current_stream(&st);
+    if (rc) return rc;
+    PRH_HIP(hipMemcpyAsync(op->vx, x, sizeof(double) * op->n, hipMemcpyHostToDevice, st));
+    if ((rc = apply_device(op, mode, op->vx, op->vy, st))) return rc;
+    PRH_HIP(hipMemcpyAsync(y, op->vy, sizeof(double) * op->n, hipMemcpyDeviceToHost, st));
+    PRH_HIP(hipStreamSynchronize(st));
+    return PRH_OK;
+}
+
+int prh_radiosity(prh_operator *op, int32_t n_rhs, const double *emitted, const double *emissivity,
+                  int32_t emissivity_per_rhs, double relative_tolerance, int32_t max_reflections,
+                  int32_t precision, double *absorbed, double *residual, int32_t *iterations)
+{
+    PRH_CHECK_ARG(op && emitted && emissivity && absorbed && residual && iterations,
+                  "prh_radiosity: null pointer");
+    PRH_CHECK_ARG(n_rhs >= 0 && max_reflections >= 1, "prh_radiosity: bad sizes");
+    PRH_CHECK_ARG(precision == PRH_FP64, "prh_radiosity: unsupported precision %d", precision);
+    PRH_HIP(hipSetDevice(op->device));
+    hipStream_t st;
+    int rc = current_stream(&st);
+    if (rc) return rc;
+    const int64_t n = op->n;
+    const unsigned rb = blocks_for(n, ROWS_PER_BLOCK);
+    const int64_t n_part = rb;
+    const unsigned ib = (unsigned)std::min<int64_t>(blocks_for(n, RED_THREADS), 1024);
+    for (int32_t q = 0; q < n_rhs; ++q) {
+        PRH_HIP(hipMemcpyAsync(op->emit, emitted + (int64_t)q * n, sizeof(double) * n, hipMemcpyHostToDevice, st));
+        if (q == 0 || emissivity_per_rhs)
+            PRH_HIP(hipMemcpyAsync(op->eps, emissivity + (emissivity_per_rhs ? (int64_t)q * n : 0),
+                                   sizeof(double) * n, hipMemcpyHostToDevice, st));
+        k_series_init<<<ib, RED_THREADS, 0, st>>>(op->emit, op->scale, op->area, n, op->flight, op->absorbed,
+                                                  op->w0, op->part_abs);
+        k_series_start<<<1, RED_THREADS, 0, st>>>(op->part_abs, (int)ib, op->state);
+        PRH_HIP(hipGetLastError());
+        int32_t it = 0;
+        SeriesState hs{};
+        double *w_in = op->w0, *w_out = op->w1;
+        while (it < max_reflections) {
+            const int batch = std::min<int32_t>(16, max_reflections - it);
+            for (int b = 0; b < batch; ++b) {
+                k_series_step<<<rb, ROW_THREADS, 0, st>>>(op->rowptr, op->col, op->val, n, op->scale, op->area,
+                                                          op->eps, w_in, w_out, op->absorbed, op->flight,
+                                                          op->part_abs, op->part_sum, op->state);
+                k_series_check<<<1, RED_THREADS, 0, st>>>(op->part_abs, op->part_sum, n_part, relative_tolerance,
+                                                          op->state);
+                std::swap(w_in, w_out);
+            }
+            PRH_HIP(hipGetLastError());
+            it += batch;
+            PRH_HIP(hipMemcpyAsync(&hs, op->state, sizeof(hs), hipMemcpyDeviceToHost, st));
+            PRH_HIP(hipStreamSynchronize(st));
+            if (hs.done) break;
+        }
+        if (!hs.done) {
+            set_error("radiosity reflection series did not converge");
+            return PRH_ERR_SERIES;
+        }
+        PRH_HIP(hipMemcpyAsync(absorbed + (int64_t)q * n, op->absorbed, sizeof(double) * n, hipMemcpyDeviceToHost,
+                               st));
+        PRH_HIP(hipStreamSynchronize(st));
+        residual[q] = hs.residual;
+        iterations[q] = hs.iterations;
+    }
+    return PRH_OK;
+}
+
+void prh_operator_destroy(prh_operator *op)
+{
+    if (!op) return;
+    (void)hipSetDevice(op->device);
+    free_operator(op);
+    delete op;
+}
+
+}  // extern "C"

@@ -1,0 +1,279 @@
+"""ctypes binding of the porerad HIP library (``libporerad_hip.so``, C ABI in
+``include/porerad_hip.h``).
+
+The library is the only compute path of this package: there is no CPU
+fallback.  If the shared library is missing or cannot be loaded, every entry
+point raises ``RuntimeError`` naming the cause.  Status codes map to the
+exception types of the reference package (porerad/view_factors.py:268-269,
+porerad/exchange.py:70-71, 243-244).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libporerad_hip.so")
+
+PRH_OK = 0
+PRH_ERR_ARG = 1
+PRH_ERR_HIP = 2
+PRH_ERR_UNRESOLVED = 3
+PRH_ERR_COUNTS = 4
+PRH_ERR_SERIES = 5
+PRH_ERR_CAPACITY = 6
+
+APPLY_SYMMETRIC, APPLY_MATVEC, APPLY_RMATVEC, APPLY_RAW, APPLY_RAW_T = 0, 1, 2, 3, 4
+FP64 = 0
+
+# Every function the C ABI exports: name -> (argtypes, restype).
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_F64 = ctypes.c_double
+SIGNATURES = {
+    "prh_last_error": ([], ctypes.c_char_p),
+    "prh_version": ([], ctypes.c_char_p),
+    "prh_last_timings": ([_P, _I32], ctypes.c_int),
+    "prh_pcg64_seed": ([_P, _I32, _P], ctypes.c_int),
+    "prh_geometry_create": ([_P, _I32, _I32, _I32, _F64, _P], ctypes.c_int),
+    "prh_geometry_sizes": ([_P, _P, _P], ctypes.c_int),
+    "prh_geometry_patch_table": ([_P, _P, _P, _P, _P], ctypes.c_int),
+    "prh_geometry_destroy": ([_P], None),
+    "prh_generate_rays": ([_P, _I64, _I64, _I32, _P, _P, _P], ctypes.c_int),
+    "prh_trace_rays": ([_P, _P, _P, _I64, _P, _P], ctypes.c_int),
+    "prh_trace_counts": ([_P, _I64, _I64, _I32, _I32, _P, _P, _P, _P], ctypes.c_int),
+    "prh_edges_copy": ([_P, _P, _P, _P], ctypes.c_int),
+    "prh_edges_destroy": ([_P], None),
+    "prh_operator_create": ([_I64, _P, _P, _P, _I64, _P, _P, _F64, _P], ctypes.c_int),
+    "prh_operator_set_scale": ([_P, _P], ctypes.c_int),
+    "prh_operator_nnz": ([_P, _P, _P], ctypes.c_int),
+    "prh_operator_apply": ([_P, _I32, _P, _P], ctypes.c_int),
+    "prh_radiosity": ([_P, _I32, _P, _P, _I32, _F64, _I32, _I32, _P, _P, _P], ctypes.c_int),
+    "prh_operator_destroy": ([_P], None),
+}
+
+_lock = threading.Lock()
+_lib = None
+_load_error: str | None = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the library and declare every exported signature (no GPU call is made)."""
+
+    L = ctypes.CDLL(path)
+    for name, (argtypes, restype) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.argtypes = argtypes
+        fn.restype = restype
+    return L
+
+
+def lib() -> ctypes.CDLL:
+    global _lib, _load_error
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"porerad HIP library not built: {LIB_PATH} is missing "
+                    "(run __graft_entry__.build() or pore-radiation-ijhmt2026_amd/build.py)")
+            try:
+                _lib = load(LIB_PATH)
+            except OSError as exc:  # pragma: no cover - depends on the ROCm install
+                _load_error = str(exc)
+                raise RuntimeError(f"porerad HIP library failed to load: {exc}") from exc
+    return _lib
+
+
+def last_error() -> str:
+    msg = lib().prh_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(status: int, what: str = "") -> None:
+    """Raise the reference's exception type for a non-zero status."""
+
+    if status == PRH_OK:
+        return
+    message = last_error() or f"{what} failed with status {status}"
+    if status in (PRH_ERR_UNRESOLVED, PRH_ERR_SERIES):
+        raise RuntimeError(message)
+    if status in (PRH_ERR_COUNTS, PRH_ERR_ARG, PRH_ERR_CAPACITY):
+        raise ValueError(message)
+    raise RuntimeError(f"HIP error in {what}: {message}")
+
+
+def ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def seed_words(seed: int) -> np.ndarray:
+    """Little-endian uint32 words of a non-negative integer (SeedSequence entropy)."""
+
+    value = int(seed)
+    if value < 0:
+        raise ValueError("expected non-negative integer")
+    words = [0] if value == 0 else []
+    while value > 0:
+        words.append(value & 0xFFFFFFFF)
+        value >>= 32
+    return np.asarray(words, dtype=np.uint32)
+
+
+def pcg64_state(seed: int) -> np.ndarray:
+    """{state_hi, state_lo, inc_hi, inc_lo} of numpy.random.PCG64(seed)."""
+
+    words = seed_words(seed)
+    out = np.zeros(4, dtype=np.uint64)
+    check(lib().prh_pcg64_seed(ptr(words), int(words.size), ptr(out)), "prh_pcg64_seed")
+    return out
+
+
+def last_timings() -> np.ndarray:
+    out = np.zeros(3, dtype=np.float64)
+    check(lib().prh_last_timings(ptr(out), 3), "prh_last_timings")
+    return out
+
+
+class Geometry:
+    """Owner of a device patch table (``prh_geometry``)."""
+
+    def __init__(self, solid_x: np.ndarray, H_m: float) -> None:
+        solid = np.ascontiguousarray(solid_x, dtype=np.uint8)
+        if solid.ndim != 3:
+            raise ValueError("expected a 3-D phase array")
+        self.shape = tuple(int(s) for s in solid.shape)
+        handle = ctypes.c_void_p()
+        L = lib()
+        check(L.prh_geometry_create(ptr(solid), *self.shape, float(H_m), ctypes.byref(handle)),
+              "prh_geometry_create")
+        self._lib = L
+        self.handle = handle
+        n, nw = ctypes.c_int64(), ctypes.c_int64()
+        check(L.prh_geometry_sizes(handle, ctypes.byref(n), ctypes.byref(nw)), "prh_geometry_sizes")
+        self.n_patches, self.n_wall = int(n.value), int(nw.value)
+
+    def patch_table(self):
+        n = self.n_patches
+        cls = np.empty(n, np.int8)
+        cell = np.empty(n, np.int32)
+        face = np.empty(n, np.int8)
+        origin = np.empty((n, 3), np.float64)
+        check(self._lib.prh_geometry_patch_table(self.handle, ptr(cls), ptr(cell), ptr(face), ptr(origin)),
+              "prh_geometry_patch_table")
+        return cls, cell, face, origin
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self._lib.prh_geometry_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Edges:
+    """Owner of a device-resident edge table (``prh_edges``)."""
+
+    def __init__(self, handle: ctypes.c_void_p, nnz: int, occupancy_reads: int) -> None:
+        self._lib = lib()
+        self.handle = handle
+        self.nnz = int(nnz)
+        self.occupancy_reads = int(occupancy_reads)
+
+    def to_host(self):
+        origin = np.empty(self.nnz, np.int32)
+        receiver = np.empty(self.nnz, np.int32)
+        count = np.empty(self.nnz, np.int64)
+        check(self._lib.prh_edges_copy(self.handle, ptr(origin), ptr(receiver), ptr(count)), "prh_edges_copy")
+        return origin, receiver, count
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self._lib.prh_edges_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def trace_counts(geometry: Geometry, p0: int, p1: int, rays_per_patch: int, seeds) -> Edges:
+    """Fused generate + trace + count of patches [p0, p1) over one or more seeds."""
+
+    states = np.ascontiguousarray(np.stack([pcg64_state(s) for s in seeds]), dtype=np.uint64)
+    handle = ctypes.c_void_p()
+    nnz = ctypes.c_int64()
+    reads = ctypes.c_int64()
+    check(lib().prh_trace_counts(geometry.handle, int(p0), int(p1), int(rays_per_patch), int(len(seeds)),
+                                 ptr(states), ctypes.byref(handle), ctypes.byref(nnz), ctypes.byref(reads)),
+          "prh_trace_counts")
+    return Edges(handle, nnz.value, reads.value)
+
+
+class Operator:
+    """Owner of a device exchange operator (``prh_operator``)."""
+
+    def __init__(self, n: int, origin, receiver, count, rays, area, delta: float) -> None:
+        self._lib = lib()
+        self.n = int(n)
+        origin = np.ascontiguousarray(origin, dtype=np.int32)
+        receiver = np.ascontiguousarray(receiver, dtype=np.int32)
+        count = np.ascontiguousarray(count, dtype=np.int64)
+        rays = np.ascontiguousarray(rays, dtype=np.int64)
+        area = np.ascontiguousarray(area, dtype=np.float64)
+        handle = ctypes.c_void_p()
+        check(self._lib.prh_operator_create(self.n, ptr(origin), ptr(receiver), ptr(count), int(count.size),
+                                            ptr(rays), ptr(area), float(delta), ctypes.byref(handle)),
+              "prh_operator_create")
+        self.handle = handle
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        check(self._lib.prh_operator_nnz(handle, ctypes.byref(a), ctypes.byref(b)), "prh_operator_nnz")
+        self.nnz_symmetric, self.nnz_raw = int(a.value), int(b.value)
+
+    def set_scale(self, scale: np.ndarray) -> None:
+        scale = np.ascontiguousarray(scale, dtype=np.float64)
+        check(self._lib.prh_operator_set_scale(self.handle, ptr(scale)), "prh_operator_set_scale")
+
+    def apply(self, mode: int, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.shape != (self.n,):
+            raise ValueError(f"expected a vector of {self.n} values")
+        y = np.empty(self.n, np.float64)
+        check(self._lib.prh_operator_apply(self.handle, int(mode), ptr(x), ptr(y)), "prh_operator_apply")
+        return y
+
+    def radiosity(self, emitted: np.ndarray, emissivity: np.ndarray, relative_tolerance: float,
+                  max_reflections: int, precision: int = FP64):
+        E = np.ascontiguousarray(np.atleast_2d(emitted), dtype=np.float64)
+        n_rhs = E.shape[0]
+        eps = np.ascontiguousarray(emissivity, dtype=np.float64)
+        per_rhs = 1 if eps.ndim == 2 else 0
+        absorbed = np.empty_like(E)
+        residual = np.empty(n_rhs, np.float64)
+        iterations = np.empty(n_rhs, np.int32)
+        check(self._lib.prh_radiosity(self.handle, n_rhs, ptr(E), ptr(eps), per_rhs, float(relative_tolerance),
+                                      int(max_reflections), int(precision), ptr(absorbed), ptr(residual),
+                                      ptr(iterations)), "prh_radiosity")
+        return absorbed, residual, iterations
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self._lib.prh_operator_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass

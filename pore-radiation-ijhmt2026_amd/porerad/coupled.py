@@ -1,0 +1,478 @@
+"""Prescribed-field radiation channels and the strictly coupled radiation-conduction solve.
+
+Drop-in for the reference module porerad/coupled.py (lines 30-490).  The net
+radiative power of each internal patch goes to the solid cell that owns it.
+Under a prescribed linear field this gives the channels q_h, q_c, q_i; in the
+coupled problem R(T) = A T - b - Q_rad(T) = 0 is solved by Newton-Krylov with
+flexible GMRES, right-preconditioned by a fixed number of Jacobi-PCG steps on
+A + diag(emission derivative), and a backtracking line search that keeps the
+field between the wall temperatures.  Every radiosity evaluation and JVP is a
+device reflection series (porerad.exchange).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .conduction import ConductionSystem
+from .constants import SIGMA
+from .exchange import ExchangeOperator, RadiosityResult, net_patch_power, net_patch_power_jvp
+from .view_factors import COLD, HOT, INTERNAL, SurfacePatches
+
+_TINY = np.finfo(np.float64).tiny
+
+NEWTON_MAX_ITERATIONS = 8
+NEWTON_RESIDUAL_W = 1.0e-13
+NEWTON_STEP_K = 1.0e-10
+GMRES_RESTART = 30
+GMRES_MAX_ITERATIONS = 90
+GMRES_RTOL = 1.0e-8
+GMRES_ATOL = 1.0e-13
+RADIOSITY_RTOL_NEWTON = 1.0e-8
+RADIOSITY_RTOL_FINAL = 1.0e-13
+PCG_STEPS = (10, 20)
+"""Preconditioner steps of the first attempt and of the restart (coupled.py:39)."""
+
+
+# -- patch <-> cell maps (coupled.py:45-73) ---------------------------------------
+
+def patch_temperature(patches: SurfacePatches, T_cells: np.ndarray, T_hot_K: float, T_cold_K: float) -> np.ndarray:
+    """Wall temperatures on wall patches, the owning-cell temperature on internal patches."""
+
+    cls = patches.patch_class
+    T = np.empty(patches.n_patches, dtype=np.float64)
+    T[cls == HOT] = float(T_hot_K)
+    T[cls == COLD] = float(T_cold_K)
+    inner = cls == INTERNAL
+    T[inner] = np.asarray(T_cells, dtype=np.float64).ravel()[patches.patch_cell[inner]]
+    return T
+
+
+def cell_power(patches: SurfacePatches, patch_power_W: np.ndarray) -> np.ndarray:
+    """Internal patch power summed into the owning solid cells."""
+
+    inner = patches.patch_class == INTERNAL
+    total = np.zeros(patches.solid.size, dtype=np.float64)
+    np.add.at(total, patches.patch_cell[inner], patch_power_W[inner])
+    return total.reshape(patches.solid.shape)
+
+
+def emission_diagonal(patches: SurfacePatches, T_cells: np.ndarray, epsilon: float) -> np.ndarray:
+    """sum_p 4 eps sigma A_p T^3 of every cell (W/K)."""
+
+    inner = patches.patch_class == INTERNAL
+    owner = patches.patch_cell[inner].astype(np.int64)
+    T = np.asarray(T_cells, dtype=np.float64).ravel()[owner]
+    diag = np.zeros(patches.solid.size, dtype=np.float64)
+    np.add.at(diag, owner, 4.0 * float(epsilon) * SIGMA * patches.patch_area_m2[inner] * T**3)
+    return diag
+
+
+@dataclass
+class RadiationState:
+    """Patch temperatures, net patch power and net cell power of one field (coupled.py:76-83)."""
+
+    patch_temperature_K: np.ndarray
+    patch_net_power_W: np.ndarray
+    cell_net_power_W: np.ndarray
+    radiosity: RadiosityResult
+
+
+def evaluate_radiation(
+    operator: ExchangeOperator,
+    patches: SurfacePatches,
+    T_cells: np.ndarray,
+    T_hot_K: float,
+    T_cold_K: float,
+    epsilon: float,
+    *,
+    relative_tolerance: float,
+    reference_temperature_K: float | None = None,
+) -> RadiationState:
+    """Net radiative power of patches and cells; uniform emissivity (coupled.py:86-105)."""
+
+    T_patch = patch_temperature(patches, T_cells, T_hot_K, T_cold_K)
+    eps = np.full(patches.n_patches, float(epsilon))
+    result = net_patch_power(operator, T_patch, eps, patches.patch_area_m2, relative_tolerance=relative_tolerance,
+                             reference_temperature_K=reference_temperature_K)
+    return RadiationState(T_patch, result.net_power_W, cell_power(patches, result.net_power_W), result)
+
+
+def radiation_jvp(
+    operator: ExchangeOperator,
+    patches: SurfacePatches,
+    T_patch: np.ndarray,
+    dT_cells: np.ndarray,
+    epsilon: float,
+    *,
+    relative_tolerance: float,
+) -> np.ndarray:
+    """Change of the net cell power for a cell temperature perturbation, walls fixed (coupled.py:108-126)."""
+
+    inner = patches.patch_class == INTERNAL
+    dT_patch = np.zeros(patches.n_patches, dtype=np.float64)
+    dT_patch[inner] = np.asarray(dT_cells, dtype=np.float64).ravel()[patches.patch_cell[inner]]
+    eps = np.full(patches.n_patches, float(epsilon))
+    jvp = net_patch_power_jvp(operator, T_patch, dT_patch, eps, patches.patch_area_m2,
+                              relative_tolerance=relative_tolerance)
+    return cell_power(patches, jvp)
+
+
+def cross_section_area(patches: SurfacePatches) -> float:
+    """Projected wall area N_y N_z (H/N_x)^2 (coupled.py:129-133)."""
+
+    nx, ny, nz = patches.solid.shape
+    return float(ny * nz) * (patches.H_m / float(nx)) ** 2
+
+
+# -- prescribed temperature field (coupled.py:138-219) ----------------------------
+
+def linear_cell_temperature(shape: tuple[int, int, int], T_mean_K: float, delta_T_K: float) -> np.ndarray:
+    """T_h + (T_c - T_h) s / H at the cell centres."""
+
+    nx = int(shape[0])
+    axial = (T_mean_K + delta_T_K / 2.0) - (np.arange(nx, dtype=np.float64) + 0.5) * delta_T_K / nx
+    return np.broadcast_to(axial[:, None, None], tuple(int(n) for n in shape)).copy()
+
+
+def coarse_source(cell_power_W: np.ndarray, bins: int = 8) -> np.ndarray:
+    """Cell power summed over a bins^3 grid of equal blocks."""
+
+    a = np.asarray(cell_power_W, dtype=np.float64)
+    parts = [np.array_split(np.arange(n), min(bins, n)) for n in a.shape]
+    sums = [float(np.sum(a[np.ix_(xs, ys, zs)])) for xs in parts[0] for ys in parts[1] for zs in parts[2]]
+    return np.asarray(sums).reshape(tuple(len(p) for p in parts))
+
+
+@dataclass
+class PrescribedChannels:
+    """Radiation channels (W m^-2) under the prescribed linear field, q_h + q_c + q_i = 0."""
+
+    T_mean_K: float
+    delta_T_K: float
+    epsilon: float
+    q_h: float
+    q_c: float
+    q_i: float
+    closure_rel: float
+    coarse_source: np.ndarray = field(repr=False)
+
+
+def _channels_from_net(patches: SurfacePatches, net: np.ndarray) -> tuple[float, float, float, float]:
+    area = cross_section_area(patches)
+    cls = patches.patch_class
+    q_h = -float(np.sum(net[cls == HOT])) / area
+    q_c = -float(np.sum(net[cls == COLD])) / area
+    q_i = -float(np.sum(net[cls == INTERNAL])) / area
+    closure = abs(q_h + q_c + q_i) / max(abs(q_h), abs(q_c), abs(q_i), 1.0e-30)
+    return q_h, q_c, q_i, closure
+
+
+def prescribed_field_channels(
+    operator: ExchangeOperator, patches: SurfacePatches, T_mean_K: float, delta_T_K: float, epsilon: float
+) -> PrescribedChannels:
+    """Net power leaving the hot wall, the cold wall and the internal surfaces (coupled.py:170-194).
+
+    Walls at T_m +/- dT/2, internal surfaces at the linear field of their
+    cell, one radiosity evaluation to 1e-13 with the isothermal state at T_m
+    subtracted.
+    """
+
+    state = evaluate_radiation(
+        operator, patches, linear_cell_temperature(patches.solid.shape, T_mean_K, delta_T_K),
+        T_mean_K + delta_T_K / 2.0, T_mean_K - delta_T_K / 2.0, epsilon,
+        relative_tolerance=RADIOSITY_RTOL_FINAL, reference_temperature_K=T_mean_K,
+    )
+    q_h, q_c, q_i, closure = _channels_from_net(patches, state.patch_net_power_W)
+    return PrescribedChannels(float(T_mean_K), float(delta_T_K), float(epsilon), q_h, q_c, q_i, closure,
+                              coarse_source(state.cell_net_power_W) / cross_section_area(patches))
+
+
+def relative_difference(a: float, b: float) -> float:
+    return abs(float(a) - float(b)) / max(abs(float(a)), abs(float(b)), _TINY)
+
+
+def ab_differences(set_a: list[PrescribedChannels], set_b: list[PrescribedChannels]) -> tuple[float, float]:
+    """Largest A/B relative difference of q_h, q_c and of the 8^3 coarse source, L1 (coupled.py:201-213)."""
+
+    channel = max(relative_difference(getattr(a, k), getattr(b, k))
+                  for a, b in zip(set_a, set_b) for k in ("q_h", "q_c"))
+    source = max(
+        float(np.sum(np.abs(a.coarse_source - b.coarse_source)))
+        / max(float(np.sum(np.abs(a.coarse_source))), float(np.sum(np.abs(b.coarse_source))), _TINY)
+        for a, b in zip(set_a, set_b)
+    )
+    return channel, source
+
+
+def append_required(channel_difference: float, source_difference: float) -> bool:
+    """One symmetric append of a further ray block to each set when the sets disagree (coupled.py:216-219)."""
+
+    return channel_difference > 0.01 or source_difference > 0.05
+
+
+# -- strictly coupled solve (coupled.py:224-447) -----------------------------------
+
+@dataclass
+class GMRESResult:
+    solution: np.ndarray
+    converged: bool
+    iterations: int
+    true_residual_relative: float
+
+
+def flexible_gmres(matvec, rhs: np.ndarray, preconditioner, *, restart: int, max_iterations: int,
+                   relative_tolerance: float, absolute_tolerance: float) -> GMRESResult:
+    """Right-preconditioned FGMRES with modified Gram-Schmidt and one re-orthogonalisation (coupled.py:232-289).
+
+    The preconditioned vectors are kept (the preconditioner may vary); a
+    small least-squares residual is confirmed with the true residual.
+    """
+
+    b = np.asarray(rhs, dtype=np.float64)
+    n = b.size
+    x = np.zeros_like(b)
+    b_norm = max(float(np.linalg.norm(b)), _TINY)
+    goal = max(float(absolute_tolerance), float(relative_tolerance) * b_norm)
+    eps = np.finfo(np.float64).eps
+    done = 0
+    while done < max_iterations:
+        r = b - matvec(x)
+        beta = float(np.linalg.norm(r))
+        if beta <= goal:
+            return GMRESResult(x, True, done, beta / b_norm)
+        m = min(restart, max_iterations - done)
+        V = np.zeros((m + 1, n))
+        Z = np.zeros((m, n))
+        H = np.zeros((m + 1, m))
+        g = np.zeros(m + 1)
+        V[0] = r / beta
+        g[0] = beta
+        restart_at = None
+        for j in range(m):
+            Z[j] = preconditioner(V[j])
+            w = matvec(Z[j])
+            for _ in range(2):
+                for i in range(j + 1):
+                    h = float(np.dot(V[i], w))
+                    H[i, j] += h
+                    w -= h * V[i]
+            h_next = float(np.linalg.norm(w))
+            H[j + 1, j] = h_next
+            if h_next > eps * beta:
+                V[j + 1] = w / h_next
+            Hj = H[: j + 2, : j + 1]
+            y = np.linalg.lstsq(Hj, g[: j + 2], rcond=None)[0]
+            estimate = float(np.linalg.norm(g[: j + 2] - Hj @ y))
+            done += 1
+            if estimate <= goal or h_next <= eps * beta:
+                candidate = x + y @ Z[: j + 1]
+                true_norm = float(np.linalg.norm(b - matvec(candidate)))
+                if true_norm <= goal:
+                    return GMRESResult(candidate, True, done, true_norm / b_norm)
+                restart_at = candidate
+                break
+        if restart_at is not None:
+            x = restart_at
+            continue
+        x += np.linalg.lstsq(H, g, rcond=None)[0] @ Z
+    true_norm = float(np.linalg.norm(b - matvec(x)))
+    return GMRESResult(x, true_norm <= goal, done, true_norm / b_norm)
+
+
+class FixedJacobiPCG:
+    """A fixed number of Jacobi-preconditioned CG steps on A + diag(shift), from x = 0 (coupled.py:292-315)."""
+
+    def __init__(self, system: ConductionSystem, shift: np.ndarray, steps: int) -> None:
+        self.matrix, self.shift, self.steps = system.matrix, shift, int(steps)
+        self.inverse_diagonal = 1.0 / (system.matrix.diagonal() + shift)
+
+    def __call__(self, rhs: np.ndarray) -> np.ndarray:
+        x = np.zeros_like(rhs)
+        r = rhs.copy()
+        z = self.inverse_diagonal * r
+        p = z.copy()
+        rz = r @ z
+        for step in range(self.steps):
+            Ap = self.matrix @ p + self.shift * p
+            alpha = rz / (p @ Ap)
+            x += alpha * p
+            r -= alpha * Ap
+            z = self.inverse_diagonal * r
+            if step + 1 < self.steps:
+                rz_next = r @ z
+                p = z + (rz_next / rz) * p
+                rz = rz_next
+        return x
+
+
+@dataclass
+class CoupledResult:
+    """Converged coupled field, its radiation state and the iteration record (coupled.py:318-338)."""
+
+    temperature_K: np.ndarray
+    state: RadiationState
+    converged: bool
+    newton_iterations: int
+    linear_solves: list[dict] = field(default_factory=list)
+    final_residual_inf_W: float = float("nan")
+    preconditioner_steps: int = 0
+    failure: str = ""
+
+    @property
+    def linear_residuals_pass(self) -> bool:
+        """Every linear solve met ||r||_2 <= max(1e-13, 1e-8 ||b||_2)."""
+
+        return bool(self.linear_solves) and all(
+            s["converged"] and s["true_residual_l2_W"] <= max(1.0e-13, 1.0e-8 * s["rhs_l2_W"])
+            for s in self.linear_solves
+        )
+
+
+def newton_krylov(
+    system: ConductionSystem,
+    operator: ExchangeOperator,
+    patches: SurfacePatches,
+    T_hot_K: float,
+    T_cold_K: float,
+    epsilon: float,
+    T_initial: np.ndarray,
+    preconditioner_steps: int,
+) -> CoupledResult:
+    """Newton-Krylov solve of A T - b - Q_rad(T) = 0 from the pure-conduction field (coupled.py:341-425).
+
+    Converged when ||R||_inf <= 1e-13 W and the last update is <= 1e-10 K;
+    radiosity to 1e-8 inside the iteration, 1e-13 for the returned state;
+    trial fields outside [T_c, T_h] are rejected by the line search.
+    """
+
+    A = system.matrix
+    b = system.boundary_rhs(T_hot_K, T_cold_K)
+    lower, upper = float(T_cold_K), float(T_hot_K)
+    pcg = FixedJacobiPCG(system, emission_diagonal(patches, T_initial, epsilon), preconditioner_steps)
+
+    def radiation(T: np.ndarray, rtol: float) -> RadiationState:
+        return evaluate_radiation(operator, patches, T, T_hot_K, T_cold_K, epsilon, relative_tolerance=rtol)
+
+    T = np.asarray(T_initial, dtype=np.float64).copy()
+    state = radiation(T, RADIOSITY_RTOL_NEWTON)
+    residual = A @ T.ravel() - b - state.cell_net_power_W.ravel()
+    target = NEWTON_RESIDUAL_W
+    last_step = 0.0 if max(float(np.linalg.norm(residual, np.inf)), _TINY) <= target else float("inf")
+    result = CoupledResult(T, state, False, 0, preconditioner_steps=int(preconditioner_steps))
+
+    def finish(converged: bool, failure: str = "") -> CoupledResult:
+        final = radiation(T, RADIOSITY_RTOL_FINAL)
+        result.temperature_K, result.state, result.converged, result.failure = T, final, converged, failure
+        result.final_residual_inf_W = float(np.linalg.norm(A @ T.ravel() - b - final.cell_net_power_W.ravel(), np.inf))
+        return result
+
+    for _ in range(NEWTON_MAX_ITERATIONS):
+        r_norm = float(np.linalg.norm(residual, np.inf))
+        if r_norm <= target and last_step <= NEWTON_STEP_K:
+            return finish(True)
+        T_patch = state.patch_temperature_K.copy()
+
+        def jacobian(v: np.ndarray) -> np.ndarray:
+            jvp = radiation_jvp(operator, patches, T_patch, v.reshape(system.shape), epsilon,
+                                relative_tolerance=RADIOSITY_RTOL_NEWTON)
+            return A @ v - jvp.ravel()
+
+        gm = flexible_gmres(jacobian, -residual, pcg, restart=GMRES_RESTART, max_iterations=GMRES_MAX_ITERATIONS,
+                            relative_tolerance=GMRES_RTOL, absolute_tolerance=GMRES_ATOL)
+        step = gm.solution
+        result.linear_solves.append({
+            "converged": gm.converged,
+            "gmres_iterations": gm.iterations,
+            "rhs_l2_W": float(np.linalg.norm(residual)),
+            "true_residual_l2_W": float(np.linalg.norm(jacobian(step) + residual)),
+            "true_residual_relative": gm.true_residual_relative,
+        })
+        if not gm.converged:
+            return finish(False, "linear solve did not converge")
+        if r_norm <= target and float(np.max(np.abs(step))) <= NEWTON_STEP_K:
+            return finish(True)
+
+        alpha, accepted = 1.0, False
+        while alpha >= 2.0**-20:
+            candidate = T.ravel() + alpha * step
+            if np.all(np.isfinite(candidate)) and float(np.min(candidate)) >= lower and float(np.max(candidate)) <= upper:
+                trial = candidate.reshape(system.shape)
+                trial_state = radiation(trial, RADIOSITY_RTOL_NEWTON)
+                trial_residual = A @ candidate - b - trial_state.cell_net_power_W.ravel()
+                trial_norm = float(np.linalg.norm(trial_residual, np.inf))
+                if trial_norm <= (1.0 - 1.0e-4 * alpha) * r_norm:
+                    accepted = True
+                    break
+            alpha *= 0.5
+        if not accepted:
+            return finish(False, "line search failed")
+        last_step = float(np.max(np.abs(trial - T)))
+        T, state, residual = trial, trial_state, trial_residual
+        result.newton_iterations += 1
+        if trial_norm <= target and last_step <= NEWTON_STEP_K:
+            return finish(True)
+    return finish(False, "maximum Newton iterations reached")
+
+
+def solve_coupled(
+    system: ConductionSystem,
+    operator: ExchangeOperator,
+    patches: SurfacePatches,
+    T_hot_K: float,
+    T_cold_K: float,
+    epsilon: float,
+    T_initial: np.ndarray,
+) -> CoupledResult:
+    """Coupled solve with 10 preconditioner steps, restarted with 20 if needed (coupled.py:428-447)."""
+
+    for steps in PCG_STEPS:
+        result = newton_krylov(system, operator, patches, T_hot_K, T_cold_K, epsilon, T_initial, steps)
+        if result.converged and result.linear_residuals_pass:
+            break
+    return result
+
+
+def wall_flux_metrics(
+    system: ConductionSystem, patches: SurfacePatches, result: CoupledResult, T_hot_K: float, T_cold_K: float
+) -> dict[str, float]:
+    """Conductive, radiative and total wall fluxes and the acceptance quantities (coupled.py:450-477)."""
+
+    q_cond_hot, q_cond_cold = system.wall_fluxes(result.temperature_K, T_hot_K, T_cold_K)
+    area = cross_section_area(patches)
+    net = result.state.patch_net_power_W
+    q_rad_hot = -float(np.sum(net[patches.patch_class == HOT])) / area
+    q_rad_cold = float(np.sum(net[patches.patch_class == COLD])) / area
+    q_total_hot, q_total_cold = q_cond_hot + q_rad_hot, q_cond_cold + q_rad_cold
+    scale = max(abs(q_total_hot), abs(q_total_cold), _TINY)
+    T = result.temperature_K
+    return {
+        "q_cond_hot_W_m2": q_cond_hot,
+        "q_rad_hot_W_m2": q_rad_hot,
+        "q_total_hot_W_m2": q_total_hot,
+        "q_cond_cold_W_m2": q_cond_cold,
+        "q_rad_cold_W_m2": q_rad_cold,
+        "q_total_cold_W_m2": q_total_cold,
+        "radiative_fraction_hot": q_rad_hot / q_total_hot,
+        "k_eff_apparent_W_mK": q_total_hot * patches.H_m / (float(T_hot_K) - float(T_cold_K)),
+        "wall_flux_mismatch_rel": abs(q_total_hot - q_total_cold) / scale,
+        "energy_residual_rel": abs(result.state.radiosity.global_energy_residual_W) / max(scale * area, _TINY),
+        "T_min_K": float(np.min(T)),
+        "T_max_K": float(np.max(T)),
+        "temperature_bound_violations": int(np.count_nonzero((T < float(T_cold_K)) | (T > float(T_hot_K)))),
+    }
+
+
+def acceptance_failures(result: CoupledResult, metrics: dict[str, float]) -> list[str]:
+    """Acceptance conditions not met (empty when accepted) (coupled.py:480-490)."""
+
+    checks = {
+        "Newton iteration converged": result.converged,
+        "linear residuals within max(1e-13, 1e-8 ||b||)": result.linear_residuals_pass,
+        "wall-flux mismatch <= 0.5 %": metrics["wall_flux_mismatch_rel"] <= 0.005,
+        "global energy residual <= 1e-3": metrics["energy_residual_rel"] <= 1.0e-3,
+        "temperature within the wall temperatures": metrics["temperature_bound_violations"] == 0,
+    }
+    return [name for name, ok in checks.items() if not ok]

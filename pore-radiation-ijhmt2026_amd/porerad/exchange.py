@@ -1,0 +1,307 @@
+"""Projected view-factor operator and gray diffuse radiosity on MI355X.
+
+Drop-in for the reference module porerad/exchange.py.  With raw view factors
+F_ij = n_ij / R_i and patch areas A the exchange matrix is replaced by its
+symmetric part plus a small diagonal support term,
+
+    S = 0.5 (diag(A) F + F^T diag(A)) + delta diag(A),    delta = 1e-15,
+
+and one positive diagonal factor d restores row closure, d * (S d) = A.  The
+projected view factors F_p = diag(1/A) diag(d) S diag(d) are applied as
+F_p x = (d / A) S(d x) and F_p^T x = d S(d x / A).
+
+S lives in HBM as an FP64 CSR built by the HIP library (``prh_operator_*``);
+every product with it, and the reflection series J = E + (1 - eps) F_p^T J,
+runs on the GPU.  The balancing factor is found by Anderson acceleration on
+y = log d (SciPy, exchange.py:175-183) with a damped multiplicative iteration
+as fallback, each residual being one device product.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+from scipy.optimize import NoConvergence, anderson
+
+from . import _native
+from .constants import SIGMA
+
+SUPPORT_DIAGONAL_FRACTION = 1.0e-15
+CLOSURE_TOLERANCE = 1.0e-10
+_TINY = np.finfo(np.float64).tiny
+_LOG_MAX = float(np.log(np.finfo(np.float64).max))
+_LOG_MIN = float(np.log(np.nextafter(0.0, 1.0)))
+
+
+def _merge_edges(origin, receiver, count, n: int):
+    """Sort by (origin, receiver) and sum duplicate pairs (exchange.py:59-67)."""
+
+    o = np.asarray(origin, dtype=np.int64)
+    r = np.asarray(receiver, dtype=np.int64)
+    c = np.asarray(count, dtype=np.int64)
+    key = o * n + r
+    if key.size and not np.all(key[1:] > key[:-1]):
+        order = np.argsort(key, kind="stable")
+        key, c = key[order], c[order]
+        first = np.concatenate(([True], key[1:] != key[:-1]))
+        starts = np.flatnonzero(first)
+        c = np.add.reduceat(c, starts)
+        key = key[starts]
+    return (key // n).astype(np.int32), (key % n).astype(np.int32), c
+
+
+class ExchangeOperator:
+    """Projected view-factor matrix F_p built from integer first-hit counts (exchange.py:37-197).
+
+    ``origin``, ``receiver`` and ``count`` hold the counts n_ij and
+    ``rays_per_origin`` the rays R_i; every row must account for all of its
+    rays (ValueError otherwise).  The projected rows must close to 1e-10
+    (RuntimeError otherwise).
+    """
+
+    def __init__(
+        self,
+        origin: np.ndarray,
+        receiver: np.ndarray,
+        count: np.ndarray,
+        rays_per_origin: np.ndarray,
+        patch_area_m2: np.ndarray,
+        *,
+        support_diagonal_fraction: float = SUPPORT_DIAGONAL_FRACTION,
+        balance_tolerance: float = 1.0e-11,
+        event_chunk_size: int = 4_000_000,
+    ) -> None:
+        rays = np.asarray(rays_per_origin, dtype=np.int64)
+        n = int(rays.size)
+        self.origin, self.receiver, merged = _merge_edges(origin, receiver, count, n)
+        observed = np.bincount(self.origin, weights=merged, minlength=n) if merged.size else np.zeros(n)
+        if observed.shape[0] != n or not np.array_equal(observed.astype(np.int64), rays):
+            raise ValueError("first-hit counts do not account for every emitted ray")
+        self.count = merged
+        self.rays_per_origin = rays
+        self.weight = merged.astype(np.float64) / rays[self.origin]
+        self.patch_area_m2 = np.asarray(patch_area_m2, dtype=np.float64).copy()
+        self.support_diagonal_fraction = float(support_diagonal_fraction)
+        self.event_chunk_size = int(event_chunk_size)
+        self.n_patches = n
+        self.device = _native.Operator(n, self.origin, self.receiver, merged, rays, self.patch_area_m2,
+                                       self.support_diagonal_fraction)
+        self.scale, self.balance_method, self.balance_iterations = self._balance(float(balance_tolerance))
+        self.device.set_scale(self.scale)
+        self.row_sum_error = float(np.max(np.abs(self.matvec(np.ones(n)) - 1.0)))
+        if self.row_sum_error > CLOSURE_TOLERANCE:
+            raise RuntimeError(f"projected rows do not close: {self.row_sum_error:.3e}")
+
+    # -- products (all on the GPU) -------------------------------------------------
+    def raw_matvec(self, x: np.ndarray) -> np.ndarray:
+        """F x (exchange.py:82-93)."""
+
+        return self.device.apply(_native.APPLY_RAW, x)
+
+    def raw_rmatvec(self, x: np.ndarray) -> np.ndarray:
+        """F^T x (exchange.py:95-106)."""
+
+        return self.device.apply(_native.APPLY_RAW_T, x)
+
+    def symmetric_matvec(self, x: np.ndarray) -> np.ndarray:
+        """S x (exchange.py:108-112)."""
+
+        return self.device.apply(_native.APPLY_SYMMETRIC, x)
+
+    def matvec(self, x: np.ndarray) -> np.ndarray:
+        """F_p x = (d / A) S(d x) (exchange.py:114-117)."""
+
+        return self.device.apply(_native.APPLY_MATVEC, x)
+
+    def rmatvec(self, x: np.ndarray) -> np.ndarray:
+        """F_p^T x = d S(d x / A) (exchange.py:119-122)."""
+
+        return self.device.apply(_native.APPLY_RMATVEC, x)
+
+    def reciprocity_error(self) -> float:
+        """Largest relative defect of A_i F_p,ij = A_j F_p,ji over the sampled pairs (exchange.py:124-140)."""
+
+        n, d, A = self.n_patches, self.scale, self.patch_area_m2
+        key = self.origin.astype(np.int64) * n + self.receiver
+        back = self.receiver.astype(np.int64) * n + self.origin
+        at = np.minimum(np.searchsorted(key, back), key.size - 1)
+        w_back = np.where(key[at] == back, self.weight[at], 0.0)
+        i, j = self.origin, self.receiver
+        s_ij = 0.5 * (A[i] * self.weight + A[j] * w_back)
+        s_ji = 0.5 * (A[j] * w_back + A[i] * self.weight)
+        on_diag = i == j
+        s_ij[on_diag] += self.support_diagonal_fraction * A[i[on_diag]]
+        s_ji[on_diag] += self.support_diagonal_fraction * A[j[on_diag]]
+        pair_ij = A[i] * ((d[i] / A[i]) * s_ij * d[j])
+        pair_ji = A[j] * ((d[j] / A[j]) * s_ji * d[i])
+        return float(np.max(np.abs(pair_ij - pair_ji)) / max(float(np.max(np.abs(pair_ij))), _TINY))
+
+    # -- balancing ------------------------------------------------------------------
+    def _closure_error(self, y: np.ndarray) -> float:
+        """max |d (S d) / A - 1| for d = exp(y) (exchange.py:142-151)."""
+
+        if not (np.all(np.isfinite(y)) and np.max(y) < _LOG_MAX and np.min(y) > _LOG_MIN):
+            return float("inf")
+        d = np.exp(y)
+        row = d * self.symmetric_matvec(d)
+        if not np.all(np.isfinite(row)) or np.any(row <= 0.0):
+            return float("inf")
+        return float(np.max(np.abs(row / self.patch_area_m2 - 1.0)))
+
+    def _balance(self, tolerance: float) -> tuple[np.ndarray, str, int]:
+        """Solve d * (S d) = A for d > 0 (exchange.py:153-197)."""
+
+        n = self.n_patches
+        log_area = np.log(self.patch_area_m2)
+        y0 = np.zeros(n, dtype=np.float64)
+        if self._closure_error(y0) <= tolerance:
+            return np.exp(y0), "none", 0
+        calls = [0]
+
+        def residual(y: np.ndarray) -> np.ndarray:
+            y = np.asarray(y, dtype=np.float64)
+            if not (np.all(np.isfinite(y)) and np.max(y) < _LOG_MAX and np.min(y) > _LOG_MIN):
+                return np.full(n, np.inf)
+            product = self.symmetric_matvec(np.exp(y))
+            if not np.all(np.isfinite(product)) or np.any(product <= 0.0):
+                return np.full(n, np.inf)
+            return y + np.log(product) - log_area
+
+        def on_iteration(_x, _f) -> None:
+            calls[0] += 1
+
+        try:
+            y = np.asarray(anderson(residual, y0, M=10, w0=0.01, maxiter=500, f_tol=tolerance,
+                                    callback=on_iteration), dtype=np.float64)
+            accept = min(max(10.0 * tolerance, 10.0 * np.finfo(float).eps), CLOSURE_TOLERANCE)
+            if self._closure_error(y) <= accept:
+                return np.exp(y), "anderson", calls[0]
+        except (NoConvergence, ValueError, FloatingPointError, np.linalg.LinAlgError):
+            pass
+
+        y = y0.copy()
+        for step in range(1, 20_001):
+            d = np.exp(y)
+            row = d * self.symmetric_matvec(d)
+            if not np.all(np.isfinite(row)) or np.any(row <= 0.0):
+                break
+            y = y + 0.5 * (log_area - np.log(row))
+            err = self._closure_error(y)
+            if not np.isfinite(err):
+                break
+            if err <= min(tolerance, CLOSURE_TOLERANCE):
+                return np.exp(y), "multiplicative", step
+        raise RuntimeError("diagonal balancing of the exchange matrix did not converge")
+
+
+@dataclass
+class RadiosityResult:
+    """Emitted, absorbed and net (absorbed minus emitted) patch power (exchange.py:200-214)."""
+
+    emitted_power_W: np.ndarray
+    absorbed_power_W: np.ndarray
+    net_power_W: np.ndarray
+    residual_power_W: float
+    iterations: int
+
+    @property
+    def global_energy_residual_W(self) -> float:
+        """Net power of all patches plus the power still in flight."""
+
+        return float(np.sum(self.net_power_W) + self.residual_power_W)
+
+
+def _emissivity_vector(emissivity, n: int) -> np.ndarray:
+    return np.ascontiguousarray(np.broadcast_to(np.asarray(emissivity, dtype=np.float64), (n,)))
+
+
+def reflection_series(
+    operator: ExchangeOperator,
+    emitted_W: np.ndarray,
+    emissivity: np.ndarray,
+    *,
+    relative_tolerance: float,
+    max_reflections: int = 10000,
+) -> RadiosityResult:
+    """Absorbed power after all diffuse reflections of a (possibly signed) emission (exchange.py:217-245).
+
+    Power in flight is passed on by F_p^T; each patch absorbs eps and reflects
+    1 - eps.  Stops when ||flight||_1 <= relative_tolerance ||E||_1; raises
+    RuntimeError after ``max_reflections`` reflections.
+    """
+
+    emitted = np.asarray(emitted_W, dtype=np.float64)
+    n = operator.n_patches
+    eps = _emissivity_vector(emissivity, n)
+    absorbed, residual, iterations = operator.device.radiosity(emitted.reshape(1, n), eps, relative_tolerance,
+                                                               max_reflections)
+    absorbed = absorbed[0]
+    return RadiosityResult(emitted, absorbed, absorbed - emitted, float(residual[0]), int(iterations[0]))
+
+
+def reflection_series_batch(
+    operator: ExchangeOperator,
+    emitted_W: np.ndarray,
+    emissivity: np.ndarray,
+    *,
+    relative_tolerance: float,
+    max_reflections: int = 10000,
+) -> list[RadiosityResult]:
+    """``reflection_series`` for k emissions (rows of ``emitted_W``) in one device call.
+
+    ``emissivity`` is one vector for all rows or one row per emission.
+    """
+
+    E = np.ascontiguousarray(np.atleast_2d(emitted_W), dtype=np.float64)
+    n = operator.n_patches
+    eps = np.asarray(emissivity, dtype=np.float64)
+    eps = np.ascontiguousarray(np.broadcast_to(eps, E.shape) if eps.ndim == 2 else _emissivity_vector(eps, n))
+    absorbed, residual, iterations = operator.device.radiosity(E, eps, relative_tolerance, max_reflections)
+    return [RadiosityResult(E[q], absorbed[q], absorbed[q] - E[q], float(residual[q]), int(iterations[q]))
+            for q in range(E.shape[0])]
+
+
+def emission(emissivity, area_m2, temperature_K, reference_temperature_K: float | None = None) -> np.ndarray:
+    """eps sigma A T^4, or eps sigma A (T - T_ref)(T + T_ref)(T^2 + T_ref^2) (exchange.py:265-270)."""
+
+    T = np.asarray(temperature_K, dtype=np.float64)
+    if reference_temperature_K is None:
+        return emissivity * SIGMA * area_m2 * T**4
+    ref = float(reference_temperature_K)
+    return emissivity * SIGMA * area_m2 * ((T - ref) * (T + ref) * (T * T + ref * ref))
+
+
+def net_patch_power(
+    operator: ExchangeOperator,
+    temperature_K: np.ndarray,
+    emissivity: np.ndarray,
+    area_m2: np.ndarray,
+    *,
+    relative_tolerance: float,
+    reference_temperature_K: float | None = None,
+) -> RadiosityResult:
+    """Net radiative power of every patch for given patch temperatures (exchange.py:248-271).
+
+    With ``reference_temperature_K`` the isothermal state at that temperature
+    (zero net power) is subtracted exactly before the series is summed.
+    """
+
+    emitted = emission(emissivity, area_m2, temperature_K, reference_temperature_K)
+    return reflection_series(operator, emitted, emissivity, relative_tolerance=relative_tolerance)
+
+
+def net_patch_power_jvp(
+    operator: ExchangeOperator,
+    temperature_K: np.ndarray,
+    delta_temperature_K: np.ndarray,
+    emissivity: np.ndarray,
+    area_m2: np.ndarray,
+    *,
+    relative_tolerance: float,
+) -> np.ndarray:
+    """Derivative of the net patch power along a patch temperature perturbation (exchange.py:274-287)."""
+
+    derivative = 4.0 * emissivity * SIGMA * area_m2 * temperature_K**3 * delta_temperature_K
+    response = reflection_series(operator, derivative, emissivity, relative_tolerance=relative_tolerance)
+    return response.absorbed_power_W - derivative

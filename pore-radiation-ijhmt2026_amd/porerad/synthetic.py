@@ -1,0 +1,155 @@
+"""Deterministic synthetic microstructures for the benchmark workloads.
+
+These generators are this package's own (the reference builds its geometries
+with porerad/geometry.py, which is not part of the radiation path); they only
+produce inputs in the reference's format: uint8 arrays (nx, ny, nz) with
+1 = solid, 0 = pore, periodic in every direction.
+
+  random_fibers   straight cylinders of given radius, isotropic uniform axes
+                  and uniform centres, added until the solid fraction is met
+  voronoi_foam    open-cell foam: struts along the edges of the periodic
+                  Voronoi tessellation of uniform seeds, strut radius bisected
+                  to the solid fraction
+  aligned_fibers  cylinders along z with Poisson-disk centres
+  layered_plates  plates normal to x with square holes (tortuous layers)
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def _centres(n: int) -> np.ndarray:
+    return np.arange(n, dtype=np.float64) + 0.5
+
+
+def random_fibers(N: int, solid_fraction: float, radius: float, seed: int, max_fibers: int = 100000) -> np.ndarray:
+    """Straight cylinders (length N, minimum-image periodic) until the solid fraction is reached."""
+
+    rng = np.random.default_rng(seed)
+    c = _centres(N)
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    P = np.stack((X, Y, Z), axis=-1).reshape(-1, 3)
+    solid = np.zeros(P.shape[0], dtype=bool)
+    half = N / 2.0
+    for _ in range(max_fibers):
+        if solid.mean() >= solid_fraction:
+            break
+        centre = rng.uniform(0.0, N, size=3)
+        axis = rng.normal(size=3)
+        axis /= np.linalg.norm(axis)
+        v = P - centre
+        v -= N * np.round(v / N)
+        t = np.clip(v @ axis, -half, half)
+        dist2 = np.sum((v - t[:, None] * axis[None, :]) ** 2, axis=1)
+        solid |= dist2 <= radius * radius
+    return solid.reshape(N, N, N).astype(np.uint8)
+
+
+def _segment_distance_field(N: int, segments: np.ndarray, reach: float) -> np.ndarray:
+    """Distance from every voxel centre to the nearest segment, capped at ``reach``."""
+
+    c = _centres(N)
+    best = np.full((N, N, N), np.inf)
+    for a, b in segments:
+        lo = np.floor(np.minimum(a, b) - reach).astype(int)
+        hi = np.ceil(np.maximum(a, b) + reach).astype(int)
+        lo = np.maximum(lo, 0)
+        hi = np.minimum(hi, N)
+        if np.any(hi <= lo):
+            continue
+        xs, ys, zs = c[lo[0]:hi[0]], c[lo[1]:hi[1]], c[lo[2]:hi[2]]
+        X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+        ab = b - a
+        L2 = float(ab @ ab)
+        px, py, pz = X - a[0], Y - a[1], Z - a[2]
+        t = (px * ab[0] + py * ab[1] + pz * ab[2]) / L2 if L2 > 0 else np.zeros_like(px)
+        t = np.clip(t, 0.0, 1.0)
+        d = np.sqrt((px - t * ab[0]) ** 2 + (py - t * ab[1]) ** 2 + (pz - t * ab[2]) ** 2)
+        view = best[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+        np.minimum(view, d, out=view)
+    return best
+
+
+def voronoi_foam(N: int, n_seeds: int, solid_fraction: float, seed: int, reach: float = 10.0) -> np.ndarray:
+    """Open-cell foam from the edges of the periodic Voronoi tessellation of ``n_seeds`` uniform points."""
+
+    from scipy.spatial import Voronoi
+
+    rng = np.random.default_rng(seed)
+    pts = rng.uniform(0.0, N, size=(n_seeds, 3))
+    shifts = np.array([(i, j, k) for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)], dtype=np.float64)
+    images = (pts[None, :, :] + N * shifts[:, None, :]).reshape(-1, 3)
+    vor = Voronoi(images)
+    verts = vor.vertices
+    edges = set()
+    for ridge in vor.ridge_vertices:
+        if -1 in ridge:
+            continue
+        for u, v in zip(ridge, ridge[1:] + ridge[:1]):
+            edges.add((min(u, v), max(u, v)))
+    segs = []
+    lo, hi = -reach, N + reach
+    for u, v in sorted(edges):
+        a, b = verts[u], verts[v]
+        if np.all(np.minimum(a, b) < hi) and np.all(np.maximum(a, b) > lo):
+            segs.append((a, b))
+    dist = _segment_distance_field(N, segs, reach)
+    target = int(round(solid_fraction * N**3))
+    r_lo, r_hi = 0.0, reach
+    for _ in range(60):
+        r = 0.5 * (r_lo + r_hi)
+        if np.count_nonzero(dist <= r) < target:
+            r_lo = r
+        else:
+            r_hi = r
+    return (dist <= r_hi).astype(np.uint8)
+
+
+def aligned_fibers(N: int, solid_fraction: float, radius: float, seed: int, gap: float = 1.0) -> np.ndarray:
+    """Cylinders along z with Poisson-disk centres (minimum spacing 2 r + gap), periodic in x and y."""
+
+    rng = np.random.default_rng(seed)
+    c = _centres(N)
+    X, Y = np.meshgrid(c, c, indexing="ij")
+    section = np.zeros((N, N), dtype=bool)
+    centres: list[np.ndarray] = []
+    min_d = 2.0 * radius + gap
+    attempts = 0
+    while section.mean() < solid_fraction and attempts < 200000:
+        attempts += 1
+        p = rng.uniform(0.0, N, size=2)
+        ok = True
+        for q in centres:
+            d = p - q
+            d -= N * np.round(d / N)
+            if d @ d < min_d * min_d:
+                ok = False
+                break
+        if not ok:
+            continue
+        centres.append(p)
+        dx = X - p[0]
+        dy = Y - p[1]
+        dx -= N * np.round(dx / N)
+        dy -= N * np.round(dy / N)
+        section |= dx * dx + dy * dy <= radius * radius
+    return np.broadcast_to(section[:, :, None], (N, N, N)).astype(np.uint8).copy()
+
+
+def layered_plates(N: int, solid_fraction: float, seed: int, thickness: int = 2, hole: int = 6,
+                   hole_fraction: float = 0.15) -> np.ndarray:
+    """Plates normal to x, ``thickness`` cells thick, with random square holes covering ``hole_fraction``."""
+
+    rng = np.random.default_rng(seed)
+    period = max(thickness + 1, int(round(thickness * (1.0 - hole_fraction) / solid_fraction)))
+    out = np.zeros((N, N, N), dtype=np.uint8)
+    for start in range(period // 2, N - thickness + 1, period):
+        plate = np.ones((N, N), dtype=bool)
+        while 1.0 - plate.mean() < hole_fraction:
+            j, k = rng.integers(0, N, size=2)
+            jj = (j + np.arange(hole)) % N
+            kk = (k + np.arange(hole)) % N
+            plate[np.ix_(jj, kk)] = False
+        out[start:start + thickness] = plate[None, :, :]
+    return out

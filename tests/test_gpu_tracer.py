@@ -1,0 +1,175 @@
+"""Parity of the HIP tracer with the CPU oracle and the reference's archived hashes.
+
+All calls go through the C ABI (libporerad_hip.so via porerad._native).
+Integer outputs must be bit-identical: patch tables, receivers, counts.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from golden_data import cases, phases_x
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def vf():
+    from porerad import _native, view_factors
+
+    _native.lib()
+    return view_factors
+
+
+_cache = {}
+
+
+def _both(vf, key, phases, H=0.01):
+    if key not in _cache:
+        _cache.clear()
+        _cache[key] = (vf.build_surface_patches(phases, H), O.build_surface_patches(phases, H))
+    return _cache[key]
+
+
+def _fiber_c1():
+    from porerad import synthetic
+
+    return synthetic.random_fibers(64, 0.10, 3.0, 0)
+
+
+def _small_random(seed=0, shape=(12, 10, 11), density=0.2):
+    rng = np.random.default_rng(seed)
+    return (rng.random(shape) < density).astype(np.uint8)
+
+
+@pytest.mark.parametrize("name,axis", [("OF_canonical_phi0.10_N128", "x"), ("FI_canonical_phi0.10_N128", "z"),
+                                       ("synthetic_fiber_N64", "x"), ("small_random", "x")])
+def test_patch_table_matches_oracle(vf, name, axis):
+    if name == "synthetic_fiber_N64":
+        phases = _fiber_c1()
+    elif name == "small_random":
+        phases = _small_random()
+    else:
+        phases = phases_x(name, axis)
+    P, Q = _both(vf, (name, axis), phases)
+    assert P.n_patches == Q.n_patches
+    for field in ("patch_class", "patch_cell", "patch_face", "patch_origin", "patch_normal", "cell_face_patch",
+                  "hot_wall_patch", "cold_wall_patch"):
+        a, b = getattr(P, field), getattr(Q, field)
+        assert a.dtype == b.dtype or field == "patch_normal", field
+        assert np.array_equal(a, b), field
+
+
+def test_generate_rays_matches_oracle(vf):
+    phases = phases_x("OF_canonical_phi0.10_N128", "x")
+    P, Q = _both(vf, ("OF_canonical_phi0.10_N128", "x"), phases)
+    n = P.n_patches
+    for p0, p1 in ((0, 3000), (n - 3000, n)):
+        o_gpu, d_gpu = vf.generate_rays(P, p0, p1, 64, 4734788061198622954)
+        o_ref, d_ref = O.generate_rays(Q, p0, p1, 64, 4734788061198622954)
+        assert np.array_equal(o_gpu, o_ref)
+        ulp = np.abs(d_gpu.view(np.int64) - d_ref.view(np.int64))
+        mismatch = float(np.mean(ulp != 0))
+        print(f"direction components differing from glibc: {mismatch:.2e} (max {int(ulp.max())} ulp)")
+        assert ulp.max() <= 2
+        assert mismatch < 0.01
+
+
+def test_trace_given_rays_matches_oracle(vf):
+    phases = phases_x("FA_canonical_phi0.10_N128", "y")
+    P, Q = _both(vf, ("FA_canonical_phi0.10_N128", "y"), phases)
+    o, d = O.generate_rays(Q, 0, P.n_patches, 4, 17)
+    rec_ref, _ = O.trace_rays(Q, o, d)
+    assert np.all(rec_ref >= 0)
+    rec = vf.trace_first_hit(P, o, d)
+    assert np.array_equal(rec, rec_ref)
+
+
+def _hash_params():
+    out = []
+    for h in cases()["hash_cases"]:
+        for s in "AB":
+            out.append(pytest.param(h, s, id=f"{h['structure']}-{h['axis']}-{s}"))
+    return out
+
+
+@pytest.mark.parametrize("case,which", _hash_params())
+def test_counts_reproduce_archived_hash(vf, case, which):
+    phases = phases_x(case["microstructure"], case["axis"])
+    key = (case["microstructure"], case["axis"])
+    if key not in _cache:
+        _cache.clear()
+        _cache[key] = (vf.build_surface_patches(phases, case["H_m"]), None)
+    P = _cache[key][0]
+    R = case["rays_per_patch_per_block"]
+    seeds = case[f"seeds_{which}"]
+    o, r, c = vf.first_hit_counts_pooled(P, R, seeds)
+    rays = np.full(P.n_patches, R * len(seeds), dtype=np.int64)
+    assert vf.count_sha256(o, r, c, rays) == case[f"raw_count_sha256_{which}"]
+
+
+def test_c1_fibre_counts_and_reads_bit_exact(vf):
+    from porerad import _native
+
+    phases = _fiber_c1()
+    P, Q = _both(vf, ("synthetic_fiber_N64", "x"), phases)
+    edges = vf.first_hit_edges(P, 200, [1])
+    got = edges.to_host()
+    reads = edges.occupancy_reads
+    edges.close()
+    (o, r, c), reads_ref = O.first_hit_counts_range(Q, 200, 1, 0, Q.n_patches)
+    assert np.array_equal(got[0], o) and np.array_equal(got[1], r) and np.array_equal(got[2], c)
+    assert got[2].dtype == np.int64
+    assert reads == reads_ref
+    assert np.all(_native.last_timings()[:2] > 0)
+
+
+def test_single_seed_api_equals_pooled_singles(vf):
+    phases = _small_random(3, (14, 12, 12), 0.15)
+    P, Q = _both(vf, ("small3", "x"), phases)
+    seeds = [5, 99, 2**63 + 7]
+    tables = [vf.first_hit_counts(P, 37, s) for s in seeds]
+    pooled = vf.first_hit_counts_pooled(P, 37, seeds)
+    ref = O.pool_counts([O.first_hit_counts(Q, 37, s) for s in seeds], Q.n_patches)
+    for a, b, c in zip(vf.pool_counts(tables, P.n_patches), pooled, ref):
+        assert np.array_equal(a, c) and np.array_equal(b, c)
+
+
+@pytest.mark.parametrize("R", [1, 7, 64, 1000, 4096])
+def test_partial_ranges_and_ray_counts(vf, R):
+    phases = _small_random(4, (16, 9, 13), 0.2)
+    P, Q = _both(vf, ("small4", "x"), phases)
+    n = P.n_patches
+    for p0, p1 in ((0, n), (n // 3, n // 3 + 17), (n - 1, n), (5, 5)):
+        got = vf.first_hit_counts_pooled(P, R, [123], patch_range=(p0, p1))
+        (o, r, c), _ = O.first_hit_counts_range(Q, R, 123, p0, p1)
+        assert all(np.array_equal(x, y) for x, y in zip(got, (o, r, c))), (p0, p1)
+
+
+def test_counts_are_deterministic(vf):
+    phases = _small_random(5, (20, 16, 16), 0.12)
+    P, _ = _both(vf, ("small5", "x"), phases)
+    a = vf.first_hit_counts_pooled(P, 300, [8, 9])
+    b = vf.first_hit_counts_pooled(P, 300, [8, 9])
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def test_unresolved_rays_raise(vf):
+    phases = np.zeros((4, 4, 4), np.uint8)
+    phases[1, 1, 1] = 1
+    P = vf.build_surface_patches(phases, 0.004)
+    o = np.array([[0.5, 0.5, 0.5], [1.5, 1.5, 1.5], [0.5, 0.5, 0.5]])
+    d = np.array([[0.0, 1.0, 0.0], [1.0, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    with pytest.raises(RuntimeError, match="2 rays did not resolve a receiving patch"):
+        vf.trace_first_hit(P, o, d)
+
+
+def test_empty_box_counts(vf):
+    phases = np.zeros((8, 6, 6), np.uint8)
+    P, Q = _both(vf, ("empty", "x"), phases, 0.008)
+    got = vf.first_hit_counts(P, 50, 3)
+    ref = O.first_hit_counts(Q, 50, 3)
+    assert all(np.array_equal(x, y) for x, y in zip(got, ref))
+    assert np.all(P.patch_class[got[0]] != P.patch_class[got[1]])

@@ -1,0 +1,88 @@
+"""Host-side logic and the C ABI library, without GPU compute.
+
+* libporerad_hip.so loads and exports every entry point declared in
+  include/porerad_hip.h, and the ctypes table declares each of them.
+* prh_pcg64_seed (host code of the library) equals numpy's PCG64 seeding.
+* count_sha256 / pool_counts of the drop-in package equal the oracle's and
+  hashlib's digests.
+"""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "porerad_hip.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(prh_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from porerad import _native
+
+    assert os.path.exists(_native.LIB_PATH), "build the library first (__graft_entry__.build())"
+    raw = ctypes.CDLL(_native.LIB_PATH)
+    declared = _declared()
+    assert len(declared) >= 18
+    for name in declared:
+        assert hasattr(raw, name), name
+        assert name in _native.SIGNATURES, name
+    _native.load()  # every declared signature binds
+
+
+def test_library_seeding_matches_numpy():
+    from porerad import _native
+
+    for seed in [0, 5, 2**40 + 1, 4734788061198622954, 2**70 + 3]:
+        st = np.random.PCG64(seed).state["state"]
+        ref = [st["state"] >> 64, st["state"] & (2**64 - 1), st["inc"] >> 64, st["inc"] & (2**64 - 1)]
+        assert [int(v) for v in _native.pcg64_state(seed)] == ref
+
+
+def test_count_sha256_matches_oracle_and_hashlib():
+    import hashlib
+
+    from porerad import view_factors as vf
+
+    o = np.array([0, 0, 1, 2], np.int32)
+    r = np.array([1, 2, 0, 0], np.int32)
+    c = np.array([3, 1, 4, 4], np.int64)
+    rays = np.full(3, 4, np.int64)
+    h = hashlib.sha256()
+    for name, a in (("origin_patch_id", o), ("receiver_patch_id", r), ("count", c), ("rays_per_origin", rays)):
+        h.update(name.encode())
+        h.update(a.dtype.str.encode())
+        h.update(np.asarray(a.shape, np.int64).tobytes())
+        h.update(a.tobytes())
+    assert vf.count_sha256(o, r, c, rays) == h.hexdigest() == O.count_sha256(o, r, c, rays)
+
+
+def test_pool_counts_matches_oracle():
+    from porerad import view_factors as vf
+
+    rng = np.random.default_rng(0)
+    tables = []
+    for _ in range(3):
+        o = np.sort(rng.integers(0, 50, 400)).astype(np.int32)
+        r = rng.integers(0, 50, 400).astype(np.int32)
+        c = rng.integers(1, 5, 400).astype(np.int64)
+        tables.append((o, r, c))
+    a = vf.pool_counts(tables, 50)
+    b = O.pool_counts(tables, 50)
+    for x, y in zip(a, b):
+        assert x.dtype == y.dtype and np.array_equal(x, y)
+
+
+def test_native_errors_map_to_reference_exceptions():
+    from porerad import _native
+
+    with pytest.raises(ValueError):
+        _native.seed_words(-1)
