@@ -104,16 +104,17 @@ def cpu_baseline(phases, R, seed, budget_s=12.0):
     Q = O.build_surface_patches(phases, 0.01)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     n_p = 200
-    start = Q.n_patches // 3
+    start = 0
     done_rays, spent = 0, 0.0
-    while spent < budget_s and start + n_p <= Q.n_patches:
+    while spent < budget_s:
+        n_p = min(n_p, Q.n_patches - start)
         t = time.perf_counter()
         O.first_hit_counts_range(Q, R, seed, start, start + n_p)
         dt = time.perf_counter() - t
         done_rays += n_p * R
         spent += dt
-        start += n_p
-        n_p = int(min(max(n_p, n_p * 0.5 * (budget_s - spent) / max(dt, 1e-3)), 20000))
+        start = (start + n_p) % Q.n_patches
+        n_p = int(min(max(n_p, n_p * 0.5 * (budget_s - spent) / max(dt, 1e-3)), 50000))
     return {
         "value": float(done_rays / spent),
         "unit": "rays/s",
