@@ -48,18 +48,16 @@ def run_tracer(P, R, seed, steps, warmup):
 
     for _ in range(warmup):
         view_factors.first_hit_edges(P, R, [seed]).close()
-    wall, kernel, count_ms = [], [], []
+    wall, parts = [], []
     reads = nnz = 0
     for _ in range(steps):
         t0 = time.perf_counter()
         e = view_factors.first_hit_edges(P, R, [seed])
         wall.append(time.perf_counter() - t0)
-        ms = _native.last_timings()
-        kernel.append(ms[0] * 1e-3)
-        count_ms.append((ms[1] + ms[2]) * 1e-3)
+        parts.append(_native.last_timings() * 1e-3)
         reads, nnz = e.occupancy_reads, e.nnz
         e.close()
-    return np.array(wall), np.array(kernel), np.array(count_ms), reads, nnz
+    return np.array(wall), np.array(parts), reads, nnz
 
 
 def run_radiosity(P, R, seed, eps_list, repeats):
@@ -133,6 +131,7 @@ def main() -> None:
     ap.add_argument("--rays", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-radiosity", action="store_true", help="skip the radiosity solves")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tracer variant")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU; multi-GPU runs are not part of this build "
@@ -149,12 +148,22 @@ def main() -> None:
     R, seed = args.rays, 3
     n_rays = P.n_patches * R
 
-    wall, kernel, count_s, reads, nnz = run_tracer(P, R, seed, args.steps, args.warmup)
+    tune = None
+    if not args.no_tune:
+        from porerad import autotune
+
+        tune = autotune.tune_tracer(P, R, [seed], use_cache=False)
+        log("tracer variants: " + ", ".join(f"v{r['variant']}(refill {r['refill_min']}, wpe {r['waves_per_eu']})="
+                                            f"{r['trace_ms']:.2f}ms" for r in tune["table"]))
+    wall, parts, reads, nnz = run_tracer(P, R, seed, args.steps, args.warmup)
     step = float(np.mean(wall))
-    log(f"trace step {step * 1e3:.2f} ms (kernel {np.mean(kernel) * 1e3:.2f} ms, counting "
-        f"{np.mean(count_s) * 1e3:.2f} ms), {nnz} edges, {reads / n_rays:.1f} occupancy reads/ray")
+    gen_s, trace_s, count_s, compact_s = (float(x) for x in parts.mean(axis=0))
+    log(f"trace step {step * 1e3:.2f} ms (generation {gen_s * 1e3:.2f}, traversal {trace_s * 1e3:.2f}, "
+        f"counting {count_s * 1e3:.2f}, compaction {compact_s * 1e3:.2f} ms), {nnz} edges, "
+        f"{reads / n_rays:.1f} occupancy reads/ray")
+    kernel = trace_s
     algo_bytes = 4.0 * n_rays + float(reads)
-    achieved = algo_bytes / float(np.mean(kernel)) / 1e9
+    achieved = algo_bytes / kernel / 1e9
     record = {
         "metric": METRIC,
         "value": n_rays / step,
@@ -178,13 +187,17 @@ def main() -> None:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": None,
-            "kernel": "k_trace_fused",
-            "kernel_ms": float(np.mean(kernel) * 1e3),
+            "kernel": "k_trace_stream",
+            "kernel_ms": kernel * 1e3,
             "algorithmic_bytes_per_launch": algo_bytes,
             "occupancy_reads_per_ray": reads / n_rays,
-            "dda_steps_per_s": float(reads) / float(np.mean(kernel)),
+            "dda_steps_per_s": float(reads) / kernel,
+            "stage_ms": {"generation": gen_s * 1e3, "traversal": trace_s * 1e3, "counting": count_s * 1e3,
+                         "compaction": compact_s * 1e3},
         },
     }
+    if tune is not None:
+        record["tracer_variant"] = {"chosen": tune["variant"], "table": tune["table"]}
     if not args.no_radiosity:
         record["radiosity"] = run_radiosity(P, R, seed, [0.3, 0.6, 0.9], 2)
         log(f"radiosity: {record['radiosity']['solves_per_s']:.1f} solves/s, p50 "

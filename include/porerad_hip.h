@@ -68,9 +68,18 @@ const char *prh_last_error(void);
 const char *prh_version(void);
 
 /* Kernel durations (ms, HIP events on the library stream) of the last
- * prh_trace_counts call: [0] fused generate+trace kernel, [1] per-patch
- * count kernel, [2] edge compaction.  n = number of slots to fill. */
+ * prh_trace_counts call: [0] ray generation, [1] traversal, [2] per-patch
+ * counting, [3] scan + edge compaction.  n = number of slots to fill. */
 int prh_last_timings(double *ms, int32_t n);
+
+/* Tuning knob of the traversal kernel: variant index in [0, n_variants).
+ * Each variant fixes the refill threshold (idle lanes of a wavefront before
+ * it loads new rays) and the register budget (waves per SIMD).  No
+ * counterpart in the reference (a launch parameter of this implementation);
+ * results do not depend on it.  prh_tracer_variants fills the tables when
+ * the pointers are non-NULL (arrays of n_variants entries). */
+int prh_tracer_set_variant(int32_t variant);
+int prh_tracer_variants(int32_t *n_variants, int32_t *current, int32_t *refill_min, int32_t *waves_per_eu);
 
 /* numpy.random.PCG64(seed) seeding: SeedSequence over the little-endian
  * uint32 words of the seed.  state_inc = {state_hi, state_lo, inc_hi, inc_lo}.

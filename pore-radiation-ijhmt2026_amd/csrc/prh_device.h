@@ -16,7 +16,7 @@ constexpr double TWO_PI = 2.0 * 3.141592653589793;
 // np.remainder(a, b) for b > 0: fmod, plus b when the signs differ, +0.0 for
 // an exact zero.  Origins lie within one period of [0, b), where the
 // remainder is exact (Sterbenz) and needs no library fmod.
-__host__ __device__ inline double py_mod(double a, double b)
+__host__ __device__ __forceinline__ double py_mod(double a, double b)
 {
     double m;
     if (a >= 0.0 && a < b) m = a;
@@ -31,7 +31,7 @@ __host__ __device__ inline double py_mod(double a, double b)
     return m;
 }
 
-__device__ inline int64_t py_imod64(int64_t a, int64_t b)
+__device__ __forceinline__ int64_t py_imod64(int64_t a, int64_t b)
 {
     int64_t m = a % b;
     if (m < 0) m += b;
@@ -39,7 +39,7 @@ __device__ inline int64_t py_imod64(int64_t a, int64_t b)
 }
 
 // PCG64 XSL-RR output after one LCG step.
-__device__ inline uint64_t pcg_next(u128 &s, const u128 inc)
+__device__ __forceinline__ uint64_t pcg_next(u128 &s, const u128 inc)
 {
     const u128 mult = make_u128(PCG_MULT_HI, PCG_MULT_LO);
     s = add128(mul128(s, mult), inc);
@@ -49,13 +49,13 @@ __device__ inline uint64_t pcg_next(u128 &s, const u128 inc)
 }
 
 // Generator.random(): (next64 >> 11) * 2^-53.
-__device__ inline double pcg_uniform(u128 &s, const u128 inc)
+__device__ __forceinline__ double pcg_uniform(u128 &s, const u128 inc)
 {
     return (double)(pcg_next(s, inc) >> 11) * (1.0 / 9007199254740992.0);
 }
 
 // Advance the LCG by delta steps using the 2^b jump tables.
-__device__ inline u128 pcg_jump(u128 s, const u128 inc, uint64_t delta, const JumpTables &T)
+__device__ __forceinline__ u128 pcg_jump(u128 s, const u128 inc, uint64_t delta, const JumpTables &T)
 {
     for (int b = 0; delta; ++b, delta >>= 1) {
         if (delta & 1u) s = add128(mul128(T.A[b], s), mul128(T.G[b], inc));
@@ -63,7 +63,7 @@ __device__ inline u128 pcg_jump(u128 s, const u128 inc, uint64_t delta, const Ju
     return s;
 }
 
-__device__ inline double clip_uniform(double u)
+__device__ __forceinline__ double clip_uniform(double u)
 {
     const double lo = FACE_EDGE_INSET, hi = 1.0 - FACE_EDGE_INSET;
     return u < lo ? lo : (u > hi ? hi : u);
@@ -71,7 +71,7 @@ __device__ inline double clip_uniform(double u)
 
 // One Lambertian ray of a patch (generate_rays, view_factors.py:147-181).
 // normal_code: 0..5 = +x, -x, +y, -y, +z, -z (walls: hot 0, cold 1).
-__device__ inline void make_ray(u128 &s, const u128 inc, const double po0, const double po1, const double po2,
+__device__ __forceinline__ void make_ray(u128 &s, const u128 inc, const double po0, const double po1, const double po2,
                                 int normal_code, double fny, double fnz, double o[3], double d[3])
 {
     const double u0 = pcg_uniform(s, inc);
@@ -124,7 +124,7 @@ __device__ inline void make_ray(u128 &s, const u128 inc, const double po0, const
 }
 
 // _initial_cell, view_factors.py:185-190 (round() = round-half-even).
-__device__ inline int initial_cell(double position, double direction)
+__device__ __forceinline__ int initial_cell(double position, double direction)
 {
     int index = (int)floor(position);
     if (direction < 0.0 && fabs(position - rint(position)) <= TIE_TOLERANCE) index -= 1;
@@ -132,7 +132,7 @@ __device__ inline int initial_cell(double position, double direction)
 }
 
 // _axis_state, view_factors.py:193-199.
-__device__ inline void axis_state(double position, double direction, int index, int &step, double &t_max,
+__device__ __forceinline__ void axis_state(double position, double direction, int index, int &step, double &t_max,
                                   double &t_delta)
 {
     if (direction > 0.0) {
@@ -151,7 +151,7 @@ __device__ inline void axis_state(double position, double direction, int index, 
 }
 
 // Step cap of _trace_kernel, view_factors.py:223-224.
-__device__ inline uint32_t max_steps_of(double dx, double dy, double dz, int nx, int ny, int nz)
+__device__ __forceinline__ uint32_t max_steps_of(double dx, double dy, double dz, int nx, int ny, int nz)
 {
     const double crossing_scale = 8.0 * (double)nx * (1.0 + (fabs(dy) + fabs(dz)) / fabs(dx)) + 100.0;
     const double capped = crossing_scale < 100000000.0 ? crossing_scale : 100000000.0;

@@ -7,17 +7,18 @@
 // FP64 operations one for one (no contraction), so the integer counts are
 // bit-identical to the reference CPU implementation on the same seeds.
 //
-// Work decomposition of the fused kernel (k_trace_fused):
-//   * ray g of a launch is (seed s, patch p, ray r) with
-//     g = s*(n_p*R) + (p-p0)*R + r; within one seed consecutive g are
-//     consecutive positions of that seed's PCG64 stream (4 draws per ray);
-//   * every lane owns a contiguous run of rays, so it jumps its stream once
-//     (2^b jump tables) and then only draws; its receiver writes are
-//     sequential, which keeps partial cache lines merged in L2;
-//   * a wave keeps tracing while lanes are busy and refills idle lanes in
-//     one batch once at least REFILL_MIN of them wait, so the ray set-up
-//     (PCG64, sincos, sqrt, 9 FP64 divides) runs once per batch instead of
-//     once per finishing lane, and lanes do not wait for the longest ray.
+// Pipeline of one prh_trace_counts call (all on the library stream):
+//   k_gen_rays       seeded Lambertian rays -> FP64 origin/direction records
+//                    in HBM (48 B per ray; fully converged waves, each thread
+//                    jumps its PCG64 stream once and then only draws);
+//   k_trace_stream   FP64 DDA over the bit-packed occupancy grid; each wave
+//                    streams through its chunk of rays and refills idle lanes
+//                    in batches, so lanes do not wait for the longest ray.
+//                    Generation is kept out of this kernel: its temporaries
+//                    would otherwise stay allocated next to the traversal
+//                    state of the busy lanes and halve the occupancy;
+//   k_count_patches  per-patch LDS sort of the receivers -> unique counts;
+//   k_compact_edges  prefix-summed edge table (origin, receiver, count).
 #include <climits>
 
 #include "prh_device.h"
@@ -52,25 +53,29 @@ GeoView view_of(const Geometry &g)
 
 enum : int { RUNNING = 0, AT_WALL = 1, AT_SOLID = 2, FAILED = 3 };
 
-// Per-lane traversal state (the loop variables of _trace_kernel).
+// Per-lane traversal state (the loop variables of _trace_kernel).  The face
+// of the last crossing is kept as its axis (priority x > y > z) and turned
+// into the face index 0..5 only when the ray ends; the step cap is a
+// countdown, and the number of occupancy reads follows from the steps taken.
 struct Ray {
     double oy, oz, dy, dz;
     double tx, ty, tz, ddx, ddy, ddz, time;
     int i, jj, kk;
     int sx, sy, sz;
-    int face, fx, fy, fz;
-    uint32_t it, max_it;
+    int axis;       // axis of the last crossing, -1 before the first
+    uint32_t left;  // iterations left before the step cap
+    uint32_t cap;   // step cap of this ray
     int term;
 };
 
-__device__ inline int normal_code_of(const GeoView &G, int64_t p)
+__device__ __forceinline__ int normal_code_of(const GeoView &G, int64_t p)
 {
     const int cls = G.patch_class[p];
     return cls == 2 ? (int)G.patch_face[p] : cls;
 }
 
 // view_factors.py:211-224.  Returns false for dx == 0 (failed ray).
-__device__ inline bool ray_setup(Ray &R, const double o[3], const double d[3], const GeoView &G)
+__device__ __forceinline__ bool ray_setup(Ray &R, const double o[3], const double d[3], const GeoView &G)
 {
     const double ox = o[0], dx = d[0];
     R.oy = o[1];
@@ -86,59 +91,55 @@ __device__ inline bool ray_setup(Ray &R, const double o[3], const double d[3], c
     axis_state(R.oz, R.dz, k, R.sz, R.tz, R.ddz);
     R.jj = (int)py_imod64(j, G.ny);
     R.kk = (int)py_imod64(k, G.nz);
-    R.fx = R.sx > 0 ? 1 : 0;
-    R.fy = R.sy > 0 ? 3 : 2;
-    R.fz = R.sz > 0 ? 5 : 4;
-    R.face = -1;
+    R.axis = -1;
     R.time = 0.0;
-    R.it = 0;
-    R.max_it = max_steps_of(dx, R.dy, R.dz, G.nx, G.ny, G.nz);
+    R.cap = max_steps_of(dx, R.dy, R.dz, G.nx, G.ny, G.nz);
+    R.left = R.cap;
     R.term = RUNNING;
     return true;
 }
 
-// One iteration of the traversal loop, view_factors.py:225-255.
-__device__ inline void ray_step(Ray &R, const GeoView &G, uint64_t &reads)
+// One iteration of the traversal loop, view_factors.py:225-255.  The minimum
+// of the three crossing times is the value of one of them, so at least one
+// axis crosses; updates are written as selects to keep the wave converged.
+__device__ __forceinline__ void ray_step(Ray &R, const GeoView &G)
 {
     if ((unsigned)R.i >= (unsigned)G.nx) {
         R.term = AT_WALL;
         return;
     }
-    const uint32_t word = G.occ[((unsigned)R.i * (unsigned)G.ny + (unsigned)R.jj) * (unsigned)G.nzw + ((unsigned)R.kk >> 5)];
-    ++reads;
+    const unsigned row = (unsigned)R.i * (unsigned)G.ny + (unsigned)R.jj;
+    const uint32_t word = G.occ[row * (unsigned)G.nzw + ((unsigned)R.kk >> 5)];
     if ((word >> (R.kk & 31)) & 1u) {
         R.term = AT_SOLID;
         return;
     }
-    double nt = R.tx;
-    if (R.ty < nt) nt = R.ty;
-    if (R.tz < nt) nt = R.tz;
+    const double nt = fmin(fmin(R.tx, R.ty), R.tz);
     const bool cx = fabs(R.tx - nt) <= TIE_TOLERANCE;
     const bool cy = fabs(R.ty - nt) <= TIE_TOLERANCE;
     const bool cz = fabs(R.tz - nt) <= TIE_TOLERANCE;
-    if (cx) {
-        R.i += R.sx;
-        R.tx = R.tx + R.ddx;
-        R.face = R.fx;
-    }
-    if (cy) {
-        int jn = R.jj + R.sy;
-        R.jj = jn == G.ny ? 0 : (jn < 0 ? G.ny - 1 : jn);
-        R.ty = R.ty + R.ddy;
-        if (!cx) R.face = R.fy;
-    }
-    if (cz) {
-        int kn = R.kk + R.sz;
-        R.kk = kn == G.nz ? 0 : (kn < 0 ? G.nz - 1 : kn);
-        R.tz = R.tz + R.ddz;
-        if (!cx && !cy) R.face = R.fz;
-    }
+    R.i += cx ? R.sx : 0;
+    R.tx = cx ? R.tx + R.ddx : R.tx;
+    int jn = R.jj + (cy ? R.sy : 0);
+    R.jj = jn >= G.ny ? 0 : (jn < 0 ? G.ny - 1 : jn);
+    R.ty = cy ? R.ty + R.ddy : R.ty;
+    int kn = R.kk + (cz ? R.sz : 0);
+    R.kk = kn >= G.nz ? 0 : (kn < 0 ? G.nz - 1 : kn);
+    R.tz = cz ? R.tz + R.ddz : R.tz;
+    R.axis = cx ? 0 : (cy ? 1 : 2);
     R.time = nt;
-    if (++R.it >= R.max_it) R.term = FAILED;
+    if (--R.left == 0u) R.term = FAILED;
+}
+
+// Occupancy reads of a finished traversal: one per step plus the read that
+// found the solid cell.
+__device__ __forceinline__ uint32_t ray_reads(const Ray &R)
+{
+    return (R.cap - R.left) + (R.term == AT_SOLID ? 1u : 0u);
 }
 
 // Receiving patch of a finished traversal (view_factors.py:226-236), -1 on failure.
-__device__ inline int ray_receiver(const Ray &R, const GeoView &G)
+__device__ __forceinline__ int ray_receiver(const Ray &R, const GeoView &G)
 {
     if (R.term == AT_WALL) {
         const double yw = R.oy + R.dy * R.time;
@@ -149,122 +150,176 @@ __device__ inline int ray_receiver(const Ray &R, const GeoView &G)
         return R.i < 0 ? G.hot_map[c] : G.cold_map[c];
     }
     if (R.term == AT_SOLID) {
-        if (R.face < 0) return -1;
+        const int axis = R.axis;
+        if (axis < 0) return -1;
+        // Direction bits as values (a select between struct fields would be
+        // lowered to an indexed stack access).
+        const uint32_t up = (R.sx > 0 ? 1u : 0u) | (R.sy > 0 ? 2u : 0u) | (R.sz > 0 ? 4u : 0u);
+        const int face = 2 * axis + (int)((up >> axis) & 1u);
         const int64_t cell = ((int64_t)R.i * G.ny + R.jj) * G.nz + R.kk;
         const uint32_t code = G.cell_code[cell];
         const uint32_t m = code & 63u;
-        if (!((m >> R.face) & 1u)) return -1;
-        return (int)(code >> 6) + __popc(m & ((1u << R.face) - 1u));
+        if (!((m >> face) & 1u)) return -1;
+        return (int)(code >> 6) + __popc(m & ((1u << face) - 1u));
     }
     return -1;
 }
 
-struct FusedArgs {
-    GeoView G;
-    int64_t p0, n_p;
-    int R, S;
-    const uint64_t *seeds;  // S x {state_hi, state_lo, inc_hi, inc_lo}
-    int64_t total, per_lane;
-    int32_t *receiver;
-    unsigned long long *reads;
-    unsigned long long *failed;
-};
-
-__device__ inline void load_seed(const uint64_t *seeds, int s, u128 &st, u128 &inc)
+__device__ __forceinline__ void load_seed(const uint64_t *seeds, int s, u128 &st, u128 &inc)
 {
     st = make_u128(seeds[4 * s + 0], seeds[4 * s + 1]);
     inc = make_u128(seeds[4 * s + 2], seeds[4 * s + 3]);
 }
 
-__device__ inline uint64_t wave_sum_u64(uint64_t v)
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     return v;
 }
 
-template <int REFILL_MIN>
-__global__ __launch_bounds__(TRACE_THREADS) void k_trace_fused(FusedArgs a)
+// ------------------------------------------------------------------ ray generation
+// Ray g of a launch is (seed s, patch p, ray r), g = s*(n_p*R) + (p-p0)*R + r.
+// Within one seed consecutive g are consecutive positions of that seed's
+// PCG64 stream (4 draws per ray), so a thread that owns GEN_PER_THREAD
+// consecutive rays jumps its stream once and then only draws.
+constexpr uint32_t GEN_PER_THREAD = 16;
+
+struct GenArgs {
+    GeoView G;
+    uint32_t p0, n_p, R;
+    const uint64_t *seeds;  // S x {state_hi, state_lo, inc_hi, inc_lo}
+    uint32_t total;
+    double *origins;     // total x 3
+    double *directions;  // total x 3
+};
+
+__global__ __launch_bounds__(TRACE_THREADS) void k_gen_rays(GenArgs a)
 {
     const GeoView &G = a.G;
-    const int64_t lane_id = (int64_t)blockIdx.x * TRACE_THREADS + threadIdx.x;
-    int64_t g = lane_id * a.per_lane;
-    int64_t g_end = g + a.per_lane;
-    if (g_end > a.total) g_end = a.total;
-    const int64_t seg = a.n_p * a.R;
+    const uint64_t g0 = (uint64_t)(blockIdx.x * TRACE_THREADS + threadIdx.x) * GEN_PER_THREAD;
+    if (g0 >= a.total) return;
+    const uint32_t g_end = (uint32_t)(g0 + GEN_PER_THREAD < a.total ? g0 + GEN_PER_THREAD : a.total);
+    const uint32_t seg = a.n_p * a.R;
+    uint32_t g = (uint32_t)g0;
+    uint32_t s = g / seg;
+    const uint32_t rem = g - s * seg;
+    const uint32_t q = rem / a.R;
+    uint32_t r = rem - q * a.R;
+    uint32_t p = a.p0 + q;
+    u128 st0, inc;
+    load_seed(a.seeds, (int)s, st0, inc);
+    u128 st = pcg_jump(st0, inc, 4ull * ((uint64_t)p * a.R + r), c_jump);
     const double fny = (double)G.ny, fnz = (double)G.nz;
-
-    int s = 0, r = 0;
-    int64_t p = 0;
-    u128 st = make_u128(0, 0), inc = make_u128(0, 0);
-    if (g < g_end) {
-        s = (int)(g / seg);
-        const int64_t rem = g - (int64_t)s * seg;
-        const int64_t q = rem / a.R;
-        r = (int)(rem - q * a.R);
-        p = a.p0 + q;
-        u128 st0;
-        load_seed(a.seeds, s, st0, inc);
-        st = pcg_jump(st0, inc, 4ull * (uint64_t)(p * a.R + r), c_jump);
+    for (; g < g_end; ++g) {
+        const double *po = G.patch_origin + 3 * (size_t)p;
+        double o[3], d[3];
+        make_ray(st, inc, po[0], po[1], po[2], normal_code_of(G, p), fny, fnz, o, d);
+        double *oo = a.origins + 3 * (size_t)g;
+        double *dd = a.directions + 3 * (size_t)g;
+        oo[0] = o[0];
+        oo[1] = o[1];
+        oo[2] = o[2];
+        dd[0] = d[0];
+        dd[1] = d[1];
+        dd[2] = d[2];
+        if (++r == a.R) {
+            r = 0;
+            if (++p == a.p0 + a.n_p) {
+                p = a.p0;
+                ++s;
+                if (g + 1 < g_end) {
+                    load_seed(a.seeds, (int)s, st0, inc);
+                    st = pcg_jump(st0, inc, 4ull * ((uint64_t)a.p0 * a.R), c_jump);
+                }
+            }
+        }
     }
+}
+
+// ------------------------------------------------------------------ traversal
+// Each wavefront owns a contiguous chunk of rays and a cursor into it.  It
+// keeps stepping while its lanes are busy; once at least REFILL_MIN lanes
+// are idle, those lanes write their receivers and take the next rays of the
+// chunk in lane order (adjacent ray records, so the refill loads coalesce).
+struct TraceArgs {
+    GeoView G;
+    uint32_t total, per_wave;
+    const double *origins;
+    const double *directions;
+    int32_t *receiver;
+    unsigned long long *reads;
+    unsigned long long *failed;
+};
+
+template <int REFILL_MIN, int WAVES_PER_EU>
+__global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU)))
+void k_trace_stream(TraceArgs a)
+{
+    const GeoView &G = a.G;
+    const uint32_t wave = (blockIdx.x * TRACE_THREADS + threadIdx.x) >> 6;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint64_t wb = (uint64_t)wave * a.per_wave;
+    const uint32_t w_begin = wb < a.total ? (uint32_t)wb : a.total;
+    const uint32_t w_end = wb + a.per_wave < a.total ? (uint32_t)(wb + a.per_wave) : a.total;
+    uint32_t cursor = w_begin;
 
     Ray R;
     R.term = RUNNING;
     R.i = 0;
-    bool active = false;
-    int64_t g_cur = -1;
-    uint64_t reads = 0, failed = 0;
+    bool active = false, pending = false;
+    uint32_t g_cur = 0;
+    uint64_t reads = 0;
+    uint32_t failed = 0;
 
     while (true) {
-        if (!active) {
-            if (g_cur >= 0) {
-                const int rec = ray_receiver(R, G);
-                failed += rec < 0 ? 1u : 0u;
-                a.receiver[g_cur] = rec;
-                g_cur = -1;
-            }
-            while (g < g_end) {
-                const double *po = G.patch_origin + 3 * p;
-                double o[3], d[3];
-                make_ray(st, inc, po[0], po[1], po[2], normal_code_of(G, p), fny, fnz, o, d);
-                g_cur = g;
-                ++g;
-                if (++r == a.R) {
-                    r = 0;
-                    if (++p == a.p0 + a.n_p) {
-                        p = a.p0;
-                        ++s;
-                        if (g < g_end) {
-                            u128 st0;
-                            load_seed(a.seeds, s, st0, inc);
-                            st = pcg_jump(st0, inc, 4ull * (uint64_t)(a.p0 * a.R), c_jump);
-                        }
+        if (!active && pending) {
+            const int rec = ray_receiver(R, G);
+            failed += rec < 0 ? 1u : 0u;
+            reads += ray_reads(R);
+            a.receiver[g_cur] = rec;
+            pending = false;
+        }
+        if (cursor < w_end) {
+            const uint64_t idle = __ballot(!active);
+            if (!active) {
+                const uint32_t g = cursor + (uint32_t)__popcll(idle & below);
+                if (g < w_end) {
+                    const double *po = a.origins + 3 * (size_t)g;
+                    const double *pd = a.directions + 3 * (size_t)g;
+                    const double o[3] = {po[0], po[1], po[2]};
+                    const double d[3] = {pd[0], pd[1], pd[2]};
+                    if (ray_setup(R, o, d, G)) {
+                        active = true;
+                        pending = true;
+                        g_cur = g;
+                    } else {
+                        a.receiver[g] = -1;
+                        ++failed;
                     }
                 }
-                if (!ray_setup(R, o, d, G)) {
-                    a.receiver[g_cur] = -1;
-                    ++failed;
-                    g_cur = -1;
-                    continue;
-                }
-                active = true;
-                break;
             }
+            cursor += (uint32_t)__popcll(idle);
         }
-        if (__ballot(active) == 0) break;
+        if (__ballot(active) == 0) {
+            if (cursor >= w_end) break;
+            continue;
+        }
         while (true) {
             if (active) {
-                ray_step(R, G, reads);
-                if (R.term != RUNNING) active = false;
+                ray_step(R, G);
+                active = R.term == RUNNING;
             }
-            if (__ballot(active) == 0) break;
-            if (__popcll(__ballot(!active && g < g_end)) >= REFILL_MIN) break;
+            const uint64_t act = __ballot(active);
+            if (act == 0) break;
+            if (cursor < w_end && __popcll(~act) >= REFILL_MIN) break;
         }
     }
     reads = wave_sum_u64(reads);
-    failed = wave_sum_u64(failed);
-    if ((threadIdx.x & 63) == 0) {
+    const uint64_t f = wave_sum_u64((uint64_t)failed);
+    if (lane == 0) {
         atomicAdd(a.reads, (unsigned long long)reads);
-        if (failed) atomicAdd(a.failed, (unsigned long long)failed);
+        if (f) atomicAdd(a.failed, (unsigned long long)f);
     }
 }
 
@@ -355,45 +410,6 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_compact_edges(const int32_t *
     }
 }
 
-// generate_rays for the API (one thread per ray, full jump per ray).
-__global__ __launch_bounds__(TRACE_THREADS) void k_generate(GeoView G, int64_t p0, int64_t n_p, int R, u128 st0,
-                                                            u128 inc, double *origins, double *directions)
-{
-    const int64_t m = (int64_t)blockIdx.x * TRACE_THREADS + threadIdx.x;
-    if (m >= n_p * R) return;
-    const int64_t q = m / R;
-    const int64_t r = m - q * R;
-    const int64_t p = p0 + q;
-    u128 st = pcg_jump(st0, inc, 4ull * (uint64_t)(p * R + r), c_jump);
-    const double *po = G.patch_origin + 3 * p;
-    double o[3], d[3];
-    make_ray(st, inc, po[0], po[1], po[2], normal_code_of(G, p), (double)G.ny, (double)G.nz, o, d);
-    for (int c = 0; c < 3; ++c) {
-        origins[3 * m + c] = o[c];
-        directions[3 * m + c] = d[c];
-    }
-}
-
-// trace_first_hit for given rays (one thread per ray).
-__global__ __launch_bounds__(TRACE_THREADS) void k_trace_given(GeoView G, const double *origins,
-                                                               const double *directions, int64_t n_rays,
-                                                               int32_t *receiver, unsigned long long *failed)
-{
-    const int64_t m = (int64_t)blockIdx.x * TRACE_THREADS + threadIdx.x;
-    if (m >= n_rays) return;
-    double o[3] = {origins[3 * m], origins[3 * m + 1], origins[3 * m + 2]};
-    double d[3] = {directions[3 * m], directions[3 * m + 1], directions[3 * m + 2]};
-    Ray R;
-    int rec = -1;
-    if (ray_setup(R, o, d, G)) {
-        uint64_t reads = 0;
-        while (R.term == RUNNING) ray_step(R, G, reads);
-        rec = ray_receiver(R, G);
-    }
-    receiver[m] = rec;
-    if (rec < 0) atomicAdd(failed, 1ull);
-}
-
 // ------------------------------------------------------------------ host side
 bool g_tables_ready[64] = {};
 
@@ -409,36 +425,82 @@ int ensure_tables()
     return PRH_OK;
 }
 
+struct Buf {
+    void *p = nullptr;
+    size_t cap = 0;
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+// Grow-only per-device workspaces of the tracer (one library stream per
+// device, so calls on one device are serialised).
 struct Workspace {
-    void *seeds = nullptr;
-    size_t seeds_cap = 0;
-    void *receiver = nullptr;
-    size_t receiver_cap = 0;
-    void *uniq_recv = nullptr;
-    size_t uniq_recv_cap = 0;
-    void *uniq_cnt = nullptr;
-    size_t uniq_cnt_cap = 0;
-    void *n_unique = nullptr;
-    size_t n_unique_cap = 0;
-    void *offsets = nullptr;
-    size_t offsets_cap = 0;
-    void *counters = nullptr;
-    size_t counters_cap = 0;
-    hipEvent_t ev[4] = {};
+    Buf seeds, origins, directions, receiver, uniq_recv, uniq_cnt, n_unique, offsets, counters;
+    hipEvent_t ev[5] = {};
     bool events = false;
 };
 
 Workspace g_ws[64];
-double g_last_ms[3] = {0, 0, 0};
+double g_last_ms[4] = {0, 0, 0, 0};
 
-int resident_blocks(const void *kernel, int threads, int *blocks)
+int grow(Buf &b, size_t bytes) { return ensure_capacity(&b.p, &b.cap, bytes); }
+
+int resident_waves(const void *kernel, int64_t *waves)
 {
     int dev = 0, cus = 0, per_cu = 0;
     PRH_HIP(hipGetDevice(&dev));
     PRH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    PRH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
+    PRH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, TRACE_THREADS, 0));
     if (per_cu < 1) per_cu = 1;
-    *blocks = cus * per_cu;
+    *waves = (int64_t)cus * per_cu * (TRACE_THREADS / 64);
+    return PRH_OK;
+}
+
+// Traversal kernel variants: refill threshold x register budget (7 waves per
+// SIMD without spills, or 8 with the refill path spilling a few registers).
+// The variant is a tuning knob (prh_tracer_set_variant, chosen by the
+// autotuner in porerad/autotune.py).
+using TraceKernel = void (*)(TraceArgs);
+struct TraceVariant {
+    TraceKernel fn;
+    int refill_min, waves_per_eu;
+};
+const TraceVariant TRACE_VARIANTS[] = {
+    {k_trace_stream<16, 1>, 16, 1}, {k_trace_stream<8, 1>, 8, 1},  {k_trace_stream<32, 1>, 32, 1},
+    {k_trace_stream<16, 8>, 16, 8}, {k_trace_stream<8, 8>, 8, 8},  {k_trace_stream<32, 8>, 32, 8},
+    {k_trace_stream<4, 8>, 4, 8},   {k_trace_stream<24, 8>, 24, 8},
+};
+constexpr int N_TRACE_VARIANTS = sizeof(TRACE_VARIANTS) / sizeof(TRACE_VARIANTS[0]);
+int g_trace_variant = 0;
+
+int launch_gen(const Geometry &g, uint32_t p0, uint32_t n_p, uint32_t R, const uint64_t *d_seeds, uint32_t total,
+               double *d_orig, double *d_dir, hipStream_t st)
+{
+    if (total == 0) return PRH_OK;
+    const uint64_t threads = (total + GEN_PER_THREAD - 1) / GEN_PER_THREAD;
+    const unsigned grid = (unsigned)((threads + TRACE_THREADS - 1) / TRACE_THREADS);
+    GenArgs a{view_of(g), p0, n_p, R, d_seeds, total, d_orig, d_dir};
+    k_gen_rays<<<grid, TRACE_THREADS, 0, st>>>(a);
+    PRH_HIP(hipGetLastError());
+    return PRH_OK;
+}
+
+int launch_trace(const Geometry &g, const double *d_orig, const double *d_dir, uint32_t total, int32_t *d_receiver,
+                 unsigned long long *d_counters, hipStream_t st)
+{
+    if (total == 0) return PRH_OK;
+    const TraceVariant &V = TRACE_VARIANTS[g_trace_variant];
+    int64_t waves = 0;
+    int rc = resident_waves((const void *)V.fn, &waves);
+    if (rc) return rc;
+    // Enough work per wave to amortise the tail, enough waves to fill the chip.
+    int64_t per_wave = (total + waves - 1) / waves;
+    if (per_wave < 64) per_wave = 64;
+    const int64_t n_waves = (total + per_wave - 1) / per_wave;
+    const unsigned grid = (unsigned)((n_waves * 64 + TRACE_THREADS - 1) / TRACE_THREADS);
+    TraceArgs a{view_of(g), total, (uint32_t)per_wave, d_orig, d_dir, d_receiver, d_counters, d_counters + 1};
+    V.fn<<<grid, TRACE_THREADS, 0, st>>>(a);
+    PRH_HIP(hipGetLastError());
     return PRH_OK;
 }
 
@@ -461,7 +523,26 @@ extern "C" {
 int prh_last_timings(double *ms, int32_t n)
 {
     PRH_CHECK_ARG(ms && n >= 0, "prh_last_timings: null output");
-    for (int i = 0; i < n && i < 3; ++i) ms[i] = g_last_ms[i];
+    for (int i = 0; i < n && i < 4; ++i) ms[i] = g_last_ms[i];
+    return PRH_OK;
+}
+
+int prh_tracer_set_variant(int32_t variant)
+{
+    PRH_CHECK_ARG(variant >= 0 && variant < N_TRACE_VARIANTS, "prh_tracer_set_variant: variant %d out of range [0, %d)",
+                  variant, N_TRACE_VARIANTS);
+    g_trace_variant = variant;
+    return PRH_OK;
+}
+
+int prh_tracer_variants(int32_t *n_variants, int32_t *current, int32_t *refill_min, int32_t *waves_per_eu)
+{
+    if (n_variants) *n_variants = N_TRACE_VARIANTS;
+    if (current) *current = g_trace_variant;
+    for (int v = 0; v < N_TRACE_VARIANTS; ++v) {
+        if (refill_min) refill_min[v] = TRACE_VARIANTS[v].refill_min;
+        if (waves_per_eu) waves_per_eu[v] = TRACE_VARIANTS[v].waves_per_eu;
+    }
     return PRH_OK;
 }
 
@@ -474,23 +555,22 @@ int prh_generate_rays(const prh_geometry *geo, int64_t p0, int64_t p1, int32_t r
     PRH_CHECK_ARG(rays_per_patch > 0, "prh_generate_rays: rays_per_patch must be positive");
     const int64_t m = (p1 - p0) * (int64_t)rays_per_patch;
     if (m == 0) return PRH_OK;
+    PRH_CHECK_ARG(m < (int64_t(1) << 32), "prh_generate_rays: more than 2^32 rays in one call");
     PRH_HIP(hipSetDevice(g.device));
     int rc = ensure_tables();
     if (rc) return rc;
     hipStream_t st;
     if ((rc = current_stream(&st))) return rc;
-    double *d_o = nullptr, *d_d = nullptr;
-    PRH_HIP(hipMallocAsync((void **)&d_o, m * 3 * sizeof(double), st));
-    PRH_HIP(hipMallocAsync((void **)&d_d, m * 3 * sizeof(double), st));
-    const unsigned blocks = (unsigned)((m + TRACE_THREADS - 1) / TRACE_THREADS);
-    k_generate<<<blocks, TRACE_THREADS, 0, st>>>(view_of(g), p0, p1 - p0, rays_per_patch,
-                                                 make_u128(state_inc[0], state_inc[1]),
-                                                 make_u128(state_inc[2], state_inc[3]), d_o, d_d);
-    PRH_HIP(hipGetLastError());
-    PRH_HIP(hipMemcpyAsync(origins, d_o, m * 3 * sizeof(double), hipMemcpyDeviceToHost, st));
-    PRH_HIP(hipMemcpyAsync(directions, d_d, m * 3 * sizeof(double), hipMemcpyDeviceToHost, st));
-    PRH_HIP(hipFreeAsync(d_o, st));
-    PRH_HIP(hipFreeAsync(d_d, st));
+    Workspace &W = g_ws[g.device];
+    if ((rc = grow(W.seeds, 4 * sizeof(uint64_t)))) return rc;
+    if ((rc = grow(W.origins, m * 3 * sizeof(double)))) return rc;
+    if ((rc = grow(W.directions, m * 3 * sizeof(double)))) return rc;
+    PRH_HIP(hipMemcpyAsync(W.seeds.p, state_inc, 4 * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    if ((rc = launch_gen(g, (uint32_t)p0, (uint32_t)(p1 - p0), (uint32_t)rays_per_patch, W.seeds.as<uint64_t>(),
+                         (uint32_t)m, W.origins.as<double>(), W.directions.as<double>(), st)))
+        return rc;
+    PRH_HIP(hipMemcpyAsync(origins, W.origins.p, m * 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+    PRH_HIP(hipMemcpyAsync(directions, W.directions.p, m * 3 * sizeof(double), hipMemcpyDeviceToHost, st));
     PRH_HIP(hipStreamSynchronize(st));
     return PRH_OK;
 }
@@ -499,7 +579,7 @@ int prh_trace_rays(const prh_geometry *geo, const double *origins, const double 
                    int32_t *receiver, int64_t *n_failed)
 {
     PRH_CHECK_ARG(geo && receiver && (n_rays == 0 || (origins && directions)), "prh_trace_rays: null pointer");
-    PRH_CHECK_ARG(n_rays >= 0, "prh_trace_rays: negative ray count");
+    PRH_CHECK_ARG(n_rays >= 0 && n_rays < (int64_t(1) << 32), "prh_trace_rays: ray count out of range");
     if (n_failed) *n_failed = 0;
     if (n_rays == 0) return PRH_OK;
     const Geometry &g = geo->g;
@@ -507,30 +587,24 @@ int prh_trace_rays(const prh_geometry *geo, const double *origins, const double 
     hipStream_t st;
     int rc = current_stream(&st);
     if (rc) return rc;
-    double *d_o = nullptr, *d_d = nullptr;
-    int32_t *d_r = nullptr;
-    unsigned long long *d_f = nullptr;
-    PRH_HIP(hipMallocAsync((void **)&d_o, n_rays * 3 * sizeof(double), st));
-    PRH_HIP(hipMallocAsync((void **)&d_d, n_rays * 3 * sizeof(double), st));
-    PRH_HIP(hipMallocAsync((void **)&d_r, n_rays * sizeof(int32_t), st));
-    PRH_HIP(hipMallocAsync((void **)&d_f, sizeof(unsigned long long), st));
-    PRH_HIP(hipMemcpyAsync(d_o, origins, n_rays * 3 * sizeof(double), hipMemcpyHostToDevice, st));
-    PRH_HIP(hipMemcpyAsync(d_d, directions, n_rays * 3 * sizeof(double), hipMemcpyHostToDevice, st));
-    PRH_HIP(hipMemsetAsync(d_f, 0, sizeof(unsigned long long), st));
-    const unsigned blocks = (unsigned)((n_rays + TRACE_THREADS - 1) / TRACE_THREADS);
-    k_trace_given<<<blocks, TRACE_THREADS, 0, st>>>(view_of(g), d_o, d_d, n_rays, d_r, d_f);
-    PRH_HIP(hipGetLastError());
-    unsigned long long failed = 0;
-    PRH_HIP(hipMemcpyAsync(receiver, d_r, n_rays * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    PRH_HIP(hipMemcpyAsync(&failed, d_f, sizeof(failed), hipMemcpyDeviceToHost, st));
-    PRH_HIP(hipFreeAsync(d_o, st));
-    PRH_HIP(hipFreeAsync(d_d, st));
-    PRH_HIP(hipFreeAsync(d_r, st));
-    PRH_HIP(hipFreeAsync(d_f, st));
+    Workspace &W = g_ws[g.device];
+    if ((rc = grow(W.origins, n_rays * 3 * sizeof(double)))) return rc;
+    if ((rc = grow(W.directions, n_rays * 3 * sizeof(double)))) return rc;
+    if ((rc = grow(W.receiver, n_rays * sizeof(int32_t)))) return rc;
+    if ((rc = grow(W.counters, 2 * sizeof(unsigned long long)))) return rc;
+    PRH_HIP(hipMemcpyAsync(W.origins.p, origins, n_rays * 3 * sizeof(double), hipMemcpyHostToDevice, st));
+    PRH_HIP(hipMemcpyAsync(W.directions.p, directions, n_rays * 3 * sizeof(double), hipMemcpyHostToDevice, st));
+    PRH_HIP(hipMemsetAsync(W.counters.p, 0, 2 * sizeof(unsigned long long), st));
+    if ((rc = launch_trace(g, W.origins.as<double>(), W.directions.as<double>(), (uint32_t)n_rays,
+                           W.receiver.as<int32_t>(), W.counters.as<unsigned long long>(), st)))
+        return rc;
+    unsigned long long counters[2] = {0, 0};
+    PRH_HIP(hipMemcpyAsync(receiver, W.receiver.p, n_rays * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    PRH_HIP(hipMemcpyAsync(counters, W.counters.p, sizeof(counters), hipMemcpyDeviceToHost, st));
     PRH_HIP(hipStreamSynchronize(st));
-    if (n_failed) *n_failed = (int64_t)failed;
-    if (failed) {
-        set_error("%lld rays did not resolve a receiving patch", (long long)failed);
+    if (n_failed) *n_failed = (int64_t)counters[1];
+    if (counters[1]) {
+        set_error("%llu rays did not resolve a receiving patch", counters[1]);
         return PRH_ERR_UNRESOLVED;
     }
     return PRH_OK;
@@ -543,15 +617,22 @@ int prh_trace_counts(const prh_geometry *geo, int64_t p0, int64_t p1, int32_t ra
     const Geometry &g = geo->g;
     PRH_CHECK_ARG(0 <= p0 && p0 <= p1 && p1 <= g.n_patches, "prh_trace_counts: patch range out of bounds");
     PRH_CHECK_ARG(rays_per_patch > 0 && n_seeds > 0, "prh_trace_counts: rays_per_patch and n_seeds must be positive");
-    const int SR = rays_per_patch * n_seeds;
     if ((int64_t)rays_per_patch * n_seeds > MAX_RAYS_PER_PATCH_CALL) {
         set_error("prh_trace_counts: rays_per_patch * n_seeds = %lld exceeds %d",
                   (long long)rays_per_patch * n_seeds, MAX_RAYS_PER_PATCH_CALL);
         return PRH_ERR_CAPACITY;
     }
+    const int SR = rays_per_patch * n_seeds;
+    const int64_t n_p = p1 - p0;
+    const int64_t total = n_p * SR;
+    if (total >= (int64_t(1) << 32)) {
+        set_error("prh_trace_counts: %lld rays exceed the 2^32 rays of one call; split the patch range",
+                  (long long)total);
+        return PRH_ERR_CAPACITY;
+    }
     *out = nullptr;
     PRH_HIP(hipSetDevice(g.device));
-    int dev = g.device;
+    const int dev = g.device;
     int rc = ensure_tables();
     if (rc) return rc;
     hipStream_t st;
@@ -561,75 +642,68 @@ int prh_trace_counts(const prh_geometry *geo, int64_t p0, int64_t p1, int32_t ra
         for (auto &e : W.ev) PRH_HIP(hipEventCreate(&e));
         W.events = true;
     }
-    const int64_t n_p = p1 - p0;
-    const int64_t total = n_p * SR;
     int SRp2 = 1;
     while (SRp2 < SR) SRp2 <<= 1;
-
-    if ((rc = ensure_capacity(&W.seeds, &W.seeds_cap, sizeof(uint64_t) * 4 * n_seeds))) return rc;
-    if ((rc = ensure_capacity(&W.receiver, &W.receiver_cap, sizeof(int32_t) * (total > 0 ? total : 1)))) return rc;
-    if ((rc = ensure_capacity(&W.uniq_recv, &W.uniq_recv_cap, sizeof(int32_t) * (total > 0 ? total : 1)))) return rc;
-    if ((rc = ensure_capacity(&W.uniq_cnt, &W.uniq_cnt_cap, sizeof(int32_t) * (total > 0 ? total : 1)))) return rc;
-    if ((rc = ensure_capacity(&W.n_unique, &W.n_unique_cap, sizeof(int32_t) * (n_p > 0 ? n_p : 1)))) return rc;
-    if ((rc = ensure_capacity(&W.offsets, &W.offsets_cap, sizeof(int64_t) * (n_p > 0 ? n_p : 1)))) return rc;
-    if ((rc = ensure_capacity(&W.counters, &W.counters_cap, sizeof(unsigned long long) * 2))) return rc;
-    PRH_HIP(hipMemcpyAsync(W.seeds, state_inc, sizeof(uint64_t) * 4 * n_seeds, hipMemcpyHostToDevice, st));
-    PRH_HIP(hipMemsetAsync(W.counters, 0, sizeof(unsigned long long) * 2, st));
-    unsigned long long *d_reads = (unsigned long long *)W.counters;
-    unsigned long long *d_failed = d_reads + 1;
+    const size_t t1 = total > 0 ? (size_t)total : 1, n1 = n_p > 0 ? (size_t)n_p : 1;
+    if ((rc = grow(W.seeds, sizeof(uint64_t) * 4 * n_seeds))) return rc;
+    if ((rc = grow(W.origins, t1 * 3 * sizeof(double)))) return rc;
+    if ((rc = grow(W.directions, t1 * 3 * sizeof(double)))) return rc;
+    if ((rc = grow(W.receiver, t1 * sizeof(int32_t)))) return rc;
+    if ((rc = grow(W.uniq_recv, t1 * sizeof(int32_t)))) return rc;
+    if ((rc = grow(W.uniq_cnt, t1 * sizeof(int32_t)))) return rc;
+    if ((rc = grow(W.n_unique, n1 * sizeof(int32_t)))) return rc;
+    if ((rc = grow(W.offsets, n1 * sizeof(int64_t)))) return rc;
+    if ((rc = grow(W.counters, 2 * sizeof(unsigned long long)))) return rc;
+    PRH_HIP(hipMemcpyAsync(W.seeds.p, state_inc, sizeof(uint64_t) * 4 * n_seeds, hipMemcpyHostToDevice, st));
+    PRH_HIP(hipMemsetAsync(W.counters.p, 0, 2 * sizeof(unsigned long long), st));
 
     int64_t nnz = 0;
     int32_t *d_origin = nullptr, *d_receiver = nullptr, *d_count = nullptr;
     unsigned long long counters[2] = {0, 0};
     if (total > 0) {
-        constexpr int REFILL = 16;
-        int blocks = 0;
-        if ((rc = resident_blocks((const void *)k_trace_fused<REFILL>, TRACE_THREADS, &blocks))) return rc;
-        const int64_t lanes_max = (int64_t)blocks * TRACE_THREADS;
-        const int64_t per_lane = (total + lanes_max - 1) / lanes_max;
-        const int64_t lanes = (total + per_lane - 1) / per_lane;
-        const unsigned grid = (unsigned)((lanes + TRACE_THREADS - 1) / TRACE_THREADS);
-        FusedArgs a{view_of(g), p0, n_p, rays_per_patch, n_seeds, (const uint64_t *)W.seeds, total, per_lane,
-                    (int32_t *)W.receiver, d_reads, d_failed};
         PRH_HIP(hipEventRecord(W.ev[0], st));
-        k_trace_fused<REFILL><<<grid, TRACE_THREADS, 0, st>>>(a);
-        PRH_HIP(hipGetLastError());
+        if ((rc = launch_gen(g, (uint32_t)p0, (uint32_t)n_p, (uint32_t)rays_per_patch, W.seeds.as<uint64_t>(),
+                             (uint32_t)total, W.origins.as<double>(), W.directions.as<double>(), st)))
+            return rc;
         PRH_HIP(hipEventRecord(W.ev[1], st));
+        if ((rc = launch_trace(g, W.origins.as<double>(), W.directions.as<double>(), (uint32_t)total,
+                               W.receiver.as<int32_t>(), W.counters.as<unsigned long long>(), st)))
+            return rc;
+        PRH_HIP(hipEventRecord(W.ev[2], st));
         const unsigned cgrid = (unsigned)(n_p < 65536 ? n_p : 65536);
         const size_t lds = sizeof(int32_t) * (SRp2 + COUNT_THREADS);
-        k_count_patches<<<cgrid, COUNT_THREADS, lds, st>>>((const int32_t *)W.receiver, n_p, rays_per_patch,
-                                                           n_seeds, SRp2, (int32_t *)W.uniq_recv,
-                                                           (int32_t *)W.uniq_cnt, (int32_t *)W.n_unique);
+        k_count_patches<<<cgrid, COUNT_THREADS, lds, st>>>(W.receiver.as<int32_t>(), n_p, rays_per_patch, n_seeds,
+                                                           SRp2, W.uniq_recv.as<int32_t>(), W.uniq_cnt.as<int32_t>(),
+                                                           W.n_unique.as<int32_t>());
         PRH_HIP(hipGetLastError());
-        PRH_HIP(hipEventRecord(W.ev[2], st));
-        PRH_HIP(hipMemcpyAsync(counters, W.counters, sizeof(counters), hipMemcpyDeviceToHost, st));
+        PRH_HIP(hipEventRecord(W.ev[3], st));
+        PRH_HIP(hipMemcpyAsync(counters, W.counters.p, sizeof(counters), hipMemcpyDeviceToHost, st));
         PRH_HIP(hipStreamSynchronize(st));
         if (counters[1]) {
             set_error("%llu rays did not resolve a receiving patch", counters[1]);
             return PRH_ERR_UNRESOLVED;
         }
-        if ((rc = exclusive_scan_i32_to_i64((const int32_t *)W.n_unique, (int64_t *)W.offsets, n_p, &nnz, st)))
+        if ((rc = exclusive_scan_i32_to_i64(W.n_unique.as<int32_t>(), W.offsets.as<int64_t>(), n_p, &nnz, st)))
             return rc;
         const int64_t alloc = nnz > 0 ? nnz : 1;
         PRH_HIP(hipMalloc((void **)&d_origin, alloc * sizeof(int32_t)));
         PRH_HIP(hipMalloc((void **)&d_receiver, alloc * sizeof(int32_t)));
         PRH_HIP(hipMalloc((void **)&d_count, alloc * sizeof(int32_t)));
-        k_compact_edges<<<cgrid, COUNT_THREADS, 0, st>>>((const int32_t *)W.uniq_recv, (const int32_t *)W.uniq_cnt,
-                                                         (const int32_t *)W.n_unique, (const int64_t *)W.offsets,
-                                                         n_p, SR, p0, d_origin, d_receiver, d_count);
+        k_compact_edges<<<cgrid, COUNT_THREADS, 0, st>>>(W.uniq_recv.as<int32_t>(), W.uniq_cnt.as<int32_t>(),
+                                                         W.n_unique.as<int32_t>(), W.offsets.as<int64_t>(), n_p, SR,
+                                                         p0, d_origin, d_receiver, d_count);
         PRH_HIP(hipGetLastError());
-        PRH_HIP(hipEventRecord(W.ev[3], st));
+        PRH_HIP(hipEventRecord(W.ev[4], st));
         PRH_HIP(hipStreamSynchronize(st));
         float ms = 0.f;
-        PRH_HIP(hipEventElapsedTime(&ms, W.ev[0], W.ev[1]));
-        g_last_ms[0] = ms;
-        PRH_HIP(hipEventElapsedTime(&ms, W.ev[1], W.ev[2]));
-        g_last_ms[1] = ms;
-        PRH_HIP(hipEventElapsedTime(&ms, W.ev[2], W.ev[3]));
-        g_last_ms[2] = ms;
+        for (int k = 0; k < 3; ++k) {
+            PRH_HIP(hipEventElapsedTime(&ms, W.ev[k], W.ev[k + 1]));
+            g_last_ms[k] = ms;
+        }
+        PRH_HIP(hipEventElapsedTime(&ms, W.ev[3], W.ev[4]));
+        g_last_ms[3] = ms;
     }
-    prh_edges *e = new prh_edges{dev, nnz, d_origin, d_receiver, d_count};
-    *out = e;
+    *out = new prh_edges{dev, nnz, d_origin, d_receiver, d_count};
     if (nnz_out) *nnz_out = nnz;
     if (occupancy_reads) *occupancy_reads = (int64_t)counters[0];
     return PRH_OK;
@@ -644,7 +718,8 @@ int prh_edges_copy(const prh_edges *e, int32_t *origin, int32_t *receiver, int64
     if (receiver) PRH_HIP(hipMemcpy(receiver, e->receiver, e->nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
     if (count) {
         // Widen the int32 device counts into the int64 host array in place:
-        // copy into the upper half of the output buffer, then expand upwards.
+        // copy into the upper half of the output buffer, then expand upwards
+        // (element k is read before the write of element k can reach it).
         int32_t *tmp = reinterpret_cast<int32_t *>(count) + e->nnz;
         PRH_HIP(hipMemcpy(tmp, e->count, e->nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
         for (int64_t k = 0; k < e->nnz; ++k) count[k] = (int64_t)tmp[k];

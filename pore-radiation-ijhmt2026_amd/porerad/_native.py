@@ -39,6 +39,8 @@ SIGNATURES = {
     "prh_version": ([], ctypes.c_char_p),
     "prh_last_timings": ([_P, _I32], ctypes.c_int),
     "prh_pcg64_seed": ([_P, _I32, _P], ctypes.c_int),
+    "prh_tracer_set_variant": ([_I32], ctypes.c_int),
+    "prh_tracer_variants": ([_P, _P, _P, _P], ctypes.c_int),
     "prh_geometry_create": ([_P, _I32, _I32, _I32, _F64, _P], ctypes.c_int),
     "prh_geometry_sizes": ([_P, _P, _P], ctypes.c_int),
     "prh_geometry_patch_table": ([_P, _P, _P, _P, _P], ctypes.c_int),
@@ -135,9 +137,26 @@ def pcg64_state(seed: int) -> np.ndarray:
 
 
 def last_timings() -> np.ndarray:
-    out = np.zeros(3, dtype=np.float64)
-    check(lib().prh_last_timings(ptr(out), 3), "prh_last_timings")
+    """[generation, traversal, counting, compaction] ms of the last prh_trace_counts call."""
+
+    out = np.zeros(4, dtype=np.float64)
+    check(lib().prh_last_timings(ptr(out), 4), "prh_last_timings")
     return out
+
+
+def tracer_variants() -> list[dict]:
+    n = ctypes.c_int32()
+    cur = ctypes.c_int32()
+    check(lib().prh_tracer_variants(ctypes.byref(n), ctypes.byref(cur), None, None), "prh_tracer_variants")
+    refill = np.zeros(n.value, np.int32)
+    waves = np.zeros(n.value, np.int32)
+    check(lib().prh_tracer_variants(None, None, ptr(refill), ptr(waves)), "prh_tracer_variants")
+    return [{"variant": v, "refill_min": int(refill[v]), "waves_per_eu": int(waves[v]), "current": v == cur.value}
+            for v in range(n.value)]
+
+
+def set_tracer_variant(variant: int) -> None:
+    check(lib().prh_tracer_set_variant(int(variant)), "prh_tracer_set_variant")
 
 
 class Geometry:

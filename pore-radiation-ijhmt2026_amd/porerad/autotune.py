@@ -1,0 +1,85 @@
+"""Launch-shape autotuner of the HIP tracer.
+
+The traversal kernel comes in a few compiled variants (refill threshold x
+register budget, see ``prh_tracer_variants``).  Which one is fastest depends
+on the geometry (mean ray length, divergence of ray lengths) and on the ray
+count per patch.  ``tune_tracer`` times every variant on a sample of the
+patches, keeps the fastest and records the choice in a small JSON cache
+keyed by the workload signature, so later runs start from it.
+
+Results never depend on the variant: every variant produces bit-identical
+counts (checked by tests/test_gpu_tracer.py).
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import time
+
+from . import _native
+
+CACHE_ENV = "PORERAD_TUNE_CACHE"
+
+
+def _cache_path() -> str:
+    return os.environ.get(CACHE_ENV) or os.path.join(os.path.expanduser("~"), ".cache", "porerad", "tracer_tune.json")
+
+
+def _load() -> dict:
+    try:
+        with open(_cache_path()) as handle:
+            return json.load(handle)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(cache: dict) -> None:
+    path = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        with open(path, "w") as handle:
+            json.dump(cache, handle, indent=1)
+    except OSError:
+        pass
+
+
+def signature(patches, rays_per_patch: int, n_seeds: int) -> str:
+    nx, ny, nz = patches.solid.shape
+    return f"{nx}x{ny}x{nz}/p{patches.n_patches}/r{int(rays_per_patch) * int(n_seeds)}"
+
+
+def tune_tracer(patches, rays_per_patch: int, seeds, *, sample_patches: int = 40000, repeats: int = 2,
+                use_cache: bool = True) -> dict:
+    """Pick the fastest traversal variant for this workload; returns the timing table."""
+
+    from .view_factors import first_hit_edges
+
+    seeds = [int(s) for s in seeds]
+    key = signature(patches, rays_per_patch, len(seeds))
+    cache = _load() if use_cache else {}
+    if key in cache:
+        _native.set_tracer_variant(cache[key]["variant"])
+        return cache[key]
+    n = patches.n_patches
+    p0 = max(0, n // 2 - sample_patches // 2)
+    p1 = min(n, p0 + sample_patches)
+    table = []
+    for v in _native.tracer_variants():
+        _native.set_tracer_variant(v["variant"])
+        first_hit_edges(patches, rays_per_patch, seeds, patch_range=(p0, p1)).close()
+        best = float("inf")
+        for _ in range(repeats):
+            t = time.perf_counter()
+            first_hit_edges(patches, rays_per_patch, seeds, patch_range=(p0, p1)).close()
+            best = min(best, _native.last_timings()[1])
+            _ = time.perf_counter() - t
+        table.append({"variant": v["variant"], "refill_min": v["refill_min"], "waves_per_eu": v["waves_per_eu"],
+                      "trace_ms": best})
+    winner = min(table, key=lambda row: row["trace_ms"])
+    _native.set_tracer_variant(winner["variant"])
+    result = {"variant": winner["variant"], "table": table, "sample": [p0, p1]}
+    if use_cache:
+        cache[key] = result
+        _store(cache)
+    return result
