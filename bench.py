@@ -123,6 +123,20 @@ def cpu_baseline(phases, R, seed, budget_s=12.0):
     }
 
 
+def copy_probe(n_bytes: int = 1 << 30) -> None:
+    """One plain device copy of n_bytes (a torch elementwise copy kernel): the
+    reference point for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+
+    import torch
+
+    src = torch.ones(n_bytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy probe: {n_bytes} bytes read, {n_bytes} bytes written")
+
+
 def main() -> None:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,7 +146,11 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-radiosity", action="store_true", help="skip the radiosity solves")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tracer variant")
+    ap.add_argument("--copy-probe", action="store_true", help="only run one plain 1 GiB device copy")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU; multi-GPU runs are not part of this build "
               "(the tracer shards by patch range, the radiosity by request)", file=sys.stderr)
@@ -153,7 +171,7 @@ def main() -> None:
         from porerad import autotune
 
         tune = autotune.tune_tracer(P, R, [seed], use_cache=False)
-        log("tracer variants: " + ", ".join(f"v{r['variant']}(refill {r['refill_min']}, wpe {r['waves_per_eu']})="
+        log("tracer variants: " + ", ".join(f"v{r['variant']}(refill {r['refill_min']}, coarse {r['coarse']})="
                                             f"{r['trace_ms']:.2f}ms" for r in tune["table"]))
     wall, parts, reads, nnz = run_tracer(P, R, seed, args.steps, args.warmup)
     step = float(np.mean(wall))

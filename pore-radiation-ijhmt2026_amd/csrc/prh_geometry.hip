@@ -9,7 +9,7 @@
 //   * origins sit 1e-9 off the face and are wrapped with float modulo in y, z
 //     (view_factors.py:88-90, 105-109).
 //
-// Layout in HBM (see prh_internal.h): a bit-packed occupancy grid for the
+// Layout in HBM (see prh_internal.h): a flat occupancy bitset for the
 // traversal, one 32-bit code per cell (first patch id << 6 | face mask) to
 // turn a hit face into a patch id, and per-patch origin/face/class arrays.
 #include "prh_internal.h"
@@ -46,18 +46,48 @@ __global__ __launch_bounds__(GEO_THREADS) void k_cell_masks(const uint8_t *solid
     }
 }
 
-__global__ __launch_bounds__(GEO_THREADS) void k_pack_occupancy(const uint8_t *solid, int nx, int ny, int nz,
-                                                                 int nzw, uint32_t *occ)
+// Flat bitset: bit c of the occupancy is cell c = (i*ny + j)*nz + k.
+__global__ __launch_bounds__(GEO_THREADS) void k_pack_occupancy(const uint8_t *solid, int64_t n_cells, uint32_t *occ)
 {
-    const int64_t n_words = (int64_t)nx * ny * nzw;
+    const int64_t n_words = (n_cells + 31) / 32;
     for (int64_t w = (int64_t)blockIdx.x * GEO_THREADS + threadIdx.x; w < n_words;
          w += (int64_t)gridDim.x * GEO_THREADS) {
-        const int64_t row = w / nzw;
-        const int k0 = (int)(w % nzw) * 32;
+        const int64_t c0 = w * 32;
         uint32_t bits = 0;
-        for (int b = 0; b < 32 && k0 + b < nz; ++b)
-            if (solid[row * nz + k0 + b]) bits |= 1u << b;
+        for (int b = 0; b < 32 && c0 + b < n_cells; ++b)
+            if (solid[c0 + b]) bits |= 1u << b;
         occ[w] = bits;
+    }
+}
+
+// Brick bitset: bit b is set when brick b (2^shift cells per side, clipped
+// at the grid end) holds a solid cell.
+__global__ __launch_bounds__(GEO_THREADS) void k_pack_coarse(const uint8_t *solid, int nx, int ny, int nz, int shift,
+                                                              int cnx, int cny, int cnz, uint32_t *coarse)
+{
+    const int64_t n_bricks = (int64_t)cnx * cny * cnz;
+    const int64_t n_words = (n_bricks + 31) / 32;
+    const int B = 1 << shift;
+    for (int64_t w = (int64_t)blockIdx.x * GEO_THREADS + threadIdx.x; w < n_words;
+         w += (int64_t)gridDim.x * GEO_THREADS) {
+        uint32_t bits = 0;
+        for (int t = 0; t < 32; ++t) {
+            const int64_t b = w * 32 + t;
+            if (b >= n_bricks) break;
+            const int bk = (int)(b % cnz);
+            const int bj = (int)((b / cnz) % cny);
+            const int bi = (int)(b / ((int64_t)cnz * cny));
+            bool any = false;
+            for (int i = bi * B; i < (bi + 1) * B && i < nx && !any; ++i)
+                for (int j = bj * B; j < (bj + 1) * B && j < ny && !any; ++j)
+                    for (int k = bk * B; k < (bk + 1) * B && k < nz; ++k)
+                        if (solid[((int64_t)i * ny + j) * nz + k]) {
+                            any = true;
+                            break;
+                        }
+            if (any) bits |= 1u << t;
+        }
+        coarse[w] = bits;
     }
 }
 
@@ -144,11 +174,12 @@ unsigned grid_for(int64_t n)
 
 void free_geometry(Geometry &g)
 {
-    void *ptrs[] = {g.occ, g.cell_code, g.patch_origin, g.patch_face, g.patch_cell, g.patch_class, g.hot_map,
+    void *ptrs[] = {g.occ, g.coarse, g.cell_code, g.patch_origin, g.patch_face, g.patch_cell, g.patch_class, g.hot_map,
                     g.cold_map};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     g.occ = nullptr;
+    g.coarse = nullptr;
     g.cell_code = nullptr;
     g.patch_origin = nullptr;
     g.patch_face = nullptr;
@@ -166,7 +197,17 @@ int build(const uint8_t *solid_host, int nx, int ny, int nz, double H_m, Geometr
     g.nx = nx;
     g.ny = ny;
     g.nz = nz;
-    g.nzw = (nz + 31) / 32;
+    g.occ_words = ((int64_t)nx * ny * nz + 31) / 32;
+    // Smallest brick whose bitset stays within 72 KB, so that two 1024-lane
+    // workgroups of the traversal kernel each hold a copy in one CU's LDS.
+    for (g.coarse_shift = 1;; ++g.coarse_shift) {
+        const int B = 1 << g.coarse_shift;
+        g.cnx = (nx + B - 1) / B;
+        g.cny = (ny + B - 1) / B;
+        g.cnz = (nz + B - 1) / B;
+        g.coarse_words = ((int64_t)g.cnx * g.cny * g.cnz + 31) / 32;
+        if (g.coarse_words * 4 <= 72 * 1024) break;
+    }
     g.H_m = H_m;
     const int64_t n_cells = (int64_t)nx * ny * nz;
     const int64_t n_lat = (int64_t)ny * nz;
@@ -193,14 +234,17 @@ int build(const uint8_t *solid_host, int nx, int ny, int nz, double H_m, Geometr
     PRH_HIP(hipMalloc((void **)&face_base, n_cells * sizeof(int64_t)));
     PRH_HIP(hipMalloc((void **)&wall_count, n_lat * sizeof(int32_t)));
     PRH_HIP(hipMalloc((void **)&wall_base, n_lat * sizeof(int64_t)));
-    PRH_HIP(hipMalloc((void **)&g.occ, (size_t)nx * ny * g.nzw * sizeof(uint32_t)));
+    PRH_HIP(hipMalloc((void **)&g.occ, (size_t)g.occ_words * sizeof(uint32_t)));
+    PRH_HIP(hipMalloc((void **)&g.coarse, (size_t)g.coarse_words * sizeof(uint32_t)));
     PRH_HIP(hipMalloc((void **)&g.cell_code, n_cells * sizeof(uint32_t)));
     PRH_HIP(hipMalloc((void **)&g.hot_map, n_lat * sizeof(int32_t)));
     PRH_HIP(hipMalloc((void **)&g.cold_map, n_lat * sizeof(int32_t)));
     PRH_HIP(hipMemcpyAsync(solid, solid_host, n_cells, hipMemcpyHostToDevice, st));
 
     k_cell_masks<<<grid_for(n_cells), GEO_THREADS, 0, st>>>(solid, nx, ny, nz, mask, face_count);
-    k_pack_occupancy<<<grid_for((int64_t)nx * ny * g.nzw), GEO_THREADS, 0, st>>>(solid, nx, ny, nz, g.nzw, g.occ);
+    k_pack_occupancy<<<grid_for(g.occ_words), GEO_THREADS, 0, st>>>(solid, n_cells, g.occ);
+    k_pack_coarse<<<grid_for(g.coarse_words), GEO_THREADS, 0, st>>>(solid, nx, ny, nz, g.coarse_shift, g.cnx, g.cny,
+                                                                    g.cnz, g.coarse);
     k_wall_counts<<<grid_for(n_lat), GEO_THREADS, 0, st>>>(solid, nx, ny, nz, wall_count);
     PRH_HIP(hipGetLastError());
     int64_t n_internal = 0, n_wall = 0;

@@ -75,12 +75,18 @@ void host_seed_words(const uint32_t *words, int n_words, uint64_t out[4]);
 
 // ------------------------------------------------------------------ geometry
 // Device-resident patch table of one x-oriented microstructure.
-//   occ        bit-packed occupancy, word ((i*ny + j)*nzw + (k >> 5)), bit k & 31
+//   occ        occupancy bitset: bit c = cell (i*ny + j)*nz + k (word c >> 5, bit c & 31)
+//   coarse     brick bitset: bit b = brick ((i>>s)*cny + (j>>s))*cnz + (k>>s) is set when
+//              the 2^s-cell brick holds any solid cell; small enough to sit in LDS
 //   cell_code  per cell: (first patch id of the cell << 6) | exposed-face mask
 //   patch_*    per patch: origin (x,y,z) FP64, face 0..5 (walls: 0 hot, 1 cold)
 //   hot_map / cold_map  wall patch id per (j, k), -1 where the boundary cell is solid
 struct Geometry {
-    int32_t nx, ny, nz, nzw;
+    int32_t nx, ny, nz;
+    int64_t occ_words;
+    int32_t coarse_shift, cnx, cny, cnz;
+    int64_t coarse_words;
+    uint32_t *coarse;
     double H_m;
     int64_t n_patches, n_wall;
     int device;

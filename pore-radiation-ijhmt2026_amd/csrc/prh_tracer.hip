@@ -31,32 +31,36 @@ __constant__ JumpTables c_jump;
 namespace {
 
 constexpr int TRACE_THREADS = 256;
+constexpr int STREAM_THREADS = 1024;  // traversal: 16 waves share one LDS brick bitset
 constexpr int COUNT_THREADS = 256;
 constexpr int MAX_RAYS_PER_PATCH_CALL = 16384;
 
 struct GeoView {
     const uint32_t *occ;
+    const uint32_t *coarse;
     const uint32_t *cell_code;
     const int32_t *hot_map;
     const int32_t *cold_map;
     const double *patch_origin;
     const int8_t *patch_face;
     const int8_t *patch_class;
-    int nx, ny, nz, nzw;
+    int nx, ny, nz;
+    int coarse_shift, cny, cnz;
+    uint32_t coarse_words;
 };
 
 GeoView view_of(const Geometry &g)
 {
-    return GeoView{g.occ, g.cell_code, g.hot_map, g.cold_map, g.patch_origin, g.patch_face, g.patch_class,
-                   g.nx, g.ny, g.nz, g.nzw};
+    return GeoView{g.occ, g.coarse, g.cell_code, g.hot_map, g.cold_map, g.patch_origin, g.patch_face,
+                   g.patch_class, g.nx, g.ny, g.nz, g.coarse_shift, g.cny, g.cnz, (uint32_t)g.coarse_words};
 }
-
-enum : int { RUNNING = 0, AT_WALL = 1, AT_SOLID = 2, FAILED = 3 };
 
 // Per-lane traversal state (the loop variables of _trace_kernel).  The face
 // of the last crossing is kept as its axis (priority x > y > z) and turned
-// into the face index 0..5 only when the ray ends; the step cap is a
-// countdown, and the number of occupancy reads follows from the steps taken.
+// into the face index 0..5 only when the ray ends.  The step cap is a
+// countdown; why a ray stopped follows from the state: left == 0 is the step
+// cap, i outside [0, nx) the wall, anything else a solid cell.  The number
+// of occupancy reads follows from the steps taken.
 struct Ray {
     double oy, oz, dy, dz;
     double tx, ty, tz, ddx, ddy, ddz, time;
@@ -65,7 +69,6 @@ struct Ray {
     int axis;       // axis of the last crossing, -1 before the first
     uint32_t left;  // iterations left before the step cap
     uint32_t cap;   // step cap of this ray
-    int term;
 };
 
 __device__ __forceinline__ int normal_code_of(const GeoView &G, int64_t p)
@@ -74,8 +77,38 @@ __device__ __forceinline__ int normal_code_of(const GeoView &G, int64_t p)
     return cls == 2 ? (int)G.patch_face[p] : cls;
 }
 
+// v_min_f64 without the NaN canonicalisation the compiler adds around fmin
+// for loop-carried values; the crossing times are never NaN (finite or the
+// 1e300 sentinel), and for equal values either operand is the same number.
+__device__ __forceinline__ double vmin_f64(double a, double b)
+{
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
+// Launch constants of the traversal.
+struct TraversalConst {
+    uint32_t min_cap;    // 10 nx ny nz + 10 (view_factors.py:224)
+    double grazing_dx;   // below this |dx| the crossing-scale cap may exceed min_cap
+};
+
+TraversalConst traversal_const(const Geometry &g)
+{
+    TraversalConst c;
+    const int64_t min_cap = 10 * (int64_t)g.nx * g.ny * g.nz + 10;
+    c.min_cap = (uint32_t)(min_cap < 0xFFFFFFFFll ? min_cap : 0xFFFFFFFFll);
+    // crossing_scale = 8 nx (1 + (|dy|+|dz|)/|dx|) + 100 <= 8 nx + 16 nx/|dx| + 100,
+    // which is <= min_cap once |dx| >= 16 nx / (min_cap - 8 nx - 100); a factor 2
+    // keeps the test conservative against rounding.
+    const double room = (double)min_cap - 8.0 * g.nx - 100.0;
+    c.grazing_dx = room > 0.0 ? 2.0 * 16.0 * g.nx / room : 2.0;
+    return c;
+}
+
 // view_factors.py:211-224.  Returns false for dx == 0 (failed ray).
-__device__ __forceinline__ bool ray_setup(Ray &R, const double o[3], const double d[3], const GeoView &G)
+__device__ __forceinline__ bool ray_setup(Ray &R, const double o[3], const double d[3], const GeoView &G,
+                                          const TraversalConst &C)
 {
     const double ox = o[0], dx = d[0];
     R.oy = o[1];
@@ -89,59 +122,80 @@ __device__ __forceinline__ bool ray_setup(Ray &R, const double o[3], const doubl
     axis_state(ox, dx, R.i, R.sx, R.tx, R.ddx);
     axis_state(R.oy, R.dy, j, R.sy, R.ty, R.ddy);
     axis_state(R.oz, R.dz, k, R.sz, R.tz, R.ddz);
-    R.jj = (int)py_imod64(j, G.ny);
-    R.kk = (int)py_imod64(k, G.nz);
+    // Origins lie in [0, n] (wrapped by float modulo), so j, k are in [-1, n].
+    R.jj = j < 0 ? j + G.ny : (j >= G.ny ? j - G.ny : j);
+    R.kk = k < 0 ? k + G.nz : (k >= G.nz ? k - G.nz : k);
     R.axis = -1;
     R.time = 0.0;
-    R.cap = max_steps_of(dx, R.dy, R.dz, G.nx, G.ny, G.nz);
+    R.cap = C.min_cap;
+    if (fabs(dx) < C.grazing_dx) R.cap = max_steps_of(dx, R.dy, R.dz, G.nx, G.ny, G.nz);
     R.left = R.cap;
-    R.term = RUNNING;
     return true;
 }
 
-// One iteration of the traversal loop, view_factors.py:225-255.  The minimum
-// of the three crossing times is the value of one of them, so at least one
-// axis crosses; updates are written as selects to keep the wave converged.
-__device__ __forceinline__ void ray_step(Ray &R, const GeoView &G)
+// Occupancy test of the current cell.  With COARSE, the brick bitset held in
+// LDS answers first; only lanes in a brick that holds solid read the cell's
+// bit from the global bitset (L2-resident), so most steps never reach the
+// vector memory pipeline.
+template <bool COARSE>
+__device__ __forceinline__ bool cell_is_solid(const Ray &R, const GeoView &G, const uint32_t *lds_coarse)
 {
-    if ((unsigned)R.i >= (unsigned)G.nx) {
-        R.term = AT_WALL;
-        return;
+    if (COARSE) {
+        const unsigned s = (unsigned)G.coarse_shift;
+        const uint32_t b = (((uint32_t)R.i >> s) * (uint32_t)G.cny + ((uint32_t)R.jj >> s)) * (uint32_t)G.cnz +
+                           ((uint32_t)R.kk >> s);
+        if (!((lds_coarse[b >> 5] >> (b & 31u)) & 1u)) return false;
     }
-    const unsigned row = (unsigned)R.i * (unsigned)G.ny + (unsigned)R.jj;
-    const uint32_t word = G.occ[row * (unsigned)G.nzw + ((unsigned)R.kk >> 5)];
-    if ((word >> (R.kk & 31)) & 1u) {
-        R.term = AT_SOLID;
-        return;
-    }
-    const double nt = fmin(fmin(R.tx, R.ty), R.tz);
+    const uint32_t cell = ((uint32_t)R.i * (uint32_t)G.ny + (uint32_t)R.jj) * (uint32_t)G.nz + (uint32_t)R.kk;
+    const uint32_t word = *(const uint32_t *)((const char *)G.occ + ((cell >> 3) & ~3u));
+    return (word >> (cell & 31u)) & 1u;
+}
+
+// One iteration of the traversal loop, view_factors.py:225-255; returns
+// whether the ray goes on.  The minimum of the three crossing times is the
+// value of one of them, so at least one axis crosses.  Updates are selects
+// (the wave stays converged); a lateral index leaving [0, n) is wrapped by
+// subtracting step * n.
+template <bool COARSE>
+__device__ __forceinline__ bool ray_step(Ray &R, const GeoView &G, const uint32_t *lds_coarse)
+{
+    if ((unsigned)R.i >= (unsigned)G.nx) return false;
+    if (cell_is_solid<COARSE>(R, G, lds_coarse)) return false;
+    const double nt = vmin_f64(vmin_f64(R.tx, R.ty), R.tz);
     const bool cx = fabs(R.tx - nt) <= TIE_TOLERANCE;
     const bool cy = fabs(R.ty - nt) <= TIE_TOLERANCE;
     const bool cz = fabs(R.tz - nt) <= TIE_TOLERANCE;
     R.i += cx ? R.sx : 0;
     R.tx = cx ? R.tx + R.ddx : R.tx;
-    int jn = R.jj + (cy ? R.sy : 0);
-    R.jj = jn >= G.ny ? 0 : (jn < 0 ? G.ny - 1 : jn);
+    const int jn = R.jj + (cy ? R.sy : 0);
+    R.jj = (unsigned)jn < (unsigned)G.ny ? jn : jn - __mul24(R.sy, G.ny);
     R.ty = cy ? R.ty + R.ddy : R.ty;
-    int kn = R.kk + (cz ? R.sz : 0);
-    R.kk = kn >= G.nz ? 0 : (kn < 0 ? G.nz - 1 : kn);
+    const int kn = R.kk + (cz ? R.sz : 0);
+    R.kk = (unsigned)kn < (unsigned)G.nz ? kn : kn - __mul24(R.sz, G.nz);
     R.tz = cz ? R.tz + R.ddz : R.tz;
     R.axis = cx ? 0 : (cy ? 1 : 2);
     R.time = nt;
-    if (--R.left == 0u) R.term = FAILED;
+    return --R.left != 0u;
+}
+
+__device__ __forceinline__ bool ray_at_wall(const Ray &R, const GeoView &G)
+{
+    return R.left != 0u && (unsigned)R.i >= (unsigned)G.nx;
 }
 
 // Occupancy reads of a finished traversal: one per step plus the read that
 // found the solid cell.
-__device__ __forceinline__ uint32_t ray_reads(const Ray &R)
+__device__ __forceinline__ uint32_t ray_reads(const Ray &R, const GeoView &G)
 {
-    return (R.cap - R.left) + (R.term == AT_SOLID ? 1u : 0u);
+    const bool at_solid = R.left != 0u && (unsigned)R.i < (unsigned)G.nx;
+    return (R.cap - R.left) + (at_solid ? 1u : 0u);
 }
 
 // Receiving patch of a finished traversal (view_factors.py:226-236), -1 on failure.
 __device__ __forceinline__ int ray_receiver(const Ray &R, const GeoView &G)
 {
-    if (R.term == AT_WALL) {
+    if (R.left == 0u) return -1;
+    if ((unsigned)R.i >= (unsigned)G.nx) {
         const double yw = R.oy + R.dy * R.time;
         const double zw = R.oz + R.dz * R.time;
         const int64_t jw = py_imod64((int64_t)floor(yw), G.ny);
@@ -149,20 +203,17 @@ __device__ __forceinline__ int ray_receiver(const Ray &R, const GeoView &G)
         const int64_t c = jw * G.nz + kw;
         return R.i < 0 ? G.hot_map[c] : G.cold_map[c];
     }
-    if (R.term == AT_SOLID) {
-        const int axis = R.axis;
-        if (axis < 0) return -1;
-        // Direction bits as values (a select between struct fields would be
-        // lowered to an indexed stack access).
-        const uint32_t up = (R.sx > 0 ? 1u : 0u) | (R.sy > 0 ? 2u : 0u) | (R.sz > 0 ? 4u : 0u);
-        const int face = 2 * axis + (int)((up >> axis) & 1u);
-        const int64_t cell = ((int64_t)R.i * G.ny + R.jj) * G.nz + R.kk;
-        const uint32_t code = G.cell_code[cell];
-        const uint32_t m = code & 63u;
-        if (!((m >> face) & 1u)) return -1;
-        return (int)(code >> 6) + __popc(m & ((1u << face) - 1u));
-    }
-    return -1;
+    const int axis = R.axis;
+    if (axis < 0) return -1;
+    // Direction bits as values (a select between struct fields would be
+    // lowered to an indexed stack access).
+    const uint32_t up = (R.sx > 0 ? 1u : 0u) | (R.sy > 0 ? 2u : 0u) | (R.sz > 0 ? 4u : 0u);
+    const int face = 2 * axis + (int)((up >> axis) & 1u);
+    const int64_t cell = ((int64_t)R.i * G.ny + R.jj) * G.nz + R.kk;
+    const uint32_t code = G.cell_code[cell];
+    const uint32_t m = code & 63u;
+    if (!((m >> face) & 1u)) return -1;
+    return (int)(code >> 6) + __popc(m & ((1u << face) - 1u));
 }
 
 __device__ __forceinline__ void load_seed(const uint64_t *seeds, int s, u128 &st, u128 &inc)
@@ -178,11 +229,13 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 }
 
 // ------------------------------------------------------------------ ray generation
-// Ray g of a launch is (seed s, patch p, ray r), g = s*(n_p*R) + (p-p0)*R + r.
-// Within one seed consecutive g are consecutive positions of that seed's
-// PCG64 stream (4 draws per ray), so a thread that owns GEN_PER_THREAD
-// consecutive rays jumps its stream once and then only draws.
-constexpr uint32_t GEN_PER_THREAD = 16;
+// Ray g of a launch is (seed s, patch p, ray r), g = s*(n_p*R) + (p-p0)*R + r,
+// and uses draws 4 (p R + r) .. 4 (p R + r) + 3 of seed s's PCG64 stream.  A
+// wavefront generates GEN_ROUNDS x 64 consecutive rays; lane l takes rays
+// base + l + 64 k, so every round stores 64 adjacent records (coalesced) and
+// a lane's next ray is 256 draws further on its stream: one jump with the
+// fixed 2^8 table entry instead of a full jump-ahead.
+constexpr uint32_t GEN_ROUNDS = 16;
 
 struct GenArgs {
     GeoView G;
@@ -196,24 +249,35 @@ struct GenArgs {
 __global__ __launch_bounds__(TRACE_THREADS) void k_gen_rays(GenArgs a)
 {
     const GeoView &G = a.G;
-    const uint64_t g0 = (uint64_t)(blockIdx.x * TRACE_THREADS + threadIdx.x) * GEN_PER_THREAD;
-    if (g0 >= a.total) return;
-    const uint32_t g_end = (uint32_t)(g0 + GEN_PER_THREAD < a.total ? g0 + GEN_PER_THREAD : a.total);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t wave = ((uint64_t)blockIdx.x * TRACE_THREADS + threadIdx.x) >> 6;
+    const uint64_t base = wave * (64ull * GEN_ROUNDS) + lane;
     const uint32_t seg = a.n_p * a.R;
-    uint32_t g = (uint32_t)g0;
-    uint32_t s = g / seg;
-    const uint32_t rem = g - s * seg;
-    const uint32_t q = rem / a.R;
-    uint32_t r = rem - q * a.R;
-    uint32_t p = a.p0 + q;
-    u128 st0, inc;
-    load_seed(a.seeds, (int)s, st0, inc);
-    u128 st = pcg_jump(st0, inc, 4ull * ((uint64_t)p * a.R + r), c_jump);
     const double fny = (double)G.ny, fnz = (double)G.nz;
-    for (; g < g_end; ++g) {
+    uint32_t s_cur = 0xFFFFFFFFu;
+    u128 st = make_u128(0, 0), inc = make_u128(0, 0), step_inc = make_u128(0, 0);
+    for (uint32_t k = 0; k < GEN_ROUNDS; ++k) {
+        const uint64_t g64 = base + 64ull * k;
+        if (g64 >= a.total) break;
+        const uint32_t g = (uint32_t)g64;
+        const uint32_t s = g / seg;
+        const uint32_t rem = g - s * seg;
+        const uint32_t q = rem / a.R;
+        const uint32_t r = rem - q * a.R;
+        const uint32_t p = a.p0 + q;
+        if (s != s_cur) {
+            // first ray of this lane, or the stride crossed into the next seed
+            u128 st0;
+            load_seed(a.seeds, (int)s, st0, inc);
+            st = pcg_jump(st0, inc, 4ull * ((uint64_t)p * a.R + r), c_jump);
+            step_inc = mul128(c_jump.G[8], inc);
+            s_cur = s;
+        }
+        u128 next = add128(mul128(c_jump.A[8], st), step_inc);  // 64 rays = 256 draws on
         const double *po = G.patch_origin + 3 * (size_t)p;
         double o[3], d[3];
         make_ray(st, inc, po[0], po[1], po[2], normal_code_of(G, p), fny, fnz, o, d);
+        st = next;
         double *oo = a.origins + 3 * (size_t)g;
         double *dd = a.directions + 3 * (size_t)g;
         oo[0] = o[0];
@@ -222,17 +286,6 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_gen_rays(GenArgs a)
         dd[0] = d[0];
         dd[1] = d[1];
         dd[2] = d[2];
-        if (++r == a.R) {
-            r = 0;
-            if (++p == a.p0 + a.n_p) {
-                p = a.p0;
-                ++s;
-                if (g + 1 < g_end) {
-                    load_seed(a.seeds, (int)s, st0, inc);
-                    st = pcg_jump(st0, inc, 4ull * ((uint64_t)a.p0 * a.R), c_jump);
-                }
-            }
-        }
     }
 }
 
@@ -243,6 +296,7 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_gen_rays(GenArgs a)
 // chunk in lane order (adjacent ray records, so the refill loads coalesce).
 struct TraceArgs {
     GeoView G;
+    TraversalConst C;
     uint32_t total, per_wave;
     const double *origins;
     const double *directions;
@@ -251,12 +305,17 @@ struct TraceArgs {
     unsigned long long *failed;
 };
 
-template <int REFILL_MIN, int WAVES_PER_EU>
-__global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU)))
+template <int REFILL_MIN, bool COARSE>
+__global__ __launch_bounds__(STREAM_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
 void k_trace_stream(TraceArgs a)
 {
     const GeoView &G = a.G;
-    const uint32_t wave = (blockIdx.x * TRACE_THREADS + threadIdx.x) >> 6;
+    extern __shared__ uint32_t lds_coarse[];
+    if (COARSE) {
+        for (uint32_t w = threadIdx.x; w < G.coarse_words; w += STREAM_THREADS) lds_coarse[w] = G.coarse[w];
+        __syncthreads();
+    }
+    const uint32_t wave = (blockIdx.x * STREAM_THREADS + threadIdx.x) >> 6;
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
     const uint64_t wb = (uint64_t)wave * a.per_wave;
@@ -265,8 +324,8 @@ void k_trace_stream(TraceArgs a)
     uint32_t cursor = w_begin;
 
     Ray R;
-    R.term = RUNNING;
     R.i = 0;
+    R.left = 1;
     bool active = false, pending = false;
     uint32_t g_cur = 0;
     uint64_t reads = 0;
@@ -276,7 +335,7 @@ void k_trace_stream(TraceArgs a)
         if (!active && pending) {
             const int rec = ray_receiver(R, G);
             failed += rec < 0 ? 1u : 0u;
-            reads += ray_reads(R);
+            reads += ray_reads(R, G);
             a.receiver[g_cur] = rec;
             pending = false;
         }
@@ -289,7 +348,7 @@ void k_trace_stream(TraceArgs a)
                     const double *pd = a.directions + 3 * (size_t)g;
                     const double o[3] = {po[0], po[1], po[2]};
                     const double d[3] = {pd[0], pd[1], pd[2]};
-                    if (ray_setup(R, o, d, G)) {
+                    if (ray_setup(R, o, d, G, a.C)) {
                         active = true;
                         pending = true;
                         g_cur = g;
@@ -306,10 +365,7 @@ void k_trace_stream(TraceArgs a)
             continue;
         }
         while (true) {
-            if (active) {
-                ray_step(R, G);
-                active = R.term == RUNNING;
-            }
+            if (active) active = ray_step<COARSE>(R, G, lds_coarse);
             const uint64_t act = __ballot(active);
             if (act == 0) break;
             if (cursor < w_end && __popcll(~act) >= REFILL_MIN) break;
@@ -445,30 +501,29 @@ double g_last_ms[4] = {0, 0, 0, 0};
 
 int grow(Buf &b, size_t bytes) { return ensure_capacity(&b.p, &b.cap, bytes); }
 
-int resident_waves(const void *kernel, int64_t *waves)
+int resident_waves(const void *kernel, int threads, size_t lds, int64_t *waves)
 {
     int dev = 0, cus = 0, per_cu = 0;
     PRH_HIP(hipGetDevice(&dev));
     PRH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    PRH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, TRACE_THREADS, 0));
+    PRH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds));
     if (per_cu < 1) per_cu = 1;
-    *waves = (int64_t)cus * per_cu * (TRACE_THREADS / 64);
+    *waves = (int64_t)cus * per_cu * (threads / 64);
     return PRH_OK;
 }
 
-// Traversal kernel variants: refill threshold x register budget (7 waves per
-// SIMD without spills, or 8 with the refill path spilling a few registers).
-// The variant is a tuning knob (prh_tracer_set_variant, chosen by the
-// autotuner in porerad/autotune.py).
+// Traversal kernel variants: refill threshold x occupancy test (LDS brick
+// bitset first, or every test from the global bitset).  The variant is a
+// tuning knob (prh_tracer_set_variant, chosen by porerad/autotune.py).
 using TraceKernel = void (*)(TraceArgs);
 struct TraceVariant {
     TraceKernel fn;
-    int refill_min, waves_per_eu;
+    int refill_min, coarse;
 };
 const TraceVariant TRACE_VARIANTS[] = {
-    {k_trace_stream<16, 1>, 16, 1}, {k_trace_stream<8, 1>, 8, 1},  {k_trace_stream<32, 1>, 32, 1},
-    {k_trace_stream<16, 8>, 16, 8}, {k_trace_stream<8, 8>, 8, 8},  {k_trace_stream<32, 8>, 32, 8},
-    {k_trace_stream<4, 8>, 4, 8},   {k_trace_stream<24, 8>, 24, 8},
+    {k_trace_stream<32, true>, 32, 1},  {k_trace_stream<16, true>, 16, 1}, {k_trace_stream<24, true>, 24, 1},
+    {k_trace_stream<40, true>, 40, 1},  {k_trace_stream<48, true>, 48, 1}, {k_trace_stream<32, false>, 32, 0},
+    {k_trace_stream<16, false>, 16, 0}, {k_trace_stream<24, false>, 24, 0},
 };
 constexpr int N_TRACE_VARIANTS = sizeof(TRACE_VARIANTS) / sizeof(TRACE_VARIANTS[0]);
 int g_trace_variant = 0;
@@ -477,8 +532,8 @@ int launch_gen(const Geometry &g, uint32_t p0, uint32_t n_p, uint32_t R, const u
                double *d_orig, double *d_dir, hipStream_t st)
 {
     if (total == 0) return PRH_OK;
-    const uint64_t threads = (total + GEN_PER_THREAD - 1) / GEN_PER_THREAD;
-    const unsigned grid = (unsigned)((threads + TRACE_THREADS - 1) / TRACE_THREADS);
+    const uint64_t waves = (total + 64ull * GEN_ROUNDS - 1) / (64ull * GEN_ROUNDS);
+    const unsigned grid = (unsigned)((waves * 64 + TRACE_THREADS - 1) / TRACE_THREADS);
     GenArgs a{view_of(g), p0, n_p, R, d_seeds, total, d_orig, d_dir};
     k_gen_rays<<<grid, TRACE_THREADS, 0, st>>>(a);
     PRH_HIP(hipGetLastError());
@@ -490,16 +545,18 @@ int launch_trace(const Geometry &g, const double *d_orig, const double *d_dir, u
 {
     if (total == 0) return PRH_OK;
     const TraceVariant &V = TRACE_VARIANTS[g_trace_variant];
+    const size_t lds = V.coarse ? (size_t)g.coarse_words * sizeof(uint32_t) : 0;
     int64_t waves = 0;
-    int rc = resident_waves((const void *)V.fn, &waves);
+    int rc = resident_waves((const void *)V.fn, STREAM_THREADS, lds, &waves);
     if (rc) return rc;
     // Enough work per wave to amortise the tail, enough waves to fill the chip.
     int64_t per_wave = (total + waves - 1) / waves;
     if (per_wave < 64) per_wave = 64;
     const int64_t n_waves = (total + per_wave - 1) / per_wave;
-    const unsigned grid = (unsigned)((n_waves * 64 + TRACE_THREADS - 1) / TRACE_THREADS);
-    TraceArgs a{view_of(g), total, (uint32_t)per_wave, d_orig, d_dir, d_receiver, d_counters, d_counters + 1};
-    V.fn<<<grid, TRACE_THREADS, 0, st>>>(a);
+    const unsigned grid = (unsigned)((n_waves * 64 + STREAM_THREADS - 1) / STREAM_THREADS);
+    TraceArgs a{view_of(g), traversal_const(g), total, (uint32_t)per_wave, d_orig, d_dir, d_receiver, d_counters,
+                d_counters + 1};
+    V.fn<<<grid, STREAM_THREADS, lds, st>>>(a);
     PRH_HIP(hipGetLastError());
     return PRH_OK;
 }
@@ -535,13 +592,13 @@ int prh_tracer_set_variant(int32_t variant)
     return PRH_OK;
 }
 
-int prh_tracer_variants(int32_t *n_variants, int32_t *current, int32_t *refill_min, int32_t *waves_per_eu)
+int prh_tracer_variants(int32_t *n_variants, int32_t *current, int32_t *refill_min, int32_t *coarse)
 {
     if (n_variants) *n_variants = N_TRACE_VARIANTS;
     if (current) *current = g_trace_variant;
     for (int v = 0; v < N_TRACE_VARIANTS; ++v) {
         if (refill_min) refill_min[v] = TRACE_VARIANTS[v].refill_min;
-        if (waves_per_eu) waves_per_eu[v] = TRACE_VARIANTS[v].waves_per_eu;
+        if (coarse) coarse[v] = TRACE_VARIANTS[v].coarse;
     }
     return PRH_OK;
 }

@@ -176,3 +176,20 @@ def test_empty_box_counts(vf):
     ref = O.first_hit_counts(Q, 50, 3)
     assert all(np.array_equal(x, y) for x, y in zip(got, ref))
     assert np.all(P.patch_class[got[0]] != P.patch_class[got[1]])
+
+
+def test_every_traversal_variant_gives_identical_counts(vf):
+    from porerad import _native
+
+    phases = _small_random(6, (24, 20, 20), 0.12)
+    P, Q = _both(vf, ("small6", "x"), phases)
+    ref = O.first_hit_counts(Q, 300, 77)
+    variants = _native.tracer_variants()
+    assert len(variants) >= 2
+    try:
+        for v in variants:
+            _native.set_tracer_variant(v["variant"])
+            got = vf.first_hit_counts(P, 300, 77)
+            assert all(np.array_equal(a, b) for a, b in zip(got, ref)), v
+    finally:
+        _native.set_tracer_variant(0)
