@@ -55,11 +55,13 @@ extern "C" {
 #define PRH_APPLY_RAW_T 4     /* F^T x            (raw_rmatvec)      */
 
 /* ---- radiosity precisions (prh_radiosity) ---------------------------- */
-#define PRH_FP64 0 /* plain FP64 reflection series */
+#define PRH_FP64 0  /* plain FP64 reflection series */
+#define PRH_MIXED 1 /* batched FP32 Chebyshev sweeps + FP64 residual refinement */
 
 typedef struct prh_geometry prh_geometry;
 typedef struct prh_edges prh_edges;
 typedef struct prh_operator prh_operator;
+typedef struct prh_batch prh_batch;
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *prh_last_error(void);
@@ -153,6 +155,31 @@ int prh_radiosity(prh_operator *op, int32_t n_rhs, const double *emitted, const 
                   int32_t emissivity_per_rhs, double relative_tolerance, int32_t max_reflections,
                   int32_t precision, double *absorbed, double *residual, int32_t *iterations);
 void prh_operator_destroy(prh_operator *op);
+
+/* ---- batched radiosity engine (continuous batching) -------------------
+ * Up to 16 emissions ("slots") are solved together on one operator: each
+ * epoch runs FP32 Chebyshev sweeps for every running slot and one FP64
+ * residual sweep (iterative refinement); a slot finishes when
+ * ||E - (I - diag(1-eps) F_p^T) J||_1 <= relative_tolerance ||E||_1, and
+ * returns the absorbed power eps * F_p^T J.  Slots are admitted between
+ * epochs while others run.  Same contract as reflection_series
+ * (porerad/exchange.py:217-245); the serving loop is built on it.
+ * The batch is bound to its operator and must be destroyed before it. */
+int prh_batch_create(prh_operator *op, prh_batch **out);
+/* Device bytes held by the batch (for HBM-budget admission). */
+int prh_batch_bytes(const prh_batch *b, int64_t *bytes);
+/* Bit q set = slot q free. */
+int prh_batch_free_slots(const prh_batch *b, uint32_t *free_mask);
+/* Admit one emission (n values) with emissivity per patch (eps_per_patch)
+ * or one value; returns the slot, PRH_ERR_CAPACITY when all are busy. */
+int prh_batch_admit(prh_batch *b, const double *emitted, const double *emissivity, int32_t eps_per_patch,
+                    double relative_tolerance, int32_t *slot);
+/* Run one epoch; done_mask: finished slots waiting for prh_batch_result. */
+int prh_batch_epoch(prh_batch *b, uint32_t *done_mask, uint32_t *active_mask);
+/* Absorbed power (n), power still in flight and sweep count of a finished
+ * slot; frees the slot.  PRH_ERR_SERIES if it did not converge. */
+int prh_batch_result(prh_batch *b, int32_t slot, double *absorbed, double *residual, int32_t *iterations);
+void prh_batch_destroy(prh_batch *b);
 
 #ifdef __cplusplus
 }

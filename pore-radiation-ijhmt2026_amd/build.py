@@ -29,6 +29,7 @@ SOURCES = [
     "prh_geometry.hip",
     "prh_tracer.hip",
     "prh_operator.hip",
+    "prh_batch.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "prh_internal.h"
+#include "prh_operator_impl.h"
 
 namespace prh {
 namespace {
@@ -78,14 +79,6 @@ __global__ void k_scale_out(const double *sx, const double *d, const double *are
     if (i >= n) return;
     y[i] = mode == PRH_APPLY_MATVEC ? (d[i] / area[i]) * sx[i] : d[i] * sx[i];
 }
-
-struct SeriesState {
-    double initial;   // max(||E||_1, tiny)
-    double l1;        // ||flight||_1 of the last reflection
-    double residual;  // sum(flight) at convergence
-    int32_t done;
-    int32_t iterations;
-};
 
 // flight = E, absorbed = 0, w = d (E / A), per-block sums of |E|.
 __global__ __launch_bounds__(RED_THREADS) void k_series_init(const double *E, const double *d, const double *area,
@@ -215,28 +208,6 @@ unsigned blocks_for(int64_t n, int per_block)
 }  // namespace
 }  // namespace prh
 
-struct prh_operator {
-    int device = 0;
-    int64_t n = 0;
-    // symmetric S (FP64 CSR)
-    int64_t nnz_s = 0;
-    int64_t *rowptr = nullptr;
-    int32_t *col = nullptr;
-    double *val = nullptr;
-    // raw counts: C by origin and C^T by receiver
-    int64_t nnz_c = 0;
-    int64_t *c_rowptr = nullptr, *t_rowptr = nullptr;
-    int32_t *c_col = nullptr, *c_cnt = nullptr, *t_col = nullptr, *t_cnt = nullptr;
-    double *rays = nullptr, *area = nullptr, *scale = nullptr;
-    double delta = 0.0;
-    // work vectors (n each)
-    double *vx = nullptr, *vy = nullptr, *vt = nullptr;
-    // series work
-    double *flight = nullptr, *absorbed = nullptr, *w0 = nullptr, *w1 = nullptr, *eps = nullptr, *emit = nullptr;
-    double *part_abs = nullptr, *part_sum = nullptr;
-    prh::SeriesState *state = nullptr;
-};
-
 using namespace prh;
 
 namespace {
@@ -246,7 +217,7 @@ void free_operator(prh_operator *op)
     void *ptrs[] = {op->rowptr, op->col, op->val, op->c_rowptr, op->t_rowptr, op->c_col, op->c_cnt,
                     op->t_col, op->t_cnt, op->rays, op->area, op->scale, op->vx, op->vy, op->vt,
                     op->flight, op->absorbed, op->w0, op->w1, op->eps, op->emit, op->part_abs,
-                    op->part_sum, op->state};
+                    op->part_sum, op->state, op->kval, op->val32, op->scale32};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -256,6 +227,7 @@ struct HostCSR {
     std::vector<int64_t> c_rowptr, t_rowptr, s_rowptr;
     std::vector<int32_t> t_col, t_cnt, s_col;
     std::vector<double> s_val;
+    std::vector<int64_t> s_k;  // K_ij = C_ij + C_ji on the sparsity of S
 };
 
 void assemble(int64_t n, const int32_t *origin, const int32_t *receiver, const int64_t *count, int64_t nnz,
@@ -313,6 +285,7 @@ void assemble(int64_t n, const int32_t *origin, const int32_t *receiver, const i
     const int64_t nnz_s = H.s_rowptr[n];
     H.s_col.resize(nnz_s);
     H.s_val.resize(nnz_s);
+    H.s_k.resize(nnz_s);
     par([&](int64_t i) {
         int64_t a = H.c_rowptr[i], ae = H.c_rowptr[i + 1], b = H.t_rowptr[i], be = H.t_rowptr[i + 1];
         int64_t out = H.s_rowptr[i];
@@ -326,6 +299,7 @@ void assemble(int64_t n, const int32_t *origin, const int32_t *receiver, const i
             if (j == i) v += delta * Ai;
             H.s_col[out] = j;
             H.s_val[out] = v;
+            H.s_k[out] = cij + cji;
             ++out;
         };
         while (a < ae || b < be) {
@@ -451,6 +425,32 @@ int prh_operator_create(int64_t n, const int32_t *origin, const int32_t *receive
     if ((rc = upload(&op->rays, rays_d.data(), n, st))) return fail(rc);
     if ((rc = upload(&op->area, area, n, st))) return fail(rc);
     if ((rc = upload(&op->scale, ones.data(), n, st))) return fail(rc);
+    // Compact forms for the batched engine: exact integer K when every patch
+    // has the same area and the same ray count (S = A/(2R) K + A delta I),
+    // otherwise an FP32 copy of S.
+    {
+        bool uniform = true;
+        for (int64_t i = 1; i < n && uniform; ++i)
+            uniform = area[i] == area[0] && rays_per_origin[i] == rays_per_origin[0];
+        int64_t kmax = 0;
+        for (int64_t e = 0; e < op->nnz_s; ++e) kmax = std::max(kmax, H.s_k[e]);
+        op->uniform = uniform && kmax <= 65535;
+        if (op->uniform) {
+            op->inv_2R = 1.0 / (2.0 * (double)rays_per_origin[0]);
+            std::vector<uint16_t> k16(op->nnz_s);
+            for (int64_t e = 0; e < op->nnz_s; ++e) k16[e] = (uint16_t)H.s_k[e];
+            if ((rc = upload(&op->kval, k16.data(), op->nnz_s, st))) return fail(rc);
+            if (hipStreamSynchronize(st) != hipSuccess) return fail(PRH_ERR_HIP);
+        } else {
+            std::vector<float> v32(op->nnz_s);
+            for (int64_t e = 0; e < op->nnz_s; ++e) v32[e] = (float)H.s_val[e];
+            if ((rc = upload(&op->val32, v32.data(), op->nnz_s, st))) return fail(rc);
+            if (hipStreamSynchronize(st) != hipSuccess) return fail(PRH_ERR_HIP);
+        }
+        std::vector<float> ones32(n, 1.0f);
+        if ((rc = upload(&op->scale32, ones32.data(), n, st))) return fail(rc);
+        if (hipStreamSynchronize(st) != hipSuccess) return fail(PRH_ERR_HIP);
+    }
     double **work[] = {&op->vx, &op->vy, &op->vt, &op->flight, &op->absorbed, &op->w0, &op->w1, &op->eps, &op->emit};
     for (double **p : work)
         if (hipMalloc((void **)p, sizeof(double) * n) != hipSuccess) {
@@ -485,6 +485,9 @@ int prh_operator_set_scale(prh_operator *op, const double *scale)
     PRH_CHECK_ARG(op && scale, "prh_operator_set_scale: null pointer");
     PRH_HIP(hipSetDevice(op->device));
     PRH_HIP(hipMemcpy(op->scale, scale, sizeof(double) * op->n, hipMemcpyHostToDevice));
+    std::vector<float> s32(op->n);
+    for (int64_t i = 0; i < op->n; ++i) s32[i] = (float)scale[i];
+    PRH_HIP(hipMemcpy(op->scale32, s32.data(), sizeof(float) * op->n, hipMemcpyHostToDevice));
     return PRH_OK;
 }
 
@@ -509,8 +512,12 @@ int prh_radiosity(prh_operator *op, int32_t n_rhs, const double *emitted, const 
     PRH_CHECK_ARG(op && emitted && emissivity && absorbed && residual && iterations,
                   "prh_radiosity: null pointer");
     PRH_CHECK_ARG(n_rhs >= 0 && max_reflections >= 1, "prh_radiosity: bad sizes");
-    PRH_CHECK_ARG(precision == PRH_FP64, "prh_radiosity: unsupported precision %d", precision);
+    PRH_CHECK_ARG(precision == PRH_FP64 || precision == PRH_MIXED, "prh_radiosity: unsupported precision %d",
+                  precision);
     PRH_HIP(hipSetDevice(op->device));
+    if (precision == PRH_MIXED)
+        return prh_radiosity_mixed(op, n_rhs, emitted, emissivity, emissivity_per_rhs, relative_tolerance, absorbed,
+                                   residual, iterations);
     hipStream_t st;
     int rc = current_stream(&st);
     if (rc) return rc;
@@ -563,6 +570,7 @@ void prh_operator_destroy(prh_operator *op)
 {
     if (!op) return;
     (void)hipSetDevice(op->device);
+    if (op->batch) prh_batch_destroy(op->batch);
     free_operator(op);
     delete op;
 }

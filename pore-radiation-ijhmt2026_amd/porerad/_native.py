@@ -27,7 +27,8 @@ PRH_ERR_SERIES = 5
 PRH_ERR_CAPACITY = 6
 
 APPLY_SYMMETRIC, APPLY_MATVEC, APPLY_RMATVEC, APPLY_RAW, APPLY_RAW_T = 0, 1, 2, 3, 4
-FP64 = 0
+FP64, MIXED = 0, 1
+BATCH_SLOTS = 16
 
 # Every function the C ABI exports: name -> (argtypes, restype).
 _P = ctypes.c_void_p
@@ -56,6 +57,13 @@ SIGNATURES = {
     "prh_operator_apply": ([_P, _I32, _P, _P], ctypes.c_int),
     "prh_radiosity": ([_P, _I32, _P, _P, _I32, _F64, _I32, _I32, _P, _P, _P], ctypes.c_int),
     "prh_operator_destroy": ([_P], None),
+    "prh_batch_create": ([_P, _P], ctypes.c_int),
+    "prh_batch_bytes": ([_P, _P], ctypes.c_int),
+    "prh_batch_free_slots": ([_P, _P], ctypes.c_int),
+    "prh_batch_admit": ([_P, _P, _P, _I32, _F64, _P], ctypes.c_int),
+    "prh_batch_epoch": ([_P, _P, _P], ctypes.c_int),
+    "prh_batch_result": ([_P, _I32, _P, _P, _P], ctypes.c_int),
+    "prh_batch_destroy": ([_P], None),
 }
 
 _lock = threading.Lock()
@@ -289,6 +297,61 @@ class Operator:
     def close(self) -> None:
         if getattr(self, "handle", None) is not None and self.handle.value:
             self._lib.prh_operator_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Owner of a batched radiosity engine (``prh_batch``) bound to an Operator."""
+
+    def __init__(self, operator: Operator) -> None:
+        self._lib = lib()
+        self.operator = operator
+        self.n = operator.n
+        handle = ctypes.c_void_p()
+        check(self._lib.prh_batch_create(operator.handle, ctypes.byref(handle)), "prh_batch_create")
+        self.handle = handle
+        b = ctypes.c_int64()
+        check(self._lib.prh_batch_bytes(handle, ctypes.byref(b)), "prh_batch_bytes")
+        self.bytes = int(b.value)
+
+    def free_slots(self) -> int:
+        m = ctypes.c_uint32()
+        check(self._lib.prh_batch_free_slots(self.handle, ctypes.byref(m)), "prh_batch_free_slots")
+        return int(m.value)
+
+    def admit(self, emitted: np.ndarray, emissivity, relative_tolerance: float) -> int:
+        E = np.ascontiguousarray(emitted, dtype=np.float64)
+        eps = np.ascontiguousarray(np.atleast_1d(np.asarray(emissivity, dtype=np.float64)))
+        per_patch = 1 if eps.size == self.n else 0
+        if E.shape != (self.n,) or (per_patch == 0 and eps.size != 1):
+            raise ValueError("emission and emissivity must have one value per patch")
+        slot = ctypes.c_int32()
+        check(self._lib.prh_batch_admit(self.handle, ptr(E), ptr(eps), per_patch, float(relative_tolerance),
+                                        ctypes.byref(slot)), "prh_batch_admit")
+        return int(slot.value)
+
+    def epoch(self) -> tuple[int, int]:
+        done, active = ctypes.c_uint32(), ctypes.c_uint32()
+        check(self._lib.prh_batch_epoch(self.handle, ctypes.byref(done), ctypes.byref(active)), "prh_batch_epoch")
+        return int(done.value), int(active.value)
+
+    def result(self, slot: int):
+        absorbed = np.empty(self.n, np.float64)
+        residual = ctypes.c_double()
+        iterations = ctypes.c_int32()
+        check(self._lib.prh_batch_result(self.handle, int(slot), ptr(absorbed), ctypes.byref(residual),
+                                         ctypes.byref(iterations)), "prh_batch_result")
+        return absorbed, float(residual.value), int(iterations.value)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self._lib.prh_batch_destroy(self.handle)
             self.handle = None
 
     def __del__(self):  # pragma: no cover

@@ -247,17 +247,24 @@ def reflection_series_batch(
     *,
     relative_tolerance: float,
     max_reflections: int = 10000,
+    precision: str = "fp64",
 ) -> list[RadiosityResult]:
     """``reflection_series`` for k emissions (rows of ``emitted_W``) in one device call.
 
     ``emissivity`` is one vector for all rows or one row per emission.
+    ``precision="fp64"`` sums the reflection series in FP64 exactly as
+    ``reflection_series``; ``"mixed"`` solves the same radiosity balance with
+    batched FP32 Chebyshev sweeps and FP64 residual refinement until
+    ||E - (I - (1-eps) F_p^T) J||_1 <= relative_tolerance ||E||_1 (the
+    iteration count is then the number of sweeps).
     """
 
     E = np.ascontiguousarray(np.atleast_2d(emitted_W), dtype=np.float64)
     n = operator.n_patches
     eps = np.asarray(emissivity, dtype=np.float64)
     eps = np.ascontiguousarray(np.broadcast_to(eps, E.shape) if eps.ndim == 2 else _emissivity_vector(eps, n))
-    absorbed, residual, iterations = operator.device.radiosity(E, eps, relative_tolerance, max_reflections)
+    mode = {"fp64": _native.FP64, "mixed": _native.MIXED}[precision]
+    absorbed, residual, iterations = operator.device.radiosity(E, eps, relative_tolerance, max_reflections, mode)
     return [RadiosityResult(E[q], absorbed[q], absorbed[q] - E[q], float(residual[q]), int(iterations[q]))
             for q in range(E.shape[0])]
 

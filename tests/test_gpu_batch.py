@@ -1,0 +1,149 @@
+"""The batched mixed-precision radiosity engine (prh_batch_*) against the FP64 oracle.
+
+Tolerances: absorbed power within 1e-8 of max|absorbed| and channels within
+1e-8 of max(|q_h|, |q_c|) of the converged FP64 series (oracle, rtol 1e-13),
+at solver tolerance 1e-9.  A solve must return identical bits alone, in a
+mix of other solves, and when admitted while others are running.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from golden_data import cases, phases_x
+from oracle import exchange_oracle as X
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mods():
+    from porerad import _native, coupled, exchange, view_factors
+
+    _native.lib()
+    return _native, view_factors, exchange, coupled
+
+
+def _pair(seed=4, shape=(16, 14, 14), density=0.14, R=64, nonuniform=False):
+    from porerad import exchange
+
+    rng = np.random.default_rng(seed)
+    phases = (rng.random(shape) < density).astype(np.uint8)
+    Q = O.build_surface_patches(phases, 0.01)
+    o, r, c = O.first_hit_counts(Q, R, 91)
+    rays = np.full(Q.n_patches, R, np.int64)
+    if nonuniform:
+        # doubling every count of some origins keeps each row a valid ray set
+        heavy = (o % 3) == 0
+        c = np.where(heavy, 2 * c, c)
+        rays = np.where(np.arange(Q.n_patches) % 3 == 0, 2 * R, R)
+    op = exchange.ExchangeOperator(o, r, c, rays, Q.patch_area_m2)
+    ref = X.OracleOperator(o, r, c, rays, Q.patch_area_m2)
+    op.device.set_scale(ref.scale)
+    op.scale = ref.scale
+    return Q, op, ref
+
+
+@pytest.mark.parametrize("nonuniform", [False, True])
+@pytest.mark.parametrize("eps", [0.3, 0.6, 0.9, "mixed"])
+def test_mixed_matches_fp64_oracle(mods, eps, nonuniform):
+    _, _, ex, _ = mods
+    Q, op, ref = _pair(nonuniform=nonuniform)
+    assert op.device.nnz_symmetric > 0
+    n = Q.n_patches
+    rng = np.random.default_rng(5)
+    e = rng.uniform(0.2, 0.95, n) if eps == "mixed" else np.full(n, eps)
+    E = rng.normal(size=n) * 1e-3 + 2e-3
+    got = ex.reflection_series_batch(op, E[None, :], e, relative_tolerance=1e-9, precision="mixed")[0]
+    absorbed, _, _ = X.reflection_series(ref, E, e, relative_tolerance=1e-13)
+    assert np.max(np.abs(got.absorbed_power_W - absorbed)) <= 1e-8 * np.max(np.abs(absorbed))
+    assert abs(got.global_energy_residual_W) <= 1e-8 * np.sum(np.abs(E))
+
+
+def test_alone_equals_in_mix_bitwise(mods):
+    _native, _, ex, _ = mods
+    Q, op, _ = _pair(6)
+    n = Q.n_patches
+    rng = np.random.default_rng(8)
+    E = rng.normal(size=(5, n))
+    eps = np.stack([np.full(n, v) for v in (0.3, 0.9, 0.6, 0.45, 0.75)])
+    alone = [ex.reflection_series_batch(op, E[q], eps[q], relative_tolerance=1e-9, precision="mixed")[0]
+             for q in range(5)]
+    mixed = ex.reflection_series_batch(op, E, eps, relative_tolerance=1e-9, precision="mixed")
+    for a, b in zip(alone, mixed):
+        assert np.array_equal(a.absorbed_power_W, b.absorbed_power_W)
+        assert a.iterations == b.iterations
+    # admitted while others run
+    batch = _native.Batch(op.device)
+    try:
+        s0 = batch.admit(E[0], eps[0], 1e-9)
+        s1 = batch.admit(E[1], eps[1], 1e-9)
+        batch.epoch()
+        s2 = batch.admit(E[2], eps[2], 1e-9)
+        results = {}
+        while len(results) < 3:
+            done, _ = batch.epoch()
+            for s in (s0, s1, s2):
+                if (done >> s) & 1 and s not in results:
+                    results[s] = batch.result(s)
+        for q, s in enumerate((s0, s1, s2)):
+            assert np.array_equal(results[s][0], alone[q].absorbed_power_W)
+    finally:
+        batch.close()
+
+
+def test_more_requests_than_slots(mods):
+    _native, _, ex, _ = mods
+    Q, op, ref = _pair(7, shape=(12, 10, 10))
+    n = Q.n_patches
+    rng = np.random.default_rng(9)
+    k = _native.BATCH_SLOTS + 7
+    E = rng.normal(size=(k, n))
+    eps = rng.uniform(0.25, 0.95, size=(k, n))
+    got = ex.reflection_series_batch(op, E, eps, relative_tolerance=1e-9, precision="mixed")
+    for q in (0, k // 2, k - 1):
+        absorbed, _, _ = X.reflection_series(ref, E[q], eps[q], relative_tolerance=1e-13)
+        assert np.max(np.abs(got[q].absorbed_power_W - absorbed)) <= 1e-8 * np.max(np.abs(absorbed))
+
+
+def test_channels_of_archived_case_within_1e8(mods):
+    _, vf, ex, cp = mods
+    case = next(c for c in cases()["prescribed_field"] if c["case_id"] == "pf_OF_core_phi0.05_s3_x")
+    P = vf.build_surface_patches(phases_x(case["microstructure"], case["axis"]), case["H_m"])
+    R = case["rays_per_patch_per_block"]
+    o, r, c = vf.first_hit_counts_pooled(P, R, case["final_seeds"])
+    op = ex.ExchangeOperator(o, r, c, np.full(P.n_patches, R * 2, np.int64), P.patch_area_m2)
+    assert op.device.nnz_symmetric > 0
+    rows = case["results"]
+    T_field = []
+    E, eps = [], []
+    for ref in rows:
+        T = cp.linear_cell_temperature(P.solid.shape, ref["T_mean_K"], ref["delta_T_K"])
+        T_patch = cp.patch_temperature(P, T, ref["T_mean_K"] + ref["delta_T_K"] / 2, ref["T_mean_K"] - ref["delta_T_K"] / 2)
+        e = np.full(P.n_patches, ref["epsilon"])
+        E.append(ex.emission(e, P.patch_area_m2, T_patch, ref["T_mean_K"]))
+        eps.append(e)
+        T_field.append(T)
+    got = ex.reflection_series_batch(op, np.stack(E), np.stack(eps), relative_tolerance=1e-9, precision="mixed")
+    for ref, res in zip(rows, got):
+        q_h, q_c, q_i, _ = cp._channels_from_net(P, res.net_power_W)
+        scale = max(abs(ref["q_h_W_m2"]), abs(ref["q_c_W_m2"]))
+        for a, b in ((q_h, ref["q_h_W_m2"]), (q_c, ref["q_c_W_m2"]), (q_i, ref["q_i_W_m2"])):
+            assert abs(a - b) <= 1e-8 * scale
+
+
+def test_admission_errors(mods):
+    _native, _, _, _ = mods
+    Q, op, _ = _pair(10, shape=(8, 6, 6))
+    batch = _native.Batch(op.device)
+    try:
+        with pytest.raises(ValueError):
+            batch.admit(np.ones(Q.n_patches), 0.0, 1e-9)  # nothing absorbs
+        for _ in range(_native.BATCH_SLOTS):
+            batch.admit(np.ones(Q.n_patches), 0.5, 1e-9)
+        with pytest.raises(ValueError, match="slots are busy"):
+            batch.admit(np.ones(Q.n_patches), 0.5, 1e-9)
+    finally:
+        batch.close()
