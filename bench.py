@@ -60,39 +60,82 @@ def run_tracer(P, R, seed, steps, warmup):
     return np.array(wall), np.array(parts), reads, nnz
 
 
-def run_radiosity(P, R, seed, eps_list, repeats):
-    from porerad import coupled, exchange, view_factors
+def build_operator(P, R, seed):
+    from porerad import exchange, view_factors
+
+    o, r, c = view_factors.first_hit_counts_pooled(P, R, [seed])
+    return exchange.ExchangeOperator(o, r, c, np.full(P.n_patches, R, np.int64), P.patch_area_m2)
+
+
+def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
+    """Plain FP64 series vs served mixed-precision solves (Poisson arrivals)."""
+
+    from porerad import coupled, exchange, serving, synthetic, view_factors
 
     t0 = time.perf_counter()
-    o, r, c = view_factors.first_hit_counts_pooled(P, R, [seed])
-    op = exchange.ExchangeOperator(o, r, c, np.full(P.n_patches, R, np.int64), P.patch_area_m2)
+    geoms = {"foam128": (foam_P, build_operator(foam_P, R, seed))}
+    fibre = view_factors.build_surface_patches(synthetic.aligned_fibers(128, 0.10, 4.0, 0), 0.01)
+    geoms["aligned128"] = (fibre, build_operator(fibre, R, seed))
     setup = time.perf_counter() - t0
-    log(f"operator: {op.device.nnz_symmetric} symmetric nonzeros, balance {op.balance_method} "
-        f"({op.balance_iterations}), built in {setup:.1f} s")
-    nx = P.solid.shape[0]
-    T_h, T_c = 700.0, 300.0
-    lat, iters = [], []
-    for _ in range(repeats):
-        for eps in eps_list:
-            t = time.perf_counter()
-            state = coupled.evaluate_radiation(op, P, coupled.linear_cell_temperature(P.solid.shape, 500.0, 400.0),
-                                               T_h, T_c, eps, relative_tolerance=1e-8,
-                                               reference_temperature_K=500.0)
-            lat.append(time.perf_counter() - t)
-            iters.append(state.radiosity.iterations)
-    lat = np.array(lat)
+    log(f"operators: {', '.join(f'{k}: {P.n_patches} patches, {op.device.nnz_symmetric} nnz' for k, (P, op) in geoms.items())}"
+        f" ({setup:.1f} s)")
+    combos = [(Th, eps) for Th in (500.0, 700.0, 900.0) for eps in (0.3, 0.6, 0.9)]
+
+    # plain FP64 path: the reference algorithm (reflection series) per request
+    P, op = geoms["foam128"]
+    plain_lat, refs = [], {}
+    for Th, eps in combos:
+        T_m = 0.5 * (Th + 300.0)
+        t = time.perf_counter()
+        st = coupled.evaluate_radiation(op, P, coupled.linear_cell_temperature(P.solid.shape, T_m, Th - 300.0),
+                                        Th, 300.0, eps, relative_tolerance=1e-8, reference_temperature_K=T_m)
+        plain_lat.append(time.perf_counter() - t)
+        _ = st
+    plain = {"solves_per_s": len(plain_lat) / sum(plain_lat), "p50_ms": float(np.percentile(plain_lat, 50) * 1e3),
+             "p99_ms": float(np.percentile(plain_lat, 99) * 1e3), "precision": "fp64 reflection series, rtol 1e-8",
+             "geometry": "foam128"}
+
+    # FP64 oracle values at 1e-13 for the accuracy check
+    for Th, eps in combos:
+        T_m = 0.5 * (Th + 300.0)
+        refs[(Th, eps)] = coupled.prescribed_field_channels(op, P, T_m, Th - 300.0, eps)
+
+    def make_requests(count, seed_):
+        rng = np.random.default_rng(seed_)
+        keys = list(geoms)
+        arrivals = serving.poisson_arrivals(rate, count, seed_)
+        return [serving.RadiosityRequest(str(rng.choice(keys)), float(rng.choice([500.0, 700.0, 900.0])), 300.0,
+                                         float(rng.choice([0.3, 0.6, 0.9])), 1e-9, float(t)) for t in arrivals]
+
+    srv = serving.RadiosityServer()
+    for k, (Pk, opk) in geoms.items():
+        srv.add_geometry(k, Pk, opk)
+    srv.run(make_requests(32, 99), real_time=False)  # warm-up: engines allocated
+    reqs = make_requests(n_requests, 7)
+    t = time.perf_counter()
+    replies = srv.run(reqs, real_time=True)
+    served = serving.latency_summary(replies, time.perf_counter() - t)
+    served.update({"arrival_rate_per_s": rate, "requests": n_requests, "geometries": list(geoms),
+                   "precision": "fp32 chebyshev sweeps + fp64 refinement, rtol 1e-9", "epochs": srv.epochs})
+    t = time.perf_counter()
+    sat = srv.run(make_requests(n_requests, 8), real_time=False)
+    saturation = serving.latency_summary(sat, time.perf_counter() - t)
+    err = {"q_h": 0.0, "q_c": 0.0, "q_i": 0.0}
+    for rep in replies + sat:
+        if rep.request.key != "foam128":
+            continue
+        ref = refs[(rep.request.T_hot_K, rep.request.epsilon)]
+        scale = max(abs(ref.q_h), abs(ref.q_c))
+        for name, got, want in (("q_h", rep.q_h, ref.q_h), ("q_c", rep.q_c, ref.q_c), ("q_i", rep.q_i, ref.q_i)):
+            err[name] = max(err[name], abs(got - want) / scale)
+    srv.close()
     return {
-        "solves": int(lat.size),
-        "solves_per_s": float(lat.size / lat.sum()),
-        "p50_ms": float(np.percentile(lat, 50) * 1e3),
-        "p99_ms": float(np.percentile(lat, 99) * 1e3),
-        "precision": "fp64 reflection series, rtol 1e-8",
-        "emissivities": list(eps_list),
-        "reflections": [int(i) for i in iters[: len(eps_list)]],
-        "nnz_symmetric": int(op.device.nnz_symmetric),
-        "operator_build_s": round(setup, 2),
-        "n_patches": int(P.n_patches),
-        "nx": int(nx),
+        "plain_fp64": plain,
+        "served": served,
+        "saturation": saturation,
+        "speedup_vs_plain_fp64": saturation["solves_per_s"] / plain["solves_per_s"],
+        "max_rel_flux_error_vs_fp64": err,
+        "operator_setup_s": round(setup, 2),
     }
 
 
@@ -217,9 +260,13 @@ def main() -> None:
     if tune is not None:
         record["tracer_variant"] = {"chosen": tune["variant"], "table": tune["table"]}
     if not args.no_radiosity:
-        record["radiosity"] = run_radiosity(P, R, seed, [0.3, 0.6, 0.9], 2)
-        log(f"radiosity: {record['radiosity']['solves_per_s']:.1f} solves/s, p50 "
-            f"{record['radiosity']['p50_ms']:.2f} ms, p99 {record['radiosity']['p99_ms']:.2f} ms")
+        rad = run_radiosity(P, R, seed)
+        record["radiosity"] = rad
+        log(f"radiosity plain fp64 {rad['plain_fp64']['solves_per_s']:.1f} solves/s; served "
+            f"{rad['served']['solves_per_s']:.1f} solves/s at {rad['served']['arrival_rate_per_s']:.0f}/s offered, "
+            f"p50 {rad['served']['p50_ms']:.1f} ms p99 {rad['served']['p99_ms']:.1f} ms; saturation "
+            f"{rad['saturation']['solves_per_s']:.1f} solves/s ({rad['speedup_vs_plain_fp64']:.1f}x plain); "
+            f"max flux error {max(rad['max_rel_flux_error_vs_fp64'].values()):.1e}")
     if not args.no_cpu:
         record["cpu_baseline"] = cpu_baseline(phases, R, seed)
         log(f"cpu baseline {record['cpu_baseline']['value']:.3e} rays/s")

@@ -12,7 +12,8 @@
 // form), adds the correction to the FP64 solution, and recomputes the FP64
 // residual r = E - (x - rho * F_p^T x) with an FP64 sweep; epochs repeat
 // until ||r||_1 <= rtol ||E||_1 (iterative refinement).  m is the Chebyshev
-// count that reduces the error bound by 1e-6 for the slot's rho_max.
+// count whose error bound, for the slot's rho_max, covers what is left to
+// reach the tolerance (at most a 1e-6 reduction, what FP32 sweeps resolve).
 //
 // Sweeps read the operator once for all 16 slots: the sparsity of S with the
 // exact integer K = C + C^T as uint16 when all patches share area and ray
@@ -276,8 +277,7 @@ unsigned grid_of(int64_t n, int b) { return (unsigned)std::max<int64_t>(1, (n + 
 struct SlotHost {
     bool active = false, done = false, failed = false;
     double rtol = 0, e_norm = 0, rho_max = 0, r_norm = 0, r_sum = 0;
-    int epochs = 0, ticks = 0;
-    bool needs_residual = false;  // admitted this epoch: r64 = E already
+    int epochs = 0, ticks = 0, epoch_ticks = 0;
 };
 
 }  // namespace
@@ -330,13 +330,21 @@ void free_batch(prh_batch *b)
 }
 
 // Chebyshev iterations that shrink the error bound 2 c^k / (1 + c^2k) below tau.
-int ticks_for(double rho_max)
+int ticks_for(double rho_max, double tau)
 {
     if (rho_max <= 0.0) return 1;
     const double kappa = (1.0 + rho_max) / (1.0 - rho_max);
     const double c = (std::sqrt(kappa) - 1.0) / (std::sqrt(kappa) + 1.0);
-    const int m = (int)std::ceil(std::log(INNER_REDUCTION / 2.0) / std::log(c));
+    const int m = (int)std::ceil(std::log(tau / 2.0) / std::log(c));
     return std::min(std::max(m, 1), MAX_TICKS);
+}
+
+// Reduction an epoch aims for: what is left to reach the tolerance, with a
+// factor 10 of margin, but no finer than FP32 sweeps resolve.
+double epoch_target(const SlotHost &S)
+{
+    const double left = 0.1 * S.rtol * S.e_norm / std::max(S.r_norm, 1e-300);
+    return std::min(std::max(left, INNER_REDUCTION), 0.5);
 }
 
 // alpha_k, beta_k of Chebyshev acceleration on [1 - delta, 1 + delta]
@@ -471,7 +479,6 @@ int prh_batch_admit(prh_batch *b, const double *emitted, const double *emissivit
     S.e_norm = std::max(e_norm, 2.2250738585072014e-308);
     S.rho_max = rho_max;
     S.r_norm = e_norm;
-    S.needs_residual = false;
     S.done = e_norm == 0.0;
     *slot_out = q;
     return PRH_OK;
@@ -500,7 +507,8 @@ int prh_batch_epoch(prh_batch *b, uint32_t *done_mask, uint32_t *active_mask)
             continue;
         }
         mask |= 1u << q;
-        m = std::max(m, ticks_for(S.rho_max));
+        S.epoch_ticks = ticks_for(S.rho_max, epoch_target(S));
+        m = std::max(m, S.epoch_ticks);
         const double scale = S.r_norm / (double)b->n;
         s[q] = scale > 0.0 ? scale : 1.0;
         inv_s[q] = 1.0 / s[q];
@@ -510,7 +518,7 @@ int prh_batch_epoch(prh_batch *b, uint32_t *done_mask, uint32_t *active_mask)
         std::vector<float> coef((size_t)m * SLOTS * 2, 0.f), al, be;
         for (int q = 0; q < SLOTS; ++q) {
             if (!((mask >> q) & 1u)) continue;
-            const int mq = ticks_for(b->slot[q].rho_max);
+            const int mq = b->slot[q].epoch_ticks;
             cheb_coefficients(b->slot[q].rho_max, mq, al, be);
             // The direction after the slot's last own tick is zero, so longer
             // slots in the same epoch leave this slot untouched.
@@ -547,7 +555,7 @@ int prh_batch_epoch(prh_batch *b, uint32_t *done_mask, uint32_t *active_mask)
             if (!((mask >> q) & 1u)) continue;
             SlotHost &S = b->slot[q];
             S.epochs += 1;
-            S.ticks += ticks_for(S.rho_max);
+            S.ticks += S.epoch_ticks;
             S.r_norm = tot[q];
             S.r_sum = tot[SLOTS + q];
             if (S.r_norm <= S.rtol * S.e_norm) S.done = true;

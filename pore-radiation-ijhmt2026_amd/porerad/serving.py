@@ -1,0 +1,265 @@
+"""Continuous-batching server of radiosity requests over device-resident exchange operators.
+
+A request names a resident geometry (its ``SurfacePatches`` and
+``ExchangeOperator``), the two wall temperatures and the emissivity; it asks
+for the radiation channels of the prescribed linear field between the walls
+(porerad/coupled.py:170-194), solved to a relative tolerance with the
+batched mixed-precision engine (``prh_batch_*``: FP32 Chebyshev sweeps over
+16 slots per operator, FP64 residual refinement).
+
+Scheduling.  Every geometry has one engine with 16 slots.  Each scheduler
+step (1) admits queued requests, oldest first, into free slots of their
+geometry's engine, creating the engine first if its device memory fits the
+HBM budget (idle engines are released to make room); (2) runs one epoch on
+every engine that has running slots; (3) retires finished slots and replies.
+Requests that arrive while others are on the GPU join at the next epoch
+boundary, so their work overlaps on the device with the requests already
+running; a solve's bits do not depend on what else is in its batch.
+
+The engine type is injectable (``engine_factory``) so that the admission and
+retirement logic can be exercised without a GPU.
+"""
+
+from __future__ import annotations
+
+import collections
+import dataclasses
+import time
+from typing import Callable
+
+import numpy as np
+
+from .constants import SIGMA
+from .view_factors import COLD, HOT, INTERNAL
+
+SLOTS = 16
+
+
+@dataclasses.dataclass
+class RadiosityRequest:
+    """Channels of the linear field between T_hot and T_cold with uniform emissivity."""
+
+    key: str
+    T_hot_K: float
+    T_cold_K: float
+    epsilon: float
+    relative_tolerance: float = 1.0e-9
+    arrival_s: float = 0.0
+    tag: object = None
+
+
+@dataclasses.dataclass
+class RadiosityReply:
+    request: RadiosityRequest
+    q_h: float
+    q_c: float
+    q_i: float
+    net_power_W: np.ndarray
+    residual_power_W: float
+    sweeps: int
+    admitted_s: float
+    finished_s: float
+
+    @property
+    def latency_s(self) -> float:
+        return self.finished_s - self.request.arrival_s
+
+
+class Geometry:
+    """A resident geometry: patch table, operator and the cached emission ingredients."""
+
+    def __init__(self, key: str, patches, operator) -> None:
+        self.key = key
+        self.patches = patches
+        self.operator = operator
+        nx, ny, nz = patches.solid.shape
+        self.n = patches.n_patches
+        cls = patches.patch_class
+        self.hot = cls == HOT
+        self.cold = cls == COLD
+        self.internal = cls == INTERNAL
+        # i + 1/2 of the internal patches' owning cells along x
+        self.nx = nx
+        self.half_index = np.zeros(self.n)
+        self.half_index[self.internal] = (patches.patch_cell[self.internal] // (ny * nz)).astype(np.float64) + 0.5
+        self.area = patches.patch_area_m2
+        self.cross_section = float(ny * nz) * (patches.H_m / float(nx)) ** 2
+
+    def emission(self, req: RadiosityRequest) -> np.ndarray:
+        """eps sigma A (T^4 - T_m^4) of the prescribed linear field.
+
+        Same operations as prescribed_field_channels: walls at T_m +/- dT/2,
+        cells at (T_m + dT/2) - (i + 1/2) dT / nx (coupled.py:138-143,
+        180-184), isothermal state at T_m subtracted (exchange.py:265-270).
+        """
+
+        T_m = 0.5 * (req.T_hot_K + req.T_cold_K)
+        dT = req.T_hot_K - req.T_cold_K
+        T = np.empty(self.n)
+        T[self.hot] = T_m + dT / 2.0
+        T[self.cold] = T_m - dT / 2.0
+        T[self.internal] = (T_m + dT / 2.0) - self.half_index[self.internal] * dT / self.nx
+        return req.epsilon * SIGMA * self.area * ((T - T_m) * (T + T_m) * (T * T + T_m * T_m))
+
+    def channels(self, net: np.ndarray) -> tuple[float, float, float]:
+        a = self.cross_section
+        return (-float(np.sum(net[self.hot])) / a, -float(np.sum(net[self.cold])) / a,
+                -float(np.sum(net[self.internal])) / a)
+
+
+def device_engine(geometry: Geometry):
+    from . import _native
+
+    return _native.Batch(geometry.operator.device)
+
+
+def device_engine_bytes(geometry: Geometry) -> int:
+    """Device bytes of one engine: 16 slots x (6 FP32 + 6 FP64 vectors) per patch, plus partial sums."""
+
+    n = geometry.n
+    blocks = ((n + 3) // 4 + 7) // 8 * 8
+    return n * SLOTS * (6 * 4 + 6 * 8) + blocks * SLOTS * 16 + n * 16 + 4096
+
+
+class RadiosityServer:
+    """Scheduler of radiosity requests (see the module docstring)."""
+
+    def __init__(self, hbm_budget_bytes: int | None = None,
+                 engine_factory: Callable[[Geometry], object] = device_engine,
+                 engine_bytes: Callable[[Geometry], int] = device_engine_bytes,
+                 clock: Callable[[], float] = time.perf_counter) -> None:
+        self.budget = hbm_budget_bytes
+        self.engine_factory = engine_factory
+        self.engine_size = engine_bytes
+        self.clock = clock
+        self.geometries: dict[str, Geometry] = {}
+        self.engines: dict[str, object] = {}
+        self.slots: dict[str, dict[int, tuple[RadiosityRequest, np.ndarray, float]]] = {}
+        self.queue: collections.deque[RadiosityRequest] = collections.deque()
+        self.replies: list[RadiosityReply] = []
+        self.epochs = 0
+        self.evictions = 0
+
+    # -- registration ------------------------------------------------------------
+    def add_geometry(self, key: str, patches, operator) -> Geometry:
+        g = Geometry(key, patches, operator)
+        self.geometries[key] = g
+        self.slots[key] = {}
+        return g
+
+    def engine_bytes(self) -> int:
+        return sum(self.engine_size(self.geometries[k]) for k in self.engines)
+
+    # -- admission -----------------------------------------------------------------
+    def submit(self, request: RadiosityRequest) -> None:
+        if request.key not in self.geometries:
+            raise KeyError(f"unknown geometry {request.key!r}")
+        self.queue.append(request)
+
+    def _engine_for(self, key: str):
+        if key in self.engines:
+            return self.engines[key]
+        need = self.engine_size(self.geometries[key])
+        if self.budget is not None:
+            while self.engine_bytes() + need > self.budget:
+                idle = [k for k in self.engines if not self.slots[k]]
+                if not idle:
+                    return None
+                self.engines.pop(idle[0]).close()
+                self.evictions += 1
+        engine = self.engine_factory(self.geometries[key])
+        self.engines[key] = engine
+        return engine
+
+    def _admit(self) -> None:
+        waiting = collections.deque()
+        while self.queue:
+            req = self.queue.popleft()
+            busy = self.slots[req.key]
+            if len(busy) >= SLOTS:
+                waiting.append(req)
+                continue
+            engine = self._engine_for(req.key)
+            if engine is None:
+                waiting.append(req)
+                continue
+            g = self.geometries[req.key]
+            E = g.emission(req)
+            slot = engine.admit(E, req.epsilon, req.relative_tolerance)
+            busy[slot] = (req, E, self.clock())
+        self.queue = waiting
+
+    # -- one scheduler step ----------------------------------------------------------
+    def step(self) -> list[RadiosityReply]:
+        self._admit()
+        done_now = []
+        for key, engine in list(self.engines.items()):
+            busy = self.slots[key]
+            if not busy:
+                continue
+            done, _ = engine.epoch()
+            self.epochs += 1
+            g = self.geometries[key]
+            for slot in list(busy):
+                if not (done >> slot) & 1:
+                    continue
+                req, E, admitted = busy.pop(slot)
+                absorbed, residual, sweeps = engine.result(slot)
+                net = absorbed - E
+                q_h, q_c, q_i = g.channels(net)
+                done_now.append(RadiosityReply(req, q_h, q_c, q_i, net, residual, sweeps, admitted, self.clock()))
+        self.replies.extend(done_now)
+        return done_now
+
+    def pending(self) -> int:
+        return len(self.queue) + sum(len(b) for b in self.slots.values())
+
+    def run(self, requests: list[RadiosityRequest], *, real_time: bool = True) -> list[RadiosityReply]:
+        """Serve requests whose ``arrival_s`` are offsets from now; returns the replies in completion order.
+
+        With ``real_time`` a request is submitted only once its arrival time
+        has passed on the clock (an open-loop arrival process); otherwise all
+        are queued at once (closed loop, saturation throughput).
+        """
+
+        order = sorted(requests, key=lambda r: r.arrival_s)
+        start = self.clock()
+        first = len(self.replies)
+        i = 0
+        while i < len(order) or self.pending():
+            now = self.clock() - start
+            while i < len(order) and (not real_time or order[i].arrival_s <= now):
+                req = order[i]
+                req.arrival_s = start + (req.arrival_s if real_time else 0.0)
+                self.submit(req)
+                i += 1
+            if self.pending():
+                self.step()
+            elif i < len(order):
+                wait = order[i].arrival_s - (self.clock() - start)
+                if wait > 0:
+                    time.sleep(min(wait, 0.002))
+        return self.replies[first:]
+
+    def close(self) -> None:
+        for e in self.engines.values():
+            e.close()
+        self.engines.clear()
+
+
+def poisson_arrivals(rate_per_s: float, count: int, seed: int) -> np.ndarray:
+    """Arrival offsets (s) of a Poisson process with the given rate."""
+
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate_per_s, size=count))
+
+
+def latency_summary(replies: list[RadiosityReply], wall_s: float) -> dict:
+    lat = np.array([r.latency_s for r in replies])
+    return {
+        "solves": int(lat.size),
+        "solves_per_s": float(lat.size / wall_s) if wall_s > 0 else float("nan"),
+        "p50_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "mean_ms": float(lat.mean() * 1e3),
+    }

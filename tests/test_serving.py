@@ -1,0 +1,170 @@
+"""Scheduling logic of the radiosity server, on the CPU with a stand-in engine.
+
+The stand-in keeps the engine contract of prh_batch (16 slots, admit / epoch
+/ result, device bytes) and finishes a slot after a fixed number of epochs,
+so admission order, continuous batching and HBM-budget eviction can be
+checked exactly.  The solver numerics are covered by the GPU tests.
+"""
+
+import types
+
+import numpy as np
+import pytest
+
+from porerad import serving
+
+
+class StandInEngine:
+    def __init__(self, geometry, epochs_needed=2, nbytes=100):
+        self.n = geometry.n
+        self.bytes = nbytes
+        self.busy = {}
+        self.epochs_needed = epochs_needed
+        self.closed = False
+        self.log = []
+
+    def admit(self, E, eps, rtol):
+        free = [q for q in range(serving.SLOTS) if q not in self.busy]
+        if not free:
+            raise ValueError("all slots are busy")
+        q = free[0]
+        self.busy[q] = [E.copy(), 0]
+        self.log.append(("admit", q))
+        return q
+
+    def epoch(self):
+        done = 0
+        for q, rec in self.busy.items():
+            rec[1] += 1
+            if rec[1] >= self.epochs_needed:
+                done |= 1 << q
+        return done, sum(1 << q for q in self.busy)
+
+    def result(self, q):
+        E, ep = self.busy.pop(q)
+        return 0.25 * E, 0.0, ep
+
+    def close(self):
+        self.closed = True
+
+
+def _patches(nx=4, ny=3, nz=3):
+    from oracle import oracle as O
+
+    phases = np.zeros((nx, ny, nz), np.uint8)
+    phases[1, 1, 1] = 1
+    return O.build_surface_patches(phases, 0.004)
+
+
+class Clock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        self.t += 0.001
+        return self.t
+
+
+def test_requests_finish_and_channels_follow_net_power():
+    P = _patches()
+    srv = serving.RadiosityServer(engine_factory=lambda g: StandInEngine(g), clock=Clock())
+    g = srv.add_geometry("a", P, operator=None)
+    reqs = [serving.RadiosityRequest("a", 700.0, 300.0, 0.6, arrival_s=0.0) for _ in range(5)]
+    replies = srv.run(reqs, real_time=False)
+    assert len(replies) == 5 and srv.pending() == 0
+    E = g.emission(reqs[0])
+    net = 0.25 * E - E
+    assert replies[0].q_h == pytest.approx(-np.sum(net[P.patch_class == 0]) / g.cross_section)
+    assert all(r.sweeps == 2 for r in replies)
+
+
+def test_emission_matches_prescribed_field_arithmetic():
+    P = _patches(6, 4, 5)
+    srv = serving.RadiosityServer(engine_factory=lambda g: StandInEngine(g), clock=Clock())
+    g = srv.add_geometry("a", P, operator=None)
+    req = serving.RadiosityRequest("a", 900.0, 300.0, 0.3)
+    T_m, dT = 600.0, 600.0
+    nx = P.solid.shape[0]
+    axial = (T_m + dT / 2.0) - (np.arange(nx, dtype=np.float64) + 0.5) * dT / nx
+    T = np.empty(P.n_patches)
+    T[P.patch_class == 0] = T_m + dT / 2.0
+    T[P.patch_class == 1] = T_m - dT / 2.0
+    inner = P.patch_class == 2
+    T[inner] = np.broadcast_to(axial[:, None, None], P.solid.shape).ravel()[P.patch_cell[inner]]
+    expect = 0.3 * serving.SIGMA * P.patch_area_m2 * ((T - T_m) * (T + T_m) * (T * T + T_m * T_m))
+    assert np.array_equal(g.emission(req), expect)
+
+
+def test_slots_are_capped_and_refilled_between_epochs():
+    P = _patches()
+    engines = []
+
+    def factory(g):
+        engines.append(StandInEngine(g, epochs_needed=3))
+        return engines[-1]
+
+    srv = serving.RadiosityServer(engine_factory=factory, clock=Clock())
+    srv.add_geometry("a", P, operator=None)
+    for _ in range(serving.SLOTS + 4):
+        srv.submit(serving.RadiosityRequest("a", 700.0, 300.0, 0.6))
+    srv.step()
+    assert len(srv.slots["a"]) == serving.SLOTS and len(srv.queue) == 4
+    srv.step()
+    srv.step()  # first 16 finish at their 3rd epoch
+    assert len(srv.replies) == serving.SLOTS
+    srv.step()  # the 4 waiting requests are admitted while nothing else runs
+    assert len(srv.slots["a"]) == 4
+    while srv.pending():
+        srv.step()
+    assert len(srv.replies) == serving.SLOTS + 4
+
+
+def test_late_arrivals_join_running_batch():
+    P = _patches()
+    engines = []
+    srv = serving.RadiosityServer(engine_factory=lambda g: engines.append(StandInEngine(g, 4)) or engines[-1],
+                                  clock=Clock())
+    srv.add_geometry("a", P, operator=None)
+    srv.submit(serving.RadiosityRequest("a", 700.0, 300.0, 0.6))
+    srv.step()
+    srv.submit(serving.RadiosityRequest("a", 500.0, 300.0, 0.9))
+    srv.step()
+    assert len(srv.slots["a"]) == 2  # the late request runs beside the first one
+    while srv.pending():
+        srv.step()
+    assert [r.request.T_hot_K for r in srv.replies] == [700.0, 500.0]
+
+
+def test_hbm_budget_evicts_idle_engines_only():
+    P = _patches()
+    made = []
+
+    def factory(g):
+        made.append(StandInEngine(g, epochs_needed=2, nbytes=100))
+        return made[-1]
+
+    srv = serving.RadiosityServer(hbm_budget_bytes=150, engine_factory=factory, engine_bytes=lambda g: 100,
+                                  clock=Clock())
+    srv.add_geometry("a", P, operator=None)
+    srv.add_geometry("b", P, operator=None)
+    srv.submit(serving.RadiosityRequest("a", 700.0, 300.0, 0.6))
+    srv.submit(serving.RadiosityRequest("b", 700.0, 300.0, 0.6))
+    srv.step()
+    # only one engine fits: b waits while a runs
+    assert set(srv.engines) == {"a"} and len(srv.queue) == 1
+    while srv.pending():
+        srv.step()
+        assert srv.engine_bytes() <= 150
+    assert len(srv.replies) == 2 and srv.evictions == 1 and made[0].closed
+
+
+def test_unknown_geometry_is_rejected():
+    srv = serving.RadiosityServer(engine_factory=lambda g: StandInEngine(g))
+    with pytest.raises(KeyError):
+        srv.submit(serving.RadiosityRequest("nope", 700.0, 300.0, 0.6))
+
+
+def test_poisson_arrivals_have_the_requested_rate():
+    t = serving.poisson_arrivals(200.0, 20000, 7)
+    assert np.all(np.diff(t) > 0)
+    assert t[-1] / 20000 == pytest.approx(1 / 200.0, rel=0.03)
