@@ -33,6 +33,22 @@ HBM_PEAK_GBPS = 8000.0
 METRIC = "first-hit rays/s at N=128; served radiosity solves/s, p50/p99 at 1e-8 rel err"
 
 
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "k_trace_stream_foam128_hbm_traffic.md")
+
+
+def measured_traffic() -> float | None:
+    """HBM bytes per traversal launch at the default workload, from the committed counter summary."""
+
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -188,7 +204,7 @@ def main() -> None:
     ap.add_argument("--rays", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-radiosity", action="store_true", help="skip the radiosity solves")
-    ap.add_argument("--no-tune", action="store_true", help="keep the default tracer variant")
+    ap.add_argument("--tune", action="store_true", help="time every tracer variant first and use the fastest")
     ap.add_argument("--copy-probe", action="store_true", help="only run one plain 1 GiB device copy")
     args = ap.parse_args()
     if args.copy_probe:
@@ -210,7 +226,7 @@ def main() -> None:
     n_rays = P.n_patches * R
 
     tune = None
-    if not args.no_tune:
+    if args.tune:
         from porerad import autotune
 
         tune = autotune.tune_tracer(P, R, [seed], use_cache=False)
@@ -247,7 +263,9 @@ def main() -> None:
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": None,
+            # FETCH_SIZE x 2 + WRITE_SIZE x 1 (copy-calibrated) of this kernel at
+            # the default workload; null for other ray counts
+            "traffic": measured_traffic() if R == 1000 else None,
             "kernel": "k_trace_stream",
             "kernel_ms": kernel * 1e3,
             "algorithmic_bytes_per_launch": algo_bytes,

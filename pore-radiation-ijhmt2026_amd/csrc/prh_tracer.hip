@@ -526,7 +526,9 @@ const TraceVariant TRACE_VARIANTS[] = {
     {k_trace_stream<16, false>, 16, 0}, {k_trace_stream<24, false>, 24, 0},
 };
 constexpr int N_TRACE_VARIANTS = sizeof(TRACE_VARIANTS) / sizeof(TRACE_VARIANTS[0]);
-int g_trace_variant = 0;
+// Default: refill at 16 idle lanes with the LDS brick test, the fastest on
+// the foam and fibre media measured so far.
+int g_trace_variant = 1;
 
 int launch_gen(const Geometry &g, uint32_t p0, uint32_t n_p, uint32_t R, const uint64_t *d_seeds, uint32_t total,
                double *d_orig, double *d_dir, hipStream_t st)

@@ -16,7 +16,9 @@ import threading
 
 import numpy as np
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libporerad_hip.so")
+# The in-tree build; PORERAD_HIP_LIB names another copy of the library.
+LIB_PATH = os.environ.get("PORERAD_HIP_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                             "libporerad_hip.so")
 
 PRH_OK = 0
 PRH_ERR_ARG = 1
