@@ -132,7 +132,7 @@ def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
     replies = srv.run(reqs, real_time=True)
     served = serving.latency_summary(replies, time.perf_counter() - t)
     served.update({"arrival_rate_per_s": rate, "requests": n_requests, "geometries": list(geoms),
-                   "precision": "fp32 chebyshev sweeps + fp64 refinement, rtol 1e-9", "epochs": srv.epochs})
+                   "precision": "fp32 chebyshev sweeps + fp64 refinement, rtol 1e-9", "sweeps": srv.sweeps})
     t = time.perf_counter()
     sat = srv.run(make_requests(n_requests, 8), real_time=False)
     saturation = serving.latency_summary(sat, time.perf_counter() - t)

@@ -157,28 +157,49 @@ int prh_radiosity(prh_operator *op, int32_t n_rhs, const double *emitted, const 
 void prh_operator_destroy(prh_operator *op);
 
 /* ---- batched radiosity engine (continuous batching) -------------------
- * Up to 16 emissions ("slots") are solved together on one operator: each
- * epoch runs FP32 Chebyshev sweeps for every running slot and one FP64
- * residual sweep (iterative refinement); a slot finishes when
- * ||E - (I - diag(1-eps) F_p^T) J||_1 <= relative_tolerance ||E||_1, and
- * returns the absorbed power eps * F_p^T J.  Slots are admitted between
- * epochs while others run.  Same contract as reflection_series
- * (porerad/exchange.py:217-245); the serving loop is built on it.
- * The batch is bound to its operator and must be destroyed before it. */
+ * Up to 16 emissions ("slots") are solved together on one operator.  Each
+ * slot alternates epochs of FP32 Chebyshev sweeps with one FP64 refinement
+ * sweep (iterative refinement) and finishes when
+ * ||E - (I - diag(1-eps) F_p^T) J||_1 <= relative_tolerance ||E||_1; its
+ * result is the absorbed power eps * F_p^T J.  Every sweep serves all
+ * running slots; slots are admitted and retired at any sweep while the
+ * others run, and a solve returns the same bits alone or in any mix.  Same
+ * contract as reflection_series (porerad/exchange.py:217-245); the serving
+ * loop is built on it.  The batch is bound to its operator and must be
+ * destroyed before it. */
 int prh_batch_create(prh_operator *op, prh_batch **out);
 /* Device bytes held by the batch (for HBM-budget admission). */
 int prh_batch_bytes(const prh_batch *b, int64_t *bytes);
 /* Bit q set = slot q free. */
 int prh_batch_free_slots(const prh_batch *b, uint32_t *free_mask);
+/* Describe the patches for linear-field requests: class per patch (0 hot
+ * wall, 1 cold wall, 2 internal), the x index of each internal patch's cell
+ * (ignored for walls), the number of cells along x and the cross-section
+ * ny nz (H/nx)^2 that turns powers into fluxes (porerad/coupled.py:129-194). */
+int prh_batch_set_field(prh_batch *b, const int8_t *patch_class, const int32_t *cell_x, int32_t nx,
+                        double cross_section_m2);
 /* Admit one emission (n values) with emissivity per patch (eps_per_patch)
  * or one value; returns the slot, PRH_ERR_CAPACITY when all are busy. */
 int prh_batch_admit(prh_batch *b, const double *emitted, const double *emissivity, int32_t eps_per_patch,
                     double relative_tolerance, int32_t *slot);
-/* Run one epoch; done_mask: finished slots waiting for prh_batch_result. */
-int prh_batch_epoch(prh_batch *b, uint32_t *done_mask, uint32_t *active_mask);
+/* Admit the prescribed linear field between T_hot and T_cold with uniform
+ * emissivity; the emission eps sigma A (T^4 - T_m^4) is formed on the device
+ * exactly as prescribed_field_channels does (porerad/coupled.py:170-194).
+ * Needs prh_batch_set_field. */
+int prh_batch_admit_linear(prh_batch *b, double T_hot_K, double T_cold_K, double emissivity,
+                           double relative_tolerance, int32_t *slot);
+/* Run up to max_ticks sweeps, returning early once a slot has finished;
+ * done_mask: finished slots waiting for prh_batch_channels/result/release. */
+int prh_batch_step(prh_batch *b, int32_t max_ticks, uint32_t *done_mask, uint32_t *active_mask);
+/* Channels q_h, q_c, q_i (W m^-2, as prescribed_field_channels), power still
+ * in flight and sweep count of a finished slot; keeps the slot.
+ * PRH_ERR_SERIES if it did not converge. */
+int prh_batch_channels(prh_batch *b, int32_t slot, double *q_W_m2, double *residual, int32_t *iterations);
 /* Absorbed power (n), power still in flight and sweep count of a finished
  * slot; frees the slot.  PRH_ERR_SERIES if it did not converge. */
 int prh_batch_result(prh_batch *b, int32_t slot, double *absorbed, double *residual, int32_t *iterations);
+/* Free a finished slot without reading its absorbed power. */
+int prh_batch_release(prh_batch *b, int32_t slot);
 void prh_batch_destroy(prh_batch *b);
 
 #ifdef __cplusplus

@@ -1,4 +1,4 @@
-// Batched mixed-precision radiosity engine (continuous batching of solves).
+// Batched mixed-precision radiosity engine with continuous batching.
 //
 // Solves (I - diag(rho) F_p^T) J = E, rho = 1 - eps, for up to 16 emissions
 // ("slots") at once on one device-resident exchange operator, and returns the
@@ -7,26 +7,33 @@
 //
 // Method.  F_p^T x = d * S(d x / A) is similar to a symmetric matrix of
 // spectral radius 1 (F_p is row-stochastic and A F_p symmetric), so the
-// eigenvalues of I - diag(rho) F_p^T lie in [1 - rho_max, 1 + rho_max].
-// Each epoch runs m Chebyshev iterations on that interval in FP32 (residual
-// form), adds the correction to the FP64 solution, and recomputes the FP64
-// residual r = E - (x - rho * F_p^T x) with an FP64 sweep; epochs repeat
-// until ||r||_1 <= rtol ||E||_1 (iterative refinement).  m is the Chebyshev
-// count whose error bound, for the slot's rho_max, covers what is left to
-// reach the tolerance (at most a 1e-6 reduction, what FP32 sweeps resolve).
+// eigenvalues of I - diag(rho) F_p^T lie in [1 - rho_max, 1 + rho_max].  A
+// slot alternates epochs of m FP32 Chebyshev iterations on that interval
+// (residual form) with one FP64 refinement step that adds the epoch's
+// correction c to the FP64 solution x and recomputes the FP64 residual
+// r = E - (x - rho * F_p^T x), until ||r||_1 <= rtol ||E||_1.  m is the
+// Chebyshev count whose error bound covers what is left to reach the
+// tolerance, at most a 1e-6 reduction (what FP32 iterations resolve).
 //
-// Sweeps read the operator once for all 16 slots: the sparsity of S with the
-// exact integer K = C + C^T as uint16 when all patches share area and ray
-// count (S = A/(2R) K + A delta I), else FP32/FP64 values of S; the slot
-// vectors are interleaved [row][slot] so one gather serves every slot.
-// Workgroups are mapped so that each XCD sweeps a contiguous range of rows.
+// Every sweep ("tick") reads the operator once for all 16 slots and
+// accumulates the products in FP64 from FP32 slot vectors: K = C + C^T is
+// stored exactly as uint16 when all patches share area and ray count
+// (S = A/(2R) K + A delta I), else S in FP64.  So the refinement step of a
+// slot is just another tick: the slot's gathered vector is its correction,
+// whose FP64 product F_p^T c is exact up to FP64 rounding; it updates
+// x += s c and y = F_p^T x += s F_p^T c and then starts the next epoch in
+// the same pass.  Each slot runs its own schedule (Chebyshev tick, last
+// tick, refinement tick, idle) independently of the others, and a slot may
+// be admitted or retired at any tick.  Slot vectors are interleaved
+// [row][slot] so one 64 B gather serves all 16 slots; workgroups are mapped
+// so that each XCD sweeps a contiguous range of rows.
 //
-// Continuous batching: a free slot may be filled at any epoch boundary while
-// the other slots keep iterating.  A slot's arithmetic never mixes with
-// another slot's, every reduction has a fixed order, and a slot stops its
-// own Chebyshev iterations after its own count m (its direction is set to
-// zero while longer slots finish the epoch), so a solve returns the same
-// bits alone or in any mix.
+// Host scheduling is pipelined: the host launches tick t and only then
+// waits for tick t-1, so the device always has the next tick queued.  A
+// slot's decisions (epoch lengths, scales, convergence) depend only on its
+// own residual norms, read with a fixed lag of one tick; its arithmetic
+// never mixes with another slot's and every reduction has a fixed order, so
+// a solve returns the same bits alone or in any mix.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -40,23 +47,39 @@ namespace {
 constexpr int SLOTS = 16;
 constexpr int BT = 256;
 constexpr int ROWS = BT / 64;
+constexpr int RING = 8;
 constexpr int MAX_TICKS = 64;
-constexpr int MAX_EPOCHS = 30;
+constexpr int MAX_EPOCHS = 40;
 constexpr double INNER_REDUCTION = 1.0e-6;
+constexpr double SIGMA = 5.670374419e-8;  // porerad/constants.py:3
+
+enum : int32_t { MODE_IDLE = 0, MODE_CHEB = 1, MODE_LAST = 2, MODE_REFINE = 3 };
+
+// Per-slot instructions for one tick.
+struct SlotParam {
+    int32_t mode;
+    float alpha, beta;  // CHEB: dir' = alpha dir + beta r'
+    int32_t pad;
+    double s;           // REFINE: scale of the epoch that ends
+    double inv_s_next;  // REFINE: 1 / scale of the epoch that starts
+};
 
 struct MatView {
     int64_t n;
     const int64_t *rowptr;
     const int32_t *col;
     const uint16_t *kval;
-    const float *val32;
     const double *val64;
     const float *d32;
     const double *d64;
     const double *area;
-    float inv_2R_f, delta_f;
     double inv_2R, delta;
     uint32_t blocks_per_xcd;
+};
+
+struct SlotVecs {
+    float *x32, *r32, *dir32, *rho32;
+    double *x64, *y64, *r64, *E64, *eps64;
 };
 
 // Workgroups are dealt round-robin to the 8 XCDs; give XCD x the logical
@@ -68,113 +91,107 @@ __device__ __forceinline__ int64_t first_row(const MatView &M)
     return ((int64_t)(b % 8u) * M.blocks_per_xcd + (b / 8u)) * ROWS;
 }
 
-struct TickArgs {
-    MatView M;
-    const float *z_in;
-    float *z_out;
-    float *x32, *r32, *dir32;
-    const float *rho32;
-    const float *coef;  // [SLOTS][2] alpha, beta of this tick
-};
-
-// One FP32 Chebyshev iteration for all slots: y = F_p^T dir, then
-// x += dir, r -= dir - rho y, dir = alpha dir + beta r, z = scaled dir.
-template <bool UNIFORM>
-__global__ __launch_bounds__(BT) void k_cheb_tick(TickArgs a)
+// The gathered form of a slot vector v: d v (equal areas) or d v / A.
+__device__ __forceinline__ float scaled(float di, float v, float ai, bool uniform)
 {
-    const MatView &M = a.M;
-    const int64_t row = first_row(M) + (threadIdx.x >> 6);
-    if (row >= M.n) return;
-    const int lane = threadIdx.x & 63;
-    const int h = lane & 3;   // slots 4h .. 4h+3
-    const int g = lane >> 2;  // nonzero group
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    const int64_t e1 = M.rowptr[row + 1];
-    for (int64_t e = M.rowptr[row] + g; e < e1; e += 16) {
-        const int32_t j = M.col[e];
-        const float v = UNIFORM ? (float)M.kval[e] : M.val32[e];
-        const float4 z = *reinterpret_cast<const float4 *>(a.z_in + (size_t)j * SLOTS + 4 * h);
-        acc.x += v * z.x;
-        acc.y += v * z.y;
-        acc.z += v * z.z;
-        acc.w += v * z.w;
-    }
-    for (int off = 4; off < 64; off <<= 1) {
-        acc.x += __shfl_xor(acc.x, off);
-        acc.y += __shfl_xor(acc.y, off);
-        acc.z += __shfl_xor(acc.z, off);
-        acc.w += __shfl_xor(acc.w, off);
-    }
-    if (g != 0) return;
-    const float di = M.d32[row];
-    const float ai = UNIFORM ? 1.f : (float)M.area[row];
-    const float w[4] = {acc.x, acc.y, acc.z, acc.w};
-    for (int c = 0; c < 4; ++c) {
-        const int q = 4 * h + c;
-        const size_t at = (size_t)row * SLOTS + q;
-        const float y = UNIFORM ? di * (w[c] * M.inv_2R_f + M.delta_f * a.z_in[at]) : di * w[c];
-        const float dk = a.dir32[at];
-        const float Ad = dk - a.rho32[at] * y;
-        const float rn = a.r32[at] - Ad;
-        const float dn = a.coef[2 * q] * dk + a.coef[2 * q + 1] * rn;
-        a.x32[at] = a.x32[at] + dk;
-        a.r32[at] = rn;
-        a.dir32[at] = dn;
-        a.z_out[at] = UNIFORM ? di * dn : di * dn / ai;
-    }
+    return uniform ? di * v : di * v / ai;
 }
 
-struct ResArgs {
+struct TickArgs {
     MatView M;
-    const double *z64;
-    const double *x64, *E64, *eps64;
-    double *y64, *r64;
-    double *part_abs, *part_sum;  // [blocks][SLOTS]
+    SlotVecs V;
+    const float *z_in;
+    float *z_out;
+    const SlotParam *param;       // [SLOTS]
+    double *part_abs, *part_sum;  // [SLOTS][n_blocks], written when a slot refines
+    int64_t n_blocks;
+    uint32_t group_mask;  // bit h: one of the slots 4h..4h+3 is not idle
+    int32_t any_refine;
 };
 
-// FP64 residual sweep: y = F_p^T x, r = E - (x - (1 - eps) y); per-block
-// sums of |r| and r per slot.
+// One tick: y = F_p^T z for every slot (FP64 accumulation of FP32 inputs),
+// then each slot's own update (Chebyshev step, last step, or refinement).
 template <bool UNIFORM>
-__global__ __launch_bounds__(BT) void k_residual64(ResArgs a)
+__global__ __launch_bounds__(BT) void k_tick(TickArgs a)
 {
     __shared__ double s_abs[ROWS][SLOTS], s_sum[ROWS][SLOTS];
     const MatView &M = a.M;
     const int wave = threadIdx.x >> 6;
     const int64_t row = first_row(M) + wave;
     const int lane = threadIdx.x & 63;
-    const int h = lane & 7;   // slots 2h, 2h+1
-    const int g = lane >> 3;  // nonzero group
-    double2 acc = make_double2(0.0, 0.0);
-    if (row < M.n) {
+    const int h = lane & 3;   // gather: slots 4h .. 4h+3
+    const int g = lane >> 2;  // nonzero group
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
+    if (row < M.n && ((a.group_mask >> h) & 1u)) {
         const int64_t e1 = M.rowptr[row + 1];
-        for (int64_t e = M.rowptr[row] + g; e < e1; e += 8) {
+        for (int64_t e = M.rowptr[row] + g; e < e1; e += 16) {
             const int32_t j = M.col[e];
             const double v = UNIFORM ? (double)M.kval[e] : M.val64[e];
-            const double2 z = *reinterpret_cast<const double2 *>(a.z64 + (size_t)j * SLOTS + 2 * h);
-            acc.x += v * z.x;
-            acc.y += v * z.y;
+            const float4 z = *reinterpret_cast<const float4 *>(a.z_in + (size_t)j * SLOTS + 4 * h);
+            acc0 = fma(v, (double)z.x, acc0);
+            acc1 = fma(v, (double)z.y, acc1);
+            acc2 = fma(v, (double)z.z, acc2);
+            acc3 = fma(v, (double)z.w, acc3);
         }
     }
-    for (int off = 8; off < 64; off <<= 1) {
-        acc.x += __shfl_xor(acc.x, off);
-        acc.y += __shfl_xor(acc.y, off);
+    for (int off = 4; off < 64; off <<= 1) {
+        acc0 += __shfl_xor(acc0, off);
+        acc1 += __shfl_xor(acc1, off);
+        acc2 += __shfl_xor(acc2, off);
+        acc3 += __shfl_xor(acc3, off);
     }
-    if (g == 0) {
-        const double w[2] = {acc.x, acc.y};
-        for (int c = 0; c < 2; ++c) {
-            const int q = 2 * h + c;
-            double r = 0.0;
-            if (row < M.n) {
-                const size_t at = (size_t)row * SLOTS + q;
-                const double di = M.d64[row];
-                const double y = UNIFORM ? di * (w[c] * M.inv_2R + M.delta * a.z64[at]) : di * w[c];
-                r = a.E64[at] - (a.x64[at] - (1.0 - a.eps64[at]) * y);
-                a.y64[at] = y;
-                a.r64[at] = r;
+    // Every lane now holds the sums of its 4 slots; lanes g < 4 update slot 4h + g.
+    double r_abs = 0.0, r_val = 0.0;
+    const int q = 4 * h + (g & 3);
+    if (g < 4 && row < M.n) {
+        const SlotParam P = a.param[q];
+        if (P.mode != MODE_IDLE) {
+            const double w = g == 0 ? acc0 : g == 1 ? acc1 : g == 2 ? acc2 : acc3;
+            const SlotVecs &V = a.V;
+            const size_t at = (size_t)row * SLOTS + q;
+            const float zi = a.z_in[at];
+            const double di = M.d64[row];
+            const double y = UNIFORM ? di * (w * M.inv_2R + M.delta * (double)zi) : di * w;
+            const float di32 = M.d32[row];
+            const float ai32 = UNIFORM ? 1.f : (float)M.area[row];
+            if (P.mode == MODE_REFINE) {
+                const double ai = UNIFORM ? 1.0 : M.area[row];
+                const double c = UNIFORM ? (double)zi / di : (double)zi * ai / di;
+                const double x = V.x64[at] + P.s * c;
+                const double yy = V.y64[at] + P.s * y;
+                const double r = V.E64[at] - (x - (1.0 - V.eps64[at]) * yy);
+                V.x64[at] = x;
+                V.y64[at] = yy;
+                V.r64[at] = r;
+                r_abs = fabs(r);
+                r_val = r;
+                // next epoch: x32 = 0, r32 = dir = r / s
+                const float rn = (float)(r * P.inv_s_next);
+                V.x32[at] = 0.f;
+                V.r32[at] = rn;
+                V.dir32[at] = rn;
+                a.z_out[at] = scaled(di32, rn, ai32, UNIFORM);
+            } else {
+                const float dk = V.dir32[at];
+                const float Ad = dk - V.rho32[at] * (float)y;
+                const float rn = V.r32[at] - Ad;
+                const float x = V.x32[at] + dk;
+                V.x32[at] = x;
+                if (P.mode == MODE_CHEB) {
+                    const float dn = P.alpha * dk + P.beta * rn;
+                    V.r32[at] = rn;
+                    V.dir32[at] = dn;
+                    a.z_out[at] = scaled(di32, dn, ai32, UNIFORM);
+                } else {  // MODE_LAST: the epoch's correction is gathered next
+                    a.z_out[at] = scaled(di32, x, ai32, UNIFORM);
+                }
             }
-            s_abs[wave][q] = fabs(r);
-            s_sum[wave][q] = r;
         }
+    }
+    if (!a.any_refine) return;  // the same for every thread of the grid
+    if (g < 4) {
+        s_abs[wave][q] = r_abs;
+        s_sum[wave][q] = r_val;
     }
     __syncthreads();
     if (threadIdx.x < SLOTS) {
@@ -183,87 +200,124 @@ __global__ __launch_bounds__(BT) void k_residual64(ResArgs a)
             sa += s_abs[w][threadIdx.x];
             ss += s_sum[w][threadIdx.x];
         }
-        a.part_abs[(size_t)blockIdx.x * SLOTS + threadIdx.x] = sa;
-        a.part_sum[(size_t)blockIdx.x * SLOTS + threadIdx.x] = ss;
+        a.part_abs[(size_t)threadIdx.x * a.n_blocks + blockIdx.x] = sa;
+        a.part_sum[(size_t)threadIdx.x * a.n_blocks + blockIdx.x] = ss;
     }
 }
 
-// Fixed-order per-slot totals of the block partial sums.
-__global__ __launch_bounds__(256) void k_slot_totals(const double *part_abs, const double *part_sum, int64_t n_blocks,
-                                                     double *tot_abs, double *tot_sum)
+// Fixed-order totals of columns of partial sums: out[c] = sum_b part[c][b].
+__global__ __launch_bounds__(256) void k_column_totals(const double *part, int64_t n_blocks, double *out)
 {
-    __shared__ double sa[256], ss[256];
-    const int q = blockIdx.x;
-    double a = 0.0, s = 0.0;
-    for (int64_t b = threadIdx.x; b < n_blocks; b += 256) {
-        a += part_abs[b * SLOTS + q];
-        s += part_sum[b * SLOTS + q];
-    }
-    sa[threadIdx.x] = a;
-    ss[threadIdx.x] = s;
+    __shared__ double sh[256];
+    const int c = blockIdx.x;
+    double a = 0.0;
+    for (int64_t b = threadIdx.x; b < n_blocks; b += 256) a += part[(size_t)c * n_blocks + b];
+    sh[threadIdx.x] = a;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < off) {
-            sa[threadIdx.x] += sa[threadIdx.x + off];
-            ss[threadIdx.x] += ss[threadIdx.x + off];
-        }
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        tot_abs[q] = sa[0];
-        tot_sum[q] = ss[0];
+    if (threadIdx.x == 0) out[c] = sh[0];
+}
+
+// Sum of v over the 256 threads of the block, written to *part_at.
+__device__ __forceinline__ void block_sum_to(double v, double *sh, double *part_at)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) *part_at = sh[0];
+    __syncthreads();
 }
 
-// Epoch start for the slots in `mask`: r32 = r64 / s, x32 = 0, dir = r32,
-// z = scaled dir.  Other slots get zero direction (they stay put).
-__global__ void k_epoch_begin(MatView M, uint32_t mask, const double *inv_s, const double *r64, float *x32, float *r32,
-                              float *dir32, float *z32, int uniform)
+// Admission state of slot q in row i: x = y = 0, r = E, and the first
+// epoch's FP32 state r32 = dir = E / s with its gathered vector.
+__device__ __forceinline__ void admit_row(const MatView &M, const SlotVecs &V, float *z_next, int64_t i, int q,
+                                          double E, double e, double inv_s, bool uniform)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= M.n * SLOTS) return;
-    const int64_t row = t / SLOTS;
-    const int q = (int)(t % SLOTS);
-    const bool on = (mask >> q) & 1u;
-    const float r = on ? (float)(r64[t] * inv_s[q]) : 0.f;
-    x32[t] = 0.f;
-    r32[t] = r;
-    dir32[t] = r;
-    const float di = M.d32[row];
-    z32[t] = uniform ? di * r : di * r / (float)M.area[row];
+    const size_t at = (size_t)i * SLOTS + q;
+    V.E64[at] = E;
+    V.eps64[at] = e;
+    V.rho32[at] = (float)(1.0 - e);
+    V.x64[at] = 0.0;
+    V.y64[at] = 0.0;
+    V.r64[at] = E;
+    const float r = (float)(E * inv_s);
+    V.x32[at] = 0.f;
+    V.r32[at] = r;
+    V.dir32[at] = r;
+    z_next[at] = scaled(M.d32[i], r, uniform ? 1.f : (float)M.area[i], uniform);
 }
 
-// Epoch end: x64 += s * x32 and z64 = scaled x64 for the slots in mask.
-__global__ void k_epoch_accumulate(MatView M, uint32_t mask, const double *s, const float *x32, double *x64,
-                                   double *z64, int uniform)
-{
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= M.n * SLOTS) return;
-    const int64_t row = t / SLOTS;
-    const int q = (int)(t % SLOTS);
-    double x = x64[t];
-    if ((mask >> q) & 1u) x = x + s[q] * (double)x32[t];
-    x64[t] = x;
-    const double di = M.d64[row];
-    z64[t] = uniform ? di * x : di * x / M.area[row];
-}
-
-// Admission of one slot: E, eps in; x64 = 0, r64 = E.
-__global__ void k_admit(int64_t n, int q, const double *E, const double *eps, int eps_per_patch, double *E64,
-                        double *eps64, float *rho32, double *x64, double *r64)
+__global__ __launch_bounds__(256) void k_admit(MatView M, SlotVecs V, float *z_next, int q, const double *E,
+                                               const double *eps, int eps_per_patch, double inv_s, int uniform)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const size_t at = (size_t)i * SLOTS + q;
-    const double e = eps_per_patch ? eps[i] : eps[0];
-    E64[at] = E[i];
-    eps64[at] = e;
-    rho32[at] = (float)(1.0 - e);
-    x64[at] = 0.0;
-    r64[at] = E[i];
+    if (i >= M.n) return;
+    admit_row(M, V, z_next, i, q, E[i], eps_per_patch ? eps[i] : eps[0], inv_s, uniform != 0);
 }
 
-// Retirement of one slot: absorbed = eps * y (column q) into a packed vector.
+struct FieldView {
+    const int8_t *cls;      // 0 hot wall, 1 cold wall, 2 internal
+    const int32_t *cell_x;  // x index of the owning cell (internal patches)
+    double nx;
+};
+
+// Emission of the prescribed linear field between T_hot and T_cold with
+// uniform emissivity: walls at T_m +/- dT/2, internal patches at
+// (T_m + dT/2) - (i + 1/2) dT / nx, minus the isothermal state at T_m
+// (porerad/coupled.py:138-143, 180-184; porerad/exchange.py:265-270).
+// Also the block sums of |E|.
+__global__ __launch_bounds__(256) void k_admit_linear(MatView M, SlotVecs V, FieldView F, float *z_next, int q,
+                                                      double eps, double eps_sigma, double T_m, double dT,
+                                                      double inv_s, int uniform, double *part)
+{
+    __shared__ double sh[256];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double E = 0.0;
+    if (i < M.n) {
+        const int c = F.cls[i];
+        double T;
+        if (c == 0) T = T_m + dT / 2.0;
+        else if (c == 1) T = T_m - dT / 2.0;
+        else T = (T_m + dT / 2.0) - ((double)F.cell_x[i] + 0.5) * dT / F.nx;
+        E = eps_sigma * M.area[i] * ((T - T_m) * (T + T_m) * (T * T + T_m * T_m));
+        admit_row(M, V, z_next, i, q, E, eps, inv_s, uniform != 0);
+    }
+    block_sum_to(fabs(E), sh, part + blockIdx.x);
+}
+
+// Results of a finished slot, as block sums: per class sums of
+// net = eps y - E (hot, cold, internal; zero without a field) and the power
+// still in flight sum(r).
+__global__ __launch_bounds__(256) void k_finish(int64_t n, int q, FieldView F, SlotVecs V, int64_t n_part,
+                                                double *part)
+{
+    __shared__ double sh[256];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    if (i < n) {
+        const size_t at = (size_t)i * SLOTS + q;
+        const double net = V.eps64[at] * V.y64[at] - V.E64[at];
+        if (F.cls) {
+            const int c = F.cls[i];
+            s0 = c == 0 ? net : 0.0;
+            s1 = c == 1 ? net : 0.0;
+            s2 = c == 2 ? net : 0.0;
+        }
+        s3 = V.r64[at];
+    }
+    block_sum_to(s0, sh, part + blockIdx.x);
+    block_sum_to(s1, sh, part + n_part + blockIdx.x);
+    block_sum_to(s2, sh, part + 2 * n_part + blockIdx.x);
+    block_sum_to(s3, sh, part + 3 * n_part + blockIdx.x);
+}
+
+// Absorbed power eps * y of slot q into a packed vector.
 __global__ void k_collect(int64_t n, int q, const double *eps64, const double *y64, double *out)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -274,10 +328,69 @@ __global__ void k_collect(int64_t n, int q, const double *eps64, const double *y
 
 unsigned grid_of(int64_t n, int b) { return (unsigned)std::max<int64_t>(1, (n + b - 1) / b); }
 
+// Chebyshev iterations that shrink the error bound 2 c^k / (1 + c^2k) below
+// tau (at least 2: the epoch length is decided one tick after it starts).
+int ticks_for(double rho_max, double tau)
+{
+    if (rho_max <= 0.0) return 2;
+    const double kappa = (1.0 + rho_max) / (1.0 - rho_max);
+    const double c = (std::sqrt(kappa) - 1.0) / (std::sqrt(kappa) + 1.0);
+    const int m = (int)std::ceil(std::log(tau / 2.0) / std::log(c));
+    return std::min(std::max(m, 2), MAX_TICKS);
+}
+
+// Reduction an epoch aims for: what is left to reach the tolerance, with a
+// factor 10 of margin, but no finer than FP32 iterations resolve.
+double epoch_target(double rtol, double e_norm, double r_norm)
+{
+    const double left = 0.1 * rtol * e_norm / std::max(r_norm, 1e-300);
+    return std::min(std::max(left, INNER_REDUCTION), 0.5);
+}
+
+// alpha_k, beta_k of Chebyshev acceleration on [1 - delta, 1 + delta]:
+// rho_0 = delta, rho_{k+1} = 1 / (2/delta - rho_k),
+// dir_{k+1} = rho_{k+1} rho_k dir_k + (2 rho_{k+1} / delta) r_{k+1}.
+void cheb_coefficient(double delta, int k, float *alpha, float *beta)
+{
+    if (delta <= 0.0) {
+        *alpha = 0.f;
+        *beta = 1.f;
+        return;
+    }
+    double rho = delta, next = 0.0;
+    for (int i = 0; i <= k; ++i) {
+        next = 1.0 / (2.0 / delta - rho);
+        if (i < k) rho = next;
+    }
+    *alpha = (float)(next * rho);
+    *beta = (float)(2.0 * next / delta);
+}
+
+enum SlotState { FREE = 0, RUNNING, FINISHING, DONE };
+
 struct SlotHost {
-    bool active = false, done = false, failed = false;
-    double rtol = 0, e_norm = 0, rho_max = 0, r_norm = 0, r_sum = 0;
-    int epochs = 0, ticks = 0, epoch_ticks = 0;
+    SlotState state = FREE;
+    bool failed = false;
+    bool field = false;
+    bool refining = false;     // the slot's next tick is its refinement
+    double rtol = 0, rho_max = 0;
+    double e_norm = 0;         // ||E||_1 (once known)
+    int64_t e_norm_step = -1;  // step whose completion brings the device ||E||_1
+    double r_norm = 0;         // ||r||_1 after the last refinement read
+    int epochs = 0;            // refinements read
+    int ticks = 0;             // Chebyshev ticks run
+    int k = 0;                 // tick index within the current epoch
+    int m = 0;                 // length of the current epoch (0 = not known yet)
+    double s = 1.0, tau = 0.5; // scale and target reduction of the current epoch
+    int64_t refine_step = -1;  // step of a refinement not yet read
+    int64_t finish_step = -1;  // step whose completion brings the results
+};
+
+struct Pinned {
+    SlotParam param[RING][SLOTS];
+    double tot[RING][2 * SLOTS];
+    double e_norm[SLOTS];
+    double fin[SLOTS][4];
 };
 
 }  // namespace
@@ -290,10 +403,19 @@ struct prh_batch {
     int device = 0;
     int64_t n = 0, n_blocks = 0;
     uint32_t blocks_per_xcd = 0;
-    float *x32 = nullptr, *r32 = nullptr, *dir32 = nullptr, *za = nullptr, *zb = nullptr, *rho32 = nullptr;
-    double *x64 = nullptr, *E64 = nullptr, *eps64 = nullptr, *y64 = nullptr, *r64 = nullptr, *z64 = nullptr;
-    double *part_abs = nullptr, *part_sum = nullptr, *tot = nullptr, *svec = nullptr, *stage = nullptr;
-    float *coef = nullptr;
+    float *x32 = nullptr, *r32 = nullptr, *dir32 = nullptr, *rho32 = nullptr, *zbuf[2] = {nullptr, nullptr};
+    double *x64 = nullptr, *y64 = nullptr, *r64 = nullptr, *E64 = nullptr, *eps64 = nullptr;
+    double *part = nullptr, *tot = nullptr, *stage = nullptr;
+    SlotParam *param_dev = nullptr;
+    int8_t *cls = nullptr;
+    int32_t *cell_x = nullptr;
+    int32_t field_nx = 0;
+    double cross_section = 0.0, mean_area = 0.0;
+    Pinned *host = nullptr;
+    hipEvent_t ev[RING] = {};
+    bool ev_used[RING] = {};
+    int64_t step = 0;   // steps taken
+    int64_t ticks = 0;  // ticks launched
     SlotHost slot[SLOTS];
     int64_t bytes = 0;
 };
@@ -308,60 +430,194 @@ MatView view_of(const prh_batch *b)
     M.rowptr = op->rowptr;
     M.col = op->col;
     M.kval = op->kval;
-    M.val32 = op->val32;
     M.val64 = op->val;
     M.d32 = op->scale32;
     M.d64 = op->scale;
     M.area = op->area;
     M.inv_2R = op->inv_2R;
     M.delta = op->delta;
-    M.inv_2R_f = (float)op->inv_2R;
-    M.delta_f = (float)op->delta;
     M.blocks_per_xcd = b->blocks_per_xcd;
     return M;
 }
 
+SlotVecs vecs_of(const prh_batch *b)
+{
+    return SlotVecs{b->x32, b->r32, b->dir32, b->rho32, b->x64, b->y64, b->r64, b->E64, b->eps64};
+}
+
+FieldView field_of(const prh_batch *b) { return FieldView{b->cls, b->cell_x, (double)b->field_nx}; }
+
 void free_batch(prh_batch *b)
 {
-    void *ptrs[] = {b->x32, b->r32, b->dir32, b->za, b->zb, b->rho32, b->x64, b->E64, b->eps64, b->y64,
-                    b->r64, b->z64, b->part_abs, b->part_sum, b->tot, b->svec, b->stage, b->coef};
+    void *ptrs[] = {b->x32, b->r32,   b->dir32, b->rho32, b->zbuf[0], b->zbuf[1], b->x64,       b->y64, b->r64,
+                    b->E64, b->eps64, b->part,  b->tot,   b->stage,   b->cls,     b->cell_x,    b->param_dev};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    if (b->host) (void)hipHostFree(b->host);
+    for (int k = 0; k < RING; ++k)
+        if (b->ev[k]) (void)hipEventDestroy(b->ev[k]);
 }
 
-// Chebyshev iterations that shrink the error bound 2 c^k / (1 + c^2k) below tau.
-int ticks_for(double rho_max, double tau)
+// The z buffer the next tick gathers.
+float *z_next(prh_batch *b) { return b->zbuf[b->ticks & 1]; }
+
+int wait_step(prh_batch *b, int64_t s)
 {
-    if (rho_max <= 0.0) return 1;
-    const double kappa = (1.0 + rho_max) / (1.0 - rho_max);
-    const double c = (std::sqrt(kappa) - 1.0) / (std::sqrt(kappa) + 1.0);
-    const int m = (int)std::ceil(std::log(tau / 2.0) / std::log(c));
-    return std::min(std::max(m, 1), MAX_TICKS);
+    if (s < 0 || s >= b->step) return PRH_OK;
+    const int k = (int)(s % RING);
+    if (!b->ev_used[k]) return PRH_OK;
+    PRH_HIP(hipEventSynchronize(b->ev[k]));
+    return PRH_OK;
 }
 
-// Reduction an epoch aims for: what is left to reach the tolerance, with a
-// factor 10 of margin, but no finer than FP32 sweeps resolve.
-double epoch_target(const SlotHost &S)
+// Everything queued up to the end of step s has completed on the device:
+// read what it produced and move the slot schedules on.
+void read_step(prh_batch *b, int64_t s)
 {
-    const double left = 0.1 * S.rtol * S.e_norm / std::max(S.r_norm, 1e-300);
-    return std::min(std::max(left, INNER_REDUCTION), 0.5);
-}
-
-// alpha_k, beta_k of Chebyshev acceleration on [1 - delta, 1 + delta]
-// (theta = 1): rho_0 = delta, rho_{k+1} = 1 / (2/delta - rho_k),
-// dir_{k+1} = rho_{k+1} rho_k dir_k + (2 rho_{k+1} / delta) r_{k+1}.
-void cheb_coefficients(double delta, int m, std::vector<float> &alpha, std::vector<float> &beta)
-{
-    alpha.assign(m, 0.f);
-    beta.assign(m, 1.f);
-    if (delta <= 0.0) return;
-    double rho = delta;
-    for (int k = 0; k < m; ++k) {
-        const double next = 1.0 / (2.0 / delta - rho);
-        alpha[k] = (float)(next * rho);
-        beta[k] = (float)(2.0 * next / delta);
-        rho = next;
+    for (int q = 0; q < SLOTS; ++q) {
+        SlotHost &S = b->slot[q];
+        if (S.state == FREE) continue;
+        if (S.e_norm_step >= 0 && S.e_norm_step <= s) {
+            S.e_norm = b->host->e_norm[q];
+            S.r_norm = S.e_norm;
+            S.e_norm_step = -1;
+        }
+        if (S.state == FINISHING && S.finish_step >= 0 && S.finish_step <= s) S.state = DONE;
+        if (S.state != RUNNING || S.refine_step < 0 || S.refine_step > s) continue;
+        S.r_norm = b->host->tot[S.refine_step % RING][q];
+        S.refine_step = -1;
+        S.epochs += 1;
+        const bool converged = S.r_norm <= S.rtol * S.e_norm;
+        if (converged || S.epochs >= MAX_EPOCHS) {
+            // the epoch that began with the refinement is abandoned
+            S.ticks -= S.k;
+            S.failed = !converged;
+            S.state = FINISHING;  // finish kernels are queued by the caller
+        } else {
+            // length of the epoch that began with the refinement
+            S.tau = epoch_target(S.rtol, S.e_norm, S.r_norm);
+            S.m = ticks_for(S.rho_max, S.tau);
+        }
     }
+}
+
+int queue_finish(prh_batch *b, int q, hipStream_t st)
+{
+    const int64_t n = b->n;
+    const unsigned grid = grid_of(n, 256);
+    k_finish<<<grid, 256, 0, st>>>(n, q, field_of(b), vecs_of(b), (int64_t)grid, b->part);
+    k_column_totals<<<4, 256, 0, st>>>(b->part, (int64_t)grid, b->tot);
+    PRH_HIP(hipGetLastError());
+    PRH_HIP(hipMemcpyAsync(b->host->fin[q], b->tot, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+    return PRH_OK;
+}
+
+int first_free(const prh_batch *b)
+{
+    for (int k = 0; k < SLOTS; ++k)
+        if (b->slot[k].state == FREE) return k;
+    return -1;
+}
+
+// One pipelined step: launch the next tick of every running slot, then wait
+// for the previous step and read its results.
+int batch_step(prh_batch *b, hipStream_t st, bool *launched)
+{
+    const int64_t s = b->step;
+    SlotParam *P = b->host->param[s % RING];
+    uint32_t group_mask = 0;
+    int any = 0, any_refine = 0;
+    for (int q = 0; q < SLOTS; ++q) {
+        SlotHost &S = b->slot[q];
+        SlotParam &p = P[q];
+        p = SlotParam{MODE_IDLE, 0.f, 1.f, 0, 0.0, 0.0};
+        if (S.state != RUNNING) continue;
+        if (S.refining) {
+            // the epoch's correction is gathered now: refine, then the next epoch
+            const double s_next = S.s * S.tau;
+            p.mode = MODE_REFINE;
+            p.s = S.s;
+            S.s = s_next > 0.0 ? s_next : 1.0;
+            p.inv_s_next = 1.0 / S.s;
+            S.refining = false;
+            S.refine_step = s;
+            S.k = 0;
+            S.m = 0;  // known once this refinement is read, one step later
+            any_refine = 1;
+        } else {
+            // With the one-step lag the epoch length is always known by k = 1.
+            const bool last = S.m > 0 && S.k >= S.m - 1;
+            p.mode = last ? MODE_LAST : MODE_CHEB;
+            if (!last) cheb_coefficient(S.rho_max, S.k, &p.alpha, &p.beta);
+            S.k += 1;
+            S.ticks += 1;
+            S.refining = last;
+        }
+        group_mask |= 1u << (q / 4);
+        any = 1;
+    }
+    *launched = any != 0;
+    if (any) {
+        SlotParam *dev = b->param_dev + (s % RING) * SLOTS;
+        PRH_HIP(hipMemcpyAsync(dev, P, SLOTS * sizeof(SlotParam), hipMemcpyHostToDevice, st));
+        TickArgs a;
+        a.M = view_of(b);
+        a.V = vecs_of(b);
+        a.z_in = b->zbuf[b->ticks & 1];
+        a.z_out = b->zbuf[(b->ticks + 1) & 1];
+        a.param = dev;
+        a.part_abs = b->part;
+        a.part_sum = b->part + (size_t)SLOTS * b->n_blocks;
+        a.n_blocks = b->n_blocks;
+        a.group_mask = group_mask;
+        a.any_refine = any_refine;
+        if (b->op->uniform) k_tick<true><<<(unsigned)b->n_blocks, BT, 0, st>>>(a);
+        else k_tick<false><<<(unsigned)b->n_blocks, BT, 0, st>>>(a);
+        PRH_HIP(hipGetLastError());
+        b->ticks += 1;
+        if (any_refine) {
+            k_column_totals<<<2 * SLOTS, 256, 0, st>>>(b->part, b->n_blocks, b->tot);
+            PRH_HIP(hipGetLastError());
+            PRH_HIP(hipMemcpyAsync(b->host->tot[s % RING], b->tot, 2 * SLOTS * sizeof(double),
+                                   hipMemcpyDeviceToHost, st));
+        }
+    }
+    // this step's tick is queued behind the previous one: read the previous one
+    int rc = wait_step(b, s - 1);
+    if (rc) return rc;
+    read_step(b, s - 1);
+    for (int q = 0; q < SLOTS; ++q) {
+        SlotHost &S = b->slot[q];
+        if (S.state == FINISHING && S.finish_step < 0) {
+            if ((rc = queue_finish(b, q, st))) return rc;
+            S.finish_step = s;
+        }
+    }
+    PRH_HIP(hipEventRecord(b->ev[s % RING], st));
+    b->ev_used[s % RING] = true;
+    b->step = s + 1;
+    return PRH_OK;
+}
+
+// Wait for everything queued so far and read it.
+int batch_drain(prh_batch *b)
+{
+    if (b->step == 0) return PRH_OK;
+    int rc = wait_step(b, b->step - 1);
+    if (rc) return rc;
+    read_step(b, b->step - 1);
+    return PRH_OK;
+}
+
+void report(const prh_batch *b, uint32_t *done_mask, uint32_t *active_mask)
+{
+    uint32_t done = 0, active = 0;
+    for (int q = 0; q < SLOTS; ++q) {
+        if (b->slot[q].state != FREE) active |= 1u << q;
+        if (b->slot[q].state == DONE) done |= 1u << q;
+    }
+    if (done_mask) *done_mask = done;
+    if (active_mask) *active_mask = active;
 }
 
 }  // namespace
@@ -381,6 +637,7 @@ int prh_batch_create(prh_operator *op, prh_batch **out)
     b->blocks_per_xcd = (uint32_t)((nb_logical + 7) / 8);
     b->n_blocks = (int64_t)b->blocks_per_xcd * 8;
     const size_t ns = (size_t)op->n * SLOTS;
+    const int64_t row_blocks = grid_of(op->n, 256);
     bool ok = true;
     auto alloc = [&](void **p, size_t bytes) {
         if (ok && hipMalloc(p, bytes) != hipSuccess) ok = false;
@@ -389,33 +646,34 @@ int prh_batch_create(prh_operator *op, prh_batch **out)
     alloc((void **)&b->x32, ns * 4);
     alloc((void **)&b->r32, ns * 4);
     alloc((void **)&b->dir32, ns * 4);
-    alloc((void **)&b->za, ns * 4);
-    alloc((void **)&b->zb, ns * 4);
     alloc((void **)&b->rho32, ns * 4);
+    alloc((void **)&b->zbuf[0], ns * 4);
+    alloc((void **)&b->zbuf[1], ns * 4);
     alloc((void **)&b->x64, ns * 8);
-    alloc((void **)&b->E64, ns * 8);
-    alloc((void **)&b->eps64, ns * 8);
     alloc((void **)&b->y64, ns * 8);
     alloc((void **)&b->r64, ns * 8);
-    alloc((void **)&b->z64, ns * 8);
-    alloc((void **)&b->part_abs, (size_t)b->n_blocks * SLOTS * 8);
-    alloc((void **)&b->part_sum, (size_t)b->n_blocks * SLOTS * 8);
+    alloc((void **)&b->E64, ns * 8);
+    alloc((void **)&b->eps64, ns * 8);
+    // partial sums: refinement norms [2 SLOTS][n_blocks], and the row-block
+    // sums of admissions (|E|, tail of the array) and finishes ([4][row_blocks])
+    alloc((void **)&b->part, (size_t)std::max<int64_t>(2 * SLOTS * b->n_blocks, 4 * row_blocks) * 8);
     alloc((void **)&b->tot, 2 * SLOTS * 8);
-    alloc((void **)&b->svec, 2 * SLOTS * 8);
     alloc((void **)&b->stage, (size_t)op->n * 2 * 8);
-    alloc((void **)&b->coef, (size_t)MAX_TICKS * SLOTS * 2 * 4);
+    alloc((void **)&b->param_dev, (size_t)RING * SLOTS * sizeof(SlotParam));
+    if (ok && hipHostMalloc((void **)&b->host, sizeof(Pinned), hipHostMallocDefault) != hipSuccess) ok = false;
+    for (int k = 0; ok && k < RING; ++k)
+        if (hipEventCreateWithFlags(&b->ev[k], hipEventDisableTiming) != hipSuccess) ok = false;
     if (!ok) {
         free_batch(b);
         delete b;
         set_error("prh_batch_create: out of device memory");
         return PRH_ERR_HIP;
     }
-    // Zero the slot state so idle slots carry zeros through every sweep.
-    for (void *p : {(void *)b->x32, (void *)b->r32, (void *)b->dir32, (void *)b->za, (void *)b->zb,
-                    (void *)b->rho32})
+    // Zero the slot state so idle slots carry finite values through every sweep.
+    for (void *p : {(void *)b->x32, (void *)b->r32, (void *)b->dir32, (void *)b->rho32, (void *)b->zbuf[0],
+                    (void *)b->zbuf[1]})
         PRH_HIP(hipMemset(p, 0, ns * 4));
-    for (void *p : {(void *)b->x64, (void *)b->E64, (void *)b->eps64, (void *)b->y64, (void *)b->r64,
-                    (void *)b->z64})
+    for (void *p : {(void *)b->x64, (void *)b->y64, (void *)b->r64, (void *)b->E64, (void *)b->eps64})
         PRH_HIP(hipMemset(p, 0, ns * 8));
     *out = b;
     return PRH_OK;
@@ -433,8 +691,42 @@ int prh_batch_free_slots(const prh_batch *b, uint32_t *free_mask)
     PRH_CHECK_ARG(b && free_mask, "prh_batch_free_slots: null pointer");
     uint32_t m = 0;
     for (int q = 0; q < SLOTS; ++q)
-        if (!b->slot[q].active) m |= 1u << q;
+        if (b->slot[q].state == FREE) m |= 1u << q;
     *free_mask = m;
+    return PRH_OK;
+}
+
+int prh_batch_set_field(prh_batch *b, const int8_t *patch_class, const int32_t *cell_x, int32_t nx,
+                        double cross_section_m2)
+{
+    PRH_CHECK_ARG(b && patch_class && cell_x, "prh_batch_set_field: null pointer");
+    PRH_CHECK_ARG(nx > 0 && cross_section_m2 > 0.0, "prh_batch_set_field: nx and cross section must be positive");
+    const int64_t n = b->n;
+    for (int64_t i = 0; i < n; ++i) {
+        PRH_CHECK_ARG(patch_class[i] >= 0 && patch_class[i] <= 2, "prh_batch_set_field: patch class out of range");
+        PRH_CHECK_ARG(patch_class[i] != 2 || (cell_x[i] >= 0 && cell_x[i] < nx),
+                      "prh_batch_set_field: cell x index out of range");
+    }
+    PRH_HIP(hipSetDevice(b->device));
+    hipStream_t st;
+    int rc = current_stream(&st);
+    if (rc) return rc;
+    if ((rc = batch_drain(b))) return rc;
+    if (!b->cls) {
+        PRH_HIP(hipMalloc((void **)&b->cls, n));
+        PRH_HIP(hipMalloc((void **)&b->cell_x, n * 4));
+        b->bytes += n * 5;
+    }
+    PRH_HIP(hipMemcpyAsync(b->cls, patch_class, n, hipMemcpyHostToDevice, st));
+    PRH_HIP(hipMemcpyAsync(b->cell_x, cell_x, n * 4, hipMemcpyHostToDevice, st));
+    std::vector<double> area(n);
+    PRH_HIP(hipMemcpyAsync(area.data(), b->op->area, n * 8, hipMemcpyDeviceToHost, st));
+    PRH_HIP(hipStreamSynchronize(st));
+    double sum = 0.0;
+    for (double a : area) sum += a;
+    b->mean_area = n ? sum / (double)n : 0.0;
+    b->field_nx = nx;
+    b->cross_section = cross_section_m2;
     return PRH_OK;
 }
 
@@ -443,12 +735,7 @@ int prh_batch_admit(prh_batch *b, const double *emitted, const double *emissivit
 {
     PRH_CHECK_ARG(b && emitted && emissivity && slot_out, "prh_batch_admit: null pointer");
     PRH_CHECK_ARG(relative_tolerance > 0.0, "prh_batch_admit: relative_tolerance must be positive");
-    int q = -1;
-    for (int k = 0; k < SLOTS; ++k)
-        if (!b->slot[k].active) {
-            q = k;
-            break;
-        }
+    const int q = first_free(b);
     if (q < 0) {
         set_error("prh_batch_admit: all %d slots are busy", SLOTS);
         return PRH_ERR_CAPACITY;
@@ -466,115 +753,131 @@ int prh_batch_admit(prh_batch *b, const double *emitted, const double *emissivit
     hipStream_t st;
     int rc = current_stream(&st);
     if (rc) return rc;
+    const double s0 = e_norm > 0.0 ? e_norm / (double)n : 1.0;
     PRH_HIP(hipMemcpyAsync(b->stage, emitted, n * 8, hipMemcpyHostToDevice, st));
     PRH_HIP(hipMemcpyAsync(b->stage + n, emissivity, (eps_per_patch ? n : 1) * 8, hipMemcpyHostToDevice, st));
-    k_admit<<<grid_of(n, 256), 256, 0, st>>>(n, q, b->stage, b->stage + n, eps_per_patch, b->E64, b->eps64,
-                                             b->rho32, b->x64, b->r64);
+    k_admit<<<grid_of(n, 256), 256, 0, st>>>(view_of(b), vecs_of(b), z_next(b), q, b->stage, b->stage + n,
+                                             eps_per_patch, 1.0 / s0, b->op->uniform ? 1 : 0);
     PRH_HIP(hipGetLastError());
+    // the caller's arrays and the staging buffer are free again once this returns
     PRH_HIP(hipStreamSynchronize(st));
     SlotHost &S = b->slot[q];
     S = SlotHost();
-    S.active = true;
+    S.state = RUNNING;
     S.rtol = relative_tolerance;
-    S.e_norm = std::max(e_norm, 2.2250738585072014e-308);
     S.rho_max = rho_max;
+    S.e_norm = e_norm;
     S.r_norm = e_norm;
-    S.done = e_norm == 0.0;
+    S.s = s0;
+    S.tau = epoch_target(S.rtol, 1.0, 1.0);
+    S.m = ticks_for(rho_max, S.tau);
+    if (e_norm == 0.0) {
+        // nothing to iterate: the zero solution is exact
+        S.state = DONE;
+        for (int c = 0; c < 4; ++c) b->host->fin[q][c] = 0.0;
+    }
     *slot_out = q;
     return PRH_OK;
 }
 
-int prh_batch_epoch(prh_batch *b, uint32_t *done_mask, uint32_t *active_mask)
+int prh_batch_admit_linear(prh_batch *b, double T_hot_K, double T_cold_K, double emissivity,
+                           double relative_tolerance, int32_t *slot_out)
 {
-    PRH_CHECK_ARG(b, "prh_batch_epoch: null batch");
+    PRH_CHECK_ARG(b && slot_out, "prh_batch_admit_linear: null pointer");
+    PRH_CHECK_ARG(b->cls, "prh_batch_admit_linear: no field set (prh_batch_set_field)");
+    PRH_CHECK_ARG(relative_tolerance > 0.0, "prh_batch_admit_linear: relative_tolerance must be positive");
+    PRH_CHECK_ARG(emissivity > 0.0 && emissivity <= 1.0, "prh_batch_admit_linear: emissivity must lie in (0, 1]");
+    PRH_CHECK_ARG(std::isfinite(T_hot_K) && std::isfinite(T_cold_K),
+                  "prh_batch_admit_linear: temperatures must be finite");
+    const int q = first_free(b);
+    if (q < 0) {
+        set_error("prh_batch_admit_linear: all %d slots are busy", SLOTS);
+        return PRH_ERR_CAPACITY;
+    }
     PRH_HIP(hipSetDevice(b->device));
     hipStream_t st;
     int rc = current_stream(&st);
     if (rc) return rc;
-    const prh_operator *op = b->op;
-    const MatView M = view_of(b);
-    uint32_t mask = 0;
-    int m = 0;
-    double s[SLOTS], inv_s[SLOTS];
-    for (int q = 0; q < SLOTS; ++q) {
-        SlotHost &S = b->slot[q];
-        s[q] = 0.0;
-        inv_s[q] = 0.0;
-        if (!S.active || S.done) continue;
-        if (S.epochs >= MAX_EPOCHS) {
-            S.done = true;
-            S.failed = true;
+    const double T_m = 0.5 * (T_hot_K + T_cold_K);
+    const double dT = T_hot_K - T_cold_K;
+    // scale of the first epoch: the mean |E| of the linear profile is at most
+    // half of its largest value
+    const double Th4 = std::fabs(std::pow(T_hot_K, 4) - std::pow(T_m, 4));
+    const double Tc4 = std::fabs(std::pow(T_cold_K, 4) - std::pow(T_m, 4));
+    double s0 = 0.5 * emissivity * SIGMA * b->mean_area * std::max(Th4, Tc4);
+    if (!(s0 > 0.0) || !std::isfinite(s0)) s0 = 1.0;
+    const int64_t n = b->n;
+    const unsigned grid = grid_of(n, 256);
+    double *part = b->part + (size_t)2 * SLOTS * b->n_blocks - grid;
+    k_admit_linear<<<grid, 256, 0, st>>>(view_of(b), vecs_of(b), field_of(b), z_next(b), q, emissivity,
+                                         emissivity * SIGMA, T_m, dT, 1.0 / s0, b->op->uniform ? 1 : 0, part);
+    k_column_totals<<<1, 256, 0, st>>>(part, (int64_t)grid, b->tot + 2 * SLOTS - 1);
+    PRH_HIP(hipGetLastError());
+    PRH_HIP(hipMemcpyAsync(&b->host->e_norm[q], b->tot + 2 * SLOTS - 1, sizeof(double), hipMemcpyDeviceToHost,
+                           st));
+    SlotHost &S = b->slot[q];
+    S = SlotHost();
+    S.state = RUNNING;
+    S.field = true;
+    S.rtol = relative_tolerance;
+    S.rho_max = 1.0 - emissivity;
+    S.e_norm_step = b->step;  // the next step's completion covers this admission
+    S.s = s0;
+    S.tau = epoch_target(S.rtol, 1.0, 1.0);
+    S.m = ticks_for(S.rho_max, S.tau);
+    *slot_out = q;
+    return PRH_OK;
+}
+
+int prh_batch_step(prh_batch *b, int32_t max_ticks, uint32_t *done_mask, uint32_t *active_mask)
+{
+    PRH_CHECK_ARG(b, "prh_batch_step: null batch");
+    PRH_CHECK_ARG(max_ticks >= 1, "prh_batch_step: max_ticks must be positive");
+    PRH_HIP(hipSetDevice(b->device));
+    hipStream_t st;
+    int rc = current_stream(&st);
+    if (rc) return rc;
+    for (int t = 0; t < max_ticks; ++t) {
+        bool launched = false;
+        if ((rc = batch_step(b, st, &launched))) return rc;
+        if (!launched) {
+            // nothing runs: let the queued finish work complete
+            if ((rc = batch_drain(b))) return rc;
+            bool finishing = false;
+            for (int q = 0; q < SLOTS; ++q) finishing |= b->slot[q].state == FINISHING;
+            if (!finishing) break;
             continue;
         }
-        mask |= 1u << q;
-        S.epoch_ticks = ticks_for(S.rho_max, epoch_target(S));
-        m = std::max(m, S.epoch_ticks);
-        const double scale = S.r_norm / (double)b->n;
-        s[q] = scale > 0.0 ? scale : 1.0;
-        inv_s[q] = 1.0 / s[q];
+        uint32_t done = 0;
+        report(b, &done, nullptr);
+        if (done) break;
     }
-    if (mask) {
-        // Per-tick coefficients of every running slot.
-        std::vector<float> coef((size_t)m * SLOTS * 2, 0.f), al, be;
-        for (int q = 0; q < SLOTS; ++q) {
-            if (!((mask >> q) & 1u)) continue;
-            const int mq = b->slot[q].epoch_ticks;
-            cheb_coefficients(b->slot[q].rho_max, mq, al, be);
-            // The direction after the slot's last own tick is zero, so longer
-            // slots in the same epoch leave this slot untouched.
-            for (int k = 0; k + 1 < mq; ++k) {
-                coef[((size_t)k * SLOTS + q) * 2] = al[k];
-                coef[((size_t)k * SLOTS + q) * 2 + 1] = be[k];
-            }
-        }
-        PRH_HIP(hipMemcpyAsync(b->coef, coef.data(), coef.size() * 4, hipMemcpyHostToDevice, st));
-        PRH_HIP(hipMemcpyAsync(b->svec, s, SLOTS * 8, hipMemcpyHostToDevice, st));
-        PRH_HIP(hipMemcpyAsync(b->svec + SLOTS, inv_s, SLOTS * 8, hipMemcpyHostToDevice, st));
-        const int64_t ns = b->n * SLOTS;
-        k_epoch_begin<<<grid_of(ns, 256), 256, 0, st>>>(M, mask, b->svec + SLOTS, b->r64, b->x32, b->r32, b->dir32,
-                                                        b->za, op->uniform ? 1 : 0);
-        float *zin = b->za, *zout = b->zb;
-        const unsigned grid = (unsigned)b->n_blocks;
-        for (int k = 0; k < m; ++k) {
-            TickArgs a{M, zin, zout, b->x32, b->r32, b->dir32, b->rho32, b->coef + (size_t)k * SLOTS * 2};
-            if (op->uniform) k_cheb_tick<true><<<grid, BT, 0, st>>>(a);
-            else k_cheb_tick<false><<<grid, BT, 0, st>>>(a);
-            std::swap(zin, zout);
-        }
-        k_epoch_accumulate<<<grid_of(ns, 256), 256, 0, st>>>(M, mask, b->svec, b->x32, b->x64, b->z64,
-                                                             op->uniform ? 1 : 0);
-        ResArgs r{M, b->z64, b->x64, b->E64, b->eps64, b->y64, b->r64, b->part_abs, b->part_sum};
-        if (op->uniform) k_residual64<true><<<grid, BT, 0, st>>>(r);
-        else k_residual64<false><<<grid, BT, 0, st>>>(r);
-        k_slot_totals<<<SLOTS, 256, 0, st>>>(b->part_abs, b->part_sum, b->n_blocks, b->tot, b->tot + SLOTS);
-        PRH_HIP(hipGetLastError());
-        double tot[2 * SLOTS];
-        PRH_HIP(hipMemcpyAsync(tot, b->tot, sizeof(tot), hipMemcpyDeviceToHost, st));
-        PRH_HIP(hipStreamSynchronize(st));
-        for (int q = 0; q < SLOTS; ++q) {
-            if (!((mask >> q) & 1u)) continue;
-            SlotHost &S = b->slot[q];
-            S.epochs += 1;
-            S.ticks += S.epoch_ticks;
-            S.r_norm = tot[q];
-            S.r_sum = tot[SLOTS + q];
-            if (S.r_norm <= S.rtol * S.e_norm) S.done = true;
-        }
+    report(b, done_mask, active_mask);
+    return PRH_OK;
+}
+
+int prh_batch_channels(prh_batch *b, int32_t slot, double *q_W_m2, double *residual, int32_t *iterations)
+{
+    PRH_CHECK_ARG(b && q_W_m2 && residual && iterations, "prh_batch_channels: null pointer");
+    PRH_CHECK_ARG(slot >= 0 && slot < SLOTS && b->slot[slot].state == DONE,
+                  "prh_batch_channels: slot %d has no finished solve", slot);
+    PRH_CHECK_ARG(b->cls, "prh_batch_channels: no field set (prh_batch_set_field)");
+    SlotHost &S = b->slot[slot];
+    if (S.failed) {
+        set_error("radiosity reflection series did not converge");
+        return PRH_ERR_SERIES;
     }
-    uint32_t done = 0, active = 0;
-    for (int q = 0; q < SLOTS; ++q) {
-        if (b->slot[q].active) active |= 1u << q;
-        if (b->slot[q].active && b->slot[q].done) done |= 1u << q;
-    }
-    if (done_mask) *done_mask = done;
-    if (active_mask) *active_mask = active;
+    const double *f = b->host->fin[slot];
+    for (int c = 0; c < 3; ++c) q_W_m2[c] = -f[c] / b->cross_section;
+    *residual = f[3];
+    *iterations = S.ticks + S.epochs;
     return PRH_OK;
 }
 
 int prh_batch_result(prh_batch *b, int32_t slot, double *absorbed, double *residual, int32_t *iterations)
 {
     PRH_CHECK_ARG(b && absorbed && residual && iterations, "prh_batch_result: null pointer");
-    PRH_CHECK_ARG(slot >= 0 && slot < SLOTS && b->slot[slot].active && b->slot[slot].done,
+    PRH_CHECK_ARG(slot >= 0 && slot < SLOTS && b->slot[slot].state == DONE,
                   "prh_batch_result: slot %d has no finished solve", slot);
     SlotHost &S = b->slot[slot];
     if (S.failed) {
@@ -586,18 +889,22 @@ int prh_batch_result(prh_batch *b, int32_t slot, double *absorbed, double *resid
     hipStream_t st;
     int rc = current_stream(&st);
     if (rc) return rc;
-    if (S.epochs == 0) {
-        // zero emission: nothing was iterated
-        std::fill(absorbed, absorbed + b->n, 0.0);
-    } else {
-        k_collect<<<grid_of(b->n, 256), 256, 0, st>>>(b->n, slot, b->eps64, b->y64, b->stage);
-        PRH_HIP(hipGetLastError());
-        PRH_HIP(hipMemcpyAsync(absorbed, b->stage, b->n * 8, hipMemcpyDeviceToHost, st));
-        PRH_HIP(hipStreamSynchronize(st));
-    }
-    *residual = S.r_sum;
+    k_collect<<<grid_of(b->n, 256), 256, 0, st>>>(b->n, slot, b->eps64, b->y64, b->stage);
+    PRH_HIP(hipGetLastError());
+    PRH_HIP(hipMemcpyAsync(absorbed, b->stage, b->n * 8, hipMemcpyDeviceToHost, st));
+    PRH_HIP(hipStreamSynchronize(st));
+    *residual = b->host->fin[slot][3];
     *iterations = S.ticks + S.epochs;
     S = SlotHost();
+    return PRH_OK;
+}
+
+int prh_batch_release(prh_batch *b, int32_t slot)
+{
+    PRH_CHECK_ARG(b, "prh_batch_release: null batch");
+    PRH_CHECK_ARG(slot >= 0 && slot < SLOTS && b->slot[slot].state == DONE,
+                  "prh_batch_release: slot %d has no finished solve", slot);
+    b->slot[slot] = SlotHost();
     return PRH_OK;
 }
 
@@ -613,10 +920,8 @@ int prh_radiosity_mixed(prh_operator *op, int32_t n_rhs, const double *emitted, 
     }
     prh_batch *b = op->batch;
     const int64_t n = op->n;
-    int slot_of_rhs[SLOTS];
     int rhs_of_slot[SLOTS];
     for (int q = 0; q < SLOTS; ++q) rhs_of_slot[q] = -1;
-    (void)slot_of_rhs;
     int next = 0, finished = 0, status = PRH_OK;
     while (finished < n_rhs) {
         uint32_t free_mask = 0;
@@ -630,7 +935,7 @@ int prh_radiosity_mixed(prh_operator *op, int32_t n_rhs, const double *emitted, 
             free_mask &= ~(1u << q);
         }
         uint32_t done = 0, active = 0;
-        if ((rc = prh_batch_epoch(b, &done, &active))) return rc;
+        if ((rc = prh_batch_step(b, MAX_TICKS * MAX_EPOCHS, &done, &active))) return rc;
         for (int q = 0; q < SLOTS; ++q) {
             if (!((done >> q) & 1u)) continue;
             const int k = rhs_of_slot[q];
@@ -649,6 +954,8 @@ void prh_batch_destroy(prh_batch *b)
 {
     if (!b) return;
     (void)hipSetDevice(b->device);
+    hipStream_t st;
+    if (current_stream(&st) == PRH_OK) (void)hipStreamSynchronize(st);
     free_batch(b);
     delete b;
 }

@@ -63,7 +63,11 @@ SIGNATURES = {
     "prh_batch_bytes": ([_P, _P], ctypes.c_int),
     "prh_batch_free_slots": ([_P, _P], ctypes.c_int),
     "prh_batch_admit": ([_P, _P, _P, _I32, _F64, _P], ctypes.c_int),
-    "prh_batch_epoch": ([_P, _P, _P], ctypes.c_int),
+    "prh_batch_set_field": ([_P, _P, _P, _I32, _F64], ctypes.c_int),
+    "prh_batch_admit_linear": ([_P, _F64, _F64, _F64, _F64, _P], ctypes.c_int),
+    "prh_batch_step": ([_P, _I32, _P, _P], ctypes.c_int),
+    "prh_batch_channels": ([_P, _I32, _P, _P, _P], ctypes.c_int),
+    "prh_batch_release": ([_P, _I32], ctypes.c_int),
     "prh_batch_result": ([_P, _I32, _P, _P, _P], ctypes.c_int),
     "prh_batch_destroy": ([_P], None),
 }
@@ -338,10 +342,46 @@ class Batch:
                                         ctypes.byref(slot)), "prh_batch_admit")
         return int(slot.value)
 
-    def epoch(self) -> tuple[int, int]:
+    def set_field(self, patch_class: np.ndarray, cell_x: np.ndarray, nx: int, cross_section_m2: float) -> None:
+        cls = np.ascontiguousarray(patch_class, dtype=np.int8)
+        cx = np.ascontiguousarray(cell_x, dtype=np.int32)
+        if cls.shape != (self.n,) or cx.shape != (self.n,):
+            raise ValueError("patch_class and cell_x must have one value per patch")
+        check(self._lib.prh_batch_set_field(self.handle, ptr(cls), ptr(cx), int(nx), float(cross_section_m2)),
+              "prh_batch_set_field")
+
+    def admit_linear(self, T_hot_K: float, T_cold_K: float, emissivity: float, relative_tolerance: float) -> int:
+        slot = ctypes.c_int32()
+        check(self._lib.prh_batch_admit_linear(self.handle, float(T_hot_K), float(T_cold_K), float(emissivity),
+                                               float(relative_tolerance), ctypes.byref(slot)),
+              "prh_batch_admit_linear")
+        return int(slot.value)
+
+    def step(self, max_ticks: int = 1) -> tuple[int, int]:
+        """Run up to max_ticks sweeps (returns early when a slot finishes): (done mask, active mask)."""
+
         done, active = ctypes.c_uint32(), ctypes.c_uint32()
-        check(self._lib.prh_batch_epoch(self.handle, ctypes.byref(done), ctypes.byref(active)), "prh_batch_epoch")
+        check(self._lib.prh_batch_step(self.handle, int(max_ticks), ctypes.byref(done), ctypes.byref(active)),
+              "prh_batch_step")
         return int(done.value), int(active.value)
+
+    def run_until_done(self) -> int:
+        """Step until at least one slot has finished (or nothing runs); the done mask."""
+
+        while True:
+            done, active = self.step(64)
+            if done or not active:
+                return done
+
+    def channels(self, slot: int) -> tuple[float, float, float, float, int]:
+        """(q_h, q_c, q_i, residual power, sweeps) of a finished linear-field slot; keeps the slot."""
+
+        q = np.empty(3, np.float64)
+        residual = ctypes.c_double()
+        iterations = ctypes.c_int32()
+        check(self._lib.prh_batch_channels(self.handle, int(slot), ptr(q), ctypes.byref(residual),
+                                           ctypes.byref(iterations)), "prh_batch_channels")
+        return float(q[0]), float(q[1]), float(q[2]), float(residual.value), int(iterations.value)
 
     def result(self, slot: int):
         absorbed = np.empty(self.n, np.float64)
@@ -350,6 +390,9 @@ class Batch:
         check(self._lib.prh_batch_result(self.handle, int(slot), ptr(absorbed), ctypes.byref(residual),
                                          ctypes.byref(iterations)), "prh_batch_result")
         return absorbed, float(residual.value), int(iterations.value)
+
+    def release(self, slot: int) -> None:
+        check(self._lib.prh_batch_release(self.handle, int(slot)), "prh_batch_release")
 
     def close(self) -> None:
         if getattr(self, "handle", None) is not None and self.handle.value:

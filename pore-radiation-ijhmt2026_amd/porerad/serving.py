@@ -5,16 +5,20 @@ A request names a resident geometry (its ``SurfacePatches`` and
 for the radiation channels of the prescribed linear field between the walls
 (porerad/coupled.py:170-194), solved to a relative tolerance with the
 batched mixed-precision engine (``prh_batch_*``: FP32 Chebyshev sweeps over
-16 slots per operator, FP64 residual refinement).
+16 slots per operator, FP64 refinement).  The emission is formed and the
+channels are summed on the device, so a request moves a few scalars over
+PCIe; the patch powers come back only when a request asks for them.
 
 Scheduling.  Every geometry has one engine with 16 slots.  Each scheduler
 step (1) admits queued requests, oldest first, into free slots of their
 geometry's engine, creating the engine first if its device memory fits the
-HBM budget (idle engines are released to make room); (2) runs one epoch on
-every engine that has running slots; (3) retires finished slots and replies.
-Requests that arrive while others are on the GPU join at the next epoch
-boundary, so their work overlaps on the device with the requests already
-running; a solve's bits do not depend on what else is in its batch.
+HBM budget (idle engines are released to make room); (2) advances every
+engine that has running slots by one sweep (the engines queue their sweeps
+on one stream, each waiting only for its previous sweep, so the device
+always has work queued); (3) retires finished slots and replies.  Requests
+that arrive while others are on the GPU join at the next sweep, so their
+work overlaps on the device with the requests already running; a solve's
+bits do not depend on what else is in its batch.
 
 The engine type is injectable (``engine_factory``) so that the admission and
 retirement logic can be exercised without a GPU.
@@ -46,6 +50,7 @@ class RadiosityRequest:
     relative_tolerance: float = 1.0e-9
     arrival_s: float = 0.0
     tag: object = None
+    patch_power: bool = False  # also return the net power of every patch
 
 
 @dataclasses.dataclass
@@ -54,7 +59,7 @@ class RadiosityReply:
     q_h: float
     q_c: float
     q_i: float
-    net_power_W: np.ndarray
+    net_power_W: np.ndarray | None
     residual_power_W: float
     sweeps: int
     admitted_s: float
@@ -78,10 +83,12 @@ class Geometry:
         self.hot = cls == HOT
         self.cold = cls == COLD
         self.internal = cls == INTERNAL
-        # i + 1/2 of the internal patches' owning cells along x
+        # x index of the internal patches' owning cells, and i + 1/2
         self.nx = nx
+        self.cell_x = np.zeros(self.n, np.int32)
+        self.cell_x[self.internal] = patches.patch_cell[self.internal] // (ny * nz)
         self.half_index = np.zeros(self.n)
-        self.half_index[self.internal] = (patches.patch_cell[self.internal] // (ny * nz)).astype(np.float64) + 0.5
+        self.half_index[self.internal] = self.cell_x[self.internal].astype(np.float64) + 0.5
         self.area = patches.patch_area_m2
         self.cross_section = float(ny * nz) * (patches.H_m / float(nx)) ** 2
 
@@ -110,15 +117,17 @@ class Geometry:
 def device_engine(geometry: Geometry):
     from . import _native
 
-    return _native.Batch(geometry.operator.device)
+    engine = _native.Batch(geometry.operator.device)
+    engine.set_field(geometry.patches.patch_class, geometry.cell_x, geometry.nx, geometry.cross_section)
+    return engine
 
 
 def device_engine_bytes(geometry: Geometry) -> int:
-    """Device bytes of one engine: 16 slots x (6 FP32 + 6 FP64 vectors) per patch, plus partial sums."""
+    """Device bytes of one engine: 16 slots x (6 FP32 + 5 FP64 vectors) per patch, partial sums, field."""
 
     n = geometry.n
     blocks = ((n + 3) // 4 + 7) // 8 * 8
-    return n * SLOTS * (6 * 4 + 6 * 8) + blocks * SLOTS * 16 + n * 16 + 4096
+    return n * SLOTS * (6 * 4 + 5 * 8) + max(2 * SLOTS * blocks, 4 * ((n + 255) // 256)) * 8 + n * 21 + 8192
 
 
 class RadiosityServer:
@@ -134,10 +143,10 @@ class RadiosityServer:
         self.clock = clock
         self.geometries: dict[str, Geometry] = {}
         self.engines: dict[str, object] = {}
-        self.slots: dict[str, dict[int, tuple[RadiosityRequest, np.ndarray, float]]] = {}
+        self.slots: dict[str, dict[int, tuple[RadiosityRequest, float]]] = {}
         self.queue: collections.deque[RadiosityRequest] = collections.deque()
         self.replies: list[RadiosityReply] = []
-        self.epochs = 0
+        self.sweeps = 0
         self.evictions = 0
 
     # -- registration ------------------------------------------------------------
@@ -183,10 +192,8 @@ class RadiosityServer:
             if engine is None:
                 waiting.append(req)
                 continue
-            g = self.geometries[req.key]
-            E = g.emission(req)
-            slot = engine.admit(E, req.epsilon, req.relative_tolerance)
-            busy[slot] = (req, E, self.clock())
+            slot = engine.admit_linear(req.T_hot_K, req.T_cold_K, req.epsilon, req.relative_tolerance)
+            busy[slot] = (req, self.clock())
         self.queue = waiting
 
     # -- one scheduler step ----------------------------------------------------------
@@ -197,16 +204,20 @@ class RadiosityServer:
             busy = self.slots[key]
             if not busy:
                 continue
-            done, _ = engine.epoch()
-            self.epochs += 1
+            done, _ = engine.step(1)
+            self.sweeps += 1
             g = self.geometries[key]
             for slot in list(busy):
                 if not (done >> slot) & 1:
                     continue
-                req, E, admitted = busy.pop(slot)
-                absorbed, residual, sweeps = engine.result(slot)
-                net = absorbed - E
-                q_h, q_c, q_i = g.channels(net)
+                req, admitted = busy.pop(slot)
+                q_h, q_c, q_i, residual, sweeps = engine.channels(slot)
+                net = None
+                if req.patch_power:
+                    absorbed, _, _ = engine.result(slot)
+                    net = absorbed - g.emission(req)
+                else:
+                    engine.release(slot)
                 done_now.append(RadiosityReply(req, q_h, q_c, q_i, net, residual, sweeps, admitted, self.clock()))
         self.replies.extend(done_now)
         return done_now

@@ -80,11 +80,12 @@ def test_alone_equals_in_mix_bitwise(mods):
     try:
         s0 = batch.admit(E[0], eps[0], 1e-9)
         s1 = batch.admit(E[1], eps[1], 1e-9)
-        batch.epoch()
+        batch.step(5)
         s2 = batch.admit(E[2], eps[2], 1e-9)
         results = {}
         while len(results) < 3:
-            done, _ = batch.epoch()
+            done, active = batch.step(64)
+            assert active
             for s in (s0, s1, s2):
                 if (done >> s) & 1 and s not in results:
                     results[s] = batch.result(s)
@@ -134,6 +135,42 @@ def test_channels_of_archived_case_within_1e8(mods):
             assert abs(a - b) <= 1e-8 * scale
 
 
+def test_linear_field_admission_matches_host_emission(mods):
+    """Channels of a device-formed linear field agree with the same solve of the host-formed emission."""
+
+    _native, vf, ex, cp = mods
+    from porerad import serving
+
+    rng = np.random.default_rng(12)
+    phases = (rng.random((18, 12, 12)) < 0.15).astype(np.uint8)
+    P = vf.build_surface_patches(phases, 0.004)
+    o, r, c = vf.first_hit_counts_pooled(P, 96, [21])
+    op = ex.ExchangeOperator(o, r, c, np.full(P.n_patches, 96, np.int64), P.patch_area_m2)
+    g = serving.Geometry("g", P, op)
+    batch = _native.Batch(op.device)
+    try:
+        batch.set_field(P.patch_class, g.cell_x, g.nx, g.cross_section)
+        for Th, eps in ((900.0, 0.3), (500.0, 0.9), (300.0, 0.6)):
+            req = serving.RadiosityRequest("g", Th, 300.0, eps)
+            a = batch.admit_linear(Th, 300.0, eps, 1e-10)
+            assert batch.run_until_done() >> a & 1
+            q_dev = batch.channels(a)
+            absorbed, _, _ = batch.result(a)
+            E = g.emission(req)
+            b = batch.admit(E, eps, 1e-10)
+            assert batch.run_until_done() >> b & 1
+            absorbed_host, _, _ = batch.result(b)
+            scale = np.max(np.abs(E)) if Th != 300.0 else 1.0
+            assert np.max(np.abs(absorbed - absorbed_host)) <= 1e-9 * scale
+            q_host = g.channels(absorbed - E)
+            for x, y in zip(q_dev[:3], q_host):
+                assert abs(x - y) <= 1e-9 * max(abs(q_host[0]), abs(q_host[1]), 1e-300)
+            if Th == 300.0:
+                assert q_dev[:3] == (0.0, 0.0, 0.0) and not np.any(absorbed)
+    finally:
+        batch.close()
+
+
 def test_admission_errors(mods):
     _native, _, _, _ = mods
     Q, op, _ = _pair(10, shape=(8, 6, 6))
@@ -145,5 +182,9 @@ def test_admission_errors(mods):
             batch.admit(np.ones(Q.n_patches), 0.5, 1e-9)
         with pytest.raises(ValueError, match="slots are busy"):
             batch.admit(np.ones(Q.n_patches), 0.5, 1e-9)
+        with pytest.raises(ValueError, match="no field"):
+            batch.admit_linear(700.0, 300.0, 0.5, 1e-9)
+        with pytest.raises(ValueError, match="no finished solve"):
+            batch.release(0)
     finally:
         batch.close()

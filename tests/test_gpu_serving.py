@@ -60,11 +60,12 @@ def test_alone_equals_mixed_stream(world):
     alone = {}
     for i, (k, Th, eps) in enumerate(specs[:6]):
         srv = _server(serving, geoms)
-        alone[i] = srv.run([serving.RadiosityRequest(k, Th, 300.0, eps, tag=i)], real_time=False)[0]
+        alone[i] = srv.run([serving.RadiosityRequest(k, Th, 300.0, eps, tag=i, patch_power=True)],
+                           real_time=False)[0]
         srv.close()
     srv = _server(serving, geoms)
     arrivals = serving.poisson_arrivals(2000.0, len(specs), 7)
-    reqs = [serving.RadiosityRequest(k, Th, 300.0, eps, arrival_s=float(t), tag=i)
+    reqs = [serving.RadiosityRequest(k, Th, 300.0, eps, arrival_s=float(t), tag=i, patch_power=i < 6)
             for i, ((k, Th, eps), t) in enumerate(zip(specs, arrivals))]
     replies = srv.run(reqs, real_time=True)
     srv.close()
@@ -72,6 +73,8 @@ def test_alone_equals_mixed_stream(world):
     by_tag = {r.request.tag: r for r in replies}
     for i, rep in alone.items():
         assert np.array_equal(by_tag[i].net_power_W, rep.net_power_W)
+        assert (by_tag[i].q_h, by_tag[i].q_c, by_tag[i].q_i) == (rep.q_h, rep.q_c, rep.q_i)
         assert by_tag[i].sweeps == rep.sweeps
+    assert all(r.net_power_W is None for r in replies if r.request.tag >= 6)
     lat = serving.latency_summary(replies, 1.0)
     assert lat["p99_ms"] >= lat["p50_ms"] > 0

@@ -1,7 +1,8 @@
 """Scheduling logic of the radiosity server, on the CPU with a stand-in engine.
 
-The stand-in keeps the engine contract of prh_batch (16 slots, admit / epoch
-/ result, device bytes) and finishes a slot after a fixed number of epochs,
+The stand-in keeps the engine contract of prh_batch (16 slots,
+admit_linear / step / channels / result / release, device bytes) and
+finishes a slot after a fixed number of sweeps, with absorbed power = E / 4,
 so admission order, continuous batching and HBM-budget eviction can be
 checked exactly.  The solver numerics are covered by the GPU tests.
 """
@@ -16,6 +17,7 @@ from porerad import serving
 
 class StandInEngine:
     def __init__(self, geometry, epochs_needed=2, nbytes=100):
+        self.geometry = geometry
         self.n = geometry.n
         self.bytes = nbytes
         self.busy = {}
@@ -23,16 +25,17 @@ class StandInEngine:
         self.closed = False
         self.log = []
 
-    def admit(self, E, eps, rtol):
+    def admit_linear(self, T_hot, T_cold, eps, rtol):
         free = [q for q in range(serving.SLOTS) if q not in self.busy]
         if not free:
             raise ValueError("all slots are busy")
         q = free[0]
-        self.busy[q] = [E.copy(), 0]
+        E = self.geometry.emission(serving.RadiosityRequest(self.geometry.key, T_hot, T_cold, eps))
+        self.busy[q] = [E, 0]
         self.log.append(("admit", q))
         return q
 
-    def epoch(self):
+    def step(self, max_ticks=1):
         done = 0
         for q, rec in self.busy.items():
             rec[1] += 1
@@ -40,9 +43,18 @@ class StandInEngine:
                 done |= 1 << q
         return done, sum(1 << q for q in self.busy)
 
+    def channels(self, q):
+        E, ep = self.busy[q]
+        net = 0.25 * E - E
+        q_h, q_c, q_i = self.geometry.channels(net)
+        return q_h, q_c, q_i, 0.0, ep
+
     def result(self, q):
         E, ep = self.busy.pop(q)
         return 0.25 * E, 0.0, ep
+
+    def release(self, q):
+        self.busy.pop(q)
 
     def close(self):
         self.closed = True
@@ -69,12 +81,14 @@ def test_requests_finish_and_channels_follow_net_power():
     P = _patches()
     srv = serving.RadiosityServer(engine_factory=lambda g: StandInEngine(g), clock=Clock())
     g = srv.add_geometry("a", P, operator=None)
-    reqs = [serving.RadiosityRequest("a", 700.0, 300.0, 0.6, arrival_s=0.0) for _ in range(5)]
+    reqs = [serving.RadiosityRequest("a", 700.0, 300.0, 0.6, arrival_s=0.0, patch_power=(i == 0)) for i in range(5)]
     replies = srv.run(reqs, real_time=False)
     assert len(replies) == 5 and srv.pending() == 0
     E = g.emission(reqs[0])
     net = 0.25 * E - E
     assert replies[0].q_h == pytest.approx(-np.sum(net[P.patch_class == 0]) / g.cross_section)
+    assert np.array_equal(replies[0].net_power_W, net)
+    assert all(r.net_power_W is None for r in replies[1:])
     assert all(r.sweeps == 2 for r in replies)
 
 
