@@ -230,7 +230,7 @@ def main() -> None:
         from porerad import autotune
 
         tune = autotune.tune_tracer(P, R, [seed], use_cache=False)
-        log("tracer variants: " + ", ".join(f"v{r['variant']}(refill {r['refill_min']}, coarse {r['coarse']})="
+        log("tracer variants: " + ", ".join(f"v{r['variant']}(refill {r['refill_min']}, occupancy {r['occupancy']})="
                                             f"{r['trace_ms']:.2f}ms" for r in tune["table"]))
     wall, parts, reads, nnz = run_tracer(P, R, seed, args.steps, args.warmup)
     step = float(np.mean(wall))

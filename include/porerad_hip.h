@@ -76,13 +76,13 @@ int prh_last_timings(double *ms, int32_t n);
 
 /* Tuning knob of the traversal kernel: variant index in [0, n_variants).
  * Each variant fixes the refill threshold (idle lanes of a wavefront before
- * it loads new rays) and whether occupancy tests go through the LDS brick
- * bitset first (coarse = 1) or straight to the global bitset.  No
+ * it loads new rays) and the occupancy test: 0 the global bitset, 1 the LDS
+ * brick bitset first, 2 the global bitset word loaded ahead of the step.  No
  * counterpart in the reference (a launch parameter of this implementation);
  * results do not depend on it.  prh_tracer_variants fills the tables when
  * the pointers are non-NULL (arrays of n_variants entries). */
 int prh_tracer_set_variant(int32_t variant);
-int prh_tracer_variants(int32_t *n_variants, int32_t *current, int32_t *refill_min, int32_t *coarse);
+int prh_tracer_variants(int32_t *n_variants, int32_t *current, int32_t *refill_min, int32_t *occupancy_mode);
 
 /* numpy.random.PCG64(seed) seeding: SeedSequence over the little-endian
  * uint32 words of the seed.  state_inc = {state_hi, state_lo, inc_hi, inc_lo}.

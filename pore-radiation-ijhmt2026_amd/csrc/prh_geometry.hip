@@ -283,6 +283,10 @@ int prh_geometry_create(const uint8_t *solid_x, int32_t nx, int32_t ny, int32_t 
     PRH_CHECK_ARG(out && solid_x, "prh_geometry_create: null pointer");
     PRH_CHECK_ARG(nx > 0 && ny > 0 && nz > 0, "prh_geometry_create: grid sizes must be positive");
     PRH_CHECK_ARG((int64_t)nx * ny * nz < (int64_t(1) << 31), "prh_geometry_create: grid too large");
+    // the traversal forms cell indices with 24-bit multiplies ((i ny + j) <
+    // 2^24) and packs i + 1, j, k into 16-bit fields of its ray records
+    PRH_CHECK_ARG((int64_t)nx * ny < (int64_t(1) << 24) && nx < 65535 && ny < 65536 && nz < 65536,
+                  "prh_geometry_create: grid too large");
     *out = nullptr;
     prh_geometry *geo = new prh_geometry();  // value-initialised: all pointers null
     int rc = prh::build(solid_x, nx, ny, nz, H_m, geo->g);

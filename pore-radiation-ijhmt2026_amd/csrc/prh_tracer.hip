@@ -8,16 +8,18 @@
 // bit-identical to the reference CPU implementation on the same seeds.
 //
 // Pipeline of one prh_trace_counts call (all on the library stream):
-//   k_gen_rays       seeded Lambertian rays -> FP64 origin/direction records
-//                    in HBM (48 B per ray; fully converged waves, each thread
-//                    jumps its PCG64 stream once and then only draws);
+//   k_gen_rays       seeded Lambertian rays and their DDA set-up (the FP64
+//                    divisions of _axis_state) -> a 64 B start record and a
+//                    32 B wall record per ray in HBM; fully converged waves,
+//                    each thread jumps its PCG64 stream once and then only
+//                    draws;
 //   k_trace_stream   FP64 DDA over the bit-packed occupancy grid; each wave
 //                    streams through its chunk of rays and refills idle lanes
 //                    in batches, so lanes do not wait for the longest ray.
-//                    Generation is kept out of this kernel: its temporaries
-//                    would otherwise stay allocated next to the traversal
-//                    state of the busy lanes and halve the occupancy;
-//   k_count_patches  per-patch LDS sort of the receivers -> unique counts;
+//                    A refill is four coalesced loads and a finished ray
+//                    writes an 8 B final state (cell and face, or wall cell);
+//   k_count_patches  per patch: receivers resolved from the final states
+//                    (cell code or wall maps), LDS sort -> unique counts;
 //   k_compact_edges  prefix-summed edge table (origin, receiver, count).
 #include <climits>
 
@@ -47,13 +49,36 @@ struct GeoView {
     int nx, ny, nz;
     int coarse_shift, cny, cnz;
     uint32_t coarse_words;
+    double inv_ny, inv_nz;
 };
 
 GeoView view_of(const Geometry &g)
 {
-    return GeoView{g.occ, g.coarse, g.cell_code, g.hot_map, g.cold_map, g.patch_origin, g.patch_face,
-                   g.patch_class, g.nx, g.ny, g.nz, g.coarse_shift, g.cny, g.cnz, (uint32_t)g.coarse_words};
+    return GeoView{g.occ,    g.coarse, g.cell_code, g.hot_map, g.cold_map, g.patch_origin, g.patch_face,
+                   g.patch_class, g.nx, g.ny,     g.nz,      g.coarse_shift, g.cny,    g.cnz,
+                   (uint32_t)g.coarse_words,     1.0 / (double)g.ny,   1.0 / (double)g.nz};
 }
+
+// DDA start state of one ray (the set-up of _trace_kernel,
+// view_factors.py:211-224), written by the generation or set-up kernel and
+// read once by the traversal: 64 B, so a refill is four 16 B loads.
+struct alignas(16) DdaRec {
+    double tx, ty, tz, ddx, ddy, ddz;
+    uint32_t ij;   // (i + 1) | jj << 16
+    uint32_t ks;   // kk | (sx + 1) << 16 | (sy + 1) << 18 | (sz + 1) << 20
+    uint32_t cap;  // step cap; 0 for a ray with dx == 0 (it fails)
+    uint32_t pad;
+};
+
+// What a wall receiver needs (view_factors.py:226-231), read only for rays
+// that reach a wall: 32 B.
+struct alignas(16) WallRec {
+    double oy, oz, dy, dz;
+};
+
+// Final state of a traversed ray, 8 B: x = cell index (solid) or the wall
+// cell's j (walls); y = kind << 28 | face (solid) or the wall cell's k.
+enum : uint32_t { FIN_FAILED = 0, FIN_SOLID = 1, FIN_HOT = 2, FIN_COLD = 3 };
 
 // Per-lane traversal state (the loop variables of _trace_kernel).  The face
 // of the last crossing is kept as its axis (priority x > y > z) and turned
@@ -62,7 +87,6 @@ GeoView view_of(const Geometry &g)
 // cap, i outside [0, nx) the wall, anything else a solid cell.  The number
 // of occupancy reads follows from the steps taken.
 struct Ray {
-    double oy, oz, dy, dz;
     double tx, ty, tz, ddx, ddy, ddz, time;
     int i, jj, kk;
     int sx, sy, sz;
@@ -87,6 +111,25 @@ __device__ __forceinline__ double vmin_f64(double a, double b)
     return r;
 }
 
+// a[23:0] * b[23:0] + c as one v_mad_u32_u24 (b uniform); written out because the
+// compiler folds 24-bit multiply + add into v_mad_u64_u32, a quarter-rate
+// instruction.
+__device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+    return r;
+}
+
+// a * b + c rounded once, as one v_fma_f64 (the compiler would turn an fma
+// with a 0/1 factor back into an addition and two selects).
+__device__ __forceinline__ double fma_f64(double a, double b, double c)
+{
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 // Launch constants of the traversal.
 struct TraversalConst {
     uint32_t min_cap;    // 10 nx ny nz + 10 (view_factors.py:224)
@@ -106,81 +149,154 @@ TraversalConst traversal_const(const Geometry &g)
     return c;
 }
 
-// view_factors.py:211-224.  Returns false for dx == 0 (failed ray).
-__device__ __forceinline__ bool ray_setup(Ray &R, const double o[3], const double d[3], const GeoView &G,
-                                          const TraversalConst &C)
+// The DDA set-up of one ray from its origin and direction
+// (view_factors.py:211-224): initial cell (_initial_cell), per-axis step,
+// crossing time and increment (_axis_state), the periodic wrap of the
+// lateral indices and the step cap.  A ray with dx == 0 gets cap 0.
+__device__ __forceinline__ void make_dda(const double o[3], const double d[3], const GeoView &G,
+                                         const TraversalConst &C, DdaRec &rec, WallRec &w)
 {
-    const double ox = o[0], dx = d[0];
-    R.oy = o[1];
-    R.oz = o[2];
-    R.dy = d[1];
-    R.dz = d[2];
-    if (dx == 0.0) return false;
-    R.i = initial_cell(ox, dx);
-    const int j = initial_cell(R.oy, R.dy);
-    const int k = initial_cell(R.oz, R.dz);
-    axis_state(ox, dx, R.i, R.sx, R.tx, R.ddx);
-    axis_state(R.oy, R.dy, j, R.sy, R.ty, R.ddy);
-    axis_state(R.oz, R.dz, k, R.sz, R.tz, R.ddz);
-    // Origins lie in [0, n] (wrapped by float modulo), so j, k are in [-1, n].
-    R.jj = j < 0 ? j + G.ny : (j >= G.ny ? j - G.ny : j);
-    R.kk = k < 0 ? k + G.nz : (k >= G.nz ? k - G.nz : k);
+    w.oy = o[1];
+    w.oz = o[2];
+    w.dy = d[1];
+    w.dz = d[2];
+    if (d[0] == 0.0) {
+        rec.tx = rec.ty = rec.tz = rec.ddx = rec.ddy = rec.ddz = 0.0;
+        rec.ij = rec.ks = rec.cap = rec.pad = 0u;
+        return;
+    }
+    const int i = initial_cell(o[0], d[0]);
+    const int j = initial_cell(o[1], d[1]);
+    const int k = initial_cell(o[2], d[2]);
+    int sx, sy, sz;
+    axis_state(o[0], d[0], i, sx, rec.tx, rec.ddx);
+    axis_state(o[1], d[1], j, sy, rec.ty, rec.ddy);
+    axis_state(o[2], d[2], k, sz, rec.tz, rec.ddz);
+    // Origins lie in [0, n] (wrapped by float modulo), so j, k are in [-1, n]
+    // and i in [-1, nx].
+    const int jj = j < 0 ? j + G.ny : (j >= G.ny ? j - G.ny : j);
+    const int kk = k < 0 ? k + G.nz : (k >= G.nz ? k - G.nz : k);
+    rec.ij = (uint32_t)(i + 1) | (uint32_t)jj << 16;
+    rec.ks = (uint32_t)kk | (uint32_t)(sx + 1) << 16 | (uint32_t)(sy + 1) << 18 | (uint32_t)(sz + 1) << 20;
+    uint32_t cap = C.min_cap;
+    if (fabs(d[0]) < C.grazing_dx) cap = max_steps_of(d[0], d[1], d[2], G.nx, G.ny, G.nz);
+    rec.cap = cap;
+    rec.pad = 0u;
+}
+
+// Traversal state of ray g from its record; false for a ray that fails at once.
+__device__ __forceinline__ bool ray_load(Ray &R, const DdaRec *recs, uint32_t g)
+{
+    const double2 *p = reinterpret_cast<const double2 *>(recs + g);
+    const double2 a = p[0], b = p[1], c = p[2];
+    const uint4 w = reinterpret_cast<const uint4 *>(recs + g)[3];
+    R.tx = a.x;
+    R.ty = a.y;
+    R.tz = b.x;
+    R.ddx = b.y;
+    R.ddy = c.x;
+    R.ddz = c.y;
+    R.i = (int)(w.x & 0xFFFFu) - 1;
+    R.jj = (int)(w.x >> 16);
+    R.kk = (int)(w.y & 0xFFFFu);
+    R.sx = (int)((w.y >> 16) & 3u) - 1;
+    R.sy = (int)((w.y >> 18) & 3u) - 1;
+    R.sz = (int)((w.y >> 20) & 3u) - 1;
+    R.cap = w.z;
+    R.left = w.z;
     R.axis = -1;
     R.time = 0.0;
-    R.cap = C.min_cap;
-    if (fabs(dx) < C.grazing_dx) R.cap = max_steps_of(dx, R.dy, R.dz, G.nx, G.ny, G.nz);
-    R.left = R.cap;
-    return true;
+    return w.z != 0u;
 }
 
-// Occupancy test of the current cell.  With COARSE, the brick bitset held in
-// LDS answers first; only lanes in a brick that holds solid read the cell's
-// bit from the global bitset (L2-resident), so most steps never reach the
-// vector memory pipeline.
-template <bool COARSE>
+// How the traversal tests a cell's occupancy.
+enum OccMode : int {
+    OCC_GLOBAL = 0,  // read the cell's bit from the global bitset (L2-resident)
+    OCC_BRICK = 1,   // the brick bitset held in LDS first; the global bit only
+                     // for cells in bricks that hold solid
+    OCC_AHEAD = 2,   // the global bitset word loaded ahead: the step is computed
+                     // while the load is in flight and undone on a solid cell
+};
+
+// Index of the cell (i, jj, kk) in the flat bitset.  24-bit multiply-adds:
+// i ny + j and nz stay below 2^24 and the index below 2^31 (checked at
+// geometry creation).
+__device__ __forceinline__ uint32_t cell_index(const Ray &R, const GeoView &G)
+{
+    return mad_u24(mad_u24((uint32_t)R.i, (uint32_t)G.ny, (uint32_t)R.jj), (uint32_t)G.nz, (uint32_t)R.kk);
+}
+
+__device__ __forceinline__ uint32_t occupancy_word(const GeoView &G, uint32_t cell)
+{
+    return *(const uint32_t *)((const char *)G.occ + ((cell >> 3) & ~3u));
+}
+
+// Occupancy test of the current cell (OCC_GLOBAL, OCC_BRICK).
+template <int OCC>
 __device__ __forceinline__ bool cell_is_solid(const Ray &R, const GeoView &G, const uint32_t *lds_coarse)
 {
-    if (COARSE) {
+    if (OCC == OCC_BRICK) {
         const unsigned s = (unsigned)G.coarse_shift;
-        const uint32_t b = (((uint32_t)R.i >> s) * (uint32_t)G.cny + ((uint32_t)R.jj >> s)) * (uint32_t)G.cnz +
-                           ((uint32_t)R.kk >> s);
+        const uint32_t b = mad_u24(mad_u24((uint32_t)R.i >> s, (uint32_t)G.cny, (uint32_t)R.jj >> s),
+                                   (uint32_t)G.cnz, (uint32_t)R.kk >> s);
         if (!((lds_coarse[b >> 5] >> (b & 31u)) & 1u)) return false;
     }
-    const uint32_t cell = ((uint32_t)R.i * (uint32_t)G.ny + (uint32_t)R.jj) * (uint32_t)G.nz + (uint32_t)R.kk;
-    const uint32_t word = *(const uint32_t *)((const char *)G.occ + ((cell >> 3) & ~3u));
-    return (word >> (cell & 31u)) & 1u;
+    const uint32_t cell = cell_index(R, G);
+    return (occupancy_word(G, cell) >> (cell & 31u)) & 1u;
 }
 
-// One iteration of the traversal loop, view_factors.py:225-255; returns
-// whether the ray goes on.  The minimum of the three crossing times is the
-// value of one of them, so at least one axis crosses.  Updates are selects
-// (the wave stays converged); a lateral index leaving [0, n) is wrapped by
-// subtracting step * n.
-template <bool COARSE>
-__device__ __forceinline__ bool ray_step(Ray &R, const GeoView &G, const uint32_t *lds_coarse)
+// The crossing of view_factors.py:237-255: every axis whose crossing time is
+// within the tie tolerance of the minimum advances; the face is the first
+// of them in x > y > z order.  The minimum is the value of one of the three
+// times, so at least one axis crosses.  Updates are selects (the wave stays
+// converged); a lateral index leaving [0, n) is wrapped by subtracting
+// step * n.
+__device__ __forceinline__ void ray_advance(Ray &R, const GeoView &G)
 {
-    if ((unsigned)R.i >= (unsigned)G.nx) return false;
-    if (cell_is_solid<COARSE>(R, G, lds_coarse)) return false;
     const double nt = vmin_f64(vmin_f64(R.tx, R.ty), R.tz);
     const bool cx = fabs(R.tx - nt) <= TIE_TOLERANCE;
     const bool cy = fabs(R.ty - nt) <= TIE_TOLERANCE;
     const bool cz = fabs(R.tz - nt) <= TIE_TOLERANCE;
+    // t += dd on a crossing as fma(c, dd, t) with c in {0, 1}: fma(1, dd, t)
+    // rounds t + dd exactly as the addition does, and fma(0, dd, t) is t
+    // (dd is finite; a -0 start time may become +0, which no comparison or
+    // product distinguishes).
     R.i += cx ? R.sx : 0;
-    R.tx = cx ? R.tx + R.ddx : R.tx;
+    R.tx = fma_f64(cx ? 1.0 : 0.0, R.ddx, R.tx);
     const int jn = R.jj + (cy ? R.sy : 0);
     R.jj = (unsigned)jn < (unsigned)G.ny ? jn : jn - __mul24(R.sy, G.ny);
-    R.ty = cy ? R.ty + R.ddy : R.ty;
+    R.ty = fma_f64(cy ? 1.0 : 0.0, R.ddy, R.ty);
     const int kn = R.kk + (cz ? R.sz : 0);
     R.kk = (unsigned)kn < (unsigned)G.nz ? kn : kn - __mul24(R.sz, G.nz);
-    R.tz = cz ? R.tz + R.ddz : R.tz;
+    R.tz = fma_f64(cz ? 1.0 : 0.0, R.ddz, R.tz);
     R.axis = cx ? 0 : (cy ? 1 : 2);
     R.time = nt;
-    return --R.left != 0u;
 }
 
-__device__ __forceinline__ bool ray_at_wall(const Ray &R, const GeoView &G)
+// One iteration of the traversal loop, view_factors.py:225-255; returns
+// whether the ray goes on.
+template <int OCC>
+__device__ __forceinline__ bool ray_step(Ray &R, const GeoView &G, const uint32_t *lds_coarse)
 {
-    return R.left != 0u && (unsigned)R.i >= (unsigned)G.nx;
+    if ((unsigned)R.i >= (unsigned)G.nx) return false;
+    if (OCC == OCC_AHEAD) {
+        const uint32_t cell = cell_index(R, G);
+        const uint32_t word = occupancy_word(G, cell);
+        const int i0 = R.i, j0 = R.jj, k0 = R.kk, axis0 = R.axis;
+        ray_advance(R, G);
+        if ((word >> (cell & 31u)) & 1u) {
+            // the current cell was solid: the ray ends here, as if it had not moved
+            R.i = i0;
+            R.jj = j0;
+            R.kk = k0;
+            R.axis = axis0;
+            return false;
+        }
+        return --R.left != 0u;
+    }
+    if (cell_is_solid<OCC>(R, G, lds_coarse)) return false;
+    ray_advance(R, G);
+    return --R.left != 0u;
 }
 
 // Occupancy reads of a finished traversal: one per step plus the read that
@@ -191,29 +307,61 @@ __device__ __forceinline__ uint32_t ray_reads(const Ray &R, const GeoView &G)
     return (R.cap - R.left) + (at_solid ? 1u : 0u);
 }
 
-// Receiving patch of a finished traversal (view_factors.py:226-236), -1 on failure.
-__device__ __forceinline__ int ray_receiver(const Ray &R, const GeoView &G)
+// Python's floor(v) % n for the wall cell index (view_factors.py:226-231),
+// with |floor(v)| far below 2^52: r = f - n floor(f / n) from an inexact
+// quotient, then one correction either way (all values are exact integers).
+__device__ __forceinline__ uint32_t wrap_floor(double v, int n, double inv_n)
 {
-    if (R.left == 0u) return -1;
+    const double f = floor(v);
+    const double dn = (double)n;
+    double r = f - dn * floor(f * inv_n);
+    r = r < 0.0 ? r + dn : r;
+    r = r >= dn ? r - dn : r;
+    return (uint32_t)r;
+}
+
+// Final state of a finished traversal (see FIN_*).
+__device__ __forceinline__ uint2 ray_final(const Ray &R, const GeoView &G, const WallRec *walls, uint32_t g)
+{
+    if (R.left == 0u) return make_uint2(0u, FIN_FAILED << 28);
     if ((unsigned)R.i >= (unsigned)G.nx) {
-        const double yw = R.oy + R.dy * R.time;
-        const double zw = R.oz + R.dz * R.time;
-        const int64_t jw = py_imod64((int64_t)floor(yw), G.ny);
-        const int64_t kw = py_imod64((int64_t)floor(zw), G.nz);
-        const int64_t c = jw * G.nz + kw;
-        return R.i < 0 ? G.hot_map[c] : G.cold_map[c];
+        const double2 *p = reinterpret_cast<const double2 *>(walls + g);
+        const double2 o = p[0], d = p[1];
+        const double yw = o.x + d.x * R.time;
+        const double zw = o.y + d.y * R.time;
+        const uint32_t jw = wrap_floor(yw, G.ny, G.inv_ny);
+        const uint32_t kw = wrap_floor(zw, G.nz, G.inv_nz);
+        return make_uint2(jw, (R.i < 0 ? FIN_HOT : FIN_COLD) << 28 | kw);
     }
-    const int axis = R.axis;
-    if (axis < 0) return -1;
+    if (R.axis < 0) return make_uint2(0u, FIN_FAILED << 28);
     // Direction bits as values (a select between struct fields would be
     // lowered to an indexed stack access).
     const uint32_t up = (R.sx > 0 ? 1u : 0u) | (R.sy > 0 ? 2u : 0u) | (R.sz > 0 ? 4u : 0u);
-    const int face = 2 * axis + (int)((up >> axis) & 1u);
-    const int64_t cell = ((int64_t)R.i * G.ny + R.jj) * G.nz + R.kk;
-    const uint32_t code = G.cell_code[cell];
-    const uint32_t m = code & 63u;
-    if (!((m >> face) & 1u)) return -1;
-    return (int)(code >> 6) + __popc(m & ((1u << face) - 1u));
+    const uint32_t face = 2u * (uint32_t)R.axis + ((up >> R.axis) & 1u);
+    return make_uint2(cell_index(R, G), FIN_SOLID << 28 | face);
+}
+
+// Receiving patch of a final state (view_factors.py:226-236), -1 on failure:
+// the wall maps, or the patch of (cell, face) from the cell code (first
+// patch id of the cell << 6 | mask of its exposed faces).
+__device__ __forceinline__ int resolve_receiver(uint2 f, const GeoView &G)
+{
+    const uint32_t kind = f.y >> 28;
+    if (kind == FIN_SOLID) {
+        const uint32_t face = f.y & 7u;
+        if ((uint64_t)f.x >= (uint64_t)G.nx * (uint64_t)G.ny * (uint64_t)G.nz || face > 5u) return -1;
+        const uint32_t code = G.cell_code[f.x];
+        const uint32_t m = code & 63u;
+        if (!((m >> face) & 1u)) return -1;
+        return (int)(code >> 6) + __popc(m & ((1u << face) - 1u));
+    }
+    if (kind == FIN_HOT || kind == FIN_COLD) {
+        const uint32_t kw = f.y & 0x0FFFFFFFu;
+        if (f.x >= (uint32_t)G.ny || kw >= (uint32_t)G.nz) return -1;
+        const size_t c = (size_t)f.x * (size_t)G.nz + kw;
+        return kind == FIN_HOT ? G.hot_map[c] : G.cold_map[c];
+    }
+    return -1;
 }
 
 __device__ __forceinline__ void load_seed(const uint64_t *seeds, int s, u128 &st, u128 &inc)
@@ -239,13 +387,20 @@ constexpr uint32_t GEN_ROUNDS = 16;
 
 struct GenArgs {
     GeoView G;
+    TraversalConst C;
     uint32_t p0, n_p, R;
     const uint64_t *seeds;  // S x {state_hi, state_lo, inc_hi, inc_lo}
     uint32_t total;
-    double *origins;     // total x 3
-    double *directions;  // total x 3
+    double *origins;     // total x 3 (rays only)
+    double *directions;  // total x 3 (rays only)
+    DdaRec *dda;         // total (records only)
+    WallRec *walls;      // total (records only)
 };
 
+// RECORDS: write the traversal records (DDA start state, wall inputs)
+// instead of the origins and directions, so the divisions of the set-up run
+// here in converged waves rather than in the traversal's refills.
+template <bool RECORDS>
 __global__ __launch_bounds__(TRACE_THREADS) void k_gen_rays(GenArgs a)
 {
     const GeoView &G = a.G;
@@ -278,40 +433,63 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_gen_rays(GenArgs a)
         double o[3], d[3];
         make_ray(st, inc, po[0], po[1], po[2], normal_code_of(G, p), fny, fnz, o, d);
         st = next;
-        double *oo = a.origins + 3 * (size_t)g;
-        double *dd = a.directions + 3 * (size_t)g;
-        oo[0] = o[0];
-        oo[1] = o[1];
-        oo[2] = o[2];
-        dd[0] = d[0];
-        dd[1] = d[1];
-        dd[2] = d[2];
+        if (RECORDS) {
+            DdaRec rec;
+            WallRec w;
+            make_dda(o, d, G, a.C, rec, w);
+            a.dda[g] = rec;
+            a.walls[g] = w;
+        } else {
+            double *oo = a.origins + 3 * (size_t)g;
+            double *dd = a.directions + 3 * (size_t)g;
+            oo[0] = o[0];
+            oo[1] = o[1];
+            oo[2] = o[2];
+            dd[0] = d[0];
+            dd[1] = d[1];
+            dd[2] = d[2];
+        }
     }
+}
+
+// Traversal records of given rays (prh_trace_rays).
+__global__ __launch_bounds__(TRACE_THREADS) void k_setup_rays(GeoView G, TraversalConst C, const double *origins,
+                                                              const double *directions, uint32_t total, DdaRec *dda,
+                                                              WallRec *walls)
+{
+    const uint32_t g = blockIdx.x * TRACE_THREADS + threadIdx.x;
+    if (g >= total) return;
+    const double o[3] = {origins[3 * (size_t)g], origins[3 * (size_t)g + 1], origins[3 * (size_t)g + 2]};
+    const double d[3] = {directions[3 * (size_t)g], directions[3 * (size_t)g + 1], directions[3 * (size_t)g + 2]};
+    DdaRec rec;
+    WallRec w;
+    make_dda(o, d, G, C, rec, w);
+    dda[g] = rec;
+    walls[g] = w;
 }
 
 // ------------------------------------------------------------------ traversal
 // Each wavefront owns a contiguous chunk of rays and a cursor into it.  It
 // keeps stepping while its lanes are busy; once at least REFILL_MIN lanes
-// are idle, those lanes write their receivers and take the next rays of the
-// chunk in lane order (adjacent ray records, so the refill loads coalesce).
+// are idle, those lanes write their final states and load the next records
+// of the chunk in lane order (adjacent 64 B records, so the refill loads
+// coalesce).
 struct TraceArgs {
     GeoView G;
-    TraversalConst C;
     uint32_t total, per_wave;
-    const double *origins;
-    const double *directions;
-    int32_t *receiver;
+    const DdaRec *dda;
+    const WallRec *walls;
+    uint2 *fin;
     unsigned long long *reads;
-    unsigned long long *failed;
 };
 
-template <int REFILL_MIN, bool COARSE>
+template <int REFILL_MIN, int OCC>
 __global__ __launch_bounds__(STREAM_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
 void k_trace_stream(TraceArgs a)
 {
     const GeoView &G = a.G;
     extern __shared__ uint32_t lds_coarse[];
-    if (COARSE) {
+    if (OCC == OCC_BRICK) {
         for (uint32_t w = threadIdx.x; w < G.coarse_words; w += STREAM_THREADS) lds_coarse[w] = G.coarse[w];
         __syncthreads();
     }
@@ -329,14 +507,11 @@ void k_trace_stream(TraceArgs a)
     bool active = false, pending = false;
     uint32_t g_cur = 0;
     uint64_t reads = 0;
-    uint32_t failed = 0;
 
     while (true) {
         if (!active && pending) {
-            const int rec = ray_receiver(R, G);
-            failed += rec < 0 ? 1u : 0u;
             reads += ray_reads(R, G);
-            a.receiver[g_cur] = rec;
+            a.fin[g_cur] = ray_final(R, G, a.walls, g_cur);
             pending = false;
         }
         if (cursor < w_end) {
@@ -344,17 +519,12 @@ void k_trace_stream(TraceArgs a)
             if (!active) {
                 const uint32_t g = cursor + (uint32_t)__popcll(idle & below);
                 if (g < w_end) {
-                    const double *po = a.origins + 3 * (size_t)g;
-                    const double *pd = a.directions + 3 * (size_t)g;
-                    const double o[3] = {po[0], po[1], po[2]};
-                    const double d[3] = {pd[0], pd[1], pd[2]};
-                    if (ray_setup(R, o, d, G, a.C)) {
+                    if (ray_load(R, a.dda, g)) {
                         active = true;
                         pending = true;
                         g_cur = g;
                     } else {
-                        a.receiver[g] = -1;
-                        ++failed;
+                        a.fin[g] = make_uint2(0u, FIN_FAILED << 28);
                     }
                 }
             }
@@ -365,27 +535,40 @@ void k_trace_stream(TraceArgs a)
             continue;
         }
         while (true) {
-            if (active) active = ray_step<COARSE>(R, G, lds_coarse);
+            // two steps per wave-level check
+            if (active) active = ray_step<OCC>(R, G, lds_coarse);
+            if (active) active = ray_step<OCC>(R, G, lds_coarse);
             const uint64_t act = __ballot(active);
             if (act == 0) break;
             if (cursor < w_end && __popcll(~act) >= REFILL_MIN) break;
         }
     }
     reads = wave_sum_u64(reads);
-    const uint64_t f = wave_sum_u64((uint64_t)failed);
-    if (lane == 0) {
-        atomicAdd(a.reads, (unsigned long long)reads);
-        if (f) atomicAdd(a.failed, (unsigned long long)f);
+    if (lane == 0) atomicAdd(a.reads, (unsigned long long)reads);
+}
+
+// Receivers of final states (prh_trace_rays), with the count of failures.
+__global__ __launch_bounds__(TRACE_THREADS) void k_resolve(const uint2 *fin, GeoView G, uint32_t total,
+                                                           int32_t *receiver, unsigned long long *failed)
+{
+    const uint32_t g = blockIdx.x * TRACE_THREADS + threadIdx.x;
+    int rec = 0;
+    if (g < total) {
+        rec = resolve_receiver(fin[g], G);
+        receiver[g] = rec;
     }
+    const uint64_t f = wave_sum_u64(g < total && rec < 0 ? 1ull : 0ull);
+    if ((threadIdx.x & 63u) == 0 && f) atomicAdd(failed, (unsigned long long)f);
 }
 
 // Per-patch sorted unique receivers and their counts (the np.unique of
 // first_hit_counts, view_factors.py:284-286, pooled over seeds as
-// pool_counts does, view_factors.py:290-300).  One workgroup per patch,
-// bitonic sort of the patch's S*R receivers in LDS.
-__global__ __launch_bounds__(COUNT_THREADS) void k_count_patches(const int32_t *receiver, int64_t n_p, int R, int S,
+// pool_counts does, view_factors.py:290-300).  One workgroup per patch: the
+// receivers of the patch's S*R final states are resolved on load (failures
+// counted), then bitonic-sorted in LDS.
+__global__ __launch_bounds__(COUNT_THREADS) void k_count_patches(const uint2 *fin, GeoView G, int64_t n_p, int R, int S,
                                                                  int SRp2, int32_t *uniq_recv, int32_t *uniq_cnt,
-                                                                 int32_t *n_unique)
+                                                                 int32_t *n_unique, unsigned long long *failed)
 {
     extern __shared__ int32_t lds[];
     int32_t *key = lds;
@@ -393,14 +576,20 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_patches(const int32_t *
     const int SR = S * R;
     const int chunk = SRp2 >= COUNT_THREADS ? SRp2 / COUNT_THREADS : 1;
     for (int64_t q = blockIdx.x; q < n_p; q += gridDim.x) {
+        uint32_t bad = 0;
         for (int t = threadIdx.x; t < SRp2; t += COUNT_THREADS) {
             int32_t v = INT_MAX;
             if (t < SR) {
                 const int s = t / R, r = t - s * R;
-                v = receiver[(int64_t)s * n_p * R + q * R + r];
+                v = resolve_receiver(fin[(int64_t)s * n_p * R + q * R + r], G);
+                if (v < 0) {
+                    ++bad;
+                    v = INT_MAX;
+                }
             }
             key[t] = v;
         }
+        if (bad) atomicAdd(failed, (unsigned long long)bad);
         __syncthreads();
         for (int k = 2; k <= SRp2; k <<= 1) {
             for (int j = k >> 1; j > 0; j >>= 1) {
@@ -450,6 +639,113 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_patches(const int32_t *
     }
 }
 
+// The same for S*R <= 1024, one wavefront per patch with the keys in
+// registers: element e = lane * EPL + r (EPL = keys per lane, a power of 2),
+// so bitonic stages with partner distance j < EPL are register swaps and
+// the others one __shfl_xor per key.  Runs are then found with one
+// neighbour shuffle and a wave prefix sum; run starts go through LDS to
+// give each run its length.
+template <int EPL>
+__global__ __launch_bounds__(COUNT_THREADS) void k_count_wave(const uint2 *fin, GeoView G, int64_t n_p, int R, int S,
+                                                              int32_t *uniq_recv, int32_t *uniq_cnt,
+                                                              int32_t *n_unique, unsigned long long *failed)
+{
+    constexpr int N = 64 * EPL;
+    constexpr int WAVES = COUNT_THREADS / 64;
+    __shared__ int32_t s_pos[WAVES][N + 1];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    int32_t *pos = s_pos[wave];
+    const int SR = S * R;
+    for (int64_t q = (int64_t)blockIdx.x * WAVES + wave; q < n_p; q += (int64_t)gridDim.x * WAVES) {
+        int32_t key[EPL];
+        uint32_t bad = 0;
+#pragma unroll
+        for (int r = 0; r < EPL; ++r) {
+            const int t = lane * EPL + r;
+            int32_t v = INT_MAX;
+            if (t < SR) {
+                const int s = S == 1 ? 0 : t / R, u = t - s * R;
+                v = resolve_receiver(fin[(int64_t)s * n_p * R + q * R + u], G);
+                if (v < 0) {
+                    ++bad;
+                    v = INT_MAX;
+                }
+            }
+            key[r] = v;
+        }
+        bad = (uint32_t)wave_sum_u64(bad);
+        if (lane == 0 && bad) atomicAdd(failed, (unsigned long long)bad);
+#pragma unroll
+        for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                if (j < EPL) {
+#pragma unroll
+                    for (int r = 0; r < EPL; ++r) {
+                        if (r & j) continue;
+                        const int e = lane * EPL + r;
+                        const bool up = (e & k) == 0;
+                        const int32_t a = key[r], b = key[r | j];
+                        const int32_t lo = a < b ? a : b, hi = a < b ? b : a;
+                        key[r] = up ? lo : hi;
+                        key[r | j] = up ? hi : lo;
+                    }
+                } else {
+                    const int lj = j / EPL;
+                    const bool lower = (lane & lj) == 0;
+#pragma unroll
+                    for (int r = 0; r < EPL; ++r) {
+                        const int e = lane * EPL + r;
+                        const bool up = (e & k) == 0;
+                        const int32_t other = __shfl_xor(key[r], lj);
+                        const int32_t lo = key[r] < other ? key[r] : other;
+                        const int32_t hi = key[r] < other ? other : key[r];
+                        key[r] = (up == lower) ? lo : hi;
+                    }
+                }
+            }
+        }
+        // run starts among the first SR elements
+        const int32_t prev_last = __shfl_up(key[EPL - 1], 1);
+        int starts = 0;
+#pragma unroll
+        for (int r = 0; r < EPL; ++r) {
+            const int e = lane * EPL + r;
+            const int32_t prev = r == 0 ? prev_last : key[r - 1];
+            starts += (e < SR && (e == 0 || key[r] != prev)) ? 1 : 0;
+        }
+        int incl = starts;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        const int total = __shfl(incl, 63);
+        int rank = incl - starts;
+#pragma unroll
+        for (int r = 0; r < EPL; ++r) {
+            const int e = lane * EPL + r;
+            const int32_t prev = r == 0 ? prev_last : key[r - 1];
+            if (e < SR && (e == 0 || key[r] != prev)) {
+                uniq_recv[q * SR + rank] = key[r];
+                pos[rank] = e;
+                ++rank;
+            }
+        }
+        if (lane == 0) {
+            pos[total] = SR;
+            n_unique[q] = total;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (int u = lane; u < total; u += 64) uniq_cnt[q * SR + u] = pos[u + 1] - pos[u];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+}
+
 __global__ __launch_bounds__(COUNT_THREADS) void k_compact_edges(const int32_t *uniq_recv, const int32_t *uniq_cnt,
                                                                  const int32_t *n_unique, const int64_t *offsets,
                                                                  int64_t n_p, int SR, int64_t p0, int32_t *origin,
@@ -491,7 +787,7 @@ struct Buf {
 // Grow-only per-device workspaces of the tracer (one library stream per
 // device, so calls on one device are serialised).
 struct Workspace {
-    Buf seeds, origins, directions, receiver, uniq_recv, uniq_cnt, n_unique, offsets, counters;
+    Buf seeds, origins, directions, dda, walls, fin, receiver, uniq_recv, uniq_cnt, n_unique, offsets, counters;
     hipEvent_t ev[5] = {};
     bool events = false;
 };
@@ -512,42 +808,61 @@ int resident_waves(const void *kernel, int threads, size_t lds, int64_t *waves)
     return PRH_OK;
 }
 
-// Traversal kernel variants: refill threshold x occupancy test (LDS brick
-// bitset first, or every test from the global bitset).  The variant is a
-// tuning knob (prh_tracer_set_variant, chosen by porerad/autotune.py).
+// Traversal kernel variants: refill threshold x occupancy test mode
+// (OccMode).  The variant is a tuning knob (prh_tracer_set_variant, chosen
+// by porerad/autotune.py); every variant produces the same receivers.
 using TraceKernel = void (*)(TraceArgs);
 struct TraceVariant {
     TraceKernel fn;
-    int refill_min, coarse;
+    int refill_min, occ;
 };
 const TraceVariant TRACE_VARIANTS[] = {
-    {k_trace_stream<32, true>, 32, 1},  {k_trace_stream<16, true>, 16, 1}, {k_trace_stream<24, true>, 24, 1},
-    {k_trace_stream<40, true>, 40, 1},  {k_trace_stream<48, true>, 48, 1}, {k_trace_stream<32, false>, 32, 0},
-    {k_trace_stream<16, false>, 16, 0}, {k_trace_stream<24, false>, 24, 0},
+    {k_trace_stream<32, OCC_BRICK>, 32, OCC_BRICK},   {k_trace_stream<16, OCC_BRICK>, 16, OCC_BRICK},
+    {k_trace_stream<24, OCC_BRICK>, 24, OCC_BRICK},   {k_trace_stream<40, OCC_BRICK>, 40, OCC_BRICK},
+    {k_trace_stream<48, OCC_BRICK>, 48, OCC_BRICK},   {k_trace_stream<32, OCC_GLOBAL>, 32, OCC_GLOBAL},
+    {k_trace_stream<16, OCC_GLOBAL>, 16, OCC_GLOBAL}, {k_trace_stream<24, OCC_GLOBAL>, 24, OCC_GLOBAL},
+    {k_trace_stream<16, OCC_AHEAD>, 16, OCC_AHEAD},   {k_trace_stream<24, OCC_AHEAD>, 24, OCC_AHEAD},
+    {k_trace_stream<32, OCC_AHEAD>, 32, OCC_AHEAD},
 };
 constexpr int N_TRACE_VARIANTS = sizeof(TRACE_VARIANTS) / sizeof(TRACE_VARIANTS[0]);
 // Default: refill at 16 idle lanes with the LDS brick test, the fastest on
 // the foam and fibre media measured so far.
 int g_trace_variant = 1;
 
-int launch_gen(const Geometry &g, uint32_t p0, uint32_t n_p, uint32_t R, const uint64_t *d_seeds, uint32_t total,
-               double *d_orig, double *d_dir, hipStream_t st)
+unsigned gen_grid(uint32_t total)
+{
+    const uint64_t waves = (total + 64ull * GEN_ROUNDS - 1) / (64ull * GEN_ROUNDS);
+    return (unsigned)((waves * 64 + TRACE_THREADS - 1) / TRACE_THREADS);
+}
+
+// Origins and directions of the rays of patches [p0, p0 + n_p).
+int launch_gen_rays(const Geometry &g, uint32_t p0, uint32_t n_p, uint32_t R, const uint64_t *d_seeds, uint32_t total,
+                    double *d_orig, double *d_dir, hipStream_t st)
 {
     if (total == 0) return PRH_OK;
-    const uint64_t waves = (total + 64ull * GEN_ROUNDS - 1) / (64ull * GEN_ROUNDS);
-    const unsigned grid = (unsigned)((waves * 64 + TRACE_THREADS - 1) / TRACE_THREADS);
-    GenArgs a{view_of(g), p0, n_p, R, d_seeds, total, d_orig, d_dir};
-    k_gen_rays<<<grid, TRACE_THREADS, 0, st>>>(a);
+    GenArgs a{view_of(g), traversal_const(g), p0, n_p, R, d_seeds, total, d_orig, d_dir, nullptr, nullptr};
+    k_gen_rays<false><<<gen_grid(total), TRACE_THREADS, 0, st>>>(a);
     PRH_HIP(hipGetLastError());
     return PRH_OK;
 }
 
-int launch_trace(const Geometry &g, const double *d_orig, const double *d_dir, uint32_t total, int32_t *d_receiver,
-                 unsigned long long *d_counters, hipStream_t st)
+// Traversal records of the rays of patches [p0, p0 + n_p).
+int launch_gen_records(const Geometry &g, uint32_t p0, uint32_t n_p, uint32_t R, const uint64_t *d_seeds,
+                       uint32_t total, DdaRec *d_dda, WallRec *d_walls, hipStream_t st)
+{
+    if (total == 0) return PRH_OK;
+    GenArgs a{view_of(g), traversal_const(g), p0, n_p, R, d_seeds, total, nullptr, nullptr, d_dda, d_walls};
+    k_gen_rays<true><<<gen_grid(total), TRACE_THREADS, 0, st>>>(a);
+    PRH_HIP(hipGetLastError());
+    return PRH_OK;
+}
+
+int launch_trace(const Geometry &g, const DdaRec *d_dda, const WallRec *d_walls, uint32_t total, uint2 *d_fin,
+                 unsigned long long *d_reads, hipStream_t st)
 {
     if (total == 0) return PRH_OK;
     const TraceVariant &V = TRACE_VARIANTS[g_trace_variant];
-    const size_t lds = V.coarse ? (size_t)g.coarse_words * sizeof(uint32_t) : 0;
+    const size_t lds = V.occ == OCC_BRICK ? (size_t)g.coarse_words * sizeof(uint32_t) : 0;
     int64_t waves = 0;
     int rc = resident_waves((const void *)V.fn, STREAM_THREADS, lds, &waves);
     if (rc) return rc;
@@ -556,8 +871,7 @@ int launch_trace(const Geometry &g, const double *d_orig, const double *d_dir, u
     if (per_wave < 64) per_wave = 64;
     const int64_t n_waves = (total + per_wave - 1) / per_wave;
     const unsigned grid = (unsigned)((n_waves * 64 + STREAM_THREADS - 1) / STREAM_THREADS);
-    TraceArgs a{view_of(g), traversal_const(g), total, (uint32_t)per_wave, d_orig, d_dir, d_receiver, d_counters,
-                d_counters + 1};
+    TraceArgs a{view_of(g), total, (uint32_t)per_wave, d_dda, d_walls, d_fin, d_reads};
     V.fn<<<grid, STREAM_THREADS, lds, st>>>(a);
     PRH_HIP(hipGetLastError());
     return PRH_OK;
@@ -594,13 +908,13 @@ int prh_tracer_set_variant(int32_t variant)
     return PRH_OK;
 }
 
-int prh_tracer_variants(int32_t *n_variants, int32_t *current, int32_t *refill_min, int32_t *coarse)
+int prh_tracer_variants(int32_t *n_variants, int32_t *current, int32_t *refill_min, int32_t *occupancy_mode)
 {
     if (n_variants) *n_variants = N_TRACE_VARIANTS;
     if (current) *current = g_trace_variant;
     for (int v = 0; v < N_TRACE_VARIANTS; ++v) {
         if (refill_min) refill_min[v] = TRACE_VARIANTS[v].refill_min;
-        if (coarse) coarse[v] = TRACE_VARIANTS[v].coarse;
+        if (occupancy_mode) occupancy_mode[v] = TRACE_VARIANTS[v].occ;
     }
     return PRH_OK;
 }
@@ -625,8 +939,9 @@ int prh_generate_rays(const prh_geometry *geo, int64_t p0, int64_t p1, int32_t r
     if ((rc = grow(W.origins, m * 3 * sizeof(double)))) return rc;
     if ((rc = grow(W.directions, m * 3 * sizeof(double)))) return rc;
     PRH_HIP(hipMemcpyAsync(W.seeds.p, state_inc, 4 * sizeof(uint64_t), hipMemcpyHostToDevice, st));
-    if ((rc = launch_gen(g, (uint32_t)p0, (uint32_t)(p1 - p0), (uint32_t)rays_per_patch, W.seeds.as<uint64_t>(),
-                         (uint32_t)m, W.origins.as<double>(), W.directions.as<double>(), st)))
+    if ((rc = launch_gen_rays(g, (uint32_t)p0, (uint32_t)(p1 - p0), (uint32_t)rays_per_patch,
+                              W.seeds.as<uint64_t>(), (uint32_t)m, W.origins.as<double>(), W.directions.as<double>(),
+                              st)))
         return rc;
     PRH_HIP(hipMemcpyAsync(origins, W.origins.p, m * 3 * sizeof(double), hipMemcpyDeviceToHost, st));
     PRH_HIP(hipMemcpyAsync(directions, W.directions.p, m * 3 * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -649,14 +964,25 @@ int prh_trace_rays(const prh_geometry *geo, const double *origins, const double 
     Workspace &W = g_ws[g.device];
     if ((rc = grow(W.origins, n_rays * 3 * sizeof(double)))) return rc;
     if ((rc = grow(W.directions, n_rays * 3 * sizeof(double)))) return rc;
+    if ((rc = grow(W.dda, n_rays * sizeof(DdaRec)))) return rc;
+    if ((rc = grow(W.walls, n_rays * sizeof(WallRec)))) return rc;
+    if ((rc = grow(W.fin, n_rays * sizeof(uint2)))) return rc;
     if ((rc = grow(W.receiver, n_rays * sizeof(int32_t)))) return rc;
     if ((rc = grow(W.counters, 2 * sizeof(unsigned long long)))) return rc;
     PRH_HIP(hipMemcpyAsync(W.origins.p, origins, n_rays * 3 * sizeof(double), hipMemcpyHostToDevice, st));
     PRH_HIP(hipMemcpyAsync(W.directions.p, directions, n_rays * 3 * sizeof(double), hipMemcpyHostToDevice, st));
     PRH_HIP(hipMemsetAsync(W.counters.p, 0, 2 * sizeof(unsigned long long), st));
-    if ((rc = launch_trace(g, W.origins.as<double>(), W.directions.as<double>(), (uint32_t)n_rays,
-                           W.receiver.as<int32_t>(), W.counters.as<unsigned long long>(), st)))
+    const uint32_t total = (uint32_t)n_rays;
+    const unsigned grid = (unsigned)((n_rays + TRACE_THREADS - 1) / TRACE_THREADS);
+    k_setup_rays<<<grid, TRACE_THREADS, 0, st>>>(view_of(g), traversal_const(g), W.origins.as<double>(),
+                                                 W.directions.as<double>(), total, W.dda.as<DdaRec>(),
+                                                 W.walls.as<WallRec>());
+    PRH_HIP(hipGetLastError());
+    unsigned long long *ctr = W.counters.as<unsigned long long>();
+    if ((rc = launch_trace(g, W.dda.as<DdaRec>(), W.walls.as<WallRec>(), total, W.fin.as<uint2>(), ctr, st)))
         return rc;
+    k_resolve<<<grid, TRACE_THREADS, 0, st>>>(W.fin.as<uint2>(), view_of(g), total, W.receiver.as<int32_t>(), ctr + 1);
+    PRH_HIP(hipGetLastError());
     unsigned long long counters[2] = {0, 0};
     PRH_HIP(hipMemcpyAsync(receiver, W.receiver.p, n_rays * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     PRH_HIP(hipMemcpyAsync(counters, W.counters.p, sizeof(counters), hipMemcpyDeviceToHost, st));
@@ -705,9 +1031,9 @@ int prh_trace_counts(const prh_geometry *geo, int64_t p0, int64_t p1, int32_t ra
     while (SRp2 < SR) SRp2 <<= 1;
     const size_t t1 = total > 0 ? (size_t)total : 1, n1 = n_p > 0 ? (size_t)n_p : 1;
     if ((rc = grow(W.seeds, sizeof(uint64_t) * 4 * n_seeds))) return rc;
-    if ((rc = grow(W.origins, t1 * 3 * sizeof(double)))) return rc;
-    if ((rc = grow(W.directions, t1 * 3 * sizeof(double)))) return rc;
-    if ((rc = grow(W.receiver, t1 * sizeof(int32_t)))) return rc;
+    if ((rc = grow(W.dda, t1 * sizeof(DdaRec)))) return rc;
+    if ((rc = grow(W.walls, t1 * sizeof(WallRec)))) return rc;
+    if ((rc = grow(W.fin, t1 * sizeof(uint2)))) return rc;
     if ((rc = grow(W.uniq_recv, t1 * sizeof(int32_t)))) return rc;
     if ((rc = grow(W.uniq_cnt, t1 * sizeof(int32_t)))) return rc;
     if ((rc = grow(W.n_unique, n1 * sizeof(int32_t)))) return rc;
@@ -721,19 +1047,37 @@ int prh_trace_counts(const prh_geometry *geo, int64_t p0, int64_t p1, int32_t ra
     unsigned long long counters[2] = {0, 0};
     if (total > 0) {
         PRH_HIP(hipEventRecord(W.ev[0], st));
-        if ((rc = launch_gen(g, (uint32_t)p0, (uint32_t)n_p, (uint32_t)rays_per_patch, W.seeds.as<uint64_t>(),
-                             (uint32_t)total, W.origins.as<double>(), W.directions.as<double>(), st)))
+        if ((rc = launch_gen_records(g, (uint32_t)p0, (uint32_t)n_p, (uint32_t)rays_per_patch,
+                                     W.seeds.as<uint64_t>(), (uint32_t)total, W.dda.as<DdaRec>(),
+                                     W.walls.as<WallRec>(), st)))
             return rc;
         PRH_HIP(hipEventRecord(W.ev[1], st));
-        if ((rc = launch_trace(g, W.origins.as<double>(), W.directions.as<double>(), (uint32_t)total,
-                               W.receiver.as<int32_t>(), W.counters.as<unsigned long long>(), st)))
+        unsigned long long *ctr = W.counters.as<unsigned long long>();
+        if ((rc = launch_trace(g, W.dda.as<DdaRec>(), W.walls.as<WallRec>(), (uint32_t)total, W.fin.as<uint2>(), ctr,
+                               st)))
             return rc;
         PRH_HIP(hipEventRecord(W.ev[2], st));
         const unsigned cgrid = (unsigned)(n_p < 65536 ? n_p : 65536);
-        const size_t lds = sizeof(int32_t) * (SRp2 + COUNT_THREADS);
-        k_count_patches<<<cgrid, COUNT_THREADS, lds, st>>>(W.receiver.as<int32_t>(), n_p, rays_per_patch, n_seeds,
-                                                           SRp2, W.uniq_recv.as<int32_t>(), W.uniq_cnt.as<int32_t>(),
-                                                           W.n_unique.as<int32_t>());
+        if (SRp2 <= 1024) {
+            // one wavefront per patch, keys in registers
+            const int epl = SRp2 <= 64 ? 1 : SRp2 / 64;
+            const int64_t blocks = (n_p + COUNT_THREADS / 64 - 1) / (COUNT_THREADS / 64);
+            const unsigned wgrid = (unsigned)(blocks < 65536 ? blocks : 65536);
+            auto kern = epl == 1    ? k_count_wave<1>
+                        : epl == 2  ? k_count_wave<2>
+                        : epl == 4  ? k_count_wave<4>
+                        : epl == 8  ? k_count_wave<8>
+                                    : k_count_wave<16>;
+            kern<<<wgrid, COUNT_THREADS, 0, st>>>(W.fin.as<uint2>(), view_of(g), n_p, rays_per_patch, n_seeds,
+                                                  W.uniq_recv.as<int32_t>(), W.uniq_cnt.as<int32_t>(),
+                                                  W.n_unique.as<int32_t>(), ctr + 1);
+        } else {
+            const size_t lds = sizeof(int32_t) * (SRp2 + COUNT_THREADS);
+            k_count_patches<<<cgrid, COUNT_THREADS, lds, st>>>(W.fin.as<uint2>(), view_of(g), n_p, rays_per_patch,
+                                                               n_seeds, SRp2, W.uniq_recv.as<int32_t>(),
+                                                               W.uniq_cnt.as<int32_t>(), W.n_unique.as<int32_t>(),
+                                                               ctr + 1);
+        }
         PRH_HIP(hipGetLastError());
         PRH_HIP(hipEventRecord(W.ev[3], st));
         PRH_HIP(hipMemcpyAsync(counters, W.counters.p, sizeof(counters), hipMemcpyDeviceToHost, st));

@@ -163,9 +163,9 @@ def tracer_variants() -> list[dict]:
     cur = ctypes.c_int32()
     check(lib().prh_tracer_variants(ctypes.byref(n), ctypes.byref(cur), None, None), "prh_tracer_variants")
     refill = np.zeros(n.value, np.int32)
-    coarse = np.zeros(n.value, np.int32)
-    check(lib().prh_tracer_variants(None, None, ptr(refill), ptr(coarse)), "prh_tracer_variants")
-    return [{"variant": v, "refill_min": int(refill[v]), "coarse": int(coarse[v]), "current": v == cur.value}
+    occ = np.zeros(n.value, np.int32)
+    check(lib().prh_tracer_variants(None, None, ptr(refill), ptr(occ)), "prh_tracer_variants")
+    return [{"variant": v, "refill_min": int(refill[v]), "occupancy": int(occ[v]), "current": v == cur.value}
             for v in range(n.value)]
 
 

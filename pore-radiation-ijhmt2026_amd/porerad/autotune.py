@@ -1,7 +1,7 @@
 """Launch-shape autotuner of the HIP tracer.
 
 The traversal kernel comes in a few compiled variants (refill threshold x
-register budget, see ``prh_tracer_variants``).  Which one is fastest depends
+occupancy test, see ``prh_tracer_variants``).  Which one is fastest depends
 on the geometry (mean ray length, divergence of ray lengths, how much of the
 pore space the LDS brick bitset clears) and on the ray
 count per patch.  ``tune_tracer`` times every variant on a sample of the
@@ -75,7 +75,7 @@ def tune_tracer(patches, rays_per_patch: int, seeds, *, sample_patches: int = 40
             first_hit_edges(patches, rays_per_patch, seeds, patch_range=(p0, p1)).close()
             best = min(best, _native.last_timings()[1])
             _ = time.perf_counter() - t
-        table.append({"variant": v["variant"], "refill_min": v["refill_min"], "coarse": v["coarse"],
+        table.append({"variant": v["variant"], "refill_min": v["refill_min"], "occupancy": v["occupancy"],
                       "trace_ms": best})
     winner = min(table, key=lambda row: row["trace_ms"])
     _native.set_tracer_variant(winner["variant"])
