@@ -62,6 +62,7 @@ typedef struct prh_geometry prh_geometry;
 typedef struct prh_edges prh_edges;
 typedef struct prh_operator prh_operator;
 typedef struct prh_batch prh_batch;
+typedef struct prh_coupled prh_coupled;
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *prh_last_error(void);
@@ -201,6 +202,67 @@ int prh_batch_result(prh_batch *b, int32_t slot, double *absorbed, double *resid
 /* Free a finished slot without reading its absorbed power. */
 int prh_batch_release(prh_batch *b, int32_t slot);
 void prh_batch_destroy(prh_batch *b);
+
+
+/* ---- strictly coupled radiation-conduction solve ----------------------
+ * Device state of the Newton-Krylov iteration of porerad/coupled.py:232-447:
+ * a pool of n_vectors cell vectors (n_cells = nx ny nz doubles each,
+ * addressed by slot index), the conduction matrix A (CSR of
+ * porerad/conduction.py:57-97 with sorted columns) and the wall response
+ * vectors b = hot T_hot + cold T_cold.  With a geometry and its exchange
+ * operator (both or neither) the handle also evaluates radiation; without,
+ * it serves the pure-conduction solve (conduction.py:100-120).  The caller
+ * keeps the control flow (Newton steps, line search, FGMRES least squares).
+ * All reductions have a fixed order.  The handle is bound to its geometry
+ * and operator and must be destroyed before them. */
+int prh_coupled_create(const prh_geometry *geometry, prh_operator *op, int64_t n_cells, const int64_t *A_rowptr,
+                       const int32_t *A_col, const double *A_val, const double *hot_rhs, const double *cold_rhs,
+                       int32_t n_vectors, prh_coupled **out);
+void prh_coupled_destroy(prh_coupled *c);
+/* Copy a host vector into slot v, or slot v to the host; fill with a value. */
+int prh_coupled_set(prh_coupled *c, int32_t v, const double *host);
+int prh_coupled_get(prh_coupled *c, int32_t v, double *host);
+int prh_coupled_fill(prh_coupled *c, int32_t v, double value);
+/* y = a x + b y (b == 0: y = a x); y = x / d. */
+int prh_coupled_axpby(prh_coupled *c, int32_t y, double a, int32_t x, double b);
+int prh_coupled_divide(prh_coupled *c, int32_t y, int32_t x, double d);
+/* x . y; statistics of x: out[0] ||x||_2, [1] ||x||_inf, [2] min, [3] max,
+ * [4] number of non-finite values (excluded from the others). */
+int prh_coupled_dot(prh_coupled *c, int32_t x, int32_t y, double *out);
+int prh_coupled_stats(prh_coupled *c, int32_t x, double *out);
+/* y = A x. */
+int prh_coupled_matvec(prh_coupled *c, int32_t x, int32_t y);
+/* Net radiative cell power Q of the field in slot T: walls at T_hot/T_cold,
+ * uniform emissivity, FP64 reflection series to rtol (coupled.py:86-105,
+ * 45-59).  The patch state stays in the handle for prh_coupled_state. */
+int prh_coupled_radiation(prh_coupled *c, int32_t T, double T_hot_K, double T_cold_K, double emissivity,
+                          double relative_tolerance, int32_t Q, double *series_residual, int32_t *series_iterations);
+/* R = A T - b - Q (coupled.py:362-366). */
+int prh_coupled_residual(prh_coupled *c, int32_t T, int32_t Q, double T_hot_K, double T_cold_K, int32_t R);
+/* Fix the patch temperatures of the field in slot T as the linearisation
+ * point of prh_coupled_jacobian. */
+int prh_coupled_linearize(prh_coupled *c, int32_t T, double T_hot_K, double T_cold_K);
+/* out = A v - dQ/dT v with the walls fixed (coupled.py:108-126, 380-383). */
+int prh_coupled_jacobian(prh_coupled *c, int32_t v, double emissivity, double relative_tolerance, int32_t out,
+                         int32_t *series_iterations);
+/* Fixed-step Jacobi-PCG on A + diag(emission derivative at the field in
+ * slot T), started from zero (coupled.py:292-315): set, then apply. */
+int prh_coupled_set_preconditioner(prh_coupled *c, int32_t T, double emissivity, int32_t steps);
+int prh_coupled_precondition(prh_coupled *c, int32_t rhs, int32_t out);
+/* Two modified Gram-Schmidt passes of w against slots basis .. basis+count-1
+ * (count <= 60): h[i] = sum of both passes' coefficients, h[count] = ||w||_2
+ * (coupled.py:262-268). */
+int prh_coupled_orthogonalize(prh_coupled *c, int32_t basis, int32_t count, int32_t w, double *h);
+/* out = x + sum_i y[i] v_{basis+i} (count <= 256). */
+int prh_coupled_combine(prh_coupled *c, int32_t out, int32_t x, int32_t basis, int32_t count, const double *y);
+/* Jacobi-preconditioned CG for A x = b from the x given, until
+ * ||b - A x||_2 <= rtol ||b||_2 (conduction.py:100-120); the iterations and
+ * the true relative residual come back. */
+int prh_coupled_cg(prh_coupled *c, int32_t b, int32_t x, double relative_tolerance, int32_t max_iterations,
+                   int32_t *iterations, double *relative_residual);
+/* Patch temperatures, emission and absorbed power of the last
+ * prh_coupled_radiation (n_patches each; NULL skips). */
+int prh_coupled_state(prh_coupled *c, double *patch_temperature, double *emitted, double *absorbed);
 
 #ifdef __cplusplus
 }

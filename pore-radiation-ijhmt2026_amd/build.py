@@ -30,6 +30,7 @@ SOURCES = [
     "prh_tracer.hip",
     "prh_operator.hip",
     "prh_batch.hip",
+    "prh_coupled.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -358,6 +358,49 @@ int apply_device(prh_operator *op, int mode, const double *dx, double *dy, hipSt
 
 }  // namespace
 
+namespace prh {
+
+int series_fp64(prh_operator *op, double relative_tolerance, int32_t max_reflections, hipStream_t st,
+                double *residual, int32_t *iterations)
+{
+    const int64_t n = op->n;
+    const unsigned rb = blocks_for(n, ROWS_PER_BLOCK);
+    const int64_t n_part = rb;
+    const unsigned ib = (unsigned)std::min<int64_t>(blocks_for(n, RED_THREADS), 1024);
+    k_series_init<<<ib, RED_THREADS, 0, st>>>(op->emit, op->scale, op->area, n, op->flight, op->absorbed, op->w0,
+                                              op->part_abs);
+    k_series_start<<<1, RED_THREADS, 0, st>>>(op->part_abs, (int)ib, op->state);
+    PRH_HIP(hipGetLastError());
+    int32_t it = 0;
+    SeriesState hs{};
+    double *w_in = op->w0, *w_out = op->w1;
+    while (it < max_reflections) {
+        const int batch = std::min<int32_t>(16, max_reflections - it);
+        for (int b = 0; b < batch; ++b) {
+            k_series_step<<<rb, ROW_THREADS, 0, st>>>(op->rowptr, op->col, op->val, n, op->scale, op->area, op->eps,
+                                                      w_in, w_out, op->absorbed, op->flight, op->part_abs,
+                                                      op->part_sum, op->state);
+            k_series_check<<<1, RED_THREADS, 0, st>>>(op->part_abs, op->part_sum, n_part, relative_tolerance,
+                                                      op->state);
+            std::swap(w_in, w_out);
+        }
+        PRH_HIP(hipGetLastError());
+        it += batch;
+        PRH_HIP(hipMemcpyAsync(&hs, op->state, sizeof(hs), hipMemcpyDeviceToHost, st));
+        PRH_HIP(hipStreamSynchronize(st));
+        if (hs.done) break;
+    }
+    if (!hs.done) {
+        set_error("radiosity reflection series did not converge");
+        return PRH_ERR_SERIES;
+    }
+    *residual = hs.residual;
+    *iterations = hs.iterations;
+    return PRH_OK;
+}
+
+}  // namespace prh
+
 extern "C" {
 
 int prh_operator_create(int64_t n, const int32_t *origin, const int32_t *receiver, const int64_t *count,
@@ -522,46 +565,19 @@ int prh_radiosity(prh_operator *op, int32_t n_rhs, const double *emitted, const 
     int rc = current_stream(&st);
     if (rc) return rc;
     const int64_t n = op->n;
-    const unsigned rb = blocks_for(n, ROWS_PER_BLOCK);
-    const int64_t n_part = rb;
-    const unsigned ib = (unsigned)std::min<int64_t>(blocks_for(n, RED_THREADS), 1024);
     for (int32_t q = 0; q < n_rhs; ++q) {
         PRH_HIP(hipMemcpyAsync(op->emit, emitted + (int64_t)q * n, sizeof(double) * n, hipMemcpyHostToDevice, st));
         if (q == 0 || emissivity_per_rhs)
             PRH_HIP(hipMemcpyAsync(op->eps, emissivity + (emissivity_per_rhs ? (int64_t)q * n : 0),
                                    sizeof(double) * n, hipMemcpyHostToDevice, st));
-        k_series_init<<<ib, RED_THREADS, 0, st>>>(op->emit, op->scale, op->area, n, op->flight, op->absorbed,
-                                                  op->w0, op->part_abs);
-        k_series_start<<<1, RED_THREADS, 0, st>>>(op->part_abs, (int)ib, op->state);
-        PRH_HIP(hipGetLastError());
-        int32_t it = 0;
-        SeriesState hs{};
-        double *w_in = op->w0, *w_out = op->w1;
-        while (it < max_reflections) {
-            const int batch = std::min<int32_t>(16, max_reflections - it);
-            for (int b = 0; b < batch; ++b) {
-                k_series_step<<<rb, ROW_THREADS, 0, st>>>(op->rowptr, op->col, op->val, n, op->scale, op->area,
-                                                          op->eps, w_in, w_out, op->absorbed, op->flight,
-                                                          op->part_abs, op->part_sum, op->state);
-                k_series_check<<<1, RED_THREADS, 0, st>>>(op->part_abs, op->part_sum, n_part, relative_tolerance,
-                                                          op->state);
-                std::swap(w_in, w_out);
-            }
-            PRH_HIP(hipGetLastError());
-            it += batch;
-            PRH_HIP(hipMemcpyAsync(&hs, op->state, sizeof(hs), hipMemcpyDeviceToHost, st));
-            PRH_HIP(hipStreamSynchronize(st));
-            if (hs.done) break;
-        }
-        if (!hs.done) {
-            set_error("radiosity reflection series did not converge");
-            return PRH_ERR_SERIES;
-        }
+        double res = 0.0;
+        int32_t its = 0;
+        if ((rc = prh::series_fp64(op, relative_tolerance, max_reflections, st, &res, &its))) return rc;
         PRH_HIP(hipMemcpyAsync(absorbed + (int64_t)q * n, op->absorbed, sizeof(double) * n, hipMemcpyDeviceToHost,
                                st));
         PRH_HIP(hipStreamSynchronize(st));
-        residual[q] = hs.residual;
-        iterations[q] = hs.iterations;
+        residual[q] = res;
+        iterations[q] = its;
     }
     return PRH_OK;
 }

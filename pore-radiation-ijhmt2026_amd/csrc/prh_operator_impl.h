@@ -57,3 +57,11 @@ extern "C" int prh_radiosity_mixed(prh_operator *op, int32_t n_rhs, const double
                                    int32_t emissivity_per_rhs, double relative_tolerance, double *absorbed,
                                    double *residual, int32_t *iterations);
 
+
+namespace prh {
+// FP64 reflection series of the emission in op->emit with emissivity op->eps
+// (device buffers, n each); the absorbed power is left in op->absorbed.
+// PRH_ERR_SERIES when more than max_reflections are needed.
+int series_fp64(prh_operator *op, double relative_tolerance, int32_t max_reflections, hipStream_t st,
+                double *residual, int32_t *iterations);
+}  // namespace prh

@@ -382,8 +382,9 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 // wavefront generates GEN_ROUNDS x 64 consecutive rays; lane l takes rays
 // base + l + 64 k, so every round stores 64 adjacent records (coalesced) and
 // a lane's next ray is 256 draws further on its stream: one jump with the
-// fixed 2^8 table entry instead of a full jump-ahead.
-constexpr uint32_t GEN_ROUNDS = 16;
+// fixed 2^8 table entry instead of a full jump-ahead, which each lane makes
+// once (and again when its rays cross into the next seed's block).
+constexpr uint32_t GEN_ROUNDS = 64;
 
 struct GenArgs {
     GeoView G;
@@ -409,16 +410,27 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_gen_rays(GenArgs a)
     const uint64_t base = wave * (64ull * GEN_ROUNDS) + lane;
     const uint32_t seg = a.n_p * a.R;
     const double fny = (double)G.ny, fnz = (double)G.nz;
+    if (base >= a.total) return;
+    // (seed, patch, ray) of the lane's first ray; later rays are 64 further on
+    uint32_t s = (uint32_t)base / seg;
+    uint32_t q = ((uint32_t)base - s * seg) / a.R;
+    uint32_t r = (uint32_t)base - s * seg - q * a.R;
     uint32_t s_cur = 0xFFFFFFFFu;
     u128 st = make_u128(0, 0), inc = make_u128(0, 0), step_inc = make_u128(0, 0);
     for (uint32_t k = 0; k < GEN_ROUNDS; ++k) {
         const uint64_t g64 = base + 64ull * k;
         if (g64 >= a.total) break;
         const uint32_t g = (uint32_t)g64;
-        const uint32_t s = g / seg;
-        const uint32_t rem = g - s * seg;
-        const uint32_t q = rem / a.R;
-        const uint32_t r = rem - q * a.R;
+        if (k > 0) {
+            r += 64u;
+            while (r >= a.R) {
+                r -= a.R;
+                if (++q == a.n_p) {
+                    q = 0;
+                    ++s;
+                }
+            }
+        }
         const uint32_t p = a.p0 + q;
         if (s != s_cur) {
             // first ray of this lane, or the stride crossed into the next seed

@@ -70,6 +70,26 @@ SIGNATURES = {
     "prh_batch_release": ([_P, _I32], ctypes.c_int),
     "prh_batch_result": ([_P, _I32, _P, _P, _P], ctypes.c_int),
     "prh_batch_destroy": ([_P], None),
+    "prh_coupled_create": ([_P, _P, _I64, _P, _P, _P, _P, _P, _I32, _P], ctypes.c_int),
+    "prh_coupled_destroy": ([_P], None),
+    "prh_coupled_set": ([_P, _I32, _P], ctypes.c_int),
+    "prh_coupled_get": ([_P, _I32, _P], ctypes.c_int),
+    "prh_coupled_fill": ([_P, _I32, _F64], ctypes.c_int),
+    "prh_coupled_axpby": ([_P, _I32, _F64, _I32, _F64], ctypes.c_int),
+    "prh_coupled_divide": ([_P, _I32, _I32, _F64], ctypes.c_int),
+    "prh_coupled_dot": ([_P, _I32, _I32, _P], ctypes.c_int),
+    "prh_coupled_stats": ([_P, _I32, _P], ctypes.c_int),
+    "prh_coupled_matvec": ([_P, _I32, _I32], ctypes.c_int),
+    "prh_coupled_radiation": ([_P, _I32, _F64, _F64, _F64, _F64, _I32, _P, _P], ctypes.c_int),
+    "prh_coupled_residual": ([_P, _I32, _I32, _F64, _F64, _I32], ctypes.c_int),
+    "prh_coupled_linearize": ([_P, _I32, _F64, _F64], ctypes.c_int),
+    "prh_coupled_jacobian": ([_P, _I32, _F64, _F64, _I32, _P], ctypes.c_int),
+    "prh_coupled_set_preconditioner": ([_P, _I32, _F64, _I32], ctypes.c_int),
+    "prh_coupled_precondition": ([_P, _I32, _I32], ctypes.c_int),
+    "prh_coupled_orthogonalize": ([_P, _I32, _I32, _I32, _P], ctypes.c_int),
+    "prh_coupled_combine": ([_P, _I32, _I32, _I32, _I32, _P], ctypes.c_int),
+    "prh_coupled_cg": ([_P, _I32, _I32, _F64, _I32, _P, _P], ctypes.c_int),
+    "prh_coupled_state": ([_P, _P, _P, _P], ctypes.c_int),
 }
 
 _lock = threading.Lock()
@@ -397,6 +417,132 @@ class Batch:
     def close(self) -> None:
         if getattr(self, "handle", None) is not None and self.handle.value:
             self._lib.prh_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Coupled:
+    """Owner of the device state of a coupled solve (``prh_coupled``): cell-vector slots + conduction matrix.
+
+    ``geometry`` and ``operator`` (both or neither) enable the radiation
+    operations; without them the handle serves the pure-conduction solve.
+    """
+
+    def __init__(self, matrix, hot_rhs, cold_rhs, n_vectors: int, geometry: Geometry | None = None,
+                 operator: Operator | None = None) -> None:
+        self._lib = lib()
+        rowptr = np.ascontiguousarray(matrix.indptr, dtype=np.int64)
+        col = np.ascontiguousarray(matrix.indices, dtype=np.int32)
+        val = np.ascontiguousarray(matrix.data, dtype=np.float64)
+        hot = np.ascontiguousarray(hot_rhs, dtype=np.float64)
+        cold = np.ascontiguousarray(cold_rhs, dtype=np.float64)
+        self.n = int(matrix.shape[0])
+        if hot.shape != (self.n,) or cold.shape != (self.n,):
+            raise ValueError("wall response vectors must have one value per cell")
+        self.n_patches = operator.n if operator is not None else 0
+        handle = ctypes.c_void_p()
+        check(self._lib.prh_coupled_create(geometry.handle if geometry is not None else None,
+                                           operator.handle if operator is not None else None, self.n, ptr(rowptr),
+                                           ptr(col), ptr(val), ptr(hot), ptr(cold), int(n_vectors),
+                                           ctypes.byref(handle)), "prh_coupled_create")
+        self.handle = handle
+        self.geometry, self.operator = geometry, operator  # kept alive while the handle uses them
+
+    def set(self, v: int, x: np.ndarray) -> None:
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        if x.size != self.n:
+            raise ValueError(f"expected {self.n} values")
+        check(self._lib.prh_coupled_set(self.handle, int(v), ptr(x)), "prh_coupled_set")
+
+    def get(self, v: int) -> np.ndarray:
+        out = np.empty(self.n, np.float64)
+        check(self._lib.prh_coupled_get(self.handle, int(v), ptr(out)), "prh_coupled_get")
+        return out
+
+    def fill(self, v: int, value: float) -> None:
+        check(self._lib.prh_coupled_fill(self.handle, int(v), float(value)), "prh_coupled_fill")
+
+    def axpby(self, y: int, a: float, x: int, b: float) -> None:
+        check(self._lib.prh_coupled_axpby(self.handle, int(y), float(a), int(x), float(b)), "prh_coupled_axpby")
+
+    def divide(self, y: int, x: int, d: float) -> None:
+        check(self._lib.prh_coupled_divide(self.handle, int(y), int(x), float(d)), "prh_coupled_divide")
+
+    def dot(self, x: int, y: int) -> float:
+        out = ctypes.c_double()
+        check(self._lib.prh_coupled_dot(self.handle, int(x), int(y), ctypes.byref(out)), "prh_coupled_dot")
+        return float(out.value)
+
+    def stats(self, x: int) -> tuple[float, float, float, float, int]:
+        """(||x||_2, ||x||_inf, min, max, non-finite count)."""
+
+        out = np.zeros(5, np.float64)
+        check(self._lib.prh_coupled_stats(self.handle, int(x), ptr(out)), "prh_coupled_stats")
+        return float(out[0]), float(out[1]), float(out[2]), float(out[3]), int(out[4])
+
+    def matvec(self, x: int, y: int) -> None:
+        check(self._lib.prh_coupled_matvec(self.handle, int(x), int(y)), "prh_coupled_matvec")
+
+    def radiation(self, T: int, T_hot: float, T_cold: float, eps: float, rtol: float, Q: int) -> tuple[float, int]:
+        res, its = ctypes.c_double(), ctypes.c_int32()
+        check(self._lib.prh_coupled_radiation(self.handle, int(T), float(T_hot), float(T_cold), float(eps),
+                                              float(rtol), int(Q), ctypes.byref(res), ctypes.byref(its)),
+              "prh_coupled_radiation")
+        return float(res.value), int(its.value)
+
+    def residual(self, T: int, Q: int, T_hot: float, T_cold: float, R: int) -> None:
+        check(self._lib.prh_coupled_residual(self.handle, int(T), int(Q), float(T_hot), float(T_cold), int(R)),
+              "prh_coupled_residual")
+
+    def linearize(self, T: int, T_hot: float, T_cold: float) -> None:
+        check(self._lib.prh_coupled_linearize(self.handle, int(T), float(T_hot), float(T_cold)),
+              "prh_coupled_linearize")
+
+    def jacobian(self, v: int, eps: float, rtol: float, out: int) -> int:
+        its = ctypes.c_int32()
+        check(self._lib.prh_coupled_jacobian(self.handle, int(v), float(eps), float(rtol), int(out),
+                                             ctypes.byref(its)), "prh_coupled_jacobian")
+        return int(its.value)
+
+    def set_preconditioner(self, T: int, eps: float, steps: int) -> None:
+        check(self._lib.prh_coupled_set_preconditioner(self.handle, int(T), float(eps), int(steps)),
+              "prh_coupled_set_preconditioner")
+
+    def precondition(self, rhs: int, out: int) -> None:
+        check(self._lib.prh_coupled_precondition(self.handle, int(rhs), int(out)), "prh_coupled_precondition")
+
+    def orthogonalize(self, basis: int, count: int, w: int) -> np.ndarray:
+        h = np.zeros(count + 1, np.float64)
+        check(self._lib.prh_coupled_orthogonalize(self.handle, int(basis), int(count), int(w), ptr(h)),
+              "prh_coupled_orthogonalize")
+        return h
+
+    def combine(self, out: int, x: int, basis: int, y: np.ndarray) -> None:
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        check(self._lib.prh_coupled_combine(self.handle, int(out), int(x), int(basis), int(y.size), ptr(y)),
+              "prh_coupled_combine")
+
+    def cg(self, b: int, x: int, rtol: float, max_iterations: int) -> tuple[int, float]:
+        its, rel = ctypes.c_int32(), ctypes.c_double()
+        check(self._lib.prh_coupled_cg(self.handle, int(b), int(x), float(rtol), int(max_iterations),
+                                       ctypes.byref(its), ctypes.byref(rel)), "prh_coupled_cg")
+        return int(its.value), float(rel.value)
+
+    def state(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(patch temperatures, emitted, absorbed) of the last radiation evaluation."""
+
+        Tp, E, A = (np.empty(self.n_patches, np.float64) for _ in range(3))
+        check(self._lib.prh_coupled_state(self.handle, ptr(Tp), ptr(E), ptr(A)), "prh_coupled_state")
+        return Tp, E, A
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self._lib.prh_coupled_destroy(self.handle)
             self.handle = None
 
     def __del__(self):  # pragma: no cover

@@ -13,7 +13,6 @@ from dataclasses import dataclass
 
 import numpy as np
 from scipy.sparse import coo_matrix, csr_matrix
-from scipy.sparse.linalg import LinearOperator, cg
 
 
 @dataclass
@@ -93,17 +92,26 @@ def assemble_conduction(conductivity_x: np.ndarray, H_m: float) -> ConductionSys
 
 
 def solve_conduction(system: ConductionSystem, T_hot_K: float, T_cold_K: float, *, rtol: float = 1.0e-11) -> np.ndarray:
-    """Pure-conduction field by Jacobi-preconditioned CG from the linear profile (conduction.py:100-120)."""
+    """Pure-conduction field by Jacobi-preconditioned CG from the linear profile (conduction.py:100-120).
 
-    A = system.matrix
+    Runs on the device (``prh_coupled_cg``): ||b - A T||_2 <= rtol ||b||_2.
+    """
+
+    from . import _native
+
     rhs = system.boundary_rhs(T_hot_K, T_cold_K)
     nx, ny, nz = system.shape
     profile = float(T_hot_K) + (float(T_cold_K) - float(T_hot_K)) * ((np.arange(nx, dtype=np.float64) + 0.5) / nx)
     x0 = np.broadcast_to(profile[:, None, None], (nx, ny, nz)).reshape(-1).copy()
-    inv_diag = 1.0 / A.diagonal()
-    jacobi = LinearOperator(A.shape, matvec=lambda v: inv_diag * v, dtype=np.float64)
-    T, info = cg(A, rhs, x0=x0, rtol=float(rtol), atol=0.0, maxiter=max(50000, 20 * nx), M=jacobi)
-    relative = float(np.linalg.norm(A @ T - rhs) / np.linalg.norm(rhs))
-    if info != 0 or relative > 1.0e-9:
-        raise RuntimeError(f"conduction solve did not converge: info={info}, residual={relative:.3e}")
+    max_iterations = max(50000, 20 * nx)
+    D = _native.Coupled(system.matrix, system.hot_rhs, system.cold_rhs, 2)
+    try:
+        D.set(0, rhs)
+        D.set(1, x0)
+        iterations, relative = D.cg(0, 1, float(rtol), max_iterations)
+        T = D.get(1)
+    finally:
+        D.close()
+    if iterations >= max_iterations or not relative <= 1.0e-9:
+        raise RuntimeError(f"conduction solve did not converge: iterations={iterations}, residual={relative:.3e}")
     return T.reshape(system.shape)

@@ -8,6 +8,12 @@ flexible GMRES, right-preconditioned by a fixed number of Jacobi-PCG steps on
 A + diag(emission derivative), and a backtracking line search that keeps the
 field between the wall temperatures.  Every radiosity evaluation and JVP is a
 device reflection series (porerad.exchange).
+
+``newton_krylov`` keeps its cell vectors on the device (``prh_coupled``):
+conduction products, radiation, JVPs, the preconditioner and the
+Gram-Schmidt passes run there, and only scalars and the small FGMRES
+least-squares problems come to the host.  ``flexible_gmres`` and
+``FixedJacobiPCG`` remain the reference's generic host utilities.
 """
 
 from __future__ import annotations
@@ -16,6 +22,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from . import _native
 from .conduction import ConductionSystem
 from .constants import SIGMA
 from .exchange import ExchangeOperator, RadiosityResult, net_patch_power, net_patch_power_jvp
@@ -331,6 +338,68 @@ class CoupledResult:
         )
 
 
+# Device slots of one Newton-Krylov solve: FGMRES basis V (restart + 1) and
+# preconditioned vectors Z (restart), then the named cell vectors.
+_V0 = 0
+_Z0 = GMRES_RESTART + 1
+(_T, _Q, _R, _TRIAL_T, _TRIAL_Q, _TRIAL_R, _STEP, _RHS, _TMP, _W, _CAND, _DIFF) = range(
+    _Z0 + GMRES_RESTART, _Z0 + GMRES_RESTART + 12)
+_N_SLOTS = _Z0 + GMRES_RESTART + 12
+
+
+def _device_fgmres(D: _native.Coupled, jacobian, rhs: int, x: int, *, restart: int, max_iterations: int,
+                   relative_tolerance: float, absolute_tolerance: float) -> tuple[bool, int, float]:
+    """flexible_gmres on device slots (same steps; solution in slot x): (converged, iterations, relative residual)."""
+
+    b_norm = max(D.stats(rhs)[0], _TINY)
+    goal = max(float(absolute_tolerance), float(relative_tolerance) * b_norm)
+    eps = np.finfo(np.float64).eps
+    D.fill(x, 0.0)
+    done = 0
+
+    def residual_norm(v: int) -> float:
+        jacobian(v, _TMP)
+        D.axpby(_TMP, 1.0, rhs, -1.0)  # b - J v
+        return D.stats(_TMP)[0]
+
+    while done < max_iterations:
+        beta = residual_norm(x)
+        if beta <= goal:
+            return True, done, beta / b_norm
+        m = min(restart, max_iterations - done)
+        H = np.zeros((m + 1, m))
+        g = np.zeros(m + 1)
+        D.divide(_V0, _TMP, beta)
+        g[0] = beta
+        restarted = False
+        for j in range(m):
+            D.precondition(_V0 + j, _Z0 + j)
+            jacobian(_Z0 + j, _W)
+            h = D.orthogonalize(_V0, j + 1, _W)
+            H[: j + 1, j] = h[: j + 1]
+            h_next = float(h[j + 1])
+            H[j + 1, j] = h_next
+            if h_next > eps * beta:
+                D.divide(_V0 + j + 1, _W, h_next)
+            Hj = H[: j + 2, : j + 1]
+            y = np.linalg.lstsq(Hj, g[: j + 2], rcond=None)[0]
+            estimate = float(np.linalg.norm(g[: j + 2] - Hj @ y))
+            done += 1
+            if estimate <= goal or h_next <= eps * beta:
+                D.combine(_CAND, x, _Z0, y)
+                true_norm = residual_norm(_CAND)
+                D.axpby(x, 1.0, _CAND, 0.0)
+                if true_norm <= goal:
+                    return True, done, true_norm / b_norm
+                restarted = True
+                break
+        if restarted:
+            continue
+        D.combine(x, x, _Z0, np.linalg.lstsq(H, g, rcond=None)[0])
+    true_norm = residual_norm(x)
+    return true_norm <= goal, done, true_norm / b_norm
+
+
 def newton_krylov(
     system: ConductionSystem,
     operator: ExchangeOperator,
@@ -348,69 +417,92 @@ def newton_krylov(
     trial fields outside [T_c, T_h] are rejected by the line search.
     """
 
-    A = system.matrix
-    b = system.boundary_rhs(T_hot_K, T_cold_K)
-    lower, upper = float(T_cold_K), float(T_hot_K)
-    pcg = FixedJacobiPCG(system, emission_diagonal(patches, T_initial, epsilon), preconditioner_steps)
+    D = _native.Coupled(system.matrix, system.hot_rhs, system.cold_rhs, _N_SLOTS, patches.geometry(),
+                        operator.device)
+    try:
+        return _newton_krylov_device(D, system, patches, T_hot_K, T_cold_K, epsilon, T_initial,
+                                     preconditioner_steps)
+    finally:
+        D.close()
 
-    def radiation(T: np.ndarray, rtol: float) -> RadiationState:
-        return evaluate_radiation(operator, patches, T, T_hot_K, T_cold_K, epsilon, relative_tolerance=rtol)
 
-    T = np.asarray(T_initial, dtype=np.float64).copy()
-    state = radiation(T, RADIOSITY_RTOL_NEWTON)
-    residual = A @ T.ravel() - b - state.cell_net_power_W.ravel()
+def _newton_krylov_device(D: _native.Coupled, system: ConductionSystem, patches: SurfacePatches, T_hot_K: float,
+                          T_cold_K: float, epsilon: float, T_initial: np.ndarray,
+                          preconditioner_steps: int) -> CoupledResult:
+    Th, Tc, eps = float(T_hot_K), float(T_cold_K), float(epsilon)
+    lower, upper = Tc, Th
+    T, Q, R = _T, _Q, _R
+    TT, TQ, TR = _TRIAL_T, _TRIAL_Q, _TRIAL_R
+    D.set(T, np.asarray(T_initial, dtype=np.float64))
+    D.set_preconditioner(T, eps, preconditioner_steps)
+    D.radiation(T, Th, Tc, eps, RADIOSITY_RTOL_NEWTON, Q)
+    D.residual(T, Q, Th, Tc, R)
     target = NEWTON_RESIDUAL_W
-    last_step = 0.0 if max(float(np.linalg.norm(residual, np.inf)), _TINY) <= target else float("inf")
-    result = CoupledResult(T, state, False, 0, preconditioner_steps=int(preconditioner_steps))
+    last_step = 0.0 if max(D.stats(R)[1], _TINY) <= target else float("inf")
+    result = CoupledResult(np.asarray(T_initial, dtype=np.float64), None, False, 0,
+                           preconditioner_steps=int(preconditioner_steps))
 
     def finish(converged: bool, failure: str = "") -> CoupledResult:
-        final = radiation(T, RADIOSITY_RTOL_FINAL)
-        result.temperature_K, result.state, result.converged, result.failure = T, final, converged, failure
-        result.final_residual_inf_W = float(np.linalg.norm(A @ T.ravel() - b - final.cell_net_power_W.ravel(), np.inf))
+        series_residual, series_iterations = D.radiation(T, Th, Tc, eps, RADIOSITY_RTOL_FINAL, Q)
+        T_patch, emitted, absorbed = D.state()
+        net = absorbed - emitted
+        radiosity = RadiosityResult(emitted, absorbed, net, series_residual, series_iterations)
+        cell = D.get(Q).reshape(system.shape)
+        D.residual(T, Q, Th, Tc, R)
+        result.temperature_K = D.get(T).reshape(system.shape)
+        result.state = RadiationState(T_patch, net, cell, radiosity)
+        result.converged, result.failure = converged, failure
+        result.final_residual_inf_W = D.stats(R)[1]
         return result
 
+    def jacobian(v: int, out: int) -> None:
+        D.jacobian(v, eps, RADIOSITY_RTOL_NEWTON, out)
+
     for _ in range(NEWTON_MAX_ITERATIONS):
-        r_norm = float(np.linalg.norm(residual, np.inf))
+        r_norm = D.stats(R)[1]
         if r_norm <= target and last_step <= NEWTON_STEP_K:
             return finish(True)
-        T_patch = state.patch_temperature_K.copy()
-
-        def jacobian(v: np.ndarray) -> np.ndarray:
-            jvp = radiation_jvp(operator, patches, T_patch, v.reshape(system.shape), epsilon,
-                                relative_tolerance=RADIOSITY_RTOL_NEWTON)
-            return A @ v - jvp.ravel()
-
-        gm = flexible_gmres(jacobian, -residual, pcg, restart=GMRES_RESTART, max_iterations=GMRES_MAX_ITERATIONS,
-                            relative_tolerance=GMRES_RTOL, absolute_tolerance=GMRES_ATOL)
-        step = gm.solution
+        D.linearize(T, Th, Tc)
+        D.axpby(_RHS, -1.0, R, 0.0)
+        converged, iterations, relative = _device_fgmres(
+            D, jacobian, _RHS, _STEP, restart=GMRES_RESTART, max_iterations=GMRES_MAX_ITERATIONS,
+            relative_tolerance=GMRES_RTOL, absolute_tolerance=GMRES_ATOL)
+        jacobian(_STEP, _TMP)
+        D.axpby(_TMP, 1.0, R, 1.0)  # J step + residual
         result.linear_solves.append({
-            "converged": gm.converged,
-            "gmres_iterations": gm.iterations,
-            "rhs_l2_W": float(np.linalg.norm(residual)),
-            "true_residual_l2_W": float(np.linalg.norm(jacobian(step) + residual)),
-            "true_residual_relative": gm.true_residual_relative,
+            "converged": converged,
+            "gmres_iterations": iterations,
+            "rhs_l2_W": D.stats(R)[0],
+            "true_residual_l2_W": D.stats(_TMP)[0],
+            "true_residual_relative": relative,
         })
-        if not gm.converged:
+        if not converged:
             return finish(False, "linear solve did not converge")
-        if r_norm <= target and float(np.max(np.abs(step))) <= NEWTON_STEP_K:
+        if r_norm <= target and D.stats(_STEP)[1] <= NEWTON_STEP_K:
             return finish(True)
 
         alpha, accepted = 1.0, False
+        trial_norm = float("inf")
         while alpha >= 2.0**-20:
-            candidate = T.ravel() + alpha * step
-            if np.all(np.isfinite(candidate)) and float(np.min(candidate)) >= lower and float(np.max(candidate)) <= upper:
-                trial = candidate.reshape(system.shape)
-                trial_state = radiation(trial, RADIOSITY_RTOL_NEWTON)
-                trial_residual = A @ candidate - b - trial_state.cell_net_power_W.ravel()
-                trial_norm = float(np.linalg.norm(trial_residual, np.inf))
+            D.axpby(TT, 1.0, T, 0.0)
+            D.axpby(TT, alpha, _STEP, 1.0)  # T + alpha step
+            _, _, cmin, cmax, bad = D.stats(TT)
+            if bad == 0 and cmin >= lower and cmax <= upper:
+                D.radiation(TT, Th, Tc, eps, RADIOSITY_RTOL_NEWTON, TQ)
+                D.residual(TT, TQ, Th, Tc, TR)
+                trial_norm = D.stats(TR)[1]
                 if trial_norm <= (1.0 - 1.0e-4 * alpha) * r_norm:
                     accepted = True
                     break
             alpha *= 0.5
         if not accepted:
             return finish(False, "line search failed")
-        last_step = float(np.max(np.abs(trial - T)))
-        T, state, residual = trial, trial_state, trial_residual
+        D.axpby(_DIFF, 1.0, TT, 0.0)
+        D.axpby(_DIFF, -1.0, T, 1.0)
+        last_step = D.stats(_DIFF)[1]
+        T, TT = TT, T
+        Q, TQ = TQ, Q
+        R, TR = TR, R
         result.newton_iterations += 1
         if trial_norm <= target and last_step <= NEWTON_STEP_K:
             return finish(True)
