@@ -213,8 +213,11 @@ void prh_batch_destroy(prh_batch *b);
  * operator (both or neither) the handle also evaluates radiation; without,
  * it serves the pure-conduction solve (conduction.py:100-120).  The caller
  * keeps the control flow (Newton steps, line search, FGMRES least squares).
- * All reductions have a fixed order.  The handle is bound to its geometry
- * and operator and must be destroyed before them. */
+ * All reductions have a fixed order.  The handle works on a stream of its
+ * own, so it runs beside batch engines on the same operator; it uses the
+ * operator's series buffers, so FP64 prh_radiosity calls on that operator
+ * must not overlap with it.  The handle is bound to its geometry and
+ * operator and must be destroyed before them. */
 int prh_coupled_create(const prh_geometry *geometry, prh_operator *op, int64_t n_cells, const int64_t *A_rowptr,
                        const int32_t *A_col, const double *A_val, const double *hot_rhs, const double *cold_rhs,
                        int32_t n_vectors, prh_coupled **out);

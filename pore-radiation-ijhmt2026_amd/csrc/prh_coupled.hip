@@ -348,6 +348,7 @@ using namespace prh;
 
 struct prh_coupled {
     int device = 0;
+    hipStream_t stream = nullptr;  // its own stream: runs beside the library stream
     const prh_geometry *geo = nullptr;
     prh_operator *op = nullptr;
     int64_t n_cells = 0, n_patches = 0;
@@ -377,12 +378,14 @@ void free_coupled(prh_coupled *c)
                     c->shift, c->inv, c->Tp, c->Tlin, c->dE, c->emitted, c->part, c->scal, c->ycoef};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
 int stream_of(prh_coupled *c, hipStream_t *st)
 {
     PRH_HIP(hipSetDevice(c->device));
-    return current_stream(st);
+    *st = c->stream;
+    return PRH_OK;
 }
 
 // scal[k] = x . y
@@ -462,6 +465,7 @@ int prh_coupled_create(const prh_geometry *geo, prh_operator *op, int64_t n_cell
     alloc((void **)&c->part, (size_t)RED_BLOCKS * 5 * 8);
     alloc((void **)&c->scal, 128 * 8);
     alloc((void **)&c->ycoef, 256 * 8);
+    if (ok && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) ok = false;
     if (!ok) {
         free_coupled(c);
         delete c;
@@ -478,7 +482,8 @@ int prh_coupled_create(const prh_geometry *geo, prh_operator *op, int64_t n_cell
     PRH_HIP(hipMemcpy(c->diag, diag.data(), (size_t)n * 8, hipMemcpyHostToDevice));
     PRH_HIP(hipMemcpy(c->hot, hot_rhs, (size_t)n * 8, hipMemcpyHostToDevice));
     PRH_HIP(hipMemcpy(c->cold, cold_rhs, (size_t)n * 8, hipMemcpyHostToDevice));
-    PRH_HIP(hipMemset(c->vec, 0, (size_t)n_vectors * n * 8));
+    PRH_HIP(hipMemsetAsync(c->vec, 0, (size_t)n_vectors * n * 8, c->stream));
+    PRH_HIP(hipStreamSynchronize(c->stream));
     *out = c;
     return PRH_OK;
 }
@@ -487,8 +492,7 @@ void prh_coupled_destroy(prh_coupled *c)
 {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    hipStream_t st;
-    if (current_stream(&st) == PRH_OK) (void)hipStreamSynchronize(st);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_coupled(c);
     delete c;
 }
