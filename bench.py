@@ -87,6 +87,7 @@ def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
     """Plain FP64 series vs served mixed-precision solves (Poisson arrivals)."""
 
     from porerad import coupled, exchange, serving, synthetic, view_factors
+    from porerad.constants import K_GAS, K_SOLID
 
     t0 = time.perf_counter()
     geoms = {"foam128": (foam_P, build_operator(foam_P, R, seed))}
@@ -111,17 +112,30 @@ def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
              "p99_ms": float(np.percentile(plain_lat, 99) * 1e3), "precision": "fp64 reflection series, rtol 1e-8",
              "geometry": "foam128"}
 
-    # FP64 oracle values at 1e-13 for the accuracy check
-    for Th, eps in combos:
-        T_m = 0.5 * (Th + 300.0)
-        refs[(Th, eps)] = coupled.prescribed_field_channels(op, P, T_m, Th - 300.0, eps)
+    # FP64 reflection series at 1e-13 for the accuracy check, both geometries
+    for key, (Pk, opk) in geoms.items():
+        for Th, eps in combos:
+            T_m = 0.5 * (Th + 300.0)
+            refs[(key, Th, eps)] = coupled.prescribed_field_channels(opk, Pk, T_m, Th - 300.0, eps)
 
-    def make_requests(count, seed_):
+    def make_requests(count, seed_, rtol=1e-9):
         rng = np.random.default_rng(seed_)
         keys = list(geoms)
         arrivals = serving.poisson_arrivals(rate, count, seed_)
         return [serving.RadiosityRequest(str(rng.choice(keys)), float(rng.choice([500.0, 700.0, 900.0])), 300.0,
-                                         float(rng.choice([0.3, 0.6, 0.9])), 1e-9, float(t)) for t in arrivals]
+                                         float(rng.choice([0.3, 0.6, 0.9])), rtol, float(t)) for t in arrivals]
+
+    def flux_errors(replies):
+        err = {}
+        for rep in replies:
+            if not isinstance(rep, serving.RadiosityReply):
+                continue
+            ref = refs[(rep.request.key, rep.request.T_hot_K, rep.request.epsilon)]
+            scale = max(abs(ref.q_h), abs(ref.q_c))
+            e = err.setdefault(rep.request.key, {"q_h": 0.0, "q_c": 0.0, "q_i": 0.0})
+            for name, got, want in (("q_h", rep.q_h, ref.q_h), ("q_c", rep.q_c, ref.q_c), ("q_i", rep.q_i, ref.q_i)):
+                e[name] = max(e[name], abs(got - want) / scale)
+        return err
 
     srv = serving.RadiosityServer()
     for k, (Pk, opk) in geoms.items():
@@ -136,21 +150,42 @@ def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
     t = time.perf_counter()
     sat = srv.run(make_requests(n_requests, 8), real_time=False)
     saturation = serving.latency_summary(sat, time.perf_counter() - t)
-    err = {"q_h": 0.0, "q_c": 0.0, "q_i": 0.0}
-    for rep in replies + sat:
-        if rep.request.key != "foam128":
-            continue
-        ref = refs[(rep.request.T_hot_K, rep.request.epsilon)]
-        scale = max(abs(ref.q_h), abs(ref.q_c))
-        for name, got, want in (("q_h", rep.q_h, ref.q_h), ("q_c", rep.q_c, ref.q_c), ("q_i", rep.q_i, ref.q_i)):
-            err[name] = max(err[name], abs(got - want) / scale)
+
+    # precision: every (geometry, T_hot, eps) refined to 1e-10 against the FP64 series at 1e-13
+    prec = [serving.RadiosityRequest(k, Th, 300.0, eps, 1e-10) for k in geoms for Th, eps in combos]
+    precision = {"relative_tolerance": 1e-10, "reference": "fp64 reflection series, rtol 1e-13",
+                 "max_rel_error_per_flux": flux_errors(srv.run(prec, real_time=False))}
+
+    # serving plus coupling: radiosity requests at the same rate with coupled solves of the foam
+    # arriving among them (conductivities of the reference's constants, 0.20 and 0.026 W/m/K)
+    mixed_reqs = make_requests(n_requests // 2, 9)
+    coupled_reqs = [serving.CoupledRequest("foam128", Th, 300.0, eps, K_SOLID, K_GAS, arrival_s=a)
+                    for (Th, eps), a in zip(((700.0, 0.6), (900.0, 0.9)), (0.05, 0.5))]
+    t = time.perf_counter()
+    mixed = srv.run(mixed_reqs + coupled_reqs, real_time=True)
+    rad_part = [r for r in mixed if isinstance(r, serving.RadiosityReply)]
+    cpl_part = [r for r in mixed if isinstance(r, serving.CoupledReply)]
+    rad_wall = max(r.finished_s for r in rad_part) - t
+    coupled_block = {
+        "radiosity": serving.latency_summary(rad_part, rad_wall),
+        "coupled": {"solves": len(cpl_part), "latency_s": [round(r.latency_s, 3) for r in cpl_part],
+                    "newton_updates": [r.newton_updates for r in cpl_part],
+                    "converged": all(r.converged for r in cpl_part),
+                    "q_total_hot_W_m2": [r.metrics["q_total_hot_W_m2"] for r in cpl_part],
+                    "wall_flux_mismatch_rel": max(r.metrics["wall_flux_mismatch_rel"] for r in cpl_part)},
+        "workload": f"{n_requests // 2} radiosity requests at {rate:.0f}/s + 2 coupled solves of the foam "
+                    f"(k_solid {K_SOLID}, k_gas {K_GAS} W/m/K, FP64 JVP series)",
+    }
     srv.close()
+    err = flux_errors(replies + sat)
     return {
         "plain_fp64": plain,
         "served": served,
         "saturation": saturation,
         "speedup_vs_plain_fp64": saturation["solves_per_s"] / plain["solves_per_s"],
-        "max_rel_flux_error_vs_fp64": err,
+        "max_rel_flux_error_vs_fp64": {k: max(v.values()) for k, v in err.items()},
+        "precision": precision,
+        "serving_with_coupled": coupled_block,
         "operator_setup_s": round(setup, 2),
     }
 
@@ -285,6 +320,11 @@ def main() -> None:
             f"p50 {rad['served']['p50_ms']:.1f} ms p99 {rad['served']['p99_ms']:.1f} ms; saturation "
             f"{rad['saturation']['solves_per_s']:.1f} solves/s ({rad['speedup_vs_plain_fp64']:.1f}x plain); "
             f"max flux error {max(rad['max_rel_flux_error_vs_fp64'].values()):.1e}")
+        cb = rad["serving_with_coupled"]
+        log(f"precision at 1e-10: {rad['precision']['max_rel_error_per_flux']}")
+        log(f"with coupled solves: radiosity {cb['radiosity']['solves_per_s']:.1f} solves/s p50 "
+            f"{cb['radiosity']['p50_ms']:.1f} ms p99 {cb['radiosity']['p99_ms']:.1f} ms; coupled latencies "
+            f"{cb['coupled']['latency_s']} s, Newton updates {cb['coupled']['newton_updates']}")
     if not args.no_cpu:
         record["cpu_baseline"] = cpu_baseline(phases, R, seed)
         log(f"cpu baseline {record['cpu_baseline']['value']:.3e} rays/s")

@@ -20,14 +20,22 @@ that arrive while others are on the GPU join at the next sweep, so their
 work overlaps on the device with the requests already running; a solve's
 bits do not depend on what else is in its batch.
 
-The engine type is injectable (``engine_factory``) so that the admission and
-retirement logic can be exercised without a GPU.
+Coupled requests (``CoupledRequest``: the strictly coupled conduction-
+radiation solve of coupled.py:428-490 with given phase conductivities) run
+one at a time on a worker thread; the device solve has its own HIP stream,
+so its kernels run beside the radiosity sweeps.  A coupled solve starts only
+when its device working set fits the HBM budget next to the engines.
+
+The engine type and the coupled runner are injectable (``engine_factory``,
+``coupled_runner``) so that the admission and retirement logic can be
+exercised without a GPU.
 """
 
 from __future__ import annotations
 
 import collections
 import dataclasses
+import threading
 import time
 from typing import Callable
 
@@ -51,6 +59,35 @@ class RadiosityRequest:
     arrival_s: float = 0.0
     tag: object = None
     patch_power: bool = False  # also return the net power of every patch
+
+
+@dataclasses.dataclass
+class CoupledRequest:
+    """Strictly coupled conduction-radiation solve between T_hot and T_cold (coupled.py:428-490)."""
+
+    key: str
+    T_hot_K: float
+    T_cold_K: float
+    epsilon: float
+    conductivity_solid: float = 1.0
+    conductivity_gas: float = 0.026
+    arrival_s: float = 0.0
+    tag: object = None
+
+
+@dataclasses.dataclass
+class CoupledReply:
+    request: CoupledRequest
+    converged: bool
+    newton_updates: int
+    metrics: dict
+    failure: str
+    admitted_s: float
+    finished_s: float
+
+    @property
+    def latency_s(self) -> float:
+        return self.finished_s - self.request.arrival_s
 
 
 @dataclasses.dataclass
@@ -91,6 +128,7 @@ class Geometry:
         self.half_index[self.internal] = self.cell_x[self.internal].astype(np.float64) + 0.5
         self.area = patches.patch_area_m2
         self.cross_section = float(ny * nz) * (patches.H_m / float(nx)) ** 2
+        self.conduction: dict[tuple[float, float], object] = {}  # assembled systems by conductivities
 
     def emission(self, req: RadiosityRequest) -> np.ndarray:
         """eps sigma A (T^4 - T_m^4) of the prescribed linear field.
@@ -122,6 +160,31 @@ def device_engine(geometry: Geometry):
     return engine
 
 
+def device_coupled_runner(geometry: Geometry, request: CoupledRequest) -> tuple[bool, int, dict, str]:
+    """Pure-conduction start field, coupled solve and wall-flux metrics on the device."""
+
+    from . import conduction, coupled
+
+    P = geometry.patches
+    key = (float(request.conductivity_solid), float(request.conductivity_gas))
+    system = geometry.conduction.get(key)
+    if system is None:
+        k = np.where(np.asarray(P.solid) == 1, key[0], key[1])
+        system = geometry.conduction[key] = conduction.assemble_conduction(k, P.H_m)
+    Th, Tc = float(request.T_hot_K), float(request.T_cold_K)
+    T0 = conduction.solve_conduction(system, Th, Tc)
+    result = coupled.solve_coupled(system, geometry.operator, P, Th, Tc, float(request.epsilon), T0)
+    metrics = coupled.wall_flux_metrics(system, P, result, Th, Tc)
+    return bool(result.converged), int(result.newton_iterations), metrics, result.failure
+
+
+def device_coupled_bytes(geometry: Geometry) -> int:
+    """Device bytes of one coupled solve: 73 cell vectors, scratch and the 7-point conduction matrix."""
+
+    n_cells = int(np.prod(geometry.patches.solid.shape))
+    return n_cells * (83 * 8 + 7 * 12 + 8) + geometry.n * 32 + (1 << 20)
+
+
 def device_engine_bytes(geometry: Geometry) -> int:
     """Device bytes of one engine: 16 slots x (6 FP32 + 5 FP64 vectors) per patch, partial sums, field."""
 
@@ -136,7 +199,9 @@ class RadiosityServer:
     def __init__(self, hbm_budget_bytes: int | None = None,
                  engine_factory: Callable[[Geometry], object] = device_engine,
                  engine_bytes: Callable[[Geometry], int] = device_engine_bytes,
-                 clock: Callable[[], float] = time.perf_counter) -> None:
+                 clock: Callable[[], float] = time.perf_counter,
+                 coupled_runner: Callable[[Geometry, CoupledRequest], tuple] = device_coupled_runner,
+                 coupled_bytes: Callable[[Geometry], int] = device_coupled_bytes) -> None:
         self.budget = hbm_budget_bytes
         self.engine_factory = engine_factory
         self.engine_size = engine_bytes
@@ -148,6 +213,14 @@ class RadiosityServer:
         self.replies: list[RadiosityReply] = []
         self.sweeps = 0
         self.evictions = 0
+        self.coupled_runner = coupled_runner
+        self.coupled_size = coupled_bytes
+        self.coupled_queue: collections.deque[CoupledRequest] = collections.deque()
+        self._coupled_thread: threading.Thread | None = None
+        self._coupled_job: tuple[CoupledRequest, float] | None = None
+        self._coupled_done: list[CoupledReply] = []
+        self._coupled_error: BaseException | None = None
+        self._lock = threading.Lock()
 
     # -- registration ------------------------------------------------------------
     def add_geometry(self, key: str, patches, operator) -> Geometry:
@@ -157,13 +230,62 @@ class RadiosityServer:
         return g
 
     def engine_bytes(self) -> int:
-        return sum(self.engine_size(self.geometries[k]) for k in self.engines)
+        """Device bytes of the live engines and of the running coupled solve."""
+
+        total = sum(self.engine_size(self.geometries[k]) for k in self.engines)
+        job = self._coupled_job
+        if job is not None:
+            total += self.coupled_size(self.geometries[job[0].key])
+        return total
 
     # -- admission -----------------------------------------------------------------
-    def submit(self, request: RadiosityRequest) -> None:
+    def submit(self, request: RadiosityRequest | CoupledRequest) -> None:
         if request.key not in self.geometries:
             raise KeyError(f"unknown geometry {request.key!r}")
-        self.queue.append(request)
+        if isinstance(request, CoupledRequest):
+            self.coupled_queue.append(request)
+        else:
+            self.queue.append(request)
+
+    # -- coupled solves --------------------------------------------------------------
+    def _start_coupled(self) -> None:
+        if self._coupled_job is not None or not self.coupled_queue:
+            return
+        req = self.coupled_queue[0]
+        g = self.geometries[req.key]
+        if self.budget is not None:
+            need = self.coupled_size(g)
+            while self.engine_bytes() + need > self.budget:
+                idle = [k for k in self.engines if not self.slots[k]]
+                if not idle:
+                    return
+                self.engines.pop(idle[0]).close()
+                self.evictions += 1
+        self.coupled_queue.popleft()
+        self._coupled_job = (req, self.clock())
+        self._coupled_thread = threading.Thread(target=self._run_coupled, args=(g, req), daemon=True)
+        self._coupled_thread.start()
+
+    def _run_coupled(self, g: Geometry, req: CoupledRequest) -> None:
+        try:
+            converged, updates, metrics, failure = self.coupled_runner(g, req)
+            with self._lock:
+                self._coupled_done.append(CoupledReply(req, converged, updates, metrics, failure,
+                                                       self._coupled_job[1], self.clock()))
+        except BaseException as exc:  # reported by the scheduler thread
+            with self._lock:
+                self._coupled_error = exc
+        finally:
+            with self._lock:
+                self._coupled_job = None
+
+    def _collect_coupled(self) -> list[CoupledReply]:
+        with self._lock:
+            done, self._coupled_done = self._coupled_done, []
+            error, self._coupled_error = self._coupled_error, None
+        if error is not None:
+            raise error
+        return done
 
     def _engine_for(self, key: str):
         if key in self.engines:
@@ -197,9 +319,12 @@ class RadiosityServer:
         self.queue = waiting
 
     # -- one scheduler step ----------------------------------------------------------
-    def step(self) -> list[RadiosityReply]:
+    def step(self) -> list:
+        """One scheduler step; returns the replies (radiosity and coupled) finished in it."""
+
         self._admit()
-        done_now = []
+        self._start_coupled()
+        done_now: list = []
         for key, engine in list(self.engines.items()):
             busy = self.slots[key]
             if not busy:
@@ -219,11 +344,16 @@ class RadiosityServer:
                 else:
                     engine.release(slot)
                 done_now.append(RadiosityReply(req, q_h, q_c, q_i, net, residual, sweeps, admitted, self.clock()))
+        done_now.extend(self._collect_coupled())
+        if not done_now and not any(self.slots.values()) and self._coupled_job is not None:
+            time.sleep(0.001)  # only a coupled solve runs: do not spin
         self.replies.extend(done_now)
         return done_now
 
     def pending(self) -> int:
-        return len(self.queue) + sum(len(b) for b in self.slots.values())
+        with self._lock:
+            coupled = len(self._coupled_done) + (1 if self._coupled_job is not None else 0)
+        return len(self.queue) + sum(len(b) for b in self.slots.values()) + len(self.coupled_queue) + coupled
 
     def run(self, requests: list[RadiosityRequest], *, real_time: bool = True) -> list[RadiosityReply]:
         """Serve requests whose ``arrival_s`` are offsets from now; returns the replies in completion order.
@@ -253,6 +383,8 @@ class RadiosityServer:
         return self.replies[first:]
 
     def close(self) -> None:
+        if self._coupled_thread is not None:
+            self._coupled_thread.join()
         for e in self.engines.values():
             e.close()
         self.engines.clear()
@@ -265,7 +397,7 @@ def poisson_arrivals(rate_per_s: float, count: int, seed: int) -> np.ndarray:
     return np.cumsum(rng.exponential(1.0 / rate_per_s, size=count))
 
 
-def latency_summary(replies: list[RadiosityReply], wall_s: float) -> dict:
+def latency_summary(replies: list, wall_s: float) -> dict:
     lat = np.array([r.latency_s for r in replies])
     return {
         "solves": int(lat.size),

@@ -78,3 +78,37 @@ def test_alone_equals_mixed_stream(world):
     assert all(r.net_power_W is None for r in replies if r.request.tag >= 6)
     lat = serving.latency_summary(replies, 1.0)
     assert lat["p99_ms"] >= lat["p50_ms"] > 0
+
+
+def test_coupled_requests_served_beside_radiosity(world):
+    """Coupled solves on the worker thread give the bits of a direct solve, while radiosity requests keep flowing."""
+
+    serving, coupled, geoms = world
+    from porerad import conduction
+
+    creqs = [serving.CoupledRequest("fibre24", 700.0, 300.0, 0.6, conductivity_solid=1.0, arrival_s=0.002, tag="c0"),
+             serving.CoupledRequest("plates20", 900.0, 300.0, 0.3, conductivity_solid=1.0, arrival_s=0.004, tag="c1")]
+    direct = {}
+    for r in creqs:
+        P, op = geoms[r.key]
+        system = conduction.assemble_conduction(np.where(P.solid == 1, r.conductivity_solid, r.conductivity_gas),
+                                                P.H_m)
+        T0 = conduction.solve_conduction(system, r.T_hot_K, r.T_cold_K)
+        res = coupled.solve_coupled(system, op, P, r.T_hot_K, r.T_cold_K, r.epsilon, T0)
+        assert res.converged
+        direct[r.tag] = coupled.wall_flux_metrics(system, P, res, r.T_hot_K, r.T_cold_K)
+    rng = np.random.default_rng(3)
+    arrivals = serving.poisson_arrivals(2000.0, 30, 3)
+    reqs = [serving.RadiosityRequest(str(rng.choice(list(geoms))), 700.0, 300.0, float(rng.choice([0.3, 0.9])),
+                                     arrival_s=float(t), tag=i) for i, t in enumerate(arrivals)]
+    srv = _server(serving, geoms)
+    replies = srv.run(reqs + creqs, real_time=True)
+    srv.close()
+    assert len(replies) == len(reqs) + len(creqs)
+    served = {r.request.tag: r for r in replies if isinstance(r, serving.CoupledReply)}
+    assert set(served) == {"c0", "c1"}
+    for tag, m in direct.items():
+        got = served[tag]
+        assert got.converged and not got.failure
+        for key in ("q_cond_hot_W_m2", "q_rad_hot_W_m2", "q_total_cold_W_m2", "T_min_K", "T_max_K"):
+            assert got.metrics[key] == m[key], key

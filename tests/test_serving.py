@@ -182,3 +182,64 @@ def test_poisson_arrivals_have_the_requested_rate():
     t = serving.poisson_arrivals(200.0, 20000, 7)
     assert np.all(np.diff(t) > 0)
     assert t[-1] / 20000 == pytest.approx(1 / 200.0, rel=0.03)
+
+
+def _coupled_runner(log, pause=0.005):
+    import time
+
+    def runner(g, req):
+        log.append(req.T_hot_K)
+        time.sleep(pause)
+        return True, 2, {"q_total_hot_W_m2": 0.5 * req.T_hot_K}, ""
+
+    return runner
+
+
+def test_coupled_requests_run_beside_radiosity():
+    P = _patches()
+    log = []
+    srv = serving.RadiosityServer(engine_factory=lambda g: StandInEngine(g, 3), clock=Clock(),
+                                  coupled_runner=_coupled_runner(log), coupled_bytes=lambda g: 10)
+    srv.add_geometry("a", P, operator=None)
+    reqs = [serving.RadiosityRequest("a", 700.0, 300.0, 0.6) for _ in range(20)]
+    reqs += [serving.CoupledRequest("a", 700.0, 300.0, 0.6), serving.CoupledRequest("a", 900.0, 300.0, 0.3)]
+    replies = srv.run(reqs, real_time=False)
+    srv.close()
+    assert len(replies) == len(reqs) and srv.pending() == 0
+    coupled = [r for r in replies if isinstance(r, serving.CoupledReply)]
+    assert log == [700.0, 900.0]  # one at a time, in arrival order
+    assert [c.metrics["q_total_hot_W_m2"] for c in coupled] == [350.0, 450.0]
+    assert all(c.converged and c.newton_updates == 2 and c.latency_s > 0 for c in coupled)
+
+
+def test_coupled_request_waits_for_hbm_budget():
+    P = _patches()
+    log = []
+    srv = serving.RadiosityServer(hbm_budget_bytes=150, engine_factory=lambda g: StandInEngine(g, 3, nbytes=100),
+                                  engine_bytes=lambda g: 100, clock=Clock(), coupled_runner=_coupled_runner(log),
+                                  coupled_bytes=lambda g: 100)
+    srv.add_geometry("a", P, operator=None)
+    srv.submit(serving.RadiosityRequest("a", 700.0, 300.0, 0.6))
+    srv.submit(serving.CoupledRequest("a", 700.0, 300.0, 0.6))
+    srv.step()
+    # the engine holds 100 of 150 bytes while it runs: the coupled solve must wait
+    assert log == [] and len(srv.coupled_queue) == 1
+    while srv.pending():
+        srv.step()
+        assert srv.engine_bytes() <= 150
+    srv.close()
+    assert log == [700.0] and srv.evictions == 1
+
+
+def test_coupled_runner_errors_reach_the_caller():
+    P = _patches()
+
+    def runner(g, req):
+        raise RuntimeError("radiosity reflection series did not converge")
+
+    srv = serving.RadiosityServer(engine_factory=lambda g: StandInEngine(g), clock=Clock(), coupled_runner=runner,
+                                  coupled_bytes=lambda g: 1)
+    srv.add_geometry("a", P, operator=None)
+    with pytest.raises(RuntimeError, match="did not converge"):
+        srv.run([serving.CoupledRequest("a", 700.0, 300.0, 0.6)], real_time=False)
+    srv.close()
