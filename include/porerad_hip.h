@@ -63,6 +63,7 @@ typedef struct prh_edges prh_edges;
 typedef struct prh_operator prh_operator;
 typedef struct prh_batch prh_batch;
 typedef struct prh_coupled prh_coupled;
+typedef struct prh_trace_batch prh_trace_batch;
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *prh_last_error(void);
@@ -126,6 +127,37 @@ int prh_trace_rays(const prh_geometry *g, const double *origins, const double *d
  * Replaces first_hit_counts + pool_counts, porerad/view_factors.py:273-300. */
 int prh_trace_counts(const prh_geometry *g, int64_t p0, int64_t p1, int32_t rays_per_patch, int32_t n_seeds,
                      const uint64_t *state_inc, prh_edges **out, int64_t *nnz, int64_t *occupancy_reads);
+/* Many first_hit_counts calls of uneven geometries in one graph.  Segment k
+ * traces patches [patch_begin[k], patch_end[k]) of geometries[k] with
+ * rays_per_patch[k] rays from each of n_seeds[k] seeded blocks, pooled as
+ * prh_trace_counts pools them (the same edges, bit for bit).  The segments
+ * are cut at patch boundaries into chunks of at most ray_capacity ray records
+ * (0: min(2^27, all rays); 112 bytes of HBM per record); each chunk is one
+ * ray-generation launch and one persistent traversal launch over all its
+ * pieces, followed by per-piece counting and compaction into one device edge
+ * arena of edge_capacity entries (0: the worst case, rays per patch bounded by
+ * the patch count).  The whole pass is captured once into a HIP graph that
+ * prh_trace_batch_run replays.  The geometries must outlive the batch.
+ * Replaces the per-case loop of first_hit_counts + pool_counts calls,
+ * porerad/view_factors.py:273-300, as scripts/run_prescribed_field.py:81-85
+ * drives it per case. */
+int prh_trace_batch_create(int32_t n_segments, const prh_geometry *const *geometries, const int64_t *patch_begin,
+                           const int64_t *patch_end, const int32_t *rays_per_patch, const int32_t *n_seeds,
+                           int64_t ray_capacity, int64_t edge_capacity, prh_trace_batch **out);
+/* Replay the pass with seeds state_inc (each segment's n_seeds x 4 words, in
+ * segment order).  nnz[k] / occupancy_reads[k] (either may be NULL) receive
+ * segment k's edge count and occupancy reads; *elapsed_ms (may be NULL) the
+ * device time of the graph.  PRH_ERR_UNRESOLVED if a ray failed,
+ * PRH_ERR_CAPACITY if the edges overflowed the arena. */
+int prh_trace_batch_run(prh_trace_batch *b, const uint64_t *state_inc, int64_t *nnz, int64_t *occupancy_reads,
+                        double *elapsed_ms);
+/* Copy segment k's edges of the last run to host arrays of nnz[k] entries. */
+int prh_trace_batch_edges(const prh_trace_batch *b, int32_t segment, int32_t *origin, int32_t *receiver,
+                          int64_t *count);
+/* info[0..n): device bytes, chunks, pieces, ray capacity, edge capacity, rays. */
+int prh_trace_batch_info(const prh_trace_batch *b, int64_t *info, int32_t n);
+void prh_trace_batch_destroy(prh_trace_batch *b);
+
 /* Copy a device edge table to host arrays of nnz entries. */
 int prh_edges_copy(const prh_edges *e, int32_t *origin, int32_t *receiver, int64_t *count);
 void prh_edges_destroy(prh_edges *e);

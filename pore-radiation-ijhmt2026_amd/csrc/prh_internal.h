@@ -102,6 +102,13 @@ struct Geometry {
 // Device exclusive scan helpers (prh_scan.hip).
 int exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *total, hipStream_t stream);
 int exclusive_scan_i32_to_i64(const int32_t *in, int64_t *out, int64_t n, int64_t *total, hipStream_t stream);
+// Stream-ordered form without host synchronisation or allocation (usable
+// under graph capture) that continues a running device total: out[i] =
+// *carry + sum(in[0..i)), then *start = the old *carry and *carry += sum(in).
+// scratch holds scan_scratch_elems(n) values.
+int64_t scan_scratch_elems(int64_t n);
+int exclusive_scan_i32_to_i64_carry_async(const int32_t *in, int64_t *out, int64_t n, int64_t *scratch,
+                                          int64_t *carry, int64_t *start, hipStream_t stream);
 
 // Persistent workspace helpers.
 int ensure_capacity(void **buf, size_t *cap, size_t bytes);

@@ -60,6 +60,29 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_tile_sums(int64_t *tile_s
     if (threadIdx.x == 0) *grand_total = carry;
 }
 
+// k_scan_tile_sums continuing from a running total: tile offsets start at
+// *carry, the previous total goes to *start and the new one to *carry.
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_tile_sums_carry(int64_t *tile_sums, int64_t n_tiles,
+                                                                       int64_t *carry, int64_t *start)
+{
+    __shared__ int64_t lds[SCAN_THREADS];
+    const int64_t first = *carry;
+    int64_t c = first;
+    for (int64_t base = 0; base < n_tiles; base += SCAN_THREADS) {
+        int64_t idx = base + threadIdx.x;
+        int64_t v = idx < n_tiles ? tile_sums[idx] : 0;
+        int64_t total;
+        int64_t ex = block_exclusive_scan<int64_t>(v, lds, &total);
+        if (idx < n_tiles) tile_sums[idx] = c + ex;
+        c += total;
+    }
+    __syncthreads();  // every thread has read *carry
+    if (threadIdx.x == 0) {
+        *start = first;
+        *carry = c;
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(SCAN_THREADS) void k_tile_scan(const T *in, int64_t *out, int64_t n,
                                                             const int64_t *tile_offsets)
@@ -106,6 +129,24 @@ int scan_impl(const T *in, int64_t *out, int64_t n, int64_t *total, hipStream_t 
 }
 
 }  // namespace
+
+int64_t scan_scratch_elems(int64_t n)
+{
+    return n > 0 ? (n + SCAN_TILE - 1) / SCAN_TILE : 1;
+}
+
+int exclusive_scan_i32_to_i64_carry_async(const int32_t *in, int64_t *out, int64_t n, int64_t *scratch,
+                                          int64_t *carry, int64_t *start, hipStream_t stream)
+{
+    const int64_t n_tiles = n > 0 ? (n + SCAN_TILE - 1) / SCAN_TILE : 0;
+    if (n_tiles > 0)
+        k_tile_sums<int32_t><<<dim3((unsigned)n_tiles), dim3(SCAN_THREADS), 0, stream>>>(in, n, scratch);
+    k_scan_tile_sums_carry<<<1, SCAN_THREADS, 0, stream>>>(scratch, n_tiles, carry, start);
+    if (n_tiles > 0)
+        k_tile_scan<int32_t><<<dim3((unsigned)n_tiles), dim3(SCAN_THREADS), 0, stream>>>(in, out, n, scratch);
+    PRH_HIP(hipGetLastError());
+    return PRH_OK;
+}
 
 int exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *total, hipStream_t stream)
 {

@@ -391,23 +391,22 @@ struct GenArgs {
     TraversalConst C;
     uint32_t p0, n_p, R;
     const uint64_t *seeds;  // S x {state_hi, state_lo, inc_hi, inc_lo}
-    uint32_t total;
+    uint32_t total;         // rays of this launch (or segment)
+    uint32_t offset;        // index of its first record in the output arrays
     double *origins;     // total x 3 (rays only)
     double *directions;  // total x 3 (rays only)
-    DdaRec *dda;         // total (records only)
-    WallRec *walls;      // total (records only)
+    DdaRec *dda;         // records only
+    WallRec *walls;      // records only
 };
 
+// The GEN_ROUNDS rays of one lane: rays base, base + 64, ... of the launch.
 // RECORDS: write the traversal records (DDA start state, wall inputs)
 // instead of the origins and directions, so the divisions of the set-up run
 // here in converged waves rather than in the traversal's refills.
 template <bool RECORDS>
-__global__ __launch_bounds__(TRACE_THREADS) void k_gen_rays(GenArgs a)
+__device__ __forceinline__ void gen_rays_body(const GenArgs &a, uint64_t base)
 {
     const GeoView &G = a.G;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t wave = ((uint64_t)blockIdx.x * TRACE_THREADS + threadIdx.x) >> 6;
-    const uint64_t base = wave * (64ull * GEN_ROUNDS) + lane;
     const uint32_t seg = a.n_p * a.R;
     const double fny = (double)G.ny, fnz = (double)G.nz;
     if (base >= a.total) return;
@@ -420,7 +419,7 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_gen_rays(GenArgs a)
     for (uint32_t k = 0; k < GEN_ROUNDS; ++k) {
         const uint64_t g64 = base + 64ull * k;
         if (g64 >= a.total) break;
-        const uint32_t g = (uint32_t)g64;
+        const uint32_t g = (uint32_t)g64 + a.offset;
         if (k > 0) {
             r += 64u;
             while (r >= a.R) {
@@ -464,6 +463,41 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_gen_rays(GenArgs a)
     }
 }
 
+template <bool RECORDS>
+__global__ __launch_bounds__(TRACE_THREADS) void k_gen_rays(GenArgs a)
+{
+    const uint64_t wave = ((uint64_t)blockIdx.x * TRACE_THREADS + threadIdx.x) >> 6;
+    gen_rays_body<RECORDS>(a, wave * (64ull * GEN_ROUNDS) + (threadIdx.x & 63u));
+}
+
+// One segment of a multi-geometry batch: a patch range of one geometry,
+// traced with one seed.  Its rays occupy records [offset, offset + total).
+struct Segment {
+    GenArgs gen;
+    uint32_t gen_wave_begin;    // first generation wavefront of the segment
+    uint32_t trace_block_begin;  // first traversal workgroup of the segment
+    uint32_t per_wave;           // traversal rays per wavefront
+    unsigned long long *reads;   // occupancy-read counter of the segment
+};
+
+__device__ __forceinline__ int segment_of(const Segment *segs, int n_seg, uint32_t x, bool trace)
+{
+    int s = 0;
+    for (int k = 1; k < n_seg; ++k)
+        if ((trace ? segs[k].trace_block_begin : segs[k].gen_wave_begin) <= x) s = k;
+    return s;
+}
+
+// Records of every segment of a batch in one launch (each wavefront works
+// inside one segment).
+__global__ __launch_bounds__(TRACE_THREADS) void k_gen_rays_batch(const Segment *segs, int n_seg)
+{
+    const uint32_t wave = (blockIdx.x * TRACE_THREADS + threadIdx.x) >> 6;
+    const int k = segment_of(segs, n_seg, wave, false);
+    const GenArgs a = segs[k].gen;
+    gen_rays_body<true>(a, (uint64_t)(wave - segs[k].gen_wave_begin) * (64ull * GEN_ROUNDS) + (threadIdx.x & 63u));
+}
+
 // Traversal records of given rays (prh_trace_rays).
 __global__ __launch_bounds__(TRACE_THREADS) void k_setup_rays(GeoView G, TraversalConst C, const double *origins,
                                                               const double *directions, uint32_t total, DdaRec *dda,
@@ -495,22 +529,14 @@ struct TraceArgs {
     unsigned long long *reads;
 };
 
+// Rays [w_begin, w_end) of one wavefront, streamed with refills.
 template <int REFILL_MIN, int OCC>
-__global__ __launch_bounds__(STREAM_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
-void k_trace_stream(TraceArgs a)
+__device__ __forceinline__ void trace_body(const GeoView &G, uint32_t w_begin, uint32_t w_end, const DdaRec *dda,
+                                           const WallRec *walls, uint2 *fin, unsigned long long *reads_out,
+                                           const uint32_t *lds_coarse)
 {
-    const GeoView &G = a.G;
-    extern __shared__ uint32_t lds_coarse[];
-    if (OCC == OCC_BRICK) {
-        for (uint32_t w = threadIdx.x; w < G.coarse_words; w += STREAM_THREADS) lds_coarse[w] = G.coarse[w];
-        __syncthreads();
-    }
-    const uint32_t wave = (blockIdx.x * STREAM_THREADS + threadIdx.x) >> 6;
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
-    const uint64_t wb = (uint64_t)wave * a.per_wave;
-    const uint32_t w_begin = wb < a.total ? (uint32_t)wb : a.total;
-    const uint32_t w_end = wb + a.per_wave < a.total ? (uint32_t)(wb + a.per_wave) : a.total;
     uint32_t cursor = w_begin;
 
     Ray R;
@@ -523,7 +549,7 @@ void k_trace_stream(TraceArgs a)
     while (true) {
         if (!active && pending) {
             reads += ray_reads(R, G);
-            a.fin[g_cur] = ray_final(R, G, a.walls, g_cur);
+            fin[g_cur] = ray_final(R, G, walls, g_cur);
             pending = false;
         }
         if (cursor < w_end) {
@@ -531,12 +557,12 @@ void k_trace_stream(TraceArgs a)
             if (!active) {
                 const uint32_t g = cursor + (uint32_t)__popcll(idle & below);
                 if (g < w_end) {
-                    if (ray_load(R, a.dda, g)) {
+                    if (ray_load(R, dda, g)) {
                         active = true;
                         pending = true;
                         g_cur = g;
                     } else {
-                        a.fin[g] = make_uint2(0u, FIN_FAILED << 28);
+                        fin[g] = make_uint2(0u, FIN_FAILED << 28);
                     }
                 }
             }
@@ -556,7 +582,52 @@ void k_trace_stream(TraceArgs a)
         }
     }
     reads = wave_sum_u64(reads);
-    if (lane == 0) atomicAdd(a.reads, (unsigned long long)reads);
+    if (lane == 0 && reads) atomicAdd(reads_out, (unsigned long long)reads);
+}
+
+template <int REFILL_MIN, int OCC>
+__global__ __launch_bounds__(STREAM_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
+void k_trace_stream(TraceArgs a)
+{
+    const GeoView &G = a.G;
+    extern __shared__ uint32_t lds_coarse[];
+    if (OCC == OCC_BRICK) {
+        for (uint32_t w = threadIdx.x; w < G.coarse_words; w += STREAM_THREADS) lds_coarse[w] = G.coarse[w];
+        __syncthreads();
+    }
+    const uint32_t wave = (blockIdx.x * STREAM_THREADS + threadIdx.x) >> 6;
+    const uint64_t wb = (uint64_t)wave * a.per_wave;
+    const uint32_t w_begin = wb < a.total ? (uint32_t)wb : a.total;
+    const uint32_t w_end = wb + a.per_wave < a.total ? (uint32_t)(wb + a.per_wave) : a.total;
+    trace_body<REFILL_MIN, OCC>(G, w_begin, w_end, a.dda, a.walls, a.fin, a.reads, lds_coarse);
+}
+
+// Every segment of a batch in one persistent launch: each workgroup works
+// inside one segment (its LDS holds that geometry's brick bitset), so
+// geometries of uneven size share the chip without a launch each.
+template <int REFILL_MIN, int OCC>
+__global__ __launch_bounds__(STREAM_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
+void k_trace_batch(const Segment *segs, int n_seg, const DdaRec *dda, const WallRec *walls, uint2 *fin)
+{
+    const int k = segment_of(segs, n_seg, blockIdx.x, true);
+    const GeoView G = segs[k].gen.G;
+    extern __shared__ uint32_t lds_coarse[];
+    if (OCC == OCC_BRICK) {
+        for (uint32_t w = threadIdx.x; w < G.coarse_words; w += STREAM_THREADS) lds_coarse[w] = G.coarse[w];
+        __syncthreads();
+    }
+    const uint32_t base = segs[k].gen.offset, total = segs[k].gen.total, per_wave = segs[k].per_wave;
+    const uint32_t wave = (blockIdx.x - segs[k].trace_block_begin) * (STREAM_THREADS / 64) + (threadIdx.x >> 6);
+    const uint64_t wb = (uint64_t)wave * per_wave;
+    const uint32_t w_begin = base + (wb < total ? (uint32_t)wb : total);
+    const uint32_t w_end = base + (wb + per_wave < total ? (uint32_t)(wb + per_wave) : total);
+    trace_body<REFILL_MIN, OCC>(G, w_begin, w_end, dda, walls, fin, segs[k].reads, lds_coarse);
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
 }
 
 // Receivers of final states (prh_trace_rays), with the count of failures.
@@ -761,12 +832,15 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_wave(const uint2 *fin, 
 __global__ __launch_bounds__(COUNT_THREADS) void k_compact_edges(const int32_t *uniq_recv, const int32_t *uniq_cnt,
                                                                  const int32_t *n_unique, const int64_t *offsets,
                                                                  int64_t n_p, int SR, int64_t p0, int32_t *origin,
-                                                                 int32_t *receiver, int32_t *count)
+                                                                 int32_t *receiver, int32_t *count, int64_t cap)
 {
     for (int64_t q = blockIdx.x; q < n_p; q += gridDim.x) {
         const int64_t off = offsets[q];
         const int nu = n_unique[q];
-        for (int u = threadIdx.x; u < nu; u += COUNT_THREADS) {
+        // edges past the capacity are dropped; the caller reports the overflow
+        const int64_t room = cap - off;
+        const int n_write = room <= 0 ? 0 : (room < nu ? (int)room : nu);
+        for (int u = threadIdx.x; u < n_write; u += COUNT_THREADS) {
             origin[off + u] = (int32_t)(p0 + q);
             receiver[off + u] = uniq_recv[q * SR + u];
             count[off + u] = uniq_cnt[q * SR + u];
@@ -841,6 +915,26 @@ constexpr int N_TRACE_VARIANTS = sizeof(TRACE_VARIANTS) / sizeof(TRACE_VARIANTS[
 // the foam and fibre media measured so far.
 int g_trace_variant = 1;
 
+// Batch traversal kernels for the variants' (refill, occupancy) pairs.
+using BatchKernel = void (*)(const Segment *, int, const DdaRec *, const WallRec *, uint2 *);
+struct BatchVariant {
+    BatchKernel fn;
+    int refill_min, occ;
+};
+const BatchVariant BATCH_VARIANTS[] = {
+    {k_trace_batch<16, OCC_BRICK>, 16, OCC_BRICK}, {k_trace_batch<24, OCC_BRICK>, 24, OCC_BRICK},
+    {k_trace_batch<32, OCC_BRICK>, 32, OCC_BRICK}, {k_trace_batch<16, OCC_GLOBAL>, 16, OCC_GLOBAL},
+    {k_trace_batch<32, OCC_GLOBAL>, 32, OCC_GLOBAL},
+};
+
+const BatchVariant &batch_variant_for(int variant)
+{
+    const TraceVariant &V = TRACE_VARIANTS[variant];
+    for (const BatchVariant &B : BATCH_VARIANTS)
+        if (B.refill_min == V.refill_min && B.occ == V.occ) return B;
+    return BATCH_VARIANTS[0];
+}
+
 unsigned gen_grid(uint32_t total)
 {
     const uint64_t waves = (total + 64ull * GEN_ROUNDS - 1) / (64ull * GEN_ROUNDS);
@@ -852,7 +946,7 @@ int launch_gen_rays(const Geometry &g, uint32_t p0, uint32_t n_p, uint32_t R, co
                     double *d_orig, double *d_dir, hipStream_t st)
 {
     if (total == 0) return PRH_OK;
-    GenArgs a{view_of(g), traversal_const(g), p0, n_p, R, d_seeds, total, d_orig, d_dir, nullptr, nullptr};
+    GenArgs a{view_of(g), traversal_const(g), p0, n_p, R, d_seeds, total, 0u, d_orig, d_dir, nullptr, nullptr};
     k_gen_rays<false><<<gen_grid(total), TRACE_THREADS, 0, st>>>(a);
     PRH_HIP(hipGetLastError());
     return PRH_OK;
@@ -863,7 +957,7 @@ int launch_gen_records(const Geometry &g, uint32_t p0, uint32_t n_p, uint32_t R,
                        uint32_t total, DdaRec *d_dda, WallRec *d_walls, hipStream_t st)
 {
     if (total == 0) return PRH_OK;
-    GenArgs a{view_of(g), traversal_const(g), p0, n_p, R, d_seeds, total, nullptr, nullptr, d_dda, d_walls};
+    GenArgs a{view_of(g), traversal_const(g), p0, n_p, R, d_seeds, total, 0u, nullptr, nullptr, d_dda, d_walls};
     k_gen_rays<true><<<gen_grid(total), TRACE_THREADS, 0, st>>>(a);
     PRH_HIP(hipGetLastError());
     return PRH_OK;
@@ -1106,7 +1200,7 @@ int prh_trace_counts(const prh_geometry *geo, int64_t p0, int64_t p1, int32_t ra
         PRH_HIP(hipMalloc((void **)&d_count, alloc * sizeof(int32_t)));
         k_compact_edges<<<cgrid, COUNT_THREADS, 0, st>>>(W.uniq_recv.as<int32_t>(), W.uniq_cnt.as<int32_t>(),
                                                          W.n_unique.as<int32_t>(), W.offsets.as<int64_t>(), n_p, SR,
-                                                         p0, d_origin, d_receiver, d_count);
+                                                         p0, d_origin, d_receiver, d_count, nnz);
         PRH_HIP(hipGetLastError());
         PRH_HIP(hipEventRecord(W.ev[4], st));
         PRH_HIP(hipStreamSynchronize(st));
@@ -1122,6 +1216,404 @@ int prh_trace_counts(const prh_geometry *geo, int64_t p0, int64_t p1, int32_t ra
     if (nnz_out) *nnz_out = nnz;
     if (occupancy_reads) *occupancy_reads = (int64_t)counters[0];
     return PRH_OK;
+}
+
+}  // extern "C"
+
+// A multi-geometry batch: segments (geometry, patch range, rays per patch,
+// seeds) cut at patch boundaries into pieces that fill fixed-size chunks of
+// ray records; per chunk one generation launch and one persistent traversal
+// launch over all its pieces, then per piece counting, a scan that carries the
+// running edge total on the device, and compaction into one edge arena.  The
+// whole pass is captured once into a graph and replayed with new seeds.
+struct BatchPiece {
+    int seg;
+    int64_t a, b;      // patch range [a, b) of the segment's geometry
+    int64_t ray_off;   // first record within the chunk
+    int64_t patch_off; // first per-patch slot within the chunk
+};
+
+struct prh_trace_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_seg = 0;
+    std::vector<const prh_geometry *> geos;
+    std::vector<int> R, S;
+    std::vector<int64_t> seed_off;  // first seed word of each segment
+    int64_t seed_words = 0;
+    std::vector<BatchPiece> pieces;
+    std::vector<int> chunk_first;  // first piece of each chunk, then the piece count
+    std::vector<int64_t> chunk_waves, chunk_blocks;
+    int64_t ray_cap = 0, patch_cap = 0, scratch_elems = 0, edge_cap = 0, rays = 0;
+    Segment *segs_dev = nullptr;
+    uint64_t *seeds = nullptr;
+    DdaRec *dda = nullptr;
+    WallRec *walls = nullptr;
+    uint2 *fin = nullptr;
+    int32_t *uniq_recv = nullptr, *uniq_cnt = nullptr, *n_unique = nullptr;
+    int64_t *offsets = nullptr, *scan_scratch = nullptr;
+    unsigned long long *counters = nullptr;  // reads of each segment, then failures
+    int64_t *marks = nullptr;                // running edge total, then each piece's first edge
+    int32_t *e_origin = nullptr, *e_receiver = nullptr, *e_count = nullptr;
+    std::vector<int64_t> seg_nnz, piece_start;
+    int64_t edges = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int64_t bytes = 0;
+};
+
+namespace {
+
+constexpr int64_t GEN_UNIT = 64 * GEN_ROUNDS;  // records of one generation wavefront
+constexpr int64_t DEFAULT_BATCH_RAYS = int64_t(1) << 27;
+
+int64_t align_gen(int64_t rays) { return (rays + GEN_UNIT - 1) / GEN_UNIT * GEN_UNIT; }
+
+void free_trace_batch(prh_trace_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (hipEvent_t e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    void *ptrs[] = {b->segs_dev, b->seeds,        b->dda,      b->walls,    b->fin,      b->uniq_recv, b->uniq_cnt,
+                    b->n_unique, b->offsets,      b->scan_scratch, b->counters, b->marks, b->e_origin,
+                    b->e_receiver, b->e_count};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+void count_launch(const prh_trace_batch *b, const BatchPiece &P, hipStream_t st)
+{
+    const Geometry &g = b->geos[P.seg]->g;
+    const int R = b->R[P.seg], S = b->S[P.seg];
+    const int64_t np = P.b - P.a;
+    int SRp2 = 1;
+    while (SRp2 < R * S) SRp2 <<= 1;
+    const uint2 *fin = b->fin + P.ray_off;
+    int32_t *ur = b->uniq_recv + P.ray_off, *uc = b->uniq_cnt + P.ray_off, *nu = b->n_unique + P.patch_off;
+    unsigned long long *failed = b->counters + b->n_seg;
+    if (SRp2 <= 1024) {
+        const int epl = SRp2 <= 64 ? 1 : SRp2 / 64;
+        const int64_t blocks = (np + COUNT_THREADS / 64 - 1) / (COUNT_THREADS / 64);
+        const unsigned wgrid = (unsigned)(blocks < 65536 ? blocks : 65536);
+        auto kern = epl == 1 ? k_count_wave<1> : epl == 2 ? k_count_wave<2> : epl == 4 ? k_count_wave<4>
+                    : epl == 8 ? k_count_wave<8> : k_count_wave<16>;
+        kern<<<wgrid, COUNT_THREADS, 0, st>>>(fin, view_of(g), np, R, S, ur, uc, nu, failed);
+    } else {
+        const unsigned cgrid = (unsigned)(np < 65536 ? np : 65536);
+        const size_t lds = sizeof(int32_t) * (SRp2 + COUNT_THREADS);
+        k_count_patches<<<cgrid, COUNT_THREADS, lds, st>>>(fin, view_of(g), np, R, S, SRp2, ur, uc, nu, failed);
+    }
+}
+
+// The whole pass on the batch's stream (captured into the graph).
+int enqueue_batch(prh_trace_batch *b, const BatchVariant &V, size_t lds)
+{
+    hipStream_t st = b->stream;
+    const int n_pieces = (int)b->pieces.size();
+    k_zero_u64<<<(b->n_seg + 256) / 256, 256, 0, st>>>(b->counters, b->n_seg + 1);
+    k_zero_u64<<<(n_pieces + 256) / 256, 256, 0, st>>>(reinterpret_cast<unsigned long long *>(b->marks),
+                                                       n_pieces + 1);
+    const int n_chunks = (int)b->chunk_waves.size();
+    for (int c = 0; c < n_chunks; ++c) {
+        const int j0 = b->chunk_first[c], j1 = b->chunk_first[c + 1];
+        const unsigned gen_blocks = (unsigned)((b->chunk_waves[c] * 64 + TRACE_THREADS - 1) / TRACE_THREADS);
+        k_gen_rays_batch<<<gen_blocks, TRACE_THREADS, 0, st>>>(b->segs_dev + j0, j1 - j0);
+        V.fn<<<(unsigned)b->chunk_blocks[c], STREAM_THREADS, lds, st>>>(b->segs_dev + j0, j1 - j0, b->dda,
+                                                                         b->walls, b->fin);
+        PRH_HIP(hipGetLastError());
+        for (int j = j0; j < j1; ++j) {
+            const BatchPiece &P = b->pieces[j];
+            const int64_t np = P.b - P.a;
+            count_launch(b, P, st);
+            PRH_HIP(hipGetLastError());
+            int rc = exclusive_scan_i32_to_i64_carry_async(b->n_unique + P.patch_off, b->offsets + P.patch_off, np,
+                                                           b->scan_scratch, b->marks, b->marks + 1 + j, st);
+            if (rc) return rc;
+            const unsigned cgrid = (unsigned)(np < 65536 ? np : 65536);
+            k_compact_edges<<<cgrid, COUNT_THREADS, 0, st>>>(
+                b->uniq_recv + P.ray_off, b->uniq_cnt + P.ray_off, b->n_unique + P.patch_off,
+                b->offsets + P.patch_off, np, b->R[P.seg] * b->S[P.seg], P.a, b->e_origin, b->e_receiver,
+                b->e_count, b->edge_cap);
+            PRH_HIP(hipGetLastError());
+        }
+    }
+    return PRH_OK;
+}
+
+int fail_batch(prh_trace_batch *b, int rc)
+{
+    free_trace_batch(b);
+    delete b;
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int prh_trace_batch_create(int32_t n_segments, const prh_geometry *const *geometries, const int64_t *patch_begin,
+                           const int64_t *patch_end, const int32_t *rays_per_patch, const int32_t *n_seeds,
+                           int64_t ray_capacity, int64_t edge_capacity, prh_trace_batch **out)
+{
+    PRH_CHECK_ARG(out && geometries && patch_begin && patch_end && rays_per_patch && n_seeds,
+                  "prh_trace_batch_create: null pointer");
+    *out = nullptr;
+    PRH_CHECK_ARG(n_segments >= 1, "prh_trace_batch_create: no segments");
+    PRH_CHECK_ARG(ray_capacity >= 0 && edge_capacity >= 0, "prh_trace_batch_create: negative capacity");
+    const int device = geometries[0] ? geometries[0]->g.device : 0;
+    int64_t total = 0, widest = 0, worst_edges = 0;
+    for (int k = 0; k < n_segments; ++k) {
+        PRH_CHECK_ARG(geometries[k], "prh_trace_batch_create: null geometry of segment %d", k);
+        const Geometry &g = geometries[k]->g;
+        PRH_CHECK_ARG(g.device == device, "prh_trace_batch_create: segments on different devices");
+        PRH_CHECK_ARG(0 <= patch_begin[k] && patch_begin[k] <= patch_end[k] && patch_end[k] <= g.n_patches,
+                      "prh_trace_batch_create: patch range of segment %d out of bounds", k);
+        PRH_CHECK_ARG(rays_per_patch[k] >= 1 && n_seeds[k] >= 1, "prh_trace_batch_create: segment %d has no rays", k);
+        const int64_t SR = (int64_t)rays_per_patch[k] * n_seeds[k];
+        if (SR > MAX_RAYS_PER_PATCH_CALL) {
+            set_error("prh_trace_batch_create: segment %d has %lld rays per patch (at most %d)", k, (long long)SR,
+                      MAX_RAYS_PER_PATCH_CALL);
+            return PRH_ERR_CAPACITY;
+        }
+        const int64_t np = patch_end[k] - patch_begin[k];
+        total += np * SR;
+        widest = std::max(widest, align_gen(SR));
+        worst_edges += np * std::min<int64_t>(SR, g.n_patches);
+    }
+    int64_t cap = ray_capacity > 0 ? ray_capacity : std::min(DEFAULT_BATCH_RAYS, align_gen(std::max<int64_t>(total, 1)));
+    cap = std::max(cap / GEN_UNIT * GEN_UNIT, widest);
+    if (cap >= (int64_t(1) << 32)) {
+        set_error("prh_trace_batch_create: a chunk of %lld records exceeds 2^32", (long long)cap);
+        return PRH_ERR_CAPACITY;
+    }
+    PRH_HIP(hipSetDevice(device));
+    int rc = ensure_tables();
+    if (rc) return rc;
+    prh_trace_batch *b = new prh_trace_batch();
+    b->device = device;
+    b->n_seg = n_segments;
+    b->ray_cap = cap;
+    b->rays = total;
+    b->edge_cap = edge_capacity > 0 ? edge_capacity : std::max<int64_t>(worst_edges, 1);
+    for (int k = 0; k < n_segments; ++k) {
+        b->geos.push_back(geometries[k]);
+        b->R.push_back(rays_per_patch[k]);
+        b->S.push_back(n_seeds[k]);
+        b->seed_off.push_back(b->seed_words);
+        b->seed_words += 4 * (int64_t)n_seeds[k];
+    }
+    // pieces: whole patches into chunks of at most cap records
+    int64_t used = 0, patches = 0;
+    b->chunk_first.push_back(0);
+    auto close_chunk = [&]() {
+        if ((int)b->pieces.size() == b->chunk_first.back()) return;
+        b->chunk_first.push_back((int)b->pieces.size());
+        b->patch_cap = std::max(b->patch_cap, patches);
+        used = patches = 0;
+    };
+    for (int k = 0; k < n_segments; ++k) {
+        const int64_t SR = (int64_t)rays_per_patch[k] * n_seeds[k];
+        for (int64_t p = patch_begin[k]; p < patch_end[k];) {
+            const int64_t fit = std::min(patch_end[k] - p, (cap - used) / SR);
+            if (fit == 0) {
+                close_chunk();
+                continue;
+            }
+            b->pieces.push_back(BatchPiece{k, p, p + fit, used, patches});
+            used += align_gen(fit * SR);
+            patches += fit;
+            p += fit;
+            b->scratch_elems = std::max(b->scratch_elems, scan_scratch_elems(fit));
+        }
+    }
+    close_chunk();
+    const int n_pieces = (int)b->pieces.size();
+    const int n_chunks = (int)b->chunk_first.size() - 1;
+    bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreate(&b->ev[0]) == hipSuccess && hipEventCreate(&b->ev[1]) == hipSuccess;
+    auto alloc = [&](void **ptr, int64_t bytes) {
+        bytes = std::max<int64_t>(bytes, 8);
+        if (ok && hipMalloc(ptr, (size_t)bytes) != hipSuccess) ok = false;
+        b->bytes += bytes;
+    };
+    const int64_t rec = n_chunks > 0 ? cap : 1;
+    alloc((void **)&b->segs_dev, (int64_t)sizeof(Segment) * std::max(n_pieces, 1));
+    alloc((void **)&b->seeds, (int64_t)sizeof(uint64_t) * b->seed_words);
+    alloc((void **)&b->dda, (int64_t)sizeof(DdaRec) * rec);
+    alloc((void **)&b->walls, (int64_t)sizeof(WallRec) * rec);
+    alloc((void **)&b->fin, (int64_t)sizeof(uint2) * rec);
+    alloc((void **)&b->uniq_recv, (int64_t)sizeof(int32_t) * rec);
+    alloc((void **)&b->uniq_cnt, (int64_t)sizeof(int32_t) * rec);
+    alloc((void **)&b->n_unique, (int64_t)sizeof(int32_t) * std::max<int64_t>(b->patch_cap, 1));
+    alloc((void **)&b->offsets, (int64_t)sizeof(int64_t) * std::max<int64_t>(b->patch_cap, 1));
+    alloc((void **)&b->scan_scratch, (int64_t)sizeof(int64_t) * std::max<int64_t>(b->scratch_elems, 1));
+    alloc((void **)&b->counters, (int64_t)sizeof(unsigned long long) * (n_segments + 1));
+    alloc((void **)&b->marks, (int64_t)sizeof(int64_t) * (n_pieces + 1));
+    alloc((void **)&b->e_origin, (int64_t)sizeof(int32_t) * b->edge_cap);
+    alloc((void **)&b->e_receiver, (int64_t)sizeof(int32_t) * b->edge_cap);
+    alloc((void **)&b->e_count, (int64_t)sizeof(int32_t) * b->edge_cap);
+    if (!ok) {
+        set_error("prh_trace_batch_create: out of device memory (%lld bytes)", (long long)b->bytes);
+        return fail_batch(b, PRH_ERR_HIP);
+    }
+    // launch shapes: generation wavefronts per piece, and the resident
+    // traversal wavefronts of each chunk shared out in proportion to rays
+    const BatchVariant &V = batch_variant_for(g_trace_variant);
+    uint32_t coarse_words = 0;
+    if (V.occ == OCC_BRICK)
+        for (const prh_geometry *g : b->geos) coarse_words = std::max<uint32_t>(coarse_words, (uint32_t)g->g.coarse_words);
+    const size_t lds = (size_t)coarse_words * sizeof(uint32_t);
+    int64_t waves = 0;
+    if ((rc = resident_waves((const void *)V.fn, STREAM_THREADS, lds, &waves))) return fail_batch(b, rc);
+    const int64_t wpb = STREAM_THREADS / 64;
+    std::vector<Segment> segs(n_pieces);
+    for (int c = 0; c < n_chunks; ++c) {
+        const int j0 = b->chunk_first[c], j1 = b->chunk_first[c + 1];
+        int64_t chunk_rays = 0;
+        for (int j = j0; j < j1; ++j) {
+            const BatchPiece &P = b->pieces[j];
+            chunk_rays += (P.b - P.a) * b->R[P.seg] * b->S[P.seg];
+        }
+        int64_t wave_at = 0, block_at = 0;
+        for (int j = j0; j < j1; ++j) {
+            const BatchPiece &P = b->pieces[j];
+            const Geometry &g = b->geos[P.seg]->g;
+            const int64_t rays = (P.b - P.a) * b->R[P.seg] * b->S[P.seg];
+            Segment &sg = segs[j];
+            sg.gen = GenArgs{view_of(g), traversal_const(g), (uint32_t)P.a, (uint32_t)(P.b - P.a),
+                             (uint32_t)b->R[P.seg], b->seeds + b->seed_off[P.seg], (uint32_t)rays,
+                             (uint32_t)P.ray_off, nullptr, nullptr, b->dda, b->walls};
+            sg.gen_wave_begin = (uint32_t)wave_at;
+            sg.reads = b->counters + P.seg;
+            wave_at += (rays + GEN_UNIT - 1) / GEN_UNIT;
+            const double share = (double)rays / (double)chunk_rays;
+            int64_t blocks = std::max<int64_t>(1, std::llround(share * (double)waves / (double)wpb));
+            int64_t per_wave = std::max<int64_t>(64, (rays + blocks * wpb - 1) / (blocks * wpb));
+            blocks = std::max<int64_t>(1, (rays + per_wave * wpb - 1) / (per_wave * wpb));
+            sg.per_wave = (uint32_t)per_wave;
+            sg.trace_block_begin = (uint32_t)block_at;
+            block_at += blocks;
+        }
+        b->chunk_waves.push_back(wave_at);
+        b->chunk_blocks.push_back(block_at);
+    }
+    if (n_pieces > 0)
+        PRH_HIP(hipMemcpy(b->segs_dev, segs.data(), sizeof(Segment) * n_pieces, hipMemcpyHostToDevice));
+    b->piece_start.assign(n_pieces + 1, 0);
+    b->seg_nnz.assign(n_segments, 0);
+    // capture the pass once; each run replays it
+    if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        set_error("prh_trace_batch_create: stream capture did not start");
+        return fail_batch(b, PRH_ERR_HIP);
+    }
+    rc = enqueue_batch(b, V, lds);
+    hipGraph_t graph = nullptr;
+    const hipError_t end = hipStreamEndCapture(b->stream, &graph);
+    b->graph = graph;
+    if (rc) return fail_batch(b, rc);
+    if (end != hipSuccess || !graph) {
+        set_error("prh_trace_batch_create: graph capture failed: %s", hipGetErrorString(end));
+        return fail_batch(b, PRH_ERR_HIP);
+    }
+    if (hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        set_error("prh_trace_batch_create: graph instantiation failed");
+        return fail_batch(b, PRH_ERR_HIP);
+    }
+    *out = b;
+    return PRH_OK;
+}
+
+int prh_trace_batch_run(prh_trace_batch *b, const uint64_t *state_inc, int64_t *nnz, int64_t *occupancy_reads,
+                        double *elapsed_ms)
+{
+    PRH_CHECK_ARG(b && state_inc, "prh_trace_batch_run: null pointer");
+    PRH_HIP(hipSetDevice(b->device));
+    const int n = b->n_seg;
+    const int n_pieces = (int)b->pieces.size();
+    std::vector<unsigned long long> ctr(n + 1);
+    PRH_HIP(hipMemcpyAsync(b->seeds, state_inc, sizeof(uint64_t) * b->seed_words, hipMemcpyHostToDevice, b->stream));
+    PRH_HIP(hipEventRecord(b->ev[0], b->stream));
+    PRH_HIP(hipGraphLaunch(b->exec, b->stream));
+    PRH_HIP(hipEventRecord(b->ev[1], b->stream));
+    PRH_HIP(hipMemcpyAsync(ctr.data(), b->counters, sizeof(unsigned long long) * (n + 1), hipMemcpyDeviceToHost,
+                           b->stream));
+    PRH_HIP(hipMemcpyAsync(b->piece_start.data(), b->marks, sizeof(int64_t) * (n_pieces + 1),
+                           hipMemcpyDeviceToHost, b->stream));
+    PRH_HIP(hipStreamSynchronize(b->stream));
+    if (elapsed_ms) {
+        float ms = 0.f;
+        PRH_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+        *elapsed_ms = ms;
+    }
+    // marks: [0] edges written, [1 + j] first edge of piece j
+    b->edges = b->piece_start[0];
+    std::fill(b->seg_nnz.begin(), b->seg_nnz.end(), 0);
+    for (int j = 0; j < n_pieces; ++j) {
+        const int64_t next = j + 1 < n_pieces ? b->piece_start[j + 2] : b->edges;
+        b->seg_nnz[b->pieces[j].seg] += next - b->piece_start[j + 1];
+    }
+    for (int k = 0; k < n; ++k) {
+        if (nnz) nnz[k] = b->seg_nnz[k];
+        if (occupancy_reads) occupancy_reads[k] = (int64_t)ctr[k];
+    }
+    if (ctr[n]) {
+        set_error("%llu rays did not resolve a receiving patch", ctr[n]);
+        return PRH_ERR_UNRESOLVED;
+    }
+    if (b->edges > b->edge_cap) {
+        set_error("prh_trace_batch_run: %lld edges exceed the edge capacity %lld", (long long)b->edges,
+                  (long long)b->edge_cap);
+        return PRH_ERR_CAPACITY;
+    }
+    return PRH_OK;
+}
+
+int prh_trace_batch_edges(const prh_trace_batch *b, int32_t segment, int32_t *origin, int32_t *receiver,
+                          int64_t *count)
+{
+    PRH_CHECK_ARG(b && segment >= 0 && segment < b->n_seg, "prh_trace_batch_edges: bad segment");
+    PRH_CHECK_ARG(b->edges <= b->edge_cap, "prh_trace_batch_edges: the last run overflowed the edge arena");
+    PRH_HIP(hipSetDevice(b->device));
+    const int n_pieces = (int)b->pieces.size();
+    int64_t at = 0;
+    std::vector<int32_t> tmp;
+    for (int j = 0; j < n_pieces; ++j) {
+        if (b->pieces[j].seg != segment) continue;
+        const int64_t first = b->piece_start[j + 1];
+        const int64_t m = (j + 1 < n_pieces ? b->piece_start[j + 2] : b->edges) - first;
+        if (m <= 0) continue;
+        if (origin) PRH_HIP(hipMemcpy(origin + at, b->e_origin + first, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (receiver)
+            PRH_HIP(hipMemcpy(receiver + at, b->e_receiver + first, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (count) {
+            tmp.resize(m);
+            PRH_HIP(hipMemcpy(tmp.data(), b->e_count + first, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < m; ++i) count[at + i] = tmp[i];
+        }
+        at += m;
+    }
+    return PRH_OK;
+}
+
+int prh_trace_batch_info(const prh_trace_batch *b, int64_t *info, int32_t n)
+{
+    PRH_CHECK_ARG(b && info && n >= 0, "prh_trace_batch_info: null pointer");
+    const int64_t v[] = {b->bytes,   (int64_t)b->chunk_waves.size(), (int64_t)b->pieces.size(),
+                         b->ray_cap, b->edge_cap,                    b->rays};
+    for (int i = 0; i < n && i < 6; ++i) info[i] = v[i];
+    return PRH_OK;
+}
+
+void prh_trace_batch_destroy(prh_trace_batch *b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    free_trace_batch(b);
+    delete b;
 }
 
 int prh_edges_copy(const prh_edges *e, int32_t *origin, int32_t *receiver, int64_t *count)

@@ -52,6 +52,11 @@ SIGNATURES = {
     "prh_trace_rays": ([_P, _P, _P, _I64, _P, _P], ctypes.c_int),
     "prh_trace_counts": ([_P, _I64, _I64, _I32, _I32, _P, _P, _P, _P], ctypes.c_int),
     "prh_edges_copy": ([_P, _P, _P, _P], ctypes.c_int),
+    "prh_trace_batch_create": ([_I32, _P, _P, _P, _P, _P, _I64, _I64, _P], ctypes.c_int),
+    "prh_trace_batch_run": ([_P, _P, _P, _P, _P], ctypes.c_int),
+    "prh_trace_batch_edges": ([_P, _I32, _P, _P, _P], ctypes.c_int),
+    "prh_trace_batch_info": ([_P, _P, _I32], ctypes.c_int),
+    "prh_trace_batch_destroy": ([_P], None),
     "prh_edges_destroy": ([_P], None),
     "prh_operator_create": ([_I64, _P, _P, _P, _I64, _P, _P, _F64, _P], ctypes.c_int),
     "prh_operator_set_scale": ([_P, _P], ctypes.c_int),
@@ -272,6 +277,75 @@ def trace_counts(geometry: Geometry, p0: int, p1: int, rays_per_patch: int, seed
                                  ptr(states), ctypes.byref(handle), ctypes.byref(nnz), ctypes.byref(reads)),
           "prh_trace_counts")
     return Edges(handle, nnz.value, reads.value)
+
+
+class TraceBatch:
+    """Owner of a graph-captured multi-geometry trace pass (``prh_trace_batch``).
+
+    ``segments`` is a list of ``(geometry, p0, p1, rays_per_patch, n_seeds)``;
+    every :meth:`run` replays the pass with one seed list per segment and
+    leaves each segment's pooled edge table on the device.
+    """
+
+    INFO = ("device_bytes", "chunks", "pieces", "ray_capacity", "edge_capacity", "rays")
+
+    def __init__(self, segments, ray_capacity: int = 0, edge_capacity: int = 0) -> None:
+        self._lib = lib()
+        self.geometries = [seg[0] for seg in segments]  # kept alive with the batch
+        n = len(segments)
+        handles = (ctypes.c_void_p * n)(*[g.handle.value for g in self.geometries])
+        p0 = np.array([seg[1] for seg in segments], np.int64)
+        p1 = np.array([seg[2] for seg in segments], np.int64)
+        rays = np.array([seg[3] for seg in segments], np.int32)
+        self.n_seeds = np.array([seg[4] for seg in segments], np.int32)
+        handle = ctypes.c_void_p()
+        check(self._lib.prh_trace_batch_create(n, ctypes.cast(handles, ctypes.c_void_p), ptr(p0), ptr(p1), ptr(rays),
+                                               ptr(self.n_seeds), int(ray_capacity), int(edge_capacity),
+                                               ctypes.byref(handle)), "prh_trace_batch_create")
+        self.handle = handle
+        self.n = n
+        self.nnz = np.zeros(n, np.int64)
+        self.occupancy_reads = np.zeros(n, np.int64)
+        self.elapsed_ms = 0.0
+        info = np.zeros(len(self.INFO), np.int64)
+        check(self._lib.prh_trace_batch_info(handle, ptr(info), len(self.INFO)), "prh_trace_batch_info")
+        self.info = dict(zip(self.INFO, (int(v) for v in info)))
+
+    def run(self, seeds) -> None:
+        """Replay the pass; ``seeds[k]`` lists segment k's ``n_seeds[k]`` seeds."""
+
+        if len(seeds) != self.n:
+            raise ValueError(f"expected seeds for {self.n} segments")
+        rows = []
+        for k, block in enumerate(seeds):
+            if len(block) != int(self.n_seeds[k]):
+                raise ValueError(f"segment {k} takes {int(self.n_seeds[k])} seeds")
+            rows.extend(pcg64_state(s) for s in block)
+        states = np.ascontiguousarray(np.stack(rows), dtype=np.uint64)
+        ms = ctypes.c_double()
+        check(self._lib.prh_trace_batch_run(self.handle, ptr(states), ptr(self.nnz), ptr(self.occupancy_reads),
+                                            ctypes.byref(ms)), "prh_trace_batch_run")
+        self.elapsed_ms = float(ms.value)
+
+    def edges(self, segment: int):
+        """(origin, receiver, count) of a segment from the last run."""
+
+        m = int(self.nnz[segment])
+        origin, receiver, count = np.empty(m, np.int32), np.empty(m, np.int32), np.empty(m, np.int64)
+        check(self._lib.prh_trace_batch_edges(self.handle, int(segment), ptr(origin), ptr(receiver), ptr(count)),
+              "prh_trace_batch_edges")
+        return origin, receiver, count
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self._lib.prh_trace_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Operator:

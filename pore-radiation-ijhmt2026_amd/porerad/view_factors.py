@@ -203,6 +203,33 @@ def first_hit_edges(patches: SurfacePatches, rays_per_patch: int, seeds, *,
     return _native.trace_counts(patches.geometry(), p0, p1, int(rays_per_patch), seeds)
 
 
+def first_hit_counts_batch(cases, *, ray_capacity: int = 0, edge_capacity: int = 0):
+    """Counts of many geometries in one graph-captured device pass.
+
+    ``cases`` lists ``(patches, rays_per_patch, seeds)``; the result lists one
+    ``(origin, receiver, count)`` per case, identical to
+    ``first_hit_counts_pooled(patches, rays_per_patch, seeds)``.  This is the
+    per-case loop of scripts/run_prescribed_field.py:81-85 in one replay:
+    uneven geometries share persistent launches, in chunks of at most
+    ``ray_capacity`` ray records (0: up to 2^27).  For repeated passes over the
+    same layout keep a :class:`porerad._native.TraceBatch` and call ``run``.
+    """
+
+    segments, seeds = [], []
+    for patches, rays_per_patch, block in cases:
+        block = [int(s) for s in block]
+        if not block:
+            raise ValueError("need at least one seed per case")
+        segments.append((patches.geometry(), 0, patches.n_patches, int(rays_per_patch), len(block)))
+        seeds.append(block)
+    batch = _native.TraceBatch(segments, ray_capacity=ray_capacity, edge_capacity=edge_capacity)
+    try:
+        batch.run(seeds)
+        return [batch.edges(k) for k in range(len(segments))]
+    finally:
+        batch.close()
+
+
 def pool_counts(tables, n_patches: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
     """Sum the integer edge counts of several ray blocks into one sorted edge list (view_factors.py:290-300)."""
 

@@ -186,10 +186,11 @@ def test_every_traversal_variant_gives_identical_counts(vf):
     ref = O.first_hit_counts(Q, 300, 77)
     variants = _native.tracer_variants()
     assert len(variants) >= 2
+    default = next(v["variant"] for v in variants if v["current"])
     try:
         for v in variants:
             _native.set_tracer_variant(v["variant"])
             got = vf.first_hit_counts(P, 300, 77)
             assert all(np.array_equal(a, b) for a, b in zip(got, ref)), v
     finally:
-        _native.set_tracer_variant(0)
+        _native.set_tracer_variant(default)
