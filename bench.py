@@ -190,6 +190,141 @@ def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
     }
 
 
+BATCH_SIZES = (96, 128, 160, 192)
+
+
+def batching_workload():
+    """16 uneven geometries: foam, random fibres, aligned fibres and perforated plates at N = 96..192."""
+
+    from porerad import synthetic
+
+    families = [
+        # foam cells of constant size: 64 Voronoi seeds per 128^3
+        ("foam", lambda N: synthetic.voronoi_foam(N, int(round(64 * (N / 128.0) ** 3)), 0.10, 2)),
+        ("random_fibre", lambda N: synthetic.random_fibers(N, 0.10, 3.0, 0)),
+        ("aligned_fibre", lambda N: synthetic.aligned_fibers(N, 0.10, 4.0, 0)),
+        ("perforated_plates", lambda N: synthetic.layered_plates(N, 0.10, 0, thickness=2, hole=6,
+                                                                 hole_fraction=0.15)),
+    ]
+    return [(f"{name}{N}", fn(N)) for N in BATCH_SIZES for name, fn in families]
+
+
+def run_batching(args) -> dict:
+    """One graph-captured trace pass over 16 uneven geometries (seeds 10..25, R rays per patch), replayed
+    K times, against the same rays traced geometry by geometry; then 64 radiosity right-hand sides per
+    geometry through the served engines."""
+
+    from porerad import _native, autotune, exchange, serving, view_factors
+
+    R = args.rays
+    t = time.perf_counter()
+    geos = []
+    for name, phases in batching_workload():
+        geos.append((name, view_factors.build_surface_patches(phases, 0.01)))
+    seeds = [[10 + k] for k in range(len(geos))]
+    n_rays = sum(P.n_patches for _, P in geos) * R
+    log(f"batch geometries ({time.perf_counter() - t:.1f} s): " +
+        ", ".join(f"{name} {P.n_patches}" for name, P in geos) + f"; {n_rays} rays per pass")
+    segments = [(P.geometry(), 0, P.n_patches, R, 1) for _, P in geos]
+    tune, capacity = None, 0
+    if args.tune:
+        tune = autotune.tune_batch(segments, seeds, use_cache=False)
+        capacity = tune["ray_capacity"]
+        log("batch tuning: " + ", ".join(f"v{r['variant']}={r['graph_ms']:.1f}ms" for r in tune["table"]) +
+            "; chunks " + ", ".join(f"{r['ray_capacity']}={r['graph_ms']:.1f}ms" for r in tune["capacity_table"]))
+
+    # plain path: one prh_trace_counts call per geometry
+    seq_nnz = []
+    largest = max(range(len(geos)), key=lambda k: geos[k][1].n_patches)
+    view_factors.first_hit_edges(geos[largest][1], R, seeds[largest]).close()  # workspace sized
+    t = time.perf_counter()
+    for k, (_, P) in enumerate(geos):
+        e = view_factors.first_hit_edges(P, R, seeds[k])
+        seq_nnz.append(e.nnz)
+        e.close()
+    sequential_s = time.perf_counter() - t
+
+    t = time.perf_counter()
+    batch = _native.TraceBatch(segments, ray_capacity=capacity)
+    create_s = time.perf_counter() - t
+    log(f"trace batch: {batch.info} (created and captured in {create_s:.2f} s)")
+    for _ in range(args.warmup):
+        batch.run(seeds)
+    wall, dev = [], []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        batch.run(seeds)
+        wall.append(time.perf_counter() - t)
+        dev.append(batch.elapsed_ms * 1e-3)
+    step = float(np.mean(wall))
+    if list(batch.nnz) != seq_nnz:
+        raise RuntimeError("batched and per-geometry edge counts differ")
+    reads = int(batch.occupancy_reads.sum())
+    log(f"batched pass {step * 1e3:.1f} ms ({n_rays / step:.3e} rays/s, graph {np.mean(dev) * 1e3:.1f} ms); "
+        f"per-geometry calls {sequential_s * 1e3:.1f} ms ({n_rays / sequential_s:.3e} rays/s)")
+    algo_bytes = 4.0 * n_rays + float(reads)
+    graph_s = float(np.mean(dev))
+    record = {
+        "metric": "first-hit rays/s over 16 uneven geometries in one graph replay",
+        "value": n_rays / step,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "foam, random fibre, aligned fibre, perforated plates x N in "
+                               f"{list(BATCH_SIZES)}, seeds 10-25, {R} rays/patch",
+                   "geometries": {name: int(P.n_patches) for name, P in geos}, "rays_per_step": int(n_rays),
+                   "edges": int(batch.nnz.sum())},
+        "batching": {"graph_ms": graph_s * 1e3, "per_geometry_calls_ms": sequential_s * 1e3,
+                     "speedup_vs_per_geometry_calls": sequential_s / step, "create_and_capture_s": create_s,
+                     "layout": batch.info},
+        "roofline": {"bound": "hbm", "achieved": algo_bytes / graph_s / 1e9, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": algo_bytes / graph_s / 1e9 / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": "whole graph (generation + traversal + counting + compaction)",
+                     "algorithmic_bytes_per_launch": algo_bytes, "occupancy_reads_per_ray": reads / n_rays},
+    }
+    if tune is not None:
+        record["tuning"] = tune
+    if args.no_radiosity:
+        batch.close()
+        return record
+
+    # 64 right-hand sides per geometry: (T_hot, emissivity) pairs through the served engines.  The
+    # edge tables go to host memory first so that the batch's device memory is free for the operators.
+    tables = [batch.edges(k) for k in range(len(geos))]
+    batch.close()
+    t = time.perf_counter()
+    srv = serving.RadiosityServer()
+    for k, (name, P) in enumerate(geos):
+        o, r, c = tables[k]
+        tables[k] = None
+        srv.add_geometry(name, P, exchange.ExchangeOperator(o, r, c, np.full(P.n_patches, R, np.int64),
+                                                            P.patch_area_m2))
+        del o, r, c
+        log(f"operator {name}: {time.perf_counter() - t:.1f} s")
+    setup = time.perf_counter() - t
+    rng = np.random.default_rng(12)
+    reqs = [serving.RadiosityRequest(name, float(rng.uniform(400.0, 1000.0)), 300.0, float(rng.uniform(0.2, 0.95)),
+                                     1e-9, 0.0) for name, _ in geos for _ in range(64)]
+    srv.run(reqs[:: len(reqs) // 32], real_time=False)  # engines allocated
+    t = time.perf_counter()
+    replies = srv.run(reqs, real_time=False)
+    rhs = serving.latency_summary(replies, time.perf_counter() - t)
+    rhs.update({"right_hand_sides": len(reqs), "per_geometry": 64, "operator_setup_s": round(setup, 1),
+                "precision": "fp32 chebyshev sweeps + fp64 refinement, rtol 1e-9", "sweeps": srv.sweeps})
+    srv.close()
+    log(f"64 RHS x {len(geos)} geometries: {rhs['solves_per_s']:.1f} solves/s, p50 {rhs['p50_ms']:.1f} ms, "
+        f"p99 {rhs['p99_ms']:.1f} ms (operators {setup:.1f} s)")
+    record["right_hand_sides"] = rhs
+    return record
+
+
 def cpu_baseline(phases, R, seed, budget_s=12.0):
     from oracle import oracle as O
 
@@ -241,6 +376,8 @@ def main() -> None:
     ap.add_argument("--no-radiosity", action="store_true", help="skip the radiosity solves")
     ap.add_argument("--tune", action="store_true", help="time every tracer variant first and use the fastest")
     ap.add_argument("--copy-probe", action="store_true", help="only run one plain 1 GiB device copy")
+    ap.add_argument("--batching", action="store_true",
+                    help="the 16-geometry batched trace pass (+ 64 radiosity RHS per geometry) instead")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -253,6 +390,9 @@ def main() -> None:
     from porerad import _native, view_factors
 
     log(_native.lib().prh_version().decode())
+    if args.batching:
+        print(json.dumps(run_batching(args)), flush=True)
+        return
     t = time.perf_counter()
     phases = foam_workload()
     P = view_factors.build_surface_patches(phases, 0.01)

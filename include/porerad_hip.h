@@ -85,6 +85,11 @@ int prh_last_timings(double *ms, int32_t n);
  * the pointers are non-NULL (arrays of n_variants entries). */
 int prh_tracer_set_variant(int32_t variant);
 int prh_tracer_variants(int32_t *n_variants, int32_t *current, int32_t *refill_min, int32_t *occupancy_mode);
+/* Tuning knob of the traversal kernel: records a wavefront takes from the
+ * launch's shared work queue at a time (64 .. 2^20, default 2048).  No
+ * counterpart in the reference; results do not depend on it. */
+int prh_tracer_set_grain(int32_t records);
+int prh_tracer_grain(int32_t *records);
 
 /* numpy.random.PCG64(seed) seeding: SeedSequence over the little-endian
  * uint32 words of the seed.  state_inc = {state_hi, state_lo, inc_hi, inc_lo}.

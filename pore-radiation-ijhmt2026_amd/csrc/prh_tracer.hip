@@ -185,6 +185,7 @@ __device__ __forceinline__ void make_dda(const double o[3], const double d[3], c
 }
 
 // Traversal state of ray g from its record; false for a ray that fails at once.
+// false for a ray that fails (dx == 0).
 __device__ __forceinline__ bool ray_load(Ray &R, const DdaRec *recs, uint32_t g)
 {
     const double2 *p = reinterpret_cast<const double2 *>(recs + g);
@@ -476,7 +477,8 @@ struct Segment {
     GenArgs gen;
     uint32_t gen_wave_begin;    // first generation wavefront of the segment
     uint32_t trace_block_begin;  // first traversal workgroup of the segment
-    uint32_t per_wave;           // traversal rays per wavefront
+    uint32_t grain;              // records per queue grab
+    unsigned long long *queue;   // traversal queue of the segment (zero at launch)
     unsigned long long *reads;   // occupancy-read counter of the segment
 };
 
@@ -492,7 +494,8 @@ __device__ __forceinline__ int segment_of(const Segment *segs, int n_seg, uint32
 // inside one segment).
 __global__ __launch_bounds__(TRACE_THREADS) void k_gen_rays_batch(const Segment *segs, int n_seg)
 {
-    const uint32_t wave = (blockIdx.x * TRACE_THREADS + threadIdx.x) >> 6;
+    // wave-uniform, so the segment's arguments load into scalar registers
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * TRACE_THREADS + threadIdx.x) >> 6);
     const int k = segment_of(segs, n_seg, wave, false);
     const GenArgs a = segs[k].gen;
     gen_rays_body<true>(a, (uint64_t)(wave - segs[k].gen_wave_begin) * (64ull * GEN_ROUNDS) + (threadIdx.x & 63u));
@@ -515,29 +518,34 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_setup_rays(GeoView G, Travers
 }
 
 // ------------------------------------------------------------------ traversal
-// Each wavefront owns a contiguous chunk of rays and a cursor into it.  It
-// keeps stepping while its lanes are busy; once at least REFILL_MIN lanes
-// are idle, those lanes write their final states and load the next records
-// of the chunk in lane order (adjacent 64 B records, so the refill loads
-// coalesce).
+// Each wavefront takes grains of consecutive records from a shared counter
+// (the queue) and streams them through its lanes: it keeps stepping while
+// its lanes are busy; once at least REFILL_MIN lanes are idle, those lanes
+// write their final states and load the next records in lane order
+// (adjacent 64 B records, so the refill loads coalesce).  Work taken from a
+// queue rather than a fixed range keeps every wavefront busy until the
+// records run out, whatever the spread of ray lengths, so the launch ends
+// within about one grain of work.
 struct TraceArgs {
     GeoView G;
-    uint32_t total, per_wave;
+    uint32_t total, grain;
     const DdaRec *dda;
     const WallRec *walls;
     uint2 *fin;
     unsigned long long *reads;
+    unsigned long long *queue;  // records handed out so far; zero at launch
 };
 
-// Rays [w_begin, w_end) of one wavefront, streamed with refills.
+// Records [base, base + total) streamed by the wavefronts that share queue.
 template <int REFILL_MIN, int OCC>
-__device__ __forceinline__ void trace_body(const GeoView &G, uint32_t w_begin, uint32_t w_end, const DdaRec *dda,
-                                           const WallRec *walls, uint2 *fin, unsigned long long *reads_out,
-                                           const uint32_t *lds_coarse)
+__device__ __forceinline__ void trace_body(const GeoView &G, uint32_t base, uint32_t total, uint32_t grain,
+                                           unsigned long long *queue, const DdaRec *dda, const WallRec *walls,
+                                           uint2 *fin, unsigned long long *reads_out, const uint32_t *lds_coarse)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
-    uint32_t cursor = w_begin;
+    uint32_t cursor = 0, w_end = 0;  // the wavefront's grain [cursor, w_end)
+    bool more = true;                // the queue may still hold records
 
     Ray R;
     R.i = 0;
@@ -551,6 +559,19 @@ __device__ __forceinline__ void trace_body(const GeoView &G, uint32_t w_begin, u
             reads += ray_reads(R, G);
             fin[g_cur] = ray_final(R, G, walls, g_cur);
             pending = false;
+        }
+        if (more && cursor >= w_end) {
+            // next grain (the counter is 64-bit: it passes total without wrapping)
+            unsigned long long first = 0;
+            if (lane == 0) first = atomicAdd(queue, (unsigned long long)grain);
+            first = __shfl(first, 0);
+            if (first >= total) {
+                more = false;
+            } else {
+                cursor = base + (uint32_t)first;
+                const unsigned long long last = first + grain;
+                w_end = base + (last < total ? (uint32_t)last : total);
+            }
         }
         if (cursor < w_end) {
             const uint64_t idle = __ballot(!active);
@@ -569,7 +590,7 @@ __device__ __forceinline__ void trace_body(const GeoView &G, uint32_t w_begin, u
             cursor += (uint32_t)__popcll(idle);
         }
         if (__ballot(active) == 0) {
-            if (cursor >= w_end) break;
+            if (!more) break;  // no grain left (cursor >= w_end)
             continue;
         }
         while (true) {
@@ -578,7 +599,7 @@ __device__ __forceinline__ void trace_body(const GeoView &G, uint32_t w_begin, u
             if (active) active = ray_step<OCC>(R, G, lds_coarse);
             const uint64_t act = __ballot(active);
             if (act == 0) break;
-            if (cursor < w_end && __popcll(~act) >= REFILL_MIN) break;
+            if ((more || cursor < w_end) && __popcll(~act) >= REFILL_MIN) break;
         }
     }
     reads = wave_sum_u64(reads);
@@ -595,16 +616,14 @@ void k_trace_stream(TraceArgs a)
         for (uint32_t w = threadIdx.x; w < G.coarse_words; w += STREAM_THREADS) lds_coarse[w] = G.coarse[w];
         __syncthreads();
     }
-    const uint32_t wave = (blockIdx.x * STREAM_THREADS + threadIdx.x) >> 6;
-    const uint64_t wb = (uint64_t)wave * a.per_wave;
-    const uint32_t w_begin = wb < a.total ? (uint32_t)wb : a.total;
-    const uint32_t w_end = wb + a.per_wave < a.total ? (uint32_t)(wb + a.per_wave) : a.total;
-    trace_body<REFILL_MIN, OCC>(G, w_begin, w_end, a.dda, a.walls, a.fin, a.reads, lds_coarse);
+    trace_body<REFILL_MIN, OCC>(G, 0u, a.total, a.grain, a.queue, a.dda, a.walls, a.fin, a.reads, lds_coarse);
 }
 
-// Every segment of a batch in one persistent launch: each workgroup works
-// inside one segment (its LDS holds that geometry's brick bitset), so
-// geometries of uneven size share the chip without a launch each.
+// Every piece of a chunk in one launch: each workgroup works inside one piece
+// (its LDS holds that geometry's brick bitset) and its wavefronts take grains
+// from the piece's queue.  Each piece has up to a chip's worth of
+// workgroups, dispatched in piece order, so the workgroups of the next piece
+// move onto the compute units as the previous piece drains.
 template <int REFILL_MIN, int OCC>
 __global__ __launch_bounds__(STREAM_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
 void k_trace_batch(const Segment *segs, int n_seg, const DdaRec *dda, const WallRec *walls, uint2 *fin)
@@ -616,12 +635,8 @@ void k_trace_batch(const Segment *segs, int n_seg, const DdaRec *dda, const Wall
         for (uint32_t w = threadIdx.x; w < G.coarse_words; w += STREAM_THREADS) lds_coarse[w] = G.coarse[w];
         __syncthreads();
     }
-    const uint32_t base = segs[k].gen.offset, total = segs[k].gen.total, per_wave = segs[k].per_wave;
-    const uint32_t wave = (blockIdx.x - segs[k].trace_block_begin) * (STREAM_THREADS / 64) + (threadIdx.x >> 6);
-    const uint64_t wb = (uint64_t)wave * per_wave;
-    const uint32_t w_begin = base + (wb < total ? (uint32_t)wb : total);
-    const uint32_t w_end = base + (wb + per_wave < total ? (uint32_t)(wb + per_wave) : total);
-    trace_body<REFILL_MIN, OCC>(G, w_begin, w_end, dda, walls, fin, segs[k].reads, lds_coarse);
+    trace_body<REFILL_MIN, OCC>(G, segs[k].gen.offset, segs[k].gen.total, segs[k].grain, segs[k].queue, dda, walls,
+                                fin, segs[k].reads, lds_coarse);
 }
 
 __global__ void k_zero_u64(unsigned long long *p, int n)
@@ -915,24 +930,36 @@ constexpr int N_TRACE_VARIANTS = sizeof(TRACE_VARIANTS) / sizeof(TRACE_VARIANTS[
 // the foam and fibre media measured so far.
 int g_trace_variant = 1;
 
-// Batch traversal kernels for the variants' (refill, occupancy) pairs.
+// Batch traversal kernels, one per entry of TRACE_VARIANTS (same index).
 using BatchKernel = void (*)(const Segment *, int, const DdaRec *, const WallRec *, uint2 *);
 struct BatchVariant {
     BatchKernel fn;
     int refill_min, occ;
 };
 const BatchVariant BATCH_VARIANTS[] = {
-    {k_trace_batch<16, OCC_BRICK>, 16, OCC_BRICK}, {k_trace_batch<24, OCC_BRICK>, 24, OCC_BRICK},
-    {k_trace_batch<32, OCC_BRICK>, 32, OCC_BRICK}, {k_trace_batch<16, OCC_GLOBAL>, 16, OCC_GLOBAL},
-    {k_trace_batch<32, OCC_GLOBAL>, 32, OCC_GLOBAL},
+    {k_trace_batch<32, OCC_BRICK>, 32, OCC_BRICK},   {k_trace_batch<16, OCC_BRICK>, 16, OCC_BRICK},
+    {k_trace_batch<24, OCC_BRICK>, 24, OCC_BRICK},   {k_trace_batch<40, OCC_BRICK>, 40, OCC_BRICK},
+    {k_trace_batch<48, OCC_BRICK>, 48, OCC_BRICK},   {k_trace_batch<32, OCC_GLOBAL>, 32, OCC_GLOBAL},
+    {k_trace_batch<16, OCC_GLOBAL>, 16, OCC_GLOBAL}, {k_trace_batch<24, OCC_GLOBAL>, 24, OCC_GLOBAL},
+    {k_trace_batch<16, OCC_AHEAD>, 16, OCC_AHEAD},   {k_trace_batch<24, OCC_AHEAD>, 24, OCC_AHEAD},
+    {k_trace_batch<32, OCC_AHEAD>, 32, OCC_AHEAD},
 };
+static_assert(sizeof(BATCH_VARIANTS) / sizeof(BATCH_VARIANTS[0]) == N_TRACE_VARIANTS,
+              "one batch kernel per traversal variant");
 
-const BatchVariant &batch_variant_for(int variant)
+const BatchVariant &batch_variant_for(int variant) { return BATCH_VARIANTS[variant]; }
+
+// Records per queue grab of the traversal (prh_tracer_set_grain).
+int g_trace_grain = 512;
+
+// Traversal workgroups for `total` records: the resident ones, capped at one
+// grain per wavefront.
+unsigned trace_blocks(int64_t resident_waves, int64_t total)
 {
-    const TraceVariant &V = TRACE_VARIANTS[variant];
-    for (const BatchVariant &B : BATCH_VARIANTS)
-        if (B.refill_min == V.refill_min && B.occ == V.occ) return B;
-    return BATCH_VARIANTS[0];
+    const int64_t wpb = STREAM_THREADS / 64;
+    const int64_t need = (total + (int64_t)g_trace_grain * wpb - 1) / ((int64_t)g_trace_grain * wpb);
+    const int64_t have = resident_waves / wpb;
+    return (unsigned)std::max<int64_t>(1, std::min(need, have));
 }
 
 unsigned gen_grid(uint32_t total)
@@ -964,7 +991,7 @@ int launch_gen_records(const Geometry &g, uint32_t p0, uint32_t n_p, uint32_t R,
 }
 
 int launch_trace(const Geometry &g, const DdaRec *d_dda, const WallRec *d_walls, uint32_t total, uint2 *d_fin,
-                 unsigned long long *d_reads, hipStream_t st)
+                 unsigned long long *d_reads, unsigned long long *d_queue, hipStream_t st)
 {
     if (total == 0) return PRH_OK;
     const TraceVariant &V = TRACE_VARIANTS[g_trace_variant];
@@ -972,12 +999,9 @@ int launch_trace(const Geometry &g, const DdaRec *d_dda, const WallRec *d_walls,
     int64_t waves = 0;
     int rc = resident_waves((const void *)V.fn, STREAM_THREADS, lds, &waves);
     if (rc) return rc;
-    // Enough work per wave to amortise the tail, enough waves to fill the chip.
-    int64_t per_wave = (total + waves - 1) / waves;
-    if (per_wave < 64) per_wave = 64;
-    const int64_t n_waves = (total + per_wave - 1) / per_wave;
-    const unsigned grid = (unsigned)((n_waves * 64 + STREAM_THREADS - 1) / STREAM_THREADS);
-    TraceArgs a{view_of(g), total, (uint32_t)per_wave, d_dda, d_walls, d_fin, d_reads};
+    // the resident workgroups, or fewer when there is not a grain per wavefront
+    const unsigned grid = trace_blocks(waves, total);
+    TraceArgs a{view_of(g), total, (uint32_t)g_trace_grain, d_dda, d_walls, d_fin, d_reads, d_queue};
     V.fn<<<grid, STREAM_THREADS, lds, st>>>(a);
     PRH_HIP(hipGetLastError());
     return PRH_OK;
@@ -1011,6 +1035,21 @@ int prh_tracer_set_variant(int32_t variant)
     PRH_CHECK_ARG(variant >= 0 && variant < N_TRACE_VARIANTS, "prh_tracer_set_variant: variant %d out of range [0, %d)",
                   variant, N_TRACE_VARIANTS);
     g_trace_variant = variant;
+    return PRH_OK;
+}
+
+int prh_tracer_set_grain(int32_t records)
+{
+    PRH_CHECK_ARG(records >= 64 && records <= (1 << 20), "prh_tracer_set_grain: %d records out of range [64, 2^20]",
+                  records);
+    g_trace_grain = records;
+    return PRH_OK;
+}
+
+int prh_tracer_grain(int32_t *records)
+{
+    PRH_CHECK_ARG(records, "prh_tracer_grain: null pointer");
+    *records = g_trace_grain;
     return PRH_OK;
 }
 
@@ -1074,10 +1113,10 @@ int prh_trace_rays(const prh_geometry *geo, const double *origins, const double 
     if ((rc = grow(W.walls, n_rays * sizeof(WallRec)))) return rc;
     if ((rc = grow(W.fin, n_rays * sizeof(uint2)))) return rc;
     if ((rc = grow(W.receiver, n_rays * sizeof(int32_t)))) return rc;
-    if ((rc = grow(W.counters, 2 * sizeof(unsigned long long)))) return rc;
+    if ((rc = grow(W.counters, 3 * sizeof(unsigned long long)))) return rc;
     PRH_HIP(hipMemcpyAsync(W.origins.p, origins, n_rays * 3 * sizeof(double), hipMemcpyHostToDevice, st));
     PRH_HIP(hipMemcpyAsync(W.directions.p, directions, n_rays * 3 * sizeof(double), hipMemcpyHostToDevice, st));
-    PRH_HIP(hipMemsetAsync(W.counters.p, 0, 2 * sizeof(unsigned long long), st));
+    PRH_HIP(hipMemsetAsync(W.counters.p, 0, 3 * sizeof(unsigned long long), st));
     const uint32_t total = (uint32_t)n_rays;
     const unsigned grid = (unsigned)((n_rays + TRACE_THREADS - 1) / TRACE_THREADS);
     k_setup_rays<<<grid, TRACE_THREADS, 0, st>>>(view_of(g), traversal_const(g), W.origins.as<double>(),
@@ -1085,7 +1124,7 @@ int prh_trace_rays(const prh_geometry *geo, const double *origins, const double 
                                                  W.walls.as<WallRec>());
     PRH_HIP(hipGetLastError());
     unsigned long long *ctr = W.counters.as<unsigned long long>();
-    if ((rc = launch_trace(g, W.dda.as<DdaRec>(), W.walls.as<WallRec>(), total, W.fin.as<uint2>(), ctr, st)))
+    if ((rc = launch_trace(g, W.dda.as<DdaRec>(), W.walls.as<WallRec>(), total, W.fin.as<uint2>(), ctr, ctr + 2, st)))
         return rc;
     k_resolve<<<grid, TRACE_THREADS, 0, st>>>(W.fin.as<uint2>(), view_of(g), total, W.receiver.as<int32_t>(), ctr + 1);
     PRH_HIP(hipGetLastError());
@@ -1144,23 +1183,24 @@ int prh_trace_counts(const prh_geometry *geo, int64_t p0, int64_t p1, int32_t ra
     if ((rc = grow(W.uniq_cnt, t1 * sizeof(int32_t)))) return rc;
     if ((rc = grow(W.n_unique, n1 * sizeof(int32_t)))) return rc;
     if ((rc = grow(W.offsets, n1 * sizeof(int64_t)))) return rc;
-    if ((rc = grow(W.counters, 2 * sizeof(unsigned long long)))) return rc;
+    if ((rc = grow(W.counters, 3 * sizeof(unsigned long long)))) return rc;
     PRH_HIP(hipMemcpyAsync(W.seeds.p, state_inc, sizeof(uint64_t) * 4 * n_seeds, hipMemcpyHostToDevice, st));
-    PRH_HIP(hipMemsetAsync(W.counters.p, 0, 2 * sizeof(unsigned long long), st));
+    // counters: [0] occupancy reads, [1] failures, [2] traversal queue
+    PRH_HIP(hipMemsetAsync(W.counters.p, 0, 3 * sizeof(unsigned long long), st));
 
     int64_t nnz = 0;
     int32_t *d_origin = nullptr, *d_receiver = nullptr, *d_count = nullptr;
     unsigned long long counters[2] = {0, 0};
     if (total > 0) {
         PRH_HIP(hipEventRecord(W.ev[0], st));
+        unsigned long long *ctr = W.counters.as<unsigned long long>();
         if ((rc = launch_gen_records(g, (uint32_t)p0, (uint32_t)n_p, (uint32_t)rays_per_patch,
                                      W.seeds.as<uint64_t>(), (uint32_t)total, W.dda.as<DdaRec>(),
                                      W.walls.as<WallRec>(), st)))
             return rc;
         PRH_HIP(hipEventRecord(W.ev[1], st));
-        unsigned long long *ctr = W.counters.as<unsigned long long>();
         if ((rc = launch_trace(g, W.dda.as<DdaRec>(), W.walls.as<WallRec>(), (uint32_t)total, W.fin.as<uint2>(), ctr,
-                               st)))
+                               ctr + 2, st)))
             return rc;
         PRH_HIP(hipEventRecord(W.ev[2], st));
         const unsigned cgrid = (unsigned)(n_p < 65536 ? n_p : 65536);
@@ -1253,6 +1293,7 @@ struct prh_trace_batch {
     int32_t *uniq_recv = nullptr, *uniq_cnt = nullptr, *n_unique = nullptr;
     int64_t *offsets = nullptr, *scan_scratch = nullptr;
     unsigned long long *counters = nullptr;  // reads of each segment, then failures
+    unsigned long long *queues = nullptr;    // traversal queue of each piece
     int64_t *marks = nullptr;                // running edge total, then each piece's first edge
     int32_t *e_origin = nullptr, *e_receiver = nullptr, *e_count = nullptr;
     std::vector<int64_t> seg_nnz, piece_start;
@@ -1277,7 +1318,7 @@ void free_trace_batch(prh_trace_batch *b)
     for (hipEvent_t e : b->ev)
         if (e) (void)hipEventDestroy(e);
     void *ptrs[] = {b->segs_dev, b->seeds,        b->dda,      b->walls,    b->fin,      b->uniq_recv, b->uniq_cnt,
-                    b->n_unique, b->offsets,      b->scan_scratch, b->counters, b->marks, b->e_origin,
+                    b->n_unique, b->offsets,      b->scan_scratch, b->counters, b->queues, b->marks, b->e_origin,
                     b->e_receiver, b->e_count};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -1314,6 +1355,7 @@ int enqueue_batch(prh_trace_batch *b, const BatchVariant &V, size_t lds)
     hipStream_t st = b->stream;
     const int n_pieces = (int)b->pieces.size();
     k_zero_u64<<<(b->n_seg + 256) / 256, 256, 0, st>>>(b->counters, b->n_seg + 1);
+    k_zero_u64<<<(n_pieces + 255) / 256 + 1, 256, 0, st>>>(b->queues, n_pieces);
     k_zero_u64<<<(n_pieces + 256) / 256, 256, 0, st>>>(reinterpret_cast<unsigned long long *>(b->marks),
                                                        n_pieces + 1);
     const int n_chunks = (int)b->chunk_waves.size();
@@ -1451,6 +1493,7 @@ int prh_trace_batch_create(int32_t n_segments, const prh_geometry *const *geomet
     alloc((void **)&b->offsets, (int64_t)sizeof(int64_t) * std::max<int64_t>(b->patch_cap, 1));
     alloc((void **)&b->scan_scratch, (int64_t)sizeof(int64_t) * std::max<int64_t>(b->scratch_elems, 1));
     alloc((void **)&b->counters, (int64_t)sizeof(unsigned long long) * (n_segments + 1));
+    alloc((void **)&b->queues, (int64_t)sizeof(unsigned long long) * std::max(n_pieces, 1));
     alloc((void **)&b->marks, (int64_t)sizeof(int64_t) * (n_pieces + 1));
     alloc((void **)&b->e_origin, (int64_t)sizeof(int32_t) * b->edge_cap);
     alloc((void **)&b->e_receiver, (int64_t)sizeof(int32_t) * b->edge_cap);
@@ -1459,8 +1502,8 @@ int prh_trace_batch_create(int32_t n_segments, const prh_geometry *const *geomet
         set_error("prh_trace_batch_create: out of device memory (%lld bytes)", (long long)b->bytes);
         return fail_batch(b, PRH_ERR_HIP);
     }
-    // launch shapes: generation wavefronts per piece, and the resident
-    // traversal wavefronts of each chunk shared out in proportion to rays
+    // launch shapes: generation wavefronts per piece, and up to the resident
+    // traversal workgroups per piece (each piece's own queue balances them)
     const BatchVariant &V = batch_variant_for(g_trace_variant);
     uint32_t coarse_words = 0;
     if (V.occ == OCC_BRICK)
@@ -1468,15 +1511,9 @@ int prh_trace_batch_create(int32_t n_segments, const prh_geometry *const *geomet
     const size_t lds = (size_t)coarse_words * sizeof(uint32_t);
     int64_t waves = 0;
     if ((rc = resident_waves((const void *)V.fn, STREAM_THREADS, lds, &waves))) return fail_batch(b, rc);
-    const int64_t wpb = STREAM_THREADS / 64;
     std::vector<Segment> segs(n_pieces);
     for (int c = 0; c < n_chunks; ++c) {
         const int j0 = b->chunk_first[c], j1 = b->chunk_first[c + 1];
-        int64_t chunk_rays = 0;
-        for (int j = j0; j < j1; ++j) {
-            const BatchPiece &P = b->pieces[j];
-            chunk_rays += (P.b - P.a) * b->R[P.seg] * b->S[P.seg];
-        }
         int64_t wave_at = 0, block_at = 0;
         for (int j = j0; j < j1; ++j) {
             const BatchPiece &P = b->pieces[j];
@@ -1488,14 +1525,11 @@ int prh_trace_batch_create(int32_t n_segments, const prh_geometry *const *geomet
                              (uint32_t)P.ray_off, nullptr, nullptr, b->dda, b->walls};
             sg.gen_wave_begin = (uint32_t)wave_at;
             sg.reads = b->counters + P.seg;
+            sg.queue = b->queues + j;
+            sg.grain = (uint32_t)g_trace_grain;
             wave_at += (rays + GEN_UNIT - 1) / GEN_UNIT;
-            const double share = (double)rays / (double)chunk_rays;
-            int64_t blocks = std::max<int64_t>(1, std::llround(share * (double)waves / (double)wpb));
-            int64_t per_wave = std::max<int64_t>(64, (rays + blocks * wpb - 1) / (blocks * wpb));
-            blocks = std::max<int64_t>(1, (rays + per_wave * wpb - 1) / (per_wave * wpb));
-            sg.per_wave = (uint32_t)per_wave;
             sg.trace_block_begin = (uint32_t)block_at;
-            block_at += blocks;
+            block_at += trace_blocks(waves, rays);
         }
         b->chunk_waves.push_back(wave_at);
         b->chunk_blocks.push_back(block_at);

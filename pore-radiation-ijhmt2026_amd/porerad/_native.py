@@ -44,6 +44,8 @@ SIGNATURES = {
     "prh_pcg64_seed": ([_P, _I32, _P], ctypes.c_int),
     "prh_tracer_set_variant": ([_I32], ctypes.c_int),
     "prh_tracer_variants": ([_P, _P, _P, _P], ctypes.c_int),
+    "prh_tracer_set_grain": ([_I32], ctypes.c_int),
+    "prh_tracer_grain": ([_P], ctypes.c_int),
     "prh_geometry_create": ([_P, _I32, _I32, _I32, _F64, _P], ctypes.c_int),
     "prh_geometry_sizes": ([_P, _P, _P], ctypes.c_int),
     "prh_geometry_patch_table": ([_P, _P, _P, _P, _P], ctypes.c_int),
@@ -196,6 +198,16 @@ def tracer_variants() -> list[dict]:
 
 def set_tracer_variant(variant: int) -> None:
     check(lib().prh_tracer_set_variant(int(variant)), "prh_tracer_set_variant")
+
+
+def tracer_grain() -> int:
+    g = ctypes.c_int32()
+    check(lib().prh_tracer_grain(ctypes.byref(g)), "prh_tracer_grain")
+    return int(g.value)
+
+
+def set_tracer_grain(records: int) -> None:
+    check(lib().prh_tracer_set_grain(int(records)), "prh_tracer_set_grain")
 
 
 class Geometry:

@@ -8,6 +8,9 @@ count per patch.  ``tune_tracer`` times every variant on a sample of the
 patches, keeps the fastest and records the choice in a small JSON cache
 keyed by the workload signature, so later runs start from it.
 
+``tune_batch`` does the same for a multi-geometry trace batch and also
+picks its chunk size (ray records per chunk of the captured graph).
+
 Results never depend on the variant: every variant produces bit-identical
 counts (checked by tests/test_gpu_tracer.py).
 """
@@ -80,6 +83,67 @@ def tune_tracer(patches, rays_per_patch: int, seeds, *, sample_patches: int = 40
     winner = min(table, key=lambda row: row["trace_ms"])
     _native.set_tracer_variant(winner["variant"])
     result = {"variant": winner["variant"], "table": table, "sample": [p0, p1]}
+    if use_cache:
+        cache[key] = result
+        _store(cache)
+    return result
+
+
+def batch_signature(segments, n_seeds) -> str:
+    parts = [f"{'x'.join(str(s) for s in g.shape)}:{p1 - p0}:{int(r) * int(s)}"
+             for (g, p0, p1, r, _), s in zip(segments, n_seeds)]
+    return "batch/" + "/".join(parts)
+
+
+def tune_batch(segments, seeds, *, sample_fraction: float = 0.125, capacities=(1 << 26, 1 << 27, 1 << 28),
+               use_cache: bool = True) -> dict:
+    """Pick the traversal variant and the chunk size of a multi-geometry trace batch.
+
+    ``segments`` are ``TraceBatch`` segments ``(geometry, p0, p1, rays_per_patch,
+    n_seeds)`` and ``seeds`` their seed lists.  The variant is timed on the
+    middle ``sample_fraction`` of every segment's patch range (one graph per
+    variant), then the chunk size (ray records per chunk) on the whole batch
+    with that variant.  Times are the device time of one graph replay after
+    a warm replay.  Leaves the chosen variant set; returns the tables.
+    """
+
+    key = batch_signature(segments, [len(s) for s in seeds])
+    cache = _load() if use_cache else {}
+    if key in cache:
+        _native.set_tracer_variant(cache[key]["variant"])
+        return cache[key]
+
+    def timed(segs, capacity=0):
+        batch = _native.TraceBatch(segs, ray_capacity=capacity)
+        try:
+            batch.run(seeds)
+            batch.run(seeds)
+            return batch.elapsed_ms, batch.info
+        finally:
+            batch.close()
+
+    sample = []
+    for g, p0, p1, r, s in segments:
+        span = p1 - p0
+        m = max(1, int(span * sample_fraction)) if span else 0
+        a = p0 + (span - m) // 2
+        sample.append((g, a, a + m, r, s))
+    table = []
+    for v in _native.tracer_variants():
+        _native.set_tracer_variant(v["variant"])
+        ms, _ = timed(sample)
+        table.append({"variant": v["variant"], "refill_min": v["refill_min"], "occupancy": v["occupancy"],
+                      "graph_ms": ms})
+    winner = min(table, key=lambda row: row["graph_ms"])
+    _native.set_tracer_variant(winner["variant"])
+    cap_table = []
+    for cap in capacities:
+        ms, info = timed(segments, cap)
+        cap_table.append({"ray_capacity": int(cap), "chunks": info["chunks"], "graph_ms": ms,
+                          "device_bytes": info["device_bytes"]})
+    best_cap = min(cap_table, key=lambda row: row["graph_ms"])
+    result = {"variant": winner["variant"], "ray_capacity": best_cap["ray_capacity"], "table": table,
+              "capacity_table": cap_table, "sample_fraction": sample_fraction}
     if use_cache:
         cache[key] = result
         _store(cache)

@@ -24,26 +24,54 @@ def _centres(n: int) -> np.ndarray:
 
 
 def random_fibers(N: int, solid_fraction: float, radius: float, seed: int, max_fibers: int = 100000) -> np.ndarray:
-    """Straight cylinders (length N, minimum-image periodic) until the solid fraction is reached."""
+    """Straight cylinders (length N, minimum-image periodic) until the solid fraction is reached.
+
+    A voxel centre x belongs to a fibre (centre c, unit axis a) when the
+    minimum-image offset v = x - c lies within ``radius`` of the segment
+    t a, |t| <= N/2.  Only the voxels of the fibre's tube are visited: in each
+    layer across the axis's dominant component d the tube's section is an
+    ellipse of semi-axes radius and radius / |a_d| around the axis, so a window
+    of that half-width (plus one voxel) in the two other coordinates holds
+    every voxel the fibre can reach.
+    """
 
     rng = np.random.default_rng(seed)
     c = _centres(N)
-    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
-    P = np.stack((X, Y, Z), axis=-1).reshape(-1, 3)
-    solid = np.zeros(P.shape[0], dtype=bool)
+    solid = np.zeros((N, N, N), dtype=bool)
     half = N / 2.0
+    r2 = radius * radius
+    layers = np.arange(N)
+    target = solid_fraction * solid.size
+    filled = 0
     for _ in range(max_fibers):
-        if solid.mean() >= solid_fraction:
+        if filled >= target:
             break
         centre = rng.uniform(0.0, N, size=3)
         axis = rng.normal(size=3)
         axis /= np.linalg.norm(axis)
-        v = P - centre
-        v -= N * np.round(v / N)
-        t = np.clip(v @ axis, -half, half)
-        dist2 = np.sum((v - t[:, None] * axis[None, :]) ** 2, axis=1)
-        solid |= dist2 <= radius * radius
-    return solid.reshape(N, N, N).astype(np.uint8)
+        d = int(np.argmax(np.abs(axis)))
+        e1, e2 = (d + 1) % 3, (d + 2) % 3
+        w = int(np.ceil(radius / abs(axis[d]))) + 1
+        offs = np.arange(-w, w + 1)
+        vd = c - centre[d]
+        vd -= N * np.round(vd / N)
+        idx = [None, None, None]
+        idx[d] = np.broadcast_to(layers[:, None, None], (N, offs.size, offs.size))
+        for e, shape in ((e1, (N, offs.size, 1)), (e2, (N, 1, offs.size))):
+            mid = centre[e] + vd * (axis[e] / axis[d]) - 0.5  # axis position as a fractional index
+            i = (np.floor(mid)[:, None] + offs[None, :]).astype(np.int64) % N
+            idx[e] = np.broadcast_to(i.reshape(shape), (N, offs.size, offs.size))
+        v = [c[idx[k]] - centre[k] for k in range(3)]
+        for k in range(3):
+            v[k] -= N * np.round(v[k] / N)
+        t = np.clip(v[0] * axis[0] + v[1] * axis[1] + v[2] * axis[2], -half, half)
+        dist2 = (v[0] - t * axis[0]) ** 2 + (v[1] - t * axis[1]) ** 2 + (v[2] - t * axis[2]) ** 2
+        hit = dist2 <= r2
+        flat = np.unique(np.ravel_multi_index((idx[0][hit], idx[1][hit], idx[2][hit]), solid.shape))
+        flat = flat[~solid.ravel()[flat]]
+        solid.ravel()[flat] = True
+        filled += flat.size
+    return solid.astype(np.uint8)
 
 
 def _segment_distance_field(N: int, segments: np.ndarray, reach: float) -> np.ndarray:

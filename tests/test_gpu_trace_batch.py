@@ -175,3 +175,24 @@ def test_all_archived_hash_cases_in_one_batch(env):
             assert vf.count_sha256(o, r, c, np.full(n, rays, np.int64)) == sha, k
     finally:
         batch.close()
+
+
+def test_every_traversal_variant_in_batch_mode(env, media):
+    native, vf = env
+    segs = _segments(media)[:3]
+    ref = [_single(vf, P, p0, p1, R, s)[0] for P, p0, p1, R, s in segs]
+    variants = native.tracer_variants()
+    default = next(v["variant"] for v in variants if v["current"])
+    try:
+        for v in variants:
+            native.set_tracer_variant(v["variant"])
+            batch = native.TraceBatch([(P.geometry(), p0, p1, R, len(s)) for P, p0, p1, R, s in segs],
+                                      ray_capacity=1 << 15)
+            try:
+                batch.run([s for *_, s in segs])
+                for k in range(len(segs)):
+                    assert all(np.array_equal(x, y) for x, y in zip(batch.edges(k), ref[k])), (v, k)
+            finally:
+                batch.close()
+    finally:
+        native.set_tracer_variant(default)

@@ -86,3 +86,28 @@ def test_native_errors_map_to_reference_exceptions():
 
     with pytest.raises(ValueError):
         _native.seed_words(-1)
+
+
+def _fibres_brute_force(N, solid_fraction, radius, seed):
+    """Every voxel centre against every fibre (the definition of random_fibers)."""
+
+    rng = np.random.default_rng(seed)
+    c = np.arange(N) + 0.5
+    P = np.stack(np.meshgrid(c, c, c, indexing="ij"), axis=-1).reshape(-1, 3)
+    solid = np.zeros(len(P), bool)
+    while solid.mean() < solid_fraction:
+        centre = rng.uniform(0.0, N, size=3)
+        axis = rng.normal(size=3)
+        axis /= np.linalg.norm(axis)
+        v = P - centre
+        v -= N * np.round(v / N)
+        t = np.clip(v @ axis, -N / 2.0, N / 2.0)
+        solid |= np.sum((v - t[:, None] * axis) ** 2, axis=1) <= radius * radius
+    return solid.reshape(N, N, N).astype(np.uint8)
+
+
+@pytest.mark.parametrize("N,radius,seed", [(24, 3.0, 0), (40, 2.5, 4), (64, 3.0, 0), (33, 5.0, 9)])
+def test_random_fibres_tube_windows_equal_brute_force(N, radius, seed):
+    from porerad import synthetic
+
+    assert np.array_equal(synthetic.random_fibers(N, 0.10, radius, seed), _fibres_brute_force(N, 0.10, radius, seed))
