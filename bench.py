@@ -226,12 +226,13 @@ def run_batching(args) -> dict:
     log(f"batch geometries ({time.perf_counter() - t:.1f} s): " +
         ", ".join(f"{name} {P.n_patches}" for name, P in geos) + f"; {n_rays} rays per pass")
     segments = [(P.geometry(), 0, P.n_patches, R, 1) for _, P in geos]
-    tune, capacity = None, 0
+    tune, capacity, n_streams = None, 0, 0
     if args.tune:
         tune = autotune.tune_batch(segments, seeds, use_cache=False)
-        capacity = tune["ray_capacity"]
+        capacity, n_streams = tune["ray_capacity"], tune["streams"]
         log("batch tuning: " + ", ".join(f"v{r['variant']}={r['graph_ms']:.1f}ms" for r in tune["table"]) +
-            "; chunks " + ", ".join(f"{r['ray_capacity']}={r['graph_ms']:.1f}ms" for r in tune["capacity_table"]))
+            "; chunks x streams " + ", ".join(f"{r['ray_capacity']}x{r['streams']}={r['graph_ms']:.1f}ms"
+                                             for r in tune["capacity_table"]))
 
     # plain path: one prh_trace_counts call per geometry
     seq_nnz = []
@@ -243,9 +244,10 @@ def run_batching(args) -> dict:
         seq_nnz.append(e.nnz)
         e.close()
     sequential_s = time.perf_counter() - t
+    _native.trim_tracer()  # the per-call workspace of the largest geometry
 
     t = time.perf_counter()
-    batch = _native.TraceBatch(segments, ray_capacity=capacity)
+    batch = _native.TraceBatch(segments, ray_capacity=capacity, streams=n_streams)
     create_s = time.perf_counter() - t
     log(f"trace batch: {batch.info} (created and captured in {create_s:.2f} s)")
     for _ in range(args.warmup):

@@ -89,6 +89,10 @@ int prh_tracer_variants(int32_t *n_variants, int32_t *current, int32_t *refill_m
  * launch's shared work queue at a time (64 .. 2^20, default 2048).  No
  * counterpart in the reference; results do not depend on it. */
 int prh_tracer_set_grain(int32_t records);
+/* Free the current device's tracer workspace (ray records of the largest
+ * prh_trace_counts / prh_trace_rays call so far; it grows on demand and is
+ * otherwise kept).  No counterpart in the reference. */
+int prh_tracer_trim(void);
 int prh_tracer_grain(int32_t *records);
 
 /* numpy.random.PCG64(seed) seeding: SeedSequence over the little-endian
@@ -137,29 +141,34 @@ int prh_trace_counts(const prh_geometry *g, int64_t p0, int64_t p1, int32_t rays
  * rays_per_patch[k] rays from each of n_seeds[k] seeded blocks, pooled as
  * prh_trace_counts pools them (the same edges, bit for bit).  The segments
  * are cut at patch boundaries into chunks of at most ray_capacity ray records
- * (0: min(2^27, all rays); 112 bytes of HBM per record); each chunk is one
- * ray-generation launch and one persistent traversal launch over all its
+ * (0: min(2^27, all rays); 112 bytes of HBM per record and stream); each
+ * chunk is one ray-generation launch and one traversal launch over all its
  * pieces, followed by per-piece counting and compaction into one device edge
  * arena of edge_capacity entries (0: the worst case, rays per patch bounded by
- * the patch count).  The whole pass is captured once into a HIP graph that
- * prh_trace_batch_run replays.  The geometries must outlive the batch.
- * Replaces the per-case loop of first_hit_counts + pool_counts calls,
- * porerad/view_factors.py:273-300, as scripts/run_prescribed_field.py:81-85
- * drives it per case. */
+ * the patch count).  Chunks take turns over n_streams streams (0: 2; at most
+ * 4), each with its own record buffers, so consecutive chunks overlap.  The
+ * pass is captured once into a HIP graph that prh_trace_batch_run replays
+ * (flags PRH_BATCH_DIRECT: launched directly on each run instead).  The
+ * geometries must outlive the batch.  Replaces the per-case loop of
+ * first_hit_counts + pool_counts calls, porerad/view_factors.py:273-300, as
+ * scripts/run_prescribed_field.py:81-85 drives it per case. */
+#define PRH_BATCH_DIRECT 1
 int prh_trace_batch_create(int32_t n_segments, const prh_geometry *const *geometries, const int64_t *patch_begin,
                            const int64_t *patch_end, const int32_t *rays_per_patch, const int32_t *n_seeds,
-                           int64_t ray_capacity, int64_t edge_capacity, prh_trace_batch **out);
+                           int64_t ray_capacity, int64_t edge_capacity, int32_t n_streams, int32_t flags,
+                           prh_trace_batch **out);
 /* Replay the pass with seeds state_inc (each segment's n_seeds x 4 words, in
  * segment order).  nnz[k] / occupancy_reads[k] (either may be NULL) receive
  * segment k's edge count and occupancy reads; *elapsed_ms (may be NULL) the
- * device time of the graph.  PRH_ERR_UNRESOLVED if a ray failed,
+ * device time of the pass.  PRH_ERR_UNRESOLVED if a ray failed,
  * PRH_ERR_CAPACITY if the edges overflowed the arena. */
 int prh_trace_batch_run(prh_trace_batch *b, const uint64_t *state_inc, int64_t *nnz, int64_t *occupancy_reads,
                         double *elapsed_ms);
 /* Copy segment k's edges of the last run to host arrays of nnz[k] entries. */
 int prh_trace_batch_edges(const prh_trace_batch *b, int32_t segment, int32_t *origin, int32_t *receiver,
                           int64_t *count);
-/* info[0..n): device bytes, chunks, pieces, ray capacity, edge capacity, rays. */
+/* info[0..n): device bytes, chunks, pieces, ray capacity, edge capacity, rays,
+ * streams, graph (1) or direct (0), traversal variant. */
 int prh_trace_batch_info(const prh_trace_batch *b, int64_t *info, int32_t n);
 void prh_trace_batch_destroy(prh_trace_batch *b);
 

@@ -1038,6 +1038,25 @@ int prh_tracer_set_variant(int32_t variant)
     return PRH_OK;
 }
 
+int prh_tracer_trim(void)
+{
+    int dev = 0;
+    PRH_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return PRH_OK;
+    hipStream_t st;
+    int rc = current_stream(&st);
+    if (rc) return rc;
+    PRH_HIP(hipStreamSynchronize(st));
+    Workspace &W = g_ws[dev];
+    for (Buf *b : {&W.seeds, &W.origins, &W.directions, &W.dda, &W.walls, &W.fin, &W.receiver, &W.uniq_recv,
+                   &W.uniq_cnt, &W.n_unique, &W.offsets, &W.counters}) {
+        if (b->p) PRH_HIP(hipFree(b->p));
+        b->p = nullptr;
+        b->cap = 0;
+    }
+    return PRH_OK;
+}
+
 int prh_tracer_set_grain(int32_t records)
 {
     PRH_CHECK_ARG(records >= 64 && records <= (1 << 20), "prh_tracer_set_grain: %d records out of range [64, 2^20]",
@@ -1262,10 +1281,16 @@ int prh_trace_counts(const prh_geometry *geo, int64_t p0, int64_t p1, int32_t ra
 
 // A multi-geometry batch: segments (geometry, patch range, rays per patch,
 // seeds) cut at patch boundaries into pieces that fill fixed-size chunks of
-// ray records; per chunk one generation launch and one persistent traversal
-// launch over all its pieces, then per piece counting, a scan that carries the
-// running edge total on the device, and compaction into one edge arena.  The
-// whole pass is captured once into a graph and replayed with new seeds.
+// ray records; per chunk one generation launch and one traversal launch over
+// all its pieces, then per piece counting, a scan that carries the running
+// edge total on the device, and compaction into one edge arena.  Chunks take
+// turns over a few streams, each with its own record buffers, so the grazing
+// rays that end one chunk's traversal run beside the next chunk's work; only
+// the scans are chained from chunk to chunk (the arena's running total).  Each
+// chunk is captured once as two graphs (generation + traversal + counting;
+// scans + compaction) that every run replays on the chunk's stream, ordered by
+// events: one graph over all streams would be replayed on one queue here and
+// lose the overlap.
 struct BatchPiece {
     int seg;
     int64_t a, b;      // patch range [a, b) of the segment's geometry
@@ -1273,10 +1298,24 @@ struct BatchPiece {
     int64_t patch_off; // first per-patch slot within the chunk
 };
 
+// Records and per-patch scratch of the chunks of one stream.
+struct BatchBuffers {
+    hipStream_t stream = nullptr;
+    DdaRec *dda = nullptr;
+    WallRec *walls = nullptr;
+    uint2 *fin = nullptr;
+    int32_t *uniq_recv = nullptr, *uniq_cnt = nullptr, *n_unique = nullptr;
+    int64_t *offsets = nullptr, *scan_scratch = nullptr;
+    hipEvent_t join = nullptr;
+};
+
+constexpr int MAX_BATCH_STREAMS = 4;
+
 struct prh_trace_batch {
     int device = 0;
-    hipStream_t stream = nullptr;
     int n_seg = 0;
+    int n_streams = 1;
+    bool use_graph = true;
     std::vector<const prh_geometry *> geos;
     std::vector<int> R, S;
     std::vector<int64_t> seed_off;  // first seed word of each segment
@@ -1285,13 +1324,13 @@ struct prh_trace_batch {
     std::vector<int> chunk_first;  // first piece of each chunk, then the piece count
     std::vector<int64_t> chunk_waves, chunk_blocks;
     int64_t ray_cap = 0, patch_cap = 0, scratch_elems = 0, edge_cap = 0, rays = 0;
+    size_t lds = 0;
+    int variant = 0;
+    BatchBuffers buf[MAX_BATCH_STREAMS];
+    std::vector<hipEvent_t> scanned;  // per chunk: its scans are done (the carry is final)
+    hipEvent_t fork = nullptr;
     Segment *segs_dev = nullptr;
     uint64_t *seeds = nullptr;
-    DdaRec *dda = nullptr;
-    WallRec *walls = nullptr;
-    uint2 *fin = nullptr;
-    int32_t *uniq_recv = nullptr, *uniq_cnt = nullptr, *n_unique = nullptr;
-    int64_t *offsets = nullptr, *scan_scratch = nullptr;
     unsigned long long *counters = nullptr;  // reads of each segment, then failures
     unsigned long long *queues = nullptr;    // traversal queue of each piece
     int64_t *marks = nullptr;                // running edge total, then each piece's first edge
@@ -1299,8 +1338,9 @@ struct prh_trace_batch {
     std::vector<int64_t> seg_nnz, piece_start;
     int64_t edges = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // captured graphs: [0] clears the counters, then per chunk front and back
+    std::vector<hipGraph_t> graphs;
+    std::vector<hipGraphExec_t> execs;
     int64_t bytes = 0;
 };
 
@@ -1313,27 +1353,36 @@ int64_t align_gen(int64_t rays) { return (rays + GEN_UNIT - 1) / GEN_UNIT * GEN_
 
 void free_trace_batch(prh_trace_batch *b)
 {
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (hipGraphExec_t e : b->execs)
+        if (e) (void)hipGraphExecDestroy(e);
+    for (hipGraph_t g : b->graphs)
+        if (g) (void)hipGraphDestroy(g);
     for (hipEvent_t e : b->ev)
         if (e) (void)hipEventDestroy(e);
-    void *ptrs[] = {b->segs_dev, b->seeds,        b->dda,      b->walls,    b->fin,      b->uniq_recv, b->uniq_cnt,
-                    b->n_unique, b->offsets,      b->scan_scratch, b->counters, b->queues, b->marks, b->e_origin,
-                    b->e_receiver, b->e_count};
+    for (hipEvent_t e : b->scanned)
+        if (e) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    for (BatchBuffers &B : b->buf) {
+        void *ptrs[] = {B.dda, B.walls, B.fin, B.uniq_recv, B.uniq_cnt, B.n_unique, B.offsets, B.scan_scratch};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+        if (B.join) (void)hipEventDestroy(B.join);
+        if (B.stream) (void)hipStreamDestroy(B.stream);
+    }
+    void *ptrs[] = {b->segs_dev, b->seeds, b->counters, b->queues, b->marks, b->e_origin, b->e_receiver, b->e_count};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
-void count_launch(const prh_trace_batch *b, const BatchPiece &P, hipStream_t st)
+void count_launch(const prh_trace_batch *b, const BatchBuffers &B, const BatchPiece &P, hipStream_t st)
 {
     const Geometry &g = b->geos[P.seg]->g;
     const int R = b->R[P.seg], S = b->S[P.seg];
     const int64_t np = P.b - P.a;
     int SRp2 = 1;
     while (SRp2 < R * S) SRp2 <<= 1;
-    const uint2 *fin = b->fin + P.ray_off;
-    int32_t *ur = b->uniq_recv + P.ray_off, *uc = b->uniq_cnt + P.ray_off, *nu = b->n_unique + P.patch_off;
+    const uint2 *fin = B.fin + P.ray_off;
+    int32_t *ur = B.uniq_recv + P.ray_off, *uc = B.uniq_cnt + P.ray_off, *nu = B.n_unique + P.patch_off;
     unsigned long long *failed = b->counters + b->n_seg;
     if (SRp2 <= 1024) {
         const int epl = SRp2 <= 64 ? 1 : SRp2 / 64;
@@ -1349,39 +1398,115 @@ void count_launch(const prh_trace_batch *b, const BatchPiece &P, hipStream_t st)
     }
 }
 
-// The whole pass on the batch's stream (captured into the graph).
-int enqueue_batch(prh_trace_batch *b, const BatchVariant &V, size_t lds)
+// Pieces of a pass, each enqueued on one stream (and captured as one graph).
+int enqueue_clear(prh_trace_batch *b, hipStream_t st)
 {
-    hipStream_t st = b->stream;
     const int n_pieces = (int)b->pieces.size();
     k_zero_u64<<<(b->n_seg + 256) / 256, 256, 0, st>>>(b->counters, b->n_seg + 1);
     k_zero_u64<<<(n_pieces + 255) / 256 + 1, 256, 0, st>>>(b->queues, n_pieces);
     k_zero_u64<<<(n_pieces + 256) / 256, 256, 0, st>>>(reinterpret_cast<unsigned long long *>(b->marks),
                                                        n_pieces + 1);
+    PRH_HIP(hipGetLastError());
+    return PRH_OK;
+}
+
+// Chunk c, first half: generation, traversal, per-piece counting.
+int enqueue_front(prh_trace_batch *b, int c, hipStream_t st)
+{
+    const BatchVariant &V = batch_variant_for(b->variant);
+    const BatchBuffers &B = b->buf[c % b->n_streams];
+    const int j0 = b->chunk_first[c], j1 = b->chunk_first[c + 1];
+    const unsigned gen_blocks = (unsigned)((b->chunk_waves[c] * 64 + TRACE_THREADS - 1) / TRACE_THREADS);
+    k_gen_rays_batch<<<gen_blocks, TRACE_THREADS, 0, st>>>(b->segs_dev + j0, j1 - j0);
+    V.fn<<<(unsigned)b->chunk_blocks[c], STREAM_THREADS, b->lds, st>>>(b->segs_dev + j0, j1 - j0, B.dda, B.walls,
+                                                                        B.fin);
+    PRH_HIP(hipGetLastError());
+    for (int j = j0; j < j1; ++j) count_launch(b, B, b->pieces[j], st);
+    PRH_HIP(hipGetLastError());
+    return PRH_OK;
+}
+
+// Chunk c, second half: the scans (continuing the arena's running total) and
+// the compaction of each piece.
+int enqueue_back(prh_trace_batch *b, int c, hipStream_t st)
+{
+    const BatchBuffers &B = b->buf[c % b->n_streams];
+    for (int j = b->chunk_first[c]; j < b->chunk_first[c + 1]; ++j) {
+        const BatchPiece &P = b->pieces[j];
+        const int64_t np = P.b - P.a;
+        int rc = exclusive_scan_i32_to_i64_carry_async(B.n_unique + P.patch_off, B.offsets + P.patch_off, np,
+                                                       B.scan_scratch, b->marks, b->marks + 1 + j, st);
+        if (rc) return rc;
+        const unsigned cgrid = (unsigned)(np < 65536 ? np : 65536);
+        k_compact_edges<<<cgrid, COUNT_THREADS, 0, st>>>(
+            B.uniq_recv + P.ray_off, B.uniq_cnt + P.ray_off, B.n_unique + P.patch_off, B.offsets + P.patch_off, np,
+            b->R[P.seg] * b->S[P.seg], P.a, b->e_origin, b->e_receiver, b->e_count, b->edge_cap);
+        PRH_HIP(hipGetLastError());
+    }
+    return PRH_OK;
+}
+
+// Piece `part` of the pass (0: clear; 1 + 2c: front of chunk c; 2 + 2c: back),
+// replayed from its graph or launched directly.
+int run_part(prh_trace_batch *b, int part, hipStream_t st)
+{
+    if (b->use_graph) {
+        PRH_HIP(hipGraphLaunch(b->execs[part], st));
+        return PRH_OK;
+    }
+    if (part == 0) return enqueue_clear(b, st);
+    return (part & 1) ? enqueue_front(b, (part - 1) / 2, st) : enqueue_back(b, (part - 2) / 2, st);
+}
+
+// The pass: stream 0 clears the counters and forks the others; chunk c runs
+// on stream c mod n_streams in that stream's buffers (so after the chunk that
+// last used them); its scans wait for the previous chunk's.
+int enqueue_pass(prh_trace_batch *b)
+{
+    hipStream_t s0 = b->buf[0].stream;
+    int rc = run_part(b, 0, s0);
+    if (rc) return rc;
+    PRH_HIP(hipEventRecord(b->fork, s0));
+    for (int i = 1; i < b->n_streams; ++i) PRH_HIP(hipStreamWaitEvent(b->buf[i].stream, b->fork, 0));
     const int n_chunks = (int)b->chunk_waves.size();
     for (int c = 0; c < n_chunks; ++c) {
-        const int j0 = b->chunk_first[c], j1 = b->chunk_first[c + 1];
-        const unsigned gen_blocks = (unsigned)((b->chunk_waves[c] * 64 + TRACE_THREADS - 1) / TRACE_THREADS);
-        k_gen_rays_batch<<<gen_blocks, TRACE_THREADS, 0, st>>>(b->segs_dev + j0, j1 - j0);
-        V.fn<<<(unsigned)b->chunk_blocks[c], STREAM_THREADS, lds, st>>>(b->segs_dev + j0, j1 - j0, b->dda,
-                                                                         b->walls, b->fin);
-        PRH_HIP(hipGetLastError());
-        for (int j = j0; j < j1; ++j) {
-            const BatchPiece &P = b->pieces[j];
-            const int64_t np = P.b - P.a;
-            count_launch(b, P, st);
-            PRH_HIP(hipGetLastError());
-            int rc = exclusive_scan_i32_to_i64_carry_async(b->n_unique + P.patch_off, b->offsets + P.patch_off, np,
-                                                           b->scan_scratch, b->marks, b->marks + 1 + j, st);
-            if (rc) return rc;
-            const unsigned cgrid = (unsigned)(np < 65536 ? np : 65536);
-            k_compact_edges<<<cgrid, COUNT_THREADS, 0, st>>>(
-                b->uniq_recv + P.ray_off, b->uniq_cnt + P.ray_off, b->n_unique + P.patch_off,
-                b->offsets + P.patch_off, np, b->R[P.seg] * b->S[P.seg], P.a, b->e_origin, b->e_receiver,
-                b->e_count, b->edge_cap);
-            PRH_HIP(hipGetLastError());
-        }
+        hipStream_t st = b->buf[c % b->n_streams].stream;
+        if ((rc = run_part(b, 1 + 2 * c, st))) return rc;
+        if (c > 0) PRH_HIP(hipStreamWaitEvent(st, b->scanned[c - 1], 0));
+        if ((rc = run_part(b, 2 + 2 * c, st))) return rc;
+        PRH_HIP(hipEventRecord(b->scanned[c], st));
     }
+    for (int i = 1; i < b->n_streams; ++i) {
+        PRH_HIP(hipEventRecord(b->buf[i].join, b->buf[i].stream));
+        PRH_HIP(hipStreamWaitEvent(s0, b->buf[i].join, 0));
+    }
+    return PRH_OK;
+}
+
+// Capture part `part` of the pass on st into a graph.
+int capture_part(prh_trace_batch *b, int part, hipStream_t st)
+{
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        set_error("prh_trace_batch_create: stream capture did not start");
+        return PRH_ERR_HIP;
+    }
+    const int rc = part == 0 ? enqueue_clear(b, st)
+                   : (part & 1) ? enqueue_front(b, (part - 1) / 2, st)
+                                : enqueue_back(b, (part - 2) / 2, st);
+    hipGraph_t graph = nullptr;
+    const hipError_t end = hipStreamEndCapture(st, &graph);
+    b->graphs.push_back(graph);
+    if (rc) return rc;
+    if (end != hipSuccess || !graph) {
+        set_error("prh_trace_batch_create: graph capture failed: %s", hipGetErrorString(end));
+        return PRH_ERR_HIP;
+    }
+    hipGraphExec_t exec = nullptr;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        set_error("prh_trace_batch_create: graph instantiation failed");
+        return PRH_ERR_HIP;
+    }
+    b->execs.push_back(exec);
     return PRH_OK;
 }
 
@@ -1398,13 +1523,17 @@ extern "C" {
 
 int prh_trace_batch_create(int32_t n_segments, const prh_geometry *const *geometries, const int64_t *patch_begin,
                            const int64_t *patch_end, const int32_t *rays_per_patch, const int32_t *n_seeds,
-                           int64_t ray_capacity, int64_t edge_capacity, prh_trace_batch **out)
+                           int64_t ray_capacity, int64_t edge_capacity, int32_t n_streams, int32_t flags,
+                           prh_trace_batch **out)
 {
     PRH_CHECK_ARG(out && geometries && patch_begin && patch_end && rays_per_patch && n_seeds,
                   "prh_trace_batch_create: null pointer");
     *out = nullptr;
     PRH_CHECK_ARG(n_segments >= 1, "prh_trace_batch_create: no segments");
     PRH_CHECK_ARG(ray_capacity >= 0 && edge_capacity >= 0, "prh_trace_batch_create: negative capacity");
+    PRH_CHECK_ARG(n_streams >= 0 && n_streams <= MAX_BATCH_STREAMS, "prh_trace_batch_create: 0 .. %d streams",
+                  MAX_BATCH_STREAMS);
+    PRH_CHECK_ARG((flags & ~PRH_BATCH_DIRECT) == 0, "prh_trace_batch_create: unknown flags 0x%x", flags);
     const int device = geometries[0] ? geometries[0]->g.device : 0;
     int64_t total = 0, widest = 0, worst_edges = 0;
     for (int k = 0; k < n_segments; ++k) {
@@ -1439,6 +1568,8 @@ int prh_trace_batch_create(int32_t n_segments, const prh_geometry *const *geomet
     b->n_seg = n_segments;
     b->ray_cap = cap;
     b->rays = total;
+    b->use_graph = (flags & PRH_BATCH_DIRECT) == 0;
+    b->variant = g_trace_variant;
     b->edge_cap = edge_capacity > 0 ? edge_capacity : std::max<int64_t>(worst_edges, 1);
     for (int k = 0; k < n_segments; ++k) {
         b->geos.push_back(geometries[k]);
@@ -1474,24 +1605,35 @@ int prh_trace_batch_create(int32_t n_segments, const prh_geometry *const *geomet
     close_chunk();
     const int n_pieces = (int)b->pieces.size();
     const int n_chunks = (int)b->chunk_first.size() - 1;
-    bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreate(&b->ev[0]) == hipSuccess && hipEventCreate(&b->ev[1]) == hipSuccess;
+    // streams (and record buffers): as asked, at most one per chunk, at least one
+    b->n_streams = std::max(1, std::min(n_streams > 0 ? (int)n_streams : 2, std::max(n_chunks, 1)));
+    bool ok = hipEventCreateWithFlags(&b->ev[0], hipEventDefault) == hipSuccess &&
+              hipEventCreateWithFlags(&b->ev[1], hipEventDefault) == hipSuccess &&
+              hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+    b->scanned.assign(n_chunks, nullptr);
+    for (int c = 0; c < n_chunks && ok; ++c)
+        ok = hipEventCreateWithFlags(&b->scanned[c], hipEventDisableTiming) == hipSuccess;
     auto alloc = [&](void **ptr, int64_t bytes) {
         bytes = std::max<int64_t>(bytes, 8);
         if (ok && hipMalloc(ptr, (size_t)bytes) != hipSuccess) ok = false;
         b->bytes += bytes;
     };
     const int64_t rec = n_chunks > 0 ? cap : 1;
+    for (int i = 0; i < b->n_streams && ok; ++i) {
+        BatchBuffers &B = b->buf[i];
+        ok = hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&B.join, hipEventDisableTiming) == hipSuccess;
+        alloc((void **)&B.dda, (int64_t)sizeof(DdaRec) * rec);
+        alloc((void **)&B.walls, (int64_t)sizeof(WallRec) * rec);
+        alloc((void **)&B.fin, (int64_t)sizeof(uint2) * rec);
+        alloc((void **)&B.uniq_recv, (int64_t)sizeof(int32_t) * rec);
+        alloc((void **)&B.uniq_cnt, (int64_t)sizeof(int32_t) * rec);
+        alloc((void **)&B.n_unique, (int64_t)sizeof(int32_t) * std::max<int64_t>(b->patch_cap, 1));
+        alloc((void **)&B.offsets, (int64_t)sizeof(int64_t) * std::max<int64_t>(b->patch_cap, 1));
+        alloc((void **)&B.scan_scratch, (int64_t)sizeof(int64_t) * std::max<int64_t>(b->scratch_elems, 1));
+    }
     alloc((void **)&b->segs_dev, (int64_t)sizeof(Segment) * std::max(n_pieces, 1));
     alloc((void **)&b->seeds, (int64_t)sizeof(uint64_t) * b->seed_words);
-    alloc((void **)&b->dda, (int64_t)sizeof(DdaRec) * rec);
-    alloc((void **)&b->walls, (int64_t)sizeof(WallRec) * rec);
-    alloc((void **)&b->fin, (int64_t)sizeof(uint2) * rec);
-    alloc((void **)&b->uniq_recv, (int64_t)sizeof(int32_t) * rec);
-    alloc((void **)&b->uniq_cnt, (int64_t)sizeof(int32_t) * rec);
-    alloc((void **)&b->n_unique, (int64_t)sizeof(int32_t) * std::max<int64_t>(b->patch_cap, 1));
-    alloc((void **)&b->offsets, (int64_t)sizeof(int64_t) * std::max<int64_t>(b->patch_cap, 1));
-    alloc((void **)&b->scan_scratch, (int64_t)sizeof(int64_t) * std::max<int64_t>(b->scratch_elems, 1));
     alloc((void **)&b->counters, (int64_t)sizeof(unsigned long long) * (n_segments + 1));
     alloc((void **)&b->queues, (int64_t)sizeof(unsigned long long) * std::max(n_pieces, 1));
     alloc((void **)&b->marks, (int64_t)sizeof(int64_t) * (n_pieces + 1));
@@ -1499,20 +1641,21 @@ int prh_trace_batch_create(int32_t n_segments, const prh_geometry *const *geomet
     alloc((void **)&b->e_receiver, (int64_t)sizeof(int32_t) * b->edge_cap);
     alloc((void **)&b->e_count, (int64_t)sizeof(int32_t) * b->edge_cap);
     if (!ok) {
-        set_error("prh_trace_batch_create: out of device memory (%lld bytes)", (long long)b->bytes);
+        set_error("prh_trace_batch_create: out of device memory or streams (%lld bytes)", (long long)b->bytes);
         return fail_batch(b, PRH_ERR_HIP);
     }
     // launch shapes: generation wavefronts per piece, and up to the resident
     // traversal workgroups per piece (each piece's own queue balances them)
-    const BatchVariant &V = batch_variant_for(g_trace_variant);
+    const BatchVariant &V = batch_variant_for(b->variant);
     uint32_t coarse_words = 0;
     if (V.occ == OCC_BRICK)
         for (const prh_geometry *g : b->geos) coarse_words = std::max<uint32_t>(coarse_words, (uint32_t)g->g.coarse_words);
-    const size_t lds = (size_t)coarse_words * sizeof(uint32_t);
+    b->lds = (size_t)coarse_words * sizeof(uint32_t);
     int64_t waves = 0;
-    if ((rc = resident_waves((const void *)V.fn, STREAM_THREADS, lds, &waves))) return fail_batch(b, rc);
+    if ((rc = resident_waves((const void *)V.fn, STREAM_THREADS, b->lds, &waves))) return fail_batch(b, rc);
     std::vector<Segment> segs(n_pieces);
     for (int c = 0; c < n_chunks; ++c) {
+        const BatchBuffers &B = b->buf[c % b->n_streams];
         const int j0 = b->chunk_first[c], j1 = b->chunk_first[c + 1];
         int64_t wave_at = 0, block_at = 0;
         for (int j = j0; j < j1; ++j) {
@@ -1522,7 +1665,7 @@ int prh_trace_batch_create(int32_t n_segments, const prh_geometry *const *geomet
             Segment &sg = segs[j];
             sg.gen = GenArgs{view_of(g), traversal_const(g), (uint32_t)P.a, (uint32_t)(P.b - P.a),
                              (uint32_t)b->R[P.seg], b->seeds + b->seed_off[P.seg], (uint32_t)rays,
-                             (uint32_t)P.ray_off, nullptr, nullptr, b->dda, b->walls};
+                             (uint32_t)P.ray_off, nullptr, nullptr, B.dda, B.walls};
             sg.gen_wave_begin = (uint32_t)wave_at;
             sg.reads = b->counters + P.seg;
             sg.queue = b->queues + j;
@@ -1538,23 +1681,12 @@ int prh_trace_batch_create(int32_t n_segments, const prh_geometry *const *geomet
         PRH_HIP(hipMemcpy(b->segs_dev, segs.data(), sizeof(Segment) * n_pieces, hipMemcpyHostToDevice));
     b->piece_start.assign(n_pieces + 1, 0);
     b->seg_nnz.assign(n_segments, 0);
-    // capture the pass once; each run replays it
-    if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        set_error("prh_trace_batch_create: stream capture did not start");
-        return fail_batch(b, PRH_ERR_HIP);
-    }
-    rc = enqueue_batch(b, V, lds);
-    hipGraph_t graph = nullptr;
-    const hipError_t end = hipStreamEndCapture(b->stream, &graph);
-    b->graph = graph;
-    if (rc) return fail_batch(b, rc);
-    if (end != hipSuccess || !graph) {
-        set_error("prh_trace_batch_create: graph capture failed: %s", hipGetErrorString(end));
-        return fail_batch(b, PRH_ERR_HIP);
-    }
-    if (hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
-        set_error("prh_trace_batch_create: graph instantiation failed");
-        return fail_batch(b, PRH_ERR_HIP);
+    if (b->use_graph) {
+        // capture every part once, each on the stream it will be replayed on
+        for (int part = 0; part < 1 + 2 * n_chunks; ++part) {
+            hipStream_t st = part == 0 ? b->buf[0].stream : b->buf[((part - 1) / 2) % b->n_streams].stream;
+            if ((rc = capture_part(b, part, st))) return fail_batch(b, rc);
+        }
     }
     *out = b;
     return PRH_OK;
@@ -1567,16 +1699,17 @@ int prh_trace_batch_run(prh_trace_batch *b, const uint64_t *state_inc, int64_t *
     PRH_HIP(hipSetDevice(b->device));
     const int n = b->n_seg;
     const int n_pieces = (int)b->pieces.size();
+    hipStream_t s0 = b->buf[0].stream;
     std::vector<unsigned long long> ctr(n + 1);
-    PRH_HIP(hipMemcpyAsync(b->seeds, state_inc, sizeof(uint64_t) * b->seed_words, hipMemcpyHostToDevice, b->stream));
-    PRH_HIP(hipEventRecord(b->ev[0], b->stream));
-    PRH_HIP(hipGraphLaunch(b->exec, b->stream));
-    PRH_HIP(hipEventRecord(b->ev[1], b->stream));
-    PRH_HIP(hipMemcpyAsync(ctr.data(), b->counters, sizeof(unsigned long long) * (n + 1), hipMemcpyDeviceToHost,
-                           b->stream));
-    PRH_HIP(hipMemcpyAsync(b->piece_start.data(), b->marks, sizeof(int64_t) * (n_pieces + 1),
-                           hipMemcpyDeviceToHost, b->stream));
-    PRH_HIP(hipStreamSynchronize(b->stream));
+    PRH_HIP(hipMemcpyAsync(b->seeds, state_inc, sizeof(uint64_t) * b->seed_words, hipMemcpyHostToDevice, s0));
+    PRH_HIP(hipEventRecord(b->ev[0], s0));
+    int rc = enqueue_pass(b);
+    if (rc) return rc;
+    PRH_HIP(hipEventRecord(b->ev[1], s0));
+    PRH_HIP(hipMemcpyAsync(ctr.data(), b->counters, sizeof(unsigned long long) * (n + 1), hipMemcpyDeviceToHost, s0));
+    PRH_HIP(hipMemcpyAsync(b->piece_start.data(), b->marks, sizeof(int64_t) * (n_pieces + 1), hipMemcpyDeviceToHost,
+                           s0));
+    PRH_HIP(hipStreamSynchronize(s0));
     if (elapsed_ms) {
         float ms = 0.f;
         PRH_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
@@ -1635,9 +1768,11 @@ int prh_trace_batch_edges(const prh_trace_batch *b, int32_t segment, int32_t *or
 int prh_trace_batch_info(const prh_trace_batch *b, int64_t *info, int32_t n)
 {
     PRH_CHECK_ARG(b && info && n >= 0, "prh_trace_batch_info: null pointer");
-    const int64_t v[] = {b->bytes,   (int64_t)b->chunk_waves.size(), (int64_t)b->pieces.size(),
-                         b->ray_cap, b->edge_cap,                    b->rays};
-    for (int i = 0; i < n && i < 6; ++i) info[i] = v[i];
+    const int64_t v[] = {b->bytes,    (int64_t)b->chunk_waves.size(), (int64_t)b->pieces.size(),
+                         b->ray_cap,  b->edge_cap,                    b->rays,
+                         b->n_streams, b->use_graph ? 1 : 0,          b->variant};
+    const int m = (int)(sizeof(v) / sizeof(v[0]));
+    for (int i = 0; i < n && i < m; ++i) info[i] = v[i];
     return PRH_OK;
 }
 
@@ -1645,7 +1780,8 @@ void prh_trace_batch_destroy(prh_trace_batch *b)
 {
     if (!b) return;
     (void)hipSetDevice(b->device);
-    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (const BatchBuffers &B : b->buf)
+        if (B.stream) (void)hipStreamSynchronize(B.stream);
     free_trace_batch(b);
     delete b;
 }

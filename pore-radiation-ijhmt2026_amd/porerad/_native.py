@@ -31,6 +31,7 @@ PRH_ERR_CAPACITY = 6
 APPLY_SYMMETRIC, APPLY_MATVEC, APPLY_RMATVEC, APPLY_RAW, APPLY_RAW_T = 0, 1, 2, 3, 4
 FP64, MIXED = 0, 1
 BATCH_SLOTS = 16
+BATCH_DIRECT = 1
 
 # Every function the C ABI exports: name -> (argtypes, restype).
 _P = ctypes.c_void_p
@@ -46,6 +47,7 @@ SIGNATURES = {
     "prh_tracer_variants": ([_P, _P, _P, _P], ctypes.c_int),
     "prh_tracer_set_grain": ([_I32], ctypes.c_int),
     "prh_tracer_grain": ([_P], ctypes.c_int),
+    "prh_tracer_trim": ([], ctypes.c_int),
     "prh_geometry_create": ([_P, _I32, _I32, _I32, _F64, _P], ctypes.c_int),
     "prh_geometry_sizes": ([_P, _P, _P], ctypes.c_int),
     "prh_geometry_patch_table": ([_P, _P, _P, _P, _P], ctypes.c_int),
@@ -54,7 +56,7 @@ SIGNATURES = {
     "prh_trace_rays": ([_P, _P, _P, _I64, _P, _P], ctypes.c_int),
     "prh_trace_counts": ([_P, _I64, _I64, _I32, _I32, _P, _P, _P, _P], ctypes.c_int),
     "prh_edges_copy": ([_P, _P, _P, _P], ctypes.c_int),
-    "prh_trace_batch_create": ([_I32, _P, _P, _P, _P, _P, _I64, _I64, _P], ctypes.c_int),
+    "prh_trace_batch_create": ([_I32, _P, _P, _P, _P, _P, _I64, _I64, _I32, _I32, _P], ctypes.c_int),
     "prh_trace_batch_run": ([_P, _P, _P, _P, _P], ctypes.c_int),
     "prh_trace_batch_edges": ([_P, _I32, _P, _P, _P], ctypes.c_int),
     "prh_trace_batch_info": ([_P, _P, _I32], ctypes.c_int),
@@ -210,6 +212,12 @@ def set_tracer_grain(records: int) -> None:
     check(lib().prh_tracer_set_grain(int(records)), "prh_tracer_set_grain")
 
 
+def trim_tracer() -> None:
+    """Free the tracer's grow-only device workspace."""
+
+    check(lib().prh_tracer_trim(), "prh_tracer_trim")
+
+
 class Geometry:
     """Owner of a device patch table (``prh_geometry``)."""
 
@@ -299,9 +307,11 @@ class TraceBatch:
     leaves each segment's pooled edge table on the device.
     """
 
-    INFO = ("device_bytes", "chunks", "pieces", "ray_capacity", "edge_capacity", "rays")
+    INFO = ("device_bytes", "chunks", "pieces", "ray_capacity", "edge_capacity", "rays", "streams", "graph",
+            "variant")
 
-    def __init__(self, segments, ray_capacity: int = 0, edge_capacity: int = 0) -> None:
+    def __init__(self, segments, ray_capacity: int = 0, edge_capacity: int = 0, streams: int = 0,
+                 graph: bool = True) -> None:
         self._lib = lib()
         self.geometries = [seg[0] for seg in segments]  # kept alive with the batch
         n = len(segments)
@@ -313,7 +323,8 @@ class TraceBatch:
         handle = ctypes.c_void_p()
         check(self._lib.prh_trace_batch_create(n, ctypes.cast(handles, ctypes.c_void_p), ptr(p0), ptr(p1), ptr(rays),
                                                ptr(self.n_seeds), int(ray_capacity), int(edge_capacity),
-                                               ctypes.byref(handle)), "prh_trace_batch_create")
+                                               int(streams), 0 if graph else BATCH_DIRECT, ctypes.byref(handle)),
+              "prh_trace_batch_create")
         self.handle = handle
         self.n = n
         self.nnz = np.zeros(n, np.int64)

@@ -96,15 +96,15 @@ def batch_signature(segments, n_seeds) -> str:
 
 
 def tune_batch(segments, seeds, *, sample_fraction: float = 0.125, capacities=(1 << 26, 1 << 27, 1 << 28),
-               use_cache: bool = True) -> dict:
-    """Pick the traversal variant and the chunk size of a multi-geometry trace batch.
+               streams=(1, 2, 3), use_cache: bool = True) -> dict:
+    """Pick the traversal variant, chunk size and stream count of a multi-geometry trace batch.
 
     ``segments`` are ``TraceBatch`` segments ``(geometry, p0, p1, rays_per_patch,
     n_seeds)`` and ``seeds`` their seed lists.  The variant is timed on the
-    middle ``sample_fraction`` of every segment's patch range (one graph per
-    variant), then the chunk size (ray records per chunk) on the whole batch
-    with that variant.  Times are the device time of one graph replay after
-    a warm replay.  Leaves the chosen variant set; returns the tables.
+    middle ``sample_fraction`` of every segment's patch range, then every
+    (chunk size in ray records, streams) pair on the whole batch with that
+    variant.  Times are the device time of one replay after a warm replay.
+    Leaves the chosen variant set; returns the tables.
     """
 
     key = batch_signature(segments, [len(s) for s in seeds])
@@ -113,8 +113,8 @@ def tune_batch(segments, seeds, *, sample_fraction: float = 0.125, capacities=(1
         _native.set_tracer_variant(cache[key]["variant"])
         return cache[key]
 
-    def timed(segs, capacity=0):
-        batch = _native.TraceBatch(segs, ray_capacity=capacity)
+    def timed(segs, capacity=0, n_streams=0):
+        batch = _native.TraceBatch(segs, ray_capacity=capacity, streams=n_streams)
         try:
             batch.run(seeds)
             batch.run(seeds)
@@ -138,12 +138,13 @@ def tune_batch(segments, seeds, *, sample_fraction: float = 0.125, capacities=(1
     _native.set_tracer_variant(winner["variant"])
     cap_table = []
     for cap in capacities:
-        ms, info = timed(segments, cap)
-        cap_table.append({"ray_capacity": int(cap), "chunks": info["chunks"], "graph_ms": ms,
-                          "device_bytes": info["device_bytes"]})
-    best_cap = min(cap_table, key=lambda row: row["graph_ms"])
-    result = {"variant": winner["variant"], "ray_capacity": best_cap["ray_capacity"], "table": table,
-              "capacity_table": cap_table, "sample_fraction": sample_fraction}
+        for n_streams in streams:
+            ms, info = timed(segments, cap, n_streams)
+            cap_table.append({"ray_capacity": int(cap), "streams": info["streams"], "chunks": info["chunks"],
+                              "graph_ms": ms, "device_bytes": info["device_bytes"]})
+    best = min(cap_table, key=lambda row: row["graph_ms"])
+    result = {"variant": winner["variant"], "ray_capacity": best["ray_capacity"], "streams": best["streams"],
+              "table": table, "capacity_table": cap_table, "sample_fraction": sample_fraction}
     if use_cache:
         cache[key] = result
         _store(cache)

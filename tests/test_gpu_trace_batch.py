@@ -76,14 +76,18 @@ def _single(vf, P, p0, p1, R, seeds):
         edges.close()
 
 
-@pytest.mark.parametrize("capacity", [0, 1 << 16, 3 << 14])
-def test_batch_equals_single_calls_on_every_replay(env, media, capacity):
+@pytest.mark.parametrize("capacity,streams,graph", [(0, 0, True), (1 << 16, 1, True), (3 << 14, 3, True),
+                                                    (1 << 16, 2, False), (1 << 15, 4, True)])
+def test_batch_equals_single_calls_on_every_replay(env, media, capacity, streams, graph):
     native, vf = env
     segs = _segments(media)
-    batch = native.TraceBatch([(P.geometry(), p0, p1, R, len(s)) for P, p0, p1, R, s in segs], ray_capacity=capacity)
+    batch = native.TraceBatch([(P.geometry(), p0, p1, R, len(s)) for P, p0, p1, R, s in segs], ray_capacity=capacity,
+                              streams=streams, graph=graph)
     try:
+        assert batch.info["graph"] == int(graph)
         if capacity:
             assert batch.info["chunks"] > 1 and batch.info["pieces"] > len(segs)
+            assert batch.info["streams"] == (streams or 2)
         assert batch.info["rays"] == sum((p1 - p0) * R * len(s) for _, p0, p1, R, s in segs)
         reseeded = [[x + 1000 for x in s] for *_, s in segs]
         for seeds in ([s for *_, s in segs], reseeded, [s for *_, s in segs]):
