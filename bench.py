@@ -193,20 +193,55 @@ def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
 BATCH_SIZES = (96, 128, 160, 192)
 
 
-def batching_workload():
+def batching_workload(sizes=BATCH_SIZES):
     """16 uneven geometries: foam, random fibres, aligned fibres and perforated plates at N = 96..192."""
 
     from porerad import synthetic
 
     families = [
         # foam cells of constant size: 64 Voronoi seeds per 128^3
-        ("foam", lambda N: synthetic.voronoi_foam(N, int(round(64 * (N / 128.0) ** 3)), 0.10, 2)),
+        ("foam", lambda N: synthetic.voronoi_foam(N, max(8, int(round(64 * (N / 128.0) ** 3))), 0.10, 2)),
         ("random_fibre", lambda N: synthetic.random_fibers(N, 0.10, 3.0, 0)),
         ("aligned_fibre", lambda N: synthetic.aligned_fibers(N, 0.10, 4.0, 0)),
         ("perforated_plates", lambda N: synthetic.layered_plates(N, 0.10, 0, thickness=2, hole=6,
                                                                  hole_fraction=0.15)),
     ]
-    return [(f"{name}{N}", fn(N)) for N in BATCH_SIZES for name, fn in families]
+    return [(f"{name}{N}", fn(N)) for N in sizes for name, fn in families]
+
+
+def small_batching(R, steps=5):
+    """The same four families at N = 24..48: geometries too small to fill the chip one call at a time."""
+
+    from porerad import _native, view_factors
+
+    geos = [(name, view_factors.build_surface_patches(ph, 0.01)) for name, ph in batching_workload((24, 32, 40, 48))]
+    seeds = [[10 + k] for k in range(len(geos))]
+    n_rays = sum(P.n_patches for _, P in geos) * R
+    batch = _native.TraceBatch([(P.geometry(), 0, P.n_patches, R, 1) for _, P in geos])
+    batch.run(seeds)
+    t = time.perf_counter()
+    for _ in range(steps):
+        batch.run(seeds)
+    batched = (time.perf_counter() - t) / steps
+    nnz = list(batch.nnz)
+    batch.close()
+    for k, (_, P) in enumerate(geos):
+        view_factors.first_hit_edges(P, R, seeds[k]).close()
+    t = time.perf_counter()
+    for _ in range(steps):
+        got = []
+        for k, (_, P) in enumerate(geos):
+            e = view_factors.first_hit_edges(P, R, seeds[k])
+            got.append(e.nnz)
+            e.close()
+    calls = (time.perf_counter() - t) / steps
+    if got != nnz:
+        raise RuntimeError("batched and per-geometry edge counts differ")
+    log(f"small geometries (N 24..48, {n_rays} rays): batched {batched * 1e3:.2f} ms, per-geometry calls "
+        f"{calls * 1e3:.2f} ms ({calls / batched:.2f}x)")
+    return {"geometries": {name: int(P.n_patches) for name, P in geos}, "rays": int(n_rays),
+            "batched_ms": batched * 1e3, "per_geometry_calls_ms": calls * 1e3, "speedup": calls / batched,
+            "rays_per_s_batched": n_rays / batched}
 
 
 def run_batching(args) -> dict:
@@ -293,6 +328,7 @@ def run_batching(args) -> dict:
     }
     if tune is not None:
         record["tuning"] = tune
+    record["batching"]["small_geometries"] = small_batching(R)
     if args.no_radiosity:
         batch.close()
         return record

@@ -141,7 +141,7 @@ int prh_trace_counts(const prh_geometry *g, int64_t p0, int64_t p1, int32_t rays
  * rays_per_patch[k] rays from each of n_seeds[k] seeded blocks, pooled as
  * prh_trace_counts pools them (the same edges, bit for bit).  The segments
  * are cut at patch boundaries into chunks of at most ray_capacity ray records
- * (0: min(2^27, all rays); 112 bytes of HBM per record and stream); each
+ * (0: min(2^28, all rays); 112 bytes of HBM per record and stream); each
  * chunk is one ray-generation launch and one traversal launch over all its
  * pieces, followed by per-piece counting and compaction into one device edge
  * arena of edge_capacity entries (0: the worst case, rays per patch bounded by

@@ -1347,7 +1347,7 @@ struct prh_trace_batch {
 namespace {
 
 constexpr int64_t GEN_UNIT = 64 * GEN_ROUNDS;  // records of one generation wavefront
-constexpr int64_t DEFAULT_BATCH_RAYS = int64_t(1) << 27;
+constexpr int64_t DEFAULT_BATCH_RAYS = int64_t(1) << 28;
 
 int64_t align_gen(int64_t rays) { return (rays + GEN_UNIT - 1) / GEN_UNIT * GEN_UNIT; }
 

@@ -211,7 +211,7 @@ def first_hit_counts_batch(cases, *, ray_capacity: int = 0, edge_capacity: int =
     ``first_hit_counts_pooled(patches, rays_per_patch, seeds)``.  This is the
     per-case loop of scripts/run_prescribed_field.py:81-85 in one replay:
     uneven geometries share persistent launches, in chunks of at most
-    ``ray_capacity`` ray records (0: up to 2^27).  For repeated passes over the
+    ``ray_capacity`` ray records (0: up to 2^28).  For repeated passes over the
     same layout keep a :class:`porerad._native.TraceBatch` and call ``run``.
     """
 
