@@ -101,6 +101,7 @@ def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
     # plain FP64 path: the reference algorithm (reflection series) per request
     P, op = geoms["foam128"]
     plain_lat, refs = [], {}
+    series0 = op.device.stats()
     for Th, eps in combos:
         T_m = 0.5 * (Th + 300.0)
         t = time.perf_counter()
@@ -108,6 +109,7 @@ def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
                                         Th, 300.0, eps, relative_tolerance=1e-8, reference_temperature_K=T_m)
         plain_lat.append(time.perf_counter() - t)
         _ = st
+    series1 = op.device.stats()
     plain = {"solves_per_s": len(plain_lat) / sum(plain_lat), "p50_ms": float(np.percentile(plain_lat, 50) * 1e3),
              "p99_ms": float(np.percentile(plain_lat, 99) * 1e3), "precision": "fp64 reflection series, rtol 1e-8",
              "geometry": "foam128"}
@@ -147,9 +149,11 @@ def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
     served = serving.latency_summary(replies, time.perf_counter() - t)
     served.update({"arrival_rate_per_s": rate, "requests": n_requests, "geometries": list(geoms),
                    "precision": "fp32 chebyshev sweeps + fp64 refinement, rtol 1e-9", "sweeps": srv.sweeps})
+    ticks0 = srv.engine_stats()
     t = time.perf_counter()
     sat = srv.run(make_requests(n_requests, 8), real_time=False)
     saturation = serving.latency_summary(sat, time.perf_counter() - t)
+    ticks1 = srv.engine_stats()
 
     # precision: every (geometry, T_hot, eps) refined to 1e-10 against the FP64 series at 1e-13
     prec = [serving.RadiosityRequest(k, Th, 300.0, eps, 1e-10) for k in geoms for Th, eps in combos]
@@ -178,7 +182,9 @@ def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
     }
     srv.close()
     err = flux_errors(replies + sat)
+    roofline = {"engine": engine_roofline(ticks0, ticks1), "fp64_series": series_roofline(series0, series1)}
     return {
+        "roofline": roofline,
         "plain_fp64": plain,
         "served": served,
         "saturation": saturation,
@@ -363,6 +369,42 @@ def run_batching(args) -> dict:
     return record
 
 
+def engine_roofline(before, after) -> dict:
+    """HBM roofline of the served engine's ticks over a run.
+
+    Algorithmic bytes of one tick with k running right-hand sides:
+    nnz(S) (4 B column + s_v B value) + 8 (n + 1) row pointers + 16 n k for
+    the FP32 slot vectors (4 vectors x 4 B per patch and slot); the time is
+    the ticks' device time from HIP events around each tick."""
+
+    total_bytes = total_ms = ticks = 0.0
+    for key, a in after.items():
+        b = before.get(key, {"ticks": 0, "ms": 0.0, "slot_ticks": 0})
+        n_t = a["ticks"] - b["ticks"]
+        k_t = a["slot_ticks"] - b["slot_ticks"]
+        total_bytes += n_t * (a["nnz"] * (4 + a["value_bytes"]) + 8 * (a["n"] + 1)) + 16 * a["n"] * k_t
+        total_ms += a["ms"] - b["ms"]
+        ticks += n_t
+    achieved = total_bytes / (total_ms * 1e-3) / 1e9 if total_ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS, "kernel": "k_tick", "ticks": int(ticks),
+            "ms_per_tick": total_ms / ticks if ticks else None, "algorithmic_bytes": total_bytes}
+
+
+def series_roofline(before, after) -> dict:
+    """HBM roofline of the plain FP64 reflection series: per reflection
+    nnz(S) (4 B column + 8 B value) + 8 (n + 1) + 32 n bytes (one FP64 vector
+    gathered, four read or written), over the series' device time."""
+
+    refl = after["reflections"] - before["reflections"]
+    ms = after["ms"] - before["ms"]
+    total = refl * (after["nnz"] * 12 + 8 * (after["n"] + 1) + 32 * after["n"])
+    achieved = total / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS, "kernel": "k_series_step", "reflections": int(refl),
+            "ms_per_reflection": ms / refl if refl else None}
+
+
 def cpu_baseline(phases, R, seed, budget_s=12.0):
     from oracle import oracle as O
 
@@ -498,6 +540,10 @@ def main() -> None:
             f"p50 {rad['served']['p50_ms']:.1f} ms p99 {rad['served']['p99_ms']:.1f} ms; saturation "
             f"{rad['saturation']['solves_per_s']:.1f} solves/s ({rad['speedup_vs_plain_fp64']:.1f}x plain); "
             f"max flux error {max(rad['max_rel_flux_error_vs_fp64'].values()):.1e}")
+        rf = rad["roofline"]
+        log(f"radiosity roofline: engine ticks {rf['engine']['achieved']:.0f} GB/s ({rf['engine']['frac'] * 100:.1f} % "
+            f"of HBM, {rf['engine']['ms_per_tick']:.3f} ms/tick); FP64 series {rf['fp64_series']['achieved']:.0f} GB/s "
+            f"({rf['fp64_series']['frac'] * 100:.1f} %, {rf['fp64_series']['ms_per_reflection']:.3f} ms/reflection)")
         cb = rad["serving_with_coupled"]
         log(f"precision at 1e-10: {rad['precision']['max_rel_error_per_flux']}")
         log(f"with coupled solves: radiosity {cb['radiosity']['solves_per_s']:.1f} solves/s p50 "

@@ -191,6 +191,11 @@ int prh_operator_nnz(const prh_operator *op, int64_t *nnz_symmetric, int64_t *nn
 /* y = op(x) for one of the PRH_APPLY_* modes; x, y host arrays of n. */
 int prh_operator_apply(prh_operator *op, int32_t mode, const double *x, double *y);
 
+/* Device time of the FP64 reflection series run on this operator (HIP events
+ * around each series): out[0..n) = series, reflections, milliseconds,
+ * nonzeros of S, patches, 1 if the integer form K is held.  No counterpart
+ * in the reference (measurement). */
+int prh_operator_stats(const prh_operator *op, double *out, int32_t n);
 /* Reflection series J = E + (1 - eps) F_p^T J for n_rhs emissions.
  * emitted: n_rhs x n (row-major, one emission per row); emissivity:
  * n_rhs x n if emissivity_per_rhs else n.  Outputs absorbed (n_rhs x n),
@@ -217,6 +222,12 @@ void prh_operator_destroy(prh_operator *op);
 int prh_batch_create(prh_operator *op, prh_batch **out);
 /* Device bytes held by the batch (for HBM-budget admission). */
 int prh_batch_bytes(const prh_batch *b, int64_t *bytes);
+/* Device time of the engine's ticks (HIP events around every tick, read as
+ * the pipeline retires it): out[0..n) = ticks timed, their milliseconds,
+ * slot-ticks (running slots summed over the ticks), nonzeros of S, patches,
+ * bytes per matrix value read by a tick (2: integer K, 4: FP32 S).  No
+ * counterpart in the reference (measurement). */
+int prh_batch_stats(const prh_batch *b, double *out, int32_t n);
 /* Bit q set = slot q free. */
 int prh_batch_free_slots(const prh_batch *b, uint32_t *free_mask);
 /* Describe the patches for linear-field requests: class per patch (0 hot

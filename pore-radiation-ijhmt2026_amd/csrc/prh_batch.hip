@@ -414,6 +414,11 @@ struct prh_batch {
     Pinned *host = nullptr;
     hipEvent_t ev[RING] = {};
     bool ev_used[RING] = {};
+    // device time of the ticks: events around each tick, read one step later
+    hipEvent_t tick_ev[RING][2] = {};
+    int tick_slots[RING] = {};  // running slots of the step's tick (0: none launched)
+    int64_t timed_ticks = 0, slot_ticks = 0;
+    double tick_ms = 0.0;
     int64_t step = 0;   // steps taken
     int64_t ticks = 0;  // ticks launched
     SlotHost slot[SLOTS];
@@ -454,8 +459,11 @@ void free_batch(prh_batch *b)
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->host) (void)hipHostFree(b->host);
-    for (int k = 0; k < RING; ++k)
+    for (int k = 0; k < RING; ++k) {
         if (b->ev[k]) (void)hipEventDestroy(b->ev[k]);
+        for (hipEvent_t e : b->tick_ev[k])
+            if (e) (void)hipEventDestroy(e);
+    }
 }
 
 // The z buffer the next tick gathers.
@@ -467,6 +475,14 @@ int wait_step(prh_batch *b, int64_t s)
     const int k = (int)(s % RING);
     if (!b->ev_used[k]) return PRH_OK;
     PRH_HIP(hipEventSynchronize(b->ev[k]));
+    if (b->tick_slots[k] > 0) {
+        float ms = 0.f;
+        PRH_HIP(hipEventElapsedTime(&ms, b->tick_ev[k][0], b->tick_ev[k][1]));
+        b->tick_ms += ms;
+        b->timed_ticks += 1;
+        b->slot_ticks += b->tick_slots[k];
+        b->tick_slots[k] = 0;
+    }
     return PRH_OK;
 }
 
@@ -526,7 +542,7 @@ int batch_step(prh_batch *b, hipStream_t st, bool *launched)
     const int64_t s = b->step;
     SlotParam *P = b->host->param[s % RING];
     uint32_t group_mask = 0;
-    int any = 0, any_refine = 0;
+    int any = 0, any_refine = 0, running = 0;
     for (int q = 0; q < SLOTS; ++q) {
         SlotHost &S = b->slot[q];
         SlotParam &p = P[q];
@@ -555,6 +571,7 @@ int batch_step(prh_batch *b, hipStream_t st, bool *launched)
         }
         group_mask |= 1u << (q / 4);
         any = 1;
+        running += 1;
     }
     *launched = any != 0;
     if (any) {
@@ -571,9 +588,13 @@ int batch_step(prh_batch *b, hipStream_t st, bool *launched)
         a.n_blocks = b->n_blocks;
         a.group_mask = group_mask;
         a.any_refine = any_refine;
+        hipEvent_t *tev = b->tick_ev[s % RING];
+        PRH_HIP(hipEventRecord(tev[0], st));
         if (b->op->uniform) k_tick<true><<<(unsigned)b->n_blocks, BT, 0, st>>>(a);
         else k_tick<false><<<(unsigned)b->n_blocks, BT, 0, st>>>(a);
         PRH_HIP(hipGetLastError());
+        PRH_HIP(hipEventRecord(tev[1], st));
+        b->tick_slots[s % RING] = running;
         b->ticks += 1;
         if (any_refine) {
             k_column_totals<<<2 * SLOTS, 256, 0, st>>>(b->part, b->n_blocks, b->tot);
@@ -661,8 +682,11 @@ int prh_batch_create(prh_operator *op, prh_batch **out)
     alloc((void **)&b->stage, (size_t)op->n * 2 * 8);
     alloc((void **)&b->param_dev, (size_t)RING * SLOTS * sizeof(SlotParam));
     if (ok && hipHostMalloc((void **)&b->host, sizeof(Pinned), hipHostMallocDefault) != hipSuccess) ok = false;
-    for (int k = 0; ok && k < RING; ++k)
+    for (int k = 0; ok && k < RING; ++k) {
         if (hipEventCreateWithFlags(&b->ev[k], hipEventDisableTiming) != hipSuccess) ok = false;
+        for (hipEvent_t &e : b->tick_ev[k])
+            if (ok && hipEventCreate(&e) != hipSuccess) ok = false;
+    }
     if (!ok) {
         free_batch(b);
         delete b;
@@ -683,6 +707,15 @@ int prh_batch_bytes(const prh_batch *b, int64_t *bytes)
 {
     PRH_CHECK_ARG(b && bytes, "prh_batch_bytes: null pointer");
     *bytes = b->bytes;
+    return PRH_OK;
+}
+
+int prh_batch_stats(const prh_batch *b, double *out, int32_t n)
+{
+    PRH_CHECK_ARG(b && out && n >= 0, "prh_batch_stats: null pointer");
+    const double v[] = {(double)b->timed_ticks, b->tick_ms, (double)b->slot_ticks, (double)b->op->nnz_s,
+                        (double)b->n, b->op->uniform ? 2.0 : 4.0};
+    for (int i = 0; i < n && i < (int)(sizeof(v) / sizeof(v[0])); ++i) out[i] = v[i];
     return PRH_OK;
 }
 

@@ -220,6 +220,8 @@ void free_operator(prh_operator *op)
                     op->part_sum, op->state, op->kval, op->val32, op->scale32};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (hipEvent_t e : op->series_ev)
+        if (e) (void)hipEventDestroy(e);
 }
 
 // Host assembly of S over union(C, C^T, diagonal), rows merged in parallel.
@@ -367,6 +369,11 @@ int series_fp64(prh_operator *op, double relative_tolerance, int32_t max_reflect
     const unsigned rb = blocks_for(n, ROWS_PER_BLOCK);
     const int64_t n_part = rb;
     const unsigned ib = (unsigned)std::min<int64_t>(blocks_for(n, RED_THREADS), 1024);
+    if (!op->series_ev[0]) {
+        PRH_HIP(hipEventCreate(&op->series_ev[0]));
+        PRH_HIP(hipEventCreate(&op->series_ev[1]));
+    }
+    PRH_HIP(hipEventRecord(op->series_ev[0], st));
     k_series_init<<<ib, RED_THREADS, 0, st>>>(op->emit, op->scale, op->area, n, op->flight, op->absorbed, op->w0,
                                               op->part_abs);
     k_series_start<<<1, RED_THREADS, 0, st>>>(op->part_abs, (int)ib, op->state);
@@ -386,9 +393,18 @@ int series_fp64(prh_operator *op, double relative_tolerance, int32_t max_reflect
         }
         PRH_HIP(hipGetLastError());
         it += batch;
+        PRH_HIP(hipEventRecord(op->series_ev[1], st));
         PRH_HIP(hipMemcpyAsync(&hs, op->state, sizeof(hs), hipMemcpyDeviceToHost, st));
         PRH_HIP(hipStreamSynchronize(st));
         if (hs.done) break;
+    }
+    {
+        // steps launched after convergence return at once; count the reflections made
+        float ms = 0.f;
+        PRH_HIP(hipEventElapsedTime(&ms, op->series_ev[0], op->series_ev[1]));
+        op->series_ms += ms;
+        op->series_count += 1;
+        op->series_steps += hs.iterations;
     }
     if (!hs.done) {
         set_error("radiosity reflection series did not converge");
@@ -579,6 +595,15 @@ int prh_radiosity(prh_operator *op, int32_t n_rhs, const double *emitted, const 
         residual[q] = res;
         iterations[q] = its;
     }
+    return PRH_OK;
+}
+
+int prh_operator_stats(const prh_operator *op, double *out, int32_t n)
+{
+    PRH_CHECK_ARG(op && out && n >= 0, "prh_operator_stats: null pointer");
+    const double v[] = {(double)op->series_count, (double)op->series_steps, op->series_ms, (double)op->nnz_s,
+                        (double)op->n,            op->uniform ? 1.0 : 0.0};
+    for (int i = 0; i < n && i < (int)(sizeof(v) / sizeof(v[0])); ++i) out[i] = v[i];
     return PRH_OK;
 }
 

@@ -51,6 +51,10 @@ struct prh_operator {
     float *val32 = nullptr;    // FP32 S (non-uniform only)
     float *scale32 = nullptr;
     prh_batch *batch = nullptr;  // engine of the mixed-precision path, created on first use
+    // device time of the FP64 series (HIP events around each series)
+    hipEvent_t series_ev[2] = {nullptr, nullptr};
+    int64_t series_count = 0, series_steps = 0;
+    double series_ms = 0.0;
 };
 
 extern "C" int prh_radiosity_mixed(prh_operator *op, int32_t n_rhs, const double *emitted, const double *emissivity,

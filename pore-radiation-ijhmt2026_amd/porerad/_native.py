@@ -65,6 +65,8 @@ SIGNATURES = {
     "prh_operator_create": ([_I64, _P, _P, _P, _I64, _P, _P, _F64, _P], ctypes.c_int),
     "prh_operator_set_scale": ([_P, _P], ctypes.c_int),
     "prh_operator_nnz": ([_P, _P, _P], ctypes.c_int),
+    "prh_operator_stats": ([_P, _P, _I32], ctypes.c_int),
+    "prh_batch_stats": ([_P, _P, _I32], ctypes.c_int),
     "prh_operator_apply": ([_P, _I32, _P, _P], ctypes.c_int),
     "prh_radiosity": ([_P, _I32, _P, _P, _I32, _F64, _I32, _I32, _P, _P, _P], ctypes.c_int),
     "prh_operator_destroy": ([_P], None),
@@ -391,6 +393,14 @@ class Operator:
         check(self._lib.prh_operator_nnz(handle, ctypes.byref(a), ctypes.byref(b)), "prh_operator_nnz")
         self.nnz_symmetric, self.nnz_raw = int(a.value), int(b.value)
 
+    def stats(self) -> dict:
+        """Device time of the FP64 reflection series run on this operator."""
+
+        out = np.zeros(6)
+        check(self._lib.prh_operator_stats(self.handle, ptr(out), 6), "prh_operator_stats")
+        return {"series": int(out[0]), "reflections": int(out[1]), "ms": float(out[2]), "nnz": int(out[3]),
+                "n": int(out[4]), "integer_form": bool(out[5])}
+
     def set_scale(self, scale: np.ndarray) -> None:
         scale = np.ascontiguousarray(scale, dtype=np.float64)
         check(self._lib.prh_operator_set_scale(self.handle, ptr(scale)), "prh_operator_set_scale")
@@ -442,6 +452,14 @@ class Batch:
         b = ctypes.c_int64()
         check(self._lib.prh_batch_bytes(handle, ctypes.byref(b)), "prh_batch_bytes")
         self.bytes = int(b.value)
+
+    def stats(self) -> dict:
+        """Device time of the engine's ticks (HIP events around each tick)."""
+
+        out = np.zeros(6)
+        check(self._lib.prh_batch_stats(self.handle, ptr(out), 6), "prh_batch_stats")
+        return {"ticks": int(out[0]), "ms": float(out[1]), "slot_ticks": int(out[2]), "nnz": int(out[3]),
+                "n": int(out[4]), "value_bytes": int(out[5])}
 
     def free_slots(self) -> int:
         m = ctypes.c_uint32()

@@ -229,6 +229,16 @@ class RadiosityServer:
         self.slots[key] = {}
         return g
 
+    def engine_stats(self) -> dict:
+        """Tick timings of the live engines that report them (``Batch.stats``), by geometry."""
+
+        out = {}
+        for key, engine in self.engines.items():
+            stats = getattr(engine, "stats", None)
+            if stats is not None:
+                out[key] = stats()
+        return out
+
     def engine_bytes(self) -> int:
         """Device bytes of the live engines and of the running coupled solve."""
 
