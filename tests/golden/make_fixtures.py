@@ -13,7 +13,9 @@ and writes, next to this script:
                   array) of the microstructures used by the tests, with shapes
                   and the SHA-256 of the original bytes
   cases.json      hash cases, prescribed-field cases with their archived rows,
-                  canonical coupled rows and the full-path cross-check rows
+                  canonical coupled rows and the full-path cross-check rows;
+                  every prescribed-field case and every coupled case whose
+                  microstructure is shipped, with its archived result rows
 
 Run once in a container that holds the reference:
     python tests/golden/make_fixtures.py /path/to/reference
@@ -103,7 +105,41 @@ def main(reference: str) -> None:
         for r in res_x if r["method"] == "first_hit_radiosity"
     ]
 
-    names = sorted({c["microstructure"] for c in hash_cases} | {c["microstructure"] for c in prescribed})
+    # every case of the two case tables whose microstructure is shipped, with its archived rows
+    shipped = {f[:-4] for f in os.listdir(os.path.join(reference, "data", "microstructures")) if f.endswith(".npz")}
+    prescribed_cases = []
+    for row in cases_p:
+        prescribed_cases.append({
+            **{k: row[k] for k in ("case_id", "group", "structure", "microstructure", "transfer_axis",
+                                   "phase_sha256_x", "conditions")},
+            "H_m": float(row["H_m"]),
+            "rays_per_patch_per_block": int(row["rays_per_patch_per_block"]),
+            **{k: [int(s) for s in row[k].split(";")]
+               for k in ("seeds_A", "seeds_B", "append_seeds_A", "append_seeds_B")},
+            "append_used": row["append_used"] == "true",
+            "results": [
+                {k: float(r[k]) for k in ("T_mean_K", "delta_T_K", "epsilon", "q_h_W_m2", "q_c_W_m2", "q_i_W_m2",
+                                          "ab_wall_channel_rel_difference")}
+                for r in res_p if r["case_id"] == row["case_id"]
+            ],
+        })
+    archived_c = {r["case_id"]: r for r in res_c}
+    coupled_cases = []
+    for row in cases_c:
+        if row["microstructure"] not in shipped or row["case_id"] not in archived_c:
+            continue
+        coupled_cases.append({
+            **{k: row[k] for k in ("case_id", "group", "structure", "microstructure", "transfer_axis",
+                                   "phase_sha256_x", "raw_count_sha256_A", "raw_count_sha256_B")},
+            **{k: float(row[k]) for k in ("H_m", "T_mean_K", "delta_T_K", "epsilon")},
+            **{k: int(row[k]) for k in ("repeats_along_axis", "N_axis", "rays_per_patch_per_block")},
+            **{k: [int(s) for s in row[k].split(";")] for k in ("seeds_A", "seeds_B")},
+            "archived": {k: (v if k in ("case_id", "group", "structure", "microstructure", "transfer_axis") or v == ""
+                             else float(v)) for k, v in archived_c[row["case_id"]].items()},
+        })
+
+    names = sorted({c["microstructure"] for c in hash_cases} | {c["microstructure"] for c in prescribed}
+                   | {c["microstructure"] for c in prescribed_cases} | {c["microstructure"] for c in coupled_cases})
     arrays = {}
     for name in names:
         path = os.path.join(reference, "data", "microstructures", f"{name}.npz")
@@ -117,7 +153,8 @@ def main(reference: str) -> None:
     np.savez_compressed(os.path.join(HERE, "geometries.npz"), **arrays)
     with open(os.path.join(HERE, "cases.json"), "w") as handle:
         json.dump({"hash_cases": hash_cases, "prescribed_field": prescribed, "coupled_canonical": coupled,
-                   "fullpath_crosscheck": crosscheck}, handle, indent=1)
+                   "fullpath_crosscheck": crosscheck, "prescribed_cases": prescribed_cases,
+                   "coupled_cases": coupled_cases}, handle, indent=1)
     print(f"wrote {len(names)} geometries, {len(hash_cases)} hash cases, {len(prescribed)} prescribed cases")
 
 

@@ -1,0 +1,82 @@
+"""Case drivers (porerad.cases) against the reference's archived result tables.
+
+Every row the driver writes is compared with its archived row:
+
+* prescribed field (results/prescribed_field.csv): q_h, q_c, q_i within 1e-10
+  of max(|q_h|, |q_c|) of the archived row (the FP64 operator and series to
+  rtol 1e-13, as test_gpu_exchange.py); the A/B wall-channel difference,
+  which the sets A and B evaluated on their own give, within 1e-8 absolute
+  (it is about 5e-4, a difference of channels each good to 1e-10);
+* coupled (results/coupled.csv): wall fluxes within 1e-9 of the larger total
+  flux, k_eff and the radiative fraction within 1e-9 relative, T_min/T_max
+  within 1e-8 K, the same Newton update count, and the pooled counts of sets
+  A and B identical to the stored hashes (columns the archive leaves empty,
+  the cold-side fluxes and T_min/T_max of the matched and thickness groups,
+  are not compared).
+
+By default a few cases cover each kind of case (an appended case, a matched
+pair, a doubled thickness); PORERAD_FULL=1 runs every case whose
+microstructure is shipped (48 prescribed-field cases, 36 coupled cases).
+"""
+
+from __future__ import annotations
+
+import os
+
+import pytest
+
+from golden_data import cases, phases
+
+pytestmark = pytest.mark.gpu
+
+FULL = os.environ.get("PORERAD_FULL") == "1"
+PRESCRIBED_DEFAULT = ["pf_TL_core_phi0.10_s0_y", "pf_FA_matched_pair1_x", "pf_OF_core_phi0.15_s1_z"]
+COUPLED_DEFAULT = ["coupled_FI_matched_pair1_y", "coupled_OF_x_H20"]
+
+
+def _pick(table, default):
+    return [pytest.param(c, id=c["case_id"]) for c in cases()[table] if FULL or c["case_id"] in default]
+
+
+@pytest.fixture(scope="module")
+def drivers():
+    from porerad import _native, cases as C
+
+    _native.lib()
+    return C
+
+
+@pytest.mark.parametrize("case", _pick("prescribed_cases", PRESCRIBED_DEFAULT))
+def test_prescribed_field_case_reproduces_archive(drivers, case):
+    rows = drivers.prescribed_field_case(case, phases(case["microstructure"]), ab=True)
+    archived = {(r["T_mean_K"], r["epsilon"]): r for r in case["results"]}
+    assert len(rows) == len(archived)
+    for row in rows:
+        want = archived[(float(row["T_mean_K"]), float(row["epsilon"]))]
+        scale = max(abs(want["q_h_W_m2"]), abs(want["q_c_W_m2"]))
+        for key in ("q_h_W_m2", "q_c_W_m2", "q_i_W_m2"):
+            assert abs(row[key] - want[key]) <= 1e-10 * scale, (key, row[key], want[key])
+        assert abs(row["ab_wall_channel_rel_difference"] - want["ab_wall_channel_rel_difference"]) <= 1e-8
+        assert row["delta_T_K"] == "20"
+
+
+@pytest.mark.parametrize("case", _pick("coupled_cases", COUPLED_DEFAULT))
+def test_coupled_case_reproduces_archive(drivers, case):
+    messages = []
+    row = drivers.coupled_case(case, phases(case["microstructure"]), log=messages.append)
+    for name in "AB":
+        if case[f"raw_count_sha256_{name}"]:
+            assert f"first-hit counts of set {name}: identical to the stored hash" in messages
+    # the archive leaves some columns empty for the matched and thickness groups
+    want = {k: v for k, v in case["archived"].items() if v != ""}
+    assert row["newton_updates"] == int(want["newton_updates"])
+    scale = max(abs(want[k]) for k in ("q_total_hot_W_m2", "q_total_cold_W_m2") if k in want)
+    for key in ("q_cond_hot_W_m2", "q_rad_hot_W_m2", "q_total_hot_W_m2", "q_cond_cold_W_m2", "q_rad_cold_W_m2",
+                "q_total_cold_W_m2"):
+        if key in want:
+            assert abs(row[key] - want[key]) <= 1e-9 * scale, key
+    for key in ("k_eff_apparent_W_mK", "radiative_fraction_hot"):
+        assert abs(row[key] - want[key]) <= 1e-9 * abs(want[key]), key
+    for key in ("T_min_K", "T_max_K"):
+        if key in want:
+            assert abs(row[key] - want[key]) <= 1e-8, key
