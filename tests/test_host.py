@@ -111,3 +111,21 @@ def test_random_fibres_tube_windows_equal_brute_force(N, radius, seed):
     from porerad import synthetic
 
     assert np.array_equal(synthetic.random_fibers(N, 0.10, radius, seed), _fibres_brute_force(N, 0.10, radius, seed))
+
+
+def test_case_driver_helpers(tmp_path):
+    from porerad import cases as C
+
+    assert C._seeds("5;7;11") == [5, 7, 11] and C._seeds([3, 4]) == [3, 4]
+    assert C._conditions("400/0.3;700/0.9") == [(400.0, 0.3), (700.0, 0.9)]
+    out = tmp_path / "rows.csv"
+    first = [{"case_id": "a", "q_h_W_m2": 1.0}, {"case_id": "b", "q_h_W_m2": 2.0}]
+    C.write_rows(out, first, ["case_id", "q_h_W_m2"])
+    C.write_rows(out, [{"case_id": "a", "q_h_W_m2": 3.0}], ["case_id", "q_h_W_m2"])  # replaces case a only
+    import csv
+
+    with open(out, newline="") as handle:
+        rows = list(csv.DictReader(handle))
+    assert [(r["case_id"], r["q_h_W_m2"]) for r in rows] == [("b", "2.0"), ("a", "3.0")]
+    with pytest.raises(ValueError, match="differs from the one listed"):
+        C._check_phases(np.zeros((2, 2, 2), np.uint8), "0" * 64)
