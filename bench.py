@@ -393,12 +393,14 @@ def engine_roofline(before, after) -> dict:
 
 def series_roofline(before, after) -> dict:
     """HBM roofline of the plain FP64 reflection series: per reflection
-    nnz(S) (4 B column + 8 B value) + 8 (n + 1) + 32 n bytes (one FP64 vector
-    gathered, four read or written), over the series' device time."""
+    nnz(S) (4 B column + s_v B value: 2 for the integer form K, 8 for FP64 S)
+    + 8 (n + 1) + 32 n bytes (one FP64 vector gathered, four read or written),
+    over the series' device time."""
 
     refl = after["reflections"] - before["reflections"]
     ms = after["ms"] - before["ms"]
-    total = refl * (after["nnz"] * 12 + 8 * (after["n"] + 1) + 32 * after["n"])
+    s_v = 2 if after["integer_form"] else 8
+    total = refl * (after["nnz"] * (4 + s_v) + 8 * (after["n"] + 1) + 32 * after["n"])
     achieved = total / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS, "kernel": "k_series_step", "reflections": int(refl),

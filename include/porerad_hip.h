@@ -305,6 +305,13 @@ int prh_coupled_linearize(prh_coupled *c, int32_t T, double T_hot_K, double T_co
 /* out = A v - dQ/dT v with the walls fixed (coupled.py:108-126, 380-383). */
 int prh_coupled_jacobian(prh_coupled *c, int32_t v, double emissivity, double relative_tolerance, int32_t out,
                          int32_t *series_iterations);
+/* Precision of the Jacobian-vector products' radiosity solves: PRH_FP64 (the
+ * reference's reflection series, default) or PRH_MIXED (the batched engine's
+ * FP32 Chebyshev sweeps with FP64 refinement to the same relative tolerance;
+ * the radiation of the Newton residual stays an FP64 series).  Replaces the
+ * inner solves of radiation_jvp inside newton_krylov, porerad/coupled.py:108-126,
+ * 362-425, with their precision as a choice. */
+int prh_coupled_set_precision(prh_coupled *c, int32_t precision);
 /* Fixed-step Jacobi-PCG on A + diag(emission derivative at the field in
  * slot T), started from zero (coupled.py:292-315): set, then apply. */
 int prh_coupled_set_preconditioner(prh_coupled *c, int32_t T, double emissivity, int32_t steps);

@@ -124,7 +124,24 @@ __global__ __launch_bounds__(BT) void k_tick(TickArgs a)
     double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
     if (row < M.n && ((a.group_mask >> h) & 1u)) {
         const int64_t e1 = M.rowptr[row + 1];
-        for (int64_t e = M.rowptr[row] + g; e < e1; e += 16) {
+        int64_t e = M.rowptr[row] + g;
+        // two independent gathers in flight per lane (rows hold ~10^3 nonzeros)
+        for (; e + 16 < e1; e += 32) {
+            const int32_t j0 = M.col[e], j1 = M.col[e + 16];
+            const double v0 = UNIFORM ? (double)M.kval[e] : M.val64[e];
+            const double v1 = UNIFORM ? (double)M.kval[e + 16] : M.val64[e + 16];
+            const float4 z0 = *reinterpret_cast<const float4 *>(a.z_in + (size_t)j0 * SLOTS + 4 * h);
+            const float4 z1 = *reinterpret_cast<const float4 *>(a.z_in + (size_t)j1 * SLOTS + 4 * h);
+            acc0 = fma(v0, (double)z0.x, acc0);
+            acc1 = fma(v0, (double)z0.y, acc1);
+            acc2 = fma(v0, (double)z0.z, acc2);
+            acc3 = fma(v0, (double)z0.w, acc3);
+            acc0 = fma(v1, (double)z1.x, acc0);
+            acc1 = fma(v1, (double)z1.y, acc1);
+            acc2 = fma(v1, (double)z1.z, acc2);
+            acc3 = fma(v1, (double)z1.w, acc3);
+        }
+        if (e < e1) {
             const int32_t j = M.col[e];
             const double v = UNIFORM ? (double)M.kval[e] : M.val64[e];
             const float4 z = *reinterpret_cast<const float4 *>(a.z_in + (size_t)j * SLOTS + 4 * h);
@@ -315,6 +332,23 @@ __global__ __launch_bounds__(256) void k_finish(int64_t n, int q, FieldView F, S
     block_sum_to(s1, sh, part + n_part + blockIdx.x);
     block_sum_to(s2, sh, part + 2 * n_part + blockIdx.x);
     block_sum_to(s3, sh, part + 3 * n_part + blockIdx.x);
+}
+
+// Admission of a device-resident emission with one emissivity.
+__global__ __launch_bounds__(256) void k_admit_uniform(MatView M, SlotVecs V, float *z_next, int q, const double *E,
+                                                       double eps, double inv_s, int uniform)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M.n) return;
+    admit_row(M, V, z_next, i, q, E[i], eps, inv_s, uniform != 0);
+}
+
+// Per-block sums of |x|.
+__global__ __launch_bounds__(256) void k_abs_part(const double *x, int64_t n, double *part)
+{
+    __shared__ double sh[256];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    block_sum_to(i < n ? fabs(x[i]) : 0.0, sh, part + blockIdx.x);
 }
 
 // Absorbed power eps * y of slot q into a packed vector.
@@ -642,6 +676,75 @@ void report(const prh_batch *b, uint32_t *done_mask, uint32_t *active_mask)
 }
 
 }  // namespace
+
+namespace prh {
+
+int batch_solve_device(prh_batch *b, const double *E, double emissivity, double rtol, hipStream_t st,
+                       double *absorbed, int32_t *iterations)
+{
+    PRH_CHECK_ARG(b && E && absorbed, "batch_solve_device: null pointer");
+    PRH_CHECK_ARG(emissivity > 0.0 && emissivity <= 1.0 && rtol > 0.0, "batch_solve_device: bad emissivity or tolerance");
+    const int q = first_free(b);
+    if (q < 0) {
+        set_error("batch_solve_device: all %d slots are busy", SLOTS);
+        return PRH_ERR_CAPACITY;
+    }
+    const int64_t n = b->n;
+    const unsigned grid = grid_of(n, 256);
+    // ||E||_1 sets the scale of the first epoch (as prh_batch_admit does on the host)
+    double *part = b->part + (size_t)2 * SLOTS * b->n_blocks - grid;
+    k_abs_part<<<grid, 256, 0, st>>>(E, n, part);
+    k_column_totals<<<1, 256, 0, st>>>(part, (int64_t)grid, b->tot + 2 * SLOTS - 1);
+    PRH_HIP(hipGetLastError());
+    PRH_HIP(hipMemcpyAsync(&b->host->e_norm[q], b->tot + 2 * SLOTS - 1, sizeof(double), hipMemcpyDeviceToHost, st));
+    PRH_HIP(hipStreamSynchronize(st));
+    const double e_norm = b->host->e_norm[q];
+    const unsigned ngrid = grid_of(n, 256);
+    if (!(e_norm > 0.0)) {
+        // the zero emission is absorbed as zero
+        PRH_HIP(hipMemsetAsync(absorbed, 0, (size_t)n * 8, st));
+        if (iterations) *iterations = 0;
+        return PRH_OK;
+    }
+    const double s0 = e_norm / (double)n;
+    k_admit_uniform<<<ngrid, 256, 0, st>>>(view_of(b), vecs_of(b), z_next(b), q, E, emissivity, 1.0 / s0,
+                                           b->op->uniform ? 1 : 0);
+    PRH_HIP(hipGetLastError());
+    SlotHost &S = b->slot[q];
+    S = SlotHost();
+    S.state = RUNNING;
+    S.rtol = rtol;
+    S.rho_max = 1.0 - emissivity;
+    S.e_norm = e_norm;
+    S.r_norm = e_norm;
+    S.s = s0;
+    S.tau = epoch_target(S.rtol, 1.0, 1.0);
+    S.m = ticks_for(S.rho_max, S.tau);
+    int rc = PRH_OK;
+    while (S.state != DONE) {
+        bool launched = false;
+        if ((rc = batch_step(b, st, &launched))) return rc;
+        if (!launched) {
+            if ((rc = batch_drain(b))) return rc;
+            if (S.state != DONE && S.state != FINISHING) {
+                set_error("batch_solve_device: the solve stopped without finishing");
+                return PRH_ERR_HIP;
+            }
+        }
+    }
+    if (S.failed) {
+        S = SlotHost();
+        set_error("radiosity reflection series did not converge");
+        return PRH_ERR_SERIES;
+    }
+    k_collect<<<ngrid, 256, 0, st>>>(n, q, b->eps64, b->y64, absorbed);
+    PRH_HIP(hipGetLastError());
+    if (iterations) *iterations = S.ticks + S.epochs;
+    S = SlotHost();
+    return PRH_OK;
+}
+
+}  // namespace prh
 
 extern "C" {
 

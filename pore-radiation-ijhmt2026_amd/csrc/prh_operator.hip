@@ -33,17 +33,53 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
+// S in one of its two stored forms: FP64 values, or (every patch with the
+// same area a and ray count R) the integer K = C + C^T with
+// S = a (K / (2R) + delta I), read as 2 B instead of 8 B per nonzero.  Both
+// are summed in FP64; the integer form only changes the rounding of S_ij.
+struct SView {
+    const int64_t *rowptr;
+    const int32_t *col;
+    const double *val;
+    const uint16_t *kval;
+    const double *area;
+    double inv_2R, delta;
+};
+
+// (S x)_row from the lane partials of one wavefront.
+template <bool UNIFORM>
+__device__ __forceinline__ double s_row(const SView &S, int64_t row, int lane, const double *x)
+{
+    // four independent gathers in flight per lane (rows hold ~10^3 nonzeros)
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    const int64_t e1 = S.rowptr[row + 1];
+    int64_t e = S.rowptr[row] + lane;
+    for (; e + 192 < e1; e += 256) {
+        const int32_t c0 = S.col[e], c1 = S.col[e + 64], c2 = S.col[e + 128], c3 = S.col[e + 192];
+        const double v0 = UNIFORM ? (double)S.kval[e] : S.val[e];
+        const double v1 = UNIFORM ? (double)S.kval[e + 64] : S.val[e + 64];
+        const double v2 = UNIFORM ? (double)S.kval[e + 128] : S.val[e + 128];
+        const double v3 = UNIFORM ? (double)S.kval[e + 192] : S.val[e + 192];
+        const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+        a0 += v0 * x0;
+        a1 += v1 * x1;
+        a2 += v2 * x2;
+        a3 += v3 * x3;
+    }
+    for (; e < e1; e += 64) a0 += (UNIFORM ? (double)S.kval[e] : S.val[e]) * x[S.col[e]];
+    const double acc = wave_sum((a0 + a1) + (a2 + a3));
+    return UNIFORM ? S.area[row] * (acc * S.inv_2R + S.delta * x[row]) : acc;
+}
+
 // y = S x (one wavefront per row).
-__global__ __launch_bounds__(ROW_THREADS) void k_spmv(const int64_t *rowptr, const int32_t *col, const double *val,
-                                                      int64_t n, const double *x, double *y)
+template <bool UNIFORM>
+__global__ __launch_bounds__(ROW_THREADS) void k_spmv(SView S, int64_t n, const double *x, double *y)
 {
     const int64_t row = (int64_t)blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
-    double acc = 0.0;
-    for (int64_t e = rowptr[row] + lane; e < rowptr[row + 1]; e += 64) acc += val[e] * x[col[e]];
-    acc = wave_sum(acc);
-    if (lane == 0) y[row] = acc;
+    const double sx = s_row<UNIFORM>(S, row, lane, x);
+    if (lane == 0) y[row] = sx;
 }
 
 // Raw count rows: y_i = sum_j (c_ij / R_{who}) x_j, with who = i for F and
@@ -127,12 +163,11 @@ __global__ __launch_bounds__(RED_THREADS) void k_series_start(const double *part
 // One reflection: incident = d * (S w); absorbed += eps * incident;
 // flight = (1 - eps) * incident; w_next = d * (flight / A).  Per-block
 // partial sums of |flight| and flight for the stopping rule and residual.
-__global__ __launch_bounds__(ROW_THREADS) void k_series_step(const int64_t *rowptr, const int32_t *col,
-                                                             const double *val, int64_t n, const double *d,
-                                                             const double *area, const double *eps,
-                                                             const double *w, double *w_next, double *absorbed,
-                                                             double *flight, double *part_abs, double *part_sum,
-                                                             const SeriesState *S)
+template <bool UNIFORM>
+__global__ __launch_bounds__(ROW_THREADS) void k_series_step(SView M, int64_t n, const double *d, const double *area,
+                                                             const double *eps, const double *w, double *w_next,
+                                                             double *absorbed, double *flight, double *part_abs,
+                                                             double *part_sum, const SeriesState *S)
 {
     if (S->done) return;
     __shared__ double sa[ROWS_PER_BLOCK], ss[ROWS_PER_BLOCK];
@@ -141,10 +176,7 @@ __global__ __launch_bounds__(ROW_THREADS) void k_series_step(const int64_t *rowp
     const int lane = threadIdx.x & 63;
     double fl = 0.0;
     if (row < n) {
-        double acc = 0.0;
-        for (int64_t e = rowptr[row] + lane; e < rowptr[row + 1]; e += 64) acc += val[e] * w[col[e]];
-        acc = wave_sum(acc);
-        const double incident = d[row] * acc;
+        const double incident = d[row] * s_row<UNIFORM>(M, row, lane, w);
         const double ep = eps[row];
         fl = (1.0 - ep) * incident;
         if (lane == 0) {
@@ -197,6 +229,11 @@ __global__ __launch_bounds__(RED_THREADS) void k_series_check(const double *part
             S->residual = rs[0];
         }
     }
+}
+
+SView sview_of(const prh_operator *op)
+{
+    return SView{op->rowptr, op->col, op->val, op->kval, op->area, op->inv_2R, op->delta};
 }
 
 unsigned blocks_for(int64_t n, int per_block)
@@ -336,12 +373,14 @@ int apply_device(prh_operator *op, int mode, const double *dx, double *dy, hipSt
     const unsigned rb = blocks_for(n, ROWS_PER_BLOCK), eb = blocks_for(n, 256);
     switch (mode) {
     case PRH_APPLY_SYMMETRIC:
-        k_spmv<<<rb, ROW_THREADS, 0, st>>>(op->rowptr, op->col, op->val, n, dx, dy);
+        if (op->uniform) k_spmv<true><<<rb, ROW_THREADS, 0, st>>>(sview_of(op), n, dx, dy);
+        else k_spmv<false><<<rb, ROW_THREADS, 0, st>>>(sview_of(op), n, dx, dy);
         break;
     case PRH_APPLY_MATVEC:
     case PRH_APPLY_RMATVEC:
         k_scale_in<<<eb, 256, 0, st>>>(dx, op->scale, op->area, mode, n, op->vt);
-        k_spmv<<<rb, ROW_THREADS, 0, st>>>(op->rowptr, op->col, op->val, n, op->vt, dy);
+        if (op->uniform) k_spmv<true><<<rb, ROW_THREADS, 0, st>>>(sview_of(op), n, op->vt, dy);
+        else k_spmv<false><<<rb, ROW_THREADS, 0, st>>>(sview_of(op), n, op->vt, dy);
         k_scale_out<<<eb, 256, 0, st>>>(dy, op->scale, op->area, mode, n, dy);
         break;
     case PRH_APPLY_RAW:
@@ -384,9 +423,9 @@ int series_fp64(prh_operator *op, double relative_tolerance, int32_t max_reflect
     while (it < max_reflections) {
         const int batch = std::min<int32_t>(16, max_reflections - it);
         for (int b = 0; b < batch; ++b) {
-            k_series_step<<<rb, ROW_THREADS, 0, st>>>(op->rowptr, op->col, op->val, n, op->scale, op->area, op->eps,
-                                                      w_in, w_out, op->absorbed, op->flight, op->part_abs,
-                                                      op->part_sum, op->state);
+            auto step = op->uniform ? k_series_step<true> : k_series_step<false>;
+            step<<<rb, ROW_THREADS, 0, st>>>(sview_of(op), n, op->scale, op->area, op->eps, w_in, w_out,
+                                             op->absorbed, op->flight, op->part_abs, op->part_sum, op->state);
             k_series_check<<<1, RED_THREADS, 0, st>>>(op->part_abs, op->part_sum, n_part, relative_tolerance,
                                                       op->state);
             std::swap(w_in, w_out);

@@ -68,4 +68,12 @@ namespace prh {
 // PRH_ERR_SERIES when more than max_reflections are needed.
 int series_fp64(prh_operator *op, double relative_tolerance, int32_t max_reflections, hipStream_t st,
                 double *residual, int32_t *iterations);
+// The same contract by the batched engine's mixed-precision iteration for one
+// device-resident emission E (n doubles, read on stream st) with a single
+// emissivity: FP32 Chebyshev sweeps with FP64 refinement until
+// ||E - (I - (1 - eps) F_p^T) J||_1 <= rtol ||E||_1; eps F_p^T J is written to
+// `absorbed` (device).  The engine must not be driven from another thread
+// meanwhile.
+int batch_solve_device(prh_batch *b, const double *E, double emissivity, double rtol, hipStream_t st,
+                       double *absorbed, int32_t *iterations);
 }  // namespace prh

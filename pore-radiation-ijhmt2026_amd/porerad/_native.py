@@ -96,6 +96,7 @@ SIGNATURES = {
     "prh_coupled_linearize": ([_P, _I32, _F64, _F64], ctypes.c_int),
     "prh_coupled_jacobian": ([_P, _I32, _F64, _F64, _I32, _P], ctypes.c_int),
     "prh_coupled_set_preconditioner": ([_P, _I32, _F64, _I32], ctypes.c_int),
+    "prh_coupled_set_precision": ([_P, _I32], ctypes.c_int),
     "prh_coupled_precondition": ([_P, _I32, _I32], ctypes.c_int),
     "prh_coupled_orthogonalize": ([_P, _I32, _I32, _I32, _P], ctypes.c_int),
     "prh_coupled_combine": ([_P, _I32, _I32, _I32, _I32, _P], ctypes.c_int),
@@ -623,6 +624,14 @@ class Coupled:
         check(self._lib.prh_coupled_jacobian(self.handle, int(v), float(eps), float(rtol), int(out),
                                              ctypes.byref(its)), "prh_coupled_jacobian")
         return int(its.value)
+
+    def set_precision(self, precision: str) -> None:
+        """Radiosity solves of the Jacobian-vector products: "fp64" (series) or "mixed" (engine)."""
+
+        code = {"fp64": FP64, "mixed": MIXED}.get(precision)
+        if code is None:
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        check(self._lib.prh_coupled_set_precision(self.handle, code), "prh_coupled_set_precision")
 
     def set_preconditioner(self, T: int, eps: float, steps: int) -> None:
         check(self._lib.prh_coupled_set_preconditioner(self.handle, int(T), float(eps), int(steps)),

@@ -409,17 +409,24 @@ def newton_krylov(
     epsilon: float,
     T_initial: np.ndarray,
     preconditioner_steps: int,
+    *,
+    precision: str = "fp64",
 ) -> CoupledResult:
     """Newton-Krylov solve of A T - b - Q_rad(T) = 0 from the pure-conduction field (coupled.py:341-425).
 
     Converged when ||R||_inf <= 1e-13 W and the last update is <= 1e-10 K;
     radiosity to 1e-8 inside the iteration, 1e-13 for the returned state;
     trial fields outside [T_c, T_h] are rejected by the line search.
+    ``precision="mixed"`` solves the radiosity of the Jacobian-vector products
+    with FP32 sweeps and FP64 refinement (same 1e-8 tolerance); the Newton
+    residual's radiation stays the FP64 series, so the converged field solves
+    the same equations.
     """
 
     D = _native.Coupled(system.matrix, system.hot_rhs, system.cold_rhs, _N_SLOTS, patches.geometry(),
                         operator.device)
     try:
+        D.set_precision(precision)
         return _newton_krylov_device(D, system, patches, T_hot_K, T_cold_K, epsilon, T_initial,
                                      preconditioner_steps)
     finally:
@@ -517,11 +524,14 @@ def solve_coupled(
     T_cold_K: float,
     epsilon: float,
     T_initial: np.ndarray,
+    *,
+    precision: str = "fp64",
 ) -> CoupledResult:
     """Coupled solve with 10 preconditioner steps, restarted with 20 if needed (coupled.py:428-447)."""
 
     for steps in PCG_STEPS:
-        result = newton_krylov(system, operator, patches, T_hot_K, T_cold_K, epsilon, T_initial, steps)
+        result = newton_krylov(system, operator, patches, T_hot_K, T_cold_K, epsilon, T_initial, steps,
+                               precision=precision)
         if result.converged and result.linear_residuals_pass:
             break
     return result

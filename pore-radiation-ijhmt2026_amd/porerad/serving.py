@@ -73,6 +73,7 @@ class CoupledRequest:
     conductivity_gas: float = 0.026
     arrival_s: float = 0.0
     tag: object = None
+    precision: str = "fp64"  # radiosity of the Jacobian-vector products: "fp64" series or "mixed" engine
 
 
 @dataclasses.dataclass
@@ -173,7 +174,8 @@ def device_coupled_runner(geometry: Geometry, request: CoupledRequest) -> tuple[
         system = geometry.conduction[key] = conduction.assemble_conduction(k, P.H_m)
     Th, Tc = float(request.T_hot_K), float(request.T_cold_K)
     T0 = conduction.solve_conduction(system, Th, Tc)
-    result = coupled.solve_coupled(system, geometry.operator, P, Th, Tc, float(request.epsilon), T0)
+    result = coupled.solve_coupled(system, geometry.operator, P, Th, Tc, float(request.epsilon), T0,
+                                   precision=request.precision)
     metrics = coupled.wall_flux_metrics(system, P, result, Th, Tc)
     return bool(result.converged), int(result.newton_iterations), metrics, result.failure
 

@@ -126,3 +126,29 @@ def test_archived_coupled_rows(mods, case_id):
         assert abs(m[key] - row[key]) <= 1e-9 * scale, key
     assert abs(m["T_min_K"] - row["T_min_K"]) <= 1e-8
     assert abs(m["T_max_K"] - row["T_max_K"]) <= 1e-8
+
+
+@pytest.mark.parametrize("eps", [0.3, 0.6])
+def test_mixed_precision_jacobian_solves_the_same_equations(mods, eps):
+    """FP32-sweep Jacobian-vector products (FP64 refinement to 1e-8) leave the
+    Newton residual in FP64, so the field solves the same discrete equations:
+    every wall flux within 1e-8 of the total flux of the FP64 solve and the
+    temperatures within 1e-6 K."""
+
+    _, _, cd, cp, _ = mods
+    rng = np.random.default_rng(21)
+    phases = (rng.random((18, 14, 14)) < 0.15).astype(np.uint8)
+    H, Th, Tc = 0.004, 800.0, 300.0
+    P, op, system = _product(mods, phases, H, 96, [4, 5])
+    T0 = cd.solve_conduction(system, Th, Tc)
+    ref = cp.solve_coupled(system, op, P, Th, Tc, eps, T0)
+    mixed = cp.solve_coupled(system, op, P, Th, Tc, eps, T0, precision="mixed")
+    assert ref.converged and mixed.converged
+    assert abs(mixed.newton_iterations - ref.newton_iterations) <= 1
+    m_ref = cp.wall_flux_metrics(system, P, ref, Th, Tc)
+    m_mix = cp.wall_flux_metrics(system, P, mixed, Th, Tc)
+    scale = max(abs(m_ref["q_total_hot_W_m2"]), abs(m_ref["q_total_cold_W_m2"]))
+    for key in ("q_cond_hot_W_m2", "q_rad_hot_W_m2", "q_cond_cold_W_m2", "q_rad_cold_W_m2"):
+        assert abs(m_mix[key] - m_ref[key]) <= 1e-8 * scale, key
+    assert np.max(np.abs(mixed.temperature_K - ref.temperature_K)) <= 1e-6
+    assert not cp.acceptance_failures(mixed, m_mix)
