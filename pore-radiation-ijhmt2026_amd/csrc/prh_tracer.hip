@@ -219,6 +219,11 @@ enum OccMode : int {
                      // while the load is in flight and undone on a solid cell
 };
 
+// Flag added to an occupancy mode when ny and nz are powers of two: the
+// periodic wrap of the lateral cell indices is then one AND with n - 1.
+constexpr int OCC_POW2 = 4;
+constexpr int occ_mode(int occ) { return occ & 3; }
+
 // Index of the cell (i, jj, kk) in the flat bitset.  24-bit multiply-adds:
 // i ny + j and nz stay below 2^24 and the index below 2^31 (checked at
 // geometry creation).
@@ -236,7 +241,7 @@ __device__ __forceinline__ uint32_t occupancy_word(const GeoView &G, uint32_t ce
 template <int OCC>
 __device__ __forceinline__ bool cell_is_solid(const Ray &R, const GeoView &G, const uint32_t *lds_coarse)
 {
-    if (OCC == OCC_BRICK) {
+    if (occ_mode(OCC) == OCC_BRICK) {
         const unsigned s = (unsigned)G.coarse_shift;
         const uint32_t b = mad_u24(mad_u24((uint32_t)R.i >> s, (uint32_t)G.cny, (uint32_t)R.jj >> s),
                                    (uint32_t)G.cnz, (uint32_t)R.kk >> s);
@@ -251,7 +256,8 @@ __device__ __forceinline__ bool cell_is_solid(const Ray &R, const GeoView &G, co
 // of them in x > y > z order.  The minimum is the value of one of the three
 // times, so at least one axis crosses.  Updates are selects (the wave stays
 // converged); a lateral index leaving [0, n) is wrapped by subtracting
-// step * n.
+// step * n (POW2: by masking with n - 1).
+template <bool POW2>
 __device__ __forceinline__ void ray_advance(Ray &R, const GeoView &G)
 {
     const double nt = vmin_f64(vmin_f64(R.tx, R.ty), R.tz);
@@ -265,10 +271,12 @@ __device__ __forceinline__ void ray_advance(Ray &R, const GeoView &G)
     R.i += cx ? R.sx : 0;
     R.tx = fma_f64(cx ? 1.0 : 0.0, R.ddx, R.tx);
     const int jn = R.jj + (cy ? R.sy : 0);
-    R.jj = (unsigned)jn < (unsigned)G.ny ? jn : jn - __mul24(R.sy, G.ny);
+    if (POW2) R.jj = jn & (G.ny - 1);
+    else R.jj = (unsigned)jn < (unsigned)G.ny ? jn : jn - __mul24(R.sy, G.ny);
     R.ty = fma_f64(cy ? 1.0 : 0.0, R.ddy, R.ty);
     const int kn = R.kk + (cz ? R.sz : 0);
-    R.kk = (unsigned)kn < (unsigned)G.nz ? kn : kn - __mul24(R.sz, G.nz);
+    if (POW2) R.kk = kn & (G.nz - 1);
+    else R.kk = (unsigned)kn < (unsigned)G.nz ? kn : kn - __mul24(R.sz, G.nz);
     R.tz = fma_f64(cz ? 1.0 : 0.0, R.ddz, R.tz);
     R.axis = cx ? 0 : (cy ? 1 : 2);
     R.time = nt;
@@ -279,12 +287,13 @@ __device__ __forceinline__ void ray_advance(Ray &R, const GeoView &G)
 template <int OCC>
 __device__ __forceinline__ bool ray_step(Ray &R, const GeoView &G, const uint32_t *lds_coarse)
 {
+    constexpr bool POW2 = (OCC & OCC_POW2) != 0;
     if ((unsigned)R.i >= (unsigned)G.nx) return false;
-    if (OCC == OCC_AHEAD) {
+    if (occ_mode(OCC) == OCC_AHEAD) {
         const uint32_t cell = cell_index(R, G);
         const uint32_t word = occupancy_word(G, cell);
         const int i0 = R.i, j0 = R.jj, k0 = R.kk, axis0 = R.axis;
-        ray_advance(R, G);
+        ray_advance<POW2>(R, G);
         if ((word >> (cell & 31u)) & 1u) {
             // the current cell was solid: the ray ends here, as if it had not moved
             R.i = i0;
@@ -296,7 +305,7 @@ __device__ __forceinline__ bool ray_step(Ray &R, const GeoView &G, const uint32_
         return --R.left != 0u;
     }
     if (cell_is_solid<OCC>(R, G, lds_coarse)) return false;
-    ray_advance(R, G);
+    ray_advance<POW2>(R, G);
     return --R.left != 0u;
 }
 
@@ -612,7 +621,7 @@ void k_trace_stream(TraceArgs a)
 {
     const GeoView &G = a.G;
     extern __shared__ uint32_t lds_coarse[];
-    if (OCC == OCC_BRICK) {
+    if (occ_mode(OCC) == OCC_BRICK) {
         for (uint32_t w = threadIdx.x; w < G.coarse_words; w += STREAM_THREADS) lds_coarse[w] = G.coarse[w];
         __syncthreads();
     }
@@ -631,7 +640,7 @@ void k_trace_batch(const Segment *segs, int n_seg, const DdaRec *dda, const Wall
     const int k = segment_of(segs, n_seg, blockIdx.x, true);
     const GeoView G = segs[k].gen.G;
     extern __shared__ uint32_t lds_coarse[];
-    if (OCC == OCC_BRICK) {
+    if (occ_mode(OCC) == OCC_BRICK) {
         for (uint32_t w = threadIdx.x; w < G.coarse_words; w += STREAM_THREADS) lds_coarse[w] = G.coarse[w];
         __syncthreads();
     }
@@ -926,6 +935,29 @@ const TraceVariant TRACE_VARIANTS[] = {
     {k_trace_stream<32, OCC_AHEAD>, 32, OCC_AHEAD},
 };
 constexpr int N_TRACE_VARIANTS = sizeof(TRACE_VARIANTS) / sizeof(TRACE_VARIANTS[0]);
+// The same variants for geometries whose ny and nz are powers of two.
+const TraceVariant TRACE_VARIANTS_POW2[] = {
+    {k_trace_stream<32, OCC_BRICK | OCC_POW2>, 32, OCC_BRICK},
+    {k_trace_stream<16, OCC_BRICK | OCC_POW2>, 16, OCC_BRICK},
+    {k_trace_stream<24, OCC_BRICK | OCC_POW2>, 24, OCC_BRICK},
+    {k_trace_stream<40, OCC_BRICK | OCC_POW2>, 40, OCC_BRICK},
+    {k_trace_stream<48, OCC_BRICK | OCC_POW2>, 48, OCC_BRICK},
+    {k_trace_stream<32, OCC_GLOBAL | OCC_POW2>, 32, OCC_GLOBAL},
+    {k_trace_stream<16, OCC_GLOBAL | OCC_POW2>, 16, OCC_GLOBAL},
+    {k_trace_stream<24, OCC_GLOBAL | OCC_POW2>, 24, OCC_GLOBAL},
+    {k_trace_stream<16, OCC_AHEAD | OCC_POW2>, 16, OCC_AHEAD},
+    {k_trace_stream<24, OCC_AHEAD | OCC_POW2>, 24, OCC_AHEAD},
+    {k_trace_stream<32, OCC_AHEAD | OCC_POW2>, 32, OCC_AHEAD},
+};
+static_assert(sizeof(TRACE_VARIANTS_POW2) == sizeof(TRACE_VARIANTS), "one power-of-two kernel per variant");
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// The traversal kernel of a variant for geometry g.
+const TraceVariant &trace_variant_for(const Geometry &g, int variant)
+{
+    return is_pow2(g.ny) && is_pow2(g.nz) ? TRACE_VARIANTS_POW2[variant] : TRACE_VARIANTS[variant];
+}
 // Default: refill at 16 idle lanes with the LDS brick test, the fastest on
 // the foam and fibre media measured so far.
 int g_trace_variant = 1;
@@ -946,8 +978,25 @@ const BatchVariant BATCH_VARIANTS[] = {
 };
 static_assert(sizeof(BATCH_VARIANTS) / sizeof(BATCH_VARIANTS[0]) == N_TRACE_VARIANTS,
               "one batch kernel per traversal variant");
+const BatchVariant BATCH_VARIANTS_POW2[] = {
+    {k_trace_batch<32, OCC_BRICK | OCC_POW2>, 32, OCC_BRICK},
+    {k_trace_batch<16, OCC_BRICK | OCC_POW2>, 16, OCC_BRICK},
+    {k_trace_batch<24, OCC_BRICK | OCC_POW2>, 24, OCC_BRICK},
+    {k_trace_batch<40, OCC_BRICK | OCC_POW2>, 40, OCC_BRICK},
+    {k_trace_batch<48, OCC_BRICK | OCC_POW2>, 48, OCC_BRICK},
+    {k_trace_batch<32, OCC_GLOBAL | OCC_POW2>, 32, OCC_GLOBAL},
+    {k_trace_batch<16, OCC_GLOBAL | OCC_POW2>, 16, OCC_GLOBAL},
+    {k_trace_batch<24, OCC_GLOBAL | OCC_POW2>, 24, OCC_GLOBAL},
+    {k_trace_batch<16, OCC_AHEAD | OCC_POW2>, 16, OCC_AHEAD},
+    {k_trace_batch<24, OCC_AHEAD | OCC_POW2>, 24, OCC_AHEAD},
+    {k_trace_batch<32, OCC_AHEAD | OCC_POW2>, 32, OCC_AHEAD},
+};
 
-const BatchVariant &batch_variant_for(int variant) { return BATCH_VARIANTS[variant]; }
+// pow2: every geometry of the launch has power-of-two ny and nz.
+const BatchVariant &batch_variant_for(int variant, bool pow2 = false)
+{
+    return pow2 ? BATCH_VARIANTS_POW2[variant] : BATCH_VARIANTS[variant];
+}
 
 // Records per queue grab of the traversal (prh_tracer_set_grain).
 int g_trace_grain = 512;
@@ -994,7 +1043,7 @@ int launch_trace(const Geometry &g, const DdaRec *d_dda, const WallRec *d_walls,
                  unsigned long long *d_reads, unsigned long long *d_queue, hipStream_t st)
 {
     if (total == 0) return PRH_OK;
-    const TraceVariant &V = TRACE_VARIANTS[g_trace_variant];
+    const TraceVariant &V = trace_variant_for(g, g_trace_variant);
     const size_t lds = V.occ == OCC_BRICK ? (size_t)g.coarse_words * sizeof(uint32_t) : 0;
     int64_t waves = 0;
     int rc = resident_waves((const void *)V.fn, STREAM_THREADS, lds, &waves);
@@ -1323,6 +1372,7 @@ struct prh_trace_batch {
     std::vector<BatchPiece> pieces;
     std::vector<int> chunk_first;  // first piece of each chunk, then the piece count
     std::vector<int64_t> chunk_waves, chunk_blocks;
+    std::vector<char> chunk_pow2;  // every geometry of the chunk has power-of-two ny and nz
     int64_t ray_cap = 0, patch_cap = 0, scratch_elems = 0, edge_cap = 0, rays = 0;
     size_t lds = 0;
     int variant = 0;
@@ -1413,7 +1463,7 @@ int enqueue_clear(prh_trace_batch *b, hipStream_t st)
 // Chunk c, first half: generation, traversal, per-piece counting.
 int enqueue_front(prh_trace_batch *b, int c, hipStream_t st)
 {
-    const BatchVariant &V = batch_variant_for(b->variant);
+    const BatchVariant &V = batch_variant_for(b->variant, b->chunk_pow2[c] != 0);
     const BatchBuffers &B = b->buf[c % b->n_streams];
     const int j0 = b->chunk_first[c], j1 = b->chunk_first[c + 1];
     const unsigned gen_blocks = (unsigned)((b->chunk_waves[c] * 64 + TRACE_THREADS - 1) / TRACE_THREADS);
@@ -1658,6 +1708,12 @@ int prh_trace_batch_create(int32_t n_segments, const prh_geometry *const *geomet
         const BatchBuffers &B = b->buf[c % b->n_streams];
         const int j0 = b->chunk_first[c], j1 = b->chunk_first[c + 1];
         int64_t wave_at = 0, block_at = 0;
+        bool pow2 = true;
+        for (int j = j0; j < j1; ++j) {
+            const Geometry &gj = b->geos[b->pieces[j].seg]->g;
+            pow2 = pow2 && is_pow2(gj.ny) && is_pow2(gj.nz);
+        }
+        b->chunk_pow2.push_back(pow2 ? 1 : 0);
         for (int j = j0; j < j1; ++j) {
             const BatchPiece &P = b->pieces[j];
             const Geometry &g = b->geos[P.seg]->g;
