@@ -77,10 +77,15 @@ def run_tracer(P, R, seed, steps, warmup):
 
 
 def build_operator(P, R, seed):
+    """Counts traced and the operator assembled on the device (no edge leaves HBM)."""
+
     from porerad import exchange, view_factors
 
-    o, r, c = view_factors.first_hit_counts_pooled(P, R, [seed])
-    return exchange.ExchangeOperator(o, r, c, np.full(P.n_patches, R, np.int64), P.patch_area_m2)
+    edges = view_factors.first_hit_edges(P, R, [seed])
+    try:
+        return exchange.ExchangeOperator.from_edges(edges, np.full(P.n_patches, R, np.int64), P.patch_area_m2)
+    finally:
+        edges.close()
 
 
 def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
@@ -339,18 +344,19 @@ def run_batching(args) -> dict:
         batch.close()
         return record
 
-    # 64 right-hand sides per geometry: (T_hot, emissivity) pairs through the served engines.  The
-    # edge tables go to host memory first so that the batch's device memory is free for the operators.
-    tables = [batch.edges(k) for k in range(len(geos))]
-    batch.close()
+    # 64 right-hand sides per geometry: (T_hot, emissivity) pairs through the served engines.  Each
+    # segment's edges move to a device table of their own (the batch's record buffers are then
+    # freed) and each operator is assembled on the device from them.
     t = time.perf_counter()
+    tables = [batch.edge_table(k) for k in range(len(geos))]
+    batch.close()
     srv = serving.RadiosityServer()
     for k, (name, P) in enumerate(geos):
-        o, r, c = tables[k]
-        tables[k] = None
-        srv.add_geometry(name, P, exchange.ExchangeOperator(o, r, c, np.full(P.n_patches, R, np.int64),
-                                                            P.patch_area_m2))
-        del o, r, c
+        try:
+            srv.add_geometry(name, P, exchange.ExchangeOperator.from_edges(
+                tables[k], np.full(P.n_patches, R, np.int64), P.patch_area_m2))
+        finally:
+            tables[k].close()
         log(f"operator {name}: {time.perf_counter() - t:.1f} s")
     setup = time.perf_counter() - t
     rng = np.random.default_rng(12)

@@ -167,6 +167,9 @@ int prh_trace_batch_run(prh_trace_batch *b, const uint64_t *state_inc, int64_t *
 /* Copy segment k's edges of the last run to host arrays of nnz[k] entries. */
 int prh_trace_batch_edges(const prh_trace_batch *b, int32_t segment, int32_t *origin, int32_t *receiver,
                           int64_t *count);
+/* Segment k's edges of the last run as a device edge table of their own
+ * (for prh_operator_create_device); free it with prh_edges_destroy. */
+int prh_trace_batch_edges_device(const prh_trace_batch *b, int32_t segment, prh_edges **out);
 /* info[0..n): device bytes, chunks, pieces, ray capacity, edge capacity, rays,
  * streams, graph (1) or direct (0), traversal variant. */
 int prh_trace_batch_info(const prh_trace_batch *b, int64_t *info, int32_t n);
@@ -184,6 +187,18 @@ void prh_edges_destroy(prh_edges *e);
 int prh_operator_create(int64_t n, const int32_t *origin, const int32_t *receiver, const int64_t *count,
                         int64_t nnz, const int64_t *rays_per_origin, const double *area,
                         double support_diagonal_fraction, prh_operator **out);
+/* The same operator as prh_operator_create, assembled on the device from a
+ * device edge table (prh_trace_counts' output, sorted by (origin, receiver),
+ * unique pairs): no edge leaves the device.  Row totals are checked against
+ * rays_per_origin (PRH_ERR_COUNTS, exchange.py:68-71); n, rays_per_origin and
+ * area (host arrays of n) as in prh_operator_create.  The FP64 values of S
+ * are bitwise those of prh_operator_create.  Replaces the assembly in
+ * ExchangeOperator.__init__, porerad/exchange.py:59-76. */
+int prh_operator_create_device(const prh_edges *edges, int64_t n, const int64_t *rays_per_origin, const double *area,
+                               double support_diagonal_fraction, prh_operator **out);
+/* The operator's count table (origin, receiver, count), sorted, nnz_raw
+ * entries (prh_operator_nnz): the edges it was built from. */
+int prh_operator_edges(const prh_operator *op, int32_t *origin, int32_t *receiver, int64_t *count);
 /* Diagonal balancing factor d (n), used by modes MATVEC/RMATVEC and by
  * prh_radiosity.  Defaults to ones. */
 int prh_operator_set_scale(prh_operator *op, const double *scale);

@@ -29,6 +29,7 @@ SOURCES = [
     "prh_geometry.hip",
     "prh_tracer.hip",
     "prh_operator.hip",
+    "prh_assemble.hip",
     "prh_batch.hip",
     "prh_coupled.hip",
 ]

@@ -12,6 +12,16 @@
 
 #include "porerad_hip.h"
 
+// A device-resident edge table (origin, receiver, count), sorted by
+// (origin, receiver) with unique pairs: the output of the tracer's counting.
+struct prh_edges {
+    int device;
+    int64_t nnz;
+    int32_t *origin;
+    int32_t *receiver;
+    int32_t *count;
+};
+
 namespace prh {
 
 // ------------------------------------------------------------------ errors

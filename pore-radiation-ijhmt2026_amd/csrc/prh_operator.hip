@@ -401,6 +401,27 @@ int apply_device(prh_operator *op, int mode, const double *dx, double *dy, hipSt
 
 namespace prh {
 
+int operator_alloc_work(prh_operator *op)
+{
+    const int64_t n = op->n;
+    double **work[] = {&op->vx, &op->vy, &op->vt, &op->flight, &op->absorbed, &op->w0, &op->w1, &op->eps, &op->emit};
+    for (double **p : work)
+        if (hipMalloc((void **)p, sizeof(double) * n) != hipSuccess) {
+            set_error("prh_operator_create: out of device memory");
+            return PRH_ERR_HIP;
+        }
+    const int64_t n_part = blocks_for(n, ROWS_PER_BLOCK);
+    if (hipMalloc((void **)&op->part_abs, sizeof(double) * n_part) != hipSuccess ||
+        hipMalloc((void **)&op->part_sum, sizeof(double) * n_part) != hipSuccess ||
+        hipMalloc((void **)&op->state, sizeof(SeriesState)) != hipSuccess) {
+        set_error("prh_operator_create: out of device memory");
+        return PRH_ERR_HIP;
+    }
+    return PRH_OK;
+}
+
+void operator_free(prh_operator *op) { free_operator(op); }
+
 int series_fp64(prh_operator *op, double relative_tolerance, int32_t max_reflections, hipStream_t st,
                 double *residual, int32_t *iterations)
 {
@@ -549,19 +570,7 @@ int prh_operator_create(int64_t n, const int32_t *origin, const int32_t *receive
         if ((rc = upload(&op->scale32, ones32.data(), n, st))) return fail(rc);
         if (hipStreamSynchronize(st) != hipSuccess) return fail(PRH_ERR_HIP);
     }
-    double **work[] = {&op->vx, &op->vy, &op->vt, &op->flight, &op->absorbed, &op->w0, &op->w1, &op->eps, &op->emit};
-    for (double **p : work)
-        if (hipMalloc((void **)p, sizeof(double) * n) != hipSuccess) {
-            set_error("prh_operator_create: out of device memory");
-            return fail(PRH_ERR_HIP);
-        }
-    const int64_t n_part = blocks_for(n, ROWS_PER_BLOCK);
-    if (hipMalloc((void **)&op->part_abs, sizeof(double) * n_part) != hipSuccess ||
-        hipMalloc((void **)&op->part_sum, sizeof(double) * n_part) != hipSuccess ||
-        hipMalloc((void **)&op->state, sizeof(SeriesState)) != hipSuccess) {
-        set_error("prh_operator_create: out of device memory");
-        return fail(PRH_ERR_HIP);
-    }
+    if ((rc = operator_alloc_work(op))) return fail(rc);
     if (hipStreamSynchronize(st) != hipSuccess) {
         set_error("prh_operator_create: upload failed");
         return fail(PRH_ERR_HIP);

@@ -68,6 +68,10 @@ namespace prh {
 // PRH_ERR_SERIES when more than max_reflections are needed.
 int series_fp64(prh_operator *op, double relative_tolerance, int32_t max_reflections, hipStream_t st,
                 double *residual, int32_t *iterations);
+// Work vectors and series state of an operator whose matrices are in place.
+int operator_alloc_work(prh_operator *op);
+// Every device buffer of an operator (not the struct itself).
+void operator_free(prh_operator *op);
 // The same contract by the batched engine's mixed-precision iteration for one
 // device-resident emission E (n doubles, read on stream st) with a single
 // emissivity: FP32 Chebyshev sweeps with FP64 refinement until

@@ -1060,13 +1060,6 @@ int launch_trace(const Geometry &g, const DdaRec *d_dda, const WallRec *d_walls,
 
 }  // namespace prh
 
-struct prh_edges {
-    int device;
-    int64_t nnz;
-    int32_t *origin;
-    int32_t *receiver;
-    int32_t *count;
-};
 
 using namespace prh;
 
@@ -1818,6 +1811,41 @@ int prh_trace_batch_edges(const prh_trace_batch *b, int32_t segment, int32_t *or
         }
         at += m;
     }
+    return PRH_OK;
+}
+
+int prh_trace_batch_edges_device(const prh_trace_batch *b, int32_t segment, prh_edges **out)
+{
+    PRH_CHECK_ARG(b && out && segment >= 0 && segment < b->n_seg, "prh_trace_batch_edges_device: bad argument");
+    PRH_CHECK_ARG(b->edges <= b->edge_cap, "prh_trace_batch_edges_device: the last run overflowed the edge arena");
+    *out = nullptr;
+    PRH_HIP(hipSetDevice(b->device));
+    const int n_pieces = (int)b->pieces.size();
+    const int64_t nnz = b->seg_nnz[segment];
+    const int64_t alloc = nnz > 0 ? nnz : 1;
+    int32_t *o = nullptr, *r = nullptr, *c = nullptr;
+    PRH_HIP(hipMalloc((void **)&o, alloc * sizeof(int32_t)));
+    if (hipMalloc((void **)&r, alloc * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void **)&c, alloc * sizeof(int32_t)) != hipSuccess) {
+        (void)hipFree(o);
+        if (r) (void)hipFree(r);
+        set_error("prh_trace_batch_edges_device: out of device memory");
+        return PRH_ERR_HIP;
+    }
+    hipStream_t st = b->buf[0].stream;
+    int64_t at = 0;
+    for (int j = 0; j < n_pieces; ++j) {
+        if (b->pieces[j].seg != segment) continue;
+        const int64_t first = b->piece_start[j + 1];
+        const int64_t m = (j + 1 < n_pieces ? b->piece_start[j + 2] : b->edges) - first;
+        if (m <= 0) continue;
+        PRH_HIP(hipMemcpyAsync(o + at, b->e_origin + first, m * 4, hipMemcpyDeviceToDevice, st));
+        PRH_HIP(hipMemcpyAsync(r + at, b->e_receiver + first, m * 4, hipMemcpyDeviceToDevice, st));
+        PRH_HIP(hipMemcpyAsync(c + at, b->e_count + first, m * 4, hipMemcpyDeviceToDevice, st));
+        at += m;
+    }
+    PRH_HIP(hipStreamSynchronize(st));
+    *out = new prh_edges{b->device, nnz, o, r, c};
     return PRH_OK;
 }
 

@@ -59,11 +59,14 @@ SIGNATURES = {
     "prh_trace_batch_create": ([_I32, _P, _P, _P, _P, _P, _I64, _I64, _I32, _I32, _P], ctypes.c_int),
     "prh_trace_batch_run": ([_P, _P, _P, _P, _P], ctypes.c_int),
     "prh_trace_batch_edges": ([_P, _I32, _P, _P, _P], ctypes.c_int),
+    "prh_trace_batch_edges_device": ([_P, _I32, _P], ctypes.c_int),
     "prh_trace_batch_info": ([_P, _P, _I32], ctypes.c_int),
     "prh_trace_batch_destroy": ([_P], None),
     "prh_edges_destroy": ([_P], None),
     "prh_operator_create": ([_I64, _P, _P, _P, _I64, _P, _P, _F64, _P], ctypes.c_int),
     "prh_operator_set_scale": ([_P, _P], ctypes.c_int),
+    "prh_operator_create_device": ([_P, _I64, _P, _P, _F64, _P], ctypes.c_int),
+    "prh_operator_edges": ([_P, _P, _P, _P], ctypes.c_int),
     "prh_operator_nnz": ([_P, _P, _P], ctypes.c_int),
     "prh_operator_stats": ([_P, _P, _I32], ctypes.c_int),
     "prh_batch_stats": ([_P, _P, _I32], ctypes.c_int),
@@ -362,6 +365,14 @@ class TraceBatch:
               "prh_trace_batch_edges")
         return origin, receiver, count
 
+    def edge_table(self, segment: int) -> "Edges":
+        """A segment's edges of the last run as a device edge table (no host copy)."""
+
+        handle = ctypes.c_void_p()
+        check(self._lib.prh_trace_batch_edges_device(self.handle, int(segment), ctypes.byref(handle)),
+              "prh_trace_batch_edges_device")
+        return Edges(handle, int(self.nnz[segment]), int(self.occupancy_reads[segment]))
+
     def close(self) -> None:
         if getattr(self, "handle", None) is not None and self.handle.value:
             self._lib.prh_trace_batch_destroy(self.handle)
@@ -376,6 +387,34 @@ class TraceBatch:
 
 class Operator:
     """Owner of a device exchange operator (``prh_operator``)."""
+
+    @classmethod
+    def from_edges(cls, n: int, edges: "Edges", rays, area, delta: float) -> "Operator":
+        """Assemble on the device from a device edge table (``prh_operator_create_device``)."""
+
+        self = cls.__new__(cls)
+        self._lib = lib()
+        self.n = int(n)
+        rays = np.ascontiguousarray(rays, dtype=np.int64)
+        area = np.ascontiguousarray(area, dtype=np.float64)
+        if rays.shape != (self.n,) or area.shape != (self.n,):
+            raise ValueError(f"expected {self.n} ray counts and areas")
+        handle = ctypes.c_void_p()
+        check(self._lib.prh_operator_create_device(edges.handle, self.n, ptr(rays), ptr(area), float(delta),
+                                                   ctypes.byref(handle)), "prh_operator_create_device")
+        self.handle = handle
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        check(self._lib.prh_operator_nnz(handle, ctypes.byref(a), ctypes.byref(b)), "prh_operator_nnz")
+        self.nnz_symmetric, self.nnz_raw = int(a.value), int(b.value)
+        return self
+
+    def edges(self):
+        """(origin, receiver, count) the operator was built from."""
+
+        m = self.nnz_raw
+        origin, receiver, count = np.empty(m, np.int32), np.empty(m, np.int32), np.empty(m, np.int64)
+        check(self._lib.prh_operator_edges(self.handle, ptr(origin), ptr(receiver), ptr(count)), "prh_operator_edges")
+        return origin, receiver, count
 
     def __init__(self, n: int, origin, receiver, count, rays, area, delta: float) -> None:
         self._lib = lib()

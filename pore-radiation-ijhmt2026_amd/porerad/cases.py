@@ -6,8 +6,9 @@ regenerated from the stored seeds, the same operators, solves, acceptance
 checks and output columns, so their rows can be compared with
 results/prescribed_field.csv and results/coupled.csv column for column.  The
 ray blocks of a set are traced and pooled in one device pass
-(``first_hit_counts_pooled``, equal to ``pool_counts`` over per-seed
-``first_hit_counts``).
+(``first_hit_edges``, equal to ``pool_counts`` over per-seed
+``first_hit_counts``) and the operator is assembled from them on the device
+(``ExchangeOperator.from_edges``, the same operator bit for bit).
 
 Command line, with the reference's case tables and microstructure files::
 
@@ -71,10 +72,14 @@ def _check_phases(phases_x: np.ndarray, expected_sha256: str) -> None:
 def build_operator(patches, rays_per_patch: int, seeds, log: Log = _quiet) -> exchange.ExchangeOperator:
     """Projected operator of the pooled ray blocks of ``seeds`` (run_prescribed_field.py:78-92)."""
 
-    origin, receiver, count = view_factors.first_hit_counts_pooled(patches, rays_per_patch, seeds)
+    edges = view_factors.first_hit_edges(patches, rays_per_patch, seeds)
     rays = np.full(patches.n_patches, rays_per_patch * len(seeds), dtype=np.int64)
-    operator = exchange.ExchangeOperator(origin, receiver, count, rays, patches.patch_area_m2)
-    log(f"operator of {len(seeds)} blocks: {count.size} sampled patch pairs, balancing {operator.balance_method} "
+    try:
+        operator = exchange.ExchangeOperator.from_edges(edges, rays, patches.patch_area_m2)
+        n_pairs = edges.nnz
+    finally:
+        edges.close()
+    log(f"operator of {len(seeds)} blocks: {n_pairs} sampled patch pairs, balancing {operator.balance_method} "
         f"({operator.balance_iterations} iterations), largest row-sum deviation {operator.row_sum_error:.2e}")
     return operator
 

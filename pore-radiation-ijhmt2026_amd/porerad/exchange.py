@@ -74,24 +74,85 @@ class ExchangeOperator:
     ) -> None:
         rays = np.asarray(rays_per_origin, dtype=np.int64)
         n = int(rays.size)
-        self.origin, self.receiver, merged = _merge_edges(origin, receiver, count, n)
-        observed = np.bincount(self.origin, weights=merged, minlength=n) if merged.size else np.zeros(n)
+        self._origin, self._receiver, merged = _merge_edges(origin, receiver, count, n)
+        observed = np.bincount(self._origin, weights=merged, minlength=n) if merged.size else np.zeros(n)
         if observed.shape[0] != n or not np.array_equal(observed.astype(np.int64), rays):
             raise ValueError("first-hit counts do not account for every emitted ray")
-        self.count = merged
+        self._count = merged
+        self._weight = None
+        self._setup(rays, patch_area_m2, support_diagonal_fraction, event_chunk_size)
+        self.device = _native.Operator(n, self._origin, self._receiver, merged, rays, self.patch_area_m2,
+                                       self.support_diagonal_fraction)
+        self._finish(balance_tolerance)
+
+    @classmethod
+    def from_edges(
+        cls,
+        edges: _native.Edges,
+        rays_per_origin: np.ndarray,
+        patch_area_m2: np.ndarray,
+        *,
+        support_diagonal_fraction: float = SUPPORT_DIAGONAL_FRACTION,
+        balance_tolerance: float = 1.0e-11,
+        event_chunk_size: int = 4_000_000,
+    ) -> "ExchangeOperator":
+        """The same operator from a device-resident edge table (``view_factors.first_hit_edges``).
+
+        The assembly runs on the device (``prh_operator_create_device``): the
+        counts never reach the host.  ``origin``, ``receiver``, ``count`` and
+        ``weight`` are copied back only when first read.
+        """
+
+        self = cls.__new__(cls)
+        rays = np.asarray(rays_per_origin, dtype=np.int64)
+        self._origin = self._receiver = self._count = self._weight = None
+        self._setup(rays, patch_area_m2, support_diagonal_fraction, event_chunk_size)
+        self.device = _native.Operator.from_edges(self.n_patches, edges, rays, self.patch_area_m2,
+                                                  self.support_diagonal_fraction)
+        self._finish(balance_tolerance)
+        return self
+
+    def _setup(self, rays, patch_area_m2, support_diagonal_fraction, event_chunk_size) -> None:
         self.rays_per_origin = rays
-        self.weight = merged.astype(np.float64) / rays[self.origin]
         self.patch_area_m2 = np.asarray(patch_area_m2, dtype=np.float64).copy()
         self.support_diagonal_fraction = float(support_diagonal_fraction)
         self.event_chunk_size = int(event_chunk_size)
-        self.n_patches = n
-        self.device = _native.Operator(n, self.origin, self.receiver, merged, rays, self.patch_area_m2,
-                                       self.support_diagonal_fraction)
+        self.n_patches = int(rays.size)
+
+    def _finish(self, balance_tolerance: float) -> None:
         self.scale, self.balance_method, self.balance_iterations = self._balance(float(balance_tolerance))
         self.device.set_scale(self.scale)
-        self.row_sum_error = float(np.max(np.abs(self.matvec(np.ones(n)) - 1.0)))
+        self.row_sum_error = float(np.max(np.abs(self.matvec(np.ones(self.n_patches)) - 1.0)))
         if self.row_sum_error > CLOSURE_TOLERANCE:
             raise RuntimeError(f"projected rows do not close: {self.row_sum_error:.3e}")
+
+    # -- the count table (host copies made on first use) ----------------------------
+    def _edges(self) -> None:
+        if self._origin is None:
+            self._origin, self._receiver, self._count = self.device.edges()
+
+    @property
+    def origin(self) -> np.ndarray:
+        self._edges()
+        return self._origin
+
+    @property
+    def receiver(self) -> np.ndarray:
+        self._edges()
+        return self._receiver
+
+    @property
+    def count(self) -> np.ndarray:
+        self._edges()
+        return self._count
+
+    @property
+    def weight(self) -> np.ndarray:
+        """count / R[origin] (exchange.py:73)."""
+
+        if self._weight is None:
+            self._weight = self.count.astype(np.float64) / self.rays_per_origin[self.origin]
+        return self._weight
 
     # -- products (all on the GPU) -------------------------------------------------
     def raw_matvec(self, x: np.ndarray) -> np.ndarray:

@@ -136,3 +136,53 @@ def test_error_paths(mods, small_pair):
     E = np.ones(op.n_patches)
     with pytest.raises(RuntimeError, match="did not converge"):
         ex.reflection_series(op, E, np.full(op.n_patches, 0.3), relative_tolerance=1e-13, max_reflections=1)
+
+
+@pytest.mark.parametrize("uniform", [True, False])
+def test_device_assembly_equals_host_assembly(mods, uniform):
+    """prh_operator_create_device from the tracer's device edges gives the host
+    build's operator bit for bit: count table, balancing, every product and
+    the reflection series."""
+
+    vf, ex, _ = mods
+    rng = np.random.default_rng(7)
+    phases = (rng.random((16, 13, 11)) < 0.18).astype(np.uint8)
+    P = vf.build_surface_patches(phases, 0.01)
+    R = 48
+    area = P.patch_area_m2.copy()
+    if not uniform:
+        area *= 1.0 + 0.25 * rng.random(P.n_patches)  # areas that differ: the FP64 / FP32 forms
+    rays = np.full(P.n_patches, 2 * R, np.int64)
+    o, r, c = vf.first_hit_counts_pooled(P, R, [3, 4])
+    host = ex.ExchangeOperator(o, r, c, rays, area)
+    edges = vf.first_hit_edges(P, R, [3, 4])
+    try:
+        dev = ex.ExchangeOperator.from_edges(edges, rays, area)
+    finally:
+        edges.close()
+    assert dev.device.nnz_symmetric == host.device.nnz_symmetric and dev.device.nnz_raw == host.device.nnz_raw
+    for a, b in zip((dev.origin, dev.receiver, dev.count), (host.origin, host.receiver, host.count)):
+        assert np.array_equal(a, b)
+    assert np.array_equal(dev.weight, host.weight)
+    assert dev.balance_iterations == host.balance_iterations and np.array_equal(dev.scale, host.scale)
+    x = rng.random(P.n_patches)
+    for name in ("raw_matvec", "raw_rmatvec", "symmetric_matvec", "matvec", "rmatvec"):
+        assert np.array_equal(getattr(dev, name)(x), getattr(host, name)(x)), name
+    E = rng.random(P.n_patches)
+    e = np.full(P.n_patches, 0.6)
+    a = ex.reflection_series(dev, E, e, relative_tolerance=1e-13)
+    b = ex.reflection_series(host, E, e, relative_tolerance=1e-13)
+    assert a.iterations == b.iterations and np.array_equal(a.absorbed_power_W, b.absorbed_power_W)
+
+
+def test_device_assembly_checks_row_totals(mods):
+    vf, ex, _ = mods
+    phases = np.zeros((6, 5, 5), np.uint8)
+    phases[2:4, 1:3, 1:3] = 1
+    P = vf.build_surface_patches(phases, 0.005)
+    edges = vf.first_hit_edges(P, 20, [1])
+    try:
+        with pytest.raises(ValueError, match="first-hit counts do not account for every emitted ray"):
+            ex.ExchangeOperator.from_edges(edges, np.full(P.n_patches, 21, np.int64), P.patch_area_m2)
+    finally:
+        edges.close()

@@ -200,3 +200,27 @@ def test_every_traversal_variant_in_batch_mode(env, media):
                 batch.close()
     finally:
         native.set_tracer_variant(default)
+
+
+def test_segment_edge_table_builds_the_same_operator(env, media):
+    native, vf = env
+    from porerad import exchange
+
+    P = media[0]
+    batch = native.TraceBatch([(media[1].geometry(), 0, 50, 20, 1), (P.geometry(), 0, P.n_patches, 40, 2)],
+                              ray_capacity=1 << 15)
+    try:
+        batch.run([[9], [5, 6]])
+        table = batch.edge_table(1)
+        try:
+            rays = np.full(P.n_patches, 80, np.int64)
+            dev = exchange.ExchangeOperator.from_edges(table, rays, P.patch_area_m2)
+        finally:
+            table.close()
+    finally:
+        batch.close()
+    o, r, c = vf.first_hit_counts_pooled(P, 40, [5, 6])
+    host = exchange.ExchangeOperator(o, r, c, rays, P.patch_area_m2)
+    assert np.array_equal(dev.scale, host.scale)
+    x = np.random.default_rng(1).random(P.n_patches)
+    assert np.array_equal(dev.rmatvec(x), host.rmatvec(x))
