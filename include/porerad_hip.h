@@ -86,7 +86,7 @@ int prh_last_timings(double *ms, int32_t n);
 int prh_tracer_set_variant(int32_t variant);
 int prh_tracer_variants(int32_t *n_variants, int32_t *current, int32_t *refill_min, int32_t *occupancy_mode);
 /* Tuning knob of the traversal kernel: records a wavefront takes from the
- * launch's shared work queue at a time (64 .. 2^20, default 2048).  No
+ * launch's shared work queue at a time (64 .. 2^20, default 512).  No
  * counterpart in the reference; results do not depend on it. */
 int prh_tracer_set_grain(int32_t records);
 /* Free the current device's tracer workspace (ray records of the largest

@@ -178,19 +178,26 @@ def test_empty_box_counts(vf):
     assert np.all(P.patch_class[got[0]] != P.patch_class[got[1]])
 
 
-def test_every_traversal_variant_gives_identical_counts(vf):
+@pytest.mark.parametrize("shape", [(24, 20, 20), (20, 16, 32)], ids=["lateral-20x20", "lateral-pow2"])
+def test_every_traversal_variant_gives_identical_counts(vf, shape):
+    """Counts and the occupancy-read total of every variant against the oracle,
+    on lateral sizes that are and are not powers of two (separate kernels)."""
+
     from porerad import _native
 
-    phases = _small_random(6, (24, 20, 20), 0.12)
-    P, Q = _both(vf, ("small6", "x"), phases)
-    ref = O.first_hit_counts(Q, 300, 77)
+    phases = _small_random(6, shape, 0.12)
+    P, Q = _both(vf, ("small6", shape), phases)
+    ref, reads_ref = O.first_hit_counts_range(Q, 300, 77, 0, Q.n_patches)
     variants = _native.tracer_variants()
     assert len(variants) >= 2
     default = next(v["variant"] for v in variants if v["current"])
     try:
         for v in variants:
             _native.set_tracer_variant(v["variant"])
-            got = vf.first_hit_counts(P, 300, 77)
+            edges = vf.first_hit_edges(P, 300, [77])
+            got, reads = edges.to_host(), edges.occupancy_reads
+            edges.close()
             assert all(np.array_equal(a, b) for a, b in zip(got, ref)), v
+            assert reads == reads_ref, v
     finally:
         _native.set_tracer_variant(default)
