@@ -18,6 +18,7 @@ trace kernel and the CPU oracle timed on this host.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -92,7 +93,7 @@ def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
     """Plain FP64 series vs served mixed-precision solves (Poisson arrivals)."""
 
     from porerad import coupled, exchange, serving, synthetic, view_factors
-    from porerad.constants import K_GAS, K_SOLID
+    from porerad.constants import K_GAS
 
     t0 = time.perf_counter()
     geoms = {"foam128": (foam_P, build_operator(foam_P, R, seed))}
@@ -166,10 +167,12 @@ def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
                  "max_rel_error_per_flux": flux_errors(srv.run(prec, real_time=False))}
 
     # serving plus coupling: radiosity requests at the same rate with coupled solves of the foam
-    # arriving among them (conductivities of the reference's constants, 0.20 and 0.026 W/m/K)
+    # arriving among them (solid 1.0 and gas 0.026 W/m/K; mixed-precision Jacobian solves)
     mixed_reqs = make_requests(n_requests // 2, 9)
-    coupled_reqs = [serving.CoupledRequest("foam128", Th, 300.0, eps, K_SOLID, K_GAS, arrival_s=a)
-                    for (Th, eps), a in zip(((700.0, 0.6), (900.0, 0.9)), (0.05, 0.5))]
+    cases = ((500.0, 0.3), (700.0, 0.6), (900.0, 0.9))
+    coupled_reqs = [serving.CoupledRequest("foam128", Th, 300.0, eps, C5_K_SOLID, K_GAS, arrival_s=a,
+                                           precision="mixed", preconditioner_steps=C5_PCG_STEPS)
+                    for (Th, eps), a in zip(cases, (0.05, 0.3, 0.6))]
     t = time.perf_counter()
     mixed = srv.run(mixed_reqs + coupled_reqs, real_time=True)
     rad_part = [r for r in mixed if isinstance(r, serving.RadiosityReply)]
@@ -182,9 +185,29 @@ def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
                     "converged": all(r.converged for r in cpl_part),
                     "q_total_hot_W_m2": [r.metrics["q_total_hot_W_m2"] for r in cpl_part],
                     "wall_flux_mismatch_rel": max(r.metrics["wall_flux_mismatch_rel"] for r in cpl_part)},
-        "workload": f"{n_requests // 2} radiosity requests at {rate:.0f}/s + 2 coupled solves of the foam "
-                    f"(k_solid {K_SOLID}, k_gas {K_GAS} W/m/K, FP64 JVP series)",
+        "workload": f"{n_requests // 2} radiosity requests at {rate:.0f}/s + {len(cases)} coupled solves of the "
+                    f"foam (k_solid {C5_K_SOLID}, k_gas {K_GAS} W/m/K, Jacobi-PCG {C5_PCG_STEPS[0]} steps, "
+                    f"mixed-precision Jacobian solves)",
     }
+    # each coupled request alone, with FP64 and with mixed-precision Jacobian solves: latency and
+    # the error of the mixed path per wall flux
+    alone = {}
+    for prec in ("fp64", "mixed"):
+        alone[prec] = {}
+        for r in coupled_reqs:
+            rep, = srv.run([dataclasses.replace(r, arrival_s=0.0, precision=prec)], real_time=False)
+            alone[prec][(r.T_hot_K, r.epsilon)] = rep
+    flux_keys = ("q_cond_hot_W_m2", "q_rad_hot_W_m2", "q_cond_cold_W_m2", "q_rad_cold_W_m2")
+    cpl_err = {}
+    for key, ref in alone["fp64"].items():
+        got = alone["mixed"][key]
+        scale = max(abs(ref.metrics["q_total_hot_W_m2"]), abs(ref.metrics["q_total_cold_W_m2"]))
+        cpl_err[f"T_hot {key[0]:.0f} K, eps {key[1]}"] = {k: abs(got.metrics[k] - ref.metrics[k]) / scale
+                                                          for k in flux_keys}
+    coupled_block["coupled"]["alone_latency_s"] = {
+        prec: [round(rep.latency_s, 3) for rep in alone[prec].values()] for prec in alone}
+    coupled_block["coupled"]["alone_converged"] = all(rep.converged for d in alone.values() for rep in d.values())
+    coupled_block["coupled"]["max_rel_flux_error_vs_fp64"] = cpl_err
     srv.close()
     err = flux_errors(replies + sat)
     roofline = {"engine": engine_roofline(ticks0, ticks1), "fp64_series": series_roofline(series0, series1)}
@@ -202,6 +225,10 @@ def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
 
 
 BATCH_SIZES = (96, 128, 160, 192)
+C5_K_SOLID = 1.0  # W/m/K: solid conductivity of the served coupled requests
+# Jacobi-PCG steps of their Newton-Krylov solves: the default 10/20 leave FGMRES(30) short of its
+# tolerance within 90 iterations at this conductivity ratio (k_s / k_g = 38)
+C5_PCG_STEPS = (80,)
 
 
 def batching_workload(sizes=BATCH_SIZES):
@@ -556,7 +583,9 @@ def main() -> None:
         log(f"precision at 1e-10: {rad['precision']['max_rel_error_per_flux']}")
         log(f"with coupled solves: radiosity {cb['radiosity']['solves_per_s']:.1f} solves/s p50 "
             f"{cb['radiosity']['p50_ms']:.1f} ms p99 {cb['radiosity']['p99_ms']:.1f} ms; coupled latencies "
-            f"{cb['coupled']['latency_s']} s, Newton updates {cb['coupled']['newton_updates']}")
+            f"{cb['coupled']['latency_s']} s (alone: {cb['coupled']['alone_latency_s']} s), "
+            f"Newton updates {cb['coupled']['newton_updates']}, max flux error vs FP64 "
+            f"{max(max(v.values()) for v in cb['coupled']['max_rel_flux_error_vs_fp64'].values()):.2e}")
     if not args.no_cpu:
         record["cpu_baseline"] = cpu_baseline(phases, R, seed)
         log(f"cpu baseline {record['cpu_baseline']['value']:.3e} rays/s")

@@ -224,7 +224,7 @@ int prh_radiosity(prh_operator *op, int32_t n_rhs, const double *emitted, const 
 void prh_operator_destroy(prh_operator *op);
 
 /* ---- batched radiosity engine (continuous batching) -------------------
- * Up to 16 emissions ("slots") are solved together on one operator.  Each
+ * Up to 32 emissions ("slots") are solved together on one operator.  Each
  * slot alternates epochs of FP32 Chebyshev sweeps with one FP64 refinement
  * sweep (iterative refinement) and finishes when
  * ||E - (I - diag(1-eps) F_p^T) J||_1 <= relative_tolerance ||E||_1; its
@@ -321,9 +321,10 @@ int prh_coupled_linearize(prh_coupled *c, int32_t T, double T_hot_K, double T_co
 int prh_coupled_jacobian(prh_coupled *c, int32_t v, double emissivity, double relative_tolerance, int32_t out,
                          int32_t *series_iterations);
 /* Precision of the Jacobian-vector products' radiosity solves: PRH_FP64 (the
- * reference's reflection series, default) or PRH_MIXED (the batched engine's
- * FP32 Chebyshev sweeps with FP64 refinement to the same relative tolerance;
- * the radiation of the Newton residual stays an FP64 series).  Replaces the
+ * reference's reflection series, default) or PRH_MIXED (FP32 Chebyshev
+ * sweeps over a contiguous FP32 vector with FP64 refinement to the same
+ * relative tolerance; the radiation of the Newton residual stays an FP64
+ * series).  Replaces the
  * inner solves of radiation_jvp inside newton_krylov, porerad/coupled.py:108-126,
  * 362-425, with their precision as a choice. */
 int prh_coupled_set_precision(prh_coupled *c, int32_t precision);

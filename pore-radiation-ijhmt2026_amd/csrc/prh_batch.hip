@@ -1,6 +1,6 @@
 // Batched mixed-precision radiosity engine with continuous batching.
 //
-// Solves (I - diag(rho) F_p^T) J = E, rho = 1 - eps, for up to 16 emissions
+// Solves (I - diag(rho) F_p^T) J = E, rho = 1 - eps, for up to 32 emissions
 // ("slots") at once on one device-resident exchange operator, and returns the
 // absorbed power eps * (F_p^T J) -- the quantity the reference's reflection
 // series sums reflection by reflection (porerad/exchange.py:217-245).
@@ -15,7 +15,7 @@
 // Chebyshev count whose error bound covers what is left to reach the
 // tolerance, at most a 1e-6 reduction (what FP32 iterations resolve).
 //
-// Every sweep ("tick") reads the operator once for all 16 slots and
+// Every sweep ("tick") reads the operator once for all 32 slots and
 // accumulates the products in FP64 from FP32 slot vectors: K = C + C^T is
 // stored exactly as uint16 when all patches share area and ray count
 // (S = A/(2R) K + A delta I), else S in FP64.  So the refinement step of a
@@ -25,7 +25,7 @@
 // the same pass.  Each slot runs its own schedule (Chebyshev tick, last
 // tick, refinement tick, idle) independently of the others, and a slot may
 // be admitted or retired at any tick.  Slot vectors are interleaved
-// [row][slot] so one 64 B gather serves all 16 slots; workgroups are mapped
+// [row][slot] so one 128 B line serves all 32 slots; workgroups are mapped
 // so that each XCD sweeps a contiguous range of rows.
 //
 // Host scheduling is pipelined: the host launches tick t and only then
@@ -35,6 +35,7 @@
 // never mixes with another slot's and every reduction has a fixed order, so
 // a solve returns the same bits alone or in any mix.
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <vector>
 
@@ -44,13 +45,10 @@
 namespace prh {
 namespace {
 
-constexpr int SLOTS = 16;
+constexpr int SLOTS = 32;
 constexpr int BT = 256;
 constexpr int ROWS = BT / 64;
 constexpr int RING = 8;
-constexpr int MAX_TICKS = 64;
-constexpr int MAX_EPOCHS = 40;
-constexpr double INNER_REDUCTION = 1.0e-6;
 constexpr double SIGMA = 5.670374419e-8;  // porerad/constants.py:3
 
 enum : int32_t { MODE_IDLE = 0, MODE_CHEB = 1, MODE_LAST = 2, MODE_REFINE = 3 };
@@ -97,6 +95,15 @@ __device__ __forceinline__ float scaled(float di, float v, float ai, bool unifor
     return uniform ? di * v : di * v / ai;
 }
 
+// Slots 4h .. 4h+3 of row j of an interleaved slot vector.  The byte offset
+// j * 128 + 16 h stays below 2^32 (at most 2^25 patches), so the load takes a
+// scalar base and one 32-bit offset register.
+__device__ __forceinline__ float4 gather4(const float *z, int32_t j, int h)
+{
+    const uint32_t off = (uint32_t)j * (uint32_t)(SLOTS * sizeof(float)) + 16u * (uint32_t)h;
+    return *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(z) + off);
+}
+
 struct TickArgs {
     MatView M;
     SlotVecs V;
@@ -117,41 +124,53 @@ __global__ __launch_bounds__(BT) void k_tick(TickArgs a)
     __shared__ double s_abs[ROWS][SLOTS], s_sum[ROWS][SLOTS];
     const MatView &M = a.M;
     const int wave = threadIdx.x >> 6;
-    const int64_t row = first_row(M) + wave;
+    // one row per wavefront: the row (and its CSR range) is wave-uniform
+    const int64_t row = (int64_t)__builtin_amdgcn_readfirstlane((int)(first_row(M) + wave));
     const int lane = threadIdx.x & 63;
-    const int h = lane & 3;   // gather: slots 4h .. 4h+3
-    const int g = lane >> 2;  // nonzero group
+    const int h = lane & 7;   // gather: slots 4h .. 4h+3
+    const int g = lane >> 3;  // nonzero group
     double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
-    if (row < M.n && ((a.group_mask >> h) & 1u)) {
-        const int64_t e1 = M.rowptr[row + 1];
-        int64_t e = M.rowptr[row] + g;
-        // two independent gathers in flight per lane (rows hold ~10^3 nonzeros)
-        for (; e + 16 < e1; e += 32) {
-            const int32_t j0 = M.col[e], j1 = M.col[e + 16];
-            const double v0 = UNIFORM ? (double)M.kval[e] : M.val64[e];
-            const double v1 = UNIFORM ? (double)M.kval[e + 16] : M.val64[e + 16];
-            const float4 z0 = *reinterpret_cast<const float4 *>(a.z_in + (size_t)j0 * SLOTS + 4 * h);
-            const float4 z1 = *reinterpret_cast<const float4 *>(a.z_in + (size_t)j1 * SLOTS + 4 * h);
-            acc0 = fma(v0, (double)z0.x, acc0);
-            acc1 = fma(v0, (double)z0.y, acc1);
-            acc2 = fma(v0, (double)z0.z, acc2);
-            acc3 = fma(v0, (double)z0.w, acc3);
-            acc0 = fma(v1, (double)z1.x, acc0);
-            acc1 = fma(v1, (double)z1.y, acc1);
-            acc2 = fma(v1, (double)z1.z, acc2);
-            acc3 = fma(v1, (double)z1.w, acc3);
-        }
-        if (e < e1) {
-            const int32_t j = M.col[e];
-            const double v = UNIFORM ? (double)M.kval[e] : M.val64[e];
-            const float4 z = *reinterpret_cast<const float4 *>(a.z_in + (size_t)j * SLOTS + 4 * h);
-            acc0 = fma(v, (double)z.x, acc0);
-            acc1 = fma(v, (double)z.y, acc1);
-            acc2 = fma(v, (double)z.z, acc2);
-            acc3 = fma(v, (double)z.w, acc3);
+    if (row < M.n && a.group_mask) {
+        // A tick makes one random cache-line request per nonzero: the 32 slots
+        // of a row are one 128 B line, fetched by the 8 lanes of a group, 16 B
+        // (slots 4h .. 4h+3) each.  Every lane loads a distinct column and
+        // value (64 nonzeros per load), the lanes exchange them through the LDS
+        // crossbar, and each lane then has 8 gathers in flight.  Entries past
+        // the row's end repeat its last entry with weight 0, and idle slot
+        // groups are gathered too (their sums are never used), so the loop is
+        // branch-free.
+        const int64_t e0 = M.rowptr[row], e1 = M.rowptr[row + 1];
+        const uint32_t len = (uint32_t)(e1 - e0);  // at most one entry per patch: < 2^25
+        const int32_t *col = M.col + e0;
+        const uint16_t *kval = UNIFORM ? M.kval + e0 : nullptr;
+        const double *val = UNIFORM ? nullptr : M.val64 + e0;
+        for (uint32_t base = 0; base < len; base += 64u) {
+            const uint32_t e = base + (uint32_t)lane;
+            const bool ok = e < len;
+            const uint32_t ec = ok ? e : len - 1;
+            const int32_t jl = col[ec];
+            const double w = UNIFORM ? (double)kval[ec] : val[ec];
+            const double vl = ok ? w : 0.0;
+            int32_t j[8];
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                j[u] = __shfl(jl, g + 8 * u);
+                v[u] = __shfl(vl, g + 8 * u);
+            }
+            float4 z[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) z[u] = gather4(a.z_in, j[u], h);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                acc0 = fma(v[u], (double)z[u].x, acc0);
+                acc1 = fma(v[u], (double)z[u].y, acc1);
+                acc2 = fma(v[u], (double)z[u].z, acc2);
+                acc3 = fma(v[u], (double)z[u].w, acc3);
+            }
         }
     }
-    for (int off = 4; off < 64; off <<= 1) {
+    for (int off = 8; off < 64; off <<= 1) {
         acc0 += __shfl_xor(acc0, off);
         acc1 += __shfl_xor(acc1, off);
         acc2 += __shfl_xor(acc2, off);
@@ -361,44 +380,6 @@ __global__ void k_collect(int64_t n, int q, const double *eps64, const double *y
 }
 
 unsigned grid_of(int64_t n, int b) { return (unsigned)std::max<int64_t>(1, (n + b - 1) / b); }
-
-// Chebyshev iterations that shrink the error bound 2 c^k / (1 + c^2k) below
-// tau (at least 2: the epoch length is decided one tick after it starts).
-int ticks_for(double rho_max, double tau)
-{
-    if (rho_max <= 0.0) return 2;
-    const double kappa = (1.0 + rho_max) / (1.0 - rho_max);
-    const double c = (std::sqrt(kappa) - 1.0) / (std::sqrt(kappa) + 1.0);
-    const int m = (int)std::ceil(std::log(tau / 2.0) / std::log(c));
-    return std::min(std::max(m, 2), MAX_TICKS);
-}
-
-// Reduction an epoch aims for: what is left to reach the tolerance, with a
-// factor 10 of margin, but no finer than FP32 iterations resolve.
-double epoch_target(double rtol, double e_norm, double r_norm)
-{
-    const double left = 0.1 * rtol * e_norm / std::max(r_norm, 1e-300);
-    return std::min(std::max(left, INNER_REDUCTION), 0.5);
-}
-
-// alpha_k, beta_k of Chebyshev acceleration on [1 - delta, 1 + delta]:
-// rho_0 = delta, rho_{k+1} = 1 / (2/delta - rho_k),
-// dir_{k+1} = rho_{k+1} rho_k dir_k + (2 rho_{k+1} / delta) r_{k+1}.
-void cheb_coefficient(double delta, int k, float *alpha, float *beta)
-{
-    if (delta <= 0.0) {
-        *alpha = 0.f;
-        *beta = 1.f;
-        return;
-    }
-    double rho = delta, next = 0.0;
-    for (int i = 0; i <= k; ++i) {
-        next = 1.0 / (2.0 / delta - rho);
-        if (i < k) rho = next;
-    }
-    *alpha = (float)(next * rho);
-    *beta = (float)(2.0 * next / delta);
-}
 
 enum SlotState { FREE = 0, RUNNING, FINISHING, DONE };
 
@@ -752,6 +733,10 @@ int prh_batch_create(prh_operator *op, prh_batch **out)
 {
     PRH_CHECK_ARG(op && out, "prh_batch_create: null pointer");
     *out = nullptr;
+    if (op->n > (int64_t(1) << 25)) {
+        set_error("prh_batch_create: %lld patches exceed the engine's 2^25-patch capacity", (long long)op->n);
+        return PRH_ERR_CAPACITY;
+    }
     PRH_HIP(hipSetDevice(op->device));
     prh_batch *b = new prh_batch();
     b->op = op;

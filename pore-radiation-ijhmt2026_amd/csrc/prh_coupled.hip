@@ -364,10 +364,9 @@ struct prh_coupled {
     double *part = nullptr, *scal = nullptr, *ycoef = nullptr;
     int32_t pcg_steps = 0;
     bool have_lin = false, have_pcg = false;
-    // Jacobian-vector products: FP64 reflection series (PRH_FP64) or the
-    // batched engine's FP32 sweeps with FP64 refinement (PRH_MIXED)
+    // Jacobian-vector products: FP64 reflection series (PRH_FP64) or FP32
+    // Chebyshev sweeps with FP64 refinement (PRH_MIXED, mixed_solve_single)
     int precision = PRH_FP64;
-    prh_batch *engine = nullptr;
 };
 
 namespace {
@@ -497,7 +496,6 @@ void prh_coupled_destroy(prh_coupled *c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->engine) prh_batch_destroy(c->engine);
     free_coupled(c);
     delete c;
 }
@@ -664,7 +662,7 @@ int prh_coupled_jacobian(prh_coupled *c, int32_t v, double emissivity, double rt
     PRH_HIP(hipGetLastError());
     double res = 0.0;
     int32_t its = 0;
-    if (c->precision == PRH_MIXED) rc = batch_solve_device(c->engine, op->emit, emissivity, rtol, st, op->absorbed, &its);
+    if (c->precision == PRH_MIXED) rc = mixed_solve_single(op, op->emit, emissivity, rtol, st, op->absorbed, &its);
     else rc = series_fp64(op, rtol, 10000, st, &res, &its);
     if (rc) return rc;
     k_cell_net<<<blocks_of(c->n_cells), CT, 0, st>>>(g.cell_code, op->absorbed, op->emit, c->ax, slot(c, out),
@@ -678,12 +676,6 @@ int prh_coupled_set_precision(prh_coupled *c, int32_t precision)
 {
     PRH_CHECK_ARG(c && (precision == PRH_FP64 || precision == PRH_MIXED), "prh_coupled_set_precision: bad argument");
     PRH_CHECK_ARG(c->op || precision == PRH_FP64, "prh_coupled_set_precision: no operator in this handle");
-    if (precision == PRH_MIXED && !c->engine) {
-        PRH_HIP(hipSetDevice(c->device));
-        int rc = prh_batch_create(c->op, &c->engine);
-        if (rc) return rc;
-        PRH_HIP(hipStreamSynchronize(nullptr));  // the engine's initial clears run on the null stream
-    }
     c->precision = precision;
     return PRH_OK;
 }

@@ -46,9 +46,10 @@ struct SView {
     double inv_2R, delta;
 };
 
-// (S x)_row from the lane partials of one wavefront.
-template <bool UNIFORM>
-__device__ __forceinline__ double s_row(const SView &S, int64_t row, int lane, const double *x)
+// sum_j v_j x_j over row `row` (v = integer K or FP64 S), FP64 accumulation
+// of x in its own type, from the lane partials of one wavefront.
+template <bool UNIFORM, typename V>
+__device__ __forceinline__ double row_sum(const SView &S, int64_t row, int lane, const V *x)
 {
     // four independent gathers in flight per lane (rows hold ~10^3 nonzeros)
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -60,14 +61,21 @@ __device__ __forceinline__ double s_row(const SView &S, int64_t row, int lane, c
         const double v1 = UNIFORM ? (double)S.kval[e + 64] : S.val[e + 64];
         const double v2 = UNIFORM ? (double)S.kval[e + 128] : S.val[e + 128];
         const double v3 = UNIFORM ? (double)S.kval[e + 192] : S.val[e + 192];
-        const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+        const double x0 = (double)x[c0], x1 = (double)x[c1], x2 = (double)x[c2], x3 = (double)x[c3];
         a0 += v0 * x0;
         a1 += v1 * x1;
         a2 += v2 * x2;
         a3 += v3 * x3;
     }
-    for (; e < e1; e += 64) a0 += (UNIFORM ? (double)S.kval[e] : S.val[e]) * x[S.col[e]];
-    const double acc = wave_sum((a0 + a1) + (a2 + a3));
+    for (; e < e1; e += 64) a0 += (UNIFORM ? (double)S.kval[e] : S.val[e]) * (double)x[S.col[e]];
+    return wave_sum((a0 + a1) + (a2 + a3));
+}
+
+// (S x)_row from the lane partials of one wavefront.
+template <bool UNIFORM>
+__device__ __forceinline__ double s_row(const SView &S, int64_t row, int lane, const double *x)
+{
+    const double acc = row_sum<UNIFORM, double>(S, row, lane, x);
     return UNIFORM ? S.area[row] * (acc * S.inv_2R + S.delta * x[row]) : acc;
 }
 
@@ -201,6 +209,137 @@ __global__ __launch_bounds__(ROW_THREADS) void k_series_step(SView M, int64_t n,
     }
 }
 
+// ---------------------------------------------------------------- single-emission mixed solve
+// One emission of (I - rho F_p^T) J = E with scalar rho = 1 - eps: epochs of
+// FP32 Chebyshev sweeps in residual form, each ended by an FP64 refinement
+// sweep (the batched engine's iteration for one slot).  z holds the gathered
+// form of the FP32 vector of the sweep: d v (equal areas) or d v / A.
+struct Mix1 {
+    float *x32, *r32, *dir32;
+    double *x64, *y64, *r64;
+};
+
+enum : int { MIX_CHEB = 0, MIX_LAST = 1, MIX_REFINE = 2 };
+
+__device__ __forceinline__ float mix_scaled(float di, float v, float ai, bool uniform)
+{
+    return uniform ? di * v : di * v / ai;
+}
+
+// One sweep: y = F_p^T z (FP64 accumulation of the FP32 z), then the row's
+// update.  CHEB: dir' = alpha dir + beta r'; LAST: the epoch's correction x32
+// is gathered next; REFINE: x64 += s c, y64 += s y with c the correction,
+// r64 = E - (x64 - rho y64), and the block sums of |r64|.
+template <bool UNIFORM>
+__global__ __launch_bounds__(ROW_THREADS) void k_mix1_sweep(SView M, int64_t n, const double *d, const float *d32,
+                                                            Mix1 V, const float *z_in, float *z_out, int mode,
+                                                            float alpha, float beta, float rho32, double rho,
+                                                            double s, const double *E, double *part_abs)
+{
+    __shared__ double sa[ROWS_PER_BLOCK];
+    const int wave = threadIdx.x >> 6;
+    const int64_t row = (int64_t)blockIdx.x * ROWS_PER_BLOCK + wave;
+    const int lane = threadIdx.x & 63;
+    double ra = 0.0;
+    if (row < n) {
+        const double w = row_sum<UNIFORM, float>(M, row, lane, z_in);
+        if (lane == 0) {
+            const float zi = z_in[row];
+            const double di = d[row];
+            const double y = UNIFORM ? di * (w * M.inv_2R + M.delta * (double)zi) : di * w;
+            if (mode == MIX_REFINE) {
+                const double c = UNIFORM ? (double)zi / di : (double)zi * M.area[row] / di;
+                const double x = V.x64[row] + s * c;
+                const double yy = V.y64[row] + s * y;
+                const double r = E[row] - (x - rho * yy);
+                V.x64[row] = x;
+                V.y64[row] = yy;
+                V.r64[row] = r;
+                ra = fabs(r);
+            } else {
+                const float ai32 = UNIFORM ? 1.f : (float)M.area[row];
+                const float dk = V.dir32[row];
+                const float Ad = dk - rho32 * (float)y;
+                const float rn = V.r32[row] - Ad;
+                const float x = V.x32[row] + dk;
+                V.x32[row] = x;
+                if (mode == MIX_CHEB) {
+                    const float dn = alpha * dk + beta * rn;
+                    V.r32[row] = rn;
+                    V.dir32[row] = dn;
+                    z_out[row] = mix_scaled(d32[row], dn, ai32, UNIFORM);
+                } else {
+                    z_out[row] = mix_scaled(d32[row], x, ai32, UNIFORM);
+                }
+            }
+        }
+    }
+    if (mode != MIX_REFINE) return;  // the same for every thread of the grid
+    if (lane == 0) sa[wave] = ra;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int k = 0; k < ROWS_PER_BLOCK; ++k) a += sa[k];
+        part_abs[blockIdx.x] = a;
+    }
+}
+
+// Start: x64 = y64 = 0, r64 = E, and the block sums of |E|.
+__global__ __launch_bounds__(RED_THREADS) void k_mix1_start(const double *E, int64_t n, Mix1 V, double *part_abs)
+{
+    __shared__ double red[RED_THREADS];
+    double a = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
+        const double e = E[i];
+        V.x64[i] = 0.0;
+        V.y64[i] = 0.0;
+        V.r64[i] = e;
+        a += fabs(e);
+    }
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int off = RED_THREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part_abs[blockIdx.x] = red[0];
+}
+
+// A new epoch on the scale s: x32 = 0, r32 = dir32 = r64 / s, z = its gathered form.
+__global__ void k_mix1_epoch(int64_t n, const float *d32, const double *area, int uniform, Mix1 V, double inv_s,
+                             float *z)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float r = (float)(V.r64[i] * inv_s);
+    V.x32[i] = 0.f;
+    V.r32[i] = r;
+    V.dir32[i] = r;
+    z[i] = mix_scaled(d32[i], r, uniform ? 1.f : (float)area[i], uniform != 0);
+}
+
+// Fixed-order total of n_part partial sums into *out.
+__global__ __launch_bounds__(RED_THREADS) void k_mix1_total(const double *part, int64_t n_part, double *out)
+{
+    __shared__ double red[RED_THREADS];
+    double a = 0.0;
+    for (int64_t b = threadIdx.x; b < n_part; b += RED_THREADS) a += part[b];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int off = RED_THREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+// absorbed = eps y64 (zero for the zero emission).
+__global__ void k_mix1_collect(int64_t n, double eps, const double *y64, double *absorbed)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) absorbed[i] = y64 ? eps * y64[i] : 0.0;
+}
+
 __global__ __launch_bounds__(RED_THREADS) void k_series_check(const double *part_abs, const double *part_sum,
                                                               int64_t n_part, double rtol, SeriesState *S)
 {
@@ -254,7 +393,7 @@ void free_operator(prh_operator *op)
     void *ptrs[] = {op->rowptr, op->col, op->val, op->c_rowptr, op->t_rowptr, op->c_col, op->c_cnt,
                     op->t_col, op->t_cnt, op->rays, op->area, op->scale, op->vx, op->vy, op->vt,
                     op->flight, op->absorbed, op->w0, op->w1, op->eps, op->emit, op->part_abs,
-                    op->part_sum, op->state, op->kval, op->val32, op->scale32};
+                    op->part_sum, op->state, op->kval, op->val32, op->scale32, op->mix32, op->mix64};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : op->series_ev)
@@ -472,6 +611,80 @@ int series_fp64(prh_operator *op, double relative_tolerance, int32_t max_reflect
     }
     *residual = hs.residual;
     *iterations = hs.iterations;
+    return PRH_OK;
+}
+
+int mixed_solve_single(prh_operator *op, const double *E, double emissivity, double rtol, hipStream_t st,
+                       double *absorbed, int32_t *iterations)
+{
+    PRH_CHECK_ARG(op && E && absorbed, "mixed_solve_single: null pointer");
+    PRH_CHECK_ARG(emissivity > 0.0 && emissivity <= 1.0 && rtol > 0.0, "mixed_solve_single: bad emissivity or tolerance");
+    const int64_t n = op->n;
+    if (!op->mix32) {
+        if (hipMalloc((void **)&op->mix32, sizeof(float) * 5 * n) != hipSuccess ||
+            hipMalloc((void **)&op->mix64, sizeof(double) * (3 * n + 1)) != hipSuccess) {
+            set_error("mixed_solve_single: out of device memory");
+            return PRH_ERR_HIP;
+        }
+    }
+    const Mix1 V{op->mix32, op->mix32 + n, op->mix32 + 2 * n, op->mix64, op->mix64 + n, op->mix64 + 2 * n};
+    float *z[2] = {op->mix32 + 3 * n, op->mix32 + 4 * n};
+    double *total = op->mix64 + 3 * n;
+    const unsigned rb = blocks_for(n, ROWS_PER_BLOCK), eb = blocks_for(n, 256);
+    const unsigned ib = (unsigned)std::min<int64_t>(blocks_for(n, RED_THREADS), 1024);
+    const int uniform = op->uniform ? 1 : 0;
+    double e_norm = 0.0;
+    k_mix1_start<<<ib, RED_THREADS, 0, st>>>(E, n, V, op->part_abs);
+    k_mix1_total<<<1, RED_THREADS, 0, st>>>(op->part_abs, (int64_t)ib, total);
+    PRH_HIP(hipGetLastError());
+    PRH_HIP(hipMemcpyAsync(&e_norm, total, sizeof(double), hipMemcpyDeviceToHost, st));
+    PRH_HIP(hipStreamSynchronize(st));
+    if (!(e_norm > 0.0)) {
+        k_mix1_collect<<<eb, 256, 0, st>>>(n, emissivity, nullptr, absorbed);  // the zero emission
+        PRH_HIP(hipGetLastError());
+        if (iterations) *iterations = 0;
+        return PRH_OK;
+    }
+    const double rho = 1.0 - emissivity;
+    const float rho32 = (float)rho;
+    double s = e_norm / (double)n, r_norm = e_norm;
+    int m = ticks_for(rho, epoch_target(rtol, 1.0, 1.0));
+    int sweeps = 0, epochs = 0;
+    auto sweep = op->uniform ? k_mix1_sweep<true> : k_mix1_sweep<false>;
+    k_mix1_epoch<<<eb, 256, 0, st>>>(n, op->scale32, op->area, uniform, V, 1.0 / s, z[0]);
+    while (true) {
+        int in = 0;
+        for (int k = 0; k < m; ++k) {
+            float alpha = 0.f, beta = 1.f;
+            const bool last = k == m - 1;
+            if (!last) cheb_coefficient(rho, k, &alpha, &beta);
+            sweep<<<rb, ROW_THREADS, 0, st>>>(sview_of(op), n, op->scale, op->scale32, V, z[in], z[1 - in],
+                                              last ? MIX_LAST : MIX_CHEB, alpha, beta, rho32, rho, s, E,
+                                              op->part_abs);
+            in = 1 - in;
+            ++sweeps;
+        }
+        // the refinement gathers the epoch's correction x32 (left in z[in] by the last sweep)
+        sweep<<<rb, ROW_THREADS, 0, st>>>(sview_of(op), n, op->scale, op->scale32, V, z[in], z[1 - in], MIX_REFINE,
+                                          0.f, 1.f, rho32, rho, s, E, op->part_abs);
+        k_mix1_total<<<1, RED_THREADS, 0, st>>>(op->part_abs, (int64_t)rb, total);
+        PRH_HIP(hipGetLastError());
+        PRH_HIP(hipMemcpyAsync(&r_norm, total, sizeof(double), hipMemcpyDeviceToHost, st));
+        PRH_HIP(hipStreamSynchronize(st));
+        ++epochs;
+        if (r_norm <= rtol * e_norm) break;
+        if (epochs >= MAX_EPOCHS || !(r_norm < HUGE_VAL)) {
+            set_error("radiosity reflection series did not converge");
+            return PRH_ERR_SERIES;
+        }
+        // the next epoch works on the residual scaled to mean magnitude 1
+        m = ticks_for(rho, epoch_target(rtol, e_norm, r_norm));
+        s = r_norm / (double)n;
+        k_mix1_epoch<<<eb, 256, 0, st>>>(n, op->scale32, op->area, uniform, V, 1.0 / s, z[0]);
+    }
+    k_mix1_collect<<<eb, 256, 0, st>>>(n, emissivity, V.y64, absorbed);
+    PRH_HIP(hipGetLastError());
+    if (iterations) *iterations = sweeps + epochs;
     return PRH_OK;
 }
 

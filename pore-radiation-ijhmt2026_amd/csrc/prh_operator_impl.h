@@ -4,6 +4,57 @@
 
 #include "prh_internal.h"
 
+#include <algorithm>
+#include <cmath>
+
+namespace prh {
+// Schedule of the mixed-precision iteration, shared by the batched engine and
+// the single-emission solve: epochs of FP32 Chebyshev sweeps on
+// [1 - rho_max, 1 + rho_max], each ended by an FP64 refinement.
+constexpr int MAX_TICKS = 64;
+constexpr int MAX_EPOCHS = 40;
+constexpr double INNER_REDUCTION = 1.0e-6;
+
+// Chebyshev iterations that shrink the error bound 2 c^k / (1 + c^2k) below
+// tau (at least 2: the epoch length is decided one tick after it starts).
+inline int ticks_for(double rho_max, double tau)
+{
+    if (rho_max <= 0.0) return 2;
+    const double kappa = (1.0 + rho_max) / (1.0 - rho_max);
+    const double c = (std::sqrt(kappa) - 1.0) / (std::sqrt(kappa) + 1.0);
+    const int m = (int)std::ceil(std::log(tau / 2.0) / std::log(c));
+    return std::min(std::max(m, 2), MAX_TICKS);
+}
+
+// Reduction an epoch aims for: what is left to reach the tolerance, with a
+// factor 10 of margin, but no finer than FP32 iterations resolve.
+inline double epoch_target(double rtol, double e_norm, double r_norm)
+{
+    const double left = 0.1 * rtol * e_norm / std::max(r_norm, 1e-300);
+    return std::min(std::max(left, INNER_REDUCTION), 0.5);
+}
+
+// alpha_k, beta_k of Chebyshev acceleration on [1 - delta, 1 + delta]:
+// rho_0 = delta, rho_{k+1} = 1 / (2/delta - rho_k),
+// dir_{k+1} = rho_{k+1} rho_k dir_k + (2 rho_{k+1} / delta) r_{k+1}.
+inline void cheb_coefficient(double delta, int k, float *alpha, float *beta)
+{
+    if (delta <= 0.0) {
+        *alpha = 0.f;
+        *beta = 1.f;
+        return;
+    }
+    double rho = delta, next = 0.0;
+    for (int i = 0; i <= k; ++i) {
+        next = 1.0 / (2.0 / delta - rho);
+        if (i < k) rho = next;
+    }
+    *alpha = (float)(next * rho);
+    *beta = (float)(2.0 * next / delta);
+}
+
+}  // namespace prh
+
 namespace prh {
 
 struct SeriesState {
@@ -51,6 +102,10 @@ struct prh_operator {
     float *val32 = nullptr;    // FP32 S (non-uniform only)
     float *scale32 = nullptr;
     prh_batch *batch = nullptr;  // engine of the mixed-precision path, created on first use
+    // single-emission mixed-precision solve (mixed_solve_single), allocated on first use:
+    // FP32 x, r, dir, z[2] (n each) and FP64 x, y, r (n each) + one total
+    float *mix32 = nullptr;
+    double *mix64 = nullptr;
     // device time of the FP64 series (HIP events around each series)
     hipEvent_t series_ev[2] = {nullptr, nullptr};
     int64_t series_count = 0, series_steps = 0;
@@ -79,5 +134,11 @@ void operator_free(prh_operator *op);
 // `absorbed` (device).  The engine must not be driven from another thread
 // meanwhile.
 int batch_solve_device(prh_batch *b, const double *E, double emissivity, double rtol, hipStream_t st,
+                       double *absorbed, int32_t *iterations);
+// The same iteration for one emission without the engine's slot interleaving:
+// the FP32 vectors are contiguous (4 B gathers from an L2-resident vector),
+// one wavefront per row as in the FP64 series, and the host reads one norm
+// per epoch.  Same contract as batch_solve_device.
+int mixed_solve_single(prh_operator *op, const double *E, double emissivity, double rtol, hipStream_t st,
                        double *absorbed, int32_t *iterations);
 }  // namespace prh

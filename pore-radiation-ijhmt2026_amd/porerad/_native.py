@@ -30,7 +30,7 @@ PRH_ERR_CAPACITY = 6
 
 APPLY_SYMMETRIC, APPLY_MATVEC, APPLY_RMATVEC, APPLY_RAW, APPLY_RAW_T = 0, 1, 2, 3, 4
 FP64, MIXED = 0, 1
-BATCH_SLOTS = 16
+BATCH_SLOTS = 32
 BATCH_DIRECT = 1
 
 # Every function the C ABI exports: name -> (argtypes, restype).

@@ -5,11 +5,11 @@ A request names a resident geometry (its ``SurfacePatches`` and
 for the radiation channels of the prescribed linear field between the walls
 (porerad/coupled.py:170-194), solved to a relative tolerance with the
 batched mixed-precision engine (``prh_batch_*``: FP32 Chebyshev sweeps over
-16 slots per operator, FP64 refinement).  The emission is formed and the
+32 slots per operator, FP64 refinement).  The emission is formed and the
 channels are summed on the device, so a request moves a few scalars over
 PCIe; the patch powers come back only when a request asks for them.
 
-Scheduling.  Every geometry has one engine with 16 slots.  Each scheduler
+Scheduling.  Every geometry has one engine with 32 slots.  Each scheduler
 step (1) admits queued requests, oldest first, into free slots of their
 geometry's engine, creating the engine first if its device memory fits the
 HBM budget (idle engines are released to make room); (2) advances every
@@ -44,7 +44,7 @@ import numpy as np
 from .constants import SIGMA
 from .view_factors import COLD, HOT, INTERNAL
 
-SLOTS = 16
+SLOTS = 32
 
 
 @dataclasses.dataclass
@@ -74,6 +74,7 @@ class CoupledRequest:
     arrival_s: float = 0.0
     tag: object = None
     precision: str = "fp64"  # radiosity of the Jacobian-vector products: "fp64" series or "mixed" engine
+    preconditioner_steps: tuple = (10, 20)  # Jacobi-PCG step counts tried in turn (coupled.solve_coupled)
 
 
 @dataclasses.dataclass
@@ -175,20 +176,22 @@ def device_coupled_runner(geometry: Geometry, request: CoupledRequest) -> tuple[
     Th, Tc = float(request.T_hot_K), float(request.T_cold_K)
     T0 = conduction.solve_conduction(system, Th, Tc)
     result = coupled.solve_coupled(system, geometry.operator, P, Th, Tc, float(request.epsilon), T0,
-                                   precision=request.precision)
+                                   precision=request.precision,
+                                   preconditioner_steps=tuple(request.preconditioner_steps))
     metrics = coupled.wall_flux_metrics(system, P, result, Th, Tc)
     return bool(result.converged), int(result.newton_iterations), metrics, result.failure
 
 
 def device_coupled_bytes(geometry: Geometry) -> int:
-    """Device bytes of one coupled solve: 73 cell vectors, scratch and the 7-point conduction matrix."""
+    """Device bytes of one coupled solve: 73 cell vectors, scratch and the 7-point conduction matrix,
+    patch vectors, and the mixed-precision Jacobian solve's vectors (5 FP32 + 3 FP64 per patch)."""
 
     n_cells = int(np.prod(geometry.patches.solid.shape))
-    return n_cells * (83 * 8 + 7 * 12 + 8) + geometry.n * 32 + (1 << 20)
+    return n_cells * (83 * 8 + 7 * 12 + 8) + geometry.n * (32 + 44) + (1 << 20)
 
 
 def device_engine_bytes(geometry: Geometry) -> int:
-    """Device bytes of one engine: 16 slots x (6 FP32 + 5 FP64 vectors) per patch, partial sums, field."""
+    """Device bytes of one engine: 32 slots x (6 FP32 + 5 FP64 vectors) per patch, partial sums, field."""
 
     n = geometry.n
     blocks = ((n + 3) // 4 + 7) // 8 * 8

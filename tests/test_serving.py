@@ -1,6 +1,6 @@
 """Scheduling logic of the radiosity server, on the CPU with a stand-in engine.
 
-The stand-in keeps the engine contract of prh_batch (16 slots,
+The stand-in keeps the engine contract of prh_batch (32 slots,
 admit_linear / step / channels / result / release, device bytes) and
 finishes a slot after a fixed number of sweeps, with absorbed power = E / 4,
 so admission order, continuous batching and HBM-budget eviction can be
@@ -124,7 +124,7 @@ def test_slots_are_capped_and_refilled_between_epochs():
     srv.step()
     assert len(srv.slots["a"]) == serving.SLOTS and len(srv.queue) == 4
     srv.step()
-    srv.step()  # first 16 finish at their 3rd epoch
+    srv.step()  # the first SLOTS finish at their 3rd epoch
     assert len(srv.replies) == serving.SLOTS
     srv.step()  # the 4 waiting requests are admitted while nothing else runs
     assert len(srv.slots["a"]) == 4
