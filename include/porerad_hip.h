@@ -46,6 +46,8 @@ extern "C" {
 #define PRH_ERR_SERIES 5     /* reflection series did not converge
                                 (RuntimeError, porerad/exchange.py:243-244) */
 #define PRH_ERR_CAPACITY 6   /* request exceeds a documented capacity */
+#define PRH_ERR_BALANCE 7    /* diagonal balancing did not converge
+                                (RuntimeError, porerad/exchange.py:197) */
 
 /* ---- operator application modes (prh_operator_apply) ----------------- */
 #define PRH_APPLY_SYMMETRIC 0 /* S x              (symmetric_matvec) */
@@ -202,6 +204,17 @@ int prh_operator_edges(const prh_operator *op, int32_t *origin, int32_t *receive
 /* Diagonal balancing factor d (n), used by modes MATVEC/RMATVEC and by
  * prh_radiosity.  Defaults to ones. */
 int prh_operator_set_scale(prh_operator *op, const double *scale);
+/* Balance the operator on the device and set d: solve d (S d) = A, as
+ * ExchangeOperator._balance, porerad/exchange.py:153-197 (with
+ * _closure_error, :142-151): no balancing when d = 1 already closes the rows
+ * within tolerance (method 0); else SciPy's Anderson mixing (M 10, w0 0.01,
+ * at most 500 iterations, Armijo line search; method 1) when it reaches
+ * max |F| <= tolerance and the closure min(max(10 tol, 10 eps), 1e-10); else
+ * the damped multiplicative iteration (method 2, at most 20000 steps).
+ * PRH_ERR_BALANCE when none converges.  *iterations as balance_iterations. */
+int prh_operator_balance(prh_operator *op, double tolerance, int32_t *method, int32_t *iterations);
+/* Copy the balancing factor d (n doubles) to the host. */
+int prh_operator_get_scale(const prh_operator *op, double *scale);
 int prh_operator_nnz(const prh_operator *op, int64_t *nnz_symmetric, int64_t *nnz_raw);
 /* y = op(x) for one of the PRH_APPLY_* modes; x, y host arrays of n. */
 int prh_operator_apply(prh_operator *op, int32_t mode, const double *x, double *y);

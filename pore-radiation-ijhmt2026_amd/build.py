@@ -30,6 +30,7 @@ SOURCES = [
     "prh_tracer.hip",
     "prh_operator.hip",
     "prh_assemble.hip",
+    "prh_balance.hip",
     "prh_batch.hip",
     "prh_coupled.hip",
 ]

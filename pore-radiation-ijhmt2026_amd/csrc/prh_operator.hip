@@ -614,6 +614,15 @@ int series_fp64(prh_operator *op, double relative_tolerance, int32_t max_reflect
     return PRH_OK;
 }
 
+int symmetric_product(const prh_operator *op, const double *x, double *y, hipStream_t st)
+{
+    const unsigned rb = blocks_for(op->n, ROWS_PER_BLOCK);
+    if (op->uniform) k_spmv<true><<<rb, ROW_THREADS, 0, st>>>(sview_of(op), op->n, x, y);
+    else k_spmv<false><<<rb, ROW_THREADS, 0, st>>>(sview_of(op), op->n, x, y);
+    PRH_HIP(hipGetLastError());
+    return PRH_OK;
+}
+
 int mixed_solve_single(prh_operator *op, const double *E, double emissivity, double rtol, hipStream_t st,
                        double *absorbed, int32_t *iterations)
 {

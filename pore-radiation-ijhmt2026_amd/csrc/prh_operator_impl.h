@@ -141,4 +141,6 @@ int batch_solve_device(prh_batch *b, const double *E, double emissivity, double 
 // per epoch.  Same contract as batch_solve_device.
 int mixed_solve_single(prh_operator *op, const double *E, double emissivity, double rtol, hipStream_t st,
                        double *absorbed, int32_t *iterations);
+// y = S x (symmetric_matvec) of device vectors, queued on st.
+int symmetric_product(const prh_operator *op, const double *x, double *y, hipStream_t st);
 }  // namespace prh

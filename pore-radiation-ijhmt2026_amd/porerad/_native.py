@@ -27,6 +27,7 @@ PRH_ERR_UNRESOLVED = 3
 PRH_ERR_COUNTS = 4
 PRH_ERR_SERIES = 5
 PRH_ERR_CAPACITY = 6
+PRH_ERR_BALANCE = 7
 
 APPLY_SYMMETRIC, APPLY_MATVEC, APPLY_RMATVEC, APPLY_RAW, APPLY_RAW_T = 0, 1, 2, 3, 4
 FP64, MIXED = 0, 1
@@ -65,6 +66,8 @@ SIGNATURES = {
     "prh_edges_destroy": ([_P], None),
     "prh_operator_create": ([_I64, _P, _P, _P, _I64, _P, _P, _F64, _P], ctypes.c_int),
     "prh_operator_set_scale": ([_P, _P], ctypes.c_int),
+    "prh_operator_balance": ([_P, _F64, _P, _P], ctypes.c_int),
+    "prh_operator_get_scale": ([_P, _P], ctypes.c_int),
     "prh_operator_create_device": ([_P, _I64, _P, _P, _F64, _P], ctypes.c_int),
     "prh_operator_edges": ([_P, _P, _P, _P], ctypes.c_int),
     "prh_operator_nnz": ([_P, _P, _P], ctypes.c_int),
@@ -152,7 +155,7 @@ def check(status: int, what: str = "") -> None:
     if status == PRH_OK:
         return
     message = last_error() or f"{what} failed with status {status}"
-    if status in (PRH_ERR_UNRESOLVED, PRH_ERR_SERIES):
+    if status in (PRH_ERR_UNRESOLVED, PRH_ERR_SERIES, PRH_ERR_BALANCE):
         raise RuntimeError(message)
     if status in (PRH_ERR_COUNTS, PRH_ERR_ARG, PRH_ERR_CAPACITY):
         raise ValueError(message)
@@ -440,6 +443,19 @@ class Operator:
         check(self._lib.prh_operator_stats(self.handle, ptr(out), 6), "prh_operator_stats")
         return {"series": int(out[0]), "reflections": int(out[1]), "ms": float(out[2]), "nnz": int(out[3]),
                 "n": int(out[4]), "integer_form": bool(out[5])}
+
+    def balance(self, tolerance: float) -> tuple[int, int]:
+        """Balance on the device (method 0 none, 1 Anderson, 2 multiplicative; iterations)."""
+
+        method, iterations = ctypes.c_int32(), ctypes.c_int32()
+        check(self._lib.prh_operator_balance(self.handle, float(tolerance), ctypes.byref(method),
+                                             ctypes.byref(iterations)), "prh_operator_balance")
+        return int(method.value), int(iterations.value)
+
+    def get_scale(self) -> np.ndarray:
+        out = np.empty(self.n, np.float64)
+        check(self._lib.prh_operator_get_scale(self.handle, ptr(out)), "prh_operator_get_scale")
+        return out
 
     def set_scale(self, scale: np.ndarray) -> None:
         scale = np.ascontiguousarray(scale, dtype=np.float64)

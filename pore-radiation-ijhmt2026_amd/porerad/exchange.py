@@ -22,7 +22,6 @@ from __future__ import annotations
 from dataclasses import dataclass
 
 import numpy as np
-from scipy.optimize import NoConvergence, anderson
 
 from . import _native
 from .constants import SIGMA
@@ -121,7 +120,6 @@ class ExchangeOperator:
 
     def _finish(self, balance_tolerance: float) -> None:
         self.scale, self.balance_method, self.balance_iterations = self._balance(float(balance_tolerance))
-        self.device.set_scale(self.scale)
         self.row_sum_error = float(np.max(np.abs(self.matvec(np.ones(self.n_patches)) - 1.0)))
         if self.row_sum_error > CLOSURE_TOLERANCE:
             raise RuntimeError(f"projected rows do not close: {self.row_sum_error:.3e}")
@@ -211,49 +209,18 @@ class ExchangeOperator:
         return float(np.max(np.abs(row / self.patch_area_m2 - 1.0)))
 
     def _balance(self, tolerance: float) -> tuple[np.ndarray, str, int]:
-        """Solve d * (S d) = A for d > 0 (exchange.py:153-197)."""
+        """Solve d * (S d) = A for d > 0 (exchange.py:153-197), on the device.
 
-        n = self.n_patches
-        log_area = np.log(self.patch_area_m2)
-        y0 = np.zeros(n, dtype=np.float64)
-        if self._closure_error(y0) <= tolerance:
-            return np.exp(y0), "none", 0
-        calls = [0]
+        ``prh_operator_balance`` runs the reference's steps: no balancing when
+        d = 1 closes the rows, else SciPy's Anderson mixing (M=10, w0=0.01,
+        maxiter=500, f_tol=tolerance, Armijo line search) on y = log d with
+        the reference's acceptance test, else the damped multiplicative
+        iteration; RuntimeError when none converges.  Vectors and products stay
+        on the device; the factor is left set on the device operator.
+        """
 
-        def residual(y: np.ndarray) -> np.ndarray:
-            y = np.asarray(y, dtype=np.float64)
-            if not (np.all(np.isfinite(y)) and np.max(y) < _LOG_MAX and np.min(y) > _LOG_MIN):
-                return np.full(n, np.inf)
-            product = self.symmetric_matvec(np.exp(y))
-            if not np.all(np.isfinite(product)) or np.any(product <= 0.0):
-                return np.full(n, np.inf)
-            return y + np.log(product) - log_area
-
-        def on_iteration(_x, _f) -> None:
-            calls[0] += 1
-
-        try:
-            y = np.asarray(anderson(residual, y0, M=10, w0=0.01, maxiter=500, f_tol=tolerance,
-                                    callback=on_iteration), dtype=np.float64)
-            accept = min(max(10.0 * tolerance, 10.0 * np.finfo(float).eps), CLOSURE_TOLERANCE)
-            if self._closure_error(y) <= accept:
-                return np.exp(y), "anderson", calls[0]
-        except (NoConvergence, ValueError, FloatingPointError, np.linalg.LinAlgError):
-            pass
-
-        y = y0.copy()
-        for step in range(1, 20_001):
-            d = np.exp(y)
-            row = d * self.symmetric_matvec(d)
-            if not np.all(np.isfinite(row)) or np.any(row <= 0.0):
-                break
-            y = y + 0.5 * (log_area - np.log(row))
-            err = self._closure_error(y)
-            if not np.isfinite(err):
-                break
-            if err <= min(tolerance, CLOSURE_TOLERANCE):
-                return np.exp(y), "multiplicative", step
-        raise RuntimeError("diagonal balancing of the exchange matrix did not converge")
+        method, iterations = self.device.balance(tolerance)
+        return self.device.get_scale(), ("none", "anderson", "multiplicative")[method], iterations
 
 
 @dataclass
