@@ -117,6 +117,26 @@ int prh_geometry_patch_table(const prh_geometry *g, int8_t *patch_class, int32_t
                              int8_t *patch_face, double *patch_origin);
 void prh_geometry_destroy(prh_geometry *g);
 
+/* ---- multi-bounce Monte Carlo ray statistics -------------------------- */
+/* trace_properties's ray kernel (porerad/properties.py:54-307) for one seed
+ * on the x-oriented solid of geometry g: n_rays beam rays along +x enter at
+ * (0, u ny, v nz); at a solid surface a ray is absorbed with probability
+ * epsilon, else reflected into a Lambertian direction (at most max_bounces
+ * reflections; restart_hit_point 0 = the "article" rule, 1 = "hit_point").
+ * Per ray the first path length: event_len (solid hit) or censored_len
+ * (else), -1 in the other array (host arrays of n_rays).  counters =
+ * {surface interactions, reflections}; sums = {sum mu, sum mu^2} of the
+ * cosines between incoming and reflected directions. */
+int prh_mc_properties(const prh_geometry *g, int64_t n_rays, uint32_t seed, double epsilon, int32_t max_bounces,
+                      int32_t restart_hit_point, double *event_len, double *censored_len, int64_t counters[2],
+                      double sums[2]);
+/* trace_penetration's ray kernel (porerad/penetration.py:54-305): Lambertian
+ * rays about +x through x = 0; counts = {reflected (left through x = 0),
+ * transmitted (through x = nx), absorbed, stopped at the step or reflection
+ * limit}. */
+int prh_mc_penetration(const prh_geometry *g, int64_t n_rays, uint32_t seed, double epsilon, int32_t max_bounces,
+                       int32_t restart_hit_point, int64_t counts[4]);
+
 /* Origins and directions ((p1-p0)*rays_per_patch x 3, FP64) of the seeded
  * Lambertian rays of patches [p0, p1).  Replaces generate_rays,
  * porerad/view_factors.py:135-182. */

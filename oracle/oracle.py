@@ -22,6 +22,13 @@ Pinning of each function (where the check lives):
                                   results/prescribed_field.csv to round-off;
                                   closed form of the empty box (parallel
                                   plates); isothermal zero net power
+  mc_properties / mc_penetration  archived results/radiative_properties.csv
+                                  (beta to round-off, omega and g within
+                                  their standard errors) and
+                                  results/penetration_response.csv (R, T, A
+                                  within their standard errors); closed forms
+                                  of the empty slab and of a solid wall
+                                  (tests/test_oracle_mc.py)
 """
 
 from __future__ import annotations
@@ -74,6 +81,9 @@ def lib():
         L.orc_raw_rmatvec.argtypes = [P, P, P, i64, i64, P, P]
         L.orc_reflection_series_csr.argtypes = [P, P, P, i64, P, P, P, ctypes.c_double, i64, P, P, P, P]
         L.orc_reflection_series_csr.restype = i64
+        d, u32, i32 = ctypes.c_double, ctypes.c_uint32, ctypes.c_int
+        L.orc_mc_properties.argtypes = [P, i64, i64, i64, i64, u32, d, i32, i32, P, P, P, P, P, P]
+        L.orc_mc_penetration.argtypes = [P, i64, i64, i64, i64, u32, d, i32, i32, P]
         _lib = L
     return _lib
 
@@ -256,3 +266,29 @@ def count_sha256(origin, receiver, count, rays_per_origin) -> str:
 
 def orient_along_x(array: np.ndarray, axis: str) -> np.ndarray:
     return np.ascontiguousarray(np.moveaxis(np.asarray(array), {"x": 0, "y": 1, "z": 2}[axis], 0))
+
+
+def mc_properties(solid_x: np.ndarray, n_rays: int, seed: int, epsilon: float, max_bounces: int = 1000,
+                  restart: str = "article") -> dict:
+    """Per-ray first path lengths and reflection statistics of trace_properties's
+    ray kernel (porerad/properties.py:223-287) for one seed."""
+
+    solid = np.ascontiguousarray(solid_x, dtype=np.uint8)
+    n = int(n_rays)
+    ev, ce, m1, m2 = (np.empty(n) for _ in range(4))
+    inter, scat = np.empty(n, np.int64), np.empty(n, np.int64)
+    lib().orc_mc_properties(_p(solid), *solid.shape, n, int(seed) & 0xFFFFFFFF, float(epsilon), int(max_bounces),
+                            1 if restart == "hit_point" else 0, _p(ev), _p(ce), _p(inter), _p(scat), _p(m1), _p(m2))
+    return {"event": ev, "censored": ce, "interactions": inter, "scattered": scat, "mu": m1, "mu2": m2}
+
+
+def mc_penetration(solid_x: np.ndarray, n_rays: int, seed: int, epsilon: float, max_bounces: int = 1000,
+                   restart: str = "article") -> np.ndarray:
+    """Per-ray fate of trace_penetration's ray kernel (porerad/penetration.py:200-272):
+    0 reflected, 1 transmitted, 2 absorbed, 3 stopped."""
+
+    solid = np.ascontiguousarray(solid_x, dtype=np.uint8)
+    fate = np.empty(int(n_rays), np.uint8)
+    lib().orc_mc_penetration(_p(solid), *solid.shape, int(n_rays), int(seed) & 0xFFFFFFFF, float(epsilon),
+                             int(max_bounces), 1 if restart == "hit_point" else 0, _p(fate))
+    return fate

@@ -570,3 +570,205 @@ int64_t orc_reflection_series_csr(const int64_t *rowptr, const int32_t *col, con
     }
     return -1;
 }
+
+/* ------------------------------------------------------------------------ *
+ * Multi-bounce Monte Carlo (porerad/properties.py:54-307 trace_properties,
+ * porerad/penetration.py:54-305 trace_penetration): the Taichi ray kernels
+ * restated in C, one ray at a time, with glibc cos/sin/sqrt.
+ * ------------------------------------------------------------------------ */
+
+/* _hash_u32, properties.py:81-88 / penetration.py:58-65 */
+static inline uint32_t mc_hash(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
+}
+
+/* _rand01, properties.py:90-93 */
+static inline double mc_rand01(uint32_t *state)
+{
+    *state = mc_hash(*state + 0x9E3779B9u);
+    return (double)(*state & 0x00FFFFFFu) / (double)0x01000000u;
+}
+
+typedef struct {
+    int fate; /* 0 solid, 1 exit x = 0, 2 exit x = nx, 3 step cap */
+    double t, n[3];
+} mc_hit;
+
+/* _trace, properties.py:96-194 / penetration.py:73-171 */
+static mc_hit mc_trace(const uint8_t *solid, int64_t nx, int64_t ny, int64_t nz, double ox, double oy, double oz,
+                       double dx0, double dy0, double dz0)
+{
+    const double norm = sqrt(dx0 * dx0 + dy0 * dy0 + dz0 * dz0);
+    const double d[3] = {dx0 / norm, dy0 / norm, dz0 / norm};
+    const double o[3] = {ox, oy, oz};
+    int64_t idx[3], step[3] = {0, 0, 0};
+    double tmax[3] = {1.0e300, 1.0e300, 1.0e300}, tdelta[3] = {1.0e300, 1.0e300, 1.0e300};
+    for (int a = 0; a < 3; ++a) {
+        idx[a] = (int64_t)floor(o[a]);
+        if (d[a] < 0.0 && fabs(o[a] - floor(o[a] + 0.5)) <= 1.0e-12) idx[a] -= 1;
+        const double safe = fabs(d[a]) > 0.0 ? d[a] : 1.0;
+        if (d[a] > 0.0) {
+            step[a] = 1;
+            tmax[a] = ((double)(idx[a] + 1) - o[a]) / safe;
+            tdelta[a] = 1.0 / safe;
+        } else if (d[a] < 0.0) {
+            step[a] = -1;
+            tmax[a] = ((double)idx[a] - o[a]) / safe;
+            tdelta[a] = -1.0 / safe;
+        }
+    }
+    mc_hit h = {-1, 0.0, {0.0, 0.0, 0.0}};
+    const int64_t max_steps = 10 * nx * ny * nz + 10;
+    int64_t steps = 0;
+    while (h.fate == -1 && steps < max_steps) {
+        steps += 1;
+        if (idx[0] < 0) {
+            h.fate = 1;
+        } else if (idx[0] >= nx) {
+            h.fate = 2;
+        } else if (solid[(idx[0] * ny + py_imod(idx[1], ny)) * nz + py_imod(idx[2], nz)] != 0) {
+            h.fate = 0;
+        } else {
+            const double t_next = fmin(tmax[0], fmin(tmax[1], tmax[2]));
+            int crossed[3];
+            for (int a = 0; a < 3; ++a) crossed[a] = fabs(tmax[a] - t_next) <= 1.0e-12;
+            for (int a = 0; a < 3; ++a) {
+                if (!crossed[a]) continue;
+                idx[a] += step[a];
+                tmax[a] += tdelta[a];
+                int earlier = 0;
+                for (int b = 0; b < a; ++b) earlier |= crossed[b];
+                if (!earlier) {
+                    h.n[0] = h.n[1] = h.n[2] = 0.0;
+                    h.n[a] = -(double)step[a];
+                }
+            }
+            h.t = t_next;
+        }
+    }
+    if (h.fate == -1) h.fate = 3;
+    return h;
+}
+
+/* _lambertian, properties.py:196-221 / penetration.py:173-198 */
+static void mc_lambertian(uint32_t *state, const double n[3], double d[3])
+{
+    const double xi1 = mc_rand01(state);
+    const double xi2 = mc_rand01(state);
+    double hx = 1.0, hy = 0.0, hz = 0.0;
+    if (fabs(n[0]) > 0.9) {
+        hx = 0.0;
+        hy = 1.0;
+    }
+    double u[3] = {hy * n[2] - hz * n[1], hz * n[0] - hx * n[2], hx * n[1] - hy * n[0]};
+    const double un = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    for (int a = 0; a < 3; ++a) u[a] /= un;
+    const double v[3] = {n[1] * u[2] - n[2] * u[1], n[2] * u[0] - n[0] * u[2], n[0] * u[1] - n[1] * u[0]};
+    const double radius = sqrt(xi1);
+    const double phi = 2.0 * 3.141592653589793 * xi2;
+    const double cx = radius * cos(phi), cy = radius * sin(phi), cz = sqrt(fmax(0.0, 1.0 - xi1));
+    for (int a = 0; a < 3; ++a) d[a] = cx * u[a] + cy * v[a] + cz * n[a];
+}
+
+/* The reflection loop of both kernels from a solid hit; returns 0 absorbed,
+ * 1/2 exit through x = 0 / x = nx, 3 stopped. */
+static int mc_reflect(const uint8_t *solid, int64_t nx, int64_t ny, int64_t nz, uint32_t *state, double eps,
+                      int max_bounces, int hit_point, double o[3], double d[3], mc_hit h, int64_t *interactions,
+                      int64_t *scattered, double *s1, double *s2)
+{
+    double n[3] = {h.n[0], h.n[1], h.n[2]}, dist = h.t;
+    int bounces = 0;
+    for (;;) {
+        *interactions += 1;
+        if (mc_rand01(state) < eps) return 0;
+        const double p[3] = {d[0], d[1], d[2]};
+        mc_lambertian(state, n, d);
+        const double mu = p[0] * d[0] + p[1] * d[1] + p[2] * d[2];
+        *scattered += 1;
+        *s1 += mu;
+        *s2 += mu * mu;
+        bounces += 1;
+        if (bounces >= max_bounces) return 3;
+        for (int a = 0; a < 3; ++a) o[a] = hit_point ? o[a] + n[a] * 1.0e-9 : o[a] + p[a] * dist + n[a] * 1.0e-9;
+        const mc_hit g = mc_trace(solid, nx, ny, nz, o[0], o[1], o[2], d[0], d[1], d[2]);
+        if (g.fate != 0) return g.fate;
+        dist = g.t;
+        for (int a = 0; a < 3; ++a) n[a] = g.n[a];
+        if (n[0] * n[0] + n[1] * n[1] + n[2] * n[2] <= 1.0e-24)
+            for (int a = 0; a < 3; ++a) n[a] = -d[a];
+        for (int a = 0; a < 3; ++a) o[a] = o[a] + d[a] * dist;
+    }
+}
+
+static uint32_t mc_ray_state(uint32_t seed, int64_t ray) { return mc_hash(seed + (uint32_t)(ray + 1) * 747796405u); }
+
+/* trace_properties's kernel (properties.py:223-287): per ray the event or
+ * censored first path length (-1 in the other), interactions, reflections
+ * and the sums of mu, mu^2. */
+void orc_mc_properties(const uint8_t *solid, int64_t nx, int64_t ny, int64_t nz, int64_t n_rays, uint32_t seed,
+                       double eps, int max_bounces, int hit_point, double *event_len, double *censored_len,
+                       int64_t *interactions, int64_t *scattered, double *mu1, double *mu2)
+{
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t ray = 0; ray < n_rays; ++ray) {
+        uint32_t state = mc_ray_state(seed, ray);
+        const double ry = mc_rand01(&state), rz = mc_rand01(&state);
+        double o[3] = {0.0, ry * (double)ny, rz * (double)nz}, d[3] = {1.0, 0.0, 0.0};
+        mc_hit h = mc_trace(solid, nx, ny, nz, o[0], o[1], o[2], d[0], d[1], d[2]);
+        int64_t in = 0, sc = 0;
+        double s1 = 0.0, s2 = 0.0;
+        if (h.fate == 0) {
+            if (h.n[0] * h.n[0] + h.n[1] * h.n[1] + h.n[2] * h.n[2] <= 1.0e-24)
+                for (int a = 0; a < 3; ++a) h.n[a] = -d[a];
+            event_len[ray] = h.t;
+            censored_len[ray] = -1.0;
+            if (hit_point)
+                for (int a = 0; a < 3; ++a) o[a] = o[a] + d[a] * h.t;
+            (void)mc_reflect(solid, nx, ny, nz, &state, eps, max_bounces, hit_point, o, d, h, &in, &sc, &s1, &s2);
+        } else {
+            event_len[ray] = -1.0;
+            censored_len[ray] = h.t;
+        }
+        interactions[ray] = in;
+        scattered[ray] = sc;
+        mu1[ray] = s1;
+        mu2[ray] = s2;
+    }
+}
+
+/* trace_penetration's kernel (penetration.py:200-272): per ray 0 reflected,
+ * 1 transmitted, 2 absorbed, 3 stopped. */
+void orc_mc_penetration(const uint8_t *solid, int64_t nx, int64_t ny, int64_t nz, int64_t n_rays, uint32_t seed,
+                        double eps, int max_bounces, int hit_point, uint8_t *fate)
+{
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t ray = 0; ray < n_rays; ++ray) {
+        uint32_t state = mc_ray_state(seed, ray);
+        const double ry = mc_rand01(&state), rz = mc_rand01(&state);
+        double o[3] = {0.0, ry * (double)ny, rz * (double)nz}, d[3];
+        const double normal[3] = {1.0, 0.0, 0.0};
+        mc_lambertian(&state, normal, d);
+        mc_hit h = mc_trace(solid, nx, ny, nz, o[0], o[1], o[2], d[0], d[1], d[2]);
+        int code;
+        if (h.fate != 0) {
+            code = h.fate == 1 ? 0 : (h.fate == 2 ? 1 : 3);
+        } else {
+            if (h.n[0] * h.n[0] + h.n[1] * h.n[1] + h.n[2] * h.n[2] <= 1.0e-24)
+                for (int a = 0; a < 3; ++a) h.n[a] = -d[a];
+            if (hit_point)
+                for (int a = 0; a < 3; ++a) o[a] = o[a] + d[a] * h.t;
+            int64_t in = 0, sc = 0;
+            double s1 = 0.0, s2 = 0.0;
+            const int f = mc_reflect(solid, nx, ny, nz, &state, eps, max_bounces, hit_point, o, d, h, &in, &sc, &s1,
+                                     &s2);
+            code = f == 0 ? 2 : (f == 1 ? 0 : (f == 2 ? 1 : 3));
+        }
+        fate[ray] = (uint8_t)code;
+    }
+}

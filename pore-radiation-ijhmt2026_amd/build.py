@@ -31,6 +31,7 @@ SOURCES = [
     "prh_operator.hip",
     "prh_assemble.hip",
     "prh_balance.hip",
+    "prh_mc.hip",
     "prh_batch.hip",
     "prh_coupled.hip",
 ]

@@ -53,6 +53,8 @@ SIGNATURES = {
     "prh_geometry_sizes": ([_P, _P, _P], ctypes.c_int),
     "prh_geometry_patch_table": ([_P, _P, _P, _P, _P], ctypes.c_int),
     "prh_geometry_destroy": ([_P], None),
+    "prh_mc_properties": ([_P, _I64, ctypes.c_uint32, _F64, _I32, _I32, _P, _P, _P, _P], ctypes.c_int),
+    "prh_mc_penetration": ([_P, _I64, ctypes.c_uint32, _F64, _I32, _I32, _P], ctypes.c_int),
     "prh_generate_rays": ([_P, _I64, _I64, _I32, _P, _P, _P], ctypes.c_int),
     "prh_trace_rays": ([_P, _P, _P, _I64, _P, _P], ctypes.c_int),
     "prh_trace_counts": ([_P, _I64, _I64, _I32, _I32, _P, _P, _P, _P], ctypes.c_int),

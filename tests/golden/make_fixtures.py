@@ -7,6 +7,8 @@ Reads (never imports or runs) the reference repository's shipped data:
   results/prescribed_field.csv    archived FP64 channels q_h, q_c, q_i
   results/coupled.csv             archived coupled-solve outputs
   results/fullpath_crosscheck.csv archived first-hit radiosity channels
+  results/radiative_properties.csv, results/penetration_response.csv and
+  their results/hit_point_restart/ twins: archived Monte Carlo statistics
 
 and writes, next to this script:
   geometries.npz  bit-packed phase arrays (np.packbits of the C-order uint8
@@ -15,7 +17,9 @@ and writes, next to this script:
   cases.json      hash cases, prescribed-field cases with their archived rows,
                   canonical coupled rows and the full-path cross-check rows;
                   every prescribed-field case and every coupled case whose
-                  microstructure is shipped, with its archived result rows
+                  microstructure is shipped, with its archived result rows;
+                  the Monte Carlo rows (both restart rules) of those
+                  microstructures
 
 Run once in a container that holds the reference:
     python tests/golden/make_fixtures.py /path/to/reference
@@ -150,11 +154,23 @@ def main(reference: str) -> None:
         arrays[f"{name}__bits"] = np.packbits(phases.ravel())
         arrays[f"{name}__shape"] = np.asarray(phases.shape, dtype=np.int64)
         arrays[f"{name}__sha256"] = np.frombuffer(hashlib.sha256(phases.tobytes()).digest(), dtype=np.uint8)
+    mc = {"mc_properties": [], "mc_penetration": []}
+    for restart, sub in (("article", ""), ("hit_point", "hit_point_restart")):
+        for key, table, fields in (
+                ("mc_properties", "radiative_properties.csv", ("beta_per_cell", "beta_se_per_cell", "omega", "omega_se",
+                                                                "g", "g_se")),
+                ("mc_penetration", "penetration_response.csv", ("reflectance", "transmittance", "absorptance"))):
+            for r in read_csv(os.path.join(reference, "results", sub, table)):
+                if r["microstructure"] not in names:
+                    continue
+                mc[key].append({"restart": restart, "microstructure": r["microstructure"],
+                                "transfer_axis": r["transfer_axis"], "epsilon": float(r["epsilon"]),
+                                "n_rays": int(r["n_rays"]), **{f: float(r[f]) for f in fields}})
     np.savez_compressed(os.path.join(HERE, "geometries.npz"), **arrays)
     with open(os.path.join(HERE, "cases.json"), "w") as handle:
         json.dump({"hash_cases": hash_cases, "prescribed_field": prescribed, "coupled_canonical": coupled,
                    "fullpath_crosscheck": crosscheck, "prescribed_cases": prescribed_cases,
-                   "coupled_cases": coupled_cases}, handle, indent=1)
+                   "coupled_cases": coupled_cases, **mc}, handle, indent=1)
     print(f"wrote {len(names)} geometries, {len(hash_cases)} hash cases, {len(prescribed)} prescribed cases")
 
 
