@@ -467,6 +467,42 @@ def cpu_baseline(phases, R, seed, budget_s=12.0):
     }
 
 
+def run_monte_carlo(phases, with_cpu: bool) -> dict:
+    """Multi-bounce Monte Carlo statistics of the foam (properties.py / penetration.py):
+    identify_properties (2 seeds x 200 000 beam rays) and penetration_response
+    (2 seeds x 500 000 Lambertian rays) at eps 0.6 along x, timed on the device after
+    one warm call (the cos/sin table is built then); the CPU oracle on the same rays."""
+
+    from porerad import penetration, properties
+
+    eps = 0.6
+    properties.identify_properties(phases, "x", eps, n_rays=1000)  # table and kernels warm
+    t = time.perf_counter()
+    props = properties.identify_properties(phases, "x", eps)
+    t_props = time.perf_counter() - t
+    t = time.perf_counter()
+    pen = penetration.penetration_response(phases, "x", eps)
+    t_pen = time.perf_counter() - t
+    out = {"workload": "voronoi foam N=128^3, axis x, eps 0.6, 'article' restart",
+           "properties": {"rays": props["n_rays"], "s": t_props, "rays_per_s": props["n_rays"] / t_props,
+                          "beta": props["beta"], "omega": props["omega"], "g": props["g"]},
+           "penetration": {"rays": pen["n_rays"], "s": t_pen, "rays_per_s": pen["n_rays"] / t_pen,
+                           "reflectance": pen["reflectance"], "transmittance": pen["transmittance"],
+                           "absorptance": pen["absorptance"]}}
+    if with_cpu:
+        from oracle import oracle as O
+
+        solid = (np.asarray(phases) == 1).astype(np.uint8)
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t = time.perf_counter()
+        for seed in penetration.SEEDS:
+            O.mc_penetration(solid, penetration.N_RAYS, seed, eps)
+        t_cpu = time.perf_counter() - t
+        out["cpu_penetration"] = {"rays_per_s": 2 * penetration.N_RAYS / t_cpu, "cores": threads, "kind": "port",
+                                  "s": t_cpu}
+    return out
+
+
 def copy_probe(n_bytes: int = 1 << 30) -> None:
     """One plain device copy of n_bytes (a torch elementwise copy kernel): the
     reference point for scaling the FETCH_SIZE / WRITE_SIZE counters."""
@@ -586,6 +622,13 @@ def main() -> None:
             f"{cb['coupled']['latency_s']} s (alone: {cb['coupled']['alone_latency_s']} s), "
             f"Newton updates {cb['coupled']['newton_updates']}, max flux error vs FP64 "
             f"{max(max(v.values()) for v in cb['coupled']['max_rel_flux_error_vs_fp64'].values()):.2e}")
+    if not args.no_radiosity:
+        mc = run_monte_carlo(phases, not args.no_cpu)
+        record["monte_carlo"] = mc
+        log(f"monte carlo: properties {mc['properties']['rays_per_s']:.3e} rays/s, penetration "
+            f"{mc['penetration']['rays_per_s']:.3e} rays/s"
+            + (f" (CPU oracle {mc['cpu_penetration']['rays_per_s']:.3e} rays/s on "
+               f"{mc['cpu_penetration']['cores']} threads)" if "cpu_penetration" in mc else ""))
     if not args.no_cpu:
         record["cpu_baseline"] = cpu_baseline(phases, R, seed)
         log(f"cpu baseline {record['cpu_baseline']['value']:.3e} rays/s")
