@@ -25,6 +25,10 @@
  *   prh_operator_apply        raw_matvec / raw_rmatvec / symmetric_matvec /
  *                             matvec / rmatvec, porerad/exchange.py:82-122
  *   prh_radiosity             reflection_series, porerad/exchange.py:217-245
+ *   prh_scalar_counts         scalar_descriptors / interface_area /
+ *                             mean_pore_chords / pore_connectivity /
+ *                             interface_face_counts, porerad/descriptors.py:38-79,120-123
+ *   prh_periodic_chords       periodic_chords, porerad/descriptors.py:82-109
  */
 #ifndef PORERAD_HIP_H
 #define PORERAD_HIP_H
@@ -66,6 +70,7 @@ typedef struct prh_operator prh_operator;
 typedef struct prh_batch prh_batch;
 typedef struct prh_coupled prh_coupled;
 typedef struct prh_trace_batch prh_trace_batch;
+typedef struct prh_phase_field prh_phase_field;
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *prh_last_error(void);
@@ -136,6 +141,49 @@ int prh_mc_properties(const prh_geometry *g, int64_t n_rays, uint32_t seed, doub
  * limit}. */
 int prh_mc_penetration(const prh_geometry *g, int64_t n_rays, uint32_t seed, double epsilon, int32_t max_bounces,
                        int32_t restart_hit_point, int64_t counts[4]);
+
+/* ---- morphology descriptors (porerad/descriptors.py) ----------------- */
+/* A phase field is a copy of a uint8 array (nx, ny, nz), x-major, 1 = solid,
+ * 0 = pore (the reference's phase arrays), at most 2^31 - 1 cells, on the
+ * current device.  The library returns the integer quantities the descriptors are
+ * made of; ratios, means and quantiles are formed from them by the caller. */
+int prh_phase_field_create(const uint8_t *phases, int32_t nx, int32_t ny, int32_t nz, prh_phase_field **out);
+int prh_phase_field_read(const prh_phase_field *f, uint8_t *phases);
+void prh_phase_field_destroy(prh_phase_field *f);
+
+/* Indices of prh_scalar_counts' output. */
+#define PRH_DESC_CELLS 0
+#define PRH_DESC_SOLID 1              /* cells of phase 1 */
+#define PRH_DESC_PORE 2               /* cells of phase 0 */
+#define PRH_DESC_BOX_FACES_X 3        /* +0/1/2: faces inside the box between cells whose
+                                         solid indicators differ (interface_area,
+                                         porerad/descriptors.py:38-43) */
+#define PRH_DESC_PORE_RUNS_X 6        /* +0/1/2: pore runs ended by the box faces
+                                         (mean_pore_chords, :46-48) */
+#define PRH_DESC_PERIODIC_FACES_X 9   /* +0/1/2: pore indicator differs from the next cell
+                                         with wrap-around (interface_face_counts, :120-123) */
+#define PRH_DESC_LARGEST_CLUSTER 12   /* largest 26-connected pore cluster, cells */
+#define PRH_DESC_CLUSTERS 13          /* number of 26-connected pore clusters */
+#define PRH_DESC_PERCOLATES 14        /* bit a: one cluster touches both faces of axis a */
+#define PRH_DESC_COUNT 15
+
+/* Counts of scalar_descriptors (porerad/descriptors.py:63-79) and
+ * pore_connectivity (:51-60, scipy.ndimage.label with a full 3x3x3
+ * structure).  connectivity 0 skips the cluster labelling (its three entries
+ * are then -1). */
+int prh_scalar_counts(const prh_phase_field *f, int32_t connectivity, int64_t counts[PRH_DESC_COUNT]);
+
+/* periodic_chords along axis 0/1/2 (porerad/descriptors.py:82-109): the chord
+ * lengths in the reference's order (lines in row-major order of the other two
+ * axes, runs along each line; a line whose first and last cells are pore and
+ * that is not all pore joins its end runs into one chord placed after all
+ * others, in line order), their count, the number of all-pore lines, and the
+ * chord-count histogram over lengths 0..hist_len-1 (longer chords are not
+ * binned; the reference bins to N + 1 with N = nx).  chords NULL returns the
+ * counts only; more chords than capacity is PRH_ERR_CAPACITY with *n_chords
+ * set.  Capacity lines * ceil(length / 2) always suffices. */
+int prh_periodic_chords(const prh_phase_field *f, int32_t axis, int32_t *chords, int64_t capacity, int64_t *n_chords,
+                        int64_t *n_all_pore, int64_t *histogram, int32_t hist_len);
 
 /* Origins and directions ((p1-p0)*rays_per_patch x 3, FP64) of the seeded
  * Lambertian rays of patches [p0, p1).  Replaces generate_rays,

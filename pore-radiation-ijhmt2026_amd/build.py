@@ -32,6 +32,7 @@ SOURCES = [
     "prh_assemble.hip",
     "prh_balance.hip",
     "prh_mc.hip",
+    "prh_descriptors.hip",
     "prh_batch.hip",
     "prh_coupled.hip",
 ]

@@ -55,6 +55,11 @@ SIGNATURES = {
     "prh_geometry_destroy": ([_P], None),
     "prh_mc_properties": ([_P, _I64, ctypes.c_uint32, _F64, _I32, _I32, _P, _P, _P, _P], ctypes.c_int),
     "prh_mc_penetration": ([_P, _I64, ctypes.c_uint32, _F64, _I32, _I32, _P], ctypes.c_int),
+    "prh_phase_field_create": ([_P, _I32, _I32, _I32, _P], ctypes.c_int),
+    "prh_phase_field_read": ([_P, _P], ctypes.c_int),
+    "prh_phase_field_destroy": ([_P], None),
+    "prh_scalar_counts": ([_P, _I32, _P], ctypes.c_int),
+    "prh_periodic_chords": ([_P, _I32, _P, _I64, _P, _P, _P, _I32], ctypes.c_int),
     "prh_generate_rays": ([_P, _I64, _I64, _I32, _P, _P, _P], ctypes.c_int),
     "prh_trace_rays": ([_P, _P, _P, _I64, _P, _P], ctypes.c_int),
     "prh_trace_counts": ([_P, _I64, _I64, _I32, _I32, _P, _P, _P, _P], ctypes.c_int),
@@ -261,6 +266,69 @@ class Geometry:
         if getattr(self, "handle", None) is not None and self.handle.value:
             self._lib.prh_geometry_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+DESC_CELLS, DESC_SOLID, DESC_PORE = 0, 1, 2
+DESC_BOX_FACES, DESC_PORE_RUNS, DESC_PERIODIC_FACES = 3, 6, 9
+DESC_LARGEST_CLUSTER, DESC_CLUSTERS, DESC_PERCOLATES = 12, 13, 14
+DESC_COUNT = 15
+
+
+class PhaseField:
+    """Owner of a device copy of a phase array (``prh_phase_field``)."""
+
+    def __init__(self, phases: np.ndarray) -> None:
+        p = np.ascontiguousarray(phases, dtype=np.uint8)
+        if p.ndim != 3:
+            raise ValueError("expected a 3-D phase array")
+        self.shape = tuple(int(s) for s in p.shape)
+        handle = ctypes.c_void_p()
+        L = lib()
+        check(L.prh_phase_field_create(ptr(p), *self.shape, ctypes.byref(handle)), "prh_phase_field_create")
+        self._lib = L
+        self.handle = handle
+
+    def read(self) -> np.ndarray:
+        out = np.empty(self.shape, np.uint8)
+        check(self._lib.prh_phase_field_read(self.handle, ptr(out)), "prh_phase_field_read")
+        return out
+
+    def scalar_counts(self, connectivity: bool = True) -> np.ndarray:
+        """The int64 counts of prh_scalar_counts (indices DESC_*)."""
+
+        out = np.zeros(DESC_COUNT, np.int64)
+        check(self._lib.prh_scalar_counts(self.handle, 1 if connectivity else 0, ptr(out)), "prh_scalar_counts")
+        return out
+
+    def periodic_chords(self, axis: int, hist_len: int):
+        """(chords int32 in the reference's order, number of all-pore lines, histogram int64[hist_len])."""
+
+        length = self.shape[axis]
+        lines = int(np.prod(self.shape)) // length
+        capacity = lines * ((length + 1) // 2)
+        chords = np.empty(max(capacity, 1), np.int32)
+        n, allp = ctypes.c_int64(), ctypes.c_int64()
+        hist = np.zeros(int(hist_len), np.int64)
+        check(self._lib.prh_periodic_chords(self.handle, int(axis), ptr(chords), int(capacity), ctypes.byref(n),
+                                            ctypes.byref(allp), ptr(hist), int(hist_len)), "prh_periodic_chords")
+        return chords[: n.value].copy(), int(allp.value), hist
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self._lib.prh_phase_field_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self) -> "PhaseField":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
 
     def __del__(self):  # pragma: no cover - interpreter shutdown order
         try:

@@ -9,6 +9,8 @@ Reads (never imports or runs) the reference repository's shipped data:
   results/fullpath_crosscheck.csv archived first-hit radiosity channels
   results/radiative_properties.csv, results/penetration_response.csv and
   their results/hit_point_restart/ twins: archived Monte Carlo statistics
+  data/microstructures.csv        build parameters, scalar descriptors and
+                                  phase SHA-256 of every microstructure
 
 and writes, next to this script:
   geometries.npz  bit-packed phase arrays (np.packbits of the C-order uint8
@@ -19,7 +21,7 @@ and writes, next to this script:
                   every prescribed-field case and every coupled case whose
                   microstructure is shipped, with its archived result rows;
                   the Monte Carlo rows (both restart rules) of those
-                  microstructures
+                  microstructures; every row of the microstructure table
 
 Run once in a container that holds the reference:
     python tests/golden/make_fixtures.py /path/to/reference
@@ -166,11 +168,17 @@ def main(reference: str) -> None:
                 mc[key].append({"restart": restart, "microstructure": r["microstructure"],
                                 "transfer_axis": r["transfer_axis"], "epsilon": float(r["epsilon"]),
                                 "n_rays": int(r["n_rays"]), **{f: float(r[f]) for f in fields}})
+    float_cols = ("target_phi", "realized_phi", "interfacial_area_density_per_cell", "mean_beam_length_cells",
+                  "mean_pore_chord_x_cells", "mean_pore_chord_y_cells", "mean_pore_chord_z_cells")
+    micro = [{"id": r["id"], "structure": r["structure"], "set": r["set"], "N": int(r["N"]),
+              "seed": int(r["seed"]) if r["seed"] != "" else None, "medoid": r["medoid"],
+              "phase_sha256": r["phase_sha256"], **{k: float(r[k]) for k in float_cols}}
+             for r in read_csv(os.path.join(reference, "data", "microstructures.csv"))]
     np.savez_compressed(os.path.join(HERE, "geometries.npz"), **arrays)
     with open(os.path.join(HERE, "cases.json"), "w") as handle:
         json.dump({"hash_cases": hash_cases, "prescribed_field": prescribed, "coupled_canonical": coupled,
                    "fullpath_crosscheck": crosscheck, "prescribed_cases": prescribed_cases,
-                   "coupled_cases": coupled_cases, **mc}, handle, indent=1)
+                   "coupled_cases": coupled_cases, **mc, "microstructures": micro}, handle, indent=1)
     print(f"wrote {len(names)} geometries, {len(hash_cases)} hash cases, {len(prescribed)} prescribed cases")
 
 
