@@ -29,6 +29,10 @@
  *                             mean_pore_chords / pore_connectivity /
  *                             interface_face_counts, porerad/descriptors.py:38-79,120-123
  *   prh_periodic_chords       periodic_chords, porerad/descriptors.py:82-109
+ *   prh_stamp_shapes          shape_mask (short shapes), porerad/geometry.py:106-116
+ *   prh_stamp_offsets         _long_cylinder, porerad/geometry.py:63-86
+ *   prh_threshold_field       lamellar_field smoothing + threshold,
+ *                             porerad/geometry.py:229-238
  */
 #ifndef PORERAD_HIP_H
 #define PORERAD_HIP_H
@@ -184,6 +188,35 @@ int prh_scalar_counts(const prh_phase_field *f, int32_t connectivity, int64_t co
  * set.  Capacity lines * ceil(length / 2) always suffices. */
 int prh_periodic_chords(const prh_phase_field *f, int32_t axis, int32_t *chords, int64_t capacity, int64_t *n_chords,
                         int64_t *n_all_pore, int64_t *histogram, int32_t hist_len);
+
+/* ---- microstructure generators (porerad/geometry.py) ----------------- */
+/* prh_phase_field_create with phases NULL gives a field of zeros.  Fill,
+ * copy (same shape and device) and count the cells of one value (the solid
+ * fraction of _bisect_radius, porerad/geometry.py:120-140). */
+int prh_phase_field_fill(prh_phase_field *f, uint8_t value);
+int prh_phase_field_copy(prh_phase_field *dst, const prh_phase_field *src);
+int prh_phase_field_count(const prh_phase_field *f, uint8_t value, int64_t *count);
+
+/* shape_mask's short shapes (porerad/geometry.py:106-116) OR-ed into a cubic
+ * field (cells set to 1).  shapes: n_shapes x 16 doubles = centre (3), radii
+ * (3; r_z the half length of a cylinder), R^T row-major (9; R's columns are
+ * the local axes), reach (offsets beyond it on any axis are outside the
+ * shape); kinds: 0 ellipsoid, 1 cylinder.  Membership is evaluated with the
+ * reference's FP64 operations (R^T x as an FMA chain, as OpenBLAS forms it). */
+int prh_stamp_shapes(prh_phase_field *f, int32_t n_shapes, const double *shapes, const int32_t *kinds);
+
+/* _long_cylinder's stamping (porerad/geometry.py:63-86) and the roll of
+ * shape_mask (:100-105): cells (node + offset + shift) mod N set to 1.
+ * nodes: n_nodes x 3 in [0, N); offsets: n_offsets x 3 in (-N, N); shift
+ * in [0, N). */
+int prh_stamp_offsets(prh_phase_field *f, int32_t n_nodes, const int32_t *nodes, int32_t n_offsets,
+                      const int32_t *offsets, const int32_t shift[3]);
+
+/* threshold (porerad/geometry.py:235-238): the field (N^3 FP64, host) plus,
+ * if noise is not NULL, lamellar_field's seven-point smoothing of noise
+ * (:229-232), thresholded at np.quantile(values, q) (linear rule; the value
+ * comes back in *quantile): cells >= it become 1, the others 0. */
+int prh_threshold_field(prh_phase_field *f, const double *field, const double *noise, double q, double *quantile);
 
 /* Origins and directions ((p1-p0)*rays_per_patch x 3, FP64) of the seeded
  * Lambertian rays of patches [p0, p1).  Replaces generate_rays,

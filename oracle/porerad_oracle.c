@@ -772,3 +772,68 @@ void orc_mc_penetration(const uint8_t *solid, int64_t nx, int64_t ny, int64_t nz
         fate[ray] = (uint8_t)code;
     }
 }
+
+/* ------------------------------------------------------------------ geometry stamping
+ * Short shapes of shape_mask (porerad/geometry.py:106-116): per cell the
+ * minimum-image offsets ((i - c) + N/2) mod N - N/2 with NumPy's float
+ * remainder (fmod, plus N when the result is negative), the local
+ * coordinates R^T x as OpenBLAS's dgemm forms them (an FMA chain over
+ * k = 0, 1, 2 starting from a rounded product) and the membership test with
+ * every division, square and sum rounded on its own.
+ * params per shape: cx, cy, cz, rx, ry, rz, then R^T row-major (Rt[r][k] =
+ * R[k][r]); kinds: 0 ellipsoid, 1 cylinder. */
+static inline double orc_min_image(int64_t i, double c, double N)
+{
+    const double half = N / 2.0;
+    double m = fmod(((double)i - c) + half, N);
+    if (m != 0.0) {
+        if (m < 0.0) m += N;
+    } else {
+        m = 0.0;
+    }
+    return m - half;
+}
+
+void orc_stamp_shapes(uint8_t *phases, int64_t N, int64_t n_shapes, const double *params, const int32_t *kinds)
+{
+    const double Nf = (double)N;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i)
+        for (int64_t s = 0; s < n_shapes; ++s) {
+            const double *p = params + 15 * s;
+            const double *Rt = p + 6;
+            const double x = orc_min_image(i, p[0], Nf);
+            for (int64_t j = 0; j < N; ++j) {
+                const double y = orc_min_image(j, p[1], Nf);
+                for (int64_t k = 0; k < N; ++k) {
+                    const double z = orc_min_image(k, p[2], Nf);
+                    double l[3];
+                    for (int r = 0; r < 3; ++r) l[r] = fma(Rt[3 * r + 2], z, fma(Rt[3 * r + 1], y, Rt[3 * r] * x));
+                    const double a = l[0] / p[3], b = l[1] / p[4];
+                    const double ab = a * a + b * b;
+                    int inside;
+                    if (kinds[s] == 0) {
+                        const double c = l[2] / p[5];
+                        inside = ab + c * c <= 1.0;
+                    } else {
+                        inside = ab <= 1.0 && fabs(l[2]) <= p[5];
+                    }
+                    if (inside) phases[(i * N + j) * N + k] = 1;
+                }
+            }
+        }
+}
+
+/* _long_cylinder's stamping (porerad/geometry.py:63-86) followed by the
+ * whole-cell roll of shape_mask (:100-105): cell (node + offset + shift)
+ * mod N for every node and disk offset. */
+void orc_stamp_offsets(uint8_t *phases, int64_t N, int64_t n_nodes, const int64_t *nodes, int64_t n_off,
+                       const int64_t *offsets, const int64_t shift[3])
+{
+    for (int64_t a = 0; a < n_nodes; ++a)
+        for (int64_t b = 0; b < n_off; ++b) {
+            int64_t c[3];
+            for (int d = 0; d < 3; ++d) c[d] = ((nodes[3 * a + d] + offsets[3 * b + d] + shift[d]) % N + N) % N;
+            phases[(c[0] * N + c[1]) * N + c[2]] = 1;
+        }
+}

@@ -33,6 +33,7 @@ SOURCES = [
     "prh_balance.hip",
     "prh_mc.hip",
     "prh_descriptors.hip",
+    "prh_microstructure.hip",
     "prh_batch.hip",
     "prh_coupled.hip",
 ]

@@ -38,13 +38,6 @@
 
 #include "prh_internal.h"
 
-struct prh_phase_field {
-    int device;
-    int32_t nx, ny, nz;
-    int64_t cells;
-    uint8_t *phases;
-};
-
 namespace prh {
 namespace {
 
@@ -429,7 +422,7 @@ extern "C" {
 
 int prh_phase_field_create(const uint8_t *phases, int32_t nx, int32_t ny, int32_t nz, prh_phase_field **out)
 {
-    PRH_CHECK_ARG(phases && out, "prh_phase_field_create: null pointer");
+    PRH_CHECK_ARG(out, "prh_phase_field_create: null pointer");
     PRH_CHECK_ARG(nx > 0 && ny > 0 && nz > 0, "prh_phase_field_create: extents must be positive");
     const int64_t cells = (int64_t)nx * ny * nz;
     PRH_CHECK_ARG(cells < (int64_t(1) << 31), "prh_phase_field_create: at most 2^31 - 1 cells");
@@ -441,7 +434,9 @@ int prh_phase_field_create(const uint8_t *phases, int32_t nx, int32_t ny, int32_
         set_error("prh_phase_field_create: out of device memory");
         return PRH_ERR_HIP;
     }
-    if (hipMemcpy(f->phases, phases, (size_t)cells, hipMemcpyHostToDevice) != hipSuccess) {
+    const hipError_t e = phases ? hipMemcpy(f->phases, phases, (size_t)cells, hipMemcpyHostToDevice)
+                                : hipMemset(f->phases, 0, (size_t)cells);
+    if (e != hipSuccess) {
         (void)hipFree(f->phases);
         delete f;
         set_error("prh_phase_field_create: upload failed");

@@ -130,3 +130,11 @@ int current_stream(hipStream_t *s);
 struct prh_geometry {
     prh::Geometry g;
 };
+
+// A device copy of a uint8 phase array (nx, ny, nz), x-major, 1 = solid, 0 = pore.
+struct prh_phase_field {
+    int device;
+    int32_t nx, ny, nz;
+    int64_t cells;
+    uint8_t *phases;
+};

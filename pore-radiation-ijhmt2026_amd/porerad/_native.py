@@ -60,6 +60,12 @@ SIGNATURES = {
     "prh_phase_field_destroy": ([_P], None),
     "prh_scalar_counts": ([_P, _I32, _P], ctypes.c_int),
     "prh_periodic_chords": ([_P, _I32, _P, _I64, _P, _P, _P, _I32], ctypes.c_int),
+    "prh_phase_field_fill": ([_P, ctypes.c_uint8], ctypes.c_int),
+    "prh_phase_field_copy": ([_P, _P], ctypes.c_int),
+    "prh_phase_field_count": ([_P, ctypes.c_uint8, _P], ctypes.c_int),
+    "prh_stamp_shapes": ([_P, _I32, _P, _P], ctypes.c_int),
+    "prh_stamp_offsets": ([_P, _I32, _P, _I32, _P, _P], ctypes.c_int),
+    "prh_threshold_field": ([_P, _P, _P, _F64, _P], ctypes.c_int),
     "prh_generate_rays": ([_P, _I64, _I64, _I32, _P, _P, _P], ctypes.c_int),
     "prh_trace_rays": ([_P, _P, _P, _I64, _P, _P], ctypes.c_int),
     "prh_trace_counts": ([_P, _I64, _I64, _I32, _I32, _P, _P, _P, _P], ctypes.c_int),
@@ -283,16 +289,61 @@ DESC_COUNT = 15
 class PhaseField:
     """Owner of a device copy of a phase array (``prh_phase_field``)."""
 
-    def __init__(self, phases: np.ndarray) -> None:
-        p = np.ascontiguousarray(phases, dtype=np.uint8)
-        if p.ndim != 3:
-            raise ValueError("expected a 3-D phase array")
-        self.shape = tuple(int(s) for s in p.shape)
+    def __init__(self, phases: np.ndarray | None = None, shape: tuple[int, int, int] | None = None) -> None:
+        """A copy of ``phases``, or a field of zeros of ``shape``."""
+
+        if phases is not None:
+            p = np.ascontiguousarray(phases, dtype=np.uint8)
+            if p.ndim != 3:
+                raise ValueError("expected a 3-D phase array")
+            self.shape = tuple(int(s) for s in p.shape)
+            data = ptr(p)
+        else:
+            self.shape = tuple(int(s) for s in shape)
+            data = None
         handle = ctypes.c_void_p()
         L = lib()
-        check(L.prh_phase_field_create(ptr(p), *self.shape, ctypes.byref(handle)), "prh_phase_field_create")
+        check(L.prh_phase_field_create(data, *self.shape, ctypes.byref(handle)), "prh_phase_field_create")
         self._lib = L
         self.handle = handle
+
+    def fill(self, value: int) -> None:
+        check(self._lib.prh_phase_field_fill(self.handle, int(value)), "prh_phase_field_fill")
+
+    def copy_from(self, other: "PhaseField") -> None:
+        check(self._lib.prh_phase_field_copy(self.handle, other.handle), "prh_phase_field_copy")
+
+    def count(self, value: int) -> int:
+        n = ctypes.c_int64()
+        check(self._lib.prh_phase_field_count(self.handle, int(value), ctypes.byref(n)), "prh_phase_field_count")
+        return int(n.value)
+
+    def stamp_shapes(self, shapes: np.ndarray, kinds: np.ndarray) -> None:
+        """OR short shapes into the field (rows of 16 doubles, see prh_stamp_shapes)."""
+
+        s = np.ascontiguousarray(shapes, np.float64).reshape(-1, 16)
+        k = np.ascontiguousarray(kinds, np.int32)
+        if len(k) != len(s):
+            raise ValueError("one kind per shape")
+        check(self._lib.prh_stamp_shapes(self.handle, len(k), ptr(s), ptr(k)), "prh_stamp_shapes")
+
+    def stamp_offsets(self, nodes: np.ndarray, offsets: np.ndarray, shift=(0, 0, 0)) -> None:
+        n = np.ascontiguousarray(nodes, np.int32).reshape(-1, 3)
+        o = np.ascontiguousarray(offsets, np.int32).reshape(-1, 3)
+        sh = np.ascontiguousarray(shift, np.int32)
+        check(self._lib.prh_stamp_offsets(self.handle, len(n), ptr(n), len(o), ptr(o), ptr(sh)), "prh_stamp_offsets")
+
+    def threshold(self, field: np.ndarray, q: float, noise: np.ndarray | None = None) -> float:
+        """Cells >= np.quantile(values, q) become 1, the rest 0; values = field (+ smoothed noise)."""
+
+        f = np.ascontiguousarray(field, np.float64)
+        nz = None if noise is None else np.ascontiguousarray(noise, np.float64)
+        if f.shape != self.shape or (nz is not None and nz.shape != self.shape):
+            raise ValueError("field and noise must have the shape of the phase field")
+        out = ctypes.c_double()
+        check(self._lib.prh_threshold_field(self.handle, ptr(f), None if nz is None else ptr(nz), float(q),
+                                            ctypes.byref(out)), "prh_threshold_field")
+        return float(out.value)
 
     def read(self) -> np.ndarray:
         out = np.empty(self.shape, np.uint8)
