@@ -503,6 +503,50 @@ def run_monte_carlo(phases, with_cpu: bool) -> dict:
     return out
 
 
+def run_microstructure(phases, with_cpu: bool) -> dict:
+    """The step before the path (geometry.py / descriptors.py): one core microstructure of
+    each structure at N = 128, phi_s 0.10 (OF seed 0: 32 nodes and 59 struts, three bisection
+    renders; FA seed 2: long-fibre scatter; TL seed 0: layered field threshold) built on
+    the device, and the scalar plus directional descriptors of the foam, each timed after
+    one warm call; the CPU oracle (C stamping on every host thread, NumPy / SciPy
+    descriptors) on the same work."""
+
+    from porerad import descriptors, geometry
+
+    builds = (("OF", 0), ("FA", 2), ("TL", 0))
+    geometry.build("FI", 32, 0.10, 0)  # kernels warm
+    out = {"workload": "core microstructures N=128^3 phi_s 0.10 (OF s0, FA s2, TL s0); descriptors of the foam"}
+    for s, seed in builds:
+        t = time.perf_counter()
+        p = geometry.build(s, 128, 0.10, seed)
+        out[f"build_{s}_s"] = time.perf_counter() - t
+        out[f"build_{s}_phi"] = float(np.mean(p == 1))
+    descriptors.scalar_descriptors(phases)
+    t = time.perf_counter()
+    descriptors.scalar_descriptors(phases)
+    out["scalar_descriptors_s"] = time.perf_counter() - t
+    t = time.perf_counter()
+    descriptors.directional_descriptors(phases)
+    out["directional_descriptors_s"] = time.perf_counter() - t
+    if with_cpu:
+        from oracle import descriptors_oracle as DO
+        from oracle import geometry_oracle as GO
+
+        cpu = {"cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port"}
+        for s, seed in builds:
+            t = time.perf_counter()
+            GO.build(s, 128, 0.10, seed)
+            cpu[f"build_{s}_s"] = time.perf_counter() - t
+        t = time.perf_counter()
+        DO.scalar_descriptors(phases)
+        cpu["scalar_descriptors_s"] = time.perf_counter() - t
+        t = time.perf_counter()
+        DO.directional_descriptors(phases)
+        cpu["directional_descriptors_s"] = time.perf_counter() - t
+        out["cpu"] = cpu
+    return out
+
+
 def copy_probe(n_bytes: int = 1 << 30) -> None:
     """One plain device copy of n_bytes (a torch elementwise copy kernel): the
     reference point for scaling the FETCH_SIZE / WRITE_SIZE counters."""
@@ -527,6 +571,8 @@ def main() -> None:
     ap.add_argument("--no-radiosity", action="store_true", help="skip the radiosity solves")
     ap.add_argument("--tune", action="store_true", help="time every tracer variant first and use the fastest")
     ap.add_argument("--copy-probe", action="store_true", help="only run one plain 1 GiB device copy")
+    ap.add_argument("--microstructure", action="store_true",
+                    help="only the microstructure leg (generators and descriptors)")
     ap.add_argument("--batching", action="store_true",
                     help="the 16-geometry batched trace pass (+ 64 radiosity RHS per geometry) instead")
     args = ap.parse_args()
@@ -534,8 +580,8 @@ def main() -> None:
         copy_probe()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU; multi-GPU runs are not part of this build "
-              "(the tracer shards by patch range, the radiosity by request)", file=sys.stderr)
+        print("bench.py measures one GPU; --gpus above 1 is not supported "
+              "(the tracer would shard by patch range, the radiosity by request)", file=sys.stderr)
         sys.exit(2)
 
     from porerad import _native, view_factors
@@ -543,6 +589,9 @@ def main() -> None:
     log(_native.lib().prh_version().decode())
     if args.batching:
         print(json.dumps(run_batching(args)), flush=True)
+        return
+    if args.microstructure:
+        print(json.dumps(run_microstructure(foam_workload(), not args.no_cpu)), flush=True)
         return
     t = time.perf_counter()
     phases = foam_workload()
@@ -629,6 +678,14 @@ def main() -> None:
             f"{mc['penetration']['rays_per_s']:.3e} rays/s"
             + (f" (CPU oracle {mc['cpu_penetration']['rays_per_s']:.3e} rays/s on "
                f"{mc['cpu_penetration']['cores']} threads)" if "cpu_penetration" in mc else ""))
+    if not args.no_radiosity:
+        ms = run_microstructure(phases, not args.no_cpu)
+        record["microstructure"] = ms
+        log("microstructure: build OF {build_OF_s:.3f} s, FA {build_FA_s:.3f} s, TL {build_TL_s:.3f} s; descriptors "
+            "scalar {scalar_descriptors_s:.4f} s, directional {directional_descriptors_s:.4f} s".format(**ms)
+            + (" (CPU oracle: OF {build_OF_s:.2f} s, FA {build_FA_s:.2f} s, TL {build_TL_s:.2f} s, scalar "
+               "{scalar_descriptors_s:.3f} s, directional {directional_descriptors_s:.3f} s)".format(**ms["cpu"])
+               if "cpu" in ms else ""))
     if not args.no_cpu:
         record["cpu_baseline"] = cpu_baseline(phases, R, seed)
         log(f"cpu baseline {record['cpu_baseline']['value']:.3e} rays/s")

@@ -28,8 +28,9 @@
 //     hist_len bins and the number of all-pore lines.
 //
 // Connected components use a lock-free union-find over cell indices: every
-// label points at a smaller or equal index, unions lower a root with an
-// atomic minimum, so the labels only decrease and every union ends.  The
+// label points at a smaller or equal index of the same component; unions
+// lower a root and finds halve paths, both with atomic minima, so the labels
+// only decrease and every union ends.  The
 // component of a cell is identified by its root, the smallest index in the
 // component.  Sizes, face contacts and counts are integers reduced with
 // atomics, so the results do not depend on the order of the threads.
@@ -112,14 +113,19 @@ __device__ __forceinline__ int32_t load_label(const int32_t *L, int32_t x)
     return __hip_atomic_load(&L[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ int32_t find_root(const int32_t *L, int32_t x)
+// Root of x with path halving: each visited label is lowered to its
+// grandparent by an atomic minimum (a smaller index in the same component),
+// so paths shorten as they are walked and labels still only decrease.
+__device__ __forceinline__ int32_t find_root(int32_t *L, int32_t x)
 {
-    int32_t y = load_label(L, x);
-    while (y != x) {
-        x = y;
-        y = load_label(L, x);
+    while (true) {
+        const int32_t p = load_label(L, x);
+        if (p == x) return x;
+        const int32_t g = load_label(L, p);
+        if (g == p) return p;
+        atomicMin(&L[x], g);
+        x = g;
     }
-    return x;
 }
 
 // Join the components of a and b: the larger root is lowered to the smaller
@@ -198,40 +204,55 @@ __global__ void __launch_bounds__(DESC_THREADS) k_label_merge(const uint8_t *__r
     }
 }
 
-// After the merge: every pore cell's label is its root; add the cell to the
-// root's size (one atomic per wave for the lanes that share the first active
-// lane's root) and record the box faces it touches (bit 2a: face 0 of axis a,
-// bit 2a+1: the last face).
+// After the merge: every pore cell's label becomes its root; add the cell to
+// the root's size and record the box faces it touches (bit 2a: face 0 of
+// axis a, bit 2a+1: the last face).  Each wave walks cells in wave-uniform
+// rounds and keeps a private count and face mask for one root (that of its
+// first pore cell, nearly always the spanning cluster), added with one
+// atomic each at the end; cells of other roots add their own atomics.
 __global__ void __launch_bounds__(DESC_THREADS) k_label_stats(const uint8_t *__restrict__ p, int nx, int ny, int nz,
                                                               int32_t cells, int32_t *__restrict__ L,
                                                               int32_t *__restrict__ size, uint32_t *__restrict__ faces)
 {
-    const int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-    const bool pore = c < cells && p[c] == 0;
-    int32_t root = -1;
-    if (pore) {
-        root = find_root(L, c);
-        L[c] = root;
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int32_t wave_root = -1, mine = 0;
+    uint32_t mine_faces = 0;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < cells; base += stride) {
+        const int32_t c = (int32_t)(base + lane);
+        const bool pore = c < cells && p[c] == 0;
+        int32_t root = -1;
+        if (pore) {
+            root = find_root(L, c);
+            L[c] = root;
+        }
+        if (wave_root < 0) {
+            const uint64_t active = __ballot(pore);
+            if (active) wave_root = __shfl(root, __ffsll((unsigned long long)active) - 1);
+        }
+        if (!pore) continue;
+        const Cell x = cell_of(c, ny, nz);
+        uint32_t bits = 0;
+        if (x.i == 0) bits |= 1u;
+        if (x.i == nx - 1) bits |= 2u;
+        if (x.j == 0) bits |= 4u;
+        if (x.j == ny - 1) bits |= 8u;
+        if (x.k == 0) bits |= 16u;
+        if (x.k == nz - 1) bits |= 32u;
+        if (root == wave_root) {
+            ++mine;
+            mine_faces |= bits;
+        } else {
+            atomicAdd(&size[root], 1);
+            if (bits) atomicOr(&faces[root], bits);
+        }
     }
-    const uint64_t active = __ballot(pore);
-    if (active) {
-        const int leader = __ffsll((unsigned long long)active) - 1;
-        const int32_t lead_root = __shfl(root, leader);
-        const uint64_t same = __ballot(pore && root == lead_root);
-        const int lane = threadIdx.x & 63;
-        if (lane == leader) atomicAdd(&size[lead_root], (int32_t)__popcll(same));
-        else if (pore && root != lead_root) atomicAdd(&size[root], 1);
+    for (int off = 32; off > 0; off >>= 1) {
+        mine += __shfl_xor(mine, off);
+        mine_faces |= __shfl_xor(mine_faces, off);
     }
-    if (!pore) return;
-    const Cell x = cell_of(c, ny, nz);
-    uint32_t bits = 0;
-    if (x.i == 0) bits |= 1u;
-    if (x.i == nx - 1) bits |= 2u;
-    if (x.j == 0) bits |= 4u;
-    if (x.j == ny - 1) bits |= 8u;
-    if (x.k == 0) bits |= 16u;
-    if (x.k == nz - 1) bits |= 32u;
-    if (bits) atomicOr(&faces[root], bits);
+    if (lane == 0 && mine) atomicAdd(&size[wave_root], mine);
+    if (lane == 0 && mine_faces) atomicOr(&faces[wave_root], mine_faces);
 }
 
 // out: 0 largest cluster size, 1 cluster count, 2 percolation bits (bit a: a
@@ -492,7 +513,8 @@ int prh_scalar_counts(const prh_phase_field *f, int32_t connectivity, int64_t co
         const unsigned g = grid_for(cells);
         k_label_init<<<g, DESC_THREADS, 0, st>>>(f->phases, f->nx, f->ny, f->nz, cells, L);
         k_label_merge<<<g, DESC_THREADS, 0, st>>>(f->phases, f->nx, f->ny, f->nz, cells, L);
-        k_label_stats<<<g, DESC_THREADS, 0, st>>>(f->phases, f->nx, f->ny, f->nz, cells, L, size, faces);
+        k_label_stats<<<std::min<unsigned>(g, 1024), DESC_THREADS, 0, st>>>(f->phases, f->nx, f->ny, f->nz, cells, L,
+                                                                          size, faces);
         k_label_reduce<<<g, DESC_THREADS, 0, st>>>(L, size, faces, cells, acc + N_CELL_COUNTS);
     }
     PRH_HIP(hipGetLastError());

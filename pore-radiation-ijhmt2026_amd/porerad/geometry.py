@@ -78,15 +78,20 @@ def _frame(z, helper):
 
 # Shape stamping on the periodic grid
 
-def _long_cylinder_cells(N, r, half_length, R, center=(0.0, 0.0, 0.0)):
-    """Lattice nodes along the axis (rounded half to even, mod N) and the offsets of the one-cell-thick disk."""
+def _long_cylinder_nodes(N, half_length, R, center):
+    """The cylinder's unit axis and its lattice nodes (axis samples at unit steps, rounded half to even, mod N)."""
 
     axis = _matmul(R, _EZ)
     axis /= float(np.linalg.norm(axis))
     n_steps = max(1, int(math.ceil(2.0 * half_length)))
     t = np.arange(n_steps, dtype=float) - n_steps / 2.0
     pts = t[:, None] * axis[None, :] + np.asarray(center, dtype=float)[None, :]
-    nodes = np.unique(np.rint(pts).astype(np.int64) % N, axis=0)
+    return axis, np.unique(np.rint(pts).astype(np.int64) % N, axis=0)
+
+
+def _disk_offsets(axis, r):
+    """Offsets of the cells of the one-cell-thick disk of radius r normal to the unit axis."""
+
     Rq = _frame(axis / float(np.linalg.norm(axis)), _EX)
     pad = int(math.ceil(r)) + 2
     g = np.arange(-pad, pad + 1, dtype=np.int64)
@@ -94,16 +99,25 @@ def _long_cylinder_cells(N, r, half_length, R, center=(0.0, 0.0, 0.0)):
     local = _matmul(Rq.T, off.T.astype(float)).T
     disk = (((local[:, 0] / r) ** 2 + (local[:, 1] / r) ** 2 <= 1.0 + 1e-12)
             & (np.abs(local[:, 2]) <= 0.5 + 1e-12))
-    return nodes, off[disk]
+    return off[disk]
+
+
+def _long_cylinder_cells(N, r, half_length, R, center=(0.0, 0.0, 0.0)):
+    """Lattice nodes along the axis and the disk offsets that thicken each of them."""
+
+    axis, nodes = _long_cylinder_nodes(N, half_length, R, center)
+    return nodes, _disk_offsets(axis, r)
 
 
 class _Canvas:
     """Shapes of one render, stamped into a device field: short shapes in one pass, long cylinders by scatter."""
 
-    def __init__(self, field: "_native.PhaseField") -> None:
+    def __init__(self, field: "_native.PhaseField", node_cache: dict | None = None) -> None:
         self.field = field
         self.N = field.shape[0]
         self.rows, self.kinds = [], []
+        # long-cylinder nodes do not depend on the radius: a bisection keeps them across renders
+        self.node_cache = {} if node_cache is None else node_cache
 
     def add(self, center, radii, kind, R):
         N = self.N
@@ -112,12 +126,14 @@ class _Canvas:
         if kind == _CYLINDER and float(2.0 * rz) > float(N):
             ci = tuple(int(round(c)) % N for c in center)
             if any(abs(c - float(i)) > 1e-9 for c, i in zip(center, ci)):
-                nodes, off = _long_cylinder_cells(N, rx, rz, R, center)
-                shift = (0, 0, 0)
+                origin, shift = tuple(center), (0, 0, 0)
             else:
-                nodes, off = _long_cylinder_cells(N, rx, rz, R)
-                shift = ci
-            self.field.stamp_offsets(nodes, off, shift)
+                origin, shift = (0.0, 0.0, 0.0), ci
+            key = (origin, float(rz), np.asarray(R, dtype=float).tobytes())
+            if key not in self.node_cache:
+                self.node_cache[key] = _long_cylinder_nodes(N, rz, R, origin)
+            axis, nodes = self.node_cache[key]
+            self.field.stamp_offsets(nodes, _disk_offsets(axis, rx), shift)
             return
         reach = (max(rx, ry, rz) if kind == _ELLIPSOID else max(rx, ry) + rz) + 1.0
         R = np.asarray(R, dtype=float)
@@ -204,10 +220,11 @@ def _bisect_on_device(N, draw, target_phi, r_lo, r_hi, tol=PHI_TOL, max_iter=24)
     cells = N ** 3
     with _native.PhaseField(shape=(N, N, N)) as work, _native.PhaseField(shape=(N, N, N)) as best_field:
         lo, hi, best = float(r_lo), float(r_hi), None
+        node_cache: dict = {}
         for _ in range(int(max_iter)):
             mid = 0.5 * (lo + hi)
             work.fill(0)
-            canvas = _Canvas(work)
+            canvas = _Canvas(work, node_cache)
             draw(canvas, mid)
             canvas.finish()
             phi = float(work.count(1) / cells)

@@ -18,8 +18,10 @@ and writes, next to this script:
                   and the SHA-256 of the original bytes
   cases.json      hash cases, prescribed-field cases with their archived rows,
                   canonical coupled rows and the full-path cross-check rows;
-                  every prescribed-field case and every coupled case whose
-                  microstructure is shipped, with its archived result rows;
+                  every prescribed-field case and every coupled case with its
+                  archived result rows (the 8 resolution cases use
+                  microstructures that are not shipped: they are
+                  regenerated from the microstructure table);
                   the Monte Carlo rows (both restart rules) of those
                   microstructures; every row of the microstructure table
 
@@ -132,9 +134,11 @@ def main(reference: str) -> None:
     archived_c = {r["case_id"]: r for r in res_c}
     coupled_cases = []
     for row in cases_c:
-        if row["microstructure"] not in shipped or row["case_id"] not in archived_c:
+        if row["case_id"] not in archived_c:
             continue
         coupled_cases.append({
+            # microstructures that are not shipped are regenerated from the microstructure table
+            "shipped": row["microstructure"] in shipped,
             **{k: row[k] for k in ("case_id", "group", "structure", "microstructure", "transfer_axis",
                                    "phase_sha256_x", "raw_count_sha256_A", "raw_count_sha256_B")},
             **{k: float(row[k]) for k in ("H_m", "T_mean_K", "delta_T_K", "epsilon")},
@@ -145,7 +149,8 @@ def main(reference: str) -> None:
         })
 
     names = sorted({c["microstructure"] for c in hash_cases} | {c["microstructure"] for c in prescribed}
-                   | {c["microstructure"] for c in prescribed_cases} | {c["microstructure"] for c in coupled_cases})
+                   | {c["microstructure"] for c in prescribed_cases}
+                   | {c["microstructure"] for c in coupled_cases if c["shipped"]})
     arrays = {}
     for name in names:
         path = os.path.join(reference, "data", "microstructures", f"{name}.npz")

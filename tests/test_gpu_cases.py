@@ -24,15 +24,22 @@ Every row the driver writes is compared with its archived row:
   5e-6 K (the one row where the stop counts differ is 4.3e-6 off in q_cond_hot
   and 2.1e-7 K in T_min; the other eps = 0.9 rows agree to 1e-9).
 
+The 8 resolution cases (canonical microstructures at N = 160 and 192) use
+microstructures the reference does not ship: they are generated on the device
+(porerad.geometry.canonical) and checked against the phase SHA-256 of
+data/microstructures.csv before the case runs.
+
 By default a few cases cover each kind of case (an appended case, a matched
-pair, a doubled thickness); PORERAD_FULL=1 runs every case whose
-microstructure is shipped (48 prescribed-field cases, 36 coupled cases).
+pair, a doubled thickness, a resolution case); PORERAD_FULL=1 runs all 48
+prescribed-field cases and all 44 coupled cases.
 """
 
 from __future__ import annotations
 
+import hashlib
 import os
 
+import numpy as np
 import pytest
 
 from golden_data import cases, phases
@@ -41,7 +48,7 @@ pytestmark = pytest.mark.gpu
 
 FULL = os.environ.get("PORERAD_FULL") == "1"
 PRESCRIBED_DEFAULT = ["pf_TL_core_phi0.10_s0_y", "pf_FA_matched_pair1_x", "pf_OF_core_phi0.15_s1_z"]
-COUPLED_DEFAULT = ["coupled_FI_matched_pair1_y", "coupled_OF_x_H20"]
+COUPLED_DEFAULT = ["coupled_FI_matched_pair1_y", "coupled_OF_x_H20", "coupled_TL_x_T700_eps0.6_N160"]
 
 
 def _pick(table, default):
@@ -70,10 +77,24 @@ def test_prescribed_field_case_reproduces_archive(drivers, case):
         assert row["delta_T_K"] == "20"
 
 
+def case_phases(case):
+    """The shipped microstructure, or the one generated on the device from its table row."""
+
+    if case["shipped"]:
+        return phases(case["microstructure"])
+    from porerad import geometry
+
+    row = next(r for r in cases()["microstructures"] if r["id"] == case["microstructure"])
+    assert row["set"] == "canonical"
+    p = geometry.canonical(row["structure"], row["N"])
+    assert hashlib.sha256(np.ascontiguousarray(p).tobytes()).hexdigest() == row["phase_sha256"]
+    return p
+
+
 @pytest.mark.parametrize("case", _pick("coupled_cases", COUPLED_DEFAULT))
 def test_coupled_case_reproduces_archive(drivers, case):
     messages = []
-    row = drivers.coupled_case(case, phases(case["microstructure"]), log=messages.append)
+    row = drivers.coupled_case(case, case_phases(case), log=messages.append)
     for name in "AB":
         if case[f"raw_count_sha256_{name}"]:
             assert f"first-hit counts of set {name}: identical to the stored hash" in messages
