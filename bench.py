@@ -573,6 +573,7 @@ def main() -> None:
     ap.add_argument("--copy-probe", action="store_true", help="only run one plain 1 GiB device copy")
     ap.add_argument("--microstructure", action="store_true",
                     help="only the microstructure leg (generators and descriptors)")
+    ap.add_argument("--monte-carlo", action="store_true", help="only the Monte Carlo leg")
     ap.add_argument("--batching", action="store_true",
                     help="the 16-geometry batched trace pass (+ 64 radiosity RHS per geometry) instead")
     args = ap.parse_args()
@@ -592,6 +593,9 @@ def main() -> None:
         return
     if args.microstructure:
         print(json.dumps(run_microstructure(foam_workload(), not args.no_cpu)), flush=True)
+        return
+    if args.monte_carlo:
+        print(json.dumps(run_monte_carlo(foam_workload(), not args.no_cpu)), flush=True)
         return
     t = time.perf_counter()
     phases = foam_workload()
