@@ -212,10 +212,11 @@ int prh_stamp_shapes(prh_phase_field *f, int32_t n_shapes, const double *shapes,
 int prh_stamp_offsets(prh_phase_field *f, int32_t n_nodes, const int32_t *nodes, int32_t n_offsets,
                       const int32_t *offsets, const int32_t shift[3]);
 
-/* threshold (porerad/geometry.py:235-238): the field (N^3 FP64, host) plus,
- * if noise is not NULL, lamellar_field's seven-point smoothing of noise
- * (:229-232), thresholded at np.quantile(values, q) (linear rule; the value
- * comes back in *quantile): cells >= it become 1, the others 0. */
+/* threshold (porerad/geometry.py:235-238): the field (one FP64 value per cell
+ * of f, host) plus, if noise is not NULL (f cubic), lamellar_field's
+ * seven-point smoothing of noise (:229-232), thresholded at
+ * np.quantile(values, q) (linear rule; the value comes back in *quantile):
+ * cells >= it become 1, the others 0. */
 int prh_threshold_field(prh_phase_field *f, const double *field, const double *noise, double q, double *quantile);
 
 /* Origins and directions ((p1-p0)*rays_per_patch x 3, FP64) of the seeded

@@ -377,7 +377,7 @@ int prh_threshold_field(prh_phase_field *f, const double *field, const double *n
 {
     PRH_CHECK_ARG(f && field && quantile, "prh_threshold_field: null pointer");
     PRH_CHECK_ARG(q >= 0.0 && q <= 1.0, "prh_threshold_field: quantile outside [0, 1]");
-    int rc = cubic(f, "prh_threshold_field");
+    int rc = noise ? cubic(f, "prh_threshold_field (with noise)") : PRH_OK;
     if (rc) return rc;
     const int64_t n = f->cells;
     PRH_HIP(hipSetDevice(f->device));

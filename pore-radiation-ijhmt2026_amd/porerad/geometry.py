@@ -135,8 +135,14 @@ class _Canvas:
             axis, nodes = self.node_cache[key]
             self.field.stamp_offsets(nodes, _disk_offsets(axis, rx), shift)
             return
-        reach = (max(rx, ry, rz) if kind == _ELLIPSOID else max(rx, ry) + rz) + 1.0
         R = np.asarray(R, dtype=float)
+        # A rotation keeps |x| = |R^T x|, so a point inside lies within the
+        # shape's extent of the centre on every axis; for any other R every
+        # cell is tested.
+        if np.allclose(R.T @ R, np.eye(3), rtol=0.0, atol=1e-9):
+            reach = (max(rx, ry, rz) if kind == _ELLIPSOID else max(rx, ry) + rz) + 1.0
+        else:
+            reach = 2.0 * float(N)
         self.rows.append(np.concatenate([[cx, cy, cz, rx, ry, rz], R.T.ravel(), [reach]]))
         self.kinds.append(kind)
 
@@ -373,11 +379,10 @@ def threshold(field, target_phi):
     """Solid where the field reaches its (1 - phi_s) quantile (geometry.py:235-238)."""
 
     f = np.asarray(field, dtype=np.float64)
-    if f.ndim != 3 or len(set(f.shape)) != 1:
-        raise ValueError("threshold expects an N x N x N field")
-    with _native.PhaseField(shape=f.shape) as out:
-        out.threshold(f, 1.0 - float(target_phi))
-        return out.read()
+    flat = np.ascontiguousarray(f.reshape(1, 1, -1))
+    with _native.PhaseField(shape=flat.shape) as out:
+        out.threshold(flat, 1.0 - float(target_phi))
+        return out.read().reshape(f.shape)
 
 
 def _threshold_layers(N, seed, n_layers, waviness, target_phi):

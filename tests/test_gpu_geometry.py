@@ -57,6 +57,24 @@ def test_shape_mask_against_oracle(geo, seed):
         assert np.array_equal(geo.shape_mask(N, center, radii, kind, R), canvas.finish().astype(bool))
 
 
+def test_shape_mask_general_matrix(geo):
+    """A matrix that is not a rotation: every cell is tested (no reach shortcut)."""
+
+    N = 29
+    R = np.array([[2.0, 0.3, 0.0], [0.1, 0.5, 0.2], [0.0, -0.4, 1.5]])
+    for kind, k in (("ellipsoid", G.ELLIPSOID), ("cylinder", G.CYLINDER)):
+        canvas = G.Canvas(N)
+        canvas.add((3.3, 20.1, 11.7), (4.0, 2.5, 6.0), k, R)
+        assert np.array_equal(geo.shape_mask(N, (3.3, 20.1, 11.7), (4.0, 2.5, 6.0), kind, R),
+                              canvas.finish().astype(bool))
+
+
+def test_threshold_any_shape(geo):
+    field = np.random.default_rng(4).normal(size=(7, 300))
+    q = np.quantile(field.ravel(), 1.0 - 0.3)
+    assert np.array_equal(geo.threshold(field, 0.3), (field >= q).astype(np.uint8))
+
+
 def test_long_cylinders_against_oracle(geo):
     N = 40
     v = np.array([0.2, 0.9, -0.4])
