@@ -30,7 +30,8 @@
  *                             interface_face_counts, porerad/descriptors.py:38-79,120-123
  *   prh_periodic_chords       periodic_chords, porerad/descriptors.py:82-109
  *   prh_stamp_shapes          shape_mask (short shapes), porerad/geometry.py:106-116
- *   prh_stamp_offsets         _long_cylinder, porerad/geometry.py:63-86
+ *   prh_stamp_offsets,        _long_cylinder, porerad/geometry.py:63-86
+ *   prh_stamp_long_cylinders
  *   prh_threshold_field       lamellar_field smoothing + threshold,
  *                             porerad/geometry.py:229-238
  */
@@ -211,6 +212,17 @@ int prh_stamp_shapes(prh_phase_field *f, int32_t n_shapes, const double *shapes,
  * in [0, N). */
 int prh_stamp_offsets(prh_phase_field *f, int32_t n_nodes, const int32_t *nodes, int32_t n_offsets,
                       const int32_t *offsets, const int32_t shift[3]);
+
+/* The long cylinders of one render in one call: _long_cylinder for each
+ * (porerad/geometry.py:63-86) with the roll of integer centres (:100-105).
+ * Cylinder c has the nodes [node_start[c], node_start[c+1]) of nodes (x3, in
+ * [0, N)), radius[c] in (0, N/4], the transposed disk frame rq_t[9c..9c+9)
+ * (Rq^T row-major) and shift[3c..3c+3) in [0, N); its disk is the offsets o
+ * of the (2 pad + 1)^3 box, pad = ceil(r) + 2, with (l0/r)^2 + (l1/r)^2 <=
+ * 1 + 1e-12 and |l2| <= 0.5 + 1e-12 for l = Rq^T o (an FMA chain, as NumPy's
+ * product rounds), and cells (node + o + shift) mod N are set to 1. */
+int prh_stamp_long_cylinders(prh_phase_field *f, int32_t n_cyl, const int32_t *node_start, const int32_t *nodes,
+                             const double *rq_t, const double *radius, const int32_t *shift);
 
 /* threshold (porerad/geometry.py:235-238): the field (one FP64 value per cell
  * of f, host) plus, if noise is not NULL (f cubic), lamellar_field's

@@ -116,6 +116,53 @@ __global__ void __launch_bounds__(GEN_THREADS) k_stamp_offsets(uint8_t *__restri
     phases[((int64_t)i * N + j) * N + k] = 1;
 }
 
+// Long cylinders of one render (_long_cylinder for each, with the roll of
+// integer centres): every (cylinder, node, candidate offset) of the
+// (2 pad + 1)^3 box is one work item; the offset belongs to the disk when
+// (l0/r)^2 + (l1/r)^2 <= 1 + 1e-12 and |l2| <= 0.5 + 1e-12 with l = Rq^T off
+// formed as the FMA chain NumPy's product rounds to.
+struct LongCylinders {
+    int n_cyl;
+    const int64_t *work_start;  // n_cyl + 1: first work item of each cylinder
+    const int32_t *node_start;  // n_cyl + 1: first node of each cylinder
+    const int32_t *nodes;       // x3, in [0, N)
+    const double *rq_t;         // x9: Rq^T row-major
+    const double *radius;
+    const int32_t *pad;
+    const int32_t *shift;  // x3, in [0, N)
+};
+
+__global__ void __launch_bounds__(GEN_THREADS) k_stamp_long_cylinders(uint8_t *__restrict__ phases, int N,
+                                                                      LongCylinders A)
+{
+    const int64_t total = A.work_start[A.n_cyl];
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        int c = 0;
+        while (c + 1 < A.n_cyl && A.work_start[c + 1] <= t) ++c;
+        const int pad = A.pad[c], S = 2 * pad + 1;
+        const int64_t K = (int64_t)S * S * S;
+        const int64_t rem = t - A.work_start[c];
+        const int node = A.node_start[c] + (int)(rem / K);
+        const int cand = (int)(rem % K);
+        const int ox = cand / (S * S) - pad, oy = (cand / S) % S - pad, oz = cand % S - pad;
+        const double x = (double)ox, y = (double)oy, z = (double)oz;
+        const double *Rt = A.rq_t + 9 * c;
+        const double l0 = __builtin_fma(Rt[2], z, __builtin_fma(Rt[1], y, Rt[0] * x));
+        const double l1 = __builtin_fma(Rt[5], z, __builtin_fma(Rt[4], y, Rt[3] * x));
+        const double l2 = __builtin_fma(Rt[8], z, __builtin_fma(Rt[7], y, Rt[6] * x));
+        const double r = A.radius[c];
+        const double a = l0 / r, b = l1 / r;
+        if (!(a * a + b * b <= 1.0 + 1e-12 && fabs(l2) <= 0.5 + 1e-12)) continue;
+        const int32_t *nd = A.nodes + 3 * node, *sh = A.shift + 3 * c;
+        int i = (nd[0] + ox + sh[0]) % N, j = (nd[1] + oy + sh[1]) % N, k = (nd[2] + oz + sh[2]) % N;
+        i += i < 0 ? N : 0;
+        j += j < 0 ? N : 0;
+        k += k < 0 ? N : 0;
+        phases[((int64_t)i * N + j) * N + k] = 1;
+    }
+}
+
 __global__ void k_fill(uint8_t *__restrict__ p, int64_t n, uint8_t v)
 {
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x)
@@ -368,6 +415,59 @@ int prh_stamp_offsets(prh_phase_field *f, int32_t n_nodes, const int32_t *nodes,
     PRH_HIP(hipMemcpyAsync(d_off, offsets, sizeof(int32_t) * 3 * n_offsets, hipMemcpyHostToDevice, st));
     k_stamp_offsets<<<grid_of((int64_t)n_nodes * n_offsets), GEN_THREADS, 0, st>>>(
         f->phases, N, d_nodes, n_nodes, d_off, n_offsets, shift[0], shift[1], shift[2]);
+    PRH_HIP(hipGetLastError());
+    PRH_HIP(hipStreamSynchronize(st));
+    return PRH_OK;
+}
+
+int prh_stamp_long_cylinders(prh_phase_field *f, int32_t n_cyl, const int32_t *node_start, const int32_t *nodes,
+                             const double *rq_t, const double *radius, const int32_t *shift)
+{
+    PRH_CHECK_ARG(f && (n_cyl == 0 || (node_start && nodes && rq_t && radius && shift)),
+                  "prh_stamp_long_cylinders: null pointer");
+    PRH_CHECK_ARG(n_cyl >= 0, "prh_stamp_long_cylinders: negative count");
+    int rc = cubic(f, "prh_stamp_long_cylinders");
+    if (rc) return rc;
+    if (n_cyl == 0) return PRH_OK;
+    const int N = f->nx;
+    PRH_CHECK_ARG(node_start[0] == 0, "prh_stamp_long_cylinders: node_start[0] must be 0");
+    std::vector<int64_t> work(n_cyl + 1, 0);
+    std::vector<int32_t> pad(n_cyl);
+    for (int32_t c = 0; c < n_cyl; ++c) {
+        PRH_CHECK_ARG(node_start[c + 1] >= node_start[c], "prh_stamp_long_cylinders: node ranges must ascend");
+        PRH_CHECK_ARG(radius[c] > 0.0 && radius[c] <= 0.25 * N,
+                      "prh_stamp_long_cylinders: radius of cylinder %d outside (0, N/4]", c);
+        pad[c] = (int32_t)std::ceil(radius[c]) + 2;
+        const int64_t S = 2 * (int64_t)pad[c] + 1;
+        work[c + 1] = work[c] + (int64_t)(node_start[c + 1] - node_start[c]) * S * S * S;
+        for (int d = 0; d < 3; ++d)
+            PRH_CHECK_ARG(shift[3 * c + d] >= 0 && shift[3 * c + d] < N, "prh_stamp_long_cylinders: shift outside [0, N)");
+    }
+    const int32_t n_nodes = node_start[n_cyl];
+    for (int32_t a = 0; a < 3 * n_nodes; ++a)
+        PRH_CHECK_ARG(nodes[a] >= 0 && nodes[a] < N, "prh_stamp_long_cylinders: node outside [0, N)");
+    if (work[n_cyl] == 0) return PRH_OK;
+    PRH_HIP(hipSetDevice(f->device));
+    hipStream_t st;
+    if ((rc = current_stream(&st))) return rc;
+    GenBuffers B;
+    int64_t *d_work = B.alloc<int64_t>(n_cyl + 1);
+    int32_t *d_start = B.alloc<int32_t>(n_cyl + 1), *d_nodes = B.alloc<int32_t>(3 * (int64_t)n_nodes);
+    int32_t *d_pad = B.alloc<int32_t>(n_cyl), *d_shift = B.alloc<int32_t>(3 * (int64_t)n_cyl);
+    double *d_rq = B.alloc<double>(9 * (int64_t)n_cyl), *d_r = B.alloc<double>(n_cyl);
+    if (!d_work || !d_start || !d_nodes || !d_pad || !d_shift || !d_rq || !d_r) {
+        set_error("prh_stamp_long_cylinders: out of device memory");
+        return PRH_ERR_HIP;
+    }
+    PRH_HIP(hipMemcpyAsync(d_work, work.data(), sizeof(int64_t) * (n_cyl + 1), hipMemcpyHostToDevice, st));
+    PRH_HIP(hipMemcpyAsync(d_start, node_start, sizeof(int32_t) * (n_cyl + 1), hipMemcpyHostToDevice, st));
+    PRH_HIP(hipMemcpyAsync(d_nodes, nodes, sizeof(int32_t) * 3 * (size_t)n_nodes, hipMemcpyHostToDevice, st));
+    PRH_HIP(hipMemcpyAsync(d_pad, pad.data(), sizeof(int32_t) * n_cyl, hipMemcpyHostToDevice, st));
+    PRH_HIP(hipMemcpyAsync(d_shift, shift, sizeof(int32_t) * 3 * (size_t)n_cyl, hipMemcpyHostToDevice, st));
+    PRH_HIP(hipMemcpyAsync(d_rq, rq_t, sizeof(double) * 9 * (size_t)n_cyl, hipMemcpyHostToDevice, st));
+    PRH_HIP(hipMemcpyAsync(d_r, radius, sizeof(double) * n_cyl, hipMemcpyHostToDevice, st));
+    const LongCylinders A{n_cyl, d_work, d_start, d_nodes, d_rq, d_r, d_pad, d_shift};
+    k_stamp_long_cylinders<<<stride_grid(work[n_cyl]), GEN_THREADS, 0, st>>>(f->phases, N, A);
     PRH_HIP(hipGetLastError());
     PRH_HIP(hipStreamSynchronize(st));
     return PRH_OK;

@@ -65,6 +65,7 @@ SIGNATURES = {
     "prh_phase_field_count": ([_P, ctypes.c_uint8, _P], ctypes.c_int),
     "prh_stamp_shapes": ([_P, _I32, _P, _P], ctypes.c_int),
     "prh_stamp_offsets": ([_P, _I32, _P, _I32, _P, _P], ctypes.c_int),
+    "prh_stamp_long_cylinders": ([_P, _I32, _P, _P, _P, _P, _P], ctypes.c_int),
     "prh_threshold_field": ([_P, _P, _P, _F64, _P], ctypes.c_int),
     "prh_generate_rays": ([_P, _I64, _I64, _I32, _P, _P, _P], ctypes.c_int),
     "prh_trace_rays": ([_P, _P, _P, _I64, _P, _P], ctypes.c_int),
@@ -332,6 +333,19 @@ class PhaseField:
         o = np.ascontiguousarray(offsets, np.int32).reshape(-1, 3)
         sh = np.ascontiguousarray(shift, np.int32)
         check(self._lib.prh_stamp_offsets(self.handle, len(n), ptr(n), len(o), ptr(o), ptr(sh)), "prh_stamp_offsets")
+
+    def stamp_long_cylinders(self, nodes: list, rq_t: np.ndarray, radius: np.ndarray, shift: np.ndarray) -> None:
+        """Stamp several long cylinders (see prh_stamp_long_cylinders); nodes is a list of (n_c, 3) arrays."""
+
+        counts = np.asarray([len(n) for n in nodes], np.int64)
+        start = np.zeros(len(nodes) + 1, np.int32)
+        start[1:] = np.cumsum(counts)
+        flat = np.ascontiguousarray(np.concatenate(nodes) if nodes else np.zeros((0, 3)), np.int32)
+        rq = np.ascontiguousarray(rq_t, np.float64)
+        r = np.ascontiguousarray(radius, np.float64)
+        sh = np.ascontiguousarray(shift, np.int32)
+        check(self._lib.prh_stamp_long_cylinders(self.handle, len(nodes), ptr(start), ptr(flat), ptr(rq), ptr(r),
+                                                 ptr(sh)), "prh_stamp_long_cylinders")
 
     def threshold(self, field: np.ndarray, q: float, noise: np.ndarray | None = None) -> float:
         """Cells >= np.quantile(values, q) become 1, the rest 0; values = field (+ smoothed noise)."""

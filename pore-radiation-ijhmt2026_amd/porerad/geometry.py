@@ -116,7 +116,8 @@ class _Canvas:
         self.field = field
         self.N = field.shape[0]
         self.rows, self.kinds = [], []
-        # long-cylinder nodes do not depend on the radius: a bisection keeps them across renders
+        self.long = []  # (nodes, Rq^T, radius, shift) of the render's long cylinders
+        # long-cylinder nodes and disk frames do not depend on the radius: a bisection keeps them across renders
         self.node_cache = {} if node_cache is None else node_cache
 
     def add(self, center, radii, kind, R):
@@ -131,9 +132,13 @@ class _Canvas:
                 origin, shift = (0.0, 0.0, 0.0), ci
             key = (origin, float(rz), np.asarray(R, dtype=float).tobytes())
             if key not in self.node_cache:
-                self.node_cache[key] = _long_cylinder_nodes(N, rz, R, origin)
-            axis, nodes = self.node_cache[key]
-            self.field.stamp_offsets(nodes, _disk_offsets(axis, rx), shift)
+                axis, nodes = _long_cylinder_nodes(N, rz, R, origin)
+                self.node_cache[key] = (axis, nodes, _frame(axis / float(np.linalg.norm(axis)), _EX).T.ravel())
+            axis, nodes, rq_t = self.node_cache[key]
+            if 0.0 < float(rx) <= 0.25 * N:
+                self.long.append((nodes, rq_t, float(rx), shift))
+            else:  # outside the batched kernel's radius range: disk offsets formed here
+                self.field.stamp_offsets(nodes, _disk_offsets(axis, rx), shift)
             return
         R = np.asarray(R, dtype=float)
         # A rotation keeps |x| = |R^T x|, so a point inside lies within the
@@ -150,6 +155,11 @@ class _Canvas:
         if self.rows:
             self.field.stamp_shapes(np.stack(self.rows), np.asarray(self.kinds, np.int32))
             self.rows, self.kinds = [], []
+        if self.long:
+            self.field.stamp_long_cylinders([c[0] for c in self.long], np.stack([c[1] for c in self.long]),
+                                            np.asarray([c[2] for c in self.long]),
+                                            np.asarray([c[3] for c in self.long], np.int32))
+            self.long = []
 
 
 def _kind(kind: str) -> int:
