@@ -21,6 +21,7 @@
 // The control flow (Newton steps, line search, small least-squares
 // problems) stays with the caller.
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <vector>
 
@@ -194,6 +195,61 @@ __global__ __launch_bounds__(CT) void k_csr(const int64_t *rowptr, const int32_t
     }
 }
 
+// The same product when A is the conduction stencil of an nx x ny x nz grid
+// (ny, nz >= 3): neighbours along x (not periodic) and along y, z (periodic),
+// symmetric, so A holds per cell the diagonal and the coefficients of its +x,
+// +y, +z faces (cx, cy, cz; zero at the last x plane) and the -x/-y/-z
+// coefficients are the neighbours' +faces.  Each row is summed in ascending
+// column order, as the CSR row is: the -x term first, the +x term last and
+// the five terms of the cell's own plane (-y, -z, self, +z, +y, whose order
+// changes where y or z wraps) sorted by a 9-comparator network.
+__device__ __forceinline__ void cswap(int64_t &ca, double &va, int64_t &cb, double &vb)
+{
+    const bool s = cb < ca;
+    const int64_t c0 = s ? cb : ca, c1 = s ? ca : cb;
+    const double v0 = s ? vb : va, v1 = s ? va : vb;
+    ca = c0;
+    cb = c1;
+    va = v0;
+    vb = v1;
+}
+
+__global__ __launch_bounds__(CT) void k_stencil7(const double *diag, const double *cx, const double *cy,
+                                                 const double *cz, int nx, int ny, int nz, const double *x,
+                                                 const double *shift, double *y, int64_t n)
+{
+    const int64_t sxy = (int64_t)ny * nz;
+    for (int64_t r = (int64_t)blockIdx.x * CT + threadIdx.x; r < n; r += (int64_t)gridDim.x * CT) {
+        const int k = (int)(r % nz);
+        const int j = (int)((r / nz) % ny);
+        const int i = (int)(r / sxy);
+        int64_t c0 = j > 0 ? r - nz : r + (int64_t)(ny - 1) * nz;  // -y
+        int64_t c1 = k > 0 ? r - 1 : r + (nz - 1);                   // -z
+        int64_t c2 = r;
+        int64_t c3 = k + 1 < nz ? r + 1 : r - (nz - 1);              // +z
+        int64_t c4 = j + 1 < ny ? r + nz : r - (int64_t)(ny - 1) * nz;  // +y
+        double v0 = cy[c0], v1 = cz[c1], v2 = diag[r], v3 = cz[r], v4 = cy[r];
+        cswap(c0, v0, c1, v1);
+        cswap(c3, v3, c4, v4);
+        cswap(c2, v2, c4, v4);
+        cswap(c2, v2, c3, v3);
+        cswap(c1, v1, c4, v4);
+        cswap(c0, v0, c3, v3);
+        cswap(c0, v0, c2, v2);
+        cswap(c1, v1, c3, v3);
+        cswap(c1, v1, c2, v2);
+        double a = 0.0;
+        if (i > 0) a += cx[r - sxy] * x[r - sxy];
+        a += v0 * x[c0];
+        a += v1 * x[c1];
+        a += v2 * x[c2];
+        a += v3 * x[c3];
+        a += v4 * x[c4];
+        if (i + 1 < nx) a += cx[r] * x[r + sxy];
+        y[r] = shift ? a + shift[r] * x[r] : a;
+    }
+}
+
 // R = (A T - b) - Q with b = hot T_hot + cold T_cold.
 __global__ __launch_bounds__(CT) void k_residual(const double *AT, const double *hot, const double *cold,
                                                  const double *Q, double T_hot, double T_cold, double *R, int64_t n)
@@ -362,6 +418,10 @@ struct prh_coupled {
     // patch vectors: temperatures of the last radiation, linearisation point, derivative
     double *Tp = nullptr, *Tlin = nullptr, *dE = nullptr, *emitted = nullptr;
     double *part = nullptr, *scal = nullptr, *ycoef = nullptr;
+    // the conduction stencil (k_stencil7), when A has exactly that form
+    bool stencil = false;
+    int32_t nx = 0, ny = 0, nz = 0;
+    double *cx = nullptr, *cy = nullptr, *cz = nullptr;
     int32_t pcg_steps = 0;
     bool have_lin = false, have_pcg = false;
     // Jacobian-vector products: FP64 reflection series (PRH_FP64) or FP32
@@ -377,8 +437,9 @@ bool slot_ok(const prh_coupled *c, int32_t v) { return v >= 0 && v < c->n_vector
 
 void free_coupled(prh_coupled *c)
 {
-    void *ptrs[] = {c->vec, c->rowptr, c->col, c->val, c->diag, c->hot, c->cold, c->ax, c->r, c->z, c->p, c->ap,
-                    c->shift, c->inv, c->Tp, c->Tlin, c->dE, c->emitted, c->part, c->scal, c->ycoef};
+    void *ptrs[] = {c->vec, c->rowptr, c->col,  c->val,  c->diag, c->hot,     c->cold,  c->ax,
+                    c->r,   c->z,      c->p,    c->ap,   c->shift, c->inv,    c->Tp,    c->Tlin,
+                    c->dE,  c->emitted, c->part, c->scal, c->ycoef, c->cx,    c->cy,    c->cz};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -389,6 +450,64 @@ int stream_of(prh_coupled *c, hipStream_t *st)
     PRH_HIP(hipSetDevice(c->device));
     *st = c->stream;
     return PRH_OK;
+}
+
+// y = A x (+ shift * x): the stencil form when A has it, else the CSR rows.
+int apply_conduction(prh_coupled *c, const double *x, const double *shift, double *y, hipStream_t st)
+{
+    if (c->stencil)
+        k_stencil7<<<blocks_of(c->n_cells), CT, 0, st>>>(c->diag, c->cx, c->cy, c->cz, c->nx, c->ny, c->nz, x, shift,
+                                                         y, c->n_cells);
+    else
+        k_csr<<<blocks_of(c->n_cells), CT, 0, st>>>(c->rowptr, c->col, c->val, x, shift, y, c->n_cells);
+    PRH_HIP(hipGetLastError());
+    return PRH_OK;
+}
+
+bool same_bits(double a, double b) { return std::memcmp(&a, &b, sizeof a) == 0; }
+
+// Whether the CSR matrix is the conduction stencil of the grid (exact
+// column sets per row, exact symmetry); fills the face coefficients if so.
+bool detect_stencil(const int64_t *rp, const int32_t *col, const double *val, int nx, int ny, int nz,
+                    std::vector<double> &cx, std::vector<double> &cy, std::vector<double> &cz)
+{
+    const int64_t sxy = (int64_t)ny * nz, n = (int64_t)nx * sxy;
+    if (ny < 3 || nz < 3) return false;
+    cx.assign(n, 0.0);
+    cy.assign(n, 0.0);
+    cz.assign(n, 0.0);
+    for (int pass = 0; pass < 2; ++pass)
+        for (int64_t r = 0; r < n; ++r) {
+            const int k = (int)(r % nz), j = (int)((r / nz) % ny), i = (int)(r / sxy);
+            const int64_t ym = j > 0 ? r - nz : r + (int64_t)(ny - 1) * nz, yp = j + 1 < ny ? r + nz : r - (int64_t)(ny - 1) * nz;
+            const int64_t zm = k > 0 ? r - 1 : r + (nz - 1), zp = k + 1 < nz ? r + 1 : r - (nz - 1);
+            int64_t want[7];
+            int m = 0;
+            if (i > 0) want[m++] = r - sxy;
+            want[m++] = ym;
+            want[m++] = zm;
+            want[m++] = r;
+            want[m++] = zp;
+            want[m++] = yp;
+            if (i + 1 < nx) want[m++] = r + sxy;
+            std::sort(want, want + m);
+            if (rp[r + 1] - rp[r] != m) return false;
+            for (int q = 0; q < m; ++q) {
+                const int64_t e = rp[r] + q, s = col[e];
+                if (s != want[q]) return false;
+                if (pass == 0) {
+                    if (i + 1 < nx && s == r + sxy) cx[r] = val[e];
+                    else if (s == yp) cy[r] = val[e];
+                    else if (s == zp) cz[r] = val[e];
+                } else {
+                    // the -faces must equal the neighbours' +faces bit for bit
+                    if (i > 0 && s == r - sxy && !same_bits(val[e], cx[s])) return false;
+                    if (s == ym && !same_bits(val[e], cy[s])) return false;
+                    if (s == zm && !same_bits(val[e], cz[s])) return false;
+                }
+            }
+        }
+    return true;
 }
 
 // scal[k] = x . y
@@ -483,6 +602,24 @@ int prh_coupled_create(const prh_geometry *geo, prh_operator *op, int64_t n_cell
     PRH_HIP(hipMemcpy(c->col, A_col, (size_t)nnz * 4, hipMemcpyHostToDevice));
     PRH_HIP(hipMemcpy(c->val, A_val, (size_t)nnz * 8, hipMemcpyHostToDevice));
     PRH_HIP(hipMemcpy(c->diag, diag.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+    if (geo) {  // a handle without geometry keeps the CSR rows
+        const Geometry &g = geo->g;
+        std::vector<double> fx, fy, fz;
+        if ((int64_t)g.nx * g.ny * g.nz == n && detect_stencil(A_rowptr, A_col, A_val, g.nx, g.ny, g.nz, fx, fy, fz)) {
+            alloc((void **)&c->cx, (size_t)n * 8);
+            alloc((void **)&c->cy, (size_t)n * 8);
+            alloc((void **)&c->cz, (size_t)n * 8);
+            if (ok) {
+                PRH_HIP(hipMemcpy(c->cx, fx.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+                PRH_HIP(hipMemcpy(c->cy, fy.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+                PRH_HIP(hipMemcpy(c->cz, fz.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+                c->stencil = true;
+                c->nx = g.nx;
+                c->ny = g.ny;
+                c->nz = g.nz;
+            }
+        }
+    }
     PRH_HIP(hipMemcpy(c->hot, hot_rhs, (size_t)n * 8, hipMemcpyHostToDevice));
     PRH_HIP(hipMemcpy(c->cold, cold_rhs, (size_t)n * 8, hipMemcpyHostToDevice));
     PRH_HIP(hipMemsetAsync(c->vec, 0, (size_t)n_vectors * n * 8, c->stream));
@@ -588,10 +725,7 @@ int prh_coupled_matvec(prh_coupled *c, int32_t x, int32_t y)
     hipStream_t st;
     int rc = stream_of(c, &st);
     if (rc) return rc;
-    k_csr<<<blocks_of(c->n_cells), CT, 0, st>>>(c->rowptr, c->col, c->val, slot(c, x), nullptr, slot(c, y),
-                                                c->n_cells);
-    PRH_HIP(hipGetLastError());
-    return PRH_OK;
+    return apply_conduction(c, slot(c, x), nullptr, slot(c, y), st);
 }
 
 int prh_coupled_radiation(prh_coupled *c, int32_t T, double T_hot, double T_cold, double emissivity, double rtol,
@@ -622,7 +756,7 @@ int prh_coupled_residual(prh_coupled *c, int32_t T, int32_t Q, double T_hot, dou
     hipStream_t st;
     int rc = stream_of(c, &st);
     if (rc) return rc;
-    k_csr<<<blocks_of(c->n_cells), CT, 0, st>>>(c->rowptr, c->col, c->val, slot(c, T), nullptr, c->ax, c->n_cells);
+    if ((rc = apply_conduction(c, slot(c, T), nullptr, c->ax, st))) return rc;
     k_residual<<<red_blocks_of(c->n_cells), CT, 0, st>>>(c->ax, c->hot, c->cold, slot(c, Q), T_hot, T_cold,
                                                          slot(c, R_out), c->n_cells);
     PRH_HIP(hipGetLastError());
@@ -658,8 +792,8 @@ int prh_coupled_jacobian(prh_coupled *c, int32_t v, double emissivity, double rt
     const int64_t np = c->n_patches;
     k_emission_derivative<<<blocks_of(np), CT, 0, st>>>(g.patch_class, g.patch_cell, c->Tlin, op->area, slot(c, v),
                                                         emissivity, op->emit, op->eps, np);
-    k_csr<<<blocks_of(c->n_cells), CT, 0, st>>>(c->rowptr, c->col, c->val, slot(c, v), nullptr, c->ax, c->n_cells);
     PRH_HIP(hipGetLastError());
+    if ((rc = apply_conduction(c, slot(c, v), nullptr, c->ax, st))) return rc;
     double res = 0.0;
     int32_t its = 0;
     if (c->precision == PRH_MIXED) rc = mixed_solve_single(op, op->emit, emissivity, rtol, st, op->absorbed, &its);
@@ -719,8 +853,7 @@ int prh_coupled_precondition(prh_coupled *c, int32_t rhs, int32_t out)
     PRH_HIP(hipGetLastError());
     if ((rc = dot_into(c, c->r, c->z, 0, st))) return rc;
     for (int step = 0; step < c->pcg_steps; ++step) {
-        k_csr<<<blocks_of(n), CT, 0, st>>>(c->rowptr, c->col, c->val, c->p, c->shift, c->ap, n);
-        PRH_HIP(hipGetLastError());
+        if ((rc = apply_conduction(c, c->p, c->shift, c->ap, st))) return rc;
         if ((rc = dot_into(c, c->p, c->ap, 1, st))) return rc;
         k_pcg_update<<<eb, CT, 0, st>>>(c->scal, c->p, c->ap, c->inv, x, c->r, c->z, n);
         PRH_HIP(hipGetLastError());
@@ -799,7 +932,7 @@ int prh_coupled_cg(prh_coupled *c, int32_t b, int32_t x, double rtol, int32_t ma
     PRH_HIP(hipMemcpyAsync(c->inv, d.data(), (size_t)n * 8, hipMemcpyHostToDevice, st));
     c->have_pcg = false;  // the preconditioner inverse now belongs to this solve
     if ((rc = dot_into(c, bv, bv, 3, st))) return rc;
-    k_csr<<<blocks_of(n), CT, 0, st>>>(c->rowptr, c->col, c->val, xv, nullptr, c->ax, n);
+    if ((rc = apply_conduction(c, xv, nullptr, c->ax, st))) return rc;
     k_cg_start<<<eb, CT, 0, st>>>(bv, c->ax, c->inv, c->r, c->z, c->p, n);
     PRH_HIP(hipGetLastError());
     if ((rc = dot_into(c, c->r, c->z, 0, st))) return rc;
@@ -815,8 +948,7 @@ int prh_coupled_cg(prh_coupled *c, int32_t b, int32_t x, double rtol, int32_t ma
     while (r_norm > goal && it < max_iterations) {
         const int batch = std::min(CHECK, max_iterations - it);
         for (int k = 0; k < batch; ++k) {
-            k_csr<<<blocks_of(n), CT, 0, st>>>(c->rowptr, c->col, c->val, c->p, nullptr, c->ap, n);
-            PRH_HIP(hipGetLastError());
+            if ((rc = apply_conduction(c, c->p, nullptr, c->ap, st))) return rc;
             if ((rc = dot_into(c, c->p, c->ap, 1, st))) return rc;
             k_pcg_update<<<eb, CT, 0, st>>>(c->scal, c->p, c->ap, c->inv, xv, c->r, c->z, n);
             PRH_HIP(hipGetLastError());
@@ -832,7 +964,7 @@ int prh_coupled_cg(prh_coupled *c, int32_t b, int32_t x, double rtol, int32_t ma
         r_norm = std::sqrt(h[4]);
     }
     // true residual of the returned field
-    k_csr<<<blocks_of(n), CT, 0, st>>>(c->rowptr, c->col, c->val, xv, nullptr, c->ax, n);
+    if ((rc = apply_conduction(c, xv, nullptr, c->ax, st))) return rc;
     k_axpby<<<eb, CT, 0, st>>>(c->ax, bv, n, -1.0, 1.0);
     PRH_HIP(hipGetLastError());
     if ((rc = dot_into(c, c->ax, c->ax, 4, st))) return rc;

@@ -58,6 +58,23 @@ def test_conduction_matches_oracle(mods):
     assert np.max(np.abs(T - ref.solve(700.0, 300.0))) <= 1e-6
 
 
+@pytest.mark.parametrize("shape", [(12, 9, 7), (6, 3, 3), (5, 2, 6), (4, 6, 1)])
+def test_conduction_product_bitwise(mods, shape):
+    """A x on the device equals SciPy's CSR product bit for bit: (12, 9, 7) and (6, 3, 3) take the
+    stencil form (each row summed in column order), the grids with a lateral extent below 3 the CSR rows."""
+
+    from porerad import _native
+
+    rng = np.random.default_rng(sum(shape))
+    phases = (rng.random(shape) < 0.3).astype(np.uint8)
+    P, op, system = _product(mods, phases, 0.01, 8, [5])
+    D = _native.Coupled(system.matrix, system.hot_rhs, system.cold_rhs, 2, P.geometry(), op.device)
+    x = rng.normal(size=system.matrix.shape[0]) * 300.0
+    D.set(0, x)
+    D.matvec(0, 1)
+    assert np.array_equal(D.get(1), system.matrix @ x)
+
+
 @pytest.mark.parametrize("eps", [0.3, 0.9])
 def test_plate_matches_one_dimensional_balance(mods, eps):
     _, _, _, cp, _ = mods
