@@ -760,16 +760,21 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_wave(const uint2 *fin, 
     constexpr int N = 64 * EPL;
     constexpr int WAVES = COUNT_THREADS / 64;
     __shared__ int32_t s_pos[WAVES][N + 1];
+    __shared__ int32_t s_key[WAVES][N];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     int32_t *pos = s_pos[wave];
+    int32_t *ukey = s_key[wave];
     const int SR = S * R;
     for (int64_t q = (int64_t)blockIdx.x * WAVES + wave; q < n_p; q += (int64_t)gridDim.x * WAVES) {
         int32_t key[EPL];
         uint32_t bad = 0;
+        // key r of a lane is the patch's ray lane + 64 r, so each load
+        // instruction reads 64 consecutive final states (the sort below does
+        // not depend on where a key starts)
 #pragma unroll
         for (int r = 0; r < EPL; ++r) {
-            const int t = lane * EPL + r;
+            const int t = lane + 64 * r;
             int32_t v = INT_MAX;
             if (t < SR) {
                 const int s = S == 1 ? 0 : t / R, u = t - s * R;
@@ -834,7 +839,7 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_wave(const uint2 *fin, 
             const int e = lane * EPL + r;
             const int32_t prev = r == 0 ? prev_last : key[r - 1];
             if (e < SR && (e == 0 || key[r] != prev)) {
-                uniq_recv[q * SR + rank] = key[r];
+                ukey[rank] = key[r];
                 pos[rank] = e;
                 ++rank;
             }
@@ -846,7 +851,11 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_wave(const uint2 *fin, 
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        for (int u = lane; u < total; u += 64) uniq_cnt[q * SR + u] = pos[u + 1] - pos[u];
+        // receivers and counts leave LDS in consecutive runs of 64
+        for (int u = lane; u < total; u += 64) {
+            uniq_recv[q * SR + u] = ukey[u];
+            uniq_cnt[q * SR + u] = pos[u + 1] - pos[u];
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
