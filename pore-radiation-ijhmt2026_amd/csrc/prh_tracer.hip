@@ -419,7 +419,14 @@ __device__ __forceinline__ void gen_rays_body(const GenArgs &a, uint64_t base)
     const GeoView &G = a.G;
     const uint32_t seg = a.n_p * a.R;
     const double fny = (double)G.ny, fnz = (double)G.nz;
-    if (base >= a.total) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    // records leave through LDS: each round's 64 records (4 KB + 2 KB of wall
+    // inputs) are stored by consecutive lanes in consecutive 16 B pieces
+    __shared__ uint4 s_dda[TRACE_THREADS / 64][64 * 4];
+    __shared__ uint4 s_wall[TRACE_THREADS / 64][64 * 2];
+    uint4 *sd = s_dda[threadIdx.x >> 6], *sw = s_wall[threadIdx.x >> 6];
+    const uint64_t wave_base = base - lane;
+    if (wave_base >= a.total) return;
     // (seed, patch, ray) of the lane's first ray; later rays are 64 further on
     uint32_t s = (uint32_t)base / seg;
     uint32_t q = ((uint32_t)base - s * seg) / a.R;
@@ -427,8 +434,10 @@ __device__ __forceinline__ void gen_rays_body(const GenArgs &a, uint64_t base)
     uint32_t s_cur = 0xFFFFFFFFu;
     u128 st = make_u128(0, 0), inc = make_u128(0, 0), step_inc = make_u128(0, 0);
     for (uint32_t k = 0; k < GEN_ROUNDS; ++k) {
-        const uint64_t g64 = base + 64ull * k;
-        if (g64 >= a.total) break;
+        const uint64_t round0 = wave_base + 64ull * k;  // first ray of the round (lane 0's)
+        if (round0 >= a.total) break;                   // the same for every lane of the wave
+        const uint64_t g64 = round0 + lane;
+        const bool valid = g64 < a.total;
         const uint32_t g = (uint32_t)g64 + a.offset;
         if (k > 0) {
             r += 64u;
@@ -440,27 +449,56 @@ __device__ __forceinline__ void gen_rays_body(const GenArgs &a, uint64_t base)
                 }
             }
         }
-        const uint32_t p = a.p0 + q;
-        if (s != s_cur) {
-            // first ray of this lane, or the stride crossed into the next seed
-            u128 st0;
-            load_seed(a.seeds, (int)s, st0, inc);
-            st = pcg_jump(st0, inc, 4ull * ((uint64_t)p * a.R + r), c_jump);
-            step_inc = mul128(c_jump.G[8], inc);
-            s_cur = s;
-        }
-        u128 next = add128(mul128(c_jump.A[8], st), step_inc);  // 64 rays = 256 draws on
-        const double *po = G.patch_origin + 3 * (size_t)p;
+        DdaRec rec;
+        WallRec w;
         double o[3], d[3];
-        make_ray(st, inc, po[0], po[1], po[2], normal_code_of(G, p), fny, fnz, o, d);
-        st = next;
+        if (valid) {
+            const uint32_t p = a.p0 + q;
+            if (s != s_cur) {
+                // first ray of this lane, or the stride crossed into the next seed
+                u128 st0;
+                load_seed(a.seeds, (int)s, st0, inc);
+                st = pcg_jump(st0, inc, 4ull * ((uint64_t)p * a.R + r), c_jump);
+                step_inc = mul128(c_jump.G[8], inc);
+                s_cur = s;
+            }
+            u128 next = add128(mul128(c_jump.A[8], st), step_inc);  // 64 rays = 256 draws on
+            const double *po = G.patch_origin + 3 * (size_t)p;
+            make_ray(st, inc, po[0], po[1], po[2], normal_code_of(G, p), fny, fnz, o, d);
+            st = next;
+            if (RECORDS) make_dda(o, d, G, a.C, rec, w);
+        }
         if (RECORDS) {
-            DdaRec rec;
-            WallRec w;
-            make_dda(o, d, G, a.C, rec, w);
-            a.dda[g] = rec;
-            a.walls[g] = w;
-        } else {
+            if (valid) {
+                const uint4 *rv = reinterpret_cast<const uint4 *>(&rec);
+                const uint4 *wv = reinterpret_cast<const uint4 *>(&w);
+                sd[4 * lane + 0] = rv[0];
+                sd[4 * lane + 1] = rv[1];
+                sd[4 * lane + 2] = rv[2];
+                sd[4 * lane + 3] = rv[3];
+                sw[2 * lane + 0] = wv[0];
+                sw[2 * lane + 1] = wv[1];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const uint64_t n_round = a.total - round0 < 64 ? a.total - round0 : 64;  // records of this round
+            uint4 *dd = reinterpret_cast<uint4 *>(a.dda + (uint32_t)round0 + a.offset);
+            uint4 *wd = reinterpret_cast<uint4 *>(a.walls + (uint32_t)round0 + a.offset);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const uint32_t piece = lane + 64u * m;
+                if ((piece >> 2) < n_round) dd[piece] = sd[piece];
+            }
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                const uint32_t piece = lane + 64u * m;
+                if ((piece >> 1) < n_round) wd[piece] = sw[piece];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        } else if (valid) {
             double *oo = a.origins + 3 * (size_t)g;
             double *dd = a.directions + 3 * (size_t)g;
             oo[0] = o[0];
