@@ -9,9 +9,12 @@
 //   prh_stamp_shapes    the N^3 membership tests of shape_mask's short shapes
 //                       (ellipsoids and cylinders, geometry.py:106-116) for a
 //                       whole list of shapes in one pass, OR-ed into the field
-//   prh_stamp_offsets   _long_cylinder's scatter of lattice nodes thickened by
-//                       disk offsets (geometry.py:63-86), with the whole-cell
-//                       roll of integer centres (:100-105) folded in
+//   prh_stamp_long_cylinders
+//                       every long cylinder of a render in one launch
+//                       (_long_cylinder, geometry.py:63-86, with the whole-cell
+//                       roll of integer centres, :100-105): the disk test of
+//                       each candidate offset on the device, then the scatter
+//   prh_stamp_offsets   the same scatter from host-formed disk offsets
 //   prh_threshold_field threshold (geometry.py:235-238) of a field, optionally
 //                       after lamellar_field's seven-point noise smoothing
 //                       (:229-232): the two order statistics of np.quantile's
@@ -25,10 +28,13 @@
 //                                     rounded product, then two FMAs
 //   ellipsoid  ((l0/rx)^2 + (l1/ry)^2) + (l2/rz)^2 <= 1
 //   cylinder   (l0/rx)^2 + (l1/ry)^2 <= 1  and  |l2| <= rz
-// Everything else is compiled without contraction.  A cell whose offsets lie
-// beyond a shape's reach on some axis is not tested against it (the shape's
-// points lie within its reach of the centre; rounding of the rotation is
-// far below the one-cell margin of the reach).
+// The disk test of a long cylinder forms l = Rq^T o with the same FMA chain
+// (NumPy's small product rounds so as well).  Everything else is compiled
+// without contraction.  A cell whose offsets lie beyond a shape's reach on
+// some axis is not tested against it (the shape's points lie within its
+// reach of the centre; rounding of the rotation is far below the one-cell
+// margin of the reach; the host gives a matrix that is not a rotation a
+// reach that covers every cell).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
