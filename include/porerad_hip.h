@@ -119,6 +119,11 @@ int prh_pcg64_seed(const uint32_t *words, int32_t n_words, uint64_t state_inc[4]
  * porerad/view_factors.py:70-132. */
 int prh_geometry_create(const uint8_t *solid_x, int32_t nx, int32_t ny, int32_t nz, double H_m,
                         prh_geometry **out);
+/* A geometry with the occupancy bitsets only (no patch tables): enough for
+ * prh_mc_properties / prh_mc_penetration, which read nothing else; tracing
+ * calls on it fail their patch-range checks.  No counterpart in the
+ * reference (its Monte Carlo reads the phase array directly). */
+int prh_geometry_create_occupancy(const uint8_t *solid_x, int32_t nx, int32_t ny, int32_t nz, prh_geometry **out);
 int prh_geometry_sizes(const prh_geometry *g, int64_t *n_patches, int64_t *n_wall);
 /* Download the patch table (arrays of n_patches; origin is n_patches x 3).
  * Any pointer may be NULL.  Replaces the SurfacePatches fields,

@@ -273,10 +273,72 @@ int build(const uint8_t *solid_host, int nx, int ny, int nz, double H_m, Geometr
     return PRH_OK;
 }
 
+// The occupancy bitsets only (no patch tables): what the Monte Carlo
+// kernels read.
+int build_occupancy(const uint8_t *solid_host, int nx, int ny, int nz, Geometry &g)
+{
+    hipStream_t st;
+    int rc = current_stream(&st);
+    if (rc) return rc;
+    PRH_HIP(hipGetDevice(&g.device));
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.H_m = 1.0;
+    g.n_patches = g.n_wall = 0;
+    g.occ_words = ((int64_t)nx * ny * nz + 31) / 32;
+    for (g.coarse_shift = 1;; ++g.coarse_shift) {
+        const int B = 1 << g.coarse_shift;
+        g.cnx = (nx + B - 1) / B;
+        g.cny = (ny + B - 1) / B;
+        g.cnz = (nz + B - 1) / B;
+        g.coarse_words = ((int64_t)g.cnx * g.cny * g.cnz + 31) / 32;
+        if (g.coarse_words * 4 <= 72 * 1024) break;
+    }
+    const int64_t n_cells = (int64_t)nx * ny * nz;
+    uint8_t *solid = nullptr;
+    PRH_HIP(hipMalloc((void **)&solid, n_cells));
+    const hipError_t e1 = hipMalloc((void **)&g.occ, (size_t)g.occ_words * sizeof(uint32_t));
+    const hipError_t e2 = hipMalloc((void **)&g.coarse, (size_t)g.coarse_words * sizeof(uint32_t));
+    hipError_t e3 = hipSuccess;
+    if (e1 == hipSuccess && e2 == hipSuccess) {
+        e3 = hipMemcpyAsync(solid, solid_host, n_cells, hipMemcpyHostToDevice, st);
+        k_pack_occupancy<<<grid_for(g.occ_words), GEO_THREADS, 0, st>>>(solid, n_cells, g.occ);
+        k_pack_coarse<<<grid_for(g.coarse_words), GEO_THREADS, 0, st>>>(solid, nx, ny, nz, g.coarse_shift, g.cnx,
+                                                                        g.cny, g.cnz, g.coarse);
+        if (e3 == hipSuccess) e3 = hipGetLastError();
+        if (e3 == hipSuccess) e3 = hipStreamSynchronize(st);
+    }
+    (void)hipFree(solid);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+        set_error("prh_geometry_create_occupancy: %s",
+                  hipGetErrorString(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3)));
+        return PRH_ERR_HIP;
+    }
+    return PRH_OK;
+}
+
 }  // namespace
 }  // namespace prh
 
 extern "C" {
+
+int prh_geometry_create_occupancy(const uint8_t *solid_x, int32_t nx, int32_t ny, int32_t nz, prh_geometry **out)
+{
+    PRH_CHECK_ARG(out && solid_x, "prh_geometry_create_occupancy: null pointer");
+    PRH_CHECK_ARG(nx > 0 && ny > 0 && nz > 0, "prh_geometry_create_occupancy: grid sizes must be positive");
+    PRH_CHECK_ARG((int64_t)nx * ny * nz < (int64_t(1) << 31), "prh_geometry_create_occupancy: grid too large");
+    *out = nullptr;
+    prh_geometry *geo = new prh_geometry();
+    int rc = prh::build_occupancy(solid_x, nx, ny, nz, geo->g);
+    if (rc) {
+        prh::free_geometry(geo->g);
+        delete geo;
+        return rc;
+    }
+    *out = geo;
+    return PRH_OK;
+}
 
 int prh_geometry_create(const uint8_t *solid_x, int32_t nx, int32_t ny, int32_t nz, double H_m, prh_geometry **out)
 {

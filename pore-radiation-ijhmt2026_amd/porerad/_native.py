@@ -50,6 +50,7 @@ SIGNATURES = {
     "prh_tracer_grain": ([_P], ctypes.c_int),
     "prh_tracer_trim": ([], ctypes.c_int),
     "prh_geometry_create": ([_P, _I32, _I32, _I32, _F64, _P], ctypes.c_int),
+    "prh_geometry_create_occupancy": ([_P, _I32, _I32, _I32, _P], ctypes.c_int),
     "prh_geometry_sizes": ([_P, _P, _P], ctypes.c_int),
     "prh_geometry_patch_table": ([_P, _P, _P, _P, _P], ctypes.c_int),
     "prh_geometry_destroy": ([_P], None),
@@ -258,6 +259,24 @@ class Geometry:
         n, nw = ctypes.c_int64(), ctypes.c_int64()
         check(L.prh_geometry_sizes(handle, ctypes.byref(n), ctypes.byref(nw)), "prh_geometry_sizes")
         self.n_patches, self.n_wall = int(n.value), int(nw.value)
+
+    @classmethod
+    def occupancy_only(cls, solid_x: np.ndarray) -> "Geometry":
+        """The occupancy bitsets of an x-oriented solid array, without patch tables (the Monte Carlo's input)."""
+
+        solid = np.ascontiguousarray(solid_x, dtype=np.uint8)
+        if solid.ndim != 3:
+            raise ValueError("expected a 3-D phase array")
+        self = cls.__new__(cls)
+        self.shape = tuple(int(s) for s in solid.shape)
+        handle = ctypes.c_void_p()
+        L = lib()
+        check(L.prh_geometry_create_occupancy(ptr(solid), *self.shape, ctypes.byref(handle)),
+              "prh_geometry_create_occupancy")
+        self._lib = L
+        self.handle = handle
+        self.n_patches, self.n_wall = 0, 0
+        return self
 
     def patch_table(self):
         n = self.n_patches

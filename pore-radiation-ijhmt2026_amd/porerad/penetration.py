@@ -23,7 +23,7 @@ import numpy as np
 
 from . import _native
 from .constants import SIGMA
-from .properties import seed_average, solid_along_x
+from .properties import occupancy, seed_average
 
 SEEDS = (321, 654)
 N_RAYS = 500_000
@@ -54,21 +54,21 @@ def _binomial(count: int, n: int) -> tuple[float, float]:
 
 
 def trace_penetration(phases: np.ndarray, axis: str, epsilon: float, *, n_rays: int = N_RAYS, seed: int = SEEDS[0],
-                      restart: str = "article", max_bounces: int = MAX_BOUNCES) -> dict:
+                      restart: str = "article", max_bounces: int = MAX_BOUNCES, _occupancy=None) -> dict:
     """R, T, A with binomial standard errors for one geometry, axis, eps and seed (penetration.py:281-305)."""
 
     if restart not in RESTART_RULES:
         raise ValueError(f"restart must be one of {RESTART_RULES}")
-    solid = solid_along_x(phases, axis)
     n = int(n_rays)
     counts = np.zeros(4, np.int64)
-    geo = _native.Geometry(solid, 1.0)
+    geo = _occupancy if _occupancy is not None else occupancy(phases, axis)
     try:
         _native.check(_native.lib().prh_mc_penetration(
             geo.handle, n, int(seed) & 0xFFFFFFFF, float(epsilon), int(max_bounces),
             1 if restart == "hit_point" else 0, _native.ptr(counts)), "prh_mc_penetration")
     finally:
-        geo.close()
+        if _occupancy is None:
+            geo.close()
     out = {"n_rays": n, "terminated": int(counts[3])}
     for name, count in (("reflectance", counts[0]), ("transmittance", counts[1]), ("absorptance", counts[2])):
         out[name], out[f"{name}_se"] = _binomial(int(count), n)
@@ -79,7 +79,12 @@ def penetration_response(phases: np.ndarray, axis: str, epsilon: float, *, seeds
                          restart: str = "article") -> dict:
     """R, T, A of one geometry, axis and emissivity, averaged over independent seeds (penetration.py:308-315)."""
 
-    runs = [trace_penetration(phases, axis, epsilon, n_rays=n_rays, seed=s, restart=restart) for s in seeds]
+    geo = occupancy(phases, axis)  # one upload serves every seed
+    try:
+        runs = [trace_penetration(phases, axis, epsilon, n_rays=n_rays, seed=s, restart=restart, _occupancy=geo)
+                for s in seeds]
+    finally:
+        geo.close()
     out = {"n_rays": int(sum(r["n_rays"] for r in runs)), "terminated": int(sum(r["terminated"] for r in runs))}
     for key in ("reflectance", "transmittance", "absorptance"):
         out[key], out[f"{key}_se"] = seed_average([r[key] for r in runs], [r[f"{key}_se"] for r in runs])

@@ -50,30 +50,32 @@ def right_censored_mle(events: np.ndarray, censored: np.ndarray) -> tuple[float,
     return float(n_events / exposure), float(math.sqrt(n_events) / exposure)
 
 
-def _geometry(solid: np.ndarray) -> "_native.Geometry":
-    return _native.Geometry(solid, 1.0)
+def occupancy(phases: np.ndarray, axis: str) -> "_native.Geometry":
+    """The device occupancy of the solid with the transfer axis first (all the ray kernels read)."""
+
+    return _native.Geometry.occupancy_only(solid_along_x(phases, axis))
 
 
 def trace_properties(phases: np.ndarray, axis: str, epsilon: float, *, n_rays: int = N_RAYS, seed: int = SEEDS[0],
-                     max_bounces: int = MAX_BOUNCES, restart: str = "article") -> dict:
+                     max_bounces: int = MAX_BOUNCES, restart: str = "article", _occupancy=None) -> dict:
     """beta (per cell), omega and g for one geometry, transfer axis, emissivity and seed (properties.py:54-307)."""
 
     if restart not in RESTART_RULES:
         raise ValueError(f"restart must be one of {RESTART_RULES}")
-    solid = solid_along_x(phases, axis)
     n = int(n_rays)
     events_all = np.empty(n, np.float64)
     censored_all = np.empty(n, np.float64)
     counters = np.zeros(2, np.int64)
     sums = np.zeros(2, np.float64)
-    geo = _geometry(solid)
+    geo = _occupancy if _occupancy is not None else occupancy(phases, axis)
     try:
         _native.check(_native.lib().prh_mc_properties(
             geo.handle, n, int(seed) & 0xFFFFFFFF, float(epsilon), int(max_bounces),
             1 if restart == "hit_point" else 0, _native.ptr(events_all), _native.ptr(censored_all),
             _native.ptr(counters), _native.ptr(sums)), "prh_mc_properties")
     finally:
-        geo.close()
+        if _occupancy is None:
+            geo.close()
     events = events_all[events_all >= 0.0]
     censored = censored_all[censored_all >= 0.0]
     beta, beta_se = right_censored_mle(events, censored)
@@ -102,7 +104,12 @@ def identify_properties(phases: np.ndarray, axis: str, epsilon: float, *, seeds=
                         restart: str = "article") -> dict:
     """Properties of one geometry, axis and emissivity, averaged over independent seeds (properties.py:319-329)."""
 
-    runs = [trace_properties(phases, axis, epsilon, n_rays=n_rays, seed=s, restart=restart) for s in seeds]
+    geo = occupancy(phases, axis)  # one upload serves every seed
+    try:
+        runs = [trace_properties(phases, axis, epsilon, n_rays=n_rays, seed=s, restart=restart, _occupancy=geo)
+                for s in seeds]
+    finally:
+        geo.close()
     out = {"n_rays": int(sum(r["n_rays"] for r in runs))}
     for key in ("beta", "omega", "g"):
         mean, se = seed_average([r[key] for r in runs], [r[f"{key}_se"] for r in runs])
