@@ -13,6 +13,8 @@
  *
  *   prh_pcg64_seed            numpy PCG64(seed) seeding used by
  *                             generate_rays, porerad/view_factors.py:149-150
+ *   prh_libm_sincos           np.cos / np.sin of the azimuth in generate_rays,
+ *                             porerad/view_factors.py:168-176
  *   prh_geometry_create       build_surface_patches, porerad/view_factors.py:70-132
  *   prh_geometry_patch_table  the SurfacePatches arrays, porerad/view_factors.py:43-67
  *   prh_generate_rays         generate_rays, porerad/view_factors.py:135-182
@@ -112,6 +114,13 @@ int prh_tracer_grain(int32_t *records);
  * Replaces np.random.PCG64(int(seed)), porerad/view_factors.py:149. */
 int prh_pcg64_seed(const uint32_t *words, int32_t n_words, uint64_t state_inc[4]);
 
+/* sin(x[i]) and cos(x[i]) with the host C library's bits (glibc's dbl-64
+ * sin/cos as its FMA build on x86-64 rounds them), the routine the ray
+ * generation uses for the azimuth; on the host (on_device 0, no GPU needed)
+ * or on the current device.  |x| < 105414350, finite.  Replaces NumPy's
+ * cos/sin of 2 pi u, porerad/view_factors.py:168-176. */
+int prh_libm_sincos(const double *x, int64_t n, int32_t on_device, double *sin_x, double *cos_x);
+
 /* Upload an x-oriented solid/pore grid (C order, nonzero = solid, i.e. the
  * phases == 1 mask) and build the patch table on the device: wall patches
  * first in (j, k, hot/cold) order, then exposed solid faces in (i, j, k, face)
@@ -120,9 +129,10 @@ int prh_pcg64_seed(const uint32_t *words, int32_t n_words, uint64_t state_inc[4]
 int prh_geometry_create(const uint8_t *solid_x, int32_t nx, int32_t ny, int32_t nz, double H_m,
                         prh_geometry **out);
 /* A geometry with the occupancy bitsets only (no patch tables): enough for
- * prh_mc_properties / prh_mc_penetration, which read nothing else; tracing
- * calls on it fail their patch-range checks.  No counterpart in the
- * reference (its Monte Carlo reads the phase array directly). */
+ * prh_mc_properties / prh_mc_penetration, which read nothing else.
+ * prh_trace_rays rejects it (PRH_ERR_ARG); calls that take a patch range see
+ * zero patches.  No counterpart in the reference (its Monte Carlo reads the
+ * phase array directly). */
 int prh_geometry_create_occupancy(const uint8_t *solid_x, int32_t nx, int32_t ny, int32_t nz, prh_geometry **out);
 int prh_geometry_sizes(const prh_geometry *g, int64_t *n_patches, int64_t *n_wall);
 /* Download the patch table (arrays of n_patches; origin is n_patches x 3).
@@ -243,6 +253,9 @@ int prh_generate_rays(const prh_geometry *g, int64_t p0, int64_t p1, int32_t ray
                       const uint64_t state_inc[4], double *origins, double *directions);
 
 /* First patch hit by each given ray (receiver = -1 where unresolved).
+ * Origins may lie anywhere within |o| < 2^31 (lateral indices wrap as the
+ * reference's j % ny, k % nz; an x index outside [0, nx) is the wall of that
+ * side at once); non-finite values or larger origins are PRH_ERR_ARG.
  * Returns PRH_ERR_UNRESOLVED with *n_failed > 0 if any ray failed.
  * Replaces trace_first_hit, porerad/view_factors.py:261-270. */
 int prh_trace_rays(const prh_geometry *g, const double *origins, const double *directions, int64_t n_rays,

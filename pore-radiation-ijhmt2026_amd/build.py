@@ -26,6 +26,7 @@ BUILD_DIR = os.path.join(HERE, "_build")
 SOURCES = [
     "prh_runtime.cpp",
     "prh_scan.hip",
+    "prh_libm.hip",
     "prh_geometry.hip",
     "prh_tracer.hip",
     "prh_operator.hip",

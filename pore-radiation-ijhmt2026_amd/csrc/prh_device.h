@@ -5,6 +5,7 @@
 #pragma once
 
 #include "prh_internal.h"
+#include "prh_libm.h"
 
 namespace prh {
 
@@ -99,7 +100,7 @@ __device__ __forceinline__ void make_ray(u128 &s, const u128 inc, const double p
     const double radius = sqrt(u2);
     const double phi = TWO_PI * u3;
     double sphi, cphi;
-    sincos(phi, &sphi, &cphi);
+    libm::sincos(phi, sphi, cphi);  // the C library's bits, as NumPy's cos/sin give them
     const double la = radius * cphi;
     const double lb = radius * sphi;
     const double ct = sqrt(1.0 - u2);
@@ -123,31 +124,44 @@ __device__ __forceinline__ void make_ray(u128 &s, const u128 inc, const double p
     d[2] = d2 / norm;
 }
 
-// _initial_cell, view_factors.py:185-190 (round() = round-half-even).
-__device__ __forceinline__ int initial_cell(double position, double direction)
+// _initial_cell, view_factors.py:185-190 (round() = round-half-even), as an
+// integral double: exact for any origin the host accepts (|o| < 2^31).
+__device__ __forceinline__ double initial_cell(double position, double direction)
 {
-    int index = (int)floor(position);
-    if (direction < 0.0 && fabs(position - rint(position)) <= TIE_TOLERANCE) index -= 1;
+    double index = floor(position);
+    if (direction < 0.0 && fabs(position - rint(position)) <= TIE_TOLERANCE) index -= 1.0;
     return index;
 }
 
-// _axis_state, view_factors.py:193-199.
-__device__ __forceinline__ void axis_state(double position, double direction, int index, int &step, double &t_max,
-                                  double &t_delta)
+// _axis_state, view_factors.py:193-199 (index + 1.0 is exact: index is integral).
+__device__ __forceinline__ void axis_state(double position, double direction, double index, int &step,
+                                           double &t_max, double &t_delta)
 {
     if (direction > 0.0) {
         step = 1;
-        t_max = ((double)index + 1.0 - position) / direction;
+        t_max = (index + 1.0 - position) / direction;
         t_delta = 1.0 / direction;
     } else if (direction < 0.0) {
         step = -1;
-        t_max = ((double)index - position) / direction;
+        t_max = (index - position) / direction;
         t_delta = -1.0 / direction;
     } else {
         step = 0;
         t_max = INF_T;
         t_delta = INF_T;
     }
+}
+
+// Python's f % n for an integral double f with |f| < 2^52, inv_n = 1 / n
+// rounded: the quotient's floor may be one off, so one correction either way
+// (every value is an exact integer).
+__device__ __forceinline__ int floor_mod(double f, int n, double inv_n)
+{
+    const double dn = (double)n;
+    double r = f - dn * floor(f * inv_n);
+    r = r < 0.0 ? r + dn : r;
+    r = r >= dn ? r - dn : r;
+    return (int)r;
 }
 
 // Step cap of _trace_kernel, view_factors.py:223-224.

@@ -21,18 +21,17 @@
 // fixed order (the reference accumulates with atomics in no fixed order).
 //
 // The Lambertian azimuth is phi = 2 pi m / 2^24 for the 24-bit integer m of
-// the second draw, so it takes only 2^24 values: the kernels read cos(phi)
-// and sin(phi) from a table of the host C library's values (built once per
-// device, 256 MB), the functions the reference's CPU kernel calls.  The rays
-// then follow the reference's paths bit for bit.
+// the second draw; cos(phi) and sin(phi) come from prh_libm.h, which gives
+// the host C library's bits (the functions the reference's CPU kernel calls;
+// checked on all 2^24 values of phi).  The rays then follow the reference's
+// paths bit for bit.
 #include <algorithm>
 #include <climits>
 #include <cmath>
-#include <mutex>
-#include <thread>
 #include <vector>
 
 #include "prh_internal.h"
+#include "prh_libm.h"
 
 namespace prh {
 namespace {
@@ -43,7 +42,6 @@ constexpr double RESTART_OFFSET = 1.0e-9;
 
 struct McView {
     const uint32_t *occ;
-    const double2 *trig;  // (cos, sin) of 2 pi m / 2^24
     int nx, ny, nz;
     int max_steps;
 };
@@ -65,7 +63,7 @@ __device__ __forceinline__ double rand01(uint32_t &state)
     return (double)(state & 0x00FFFFFFu) / (double)0x01000000u;
 }
 
-constexpr uint32_t TRIG_ENTRIES = 1u << 24;
+constexpr double AZIMUTH_STEPS = 16777216.0;  // 2^24
 constexpr double TWO_PI = 6.283185307179586;  // 2.0 * math.pi, exact
 
 __device__ __forceinline__ int floor_mod(int a, int n)
@@ -179,7 +177,9 @@ __device__ __forceinline__ void lambertian(const McView &V, uint32_t &state, dou
 {
     const double xi1 = rand01(state);
     (void)rand01(state);  // xi2 = m / 2^24 with m the low 24 bits of the state
-    const double2 cs = V.trig[state & 0x00FFFFFFu];
+    const double phi = TWO_PI * ((double)(state & 0x00FFFFFFu) / AZIMUTH_STEPS);
+    double sin_phi, cos_phi;
+    libm::sincos(phi, sin_phi, cos_phi);
     double hx = 1.0, hy = 0.0, hz = 0.0;
     if (fabs(nxn) > 0.9) {
         hx = 0.0;
@@ -197,8 +197,8 @@ __device__ __forceinline__ void lambertian(const McView &V, uint32_t &state, dou
     const double vy = nzn * ux - nxn * uz;
     const double vz = nxn * uy - nyn * ux;
     const double radius = sqrt(xi1);
-    const double cx = radius * cs.x;
-    const double cy = radius * cs.y;
+    const double cx = radius * cos_phi;
+    const double cy = radius * sin_phi;
     const double cz = sqrt(fmax(0.0, 1.0 - xi1));
     dx = cx * ux + cy * vx + cz * nxn;
     dy = cx * uy + cy * vy + cz * nyn;
@@ -386,44 +386,10 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_totals(const double *part, in
     if (threadIdx.x == 0) out[q] = sh[0];
 }
 
-McView view_of(const Geometry &g, const double2 *trig)
+McView view_of(const Geometry &g)
 {
     const int64_t cap = 10 * (int64_t)g.nx * g.ny * g.nz + 10;
-    return McView{g.occ, trig, g.nx, g.ny, g.nz, (int)std::min<int64_t>(cap, INT32_MAX)};
-}
-
-// The (cos, sin) table of the current device, built on first use and kept.
-double2 *g_trig[64] = {};
-std::mutex g_trig_lock;
-
-int trig_table(int device, const double2 **out)
-{
-    PRH_CHECK_ARG(device >= 0 && device < 64, "prh_mc: device %d out of range", device);
-    std::lock_guard<std::mutex> lock(g_trig_lock);
-    if (!g_trig[device]) {
-        std::vector<double2> host(TRIG_ENTRIES);
-        constexpr unsigned WORKERS = 16;
-        std::vector<std::thread> pool;
-        for (unsigned w = 0; w < WORKERS; ++w)
-            pool.emplace_back([&host, w]() {
-                for (uint32_t m = w; m < TRIG_ENTRIES; m += WORKERS) {
-                    const double phi = TWO_PI * ((double)m / (double)TRIG_ENTRIES);
-                    host[m] = make_double2(std::cos(phi), std::sin(phi));
-                }
-            });
-        for (std::thread &t : pool) t.join();
-        double2 *dev = nullptr;
-        PRH_HIP(hipMalloc((void **)&dev, sizeof(double2) * TRIG_ENTRIES));
-        const hipError_t e = hipMemcpy(dev, host.data(), sizeof(double2) * TRIG_ENTRIES, hipMemcpyHostToDevice);
-        if (e != hipSuccess) {
-            (void)hipFree(dev);
-            set_error("prh_mc: table upload failed: %s", hipGetErrorString(e));
-            return PRH_ERR_HIP;
-        }
-        g_trig[device] = dev;
-    }
-    *out = g_trig[device];
-    return PRH_OK;
+    return McView{g.occ, g.nx, g.ny, g.nz, (int)std::min<int64_t>(cap, INT32_MAX)};
 }
 
 // Device buffers of one call, freed on every return.
@@ -471,9 +437,7 @@ int prh_mc_properties(const prh_geometry *geo, int64_t n_rays, uint32_t seed, do
         set_error("prh_mc_properties: out of device memory");
         return PRH_ERR_HIP;
     }
-    const double2 *trig = nullptr;
-    if ((rc = trig_table(g.device, &trig))) return rc;
-    const McView V = view_of(g, trig);
+    const McView V = view_of(g);
     if (restart_hit_point)
         k_mc_properties<true><<<grid, MC_THREADS, 0, st>>>(V, n_rays, seed, epsilon, max_bounces, ev, ce, in, sc, m1, m2);
     else
@@ -512,9 +476,7 @@ int prh_mc_penetration(const prh_geometry *geo, int64_t n_rays, uint32_t seed, d
         set_error("prh_mc_penetration: out of device memory");
         return PRH_ERR_HIP;
     }
-    const double2 *trig = nullptr;
-    if ((rc = trig_table(g.device, &trig))) return rc;
-    const McView V = view_of(g, trig);
+    const McView V = view_of(g);
     if (restart_hit_point)
         k_mc_penetration<true><<<grid, MC_THREADS, 0, st>>>(V, n_rays, seed, epsilon, max_bounces, fate);
     else

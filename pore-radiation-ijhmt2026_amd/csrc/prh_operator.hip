@@ -629,12 +629,21 @@ int mixed_solve_single(prh_operator *op, const double *E, double emissivity, dou
     PRH_CHECK_ARG(op && E && absorbed, "mixed_solve_single: null pointer");
     PRH_CHECK_ARG(emissivity > 0.0 && emissivity <= 1.0 && rtol > 0.0, "mixed_solve_single: bad emissivity or tolerance");
     const int64_t n = op->n;
-    if (!op->mix32) {
-        if (hipMalloc((void **)&op->mix32, sizeof(float) * 5 * n) != hipSuccess ||
-            hipMalloc((void **)&op->mix64, sizeof(double) * (3 * n + 1)) != hipSuccess) {
+    if (!op->mix32 || !op->mix64) {
+        // both buffers or neither: the handle never keeps half a pair
+        float *m32 = nullptr;
+        double *m64 = nullptr;
+        if (hipMalloc((void **)&m32, sizeof(float) * 5 * n) != hipSuccess ||
+            hipMalloc((void **)&m64, sizeof(double) * (3 * n + 1)) != hipSuccess) {
+            if (m32) (void)hipFree(m32);
+            (void)hipGetLastError();
             set_error("mixed_solve_single: out of device memory");
             return PRH_ERR_HIP;
         }
+        if (op->mix32) (void)hipFree(op->mix32);
+        if (op->mix64) (void)hipFree(op->mix64);
+        op->mix32 = m32;
+        op->mix64 = m64;
     }
     const Mix1 V{op->mix32, op->mix32 + n, op->mix32 + 2 * n, op->mix64, op->mix64 + n, op->mix64 + 2 * n};
     float *z[2] = {op->mix32 + 3 * n, op->mix32 + 4 * n};

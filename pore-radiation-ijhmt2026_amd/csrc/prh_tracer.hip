@@ -165,18 +165,21 @@ __device__ __forceinline__ void make_dda(const double o[3], const double d[3], c
         rec.ij = rec.ks = rec.cap = rec.pad = 0u;
         return;
     }
-    const int i = initial_cell(o[0], d[0]);
-    const int j = initial_cell(o[1], d[1]);
-    const int k = initial_cell(o[2], d[2]);
+    const double i = initial_cell(o[0], d[0]);
+    const double j = initial_cell(o[1], d[1]);
+    const double k = initial_cell(o[2], d[2]);
     int sx, sy, sz;
     axis_state(o[0], d[0], i, sx, rec.tx, rec.ddx);
     axis_state(o[1], d[1], j, sy, rec.ty, rec.ddy);
     axis_state(o[2], d[2], k, sz, rec.tz, rec.ddz);
-    // Origins lie in [0, n] (wrapped by float modulo), so j, k are in [-1, n]
-    // and i in [-1, nx].
-    const int jj = j < 0 ? j + G.ny : (j >= G.ny ? j - G.ny : j);
-    const int kk = k < 0 ? k + G.nz : (k >= G.nz ? k - G.nz : k);
-    rec.ij = (uint32_t)(i + 1) | (uint32_t)jj << 16;
+    // The reference keeps j, k unbounded and reads j % ny, k % nz; the
+    // traversal carries the wrapped indices from the start.  A ray whose
+    // x index starts outside [0, nx) ends at once on the wall of that side
+    // (hot below, cold above), so i is clamped to [-1, nx].
+    const int jj = floor_mod(j, G.ny, G.inv_ny);
+    const int kk = floor_mod(k, G.nz, G.inv_nz);
+    const int ic = i < 0.0 ? -1 : (i >= (double)G.nx ? G.nx : (int)i);
+    rec.ij = (uint32_t)(ic + 1) | (uint32_t)jj << 16;
     rec.ks = (uint32_t)kk | (uint32_t)(sx + 1) << 16 | (uint32_t)(sy + 1) << 18 | (uint32_t)(sz + 1) << 20;
     uint32_t cap = C.min_cap;
     if (fabs(d[0]) < C.grazing_dx) cap = max_steps_of(d[0], d[1], d[2], G.nx, G.ny, G.nz);
@@ -1207,9 +1210,17 @@ int prh_trace_rays(const prh_geometry *geo, const double *origins, const double 
 {
     PRH_CHECK_ARG(geo && receiver && (n_rays == 0 || (origins && directions)), "prh_trace_rays: null pointer");
     PRH_CHECK_ARG(n_rays >= 0 && n_rays < (int64_t(1) << 32), "prh_trace_rays: ray count out of range");
+    const Geometry &g = geo->g;
+    PRH_CHECK_ARG(g.n_patches > 0 && g.cell_code && g.hot_map && g.cold_map,
+                  "prh_trace_rays: the geometry has no patch tables (an occupancy-only handle)");
     if (n_failed) *n_failed = 0;
     if (n_rays == 0) return PRH_OK;
-    const Geometry &g = geo->g;
+    // The traversal's cell indices are 32-bit: origins within |o| < 2^31 (any
+    // other finite origin lies far outside every grid), finite directions.
+    for (int64_t r = 0; r < 3 * n_rays; ++r)
+        PRH_CHECK_ARG(std::isfinite(directions[r]) && std::isfinite(origins[r]) && fabs(origins[r]) < 2147483648.0,
+                      "prh_trace_rays: ray %lld has a non-finite direction or an origin outside |o| < 2^31",
+                      (long long)(r / 3));
     PRH_HIP(hipSetDevice(g.device));
     hipStream_t st;
     int rc = current_stream(&st);

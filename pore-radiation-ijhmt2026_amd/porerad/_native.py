@@ -44,6 +44,7 @@ SIGNATURES = {
     "prh_version": ([], ctypes.c_char_p),
     "prh_last_timings": ([_P, _I32], ctypes.c_int),
     "prh_pcg64_seed": ([_P, _I32, _P], ctypes.c_int),
+    "prh_libm_sincos": ([_P, _I64, _I32, _P, _P], ctypes.c_int),
     "prh_tracer_set_variant": ([_I32], ctypes.c_int),
     "prh_tracer_variants": ([_P, _P, _P, _P], ctypes.c_int),
     "prh_tracer_set_grain": ([_I32], ctypes.c_int),
@@ -201,6 +202,17 @@ def pcg64_state(seed: int) -> np.ndarray:
     out = np.zeros(4, dtype=np.uint64)
     check(lib().prh_pcg64_seed(ptr(words), int(words.size), ptr(out)), "prh_pcg64_seed")
     return out
+
+
+def libm_sincos(x: np.ndarray, on_device: bool) -> tuple[np.ndarray, np.ndarray]:
+    """sin(x), cos(x) by the library's restatement of the C library's sin/cos
+    (the routine the ray generation runs), evaluated on the host or the device."""
+
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    check(lib().prh_libm_sincos(ptr(x), int(x.size), int(bool(on_device)), ptr(s), ptr(c)), "prh_libm_sincos")
+    return s, c
 
 
 def last_timings() -> np.ndarray:

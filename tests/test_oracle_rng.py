@@ -48,3 +48,46 @@ def test_advance_then_uniforms_match_numpy(advance):
     g.advance(advance)
     ref = np.random.Generator(g).random(2048)
     assert np.array_equal(O.uniforms(seed, advance, 2048), ref)
+
+
+def _numpy_rays(P, p0, p1, R, seed):
+    """generate_rays (porerad/view_factors.py:147-182) evaluated with NumPy's own
+    routines: Generator.random, np.clip, np.remainder, np.sqrt, np.cos, np.sin and
+    np.linalg.norm, vectorised over the patch range."""
+
+    bitgen = np.random.PCG64(seed)
+    bitgen.advance(p0 * R * 4)
+    u = np.random.Generator(bitgen).random((p1 - p0, R, 4))
+    normal = P.patch_normal[p0:p1]
+    axis = np.argmax(normal != 0.0, axis=1)
+    sign = normal[np.arange(p1 - p0), axis]
+    tang = np.clip(u[:, :, :2], 1e-12, 1.0 - 1e-12) - 0.5
+    origin = np.repeat(P.patch_origin[p0:p1, None, :], R, axis=1)
+    lateral = np.array([[1, 2], [0, 2], [0, 1]])[axis]
+    rows = np.arange(p1 - p0)
+    for t in range(2):
+        origin[rows, :, lateral[:, t]] += tang[:, :, t]
+    origin[:, :, 1] = np.remainder(origin[:, :, 1], P.solid.shape[1])
+    origin[:, :, 2] = np.remainder(origin[:, :, 2], P.solid.shape[2])
+    r = np.sqrt(u[:, :, 2])
+    phi = 2.0 * np.pi * u[:, :, 3]
+    d = np.empty((p1 - p0, R, 3))
+    d[rows, :, axis] = sign[:, None] * np.sqrt(1.0 - u[:, :, 2])
+    d[rows, :, lateral[:, 0]] = r * np.cos(phi)
+    d[rows, :, lateral[:, 1]] = sign[:, None] * (r * np.sin(phi))
+    d = d.reshape(-1, 3)
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    return origin.reshape(-1, 3), d
+
+
+def test_oracle_rays_equal_numpy_bit_for_bit():
+    # The oracle's C cos/sin must be the separate calls np.cos/np.sin make
+    # (a fused sincos differs from them in about 0.1 % of arguments).
+    rng = np.random.default_rng(12)
+    solid = (rng.random((24, 20, 22)) < 0.15).astype(np.uint8)
+    P = O.build_surface_patches(solid, 0.01)
+    for p0, p1, seed in ((0, P.n_patches, 4734788061198622954), (37, 411, 3)):
+        o_ref, d_ref = _numpy_rays(P, p0, p1, 200, seed)
+        o, d = O.generate_rays(P, p0, p1, 200, seed)
+        assert np.array_equal(o.view(np.int64), o_ref.view(np.int64))
+        assert np.array_equal(d.view(np.int64), d_ref.view(np.int64))
