@@ -57,7 +57,7 @@ def log(msg: str) -> None:
 def foam_workload():
     from porerad import synthetic
 
-    return synthetic.voronoi_foam(128, 64, 0.10, 2)
+    return synthetic.foam_n128()
 
 
 def run_tracer(P, R, seed, steps, warmup):
@@ -97,7 +97,7 @@ def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
 
     t0 = time.perf_counter()
     geoms = {"foam128": (foam_P, build_operator(foam_P, R, seed))}
-    fibre = view_factors.build_surface_patches(synthetic.aligned_fibers(128, 0.10, 4.0, 0), 0.01)
+    fibre = view_factors.build_surface_patches(synthetic.aligned_fibre_n128(), 0.01)
     geoms["aligned128"] = (fibre, build_operator(fibre, R, seed))
     setup = time.perf_counter() - t0
     log(f"operators: {', '.join(f'{k}: {P.n_patches} patches, {op.device.nnz_symmetric} nnz' for k, (P, op) in geoms.items())}"
@@ -236,15 +236,7 @@ def batching_workload(sizes=BATCH_SIZES):
 
     from porerad import synthetic
 
-    families = [
-        # foam cells of constant size: 64 Voronoi seeds per 128^3
-        ("foam", lambda N: synthetic.voronoi_foam(N, max(8, int(round(64 * (N / 128.0) ** 3))), 0.10, 2)),
-        ("random_fibre", lambda N: synthetic.random_fibers(N, 0.10, 3.0, 0)),
-        ("aligned_fibre", lambda N: synthetic.aligned_fibers(N, 0.10, 4.0, 0)),
-        ("perforated_plates", lambda N: synthetic.layered_plates(N, 0.10, 0, thickness=2, hole=6,
-                                                                 hole_fraction=0.15)),
-    ]
-    return [(f"{name}{N}", fn(N)) for N in sizes for name, fn in families]
+    return synthetic.mixed_geometries(sizes)
 
 
 def small_batching(R, steps=5):

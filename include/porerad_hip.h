@@ -104,9 +104,19 @@ int prh_tracer_variants(int32_t *n_variants, int32_t *current, int32_t *refill_m
  * counterpart in the reference; results do not depend on it. */
 int prh_tracer_set_grain(int32_t records);
 /* Free the current device's tracer workspace (ray records of the largest
- * prh_trace_counts / prh_trace_rays call so far; it grows on demand and is
+ * prh_trace_counts / prh_trace_rays pass so far; it grows on demand and is
  * otherwise kept).  No counterpart in the reference. */
 int prh_tracer_trim(void);
+/* Bound of the tracer workspace (default 32 GiB, at least 1 MiB).  A call
+ * also keeps the workspace within half of the device memory that is free or
+ * already held by it; larger inputs are traced in several passes (patch
+ * ranges, seed blocks), with the same results.  prh_tracer_workspace reports
+ * the bytes the current device holds and the bound (either pointer may be
+ * NULL), e.g. for an HBM budget that admits other work.  The workspace is
+ * the bounded counterpart of the reference's patch_chunk
+ * (porerad/view_factors.py:273-287), which bounds its host arrays. */
+int prh_tracer_set_workspace_limit(int64_t bytes);
+int prh_tracer_workspace(int64_t *held_bytes, int64_t *limit_bytes);
 int prh_tracer_grain(int32_t *records);
 
 /* numpy.random.PCG64(seed) seeding: SeedSequence over the little-endian
@@ -264,7 +274,11 @@ int prh_trace_rays(const prh_geometry *g, const double *origins, const double *d
 /* Fused ray generation + FP64 DDA + per-patch counting for patches
  * [p0, p1) and n_seeds seeded ray blocks (state_inc: n_seeds x 4 words), all
  * pooled into one device-resident edge table sorted by (origin, receiver).
- * rays_per_patch * n_seeds <= 16384.  *occupancy_reads (may be NULL) receives
+ * Any patch range and rays_per_patch * n_seeds < 2^31: a pass counts at most
+ * 16384 rays per patch, so more seeds are traced in groups (each seed's rays
+ * in parts of 16384 when rays_per_patch is larger) whose tables are pooled on
+ * the device, and the patch range is cut into passes that fit the workspace
+ * bound (prh_tracer_set_workspace_limit).  *occupancy_reads (may be NULL) receives
  * the number of occupancy reads of the traversal (the exact count of
  * solid[i, j, k] loads the reference kernel makes).
  * Replaces first_hit_counts + pool_counts, porerad/view_factors.py:273-300. */

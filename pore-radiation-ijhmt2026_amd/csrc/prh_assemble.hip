@@ -22,6 +22,7 @@
 //   5. row pointers of S from the row boundaries of the reduced keys (every
 //      row holds its diagonal), then columns, FP64 values and K per entry.
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <vector>
 
@@ -344,7 +345,106 @@ int build(prh_operator *op, const prh_edges *e, const int64_t *rays_host, const 
     return PRH_OK;
 }
 
+// ------------------------------------------------------------------ pooling
+// pool_counts (porerad/view_factors.py:290-300) of device edge tables: the
+// (origin, receiver) keys of every table radix-sorted together and equal keys'
+// counts summed (integer sums, so the order among equal keys is immaterial).
+__global__ void k_pool_keys(const int32_t *origin, const int32_t *receiver, const int32_t *count, int64_t nnz,
+                            uint64_t *key, int64_t *val)
+{
+    for (int64_t e = (int64_t)blockIdx.x * AT + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * AT) {
+        key[e] = (uint64_t)(uint32_t)origin[e] << 32 | (uint32_t)receiver[e];
+        val[e] = count[e];
+    }
+}
+
+__global__ void k_pool_split(const uint64_t *key, const int64_t *val, const int64_t *n_unique, int32_t *origin,
+                             int32_t *receiver, int32_t *count, int32_t *overflow)
+{
+    const int64_t n = *n_unique;
+    for (int64_t e = (int64_t)blockIdx.x * AT + threadIdx.x; e < n; e += (int64_t)gridDim.x * AT) {
+        origin[e] = (int32_t)(key[e] >> 32);
+        receiver[e] = (int32_t)(uint32_t)key[e];
+        if (val[e] > INT32_MAX) *overflow = 1;
+        count[e] = (int32_t)val[e];
+    }
+}
+
 }  // namespace
+
+int pool_edge_tables(const prh_edges *const *parts, int n_parts, int64_t n_patches, hipStream_t st, prh_edges **out)
+{
+    *out = nullptr;
+    int64_t total = 0;
+    int dev = 0;
+    PRH_HIP(hipGetDevice(&dev));
+    for (int k = 0; k < n_parts; ++k) total += parts[k]->nnz;
+    Scratch S;
+    int rc;
+    uint64_t *key = nullptr, *key_s = nullptr, *key_u = nullptr;
+    int64_t *val = nullptr, *val_s = nullptr, *val_u = nullptr, *n_unique = nullptr;
+    int32_t *overflow = nullptr;
+    if ((rc = S.alloc(&key, total)) || (rc = S.alloc(&val, total)) || (rc = S.alloc(&key_s, total)) ||
+        (rc = S.alloc(&val_s, total)) || (rc = S.alloc(&key_u, total)) || (rc = S.alloc(&val_u, total)) ||
+        (rc = S.alloc(&n_unique, 1)) || (rc = S.alloc(&overflow, 1)))
+        return rc;
+    int64_t at = 0;
+    for (int k = 0; k < n_parts; ++k) {
+        const prh_edges *e = parts[k];
+        if (e->nnz > 0)
+            k_pool_keys<<<grid_of(e->nnz), AT, 0, st>>>(e->origin, e->receiver, e->count, e->nnz, key + at, val + at);
+        at += e->nnz;
+    }
+    PRH_HIP(hipGetLastError());
+    int bits = 1;
+    while ((int64_t(1) << bits) < n_patches) ++bits;
+    size_t need_sort = 0, need_reduce = 0;
+    PRH_HIP(rocprim::radix_sort_pairs(nullptr, need_sort, key, key_s, val, val_s, (size_t)total, 0u,
+                                      (unsigned)(32 + bits), st));
+    PRH_HIP(rocprim::reduce_by_key(nullptr, need_reduce, key_s, val_s, (size_t)total, key_u, val_u, n_unique,
+                                   rocprim::plus<int64_t>(), rocprim::equal_to<uint64_t>(), st));
+    char *tmp = nullptr;
+    const size_t tmp_bytes = std::max(need_sort, need_reduce);
+    if ((rc = S.alloc(&tmp, (int64_t)tmp_bytes))) return rc;
+    size_t need = tmp_bytes;
+    PRH_HIP(rocprim::radix_sort_pairs((void *)tmp, need, key, key_s, val, val_s, (size_t)total, 0u,
+                                      (unsigned)(32 + bits), st));
+    need = tmp_bytes;
+    PRH_HIP(rocprim::reduce_by_key((void *)tmp, need, key_s, val_s, (size_t)total, key_u, val_u, n_unique,
+                                   rocprim::plus<int64_t>(), rocprim::equal_to<uint64_t>(), st));
+    int64_t nnz = 0;
+    PRH_HIP(hipMemcpyAsync(&nnz, n_unique, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    PRH_HIP(hipMemsetAsync(overflow, 0, sizeof(int32_t), st));
+    PRH_HIP(hipStreamSynchronize(st));
+    prh_edges *res = new prh_edges{dev, nnz, nullptr, nullptr, nullptr};
+    const int64_t alloc = nnz > 0 ? nnz : 1;
+    if (hipMalloc((void **)&res->origin, alloc * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void **)&res->receiver, alloc * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void **)&res->count, alloc * sizeof(int32_t)) != hipSuccess) {
+        prh_edges_destroy(res);
+        (void)hipGetLastError();
+        set_error("pooling the seed blocks' edges: out of device memory (%lld edges)", (long long)nnz);
+        return PRH_ERR_HIP;
+    }
+    int32_t oflow = 0;
+    if (nnz > 0) k_pool_split<<<grid_of(nnz), AT, 0, st>>>(key_u, val_u, n_unique, res->origin, res->receiver,
+                                                           res->count, overflow);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&oflow, overflow, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess || oflow) {
+        prh_edges_destroy(res);
+        if (e != hipSuccess) {
+            set_error("pooling the seed blocks' edges: %s", hipGetErrorString(e));
+            return PRH_ERR_HIP;
+        }
+        set_error("pooled count above 2^31 - 1 for one edge");
+        return PRH_ERR_CAPACITY;
+    }
+    *out = res;
+    return PRH_OK;
+}
+
 }  // namespace prh
 
 using namespace prh;

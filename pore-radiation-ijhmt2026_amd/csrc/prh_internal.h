@@ -120,6 +120,10 @@ int64_t scan_scratch_elems(int64_t n);
 int exclusive_scan_i32_to_i64_carry_async(const int32_t *in, int64_t *out, int64_t n, int64_t *scratch,
                                           int64_t *carry, int64_t *start, hipStream_t stream);
 
+// pool_counts of device edge tables (each sorted, unique pairs) into a new
+// table: keys sorted, equal pairs' counts summed (prh_assemble.hip).
+int pool_edge_tables(const prh_edges *const *parts, int n_parts, int64_t n_patches, hipStream_t st, prh_edges **out);
+
 // Persistent workspace helpers.
 int ensure_capacity(void **buf, size_t *cap, size_t bytes);
 

@@ -403,6 +403,10 @@ struct GenArgs {
     GeoView G;
     TraversalConst C;
     uint32_t p0, n_p, R;
+    // A patch's rays r = 0 .. R-1 of a launch are rays r0 + r of a block of
+    // R_stream rays per patch: stream position 4 (p R_stream + r0 + r).  A
+    // whole block is r0 = 0, R_stream = R.
+    uint32_t R_stream, r0;
     const uint64_t *seeds;  // S x {state_hi, state_lo, inc_hi, inc_lo}
     uint32_t total;         // rays of this launch (or segment)
     uint32_t offset;        // index of its first record in the output arrays
@@ -434,7 +438,9 @@ __device__ __forceinline__ void gen_rays_body(const GenArgs &a, uint64_t base)
     uint32_t s = (uint32_t)base / seg;
     uint32_t q = ((uint32_t)base - s * seg) / a.R;
     uint32_t r = (uint32_t)base - s * seg - q * a.R;
-    uint32_t s_cur = 0xFFFFFFFFu;
+    uint32_t s_cur = 0xFFFFFFFFu, q_cur = 0xFFFFFFFFu;
+    // the stream runs on from one patch's rays to the next only for whole blocks
+    const bool contiguous = a.R == a.R_stream;
     u128 st = make_u128(0, 0), inc = make_u128(0, 0), step_inc = make_u128(0, 0);
     for (uint32_t k = 0; k < GEN_ROUNDS; ++k) {
         const uint64_t round0 = wave_base + 64ull * k;  // first ray of the round (lane 0's)
@@ -457,13 +463,15 @@ __device__ __forceinline__ void gen_rays_body(const GenArgs &a, uint64_t base)
         double o[3], d[3];
         if (valid) {
             const uint32_t p = a.p0 + q;
-            if (s != s_cur) {
-                // first ray of this lane, or the stride crossed into the next seed
+            if (s != s_cur || (!contiguous && q != q_cur)) {
+                // first ray of this lane, or the stride crossed into the next
+                // seed (or, for part of a block, into the next patch)
                 u128 st0;
                 load_seed(a.seeds, (int)s, st0, inc);
-                st = pcg_jump(st0, inc, 4ull * ((uint64_t)p * a.R + r), c_jump);
+                st = pcg_jump(st0, inc, 4ull * ((uint64_t)p * a.R_stream + a.r0 + r), c_jump);
                 step_inc = mul128(c_jump.G[8], inc);
                 s_cur = s;
+                q_cur = q;
             }
             u128 next = add128(mul128(c_jump.A[8], st), step_inc);  // 64 rays = 256 draws on
             const double *po = G.patch_origin + 3 * (size_t)p;
@@ -957,6 +965,50 @@ double g_last_ms[4] = {0, 0, 0, 0};
 
 int grow(Buf &b, size_t bytes) { return ensure_capacity(&b.p, &b.cap, bytes); }
 
+// Device bytes of the tracer's workspace per ray of one pass (start record,
+// wall record, final state, unique receivers and counts) and per patch
+// (unique count and edge offset); prh_trace_rays holds origins, directions,
+// records, final states and receivers per ray.
+constexpr int64_t PASS_BYTES_PER_RAY = sizeof(DdaRec) + sizeof(WallRec) + sizeof(uint2) + 2 * sizeof(int32_t);
+constexpr int64_t PASS_BYTES_PER_PATCH = sizeof(int32_t) + sizeof(int64_t);
+constexpr int64_t GIVEN_BYTES_PER_RAY = 6 * sizeof(double) + sizeof(DdaRec) + sizeof(WallRec) + sizeof(uint2) +
+                                        sizeof(int32_t);
+
+// Upper bound of the workspace (prh_tracer_set_workspace_limit); a call
+// also keeps it within half of the device memory that is free or already
+// held by the workspace.
+int64_t g_ws_limit = int64_t(32) << 30;
+
+int64_t workspace_held(Workspace &W)
+{
+    int64_t held = 0;
+    for (Buf *b : {&W.seeds, &W.origins, &W.directions, &W.dda, &W.walls, &W.fin, &W.receiver, &W.uniq_recv,
+                   &W.uniq_cnt, &W.n_unique, &W.offsets, &W.counters})
+        held += (int64_t)b->cap;
+    return held;
+}
+
+int workspace_free(Workspace &W)
+{
+    for (Buf *b : {&W.seeds, &W.origins, &W.directions, &W.dda, &W.walls, &W.fin, &W.receiver, &W.uniq_recv,
+                   &W.uniq_cnt, &W.n_unique, &W.offsets, &W.counters}) {
+        if (b->p) PRH_HIP(hipFree(b->p));
+        b->p = nullptr;
+        b->cap = 0;
+    }
+    return PRH_OK;
+}
+
+// Workspace bytes a call may use on the current device.
+int workspace_budget(Workspace &W, int64_t *budget)
+{
+    size_t free_b = 0, total_b = 0;
+    PRH_HIP(hipMemGetInfo(&free_b, &total_b));
+    const int64_t avail = ((int64_t)free_b + workspace_held(W)) / 2;
+    *budget = std::min(g_ws_limit, avail);
+    return PRH_OK;
+}
+
 int resident_waves(const void *kernel, int threads, size_t lds, int64_t *waves)
 {
     int dev = 0, cus = 0, per_cu = 0;
@@ -1072,18 +1124,20 @@ int launch_gen_rays(const Geometry &g, uint32_t p0, uint32_t n_p, uint32_t R, co
                     double *d_orig, double *d_dir, hipStream_t st)
 {
     if (total == 0) return PRH_OK;
-    GenArgs a{view_of(g), traversal_const(g), p0, n_p, R, d_seeds, total, 0u, d_orig, d_dir, nullptr, nullptr};
+    GenArgs a{view_of(g), traversal_const(g), p0, n_p, R, R, 0u, d_seeds, total, 0u, d_orig, d_dir, nullptr, nullptr};
     k_gen_rays<false><<<gen_grid(total), TRACE_THREADS, 0, st>>>(a);
     PRH_HIP(hipGetLastError());
     return PRH_OK;
 }
 
 // Traversal records of the rays of patches [p0, p0 + n_p).
-int launch_gen_records(const Geometry &g, uint32_t p0, uint32_t n_p, uint32_t R, const uint64_t *d_seeds,
-                       uint32_t total, DdaRec *d_dda, WallRec *d_walls, hipStream_t st)
+// R rays per patch: rays r0 .. r0 + R - 1 of blocks of R_stream rays per patch.
+int launch_gen_records(const Geometry &g, uint32_t p0, uint32_t n_p, uint32_t R, uint32_t R_stream, uint32_t r0,
+                       const uint64_t *d_seeds, uint32_t total, DdaRec *d_dda, WallRec *d_walls, hipStream_t st)
 {
     if (total == 0) return PRH_OK;
-    GenArgs a{view_of(g), traversal_const(g), p0, n_p, R, d_seeds, total, 0u, nullptr, nullptr, d_dda, d_walls};
+    GenArgs a{view_of(g), traversal_const(g), p0, n_p, R, R_stream, r0, d_seeds, total, 0u,
+              nullptr,    nullptr,          d_dda, d_walls};
     k_gen_rays<true><<<gen_grid(total), TRACE_THREADS, 0, st>>>(a);
     PRH_HIP(hipGetLastError());
     return PRH_OK;
@@ -1103,6 +1157,151 @@ int launch_trace(const Geometry &g, const DdaRec *d_dda, const WallRec *d_walls,
     TraceArgs a{view_of(g), total, (uint32_t)g_trace_grain, d_dda, d_walls, d_fin, d_reads, d_queue};
     V.fn<<<grid, STREAM_THREADS, lds, st>>>(a);
     PRH_HIP(hipGetLastError());
+    return PRH_OK;
+}
+
+// Owner of edge tables made along the way; frees whatever it still holds.
+struct EdgeParts {
+    std::vector<prh_edges *> parts;
+    ~EdgeParts()
+    {
+        for (prh_edges *e : parts) prh_edges_destroy(e);
+    }
+};
+
+// One traced pass: patches [p0, p0 + n_p) with S seeds (d_seeds, 4 words
+// each), R rays per patch (rays r0 .. r0 + R - 1 of each seed's blocks of
+// R_stream rays per patch), S R <= MAX_RAYS_PER_PATCH_CALL and
+// n_p S R < 2^32: generation, traversal, per-patch counting and compaction
+// into a new device edge table.  Adds the occupancy reads and stage times.
+int trace_pass(const Geometry &g, Workspace &W, int64_t p0, int64_t n_p, int R, int R_stream, int r0, int S,
+               const uint64_t *d_seeds, hipStream_t st, prh_edges **out, unsigned long long *reads, double ms[4])
+{
+    const int dev = g.device;
+    const int SR = R * S;
+    const int64_t total = n_p * SR;
+    int SRp2 = 1;
+    while (SRp2 < SR) SRp2 <<= 1;
+    const size_t t1 = total > 0 ? (size_t)total : 1, n1 = n_p > 0 ? (size_t)n_p : 1;
+    int rc;
+    if ((rc = grow(W.dda, t1 * sizeof(DdaRec))) || (rc = grow(W.walls, t1 * sizeof(WallRec))) ||
+        (rc = grow(W.fin, t1 * sizeof(uint2))) || (rc = grow(W.uniq_recv, t1 * sizeof(int32_t))) ||
+        (rc = grow(W.uniq_cnt, t1 * sizeof(int32_t))) || (rc = grow(W.n_unique, n1 * sizeof(int32_t))) ||
+        (rc = grow(W.offsets, n1 * sizeof(int64_t))) || (rc = grow(W.counters, 3 * sizeof(unsigned long long))))
+        return rc;
+    // counters: [0] occupancy reads, [1] failures, [2] traversal queue
+    PRH_HIP(hipMemsetAsync(W.counters.p, 0, 3 * sizeof(unsigned long long), st));
+    *out = nullptr;
+    if (total == 0) {
+        *out = new prh_edges{dev, 0, nullptr, nullptr, nullptr};
+        return PRH_OK;
+    }
+    PRH_HIP(hipEventRecord(W.ev[0], st));
+    unsigned long long *ctr = W.counters.as<unsigned long long>();
+    if ((rc = launch_gen_records(g, (uint32_t)p0, (uint32_t)n_p, (uint32_t)R, (uint32_t)R_stream, (uint32_t)r0,
+                                 d_seeds, (uint32_t)total, W.dda.as<DdaRec>(), W.walls.as<WallRec>(), st)))
+        return rc;
+    PRH_HIP(hipEventRecord(W.ev[1], st));
+    if ((rc = launch_trace(g, W.dda.as<DdaRec>(), W.walls.as<WallRec>(), (uint32_t)total, W.fin.as<uint2>(), ctr,
+                           ctr + 2, st)))
+        return rc;
+    PRH_HIP(hipEventRecord(W.ev[2], st));
+    const unsigned cgrid = (unsigned)(n_p < 65536 ? n_p : 65536);
+    if (SRp2 <= 1024) {
+        // one wavefront per patch, keys in registers
+        const int epl = SRp2 <= 64 ? 1 : SRp2 / 64;
+        const int64_t blocks = (n_p + COUNT_THREADS / 64 - 1) / (COUNT_THREADS / 64);
+        const unsigned wgrid = (unsigned)(blocks < 65536 ? blocks : 65536);
+        auto kern = epl == 1    ? k_count_wave<1>
+                    : epl == 2  ? k_count_wave<2>
+                    : epl == 4  ? k_count_wave<4>
+                    : epl == 8  ? k_count_wave<8>
+                                : k_count_wave<16>;
+        kern<<<wgrid, COUNT_THREADS, 0, st>>>(W.fin.as<uint2>(), view_of(g), n_p, R, S, W.uniq_recv.as<int32_t>(),
+                                              W.uniq_cnt.as<int32_t>(), W.n_unique.as<int32_t>(), ctr + 1);
+    } else {
+        const size_t lds = sizeof(int32_t) * (SRp2 + COUNT_THREADS);
+        k_count_patches<<<cgrid, COUNT_THREADS, lds, st>>>(W.fin.as<uint2>(), view_of(g), n_p, R, S, SRp2,
+                                                           W.uniq_recv.as<int32_t>(), W.uniq_cnt.as<int32_t>(),
+                                                           W.n_unique.as<int32_t>(), ctr + 1);
+    }
+    PRH_HIP(hipGetLastError());
+    PRH_HIP(hipEventRecord(W.ev[3], st));
+    unsigned long long counters[2] = {0, 0};
+    PRH_HIP(hipMemcpyAsync(counters, W.counters.p, sizeof(counters), hipMemcpyDeviceToHost, st));
+    PRH_HIP(hipStreamSynchronize(st));
+    *reads += counters[0];
+    if (counters[1]) {
+        set_error("%llu rays did not resolve a receiving patch", counters[1]);
+        return PRH_ERR_UNRESOLVED;
+    }
+    int64_t nnz = 0;
+    if ((rc = exclusive_scan_i32_to_i64(W.n_unique.as<int32_t>(), W.offsets.as<int64_t>(), n_p, &nnz, st)))
+        return rc;
+    prh_edges *e = new prh_edges{dev, nnz, nullptr, nullptr, nullptr};
+    const int64_t alloc = nnz > 0 ? nnz : 1;
+    if (hipMalloc((void **)&e->origin, alloc * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void **)&e->receiver, alloc * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void **)&e->count, alloc * sizeof(int32_t)) != hipSuccess) {
+        prh_edges_destroy(e);
+        (void)hipGetLastError();
+        set_error("prh_trace_counts: out of device memory for %lld edges", (long long)nnz);
+        return PRH_ERR_HIP;
+    }
+    k_compact_edges<<<cgrid, COUNT_THREADS, 0, st>>>(W.uniq_recv.as<int32_t>(), W.uniq_cnt.as<int32_t>(),
+                                                     W.n_unique.as<int32_t>(), W.offsets.as<int64_t>(), n_p, SR, p0,
+                                                     e->origin, e->receiver, e->count, nnz);
+    hipError_t err = hipGetLastError();
+    if (err == hipSuccess) err = hipEventRecord(W.ev[4], st);
+    if (err == hipSuccess) err = hipStreamSynchronize(st);
+    if (err != hipSuccess) {
+        prh_edges_destroy(e);
+        set_error("prh_trace_counts: %s", hipGetErrorString(err));
+        return PRH_ERR_HIP;
+    }
+    float t = 0.f;
+    for (int k = 0; k < 3; ++k) {
+        PRH_HIP(hipEventElapsedTime(&t, W.ev[k], W.ev[k + 1]));
+        ms[k] += t;
+    }
+    PRH_HIP(hipEventElapsedTime(&t, W.ev[3], W.ev[4]));
+    ms[3] += t;
+    *out = e;
+    return PRH_OK;
+}
+
+// The tables of consecutive patch ranges, in order, as one table.
+int concatenate_edges(const std::vector<prh_edges *> &parts, int dev, hipStream_t st, prh_edges **out)
+{
+    int64_t nnz = 0;
+    for (const prh_edges *e : parts) nnz += e->nnz;
+    prh_edges *res = new prh_edges{dev, nnz, nullptr, nullptr, nullptr};
+    const int64_t alloc = nnz > 0 ? nnz : 1;
+    if (hipMalloc((void **)&res->origin, alloc * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void **)&res->receiver, alloc * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void **)&res->count, alloc * sizeof(int32_t)) != hipSuccess) {
+        prh_edges_destroy(res);
+        (void)hipGetLastError();
+        set_error("prh_trace_counts: out of device memory for %lld edges", (long long)nnz);
+        return PRH_ERR_HIP;
+    }
+    int64_t at = 0;
+    hipError_t err = hipSuccess;
+    for (const prh_edges *e : parts) {
+        if (e->nnz == 0) continue;
+        const size_t b = (size_t)e->nnz * sizeof(int32_t);
+        if (err == hipSuccess) err = hipMemcpyAsync(res->origin + at, e->origin, b, hipMemcpyDeviceToDevice, st);
+        if (err == hipSuccess) err = hipMemcpyAsync(res->receiver + at, e->receiver, b, hipMemcpyDeviceToDevice, st);
+        if (err == hipSuccess) err = hipMemcpyAsync(res->count + at, e->count, b, hipMemcpyDeviceToDevice, st);
+        at += e->nnz;
+    }
+    if (err == hipSuccess) err = hipStreamSynchronize(st);
+    if (err != hipSuccess) {
+        prh_edges_destroy(res);
+        set_error("prh_trace_counts: %s", hipGetErrorString(err));
+        return PRH_ERR_HIP;
+    }
+    *out = res;
     return PRH_OK;
 }
 
@@ -1139,14 +1338,7 @@ int prh_tracer_trim(void)
     int rc = current_stream(&st);
     if (rc) return rc;
     PRH_HIP(hipStreamSynchronize(st));
-    Workspace &W = g_ws[dev];
-    for (Buf *b : {&W.seeds, &W.origins, &W.directions, &W.dda, &W.walls, &W.fin, &W.receiver, &W.uniq_recv,
-                   &W.uniq_cnt, &W.n_unique, &W.offsets, &W.counters}) {
-        if (b->p) PRH_HIP(hipFree(b->p));
-        b->p = nullptr;
-        b->cap = 0;
-    }
-    return PRH_OK;
+    return workspace_free(g_ws[dev]);
 }
 
 int prh_tracer_set_grain(int32_t records)
@@ -1209,7 +1401,7 @@ int prh_trace_rays(const prh_geometry *geo, const double *origins, const double 
                    int32_t *receiver, int64_t *n_failed)
 {
     PRH_CHECK_ARG(geo && receiver && (n_rays == 0 || (origins && directions)), "prh_trace_rays: null pointer");
-    PRH_CHECK_ARG(n_rays >= 0 && n_rays < (int64_t(1) << 32), "prh_trace_rays: ray count out of range");
+    PRH_CHECK_ARG(n_rays >= 0, "prh_trace_rays: negative ray count");
     const Geometry &g = geo->g;
     PRH_CHECK_ARG(g.n_patches > 0 && g.cell_code && g.hot_map && g.cold_map,
                   "prh_trace_rays: the geometry has no patch tables (an occupancy-only handle)");
@@ -1226,34 +1418,46 @@ int prh_trace_rays(const prh_geometry *geo, const double *origins, const double 
     int rc = current_stream(&st);
     if (rc) return rc;
     Workspace &W = g_ws[g.device];
-    if ((rc = grow(W.origins, n_rays * 3 * sizeof(double)))) return rc;
-    if ((rc = grow(W.directions, n_rays * 3 * sizeof(double)))) return rc;
-    if ((rc = grow(W.dda, n_rays * sizeof(DdaRec)))) return rc;
-    if ((rc = grow(W.walls, n_rays * sizeof(WallRec)))) return rc;
-    if ((rc = grow(W.fin, n_rays * sizeof(uint2)))) return rc;
-    if ((rc = grow(W.receiver, n_rays * sizeof(int32_t)))) return rc;
-    if ((rc = grow(W.counters, 3 * sizeof(unsigned long long)))) return rc;
-    PRH_HIP(hipMemcpyAsync(W.origins.p, origins, n_rays * 3 * sizeof(double), hipMemcpyHostToDevice, st));
-    PRH_HIP(hipMemcpyAsync(W.directions.p, directions, n_rays * 3 * sizeof(double), hipMemcpyHostToDevice, st));
-    PRH_HIP(hipMemsetAsync(W.counters.p, 0, 3 * sizeof(unsigned long long), st));
-    const uint32_t total = (uint32_t)n_rays;
-    const unsigned grid = (unsigned)((n_rays + TRACE_THREADS - 1) / TRACE_THREADS);
-    k_setup_rays<<<grid, TRACE_THREADS, 0, st>>>(view_of(g), traversal_const(g), W.origins.as<double>(),
-                                                 W.directions.as<double>(), total, W.dda.as<DdaRec>(),
-                                                 W.walls.as<WallRec>());
-    PRH_HIP(hipGetLastError());
-    unsigned long long *ctr = W.counters.as<unsigned long long>();
-    if ((rc = launch_trace(g, W.dda.as<DdaRec>(), W.walls.as<WallRec>(), total, W.fin.as<uint2>(), ctr, ctr + 2, st)))
+    int64_t budget = 0;
+    if ((rc = workspace_budget(W, &budget))) return rc;
+    if (workspace_held(W) > budget && (rc = workspace_free(W))) return rc;
+    // rays in chunks that fit the workspace budget (and 2^31 records)
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(budget / GIVEN_BYTES_PER_RAY, int64_t(1) << 31));
+    const int64_t m = std::min(chunk, n_rays);
+    if ((rc = grow(W.origins, m * 3 * sizeof(double))) || (rc = grow(W.directions, m * 3 * sizeof(double))) ||
+        (rc = grow(W.dda, m * sizeof(DdaRec))) || (rc = grow(W.walls, m * sizeof(WallRec))) ||
+        (rc = grow(W.fin, m * sizeof(uint2))) || (rc = grow(W.receiver, m * sizeof(int32_t))) ||
+        (rc = grow(W.counters, 3 * sizeof(unsigned long long))))
         return rc;
-    k_resolve<<<grid, TRACE_THREADS, 0, st>>>(W.fin.as<uint2>(), view_of(g), total, W.receiver.as<int32_t>(), ctr + 1);
-    PRH_HIP(hipGetLastError());
-    unsigned long long counters[2] = {0, 0};
-    PRH_HIP(hipMemcpyAsync(receiver, W.receiver.p, n_rays * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    PRH_HIP(hipMemcpyAsync(counters, W.counters.p, sizeof(counters), hipMemcpyDeviceToHost, st));
-    PRH_HIP(hipStreamSynchronize(st));
-    if (n_failed) *n_failed = (int64_t)counters[1];
-    if (counters[1]) {
-        set_error("%llu rays did not resolve a receiving patch", counters[1]);
+    unsigned long long failed_total = 0;
+    for (int64_t r0 = 0; r0 < n_rays; r0 += m) {
+        const int64_t k = std::min(m, n_rays - r0);
+        PRH_HIP(hipMemcpyAsync(W.origins.p, origins + 3 * r0, k * 3 * sizeof(double), hipMemcpyHostToDevice, st));
+        PRH_HIP(hipMemcpyAsync(W.directions.p, directions + 3 * r0, k * 3 * sizeof(double), hipMemcpyHostToDevice,
+                               st));
+        PRH_HIP(hipMemsetAsync(W.counters.p, 0, 3 * sizeof(unsigned long long), st));
+        const uint32_t total = (uint32_t)k;
+        const unsigned grid = (unsigned)((k + TRACE_THREADS - 1) / TRACE_THREADS);
+        k_setup_rays<<<grid, TRACE_THREADS, 0, st>>>(view_of(g), traversal_const(g), W.origins.as<double>(),
+                                                     W.directions.as<double>(), total, W.dda.as<DdaRec>(),
+                                                     W.walls.as<WallRec>());
+        PRH_HIP(hipGetLastError());
+        unsigned long long *ctr = W.counters.as<unsigned long long>();
+        if ((rc = launch_trace(g, W.dda.as<DdaRec>(), W.walls.as<WallRec>(), total, W.fin.as<uint2>(), ctr, ctr + 2,
+                               st)))
+            return rc;
+        k_resolve<<<grid, TRACE_THREADS, 0, st>>>(W.fin.as<uint2>(), view_of(g), total, W.receiver.as<int32_t>(),
+                                                  ctr + 1);
+        PRH_HIP(hipGetLastError());
+        unsigned long long counters[2] = {0, 0};
+        PRH_HIP(hipMemcpyAsync(receiver + r0, W.receiver.p, k * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        PRH_HIP(hipMemcpyAsync(counters, W.counters.p, sizeof(counters), hipMemcpyDeviceToHost, st));
+        PRH_HIP(hipStreamSynchronize(st));
+        failed_total += counters[1];
+    }
+    if (n_failed) *n_failed = (int64_t)failed_total;
+    if (failed_total) {
+        set_error("%llu rays did not resolve a receiving patch", failed_total);
         return PRH_ERR_UNRESOLVED;
     }
     return PRH_OK;
@@ -1266,17 +1470,11 @@ int prh_trace_counts(const prh_geometry *geo, int64_t p0, int64_t p1, int32_t ra
     const Geometry &g = geo->g;
     PRH_CHECK_ARG(0 <= p0 && p0 <= p1 && p1 <= g.n_patches, "prh_trace_counts: patch range out of bounds");
     PRH_CHECK_ARG(rays_per_patch > 0 && n_seeds > 0, "prh_trace_counts: rays_per_patch and n_seeds must be positive");
-    if ((int64_t)rays_per_patch * n_seeds > MAX_RAYS_PER_PATCH_CALL) {
-        set_error("prh_trace_counts: rays_per_patch * n_seeds = %lld exceeds %d",
-                  (long long)rays_per_patch * n_seeds, MAX_RAYS_PER_PATCH_CALL);
-        return PRH_ERR_CAPACITY;
-    }
-    const int SR = rays_per_patch * n_seeds;
-    const int64_t n_p = p1 - p0;
-    const int64_t total = n_p * SR;
-    if (total >= (int64_t(1) << 32)) {
-        set_error("prh_trace_counts: %lld rays exceed the 2^32 rays of one call; split the patch range",
-                  (long long)total);
+    const int64_t R = rays_per_patch, S = n_seeds;
+    if (R * S > INT32_MAX) {
+        // a pooled count must fit the int32 counts of the device edge table
+        set_error("prh_trace_counts: rays_per_patch * n_seeds = %lld exceeds 2^31 - 1 rays per patch",
+                  (long long)(R * S));
         return PRH_ERR_CAPACITY;
     }
     *out = nullptr;
@@ -1291,89 +1489,82 @@ int prh_trace_counts(const prh_geometry *geo, int64_t p0, int64_t p1, int32_t ra
         for (auto &e : W.ev) PRH_HIP(hipEventCreate(&e));
         W.events = true;
     }
-    int SRp2 = 1;
-    while (SRp2 < SR) SRp2 <<= 1;
-    const size_t t1 = total > 0 ? (size_t)total : 1, n1 = n_p > 0 ? (size_t)n_p : 1;
-    if ((rc = grow(W.seeds, sizeof(uint64_t) * 4 * n_seeds))) return rc;
-    if ((rc = grow(W.dda, t1 * sizeof(DdaRec)))) return rc;
-    if ((rc = grow(W.walls, t1 * sizeof(WallRec)))) return rc;
-    if ((rc = grow(W.fin, t1 * sizeof(uint2)))) return rc;
-    if ((rc = grow(W.uniq_recv, t1 * sizeof(int32_t)))) return rc;
-    if ((rc = grow(W.uniq_cnt, t1 * sizeof(int32_t)))) return rc;
-    if ((rc = grow(W.n_unique, n1 * sizeof(int32_t)))) return rc;
-    if ((rc = grow(W.offsets, n1 * sizeof(int64_t)))) return rc;
-    if ((rc = grow(W.counters, 3 * sizeof(unsigned long long)))) return rc;
-    PRH_HIP(hipMemcpyAsync(W.seeds.p, state_inc, sizeof(uint64_t) * 4 * n_seeds, hipMemcpyHostToDevice, st));
-    // counters: [0] occupancy reads, [1] failures, [2] traversal queue
-    PRH_HIP(hipMemsetAsync(W.counters.p, 0, 3 * sizeof(unsigned long long), st));
+    int64_t budget = 0;
+    if ((rc = workspace_budget(W, &budget))) return rc;
+    if (workspace_held(W) > budget && (rc = workspace_free(W))) return rc;
+    if ((rc = grow(W.seeds, sizeof(uint64_t) * 4 * S))) return rc;
+    PRH_HIP(hipMemcpyAsync(W.seeds.p, state_inc, sizeof(uint64_t) * 4 * S, hipMemcpyHostToDevice, st));
 
-    int64_t nnz = 0;
-    int32_t *d_origin = nullptr, *d_receiver = nullptr, *d_count = nullptr;
-    unsigned long long counters[2] = {0, 0};
-    if (total > 0) {
-        PRH_HIP(hipEventRecord(W.ev[0], st));
-        unsigned long long *ctr = W.counters.as<unsigned long long>();
-        if ((rc = launch_gen_records(g, (uint32_t)p0, (uint32_t)n_p, (uint32_t)rays_per_patch,
-                                     W.seeds.as<uint64_t>(), (uint32_t)total, W.dda.as<DdaRec>(),
-                                     W.walls.as<WallRec>(), st)))
-            return rc;
-        PRH_HIP(hipEventRecord(W.ev[1], st));
-        if ((rc = launch_trace(g, W.dda.as<DdaRec>(), W.walls.as<WallRec>(), (uint32_t)total, W.fin.as<uint2>(), ctr,
-                               ctr + 2, st)))
-            return rc;
-        PRH_HIP(hipEventRecord(W.ev[2], st));
-        const unsigned cgrid = (unsigned)(n_p < 65536 ? n_p : 65536);
-        if (SRp2 <= 1024) {
-            // one wavefront per patch, keys in registers
-            const int epl = SRp2 <= 64 ? 1 : SRp2 / 64;
-            const int64_t blocks = (n_p + COUNT_THREADS / 64 - 1) / (COUNT_THREADS / 64);
-            const unsigned wgrid = (unsigned)(blocks < 65536 ? blocks : 65536);
-            auto kern = epl == 1    ? k_count_wave<1>
-                        : epl == 2  ? k_count_wave<2>
-                        : epl == 4  ? k_count_wave<4>
-                        : epl == 8  ? k_count_wave<8>
-                                    : k_count_wave<16>;
-            kern<<<wgrid, COUNT_THREADS, 0, st>>>(W.fin.as<uint2>(), view_of(g), n_p, rays_per_patch, n_seeds,
-                                                  W.uniq_recv.as<int32_t>(), W.uniq_cnt.as<int32_t>(),
-                                                  W.n_unique.as<int32_t>(), ctr + 1);
-        } else {
-            const size_t lds = sizeof(int32_t) * (SRp2 + COUNT_THREADS);
-            k_count_patches<<<cgrid, COUNT_THREADS, lds, st>>>(W.fin.as<uint2>(), view_of(g), n_p, rays_per_patch,
-                                                               n_seeds, SRp2, W.uniq_recv.as<int32_t>(),
-                                                               W.uniq_cnt.as<int32_t>(), W.n_unique.as<int32_t>(),
-                                                               ctr + 1);
-        }
-        PRH_HIP(hipGetLastError());
-        PRH_HIP(hipEventRecord(W.ev[3], st));
-        PRH_HIP(hipMemcpyAsync(counters, W.counters.p, sizeof(counters), hipMemcpyDeviceToHost, st));
-        PRH_HIP(hipStreamSynchronize(st));
-        if (counters[1]) {
-            set_error("%llu rays did not resolve a receiving patch", counters[1]);
-            return PRH_ERR_UNRESOLVED;
-        }
-        if ((rc = exclusive_scan_i32_to_i64(W.n_unique.as<int32_t>(), W.offsets.as<int64_t>(), n_p, &nnz, st)))
-            return rc;
-        const int64_t alloc = nnz > 0 ? nnz : 1;
-        PRH_HIP(hipMalloc((void **)&d_origin, alloc * sizeof(int32_t)));
-        PRH_HIP(hipMalloc((void **)&d_receiver, alloc * sizeof(int32_t)));
-        PRH_HIP(hipMalloc((void **)&d_count, alloc * sizeof(int32_t)));
-        k_compact_edges<<<cgrid, COUNT_THREADS, 0, st>>>(W.uniq_recv.as<int32_t>(), W.uniq_cnt.as<int32_t>(),
-                                                         W.n_unique.as<int32_t>(), W.offsets.as<int64_t>(), n_p, SR,
-                                                         p0, d_origin, d_receiver, d_count, nnz);
-        PRH_HIP(hipGetLastError());
-        PRH_HIP(hipEventRecord(W.ev[4], st));
-        PRH_HIP(hipStreamSynchronize(st));
-        float ms = 0.f;
-        for (int k = 0; k < 3; ++k) {
-            PRH_HIP(hipEventElapsedTime(&ms, W.ev[k], W.ev[k + 1]));
-            g_last_ms[k] = ms;
-        }
-        PRH_HIP(hipEventElapsedTime(&ms, W.ev[3], W.ev[4]));
-        g_last_ms[3] = ms;
+    // Seed blocks whose rays a per-patch count can sort in one pass (at most
+    // MAX_RAYS_PER_PATCH_CALL per patch): all seeds at once, groups of seeds,
+    // or each seed's rays in parts.  Their tables are pooled at the end
+    // (pool_counts, view_factors.py:290-300).
+    struct Group {
+        int s0, S, R, r0;
+    };
+    std::vector<Group> groups;
+    const int64_t CAP = MAX_RAYS_PER_PATCH_CALL;
+    if (R * S <= CAP) {
+        groups.push_back({0, (int)S, (int)R, 0});
+    } else if (R <= CAP) {
+        const int64_t per = CAP / R;
+        for (int64_t s0 = 0; s0 < S; s0 += per) groups.push_back({(int)s0, (int)std::min(per, S - s0), (int)R, 0});
+    } else {
+        for (int64_t s = 0; s < S; ++s)
+            for (int64_t r0 = 0; r0 < R; r0 += CAP) groups.push_back({(int)s, 1, (int)std::min(CAP, R - r0), (int)r0});
     }
-    *out = new prh_edges{dev, nnz, d_origin, d_receiver, d_count};
-    if (nnz_out) *nnz_out = nnz;
-    if (occupancy_reads) *occupancy_reads = (int64_t)counters[0];
+
+    // Each group's patch range in chunks whose workspace fits the budget.
+    double ms[4] = {0, 0, 0, 0};
+    unsigned long long reads = 0;
+    EdgeParts parts;
+    const int64_t n_p = p1 - p0;
+    for (const Group &gr : groups) {
+        const int64_t SR = (int64_t)gr.S * gr.R;
+        const int64_t per_patch = SR * PASS_BYTES_PER_RAY + PASS_BYTES_PER_PATCH;
+        int64_t chunk = std::max<int64_t>(1, budget / per_patch);
+        chunk = std::min<int64_t>(chunk, ((int64_t(1) << 32) - 1) / SR);
+        const uint64_t *d_seeds = W.seeds.as<uint64_t>() + 4 * gr.s0;
+        for (int64_t c0 = 0; c0 < std::max<int64_t>(n_p, 1); c0 += chunk) {
+            const int64_t c1 = std::min(n_p, c0 + chunk);
+            prh_edges *e = nullptr;
+            if ((rc = trace_pass(g, W, p0 + c0, c1 - c0, gr.R, (int)R, gr.r0, gr.S, d_seeds, st, &e, &reads, ms)))
+                return rc;
+            parts.parts.push_back(e);
+        }
+    }
+    prh_edges *result = nullptr;
+    if (parts.parts.size() == 1) {
+        result = parts.parts[0];
+        parts.parts.clear();
+    } else if (groups.size() == 1) {
+        // consecutive patch ranges of one group: the tables follow each other
+        if ((rc = concatenate_edges(parts.parts, dev, st, &result))) return rc;
+    } else {
+        if ((rc = pool_edge_tables(parts.parts.data(), (int)parts.parts.size(), g.n_patches, st, &result))) return rc;
+    }
+    for (int k = 0; k < 4; ++k) g_last_ms[k] = ms[k];
+    *out = result;
+    if (nnz_out) *nnz_out = result->nnz;
+    if (occupancy_reads) *occupancy_reads = (int64_t)reads;
+    return PRH_OK;
+}
+
+int prh_tracer_set_workspace_limit(int64_t bytes)
+{
+    PRH_CHECK_ARG(bytes >= (int64_t(1) << 20), "prh_tracer_set_workspace_limit: %lld bytes is below 1 MiB",
+                  (long long)bytes);
+    g_ws_limit = bytes;
+    return PRH_OK;
+}
+
+int prh_tracer_workspace(int64_t *held_bytes, int64_t *limit_bytes)
+{
+    int dev = 0;
+    PRH_HIP(hipGetDevice(&dev));
+    PRH_CHECK_ARG(dev >= 0 && dev < 64, "prh_tracer_workspace: device %d out of range", dev);
+    if (held_bytes) *held_bytes = workspace_held(g_ws[dev]);
+    if (limit_bytes) *limit_bytes = g_ws_limit;
     return PRH_OK;
 }
 
@@ -1771,7 +1962,8 @@ int prh_trace_batch_create(int32_t n_segments, const prh_geometry *const *geomet
             const int64_t rays = (P.b - P.a) * b->R[P.seg] * b->S[P.seg];
             Segment &sg = segs[j];
             sg.gen = GenArgs{view_of(g), traversal_const(g), (uint32_t)P.a, (uint32_t)(P.b - P.a),
-                             (uint32_t)b->R[P.seg], b->seeds + b->seed_off[P.seg], (uint32_t)rays,
+                             (uint32_t)b->R[P.seg], (uint32_t)b->R[P.seg], 0u, b->seeds + b->seed_off[P.seg],
+                             (uint32_t)rays,
                              (uint32_t)P.ray_off, nullptr, nullptr, B.dda, B.walls};
             sg.gen_wave_begin = (uint32_t)wave_at;
             sg.reads = b->counters + P.seg;

@@ -50,6 +50,8 @@ SIGNATURES = {
     "prh_tracer_set_grain": ([_I32], ctypes.c_int),
     "prh_tracer_grain": ([_P], ctypes.c_int),
     "prh_tracer_trim": ([], ctypes.c_int),
+    "prh_tracer_set_workspace_limit": ([_I64], ctypes.c_int),
+    "prh_tracer_workspace": ([_P, _P], ctypes.c_int),
     "prh_geometry_create": ([_P, _I32, _I32, _I32, _F64, _P], ctypes.c_int),
     "prh_geometry_create_occupancy": ([_P, _I32, _I32, _I32, _P], ctypes.c_int),
     "prh_geometry_sizes": ([_P, _P, _P], ctypes.c_int),
@@ -252,6 +254,20 @@ def trim_tracer() -> None:
     """Free the tracer's grow-only device workspace."""
 
     check(lib().prh_tracer_trim(), "prh_tracer_trim")
+
+
+def set_tracer_workspace_limit(nbytes: int) -> None:
+    """Bound the tracer's device workspace; larger inputs are traced in several passes."""
+
+    check(lib().prh_tracer_set_workspace_limit(int(nbytes)), "prh_tracer_set_workspace_limit")
+
+
+def tracer_workspace() -> tuple[int, int]:
+    """(bytes held on the current device, bound) of the tracer's workspace."""
+
+    held, limit = ctypes.c_int64(), ctypes.c_int64()
+    check(lib().prh_tracer_workspace(ctypes.byref(held), ctypes.byref(limit)), "prh_tracer_workspace")
+    return int(held.value), int(limit.value)
 
 
 class Geometry:

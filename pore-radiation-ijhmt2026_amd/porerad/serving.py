@@ -198,6 +198,14 @@ def device_engine_bytes(geometry: Geometry) -> int:
     return n * SLOTS * (6 * 4 + 5 * 8) + max(2 * SLOTS * blocks, 4 * ((n + 255) // 256)) * 8 + n * 21 + 8192
 
 
+def device_tracer_bytes() -> int:
+    """Device bytes the tracer's workspace holds on the current device (traces run beside the server)."""
+
+    from . import _native
+
+    return _native.tracer_workspace()[0]
+
+
 class RadiosityServer:
     """Scheduler of radiosity requests (see the module docstring)."""
 
@@ -206,8 +214,13 @@ class RadiosityServer:
                  engine_bytes: Callable[[Geometry], int] = device_engine_bytes,
                  clock: Callable[[], float] = time.perf_counter,
                  coupled_runner: Callable[[Geometry, CoupledRequest], tuple] = device_coupled_runner,
-                 coupled_bytes: Callable[[Geometry], int] = device_coupled_bytes) -> None:
+                 coupled_bytes: Callable[[Geometry], int] = device_coupled_bytes,
+                 other_bytes: Callable[[], int] = device_tracer_bytes) -> None:
+        """``other_bytes`` reports device memory held outside the server that counts against the budget
+        (by default the tracer's workspace)."""
+
         self.budget = hbm_budget_bytes
+        self.other_size = other_bytes
         self.engine_factory = engine_factory
         self.engine_size = engine_bytes
         self.clock = clock
@@ -245,9 +258,9 @@ class RadiosityServer:
         return out
 
     def engine_bytes(self) -> int:
-        """Device bytes of the live engines and of the running coupled solve."""
+        """Device bytes of the live engines, of the running coupled solve and held outside the server."""
 
-        total = sum(self.engine_size(self.geometries[k]) for k in self.engines)
+        total = sum(self.engine_size(self.geometries[k]) for k in self.engines) + int(self.other_size())
         job = self._coupled_job
         if job is not None:
             total += self.coupled_size(self.geometries[job[0].key])

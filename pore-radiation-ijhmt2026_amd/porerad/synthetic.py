@@ -181,3 +181,42 @@ def layered_plates(N: int, solid_fraction: float, seed: int, thickness: int = 2,
             plate[np.ix_(jj, kk)] = False
         out[start:start + thickness] = plate[None, :, :]
     return out
+
+
+# ---------------------------------------------------------------- named workloads
+# The geometries of the benchmark and of the full-size tests, x-oriented as
+# generated (view_factors.orient_along_x turns them to another axis).
+
+def fibre_n64() -> np.ndarray:
+    """Random fibres N=64, solid fraction 0.10, radius 3, seed 0 (traced with 200 rays/patch, seed 1)."""
+
+    return random_fibers(64, 0.10, 3.0, 0)
+
+
+def foam_n128() -> np.ndarray:
+    """Voronoi-edge foam N=128 from 64 seeds, solid fraction 0.10, seed 2 (1000 rays/patch, seed 3)."""
+
+    return voronoi_foam(128, 64, 0.10, 2)
+
+
+def aligned_fibre_n128() -> np.ndarray:
+    """Cylinders of radius 4 along z with Poisson-disk centres, N=128, solid fraction 0.10, seed 0."""
+
+    return aligned_fibers(128, 0.10, 4.0, 0)
+
+
+MIXED_SIZES = (96, 128, 160, 192)
+
+
+def mixed_geometries(sizes=MIXED_SIZES) -> list[tuple[str, np.ndarray]]:
+    """Foam, random fibres, aligned fibres and perforated plates (2 cells thick, 6-cell holes over 15 %
+    of each plate) at each N of ``sizes``: 16 uneven geometries by default, traced with seeds 10, 11, ...
+    in this order.  Foam cells keep their size: 64 Voronoi seeds per 128^3."""
+
+    families = [
+        ("foam", lambda N: voronoi_foam(N, max(8, int(round(64 * (N / 128.0) ** 3))), 0.10, 2)),
+        ("random_fibre", lambda N: random_fibers(N, 0.10, 3.0, 0)),
+        ("aligned_fibre", lambda N: aligned_fibers(N, 0.10, 4.0, 0)),
+        ("perforated_plates", lambda N: layered_plates(N, 0.10, 0, thickness=2, hole=6, hole_fraction=0.15)),
+    ]
+    return [(f"{name}{N}", fn(N)) for N in sizes for name, fn in families]

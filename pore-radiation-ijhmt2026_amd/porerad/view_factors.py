@@ -169,10 +169,16 @@ def first_hit_counts(
 ) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
     """Integer counts (origin, receiver, n) of one seeded ray block, sorted by edge (view_factors.py:273-287).
 
-    The whole patch range is traced in fused device launches; ``patch_chunk``
-    is accepted for signature compatibility and does not change the result
-    (the ray stream is partition invariant).
+    The patch range is traced in fused device passes; the library cuts it
+    into as many passes as its bounded workspace needs
+    (``_native.set_tracer_workspace_limit``), which is what ``patch_chunk``
+    does for the reference's host arrays.  ``patch_chunk`` is accepted and
+    must be positive; like the reference's, it never changes the result (the
+    ray stream is partition invariant).
     """
+
+    if int(patch_chunk) <= 0:
+        raise ValueError("patch_chunk must be positive")
 
     return first_hit_counts_pooled(patches, rays_per_patch, [seed])
 

@@ -137,6 +137,41 @@ def test_trace_rejects_occupancy_only_geometry_and_bad_rays(vf):
             vf.trace_first_hit(P, bad_o, bad_d)
 
 
+def _equal_tables(a, b):
+    return all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def test_counts_tiled_under_a_small_workspace_equal_one_pass(vf):
+    from porerad import _native
+
+    phases = _fiber_c1()
+    P, Q = _both(vf, ("synthetic_fiber_N64", "x"), phases)
+    one = vf.first_hit_counts_pooled(P, 200, [1, 2])
+    _, limit = _native.tracer_workspace()
+    try:
+        # 8 MiB holds about 180 patches of 400 rays: the range is traced in ~180 passes
+        _native.set_tracer_workspace_limit(8 << 20)
+        tiled = vf.first_hit_counts_pooled(P, 200, [1, 2])
+        held, _ = _native.tracer_workspace()
+        assert held <= 8 << 20
+    finally:
+        _native.set_tracer_workspace_limit(limit)
+    assert _equal_tables(tiled, one)
+    ref = O.pool_counts([O.first_hit_counts(Q, 200, s) for s in (1, 2)], Q.n_patches)
+    assert _equal_tables(one, ref)
+
+
+@pytest.mark.parametrize("R,seeds", [(6000, [5, 6, 7]), (40000, [9]), (20000, [3, 4])])
+def test_counts_beyond_one_pass_per_patch_match_oracle(vf, R, seeds):
+    # more than 16384 rays per patch: seed groups, or parts of one seed's rays, pooled on the device
+    phases = _small_random(seed=3, shape=(10, 9, 8), density=0.25)
+    P, Q = _both(vf, ("small_random_3", "x"), phases)
+    got = vf.first_hit_counts_pooled(P, R, seeds)
+    ref = O.pool_counts([O.first_hit_counts(Q, R, s) for s in seeds], Q.n_patches)
+    assert _equal_tables(got, ref)
+    assert int(got[2].sum()) == P.n_patches * R * len(seeds)
+
+
 def _hash_params():
     out = []
     for h in cases()["hash_cases"]:

@@ -158,6 +158,7 @@ def test_hbm_budget_evicts_idle_engines_only():
         return made[-1]
 
     srv = serving.RadiosityServer(hbm_budget_bytes=150, engine_factory=factory, engine_bytes=lambda g: 100,
+                                  other_bytes=lambda: 0,
                                   clock=Clock())
     srv.add_geometry("a", P, operator=None)
     srv.add_geometry("b", P, operator=None)
@@ -216,6 +217,7 @@ def test_coupled_request_waits_for_hbm_budget():
     P = _patches()
     log = []
     srv = serving.RadiosityServer(hbm_budget_bytes=150, engine_factory=lambda g: StandInEngine(g, 3, nbytes=100),
+                                  other_bytes=lambda: 0,
                                   engine_bytes=lambda g: 100, clock=Clock(), coupled_runner=_coupled_runner(log),
                                   coupled_bytes=lambda g: 100)
     srv.add_geometry("a", P, operator=None)
@@ -243,3 +245,19 @@ def test_coupled_runner_errors_reach_the_caller():
     with pytest.raises(RuntimeError, match="did not converge"):
         srv.run([serving.CoupledRequest("a", 700.0, 300.0, 0.6)], real_time=False)
     srv.close()
+
+
+def test_memory_held_outside_the_server_counts_against_the_budget():
+    # 2 engines of 100 B would fit 250 B, but 60 B held by the tracer's workspace leave room for one
+    P = _patches()
+    srv = serving.RadiosityServer(hbm_budget_bytes=250, engine_factory=lambda g: StandInEngine(g, 2, nbytes=100),
+                                  engine_bytes=lambda g: 100, other_bytes=lambda: 60, clock=Clock())
+    srv.add_geometry("a", P, operator=None)
+    srv.add_geometry("b", P, operator=None)
+    srv.submit(serving.RadiosityRequest("a", 700.0, 300.0, 0.6))
+    srv.submit(serving.RadiosityRequest("b", 700.0, 300.0, 0.6))
+    srv.step()
+    assert set(srv.engines) == {"a"} and len(srv.queue) == 1
+    while srv.pending():
+        srv.step()
+    assert len(srv.replies) == 2
