@@ -284,6 +284,21 @@ int prh_trace_rays(const prh_geometry *g, const double *origins, const double *d
  * Replaces first_hit_counts + pool_counts, porerad/view_factors.py:273-300. */
 int prh_trace_counts(const prh_geometry *g, int64_t p0, int64_t p1, int32_t rays_per_patch, int32_t n_seeds,
                      const uint64_t *state_inc, prh_edges **out, int64_t *nnz, int64_t *occupancy_reads);
+/* prh_trace_counts without any device allocation: the edges go into
+ * caller-owned device arrays origin, receiver, count (int32, capacity
+ * entries) and *nnz receives their number (PRH_ERR_CAPACITY, nothing beyond
+ * capacity written, when they do not fit: *nnz is then the size needed).
+ * The work runs in a fixed 256 MiB device array of the library (one per
+ * device, held until the process ends) in passes of as many patches as it
+ * holds; rays_per_patch * n_seeds <= 16384 (PRH_ERR_CAPACITY above).  Calls on
+ * one device share that array and are serialised by the library; the array
+ * keeps the last call's contents.  Same edges, bit for bit, as
+ * prh_trace_counts.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops
+ * with a fixed memory plan). */
+int prh_trace_counts_into(const prh_geometry *g, int64_t p0, int64_t p1, int32_t rays_per_patch, int32_t n_seeds,
+                          const uint64_t *state_inc, int32_t *origin, int32_t *receiver, int32_t *count,
+                          int64_t capacity, int64_t *nnz, int64_t *occupancy_reads);
 /* Many first_hit_counts calls of uneven geometries in one graph.  Segment k
  * traces patches [patch_begin[k], patch_end[k]) of geometries[k] with
  * rays_per_patch[k] rays from each of n_seeds[k] seeded blocks, pooled as

@@ -22,6 +22,7 @@
 //                    (cell code or wall maps), LDS sort -> unique counts;
 //   k_compact_edges  prefix-summed edge table (origin, receiver, count).
 #include <climits>
+#include <mutex>
 
 #include "prh_device.h"
 #include "prh_internal.h"
@@ -1305,6 +1306,53 @@ int concatenate_edges(const std::vector<prh_edges *> &parts, int dev, hipStream_
     return PRH_OK;
 }
 
+// ------------------------------------------------------------------ allocation-free counting
+// prh_trace_counts_into works in this fixed device array instead of the
+// grow-only workspace: the ray records of one pass, the per-patch counts and
+// offsets, the scan's tile sums, the seeds and the counters.  Each device
+// holds its own copy until the process ends; its address comes from
+// hipGetSymbolAddress once per device (the array's name is no device address
+// in host code).
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(256) << 20;
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+unsigned char *g_scratch[64] = {};
+std::mutex g_scratch_lock[64];
+
+int scratch_address(int dev, unsigned char **out)
+{
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        PRH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch[dev];
+    return PRH_OK;
+}
+
+// Bump allocation inside the scratch array, 256-byte aligned.
+struct ScratchCarver {
+    unsigned char *base;
+    size_t used = 0;
+    template <typename T>
+    T *take(size_t count)
+    {
+        T *p = reinterpret_cast<T *>(base + used);
+        used += (count * sizeof(T) + 255) & ~size_t(255);
+        return p;
+    }
+};
+
+// Scratch bytes of a pass of n_p patches with SR rays each (aligned pieces).
+size_t noalloc_pass_bytes(int64_t n_p, int64_t SR)
+{
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t rays = (size_t)(n_p * SR);
+    return al(rays * sizeof(DdaRec)) + al(rays * sizeof(WallRec)) + al(rays * sizeof(uint2)) +
+           2 * al(rays * sizeof(int32_t)) + al((size_t)n_p * sizeof(int32_t)) + al((size_t)n_p * sizeof(int64_t)) +
+           al((size_t)scan_scratch_elems(n_p) * sizeof(int64_t));
+}
+
 }  // namespace
 
 }  // namespace prh
@@ -1547,6 +1595,115 @@ int prh_trace_counts(const prh_geometry *geo, int64_t p0, int64_t p1, int32_t ra
     *out = result;
     if (nnz_out) *nnz_out = result->nnz;
     if (occupancy_reads) *occupancy_reads = (int64_t)reads;
+    return PRH_OK;
+}
+
+int prh_trace_counts_into(const prh_geometry *geo, int64_t p0, int64_t p1, int32_t rays_per_patch, int32_t n_seeds,
+                          const uint64_t *state_inc, int32_t *origin, int32_t *receiver, int32_t *count,
+                          int64_t capacity, int64_t *nnz_out, int64_t *occupancy_reads)
+{
+    PRH_CHECK_ARG(geo && state_inc && nnz_out, "prh_trace_counts_into: null pointer");
+    const Geometry &g = geo->g;
+    PRH_CHECK_ARG(0 <= p0 && p0 <= p1 && p1 <= g.n_patches, "prh_trace_counts_into: patch range out of bounds");
+    PRH_CHECK_ARG(rays_per_patch > 0 && n_seeds > 0,
+                  "prh_trace_counts_into: rays_per_patch and n_seeds must be positive");
+    PRH_CHECK_ARG(capacity >= 0 && (capacity == 0 || (origin && receiver && count)),
+                  "prh_trace_counts_into: null output arrays or negative capacity");
+    const int64_t SR = (int64_t)rays_per_patch * n_seeds;
+    if (SR > MAX_RAYS_PER_PATCH_CALL) {
+        set_error("prh_trace_counts_into: rays_per_patch * n_seeds = %lld exceeds %d (one pass per patch)",
+                  (long long)SR, MAX_RAYS_PER_PATCH_CALL);
+        return PRH_ERR_CAPACITY;
+    }
+    // fixed part: seeds and five counters; then at least one patch per pass
+    const size_t fixed = ((4 * (size_t)n_seeds * sizeof(uint64_t) + 255) & ~size_t(255)) + 256;
+    if (fixed + noalloc_pass_bytes(1, SR) > NOALLOC_SCRATCH_BYTES) {
+        set_error("prh_trace_counts_into: one patch of %lld rays needs more than the %zu-byte scratch",
+                  (long long)SR, NOALLOC_SCRATCH_BYTES);
+        return PRH_ERR_CAPACITY;
+    }
+    PRH_HIP(hipSetDevice(g.device));
+    const int dev = g.device;
+    PRH_CHECK_ARG(dev >= 0 && dev < 64, "prh_trace_counts_into: device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_lock[dev]);
+    int rc = ensure_tables();
+    if (rc) return rc;
+    hipStream_t st;
+    if ((rc = current_stream(&st))) return rc;
+    unsigned char *base = nullptr;
+    if ((rc = scratch_address(dev, &base))) return rc;
+
+    ScratchCarver sc{base};
+    uint64_t *d_seeds = sc.take<uint64_t>(4 * (size_t)n_seeds);
+    unsigned long long *ctr = sc.take<unsigned long long>(5);  // reads, failures, queue, carry, start
+    int64_t *carry = reinterpret_cast<int64_t *>(ctr + 3), *start = reinterpret_cast<int64_t *>(ctr + 4);
+    const size_t room = NOALLOC_SCRATCH_BYTES - sc.used;
+    int64_t chunk = 1;
+    while (chunk < p1 - p0 && noalloc_pass_bytes(2 * chunk, SR) <= room) chunk *= 2;
+    if (noalloc_pass_bytes(chunk, SR) > room) chunk /= 2;
+    chunk = std::max<int64_t>(chunk, 1);
+    const size_t rays = (size_t)(chunk * SR);
+    DdaRec *dda = sc.take<DdaRec>(rays);
+    WallRec *walls = sc.take<WallRec>(rays);
+    uint2 *fin = sc.take<uint2>(rays);
+    int32_t *uniq_recv = sc.take<int32_t>(rays), *uniq_cnt = sc.take<int32_t>(rays);
+    int32_t *n_unique = sc.take<int32_t>((size_t)chunk);
+    int64_t *offsets = sc.take<int64_t>((size_t)chunk);
+    int64_t *scan_tiles = sc.take<int64_t>((size_t)scan_scratch_elems(chunk));
+
+    // the scratch keeps the last call's contents: every counter starts at zero
+    PRH_HIP(hipMemcpyAsync(d_seeds, state_inc, 4 * sizeof(uint64_t) * n_seeds, hipMemcpyHostToDevice, st));
+    k_zero_u64<<<1, 64, 0, st>>>(ctr, 5);
+    PRH_HIP(hipGetLastError());
+    const int R = rays_per_patch, S = n_seeds;
+    int SRp2 = 1;
+    while (SRp2 < SR) SRp2 <<= 1;
+    for (int64_t c0 = p0; c0 < p1; c0 += chunk) {
+        const int64_t n_p = std::min(chunk, p1 - c0);
+        const uint32_t total = (uint32_t)(n_p * SR);
+        k_zero_u64<<<1, 64, 0, st>>>(ctr + 2, 1);  // the traversal's queue starts at zero
+        if ((rc = launch_gen_records(g, (uint32_t)c0, (uint32_t)n_p, (uint32_t)R, (uint32_t)R, 0u, d_seeds, total,
+                                     dda, walls, st)) ||
+            (rc = launch_trace(g, dda, walls, total, fin, ctr, ctr + 2, st)))
+            return rc;
+        const int64_t blocks = (n_p + COUNT_THREADS / 64 - 1) / (COUNT_THREADS / 64);
+        if (SRp2 <= 1024) {
+            const int epl = SRp2 <= 64 ? 1 : SRp2 / 64;
+            auto kern = epl == 1    ? k_count_wave<1>
+                        : epl == 2  ? k_count_wave<2>
+                        : epl == 4  ? k_count_wave<4>
+                        : epl == 8  ? k_count_wave<8>
+                                    : k_count_wave<16>;
+            kern<<<(unsigned)std::min<int64_t>(blocks, 65536), COUNT_THREADS, 0, st>>>(
+                fin, view_of(g), n_p, R, S, uniq_recv, uniq_cnt, n_unique, ctr + 1);
+        } else {
+            k_count_patches<<<(unsigned)std::min<int64_t>(n_p, 65536), COUNT_THREADS,
+                              sizeof(int32_t) * (SRp2 + COUNT_THREADS), st>>>(fin, view_of(g), n_p, R, S, SRp2,
+                                                                               uniq_recv, uniq_cnt, n_unique, ctr + 1);
+        }
+        PRH_HIP(hipGetLastError());
+        // edge offsets continue the running total kept on the device
+        if ((rc = exclusive_scan_i32_to_i64_carry_async(n_unique, offsets, n_p, scan_tiles, carry, start, st)))
+            return rc;
+        k_compact_edges<<<(unsigned)std::min<int64_t>(n_p, 65536), COUNT_THREADS, 0, st>>>(
+            uniq_recv, uniq_cnt, n_unique, offsets, n_p, (int)SR, c0, origin, receiver, count, capacity);
+        PRH_HIP(hipGetLastError());
+    }
+    unsigned long long host[4] = {0, 0, 0, 0};
+    PRH_HIP(hipMemcpyAsync(host, ctr, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    PRH_HIP(hipStreamSynchronize(st));
+    const int64_t nnz = (int64_t)host[3];
+    *nnz_out = nnz;
+    if (occupancy_reads) *occupancy_reads = (int64_t)host[0];
+    if (host[1]) {
+        set_error("%llu rays did not resolve a receiving patch", host[1]);
+        return PRH_ERR_UNRESOLVED;
+    }
+    if (nnz > capacity) {
+        set_error("prh_trace_counts_into: %lld edges exceed the output capacity %lld", (long long)nnz,
+                  (long long)capacity);
+        return PRH_ERR_CAPACITY;
+    }
     return PRH_OK;
 }
 

@@ -74,6 +74,7 @@ SIGNATURES = {
     "prh_generate_rays": ([_P, _I64, _I64, _I32, _P, _P, _P], ctypes.c_int),
     "prh_trace_rays": ([_P, _P, _P, _I64, _P, _P], ctypes.c_int),
     "prh_trace_counts": ([_P, _I64, _I64, _I32, _I32, _P, _P, _P, _P], ctypes.c_int),
+    "prh_trace_counts_into": ([_P, _I64, _I64, _I32, _I32, _P, _P, _P, _P, _I64, _P, _P], ctypes.c_int),
     "prh_edges_copy": ([_P, _P, _P, _P], ctypes.c_int),
     "prh_trace_batch_create": ([_I32, _P, _P, _P, _P, _P, _I64, _I64, _I32, _I32, _P], ctypes.c_int),
     "prh_trace_batch_run": ([_P, _P, _P, _P, _P], ctypes.c_int),
