@@ -7,10 +7,21 @@ x, 1000 cosine-weighted rays per patch (seed 3).
 A step is one fused first-hit pass over every patch of the foam: seeded ray
 generation -> FP64 DDA -> per-patch integer counts -> sorted edge table in
 HBM (prh_trace_counts).  ``value`` is first-hit rays per second of the whole
-step with the geometry already resident.  Alongside it the line reports the
-served radiosity solves (FP64 reflection series to 1e-8 per request on the
-device-resident exchange operator of the same foam), the roofline of the
-trace kernel and the CPU oracle timed on this host.
+step with the geometry already resident; the step's count_sha256 is checked
+against the CPU oracle's (``parity``).  Alongside it the line reports:
+
+  radiosity    plain FP64 series and served mixed-precision solves on the
+               foam and the aligned fibres N=128 (Poisson 200/s), saturation,
+               error per flux, tick/series rooflines, coupled requests, and
+               the C oracle's FP64 series on the foam operator (solves/s)
+  batching     16 uneven geometries N=96..192 in one replayed graph pass
+               (rays/s) against 16 separate calls, then a Poisson stream
+               (200/s, seed 7) of radiosity and Newton-Krylov requests over
+               their 16 resident operators (solves/s, p50/p99, coupled
+               latencies)
+  monte_carlo, microstructure   the rows ranked next to the path
+  cpu_baseline the C oracle's first_hit_counts on this host's threads and
+               on one core (CPU model and counts stated)
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -77,6 +88,31 @@ def run_tracer(P, R, seed, steps, warmup):
     return np.array(wall), np.array(parts), reads, nnz
 
 
+def count_parity(P, R, seed):
+    """count_sha256 of one traced step against the CPU oracle's for the same workload
+    (tests/golden/configs.json, written by the oracle on the CPU); raises on a mismatch."""
+
+    from porerad import view_factors
+
+    path = os.path.join(REPO, "tests", "golden", "configs.json")
+    with open(path) as f:
+        fx = json.load(f)["foam_n128"]["x"]
+    if (R, seed, P.n_patches) != (fx["rays_per_patch"], fx["seed"], fx["n_patches"]):
+        return {"checked": False, "reason": "no oracle fixture for this ray count"}
+    e = view_factors.first_hit_edges(P, R, [seed])
+    try:
+        o, r, c = e.to_host()
+        reads = e.occupancy_reads
+    finally:
+        e.close()
+    digest = view_factors.count_sha256(o, r, c, np.full(P.n_patches, R, np.int64))
+    ok = digest == fx["count_sha256"] and o.size == fx["nnz"] and reads == fx["occupancy_reads"]
+    if not ok:
+        raise RuntimeError(f"first-hit counts differ from the CPU oracle's: {digest} vs {fx['count_sha256']}")
+    return {"checked": True, "count_sha256": digest, "edges": int(o.size), "occupancy_reads": int(reads),
+            "equal_to_cpu_oracle": True}
+
+
 def build_operator(P, R, seed):
     """Counts traced and the operator assembled on the device (no edge leaves HBM)."""
 
@@ -89,7 +125,7 @@ def build_operator(P, R, seed):
         edges.close()
 
 
-def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
+def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0, with_cpu=True):
     """Plain FP64 series vs served mixed-precision solves (Poisson arrivals)."""
 
     from porerad import coupled, exchange, serving, synthetic, view_factors
@@ -119,6 +155,10 @@ def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
     plain = {"solves_per_s": len(plain_lat) / sum(plain_lat), "p50_ms": float(np.percentile(plain_lat, 50) * 1e3),
              "p99_ms": float(np.percentile(plain_lat, 99) * 1e3), "precision": "fp64 reflection series, rtol 1e-8",
              "geometry": "foam128"}
+    cpu_rad = cpu_radiosity_baseline(P, op) if with_cpu else None
+    if cpu_rad is not None:
+        log(f"radiosity cpu baseline {cpu_rad['value']:.3f} solves/s ({cpu_rad['reflections']} reflections, "
+            f"{cpu_rad['cores']} threads)")
 
     # FP64 reflection series at 1e-13 for the accuracy check, both geometries
     for key, (Pk, opk) in geoms.items():
@@ -221,6 +261,7 @@ def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0):
         "precision": precision,
         "serving_with_coupled": coupled_block,
         "operator_setup_s": round(setup, 2),
+        "cpu_baseline": cpu_rad,
     }
 
 
@@ -274,10 +315,11 @@ def small_batching(R, steps=5):
             "rays_per_s_batched": n_rays / batched}
 
 
-def run_batching(args) -> dict:
+def run_batching(args, with_stream=True) -> dict:
     """One graph-captured trace pass over 16 uneven geometries (seeds 10..25, R rays per patch), replayed
-    K times, against the same rays traced geometry by geometry; then 64 radiosity right-hand sides per
-    geometry through the served engines."""
+    K times, against the same rays traced geometry by geometry; then the operators of all 16, assembled
+    and balanced on the device from the pass's edge tables, serve a Poisson stream of radiosity and
+    Newton-Krylov requests (run_mixed_stream)."""
 
     from porerad import _native, autotune, exchange, serving, view_factors
 
@@ -315,10 +357,11 @@ def run_batching(args) -> dict:
     batch = _native.TraceBatch(segments, ray_capacity=capacity, streams=n_streams)
     create_s = time.perf_counter() - t
     log(f"trace batch: {batch.info} (created and captured in {create_s:.2f} s)")
-    for _ in range(args.warmup):
+    steps = max(1, min(args.steps, 3))
+    for _ in range(max(1, min(args.warmup, 1))):
         batch.run(seeds)
     wall, dev = [], []
-    for _ in range(args.steps):
+    for _ in range(steps):
         t = time.perf_counter()
         batch.run(seeds)
         wall.append(time.perf_counter() - t)
@@ -332,25 +375,20 @@ def run_batching(args) -> dict:
     algo_bytes = 4.0 * n_rays + float(reads)
     graph_s = float(np.mean(dev))
     record = {
-        "metric": "first-hit rays/s over 16 uneven geometries in one graph replay",
-        "value": n_rays / step,
-        "unit": "rays/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": step * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "foam, random fibre, aligned fibre, perforated plates x N in "
-                               f"{list(BATCH_SIZES)}, seeds 10-25, {R} rays/patch",
-                   "geometries": {name: int(P.n_patches) for name, P in geos}, "rays_per_step": int(n_rays),
-                   "edges": int(batch.nnz.sum())},
-        "batching": {"graph_ms": graph_s * 1e3, "per_geometry_calls_ms": sequential_s * 1e3,
-                     "speedup_vs_per_geometry_calls": sequential_s / step, "create_and_capture_s": create_s,
-                     "layout": batch.info},
+        "workload": "foam, random fibre, aligned fibre, perforated plates x N in "
+                    f"{list(BATCH_SIZES)}, seeds 10-25, {R} rays/patch, one replayed graph pass",
+        "rays_per_s": n_rays / step,
+        "replays": steps,
+        "ms_per_replay": step * 1e3,
+        "graph_ms": graph_s * 1e3,
+        "per_geometry_calls_ms": sequential_s * 1e3,
+        "speedup_vs_per_geometry_calls": sequential_s / step,
+        "create_and_capture_s": create_s,
+        "geometries": {name: int(P.n_patches) for name, P in geos},
+        "rays_per_replay": int(n_rays),
+        "edges": int(batch.nnz.sum()),
+        "edge_counts_equal_per_geometry_calls": True,
+        "layout": batch.info,
         "roofline": {"bound": "hbm", "achieved": algo_bytes / graph_s / 1e9, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": algo_bytes / graph_s / 1e9 / HBM_PEAK_GBPS, "traffic": None,
                      "kernel": "whole graph (generation + traversal + counting + compaction)",
@@ -358,40 +396,96 @@ def run_batching(args) -> dict:
     }
     if tune is not None:
         record["tuning"] = tune
-    record["batching"]["small_geometries"] = small_batching(R)
-    if args.no_radiosity:
+    record["small_geometries"] = small_batching(R)
+    if not with_stream:
         batch.close()
         return record
 
-    # 64 right-hand sides per geometry: (T_hot, emissivity) pairs through the served engines.  Each
-    # segment's edges move to a device table of their own (the batch's record buffers are then
-    # freed) and each operator is assembled on the device from them.
+    # Each segment's edges move to a device table of their own (the batch's record buffers are then
+    # freed) and each operator is assembled and balanced on the device from them.
     t = time.perf_counter()
     tables = [batch.edge_table(k) for k in range(len(geos))]
     batch.close()
-    srv = serving.RadiosityServer()
+    operators = {}
     for k, (name, P) in enumerate(geos):
         try:
-            srv.add_geometry(name, P, exchange.ExchangeOperator.from_edges(
-                tables[k], np.full(P.n_patches, R, np.int64), P.patch_area_m2))
+            operators[name] = exchange.ExchangeOperator.from_edges(tables[k], np.full(P.n_patches, R, np.int64),
+                                                                    P.patch_area_m2)
         finally:
             tables[k].close()
-        log(f"operator {name}: {time.perf_counter() - t:.1f} s")
     setup = time.perf_counter() - t
-    rng = np.random.default_rng(12)
-    reqs = [serving.RadiosityRequest(name, float(rng.uniform(400.0, 1000.0)), 300.0, float(rng.uniform(0.2, 0.95)),
-                                     1e-9, 0.0) for name, _ in geos for _ in range(64)]
-    srv.run(reqs[:: len(reqs) // 32], real_time=False)  # engines allocated
-    t = time.perf_counter()
-    replies = srv.run(reqs, real_time=False)
-    rhs = serving.latency_summary(replies, time.perf_counter() - t)
-    rhs.update({"right_hand_sides": len(reqs), "per_geometry": 64, "operator_setup_s": round(setup, 1),
-                "precision": "fp32 chebyshev sweeps + fp64 refinement, rtol 1e-9", "sweeps": srv.sweeps})
-    srv.close()
-    log(f"64 RHS x {len(geos)} geometries: {rhs['solves_per_s']:.1f} solves/s, p50 {rhs['p50_ms']:.1f} ms, "
-        f"p99 {rhs['p99_ms']:.1f} ms (operators {setup:.1f} s)")
-    record["right_hand_sides"] = rhs
+    log(f"16 operators assembled and balanced on the device in {setup:.1f} s")
+    record["stream"] = run_mixed_stream(dict(geos), operators)
+    record["stream"]["operator_setup_s"] = round(setup, 1)
+    for op in operators.values():
+        op.device.close()
     return record
+
+
+# Newton-Krylov requests of the served stream over the 16 geometries: (geometry, T_hot, emissivity, arrival
+# index among the radiosity requests).
+STREAM_COUPLED = (("foam96", 500.0, 0.3, 20), ("random_fibre96", 700.0, 0.6, 120),
+                  ("aligned_fibre128", 900.0, 0.9, 220), ("foam128", 700.0, 0.6, 320))
+
+
+def run_mixed_stream(patches, operators, n_requests=600, rate=200.0, seed=7):
+    """A Poisson stream (rate 200/s, seed 7) of radiosity requests over the 16 resident operators
+    (T_hot 500/700/900 K, eps 0.3/0.6/0.9, solver tolerance 1e-8) with Newton-Krylov coupled requests
+    arriving among them (k_solid 1.0, k_gas 0.026 W/m/K, T_cold 300 K): served solves/s, p50/p99
+    latency, and the coupled latencies, alone and in the stream."""
+
+    from porerad import serving
+    from porerad.constants import K_GAS
+
+    def server():
+        srv = serving.RadiosityServer()
+        for name, op in operators.items():
+            srv.add_geometry(name, patches[name], op)
+        return srv
+
+    rng = np.random.default_rng(seed)
+    names = list(operators)
+    arrivals = serving.poisson_arrivals(rate, n_requests, seed)
+    reqs = [serving.RadiosityRequest(str(rng.choice(names)), float(rng.choice([500.0, 700.0, 900.0])), 300.0,
+                                     float(rng.choice([0.3, 0.6, 0.9])), 1e-8, float(t), tag=i)
+            for i, t in enumerate(arrivals)]
+    creqs = [serving.CoupledRequest(g, Th, 300.0, eps, C5_K_SOLID, K_GAS, float(arrivals[at]), tag=f"c{j}",
+                                    precision="mixed", preconditioner_steps=C5_PCG_STEPS)
+             for j, (g, Th, eps, at) in enumerate(STREAM_COUPLED)]
+    srv = server()
+    srv.run([dataclasses.replace(r, arrival_s=0.0) for r in reqs[:: n_requests // 32]], real_time=False)  # warm
+    t = time.perf_counter()
+    replies = srv.run(reqs + creqs, real_time=True)
+    wall = time.perf_counter() - t
+    srv.close()
+    rad = [r for r in replies if isinstance(r, serving.RadiosityReply)]
+    cpl = [r for r in replies if isinstance(r, serving.CoupledReply)]
+    rad_wall = max(r.finished_s for r in rad) - t
+    out = {"workload": f"{n_requests} radiosity requests (rtol 1e-8) at {rate:.0f}/s Poisson (seed {seed}) over the "
+                       f"16 geometries + {len(creqs)} Newton-Krylov requests (k_solid {C5_K_SOLID}, k_gas {K_GAS}, "
+                       f"Jacobi-PCG {C5_PCG_STEPS[0]} steps, mixed-precision Jacobian solves)",
+           "radiosity": serving.latency_summary(rad, rad_wall),
+           "coupled": {"requests": [f"{g} T_hot {Th:.0f} K eps {eps}" for g, Th, eps, _ in STREAM_COUPLED],
+                       "latency_s": {r.request.tag: round(r.latency_s, 3) for r in cpl},
+                       "newton_updates": {r.request.tag: r.newton_updates for r in cpl},
+                       "converged": all(r.converged for r in cpl)},
+           "wall_s": wall}
+    # each coupled request alone: latency, and the same result as in the stream
+    alone = {}
+    same = True
+    by_tag = {r.request.tag: r for r in cpl}
+    for r in creqs:
+        srv = server()
+        rep, = srv.run([dataclasses.replace(r, arrival_s=0.0)], real_time=False)
+        srv.close()
+        alone[r.tag] = round(rep.latency_s, 3)
+        same &= rep.metrics == by_tag[r.tag].metrics
+    out["coupled"]["alone_latency_s"] = alone
+    out["coupled"]["alone_equals_in_stream"] = bool(same)
+    log(f"stream over 16 geometries: radiosity {out['radiosity']['solves_per_s']:.1f} solves/s, p50 "
+        f"{out['radiosity']['p50_ms']:.1f} ms, p99 {out['radiosity']['p99_ms']:.1f} ms; coupled latencies "
+        f"{out['coupled']['latency_s']} s (alone {alone} s)")
+    return out
 
 
 def engine_roofline(before, after) -> dict:
@@ -432,31 +526,108 @@ def series_roofline(before, after) -> dict:
             "ms_per_reflection": ms / refl if refl else None}
 
 
-def cpu_baseline(phases, R, seed, budget_s=12.0):
+def host_cpu() -> dict:
+    """CPU model and core counts of this host."""
+
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "omp_threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
+
+
+def _omp_threads(n: int) -> None:
+    import ctypes
+
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+
+
+def _median_of(fn, runs=5):
+    fn()  # warm-up
+    times = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t)
+    return float(np.median(times)), times
+
+
+def cpu_baseline(phases, R, seed, target_s=2.0):
+    """The C oracle (first_hit_counts of a fixed patch range: generation, FP64 DDA, sorted
+    counts) on all host threads and on one: one warm-up, then the median of 5 runs."""
+
     from oracle import oracle as O
 
     Q = O.build_surface_patches(phases, 0.01)
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    n_p = 200
-    start = 0
-    done_rays, spent = 0, 0.0
-    while spent < budget_s:
-        n_p = min(n_p, Q.n_patches - start)
+    cpu = host_cpu()
+    threads = cpu["omp_threads"]
+
+    def sample(seconds):
         t = time.perf_counter()
-        O.first_hit_counts_range(Q, R, seed, start, start + n_p)
-        dt = time.perf_counter() - t
-        done_rays += n_p * R
-        spent += dt
-        start = (start + n_p) % Q.n_patches
-        n_p = int(min(max(n_p, n_p * 0.5 * (budget_s - spent) / max(dt, 1e-3)), 50000))
-    return {
-        "value": float(done_rays / spent),
-        "unit": "rays/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"C oracle (OpenMP, {threads} threads) on {done_rays // R} foam patches x {R} rays "
-                  f"({spent:.1f} s of CPU work)",
-    }
+        O.first_hit_counts_range(Q, R, seed, 0, 1000)
+        rate = 1000 / max(time.perf_counter() - t, 1e-6)  # patches per second
+        return int(min(Q.n_patches, max(200, rate * seconds)))
+
+    out = {"unit": "rays/s", "kind": "port", "cpu": cpu}
+    _omp_threads(threads)
+    n_all = sample(target_s)
+    med, runs = _median_of(lambda: O.first_hit_counts_range(Q, R, seed, 0, n_all))
+    out.update({"value": n_all * R / med, "cores": threads, "median_s": med, "runs_s": runs,
+                "sample": f"C oracle first_hit_counts of foam patches [0, {n_all}) x {R} rays (seed {seed}), "
+                          f"{threads} OpenMP threads, 1 warm-up + median of 5"})
+    _omp_threads(1)
+    try:
+        n_one = sample(target_s)
+        med1, runs1 = _median_of(lambda: O.first_hit_counts_range(Q, R, seed, 0, n_one))
+    finally:
+        _omp_threads(threads)
+    out["one_core"] = {"value": n_one * R / med1, "median_s": med1, "runs_s": runs1,
+                       "sample": f"patches [0, {n_one}) x {R} rays, 1 thread, 1 warm-up + median of 5"}
+    return out
+
+
+def cpu_radiosity_baseline(P, op, T_hot=700.0, eps=0.6, rtol=1e-8):
+    """FP64 reflection series of the C oracle (exchange.py:217-245 on the operator's own symmetric CSR,
+    row-parallel over all host threads) for one prescribed-field request: solves/s from one warm-up
+    and the median of 3 solves."""
+
+    from oracle import oracle as O
+    from porerad import coupled, exchange
+
+    rowptr, col, val = op.device.csr()
+    d = np.asarray(op.scale, np.float64)
+    rows = np.repeat(np.arange(P.n_patches, dtype=np.int64), np.diff(rowptr))
+    G = val * d[rows] * d[col]  # diag(d) S diag(d): the map of rmatvec, x -> G (x / A)
+    del rows, val
+    T_m = 0.5 * (T_hot + 300.0)
+    T = coupled.patch_temperature(P, coupled.linear_cell_temperature(P.solid.shape, T_m, T_hot - 300.0),
+                                  T_hot, 300.0)
+    e = np.full(P.n_patches, eps)
+    E = exchange.emission(e, P.patch_area_m2, T, T_m)
+    n = P.n_patches
+    absorbed, flight, work = np.empty(n), np.empty(n), np.empty(n)
+    residual = np.zeros(1)
+    area = np.ascontiguousarray(P.patch_area_m2, np.float64)
+    state = {}
+
+    def solve():
+        state["refl"] = O.lib().orc_reflection_series_csr(
+            O._p(rowptr), O._p(col), O._p(G), n, O._p(area), O._p(E), O._p(e), rtol, 10000, O._p(absorbed),
+            O._p(flight), O._p(work), O._p(residual))
+
+    med, runs = _median_of(solve, runs=3)
+    threads = host_cpu()["omp_threads"]
+    return {"value": 1.0 / med, "unit": "solves/s", "cores": threads, "kind": "port", "median_s": med,
+            "runs_s": runs, "reflections": int(state["refl"]),
+            "sample": f"C oracle FP64 reflection series (rtol {rtol:g}) on the foam operator's symmetric CSR "
+                      f"({op.device.nnz_symmetric} nonzeros), T_hot {T_hot:.0f} K, eps {eps}, {threads} OpenMP "
+                      "threads, 1 warm-up + median of 3"}
 
 
 def run_monte_carlo(phases, with_cpu: bool) -> dict:
@@ -609,6 +780,9 @@ def main() -> None:
     log(f"trace step {step * 1e3:.2f} ms (generation {gen_s * 1e3:.2f}, traversal {trace_s * 1e3:.2f}, "
         f"counting {count_s * 1e3:.2f}, compaction {compact_s * 1e3:.2f} ms), {nnz} edges, "
         f"{reads / n_rays:.1f} occupancy reads/ray")
+    parity = count_parity(P, R, seed)
+    log(f"counts equal to the CPU oracle's (count_sha256 {parity.get('count_sha256', '-')[:16]}...)"
+        if parity["checked"] else "count parity not checked: " + parity["reason"])
     kernel = trace_s
     algo_bytes = 4.0 * n_rays + float(reads)
     achieved = algo_bytes / kernel / 1e9
@@ -628,6 +802,7 @@ def main() -> None:
         "config": {"workload": "voronoi foam N=128^3, 64 seeds, phi=0.10 (seed 2), axis x, "
                                f"{R} rays/patch (seed 3)",
                    "n_patches": int(P.n_patches), "rays_per_step": int(n_rays), "edges": int(nnz)},
+        "parity": parity,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -635,8 +810,10 @@ def main() -> None:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             # FETCH_SIZE x 2 + WRITE_SIZE x 1 (copy-calibrated) of this kernel at
-            # the default workload; null for other ray counts
+            # the default workload, from the rocprofv3 --pmc passes summarised in
+            # traffic_source; null for other ray counts
             "traffic": measured_traffic() if R == 1000 else None,
+            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, REPO) + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
             "kernel": "k_trace_stream",
             "kernel_ms": kernel * 1e3,
             "algorithmic_bytes_per_launch": algo_bytes,
@@ -649,7 +826,7 @@ def main() -> None:
     if tune is not None:
         record["tracer_variant"] = {"chosen": tune["variant"], "table": tune["table"]}
     if not args.no_radiosity:
-        rad = run_radiosity(P, R, seed)
+        rad = run_radiosity(P, R, seed, with_cpu=not args.no_cpu)
         record["radiosity"] = rad
         log(f"radiosity plain fp64 {rad['plain_fp64']['solves_per_s']:.1f} solves/s; served "
             f"{rad['served']['solves_per_s']:.1f} solves/s at {rad['served']['arrival_rate_per_s']:.0f}/s offered, "
@@ -682,9 +859,14 @@ def main() -> None:
             + (" (CPU oracle: OF {build_OF_s:.2f} s, FA {build_FA_s:.2f} s, TL {build_TL_s:.2f} s, scalar "
                "{scalar_descriptors_s:.3f} s, directional {directional_descriptors_s:.3f} s)".format(**ms["cpu"])
                if "cpu" in ms else ""))
+    if not args.no_radiosity:
+        _native.trim_tracer()
+        record["batching"] = run_batching(args)
     if not args.no_cpu:
         record["cpu_baseline"] = cpu_baseline(phases, R, seed)
-        log(f"cpu baseline {record['cpu_baseline']['value']:.3e} rays/s")
+        cb = record["cpu_baseline"]
+        log(f"cpu baseline {cb['value']:.3e} rays/s on {cb['cores']} threads ({cb['one_core']['value']:.3e} on one "
+            f"core), {cb['cpu']['model']}")
     print(json.dumps(record), flush=True)
 
 

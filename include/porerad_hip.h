@@ -338,6 +338,10 @@ int prh_trace_batch_edges_device(const prh_trace_batch *b, int32_t segment, prh_
 int prh_trace_batch_info(const prh_trace_batch *b, int64_t *info, int32_t n);
 void prh_trace_batch_destroy(prh_trace_batch *b);
 
+/* *equal = 1 when two device edge tables (same device) hold the same
+ * entries, else 0; compared on the device, nothing is copied to the host.
+ * No counterpart in the reference (checks tables too large to copy). */
+int prh_edges_equal(const prh_edges *a, const prh_edges *b, int32_t *equal);
 /* Copy a device edge table to host arrays of nnz entries. */
 int prh_edges_copy(const prh_edges *e, int32_t *origin, int32_t *receiver, int64_t *count);
 void prh_edges_destroy(prh_edges *e);
@@ -377,6 +381,15 @@ int prh_operator_balance(prh_operator *op, double tolerance, int32_t *method, in
 /* Copy the balancing factor d (n doubles) to the host. */
 int prh_operator_get_scale(const prh_operator *op, double *scale);
 int prh_operator_nnz(const prh_operator *op, int64_t *nnz_symmetric, int64_t *nnz_raw);
+/* The symmetric FP64 matrix S (CSR: rowptr n + 1, col and val nnz_symmetric,
+ * columns sorted) to host arrays; any pointer may be NULL.  The matrix of
+ * symmetric_matvec, porerad/exchange.py:108-112, as stored on the device. */
+int prh_operator_csr(const prh_operator *op, int64_t *rowptr, int32_t *col, double *val);
+/* Largest relative defect of A_i F_p,ij = A_j F_p,ji over the count table's
+ * pairs, computed on the device from the stored symmetric matrix and the
+ * balancing factor.  Replaces ExchangeOperator.reciprocity_error,
+ * porerad/exchange.py:124-140. */
+int prh_operator_reciprocity(prh_operator *op, double *error);
 /* y = op(x) for one of the PRH_APPLY_* modes; x, y host arrays of n. */
 int prh_operator_apply(prh_operator *op, int32_t mode, const double *x, double *y);
 

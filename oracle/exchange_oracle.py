@@ -79,6 +79,24 @@ class OracleOperator:
     def rmatvec(self, x):
         return self.scale * self.symmetric_matvec(self.scale * (x / self.area))
 
+    def reciprocity_error(self):
+        """exchange.py:124-140 restated: max |A_i F_p,ij - A_j F_p,ji| / max |A_i F_p,ij| over the count pairs."""
+
+        n, d, A = self.n, self.scale, self.area
+        fwd = self.origin.astype(np.int64) * n + self.receiver
+        rev = self.receiver.astype(np.int64) * n + self.origin
+        at = np.minimum(np.searchsorted(fwd, rev), fwd.size - 1)
+        w_rev = np.where(fwd[at] == rev, self.weight[at], 0.0)
+        i, j = self.origin, self.receiver
+        s_ij = 0.5 * (A[i] * self.weight + A[j] * w_rev)
+        s_ji = 0.5 * (A[j] * w_rev + A[i] * self.weight)
+        diag = i == j
+        s_ij[diag] += self.delta * A[i[diag]]
+        s_ji[diag] += self.delta * A[j[diag]]
+        f_ij = A[i] * ((d[i] / A[i]) * s_ij * d[j])
+        f_ji = A[j] * ((d[j] / A[j]) * s_ji * d[i])
+        return float(np.max(np.abs(f_ij - f_ji)) / max(float(np.max(np.abs(f_ij))), _TINY))
+
     def _closure_error(self, y):
         if not (np.all(np.isfinite(y)) and np.max(y) < _LOG_MAX and np.min(y) > _LOG_MIN):
             return float("inf")

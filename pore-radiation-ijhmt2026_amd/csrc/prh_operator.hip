@@ -14,6 +14,7 @@
 // in flight: the same inputs always give the same bits.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <thread>
 #include <vector>
 
@@ -373,6 +374,55 @@ __global__ __launch_bounds__(RED_THREADS) void k_series_check(const double *part
 SView sview_of(const prh_operator *op)
 {
     return SView{op->rowptr, op->col, op->val, op->kval, op->area, op->inv_2R, op->delta};
+}
+
+// Position of column c in the sorted columns [lo, hi) of a CSR row, or -1.
+__device__ __forceinline__ int64_t find_col(const int32_t *col, int64_t lo, int64_t hi, int32_t c)
+{
+    const int64_t end = hi;
+    while (lo < hi) {
+        const int64_t mid = lo + ((hi - lo) >> 1);
+        if (col[mid] < c) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < end && col[lo] == c) ? lo : -1;
+}
+
+// Reciprocity defect of the projected view factors over the count table's
+// pairs (i, j), C_ij > 0 (exchange.py:124-140): A_i F_p,ij = A_i ((d_i / A_i)
+// S_ij d_j) against A_j F_p,ji with the mirrored entry S_ji of the symmetric
+// matrix.  Per wavefront the largest |difference| and |A_i F_p,ij|, kept as
+// the bit patterns of non-negative doubles (which order as integers).
+__global__ __launch_bounds__(ROW_THREADS) void k_reciprocity(const int64_t *c_rowptr, const int32_t *c_col,
+                                                             const int64_t *rowptr, const int32_t *col,
+                                                             const double *val, const double *d, const double *area,
+                                                             int64_t n, unsigned long long *worst_out,
+                                                             unsigned long long *largest_out)
+{
+    const int64_t i = (int64_t)blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    double worst = 0.0, largest = 0.0;
+    if (i < n) {
+        const int64_t s0 = rowptr[i], s1 = rowptr[i + 1];
+        for (int64_t e = c_rowptr[i] + lane; e < c_rowptr[i + 1]; e += 64) {
+            const int32_t j = c_col[e];
+            const int64_t ij = find_col(col, s0, s1, j);
+            const int64_t ji = find_col(col, rowptr[j], rowptr[j + 1], (int32_t)i);
+            const double s_ij = ij >= 0 ? val[ij] : 0.0, s_ji = ji >= 0 ? val[ji] : 0.0;
+            const double joint_ij = area[i] * ((d[i] / area[i]) * s_ij * d[j]);
+            const double joint_ji = area[j] * ((d[j] / area[j]) * s_ji * d[i]);
+            worst = fmax(worst, fabs(joint_ij - joint_ji));
+            largest = fmax(largest, fabs(joint_ij));
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        worst = fmax(worst, __shfl_xor(worst, off));
+        largest = fmax(largest, __shfl_xor(largest, off));
+    }
+    if (lane == 0 && i < n) {
+        atomicMax(worst_out, (unsigned long long)__double_as_longlong(worst));
+        atomicMax(largest_out, (unsigned long long)__double_as_longlong(largest));
+    }
 }
 
 unsigned blocks_for(int64_t n, int per_block)
@@ -818,6 +868,17 @@ int prh_operator_nnz(const prh_operator *op, int64_t *nnz_symmetric, int64_t *nn
     return PRH_OK;
 }
 
+int prh_operator_csr(const prh_operator *op, int64_t *rowptr, int32_t *col, double *val)
+{
+    PRH_CHECK_ARG(op, "prh_operator_csr: null operator");
+    PRH_HIP(hipSetDevice(op->device));
+    if (rowptr) PRH_HIP(hipMemcpy(rowptr, op->rowptr, sizeof(int64_t) * (op->n + 1), hipMemcpyDeviceToHost));
+    if (op->nnz_s == 0) return PRH_OK;
+    if (col) PRH_HIP(hipMemcpy(col, op->col, sizeof(int32_t) * op->nnz_s, hipMemcpyDeviceToHost));
+    if (val) PRH_HIP(hipMemcpy(val, op->val, sizeof(double) * op->nnz_s, hipMemcpyDeviceToHost));
+    return PRH_OK;
+}
+
 int prh_operator_set_scale(prh_operator *op, const double *scale)
 {
     PRH_CHECK_ARG(op && scale, "prh_operator_set_scale: null pointer");
@@ -826,6 +887,34 @@ int prh_operator_set_scale(prh_operator *op, const double *scale)
     std::vector<float> s32(op->n);
     for (int64_t i = 0; i < op->n; ++i) s32[i] = (float)scale[i];
     PRH_HIP(hipMemcpy(op->scale32, s32.data(), sizeof(float) * op->n, hipMemcpyHostToDevice));
+    return PRH_OK;
+}
+
+int prh_operator_reciprocity(prh_operator *op, double *error)
+{
+    PRH_CHECK_ARG(op && error, "prh_operator_reciprocity: null pointer");
+    PRH_HIP(hipSetDevice(op->device));
+    hipStream_t st;
+    int rc = current_stream(&st);
+    if (rc) return rc;
+    *error = 0.0;
+    if (op->n == 0) return PRH_OK;
+    // the first word of two work vectors holds the two maxima
+    unsigned long long *worst = reinterpret_cast<unsigned long long *>(op->vx);
+    unsigned long long *largest = reinterpret_cast<unsigned long long *>(op->vy);
+    PRH_HIP(hipMemsetAsync(worst, 0, sizeof(unsigned long long), st));
+    PRH_HIP(hipMemsetAsync(largest, 0, sizeof(unsigned long long), st));
+    k_reciprocity<<<blocks_for(op->n, ROWS_PER_BLOCK), ROW_THREADS, 0, st>>>(
+        op->c_rowptr, op->c_col, op->rowptr, op->col, op->val, op->scale, op->area, op->n, worst, largest);
+    PRH_HIP(hipGetLastError());
+    unsigned long long host[2] = {0, 0};
+    PRH_HIP(hipMemcpyAsync(&host[0], worst, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    PRH_HIP(hipMemcpyAsync(&host[1], largest, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    PRH_HIP(hipStreamSynchronize(st));
+    double worst_v, largest_v;
+    memcpy(&worst_v, &host[0], sizeof(double));
+    memcpy(&largest_v, &host[1], sizeof(double));
+    *error = worst_v / fmax(largest_v, 2.2250738585072014e-308);
     return PRH_OK;
 }
 

@@ -1353,6 +1353,17 @@ size_t noalloc_pass_bytes(int64_t n_p, int64_t SR)
            al((size_t)scan_scratch_elems(n_p) * sizeof(int64_t));
 }
 
+// Element-wise comparison of two device edge tables of n entries (sets *diff).
+__global__ __launch_bounds__(256) void k_edges_differ(const int32_t *ao, const int32_t *ar, const int32_t *ac,
+                                                      const int32_t *bo, const int32_t *br, const int32_t *bc,
+                                                      int64_t n, unsigned int *diff)
+{
+    bool d = false;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256)
+        d |= ao[e] != bo[e] || ar[e] != br[e] || ac[e] != bc[e];
+    if (__ballot(d) != 0 && (threadIdx.x & 63u) == 0) atomicOr(diff, 1u);
+}
+
 }  // namespace
 
 }  // namespace prh
@@ -2292,6 +2303,38 @@ int prh_edges_copy(const prh_edges *e, int32_t *origin, int32_t *receiver, int64
         PRH_HIP(hipMemcpy(tmp, e->count, e->nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
         for (int64_t k = 0; k < e->nnz; ++k) count[k] = (int64_t)tmp[k];
     }
+    return PRH_OK;
+}
+
+int prh_edges_equal(const prh_edges *a, const prh_edges *b, int32_t *equal)
+{
+    PRH_CHECK_ARG(a && b && equal, "prh_edges_equal: null pointer");
+    PRH_CHECK_ARG(a->device == b->device, "prh_edges_equal: tables on different devices");
+    *equal = 0;
+    if (a->nnz != b->nnz) return PRH_OK;
+    if (a->nnz == 0) {
+        *equal = 1;
+        return PRH_OK;
+    }
+    PRH_HIP(hipSetDevice(a->device));
+    hipStream_t st;
+    int rc = current_stream(&st);
+    if (rc) return rc;
+    unsigned int *flag = nullptr;
+    PRH_HIP(hipMalloc((void **)&flag, sizeof(unsigned int)));
+    unsigned int host = 1;
+    hipError_t e = hipMemsetAsync(flag, 0, sizeof(unsigned int), st);
+    if (e == hipSuccess) {
+        const int64_t blocks = std::min<int64_t>((a->nnz + 255) / 256, 8192);
+        k_edges_differ<<<(unsigned)blocks, 256, 0, st>>>(a->origin, a->receiver, a->count, b->origin, b->receiver,
+                                                         b->count, a->nnz, flag);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&host, flag, sizeof(unsigned int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(flag);
+    PRH_HIP(e);
+    *equal = host == 0 ? 1 : 0;
     return PRH_OK;
 }
 

@@ -76,6 +76,7 @@ SIGNATURES = {
     "prh_trace_counts": ([_P, _I64, _I64, _I32, _I32, _P, _P, _P, _P], ctypes.c_int),
     "prh_trace_counts_into": ([_P, _I64, _I64, _I32, _I32, _P, _P, _P, _P, _I64, _P, _P], ctypes.c_int),
     "prh_edges_copy": ([_P, _P, _P, _P], ctypes.c_int),
+    "prh_edges_equal": ([_P, _P, _P], ctypes.c_int),
     "prh_trace_batch_create": ([_I32, _P, _P, _P, _P, _P, _I64, _I64, _I32, _I32, _P], ctypes.c_int),
     "prh_trace_batch_run": ([_P, _P, _P, _P, _P], ctypes.c_int),
     "prh_trace_batch_edges": ([_P, _I32, _P, _P, _P], ctypes.c_int),
@@ -90,6 +91,8 @@ SIGNATURES = {
     "prh_operator_create_device": ([_P, _I64, _P, _P, _F64, _P], ctypes.c_int),
     "prh_operator_edges": ([_P, _P, _P, _P], ctypes.c_int),
     "prh_operator_nnz": ([_P, _P, _P], ctypes.c_int),
+    "prh_operator_csr": ([_P, _P, _P, _P], ctypes.c_int),
+    "prh_operator_reciprocity": ([_P, _P], ctypes.c_int),
     "prh_operator_stats": ([_P, _P, _I32], ctypes.c_int),
     "prh_batch_stats": ([_P, _P, _I32], ctypes.c_int),
     "prh_operator_apply": ([_P, _I32, _P, _P], ctypes.c_int),
@@ -466,6 +469,13 @@ class Edges:
         check(self._lib.prh_edges_copy(self.handle, ptr(origin), ptr(receiver), ptr(count)), "prh_edges_copy")
         return origin, receiver, count
 
+    def equals(self, other: "Edges") -> bool:
+        """Whether two device edge tables hold the same entries (compared on the device)."""
+
+        eq = ctypes.c_int32()
+        check(self._lib.prh_edges_equal(self.handle, other.handle, ctypes.byref(eq)), "prh_edges_equal")
+        return bool(eq.value)
+
     def close(self) -> None:
         if getattr(self, "handle", None) is not None and self.handle.value:
             self._lib.prh_edges_destroy(self.handle)
@@ -593,6 +603,22 @@ class Operator:
         check(self._lib.prh_operator_nnz(handle, ctypes.byref(a), ctypes.byref(b)), "prh_operator_nnz")
         self.nnz_symmetric, self.nnz_raw = int(a.value), int(b.value)
         return self
+
+    def reciprocity(self) -> float:
+        """Largest relative reciprocity defect over the count table's pairs (computed on the device)."""
+
+        err = ctypes.c_double()
+        check(self._lib.prh_operator_reciprocity(self.handle, ctypes.byref(err)), "prh_operator_reciprocity")
+        return float(err.value)
+
+    def csr(self):
+        """(rowptr, col, val) of the symmetric FP64 matrix S held on the device."""
+
+        rowptr = np.empty(self.n + 1, np.int64)
+        col = np.empty(self.nnz_symmetric, np.int32)
+        val = np.empty(self.nnz_symmetric, np.float64)
+        check(self._lib.prh_operator_csr(self.handle, ptr(rowptr), ptr(col), ptr(val)), "prh_operator_csr")
+        return rowptr, col, val
 
     def edges(self):
         """(origin, receiver, count) the operator was built from."""

@@ -12,9 +12,10 @@ F_p x = (d / A) S(d x) and F_p^T x = d S(d x / A).
 
 S lives in HBM as an FP64 CSR built by the HIP library (``prh_operator_*``);
 every product with it, and the reflection series J = E + (1 - eps) F_p^T J,
-runs on the GPU.  The balancing factor is found by Anderson acceleration on
-y = log d (SciPy, exchange.py:175-183) with a damped multiplicative iteration
-as fallback, each residual being one device product.
+runs on the GPU.  The balancing factor is found on the device as well
+(``prh_operator_balance``): SciPy's Anderson mixing on y = log d restated
+with device vectors (exchange.py:175-183), with the damped multiplicative
+iteration as fallback.  The reciprocity check runs on the device too.
 """
 
 from __future__ import annotations
@@ -29,8 +30,6 @@ from .constants import SIGMA
 SUPPORT_DIAGONAL_FRACTION = 1.0e-15
 CLOSURE_TOLERANCE = 1.0e-10
 _TINY = np.finfo(np.float64).tiny
-_LOG_MAX = float(np.log(np.finfo(np.float64).max))
-_LOG_MIN = float(np.log(np.nextafter(0.0, 1.0)))
 
 
 def _merge_edges(origin, receiver, count, n: int):
@@ -179,35 +178,15 @@ class ExchangeOperator:
         return self.device.apply(_native.APPLY_RMATVEC, x)
 
     def reciprocity_error(self) -> float:
-        """Largest relative defect of A_i F_p,ij = A_j F_p,ji over the sampled pairs (exchange.py:124-140)."""
+        """Largest relative defect of A_i F_p,ij = A_j F_p,ji over the count table's pairs (exchange.py:124-140).
 
-        n, d, A = self.n_patches, self.scale, self.patch_area_m2
-        key = self.origin.astype(np.int64) * n + self.receiver
-        back = self.receiver.astype(np.int64) * n + self.origin
-        at = np.minimum(np.searchsorted(key, back), key.size - 1)
-        w_back = np.where(key[at] == back, self.weight[at], 0.0)
-        i, j = self.origin, self.receiver
-        s_ij = 0.5 * (A[i] * self.weight + A[j] * w_back)
-        s_ji = 0.5 * (A[j] * w_back + A[i] * self.weight)
-        on_diag = i == j
-        s_ij[on_diag] += self.support_diagonal_fraction * A[i[on_diag]]
-        s_ji[on_diag] += self.support_diagonal_fraction * A[j[on_diag]]
-        pair_ij = A[i] * ((d[i] / A[i]) * s_ij * d[j])
-        pair_ji = A[j] * ((d[j] / A[j]) * s_ji * d[i])
-        return float(np.max(np.abs(pair_ij - pair_ji)) / max(float(np.max(np.abs(pair_ij))), _TINY))
+        Evaluated on the device (``prh_operator_reciprocity``) from the stored
+        symmetric matrix and the current balancing factor.
+        """
+
+        return self.device.reciprocity()
 
     # -- balancing ------------------------------------------------------------------
-    def _closure_error(self, y: np.ndarray) -> float:
-        """max |d (S d) / A - 1| for d = exp(y) (exchange.py:142-151)."""
-
-        if not (np.all(np.isfinite(y)) and np.max(y) < _LOG_MAX and np.min(y) > _LOG_MIN):
-            return float("inf")
-        d = np.exp(y)
-        row = d * self.symmetric_matvec(d)
-        if not np.all(np.isfinite(row)) or np.any(row <= 0.0):
-            return float("inf")
-        return float(np.max(np.abs(row / self.patch_area_m2 - 1.0)))
-
     def _balance(self, tolerance: float) -> tuple[np.ndarray, str, int]:
         """Solve d * (S d) = A for d > 0 (exchange.py:153-197), on the device.
 

@@ -38,3 +38,11 @@ def phases(name: str) -> np.ndarray:
 
 def phases_x(name: str, axis: str) -> np.ndarray:
     return np.ascontiguousarray(np.moveaxis(phases(name), AXES[axis], 0))
+
+
+@functools.lru_cache(maxsize=None)
+def configs() -> dict:
+    """CPU-oracle results of the full-size benchmark workloads (tests/golden/make_config_fixtures.py)."""
+
+    with open(os.path.join(GOLDEN, "configs.json")) as handle:
+        return json.load(handle)

@@ -66,6 +66,27 @@ def test_balancing_closes_rows_like_oracle(mods):
     assert op.reciprocity_error() <= 1e-12
 
 
+@pytest.mark.parametrize("nonuniform", [False, True])
+def test_reciprocity_error_on_device_matches_oracle(mods, nonuniform):
+    # the device evaluates the reference's expression on the stored symmetric
+    # matrix; with the oracle's factor d the same pairs give the same defect
+    _, ex, _ = mods
+    phases, Q, (o, r, c), rays = _small(11)
+    area = Q.patch_area_m2.copy()
+    if nonuniform:
+        heavy = (o % 3) == 0
+        c = np.where(heavy, 2 * c, c)
+        rays = np.where(np.arange(Q.n_patches) % 3 == 0, 2 * rays, rays)
+        area = area * (1.0 + 0.25 * (np.arange(Q.n_patches) % 4))
+    op = ex.ExchangeOperator(o, r, c, rays, area)
+    ref = X.OracleOperator(o, r, c, rays, area)
+    op.device.set_scale(ref.scale)
+    op.scale = ref.scale
+    got, want = op.reciprocity_error(), ref.reciprocity_error()
+    assert want <= 1e-12
+    assert abs(got - want) <= 1e-3 * want + 1e-300, (got, want)
+
+
 @pytest.mark.parametrize("eps", [0.3, 0.6, 0.9, "mixed"])
 def test_reflection_series_matches_oracle(small_pair, mods, eps):
     _, ex, _ = mods
