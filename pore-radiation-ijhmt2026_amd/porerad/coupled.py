@@ -12,8 +12,9 @@ device reflection series (porerad.exchange).
 ``newton_krylov`` keeps its cell vectors on the device (``prh_coupled``):
 conduction products, radiation, JVPs, the preconditioner and the
 Gram-Schmidt passes run there, and only scalars and the small FGMRES
-least-squares problems come to the host.  ``flexible_gmres`` and
-``FixedJacobiPCG`` remain the reference's generic host utilities.
+least-squares problems come to the host.  ``flexible_gmres`` (FGMRES with
+Givens rotations) and ``FixedJacobiPCG`` are the generic host utilities of
+the same names.
 """
 
 from __future__ import annotations
@@ -44,37 +45,36 @@ PCG_STEPS = (10, 20)
 
 
 # -- patch <-> cell maps (coupled.py:45-73) ---------------------------------------
+# Every internal patch belongs to one solid cell (SurfacePatches.internal_owners);
+# walls belong to none.  Gathers index the flat cell field by owner; scatters
+# accumulate with np.bincount, which adds each bin's terms in patch order.
 
 def patch_temperature(patches: SurfacePatches, T_cells: np.ndarray, T_hot_K: float, T_cold_K: float) -> np.ndarray:
     """Wall temperatures on wall patches, the owning-cell temperature on internal patches."""
 
-    cls = patches.patch_class
-    T = np.empty(patches.n_patches, dtype=np.float64)
-    T[cls == HOT] = float(T_hot_K)
-    T[cls == COLD] = float(T_cold_K)
-    inner = cls == INTERNAL
-    T[inner] = np.asarray(T_cells, dtype=np.float64).ravel()[patches.patch_cell[inner]]
+    inner, owner = patches.internal_owners()
+    wall_T = np.where(patches.patch_class == HOT, float(T_hot_K), float(T_cold_K))
+    T = np.asarray(wall_T, dtype=np.float64)
+    T[inner] = np.asarray(T_cells, dtype=np.float64).reshape(-1)[owner]
     return T
 
 
 def cell_power(patches: SurfacePatches, patch_power_W: np.ndarray) -> np.ndarray:
     """Internal patch power summed into the owning solid cells."""
 
-    inner = patches.patch_class == INTERNAL
-    total = np.zeros(patches.solid.size, dtype=np.float64)
-    np.add.at(total, patches.patch_cell[inner], patch_power_W[inner])
-    return total.reshape(patches.solid.shape)
+    inner, owner = patches.internal_owners()
+    per_cell = np.bincount(owner, weights=np.asarray(patch_power_W, dtype=np.float64)[inner],
+                           minlength=patches.solid.size)
+    return per_cell.reshape(patches.solid.shape)
 
 
 def emission_diagonal(patches: SurfacePatches, T_cells: np.ndarray, epsilon: float) -> np.ndarray:
     """sum_p 4 eps sigma A_p T^3 of every cell (W/K)."""
 
-    inner = patches.patch_class == INTERNAL
-    owner = patches.patch_cell[inner].astype(np.int64)
-    T = np.asarray(T_cells, dtype=np.float64).ravel()[owner]
-    diag = np.zeros(patches.solid.size, dtype=np.float64)
-    np.add.at(diag, owner, 4.0 * float(epsilon) * SIGMA * patches.patch_area_m2[inner] * T**3)
-    return diag
+    inner, owner = patches.internal_owners()
+    T_owner = np.asarray(T_cells, dtype=np.float64).reshape(-1)[owner]
+    slope = 4.0 * float(epsilon) * SIGMA * patches.patch_area_m2[inner] * T_owner**3
+    return np.bincount(owner, weights=slope, minlength=patches.solid.size)
 
 
 @dataclass
@@ -118,12 +118,11 @@ def radiation_jvp(
 ) -> np.ndarray:
     """Change of the net cell power for a cell temperature perturbation, walls fixed (coupled.py:108-126)."""
 
-    inner = patches.patch_class == INTERNAL
-    dT_patch = np.zeros(patches.n_patches, dtype=np.float64)
-    dT_patch[inner] = np.asarray(dT_cells, dtype=np.float64).ravel()[patches.patch_cell[inner]]
-    eps = np.full(patches.n_patches, float(epsilon))
-    jvp = net_patch_power_jvp(operator, T_patch, dT_patch, eps, patches.patch_area_m2,
-                              relative_tolerance=relative_tolerance)
+    inner, owner = patches.internal_owners()
+    dT_patch = np.zeros(patches.n_patches, dtype=np.float64)  # wall patches keep their temperatures
+    dT_patch[inner] = np.asarray(dT_cells, dtype=np.float64).reshape(-1)[owner]
+    jvp = net_patch_power_jvp(operator, T_patch, dT_patch, np.full(patches.n_patches, float(epsilon)),
+                              patches.patch_area_m2, relative_tolerance=relative_tolerance)
     return cell_power(patches, jvp)
 
 
@@ -145,12 +144,21 @@ def linear_cell_temperature(shape: tuple[int, int, int], T_mean_K: float, delta_
 
 
 def coarse_source(cell_power_W: np.ndarray, bins: int = 8) -> np.ndarray:
-    """Cell power summed over a bins^3 grid of equal blocks."""
+    """Cell power summed over a bins^3 grid of equal blocks (coupled.py:146-153).
 
-    a = np.asarray(cell_power_W, dtype=np.float64)
-    parts = [np.array_split(np.arange(n), min(bins, n)) for n in a.shape]
-    sums = [float(np.sum(a[np.ix_(xs, ys, zs)])) for xs in parts[0] for ys in parts[1] for zs in parts[2]]
-    return np.asarray(sums).reshape(tuple(len(p) for p in parts))
+    Along an axis of n cells there are min(bins, n) blocks whose sizes differ
+    by at most one, the larger ones first (the partition of np.array_split);
+    each axis is reduced in turn with np.add.reduceat.
+    """
+
+    blocks = np.asarray(cell_power_W, dtype=np.float64)
+    for axis, n in enumerate(blocks.shape):
+        k = min(int(bins), int(n))
+        sizes = np.full(k, n // k, dtype=np.int64)
+        sizes[: n % k] += 1
+        starts = np.concatenate(([0], np.cumsum(sizes)[:-1]))
+        blocks = np.add.reduceat(blocks, starts, axis=axis)
+    return blocks
 
 
 @dataclass
@@ -201,17 +209,20 @@ def relative_difference(a: float, b: float) -> float:
     return abs(float(a) - float(b)) / max(abs(float(a)), abs(float(b)), _TINY)
 
 
+def _l1_relative(a: np.ndarray, b: np.ndarray) -> float:
+    """||a - b||_1 / max(||a||_1, ||b||_1)."""
+
+    spread = float(np.sum(np.abs(a - b)))
+    return spread / max(float(np.sum(np.abs(a))), float(np.sum(np.abs(b))), _TINY)
+
+
 def ab_differences(set_a: list[PrescribedChannels], set_b: list[PrescribedChannels]) -> tuple[float, float]:
     """Largest A/B relative difference of q_h, q_c and of the 8^3 coarse source, L1 (coupled.py:201-213)."""
 
-    channel = max(relative_difference(getattr(a, k), getattr(b, k))
-                  for a, b in zip(set_a, set_b) for k in ("q_h", "q_c"))
-    source = max(
-        float(np.sum(np.abs(a.coarse_source - b.coarse_source)))
-        / max(float(np.sum(np.abs(a.coarse_source))), float(np.sum(np.abs(b.coarse_source))), _TINY)
-        for a, b in zip(set_a, set_b)
-    )
-    return channel, source
+    pairs = list(zip(set_a, set_b))
+    wall = max(relative_difference(getattr(a, q), getattr(b, q)) for a, b in pairs for q in ("q_h", "q_c"))
+    source = max(_l1_relative(a.coarse_source, b.coarse_source) for a, b in pairs)
+    return wall, source
 
 
 def append_required(channel_difference: float, source_difference: float) -> bool:
@@ -232,86 +243,125 @@ class GMRESResult:
 
 def flexible_gmres(matvec, rhs: np.ndarray, preconditioner, *, restart: int, max_iterations: int,
                    relative_tolerance: float, absolute_tolerance: float) -> GMRESResult:
-    """Right-preconditioned FGMRES with modified Gram-Schmidt and one re-orthogonalisation (coupled.py:232-289).
+    """Right-preconditioned flexible GMRES(restart) from x = 0 (coupled.py:232-289).
 
-    The preconditioned vectors are kept (the preconditioner may vary); a
-    small least-squares residual is confirmed with the true residual.
+    Saad's FGMRES: the preconditioned directions z_j are stored (the
+    preconditioner may change between calls), each new Krylov vector is
+    orthogonalised by two modified Gram-Schmidt sweeps, and the Hessenberg
+    least-squares problem is kept triangular by Givens rotations, so the
+    residual norm of the current iterate is |g_{j+1}| at every step.  When it
+    reaches max(atol, rtol ||b||) (or the basis breaks down) the iterate is
+    formed and accepted only if its true residual does too; otherwise the
+    cycle restarts from it.  At most ``max_iterations`` inner steps in all.
     """
 
     b = np.asarray(rhs, dtype=np.float64)
-    n = b.size
-    x = np.zeros_like(b)
     b_norm = max(float(np.linalg.norm(b)), _TINY)
     goal = max(float(absolute_tolerance), float(relative_tolerance) * b_norm)
-    eps = np.finfo(np.float64).eps
-    done = 0
-    while done < max_iterations:
+    breakdown = np.finfo(np.float64).eps
+    x = np.zeros_like(b)
+    steps = 0
+
+    def true_residual(v: np.ndarray) -> float:
+        return float(np.linalg.norm(b - matvec(v)))
+
+    while steps < max_iterations:
         r = b - matvec(x)
         beta = float(np.linalg.norm(r))
         if beta <= goal:
-            return GMRESResult(x, True, done, beta / b_norm)
-        m = min(restart, max_iterations - done)
-        V = np.zeros((m + 1, n))
-        Z = np.zeros((m, n))
-        H = np.zeros((m + 1, m))
+            return GMRESResult(x, True, steps, beta / b_norm)
+        m = min(int(restart), int(max_iterations) - steps)
+        basis = [r / beta]
+        directions: list[np.ndarray] = []
+        R = np.zeros((m + 1, m))        # rotated Hessenberg matrix (upper triangular part used)
+        rot = np.zeros((m, 2))          # Givens pairs (c, s)
         g = np.zeros(m + 1)
-        V[0] = r / beta
         g[0] = beta
-        restart_at = None
-        for j in range(m):
-            Z[j] = preconditioner(V[j])
-            w = matvec(Z[j])
-            for _ in range(2):
-                for i in range(j + 1):
-                    h = float(np.dot(V[i], w))
-                    H[i, j] += h
-                    w -= h * V[i]
-            h_next = float(np.linalg.norm(w))
-            H[j + 1, j] = h_next
-            if h_next > eps * beta:
-                V[j + 1] = w / h_next
-            Hj = H[: j + 2, : j + 1]
-            y = np.linalg.lstsq(Hj, g[: j + 2], rcond=None)[0]
-            estimate = float(np.linalg.norm(g[: j + 2] - Hj @ y))
-            done += 1
-            if estimate <= goal or h_next <= eps * beta:
-                candidate = x + y @ Z[: j + 1]
-                true_norm = float(np.linalg.norm(b - matvec(candidate)))
-                if true_norm <= goal:
-                    return GMRESResult(candidate, True, done, true_norm / b_norm)
-                restart_at = candidate
+        finished = False
+        k = 0
+        for k in range(m):
+            directions.append(preconditioner(basis[k]))
+            w = matvec(directions[k])
+            column = np.zeros(k + 2)
+            for _sweep in range(2):
+                for i in range(k + 1):
+                    h = float(np.dot(basis[i], w))
+                    column[i] += h
+                    w = w - h * basis[i]
+            column[k + 1] = float(np.linalg.norm(w))
+            broke = column[k + 1] <= breakdown * beta
+            if not broke:
+                basis.append(w / column[k + 1])
+            for i in range(k):  # earlier rotations on the new column
+                c_i, s_i = rot[i]
+                column[i], column[i + 1] = c_i * column[i] + s_i * column[i + 1], -s_i * column[i] + c_i * column[i + 1]
+            rho = float(np.hypot(column[k], column[k + 1]))
+            c_k, s_k = (column[k] / rho, column[k + 1] / rho) if rho > 0.0 else (1.0, 0.0)
+            rot[k] = (c_k, s_k)
+            column[k], column[k + 1] = rho, 0.0
+            g[k], g[k + 1] = c_k * g[k], -s_k * g[k]
+            R[: k + 2, k] = column
+            steps += 1
+            if abs(g[k + 1]) <= goal or broke:
+                finished = True
                 break
-        if restart_at is not None:
-            x = restart_at
-            continue
-        x += np.linalg.lstsq(H, g, rcond=None)[0] @ Z
-    true_norm = float(np.linalg.norm(b - matvec(x)))
-    return GMRESResult(x, true_norm <= goal, done, true_norm / b_norm)
+        used = k + 1
+        y = _upper_solve(R[:used, :used], g[:used])
+        candidate = x + np.asarray(y) @ np.asarray(directions[:used])
+        if finished:
+            norm = true_residual(candidate)
+            if norm <= goal:
+                return GMRESResult(candidate, True, steps, norm / b_norm)
+        x = candidate
+    norm = true_residual(x)
+    return GMRESResult(x, norm <= goal, steps, norm / b_norm)
+
+
+def _upper_solve(U: np.ndarray, rhs: np.ndarray) -> np.ndarray:
+    """Back substitution; a zero pivot (exact breakdown) contributes nothing."""
+
+    n = rhs.size
+    y = np.zeros(n)
+    for i in range(n - 1, -1, -1):
+        acc = rhs[i] - float(np.dot(U[i, i + 1:], y[i + 1:]))
+        y[i] = acc / U[i, i] if U[i, i] != 0.0 else 0.0
+    return y
 
 
 class FixedJacobiPCG:
-    """A fixed number of Jacobi-preconditioned CG steps on A + diag(shift), from x = 0 (coupled.py:292-315)."""
+    """A fixed number of Jacobi-preconditioned CG steps on A + diag(shift), started from x = 0 (coupled.py:292-315).
+
+    Used as the FGMRES preconditioner: a fixed step count makes it a fixed
+    linear map.  The last step needs no new residual or direction.
+    """
 
     def __init__(self, system: ConductionSystem, shift: np.ndarray, steps: int) -> None:
-        self.matrix, self.shift, self.steps = system.matrix, shift, int(steps)
-        self.inverse_diagonal = 1.0 / (system.matrix.diagonal() + shift)
+        self.steps = int(steps)
+        self._A = system.matrix
+        self._shift = np.asarray(shift, dtype=np.float64)
+        self._inverse_diagonal = 1.0 / (system.matrix.diagonal() + self._shift)
+
+    def _shifted(self, v: np.ndarray) -> np.ndarray:
+        return self._A @ v + self._shift * v
 
     def __call__(self, rhs: np.ndarray) -> np.ndarray:
-        x = np.zeros_like(rhs)
-        r = rhs.copy()
-        z = self.inverse_diagonal * r
-        p = z.copy()
-        rz = r @ z
-        for step in range(self.steps):
-            Ap = self.matrix @ p + self.shift * p
-            alpha = rz / (p @ Ap)
-            x += alpha * p
-            r -= alpha * Ap
-            z = self.inverse_diagonal * r
-            if step + 1 < self.steps:
-                rz_next = r @ z
-                p = z + (rz_next / rz) * p
-                rz = rz_next
+        residual = np.array(rhs, dtype=np.float64, copy=True)
+        x = np.zeros_like(residual)
+        if self.steps <= 0:
+            return x
+        z = self._inverse_diagonal * residual
+        direction = z.copy()
+        rz = float(residual @ z)
+        for k in range(self.steps):
+            image = self._shifted(direction)
+            alpha = rz / float(direction @ image)
+            x += alpha * direction
+            if k + 1 == self.steps:
+                break
+            residual -= alpha * image
+            z = self._inverse_diagonal * residual
+            rz, rz_previous = float(residual @ z), rz
+            direction = z + (rz / rz_previous) * direction
         return x
 
 
@@ -547,44 +597,53 @@ def solve_coupled(
     return result
 
 
+# Acceptance thresholds of a coupled case (coupled.py:480-490).
+WALL_FLUX_MISMATCH_MAX = 0.005
+ENERGY_RESIDUAL_MAX = 1.0e-3
+
+
 def wall_flux_metrics(
     system: ConductionSystem, patches: SurfacePatches, result: CoupledResult, T_hot_K: float, T_cold_K: float
 ) -> dict[str, float]:
-    """Conductive, radiative and total wall fluxes and the acceptance quantities (coupled.py:450-477)."""
+    """Conductive, radiative and total wall fluxes and the acceptance quantities (coupled.py:450-477).
 
-    q_cond_hot, q_cond_cold = system.wall_fluxes(result.temperature_K, T_hot_K, T_cold_K)
-    area = cross_section_area(patches)
+    Fluxes are W m^-2 over the cross section N_y N_z (H/N_x)^2, positive from
+    the hot wall into the slab and from the slab out through the cold wall.
+    """
+
+    section = cross_section_area(patches)
     net = result.state.patch_net_power_W
-    q_rad_hot = -float(np.sum(net[patches.patch_class == HOT])) / area
-    q_rad_cold = float(np.sum(net[patches.patch_class == COLD])) / area
-    q_total_hot, q_total_cold = q_cond_hot + q_rad_hot, q_cond_cold + q_rad_cold
-    scale = max(abs(q_total_hot), abs(q_total_cold), _TINY)
+    cls = patches.patch_class
+    conduct = dict(zip(("hot", "cold"), system.wall_fluxes(result.temperature_K, T_hot_K, T_cold_K)))
+    # a wall patch's net power is absorbed minus emitted: the hot wall gives its loss to the slab,
+    # the cold wall takes its gain out of it
+    radiate = {"hot": -float(np.sum(net[cls == HOT])) / section, "cold": float(np.sum(net[cls == COLD])) / section}
+    total = {side: conduct[side] + radiate[side] for side in ("hot", "cold")}
+    scale = max(abs(total["hot"]), abs(total["cold"]), _TINY)
     T = result.temperature_K
-    return {
-        "q_cond_hot_W_m2": q_cond_hot,
-        "q_rad_hot_W_m2": q_rad_hot,
-        "q_total_hot_W_m2": q_total_hot,
-        "q_cond_cold_W_m2": q_cond_cold,
-        "q_rad_cold_W_m2": q_rad_cold,
-        "q_total_cold_W_m2": q_total_cold,
-        "radiative_fraction_hot": q_rad_hot / q_total_hot,
-        "k_eff_apparent_W_mK": q_total_hot * patches.H_m / (float(T_hot_K) - float(T_cold_K)),
-        "wall_flux_mismatch_rel": abs(q_total_hot - q_total_cold) / scale,
-        "energy_residual_rel": abs(result.state.radiosity.global_energy_residual_W) / max(scale * area, _TINY),
-        "T_min_K": float(np.min(T)),
-        "T_max_K": float(np.max(T)),
-        "temperature_bound_violations": int(np.count_nonzero((T < float(T_cold_K)) | (T > float(T_hot_K)))),
-    }
+    out: dict[str, float] = {}
+    for side in ("hot", "cold"):
+        out[f"q_cond_{side}_W_m2"] = conduct[side]
+        out[f"q_rad_{side}_W_m2"] = radiate[side]
+        out[f"q_total_{side}_W_m2"] = total[side]
+    out["radiative_fraction_hot"] = radiate["hot"] / total["hot"]
+    out["k_eff_apparent_W_mK"] = total["hot"] * patches.H_m / (float(T_hot_K) - float(T_cold_K))
+    out["wall_flux_mismatch_rel"] = abs(total["hot"] - total["cold"]) / scale
+    out["energy_residual_rel"] = abs(result.state.radiosity.global_energy_residual_W) / max(scale * section, _TINY)
+    out["T_min_K"] = float(np.min(T))
+    out["T_max_K"] = float(np.max(T))
+    out["temperature_bound_violations"] = int(np.count_nonzero((T < float(T_cold_K)) | (T > float(T_hot_K))))
+    return out
 
 
 def acceptance_failures(result: CoupledResult, metrics: dict[str, float]) -> list[str]:
-    """Acceptance conditions not met (empty when accepted) (coupled.py:480-490)."""
+    """Acceptance conditions not met, in the reference's order and wording; empty when accepted (coupled.py:480-490)."""
 
-    checks = {
-        "Newton iteration converged": result.converged,
-        "linear residuals within max(1e-13, 1e-8 ||b||)": result.linear_residuals_pass,
-        "wall-flux mismatch <= 0.5 %": metrics["wall_flux_mismatch_rel"] <= 0.005,
-        "global energy residual <= 1e-3": metrics["energy_residual_rel"] <= 1.0e-3,
-        "temperature within the wall temperatures": metrics["temperature_bound_violations"] == 0,
-    }
-    return [name for name, ok in checks.items() if not ok]
+    rules = (
+        ("Newton iteration converged", lambda: result.converged),
+        ("linear residuals within max(1e-13, 1e-8 ||b||)", lambda: result.linear_residuals_pass),
+        ("wall-flux mismatch <= 0.5 %", lambda: metrics["wall_flux_mismatch_rel"] <= WALL_FLUX_MISMATCH_MAX),
+        ("global energy residual <= 1e-3", lambda: metrics["energy_residual_rel"] <= ENERGY_RESIDUAL_MAX),
+        ("temperature within the wall temperatures", lambda: metrics["temperature_bound_violations"] == 0),
+    )
+    return [label for label, holds in rules if not holds()]

@@ -58,10 +58,19 @@ class SurfacePatches:
     cold_wall_patch: np.ndarray
     H_m: float
     device: _native.Geometry | None = field(default=None, repr=False, compare=False)
+    _owners: tuple | None = field(default=None, init=False, repr=False, compare=False)
 
     @property
     def n_patches(self) -> int:
         return int(self.patch_class.size)
+
+    def internal_owners(self) -> tuple[np.ndarray, np.ndarray]:
+        """(mask of the internal patches, flat index of the solid cell owning each), built once."""
+
+        if self._owners is None:
+            inner = self.patch_class == INTERNAL
+            self._owners = (inner, self.patch_cell[inner].astype(np.intp))
+        return self._owners
 
     def geometry(self) -> _native.Geometry:
         """The device patch table, uploaded on first use."""
