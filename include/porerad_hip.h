@@ -70,6 +70,10 @@ extern "C" {
 /* ---- radiosity precisions (prh_radiosity) ---------------------------- */
 #define PRH_FP64 0  /* plain FP64 reflection series */
 #define PRH_MIXED 1 /* batched FP32 Chebyshev sweeps + FP64 residual refinement */
+#define PRH_FP64_BATCHED 2 /* the FP64 reflection series of up to 32 emissions per pass over the
+                              operator (FP64 slot vectors interleaved per patch: 16 slots = one
+                              128 B line per gathered column, 32 slots = two); sums in another
+                              order than PRH_FP64, equal to rounding */
 
 typedef struct prh_geometry prh_geometry;
 typedef struct prh_edges prh_edges;
@@ -395,8 +399,10 @@ int prh_operator_apply(prh_operator *op, int32_t mode, const double *x, double *
 
 /* Device time of the FP64 reflection series run on this operator (HIP events
  * around each series): out[0..n) = series, reflections, milliseconds,
- * nonzeros of S, patches, 1 if the integer form K is held.  No counterpart
- * in the reference (measurement). */
+ * nonzeros of S, patches, 1 if the integer form K is held, then for the
+ * batched FP64 series (PRH_FP64_BATCHED) passes, slot-weighted passes (the
+ * slot width summed over passes) and milliseconds.  No counterpart in the
+ * reference (measurement). */
 int prh_operator_stats(const prh_operator *op, double *out, int32_t n);
 /* Reflection series J = E + (1 - eps) F_p^T J for n_rhs emissions.
  * emitted: n_rhs x n (row-major, one emission per row); emissivity:

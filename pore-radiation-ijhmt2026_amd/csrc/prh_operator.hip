@@ -371,6 +371,124 @@ __global__ __launch_bounds__(RED_THREADS) void k_series_check(const double *part
     }
 }
 
+// ---------------------------------------------------------------- batched FP64 series
+// The reflection series of up to 32 emissions in one pass over the operator:
+// slot vectors interleaved [row][FS] in FP64 (FS = 16 or 32), so one gathered
+// column brings all slots in FS * 8 bytes (one 128 B line at FS = 16, the
+// mixed engine's line with FP64 slots).  Each slot follows the reference's
+// recursion (exchange.py:217-245) and stops on its own when
+// ||flight||_1 <= rtol ||E||_1; the sums run in another order than
+// k_series_step, so results agree to rounding.
+constexpr int FSLOTS_MAX = 32;
+
+struct SeriesSlots {
+    double initial[FSLOTS_MAX];  // max(||E||_1, tiny)
+    int32_t done[FSLOTS_MAX];
+    int32_t iterations[FSLOTS_MAX];
+};
+
+// One wave per row: FS / 2 lanes cover a column's slots with 16 B loads, the
+// wave's 64 / (FS / 2) lane groups take interleaved nonzeros, every lane has
+// FS / 2 gathers in flight.  Workgroups are dealt round-robin to the 8 XCDs;
+// XCD x sweeps the logical blocks [x * blocks_per_xcd, (x + 1) * blocks_per_xcd)
+// so its L2 serves a contiguous range of patches.
+template <int FS, bool UNIFORM>
+__global__ __launch_bounds__(ROW_THREADS) void k_series_batch(SView M, int64_t n, const double *d, const double *area,
+                                                              const double *eps, const double *w, double *w_next,
+                                                              double *absorbed, double *flight, double *part_abs,
+                                                              uint32_t blocks_per_xcd, const SeriesSlots *S)
+{
+    constexpr int HL = FS / 2;   // lanes per column
+    constexpr int NG = 64 / HL;  // nonzeros in flight per wave
+    constexpr int U = 64 / NG;   // gathers per lane per 64 nonzeros
+    __shared__ double sa[ROWS_PER_BLOCK][FS];
+    const int wave = threadIdx.x >> 6;
+    const uint32_t b = blockIdx.x;
+    const int64_t first = ((int64_t)(b % 8u) * blocks_per_xcd + (b / 8u)) * ROWS_PER_BLOCK;
+    const int64_t row = (int64_t)__builtin_amdgcn_readfirstlane((int)(first + wave));
+    const int lane = threadIdx.x & 63;
+    const int h = lane % HL;  // slots 2h, 2h + 1
+    const int g = lane / HL;  // nonzero group
+    double acc0 = 0.0, acc1 = 0.0;
+    if (row < n) {
+        const int64_t e0 = M.rowptr[row], e1 = M.rowptr[row + 1];
+        const uint32_t len = (uint32_t)(e1 - e0);
+        const int32_t *col = M.col + e0;
+        for (uint32_t base = 0; base < len; base += 64u) {
+            const uint32_t e = base + (uint32_t)lane;
+            const bool ok = e < len;
+            const uint32_t ec = ok ? e : len - 1;
+            const int32_t jl = col[ec];
+            const double vl = ok ? (UNIFORM ? (double)M.kval[e0 + ec] : M.val[e0 + ec]) : 0.0;
+            int32_t j[U];
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                j[u] = __shfl(jl, g + NG * u);
+                v[u] = __shfl(vl, g + NG * u);
+            }
+            double2 z[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) z[u] = *reinterpret_cast<const double2 *>(w + (size_t)j[u] * FS + 2 * h);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                acc0 = fma(v[u], z[u].x, acc0);
+                acc1 = fma(v[u], z[u].y, acc1);
+            }
+        }
+    }
+    for (int off = HL; off < 64; off <<= 1) {
+        acc0 += __shfl_xor(acc0, off);
+        acc1 += __shfl_xor(acc1, off);
+    }
+    const int q = 2 * h + (g & 1);
+    double fl = 0.0;
+    if (g < 2 && row < n) {
+        const size_t at = (size_t)row * FS + q;
+        if (!S->done[q]) {
+            const double wsum = g == 0 ? acc0 : acc1;
+            const double sw = UNIFORM ? M.area[row] * (wsum * M.inv_2R + M.delta * w[at]) : wsum;
+            const double incident = d[row] * sw;
+            const double ep = eps[at];
+            fl = (1.0 - ep) * incident;
+            absorbed[at] = absorbed[at] + ep * incident;
+            flight[at] = fl;
+            w_next[at] = d[row] * (fl / area[row]);
+        } else {
+            w_next[at] = 0.0;  // a finished slot adds nothing to later gathers
+        }
+    }
+    if (g < 2) sa[wave][q] = fabs(fl);
+    __syncthreads();
+    if (threadIdx.x < FS) {
+        double a = 0.0;
+        for (int k = 0; k < ROWS_PER_BLOCK; ++k) a += sa[k][threadIdx.x];
+        part_abs[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = a;
+    }
+}
+
+// Per slot (one block each): ||flight||_1 from the block partials in a fixed
+// order; a slot that meets its tolerance is done after this reflection.
+__global__ __launch_bounds__(RED_THREADS) void k_series_batch_check(const double *part_abs, int64_t n_blocks,
+                                                                    double rtol, int32_t active, SeriesSlots *S)
+{
+    __shared__ double ra[RED_THREADS];
+    const int q = blockIdx.x;
+    if (q >= active || S->done[q]) return;
+    double a = 0.0;
+    for (int64_t b = threadIdx.x; b < n_blocks; b += RED_THREADS) a += part_abs[(size_t)q * n_blocks + b];
+    ra[threadIdx.x] = a;
+    __syncthreads();
+    for (int off = RED_THREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) ra[threadIdx.x] += ra[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        S->iterations[q] += 1;
+        if (ra[0] <= rtol * S->initial[q]) S->done[q] = 1;
+    }
+}
+
 SView sview_of(const prh_operator *op)
 {
     return SView{op->rowptr, op->col, op->val, op->kval, op->area, op->inv_2R, op->delta};
@@ -932,6 +1050,124 @@ int prh_operator_apply(prh_operator *op, int32_t mode, const double *x, double *
     return PRH_OK;
 }
 
+namespace {
+
+// The FP64 series of n_rhs emissions in groups of up to 32 slots; a group of
+// at most 16 emissions runs on the 16-slot layout.
+int radiosity_fp64_batched(prh_operator *op, int32_t n_rhs, const double *emitted, const double *emissivity,
+                           int32_t emissivity_per_rhs, double rtol, int32_t max_reflections, hipStream_t st,
+                           double *absorbed, double *residual, int32_t *iterations)
+{
+    const int64_t n = op->n;
+    if (n_rhs == 0) return PRH_OK;
+    const uint32_t bpx = (uint32_t)((blocks_for(n, ROWS_PER_BLOCK) + 7) / 8);
+    const unsigned rb = 8u * bpx;  // every XCD the same number of blocks
+    const size_t vec = (size_t)n * FSLOTS_MAX;
+    std::vector<double> d(n), area(n), host(vec), host_eps(vec), host_w(vec), host_fl(vec);
+    PRH_HIP(hipMemcpy(d.data(), op->scale, sizeof(double) * n, hipMemcpyDeviceToHost));
+    PRH_HIP(hipMemcpy(area.data(), op->area, sizeof(double) * n, hipMemcpyDeviceToHost));
+    double *buf = nullptr;
+    SeriesSlots *S = nullptr;
+    const size_t bytes = sizeof(double) * (5 * vec + (size_t)FSLOTS_MAX * rb);
+    if (hipMalloc((void **)&buf, bytes) != hipSuccess || hipMalloc((void **)&S, sizeof(SeriesSlots)) != hipSuccess) {
+        if (buf) (void)hipFree(buf);
+        (void)hipGetLastError();
+        set_error("prh_radiosity: out of device memory for the batched FP64 series");
+        return PRH_ERR_HIP;
+    }
+    struct Free {
+        double *b;
+        SeriesSlots *s;
+        ~Free()
+        {
+            (void)hipFree(b);
+            (void)hipFree(s);
+        }
+    } release{buf, S};
+    double *w0 = buf, *w1 = buf + vec, *abs_d = buf + 2 * vec, *fl_d = buf + 3 * vec, *eps_d = buf + 4 * vec;
+    double *part = buf + 5 * vec;
+    const SView M = sview_of(op);
+    for (int32_t q0 = 0; q0 < n_rhs; q0 += FSLOTS_MAX) {
+        const int active = std::min<int32_t>(FSLOTS_MAX, n_rhs - q0);
+        const int FS = active <= 16 ? 16 : 32;
+        const size_t v = (size_t)n * FS;
+        SeriesSlots hs{};
+        std::fill(host.begin(), host.begin() + v, 0.0);
+        std::fill(host_eps.begin(), host_eps.begin() + v, 0.0);
+        std::fill(host_w.begin(), host_w.begin() + v, 0.0);
+        for (int q = 0; q < FSLOTS_MAX; ++q) {
+            hs.done[q] = q < active ? 0 : 1;
+            if (q >= active) continue;
+            const double *E = emitted + (int64_t)(q0 + q) * n;
+            const double *ep = emissivity + (emissivity_per_rhs ? (int64_t)(q0 + q) * n : 0);
+            double l1 = 0.0;
+            for (int64_t i = 0; i < n; ++i) {
+                host[(size_t)i * FS + q] = E[i];
+                host_eps[(size_t)i * FS + q] = ep[i];
+                host_w[(size_t)i * FS + q] = d[i] * (E[i] / area[i]);
+                l1 += fabs(E[i]);
+            }
+            hs.initial[q] = l1 > 2.2250738585072014e-308 ? l1 : 2.2250738585072014e-308;
+        }
+        PRH_HIP(hipMemcpyAsync(fl_d, host.data(), sizeof(double) * v, hipMemcpyHostToDevice, st));
+        PRH_HIP(hipMemcpyAsync(eps_d, host_eps.data(), sizeof(double) * v, hipMemcpyHostToDevice, st));
+        PRH_HIP(hipMemcpyAsync(w0, host_w.data(), sizeof(double) * v, hipMemcpyHostToDevice, st));
+        PRH_HIP(hipMemsetAsync(abs_d, 0, sizeof(double) * v, st));
+        PRH_HIP(hipMemcpyAsync(S, &hs, sizeof(hs), hipMemcpyHostToDevice, st));
+        if (!op->series_ev[0]) {
+            PRH_HIP(hipEventCreate(&op->series_ev[0]));
+            PRH_HIP(hipEventCreate(&op->series_ev[1]));
+        }
+        PRH_HIP(hipEventRecord(op->series_ev[0], st));
+        double *w_in = w0, *w_out = w1;
+        int32_t ticks = 0;
+        bool all_done = false;
+        while (!all_done && ticks < max_reflections) {
+            const int batch = std::min<int32_t>(8, max_reflections - ticks);
+            for (int b = 0; b < batch; ++b) {
+                auto step = FS == 16 ? (op->uniform ? k_series_batch<16, true> : k_series_batch<16, false>)
+                                     : (op->uniform ? k_series_batch<32, true> : k_series_batch<32, false>);
+                step<<<rb, ROW_THREADS, 0, st>>>(M, n, op->scale, op->area, eps_d, w_in, w_out, abs_d, fl_d, part,
+                                                 bpx, S);
+                k_series_batch_check<<<FS, RED_THREADS, 0, st>>>(part, (int64_t)rb, rtol, active, S);
+                std::swap(w_in, w_out);
+            }
+            PRH_HIP(hipGetLastError());
+            ticks += batch;
+            PRH_HIP(hipEventRecord(op->series_ev[1], st));
+            PRH_HIP(hipMemcpyAsync(&hs, S, sizeof(hs), hipMemcpyDeviceToHost, st));
+            PRH_HIP(hipStreamSynchronize(st));
+            all_done = true;
+            for (int q = 0; q < active; ++q) all_done = all_done && hs.done[q];
+        }
+        float ms = 0.f;
+        PRH_HIP(hipEventElapsedTime(&ms, op->series_ev[0], op->series_ev[1]));
+        op->batched_ms += ms;
+        op->batched_passes += ticks;
+        op->batched_slot_passes += (int64_t)ticks * FS;
+        if (!all_done) {
+            set_error("radiosity reflection series did not converge");
+            return PRH_ERR_SERIES;
+        }
+        PRH_HIP(hipMemcpyAsync(host.data(), abs_d, sizeof(double) * v, hipMemcpyDeviceToHost, st));
+        PRH_HIP(hipMemcpyAsync(host_fl.data(), fl_d, sizeof(double) * v, hipMemcpyDeviceToHost, st));
+        PRH_HIP(hipStreamSynchronize(st));
+        for (int q = 0; q < active; ++q) {
+            double *out = absorbed + (int64_t)(q0 + q) * n;
+            double sum = 0.0;
+            for (int64_t i = 0; i < n; ++i) {
+                out[i] = host[(size_t)i * FS + q];
+                sum += host_fl[(size_t)i * FS + q];
+            }
+            residual[q0 + q] = sum;
+            iterations[q0 + q] = hs.iterations[q];
+        }
+    }
+    return PRH_OK;
+}
+
+}  // namespace
+
 int prh_radiosity(prh_operator *op, int32_t n_rhs, const double *emitted, const double *emissivity,
                   int32_t emissivity_per_rhs, double relative_tolerance, int32_t max_reflections,
                   int32_t precision, double *absorbed, double *residual, int32_t *iterations)
@@ -939,8 +1175,8 @@ int prh_radiosity(prh_operator *op, int32_t n_rhs, const double *emitted, const 
     PRH_CHECK_ARG(op && emitted && emissivity && absorbed && residual && iterations,
                   "prh_radiosity: null pointer");
     PRH_CHECK_ARG(n_rhs >= 0 && max_reflections >= 1, "prh_radiosity: bad sizes");
-    PRH_CHECK_ARG(precision == PRH_FP64 || precision == PRH_MIXED, "prh_radiosity: unsupported precision %d",
-                  precision);
+    PRH_CHECK_ARG(precision == PRH_FP64 || precision == PRH_MIXED || precision == PRH_FP64_BATCHED,
+                  "prh_radiosity: unsupported precision %d", precision);
     PRH_HIP(hipSetDevice(op->device));
     if (precision == PRH_MIXED)
         return prh_radiosity_mixed(op, n_rhs, emitted, emissivity, emissivity_per_rhs, relative_tolerance, absorbed,
@@ -948,6 +1184,9 @@ int prh_radiosity(prh_operator *op, int32_t n_rhs, const double *emitted, const 
     hipStream_t st;
     int rc = current_stream(&st);
     if (rc) return rc;
+    if (precision == PRH_FP64_BATCHED)
+        return radiosity_fp64_batched(op, n_rhs, emitted, emissivity, emissivity_per_rhs, relative_tolerance,
+                                      max_reflections, st, absorbed, residual, iterations);
     const int64_t n = op->n;
     for (int32_t q = 0; q < n_rhs; ++q) {
         PRH_HIP(hipMemcpyAsync(op->emit, emitted + (int64_t)q * n, sizeof(double) * n, hipMemcpyHostToDevice, st));
@@ -969,8 +1208,9 @@ int prh_radiosity(prh_operator *op, int32_t n_rhs, const double *emitted, const 
 int prh_operator_stats(const prh_operator *op, double *out, int32_t n)
 {
     PRH_CHECK_ARG(op && out && n >= 0, "prh_operator_stats: null pointer");
-    const double v[] = {(double)op->series_count, (double)op->series_steps, op->series_ms, (double)op->nnz_s,
-                        (double)op->n,            op->uniform ? 1.0 : 0.0};
+    const double v[] = {(double)op->series_count,  (double)op->series_steps,        op->series_ms,
+                        (double)op->nnz_s,         (double)op->n,                   op->uniform ? 1.0 : 0.0,
+                        (double)op->batched_passes, (double)op->batched_slot_passes, op->batched_ms};
     for (int i = 0; i < n && i < (int)(sizeof(v) / sizeof(v[0])); ++i) out[i] = v[i];
     return PRH_OK;
 }

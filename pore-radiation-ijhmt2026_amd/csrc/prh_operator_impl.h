@@ -110,6 +110,9 @@ struct prh_operator {
     hipEvent_t series_ev[2] = {nullptr, nullptr};
     int64_t series_count = 0, series_steps = 0;
     double series_ms = 0.0;
+    // device time of the batched FP64 series: passes, slot-weighted passes (sum of FS), ms
+    int64_t batched_passes = 0, batched_slot_passes = 0;
+    double batched_ms = 0.0;
 };
 
 extern "C" int prh_radiosity_mixed(prh_operator *op, int32_t n_rhs, const double *emitted, const double *emissivity,

@@ -30,7 +30,7 @@ PRH_ERR_CAPACITY = 6
 PRH_ERR_BALANCE = 7
 
 APPLY_SYMMETRIC, APPLY_MATVEC, APPLY_RMATVEC, APPLY_RAW, APPLY_RAW_T = 0, 1, 2, 3, 4
-FP64, MIXED = 0, 1
+FP64, MIXED, FP64_BATCHED = 0, 1, 2
 BATCH_SLOTS = 32
 BATCH_DIRECT = 1
 
@@ -648,10 +648,11 @@ class Operator:
     def stats(self) -> dict:
         """Device time of the FP64 reflection series run on this operator."""
 
-        out = np.zeros(6)
-        check(self._lib.prh_operator_stats(self.handle, ptr(out), 6), "prh_operator_stats")
+        out = np.zeros(9)
+        check(self._lib.prh_operator_stats(self.handle, ptr(out), 9), "prh_operator_stats")
         return {"series": int(out[0]), "reflections": int(out[1]), "ms": float(out[2]), "nnz": int(out[3]),
-                "n": int(out[4]), "integer_form": bool(out[5])}
+                "n": int(out[4]), "integer_form": bool(out[5]), "batched_passes": int(out[6]),
+                "batched_slot_passes": int(out[7]), "batched_ms": float(out[8])}
 
     def balance(self, tolerance: float) -> tuple[int, int]:
         """Balance on the device (method 0 none, 1 Anderson, 2 multiplicative; iterations)."""

@@ -260,7 +260,10 @@ def reflection_series_batch(
 
     ``emissivity`` is one vector for all rows or one row per emission.
     ``precision="fp64"`` sums the reflection series in FP64 exactly as
-    ``reflection_series``; ``"mixed"`` solves the same radiosity balance with
+    ``reflection_series``, one emission at a time; ``"fp64_batched"`` sums
+    the same FP64 series for up to 32 emissions per pass over the operator
+    (each stops on its own tolerance; sums in another order, so equal to
+    rounding); ``"mixed"`` solves the same radiosity balance with
     batched FP32 Chebyshev sweeps and FP64 residual refinement until
     ||E - (I - (1-eps) F_p^T) J||_1 <= relative_tolerance ||E||_1 (the
     iteration count is then the number of sweeps).
@@ -270,7 +273,7 @@ def reflection_series_batch(
     n = operator.n_patches
     eps = np.asarray(emissivity, dtype=np.float64)
     eps = np.ascontiguousarray(np.broadcast_to(eps, E.shape) if eps.ndim == 2 else _emissivity_vector(eps, n))
-    mode = {"fp64": _native.FP64, "mixed": _native.MIXED}[precision]
+    mode = {"fp64": _native.FP64, "mixed": _native.MIXED, "fp64_batched": _native.FP64_BATCHED}[precision]
     absorbed, residual, iterations = operator.device.radiosity(E, eps, relative_tolerance, max_reflections, mode)
     return [RadiosityResult(E[q], absorbed[q], absorbed[q] - E[q], float(residual[q]), int(iterations[q]))
             for q in range(E.shape[0])]

@@ -188,3 +188,30 @@ def test_admission_errors(mods):
             batch.release(0)
     finally:
         batch.close()
+
+
+@pytest.mark.parametrize("nonuniform", [False, True])
+@pytest.mark.parametrize("n_rhs", [5, 20, 40])
+def test_fp64_batched_series_matches_oracle(mods, n_rhs, nonuniform):
+    """The FP64 series of 5 (16-slot layout), 20 (32-slot) and 40 (32 + 8)
+    emissions in batched passes: per emission within 1e-12 of the oracle's
+    series, the same reflection count as the one-at-a-time device series
+    (within one, the stopping sums differ in order), energy residual small."""
+
+    _, _, ex, _ = mods
+    Q, op, ref = _pair(nonuniform=nonuniform)
+    n = Q.n_patches
+    rng = np.random.default_rng(40 + n_rhs)
+    E = rng.normal(size=(n_rhs, n)) * 1e-3
+    E[1] = 0.0  # the zero emission stops after its first reflection
+    eps = rng.uniform(0.2, 0.95, (n_rhs, n))
+    got = ex.reflection_series_batch(op, E, eps, relative_tolerance=1e-12, precision="fp64_batched")
+    single = ex.reflection_series_batch(op, E, eps, relative_tolerance=1e-12, precision="fp64")
+    for q in range(n_rhs):
+        absorbed, _, iters = X.reflection_series(ref, E[q], eps[q], relative_tolerance=1e-12)
+        scale = max(np.max(np.abs(absorbed)), 1e-300)
+        assert np.max(np.abs(got[q].absorbed_power_W - absorbed)) <= 1e-12 * scale, q
+        assert abs(got[q].iterations - single[q].iterations) <= 1, q
+        assert abs(got[q].iterations - iters) <= 1, q
+        assert abs(got[q].global_energy_residual_W) <= 1e-11 * max(np.sum(np.abs(E[q])), 1e-300), q
+    assert got[1].iterations == 1 and not np.any(got[1].absorbed_power_W)
