@@ -703,6 +703,42 @@ def cpu_radiosity_baseline(P, op, T_hot=700.0, eps=0.6, rtol=1e-8):
                       "threads, 1 warm-up + median of 3"}
 
 
+def run_sweep_probes(R, reps=20) -> dict:
+    """What one batched sweep of the foam operator costs per slot format, and
+    what dense MFMA tiles would take off it.
+
+    Formats: FP32 x 32 slots (the engine's), FP16 x 64 and FP64 x 16 (the
+    same 128 B per gathered column), FP16 x 32 (64 B) and FP64 x 32 (256 B);
+    the slot-sweep rate is slots / ms.  Tiles: 16 x 4 blocks of K holding at
+    least 4, 8 or 16 nonzeros multiplied by v_mfma_f32_16x16x4f32, the rest
+    gathered; the plain gather's time against tiles + remainder."""
+
+    from porerad import _native, view_factors
+
+    P = view_factors.build_surface_patches(foam_workload(), 0.01)
+    op = build_operator(P, R, 3)
+    slots = {"fp32x32": 32, "fp16x64": 64, "fp16x32": 32, "fp64x16": 16, "fp64x32": 32}
+    formats = {}
+    for name, fmt in _native.PROBE_FORMATS.items():
+        ms = op.device.probe_gather(fmt, reps)
+        formats[name] = {"ms_per_sweep": ms, "slot_sweeps_per_ms": slots[name] / ms}
+        log(f"sweep {name}: {ms:.3f} ms, {slots[name] / ms:.1f} slot-sweeps/ms")
+    base = formats["fp32x32"]["slot_sweeps_per_ms"]
+    for row in formats.values():
+        row["vs_fp32x32"] = row["slot_sweeps_per_ms"] / base
+    tiles = {}
+    for fill in (4, 8, 16):
+        t = op.device.probe_tiles(fill, reps)
+        t["tiles_plus_remainder_ms"] = t["tiles_ms"] + t["remainder_ms"]
+        t["speedup_vs_plain"] = t["plain_ms"] / t["tiles_plus_remainder_ms"]
+        tiles[f"min_fill_{fill}"] = t
+        log(f"tiles >= {fill}: {t['tiles']} tiles hold {t['nnz_fraction_in_tiles']:.3f} of nnz; plain "
+            f"{t['plain_ms']:.3f} ms vs tiles {t['tiles_ms']:.3f} + remainder {t['remainder_ms']:.3f} ms")
+    op.device.close()
+    return {"workload": f"foam N=128, {P.n_patches} patches, {R} rays/patch (seed 3)", "reps": reps,
+            "formats": formats, "mfma_tiles": tiles}
+
+
 def run_monte_carlo(phases, with_cpu: bool) -> dict:
     """Multi-bounce Monte Carlo statistics of the foam (properties.py / penetration.py):
     identify_properties (2 seeds x 200 000 beam rays) and penetration_response
@@ -810,6 +846,8 @@ def main() -> None:
     ap.add_argument("--microstructure", action="store_true",
                     help="only the microstructure leg (generators and descriptors)")
     ap.add_argument("--monte-carlo", action="store_true", help="only the Monte Carlo leg")
+    ap.add_argument("--sweep-probes", action="store_true",
+                    help="only the sweep probes: slot formats and MFMA tiles on the foam operator")
     ap.add_argument("--batching", action="store_true",
                     help="the 16-geometry batched trace pass (+ 64 radiosity RHS per geometry) instead")
     args = ap.parse_args()
@@ -832,6 +870,9 @@ def main() -> None:
         return
     if args.monte_carlo:
         print(json.dumps(run_monte_carlo(foam_workload(), not args.no_cpu)), flush=True)
+        return
+    if args.sweep_probes:
+        print(json.dumps(run_sweep_probes(args.rays)), flush=True)
         return
     t = time.perf_counter()
     phases = foam_workload()

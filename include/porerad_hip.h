@@ -404,6 +404,29 @@ int prh_operator_apply(prh_operator *op, int32_t mode, const double *x, double *
  * slot width summed over passes) and milliseconds.  No counterpart in the
  * reference (measurement). */
 int prh_operator_stats(const prh_operator *op, double *out, int32_t n);
+/* Measurement probes of the batched radiosity sweep (no counterpart in the
+ * reference; they solve nothing).  Both need the integer form K (equal areas
+ * and ray counts).
+ *
+ * prh_probe_gather: device milliseconds of one sweep z' = F_p^T z over every
+ * slot of an interleaved slot vector in one of the PRH_PROBE_* formats (the
+ * batched engine's tick without its per-slot update), averaged over reps
+ * sweeps after one warm-up sweep. */
+#define PRH_PROBE_F32X32 0 /* 32 FP32 slots per patch: 128 B per gathered column */
+#define PRH_PROBE_F16X64 1 /* 64 FP16 slots: 128 B */
+#define PRH_PROBE_F16X32 2 /* 32 FP16 slots: 64 B */
+#define PRH_PROBE_F64X16 3 /* 16 FP64 slots: 128 B */
+#define PRH_PROBE_F64X32 4 /* 32 FP64 slots: 256 B */
+int prh_probe_gather(const prh_operator *op, int32_t format, int32_t reps, double *ms_per_sweep);
+/* prh_probe_tiles: the 16 x 4 blocks of K holding >= min_fill nonzeros as
+ * dense tiles multiplied by v_mfma_f32_16x16x4f32 into 32 FP32 slots, the
+ * rest as a CSR gathered like the engine's tick.  out[0..n_out) = ms of the
+ * plain gather of all of K, ms of the tiles, ms of the remainder gather
+ * (each averaged over reps), the largest difference between the plain
+ * product and tiles + remainder relative to the largest product, tiles,
+ * fraction of the nonzeros in tiles, remainder nonzeros. */
+int prh_probe_tiles(const prh_operator *op, int32_t min_fill, int32_t reps, double *out, int32_t n_out);
+
 /* Reflection series J = E + (1 - eps) F_p^T J for n_rhs emissions.
  * emitted: n_rhs x n (row-major, one emission per row); emissivity:
  * n_rhs x n if emissivity_per_rhs else n.  Outputs absorbed (n_rhs x n),

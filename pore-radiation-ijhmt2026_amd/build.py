@@ -37,6 +37,7 @@ SOURCES = [
     "prh_microstructure.hip",
     "prh_batch.hip",
     "prh_coupled.hip",
+    "prh_probe.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -1,0 +1,454 @@
+// Measurement probes of the batched radiosity sweep (not part of any solve).
+//
+// prh_probe_gather  times one sweep y = S z over every slot of an interleaved
+//                   slot vector, for several slot formats: what one tick of a
+//                   batched engine costs with FP32 x 32, FP16 x 64, FP16 x 32,
+//                   FP64 x 16 or FP64 x 32 slots per patch.  The sweep has
+//                   the structure of the engine's tick (one wave per row, 16 B
+//                   gathers, XCD-contiguous rows, FP64 accumulation) and writes
+//                   the scaled product back in the slot format.
+// prh_probe_tiles   the matrix-core alternative: the 16 x 4 blocks of S that
+//                   hold at least min_fill nonzeros become dense tiles,
+//                   multiplied into FP32 x 32 slot vectors with
+//                   v_mfma_f32_16x16x4f32, and the rest stays a CSR gathered
+//                   as before; times both parts against the plain gather and
+//                   checks that they add up to the same product.
+//
+// Both take a device-resident operator in the integer form K (equal areas and
+// ray counts).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "prh_internal.h"
+#include "prh_operator_impl.h"
+
+namespace prh {
+namespace {
+
+constexpr int PT = 256;  // 4 rows (one per wave) per workgroup
+constexpr int PROWS = PT / 64;
+
+template <typename T, int PER>
+struct alignas(16) Pack {
+    T v[PER];
+};
+
+__device__ __forceinline__ double to_f64(float x) { return (double)x; }
+__device__ __forceinline__ double to_f64(double x) { return x; }
+__device__ __forceinline__ double to_f64(_Float16 x) { return (double)(float)x; }
+template <typename T>
+__device__ __forceinline__ T from_f64(double x)
+{
+    return (T)x;
+}
+template <>
+__device__ __forceinline__ _Float16 from_f64<_Float16>(double x)
+{
+    return (_Float16)(float)x;
+}
+
+struct ProbeView {
+    int64_t n;
+    const int64_t *rowptr;
+    const int32_t *col;
+    const uint16_t *kval;
+    const double *d;
+    double inv_2R, delta;
+    uint32_t blocks_per_xcd;
+};
+
+__device__ __forceinline__ int64_t probe_row(uint32_t blocks_per_xcd)
+{
+    const uint32_t b = blockIdx.x;
+    const int64_t first = ((int64_t)(b % 8u) * blocks_per_xcd + (b / 8u)) * PROWS;
+    return (int64_t)__builtin_amdgcn_readfirstlane((int)(first + (threadIdx.x >> 6)));
+}
+
+// z_out[row][s] = (F_p^T z_in)[row][s] in the slot format T (bounded: F_p^T
+// has spectral radius 1), as the engine's tick forms it.
+template <typename T, int SL>
+__global__ __launch_bounds__(PT) void k_probe_gather(ProbeView M, const T *z_in, T *z_out)
+{
+    constexpr int PER = 16 / (int)sizeof(T);  // slots per 16 B load
+    constexpr int HL = SL / PER;               // lanes per column
+    constexpr int NG = 64 / HL;                // nonzeros in flight per wave
+    constexpr int U = 64 / NG;                 // gathers per lane per 64 nonzeros
+    static_assert(NG >= PER, "every slot needs a writing lane");
+    const int64_t row = probe_row(M.blocks_per_xcd);
+    const int lane = threadIdx.x & 63;
+    const int h = lane % HL;
+    const int g = lane / HL;
+    double acc[PER];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) acc[p] = 0.0;
+    if (row < M.n) {
+        const int64_t e0 = M.rowptr[row];
+        const uint32_t len = (uint32_t)(M.rowptr[row + 1] - e0);
+        const int32_t *col = M.col + e0;
+        const uint16_t *kv = M.kval + e0;
+        for (uint32_t base = 0; base < len; base += 64u) {
+            const uint32_t e = base + (uint32_t)lane;
+            const bool ok = e < len;
+            const uint32_t ec = ok ? e : len - 1;
+            const int32_t jl = col[ec];
+            const double vl = ok ? (double)kv[ec] : 0.0;
+            int32_t j[U];
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                j[u] = __shfl(jl, g + NG * u);
+                v[u] = __shfl(vl, g + NG * u);
+            }
+            Pack<T, PER> z[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t off = (uint32_t)j[u] * (uint32_t)(SL * sizeof(T)) + 16u * (uint32_t)h;
+                z[u] = *reinterpret_cast<const Pack<T, PER> *>(reinterpret_cast<const char *>(z_in) + off);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int p = 0; p < PER; ++p) acc[p] = fma(v[u], to_f64(z[u].v[p]), acc[p]);
+        }
+    }
+#pragma unroll
+    for (int off = HL; off < 64; off <<= 1)
+#pragma unroll
+        for (int p = 0; p < PER; ++p) acc[p] += __shfl_xor(acc[p], off);
+    if (g < PER && row < M.n) {
+        double w = acc[0];
+#pragma unroll
+        for (int p = 1; p < PER; ++p) w = g == p ? acc[p] : w;
+        const size_t at = (size_t)row * SL + (size_t)(PER * h + g);
+        const double di = M.d[row];
+        const double y = di * (w * M.inv_2R + M.delta * to_f64(z_in[at]));
+        z_out[at] = from_f64<T>(y);
+    }
+}
+
+// Values of order one in every slot.
+template <typename T>
+__global__ void k_probe_fill(T *z, size_t count)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) z[i] = from_f64<T>(0.25 + (double)((uint32_t)(i * 2654435761u) >> 8) * 0x1p-25);
+}
+
+// ------------------------------------------------------------------ tiles
+constexpr int TR = 16;  // tile rows
+constexpr int TC = 4;   // tile columns
+constexpr int TS = 32;  // slots (FP32)
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+struct TileView {
+    int64_t n, n_row_blocks;
+    const int64_t *tile_ptr;  // [n_row_blocks + 1]
+    const int32_t *tile_col;  // first column / TC of each tile
+    const uint16_t *tile_val; // [tile][k][i], k < TC, i < TR
+    uint32_t blocks_per_xcd;
+};
+
+// y32[row][s] = sum over the row block's tiles of K_tile z (FP32 MFMA
+// accumulation).  One wave per row block of 16 rows.  With
+// v_mfma_f32_16x16x4f32 lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]
+// and holds D[4 (l / 16) + r][l % 16], r < 4.
+__global__ __launch_bounds__(PT) void k_probe_tiles(TileView T, const float *z, float *y32)
+{
+    const int64_t rb = probe_row(T.blocks_per_xcd);
+    const int lane = threadIdx.x & 63;
+    const int i = lane & 15;
+    const int k = lane >> 4;
+    floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    if (rb >= T.n_row_blocks) return;
+    const int64_t t0 = T.tile_ptr[rb], t1 = T.tile_ptr[rb + 1];
+    for (int64_t t = t0; t < t1; ++t) {
+        const float a = (float)T.tile_val[(size_t)t * (TR * TC) + (size_t)(k * TR + i)];
+        const int64_t c = (int64_t)T.tile_col[t] * TC + k;
+        float b0 = 0.f, b1 = 0.f;
+        if (c < T.n) {
+            b0 = z[(size_t)c * TS + i];
+            b1 = z[(size_t)c * TS + 16 + i];
+        }
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc1, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int64_t row = rb * TR + 4 * k + r;
+        if (row < T.n) {
+            y32[(size_t)row * TS + i] = acc0[r];
+            y32[(size_t)row * TS + 16 + i] = acc1[r];
+        }
+    }
+}
+
+// y64[row][s] = (K z)[row][s] over a CSR with uint16 values (FP64 accumulation),
+// the engine's gather: 8 lanes x 16 B per column, 8 gathers in flight per lane.
+__global__ __launch_bounds__(PT) void k_probe_csr(ProbeView M, const float *z, double *y64)
+{
+    const int64_t row = probe_row(M.blocks_per_xcd);
+    const int lane = threadIdx.x & 63;
+    const int h = lane & 7;
+    const int g = lane >> 3;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (row < M.n) {
+        const int64_t e0 = M.rowptr[row];
+        const uint32_t len = (uint32_t)(M.rowptr[row + 1] - e0);
+        for (uint32_t base = 0; base < len; base += 64u) {
+            const uint32_t e = base + (uint32_t)lane;
+            const bool ok = e < len;
+            const uint32_t ec = ok ? e : len - 1;
+            const int32_t jl = M.col[e0 + ec];
+            const double vl = ok ? (double)M.kval[e0 + ec] : 0.0;
+            int32_t j[8];
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                j[u] = __shfl(jl, g + 8 * u);
+                v[u] = __shfl(vl, g + 8 * u);
+            }
+            float4 zz[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                zz[u] = *reinterpret_cast<const float4 *>(z + (size_t)j[u] * TS + 4 * h);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                acc[0] = fma(v[u], (double)zz[u].x, acc[0]);
+                acc[1] = fma(v[u], (double)zz[u].y, acc[1]);
+                acc[2] = fma(v[u], (double)zz[u].z, acc[2]);
+                acc[3] = fma(v[u], (double)zz[u].w, acc[3]);
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 8; off < 64; off <<= 1)
+#pragma unroll
+        for (int p = 0; p < 4; ++p) acc[p] += __shfl_xor(acc[p], off);
+    if (g < 4 && row < M.n) {
+        const double w = g == 0 ? acc[0] : g == 1 ? acc[1] : g == 2 ? acc[2] : acc[3];
+        y64[(size_t)row * TS + 4 * h + g] = w;
+    }
+}
+
+uint32_t xcd_blocks(int64_t units) { return (uint32_t)(((units + PROWS - 1) / PROWS + 7) / 8); }
+
+struct DeviceBuffers {
+    std::vector<void *> p;
+    ~DeviceBuffers()
+    {
+        for (void *q : p) (void)hipFree(q);
+    }
+    template <typename X>
+    int get(X **out, size_t count)
+    {
+        void *q = nullptr;
+        if (hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(X)) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("probe: out of device memory (%zu bytes)", count * sizeof(X));
+            return PRH_ERR_HIP;
+        }
+        p.push_back(q);
+        *out = static_cast<X *>(q);
+        return PRH_OK;
+    }
+};
+
+template <typename T, int SL>
+int time_gather(const prh_operator *op, int32_t reps, hipStream_t st, double *ms_per_sweep)
+{
+    PRH_CHECK_ARG((uint64_t)op->n * SL * sizeof(T) < ((uint64_t)1 << 32),
+                  "prh_probe_gather: slot vectors beyond 4 GiB");
+    DeviceBuffers buf;
+    T *z0 = nullptr, *z1 = nullptr;
+    const size_t count = (size_t)op->n * SL;
+    int rc = buf.get(&z0, count);
+    if (!rc) rc = buf.get(&z1, count);
+    if (rc) return rc;
+    k_probe_fill<T><<<(unsigned)((count + 255) / 256), 256, 0, st>>>(z0, count);
+    const uint32_t bpx = xcd_blocks(op->n);
+    const ProbeView M{op->n, op->rowptr, op->col, op->kval, op->scale, op->inv_2R, op->delta, bpx};
+    k_probe_gather<T, SL><<<8u * bpx, PT, 0, st>>>(M, z0, z1);  // warm-up
+    hipEvent_t ev[2];
+    PRH_HIP(hipEventCreate(&ev[0]));
+    PRH_HIP(hipEventCreate(&ev[1]));
+    PRH_HIP(hipEventRecord(ev[0], st));
+    for (int r = 0; r < reps; ++r) {
+        k_probe_gather<T, SL><<<8u * bpx, PT, 0, st>>>(M, z0, z1);
+        std::swap(z0, z1);
+    }
+    PRH_HIP(hipEventRecord(ev[1], st));
+    PRH_HIP(hipGetLastError());
+    PRH_HIP(hipEventSynchronize(ev[1]));
+    float ms = 0.f;
+    PRH_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    *ms_per_sweep = ms / reps;
+    return PRH_OK;
+}
+
+}  // namespace
+}  // namespace prh
+
+using namespace prh;
+
+extern "C" {
+
+int prh_probe_gather(const prh_operator *op, int32_t format, int32_t reps, double *ms_per_sweep)
+{
+    PRH_CHECK_ARG(op && ms_per_sweep, "prh_probe_gather: null pointer");
+    PRH_CHECK_ARG(op->uniform && op->kval, "prh_probe_gather: the operator has no integer form K");
+    PRH_CHECK_ARG(reps >= 1 && reps <= 10000, "prh_probe_gather: reps must be in [1, 10000]");
+    PRH_HIP(hipSetDevice(op->device));
+    hipStream_t st;
+    int rc = current_stream(&st);
+    if (rc) return rc;
+    switch (format) {
+    case PRH_PROBE_F32X32: return time_gather<float, 32>(op, reps, st, ms_per_sweep);
+    case PRH_PROBE_F16X64: return time_gather<_Float16, 64>(op, reps, st, ms_per_sweep);
+    case PRH_PROBE_F16X32: return time_gather<_Float16, 32>(op, reps, st, ms_per_sweep);
+    case PRH_PROBE_F64X16: return time_gather<double, 16>(op, reps, st, ms_per_sweep);
+    case PRH_PROBE_F64X32: return time_gather<double, 32>(op, reps, st, ms_per_sweep);
+    default: break;
+    }
+    set_error("prh_probe_gather: unknown slot format %d", format);
+    return PRH_ERR_ARG;
+}
+
+int prh_probe_tiles(const prh_operator *op, int32_t min_fill, int32_t reps, double *out, int32_t n_out)
+{
+    PRH_CHECK_ARG(op && out, "prh_probe_tiles: null pointer");
+    PRH_CHECK_ARG(op->uniform && op->kval, "prh_probe_tiles: the operator has no integer form K");
+    PRH_CHECK_ARG(min_fill >= 1 && min_fill <= TR * TC, "prh_probe_tiles: min_fill must be in [1, 64]");
+    PRH_CHECK_ARG(reps >= 1 && reps <= 10000, "prh_probe_tiles: reps must be in [1, 10000]");
+    PRH_HIP(hipSetDevice(op->device));
+    hipStream_t st;
+    int rc = current_stream(&st);
+    if (rc) return rc;
+    const int64_t n = op->n, nnz = op->nnz_s;
+    PRH_CHECK_ARG(n < ((int64_t)1 << 25) && nnz < ((int64_t)1 << 32), "prh_probe_tiles: operator too large");
+    std::vector<int64_t> rowptr(n + 1);
+    std::vector<int32_t> col(nnz);
+    std::vector<uint16_t> kval(nnz);
+    PRH_HIP(hipMemcpy(rowptr.data(), op->rowptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
+    PRH_HIP(hipMemcpy(col.data(), op->col, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+    PRH_HIP(hipMemcpy(kval.data(), op->kval, sizeof(uint16_t) * nnz, hipMemcpyDeviceToHost));
+
+    // split S into dense 16 x 4 tiles (blocks with >= min_fill nonzeros) and a remainder CSR
+    const int64_t n_rb = (n + TR - 1) / TR;
+    std::vector<int64_t> tile_ptr(n_rb + 1, 0), rem_ptr(n + 1, 0);
+    std::vector<int32_t> tile_col, rem_col;
+    std::vector<uint16_t> tile_val, rem_val;
+    rem_col.reserve(nnz);
+    rem_val.reserve(nnz);
+    std::vector<uint64_t> keys;
+    std::vector<std::vector<std::pair<int32_t, uint16_t>>> rem_rows(TR);
+    for (int64_t rb = 0; rb < n_rb; ++rb) {
+        const int64_t r0 = rb * TR, r1 = std::min<int64_t>(r0 + TR, n);
+        keys.clear();
+        for (int64_t r = r0; r < r1; ++r)
+            for (int64_t e = rowptr[r]; e < rowptr[r + 1]; ++e)
+                keys.push_back(((uint64_t)(uint32_t)col[e] << 36) | ((uint64_t)(r - r0) << 32) | (uint64_t)e);
+        std::sort(keys.begin(), keys.end());
+        for (auto &v : rem_rows) v.clear();
+        size_t a = 0;
+        while (a < keys.size()) {
+            const uint64_t cb = (keys[a] >> 36) / TC;
+            size_t b = a;
+            while (b < keys.size() && (keys[b] >> 36) / TC == cb) ++b;
+            if ((int64_t)(b - a) >= min_fill) {
+                const size_t base = tile_val.size();
+                tile_val.resize(base + TR * TC, 0);
+                tile_col.push_back((int32_t)cb);
+                for (size_t q = a; q < b; ++q) {
+                    const int kk = (int)((keys[q] >> 36) % TC);
+                    const int ii = (int)((keys[q] >> 32) & 0xF);
+                    tile_val[base + kk * TR + ii] = kval[keys[q] & 0xFFFFFFFFull];
+                }
+            } else {
+                for (size_t q = a; q < b; ++q) {
+                    const int ii = (int)((keys[q] >> 32) & 0xF);
+                    const uint64_t e = keys[q] & 0xFFFFFFFFull;
+                    rem_rows[ii].push_back({col[e], kval[e]});
+                }
+            }
+            a = b;
+        }
+        tile_ptr[rb + 1] = (int64_t)tile_col.size();
+        for (int64_t r = r0; r < r1; ++r) {
+            for (const auto &cv : rem_rows[r - r0]) {
+                rem_col.push_back(cv.first);
+                rem_val.push_back(cv.second);
+            }
+            rem_ptr[r + 1] = (int64_t)rem_col.size();
+        }
+    }
+    const int64_t n_tiles = (int64_t)tile_col.size(), n_rem = (int64_t)rem_col.size();
+
+    DeviceBuffers buf;
+    int64_t *d_tile_ptr = nullptr, *d_rem_ptr = nullptr;
+    int32_t *d_tile_col = nullptr, *d_rem_col = nullptr;
+    uint16_t *d_tile_val = nullptr, *d_rem_val = nullptr;
+    float *z = nullptr, *y32 = nullptr;
+    double *y_full = nullptr, *y_rem = nullptr;
+    const size_t vec = (size_t)n * TS;
+    if ((rc = buf.get(&d_tile_ptr, n_rb + 1)) || (rc = buf.get(&d_rem_ptr, n + 1)) ||
+        (rc = buf.get(&d_tile_col, n_tiles)) || (rc = buf.get(&d_rem_col, n_rem)) ||
+        (rc = buf.get(&d_tile_val, (size_t)n_tiles * TR * TC)) || (rc = buf.get(&d_rem_val, n_rem)) ||
+        (rc = buf.get(&z, vec)) || (rc = buf.get(&y32, vec)) || (rc = buf.get(&y_full, vec)) ||
+        (rc = buf.get(&y_rem, vec)))
+        return rc;
+    PRH_HIP(hipMemcpy(d_tile_ptr, tile_ptr.data(), sizeof(int64_t) * (n_rb + 1), hipMemcpyHostToDevice));
+    PRH_HIP(hipMemcpy(d_rem_ptr, rem_ptr.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
+    if (n_tiles) {
+        PRH_HIP(hipMemcpy(d_tile_col, tile_col.data(), sizeof(int32_t) * n_tiles, hipMemcpyHostToDevice));
+        PRH_HIP(hipMemcpy(d_tile_val, tile_val.data(), sizeof(uint16_t) * tile_val.size(), hipMemcpyHostToDevice));
+    }
+    if (n_rem) {
+        PRH_HIP(hipMemcpy(d_rem_col, rem_col.data(), sizeof(int32_t) * n_rem, hipMemcpyHostToDevice));
+        PRH_HIP(hipMemcpy(d_rem_val, rem_val.data(), sizeof(uint16_t) * n_rem, hipMemcpyHostToDevice));
+    }
+    k_probe_fill<float><<<(unsigned)((vec + 255) / 256), 256, 0, st>>>(z, vec);
+
+    const uint32_t bpx = xcd_blocks(n), bpx_t = xcd_blocks(n_rb);
+    const ProbeView full{n, op->rowptr, op->col, op->kval, op->scale, op->inv_2R, op->delta, bpx};
+    const ProbeView rem{n, d_rem_ptr, d_rem_col, d_rem_val, op->scale, op->inv_2R, op->delta, bpx};
+    const TileView tiles{n, n_rb, d_tile_ptr, d_tile_col, d_tile_val, bpx_t};
+    hipEvent_t ev[4];
+    for (auto &e : ev) PRH_HIP(hipEventCreate(&e));
+    k_probe_csr<<<8u * bpx, PT, 0, st>>>(full, z, y_full);  // warm-up
+    PRH_HIP(hipEventRecord(ev[0], st));
+    for (int r = 0; r < reps; ++r) k_probe_csr<<<8u * bpx, PT, 0, st>>>(full, z, y_full);
+    PRH_HIP(hipEventRecord(ev[1], st));
+    for (int r = 0; r < reps; ++r) k_probe_tiles<<<8u * bpx_t, PT, 0, st>>>(tiles, z, y32);
+    PRH_HIP(hipEventRecord(ev[2], st));
+    for (int r = 0; r < reps; ++r) k_probe_csr<<<8u * bpx, PT, 0, st>>>(rem, z, y_rem);
+    PRH_HIP(hipEventRecord(ev[3], st));
+    PRH_HIP(hipGetLastError());
+    PRH_HIP(hipEventSynchronize(ev[3]));
+    float ms[3];
+    for (int k = 0; k < 3; ++k) PRH_HIP(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
+    for (auto &e : ev) (void)hipEventDestroy(e);
+
+    // the two parts add up to the plain product (FP32 tile accumulation)
+    std::vector<double> hf(vec), hr(vec);
+    std::vector<float> ht(vec);
+    PRH_HIP(hipMemcpy(hf.data(), y_full, sizeof(double) * vec, hipMemcpyDeviceToHost));
+    PRH_HIP(hipMemcpy(hr.data(), y_rem, sizeof(double) * vec, hipMemcpyDeviceToHost));
+    PRH_HIP(hipMemcpy(ht.data(), y32, sizeof(float) * vec, hipMemcpyDeviceToHost));
+    double worst = 0.0, largest = 0.0;
+    for (size_t q = 0; q < vec; ++q) {
+        worst = std::max(worst, std::fabs(hf[q] - (hr[q] + (double)ht[q])));
+        largest = std::max(largest, std::fabs(hf[q]));
+    }
+    const double v[] = {ms[0] / reps, ms[1] / reps, ms[2] / reps, largest > 0.0 ? worst / largest : worst,
+                        (double)n_tiles, (double)(nnz - n_rem) / (double)std::max<int64_t>(nnz, 1),
+                        (double)n_rem};
+    for (int k = 0; k < n_out && k < (int)(sizeof(v) / sizeof(v[0])); ++k) out[k] = v[k];
+    return PRH_OK;
+}
+
+}  // extern "C"

@@ -31,6 +31,8 @@ PRH_ERR_BALANCE = 7
 
 APPLY_SYMMETRIC, APPLY_MATVEC, APPLY_RMATVEC, APPLY_RAW, APPLY_RAW_T = 0, 1, 2, 3, 4
 FP64, MIXED, FP64_BATCHED = 0, 1, 2
+# slot formats of prh_probe_gather
+PROBE_FORMATS = {"fp32x32": 0, "fp16x64": 1, "fp16x32": 2, "fp64x16": 3, "fp64x32": 4}
 BATCH_SLOTS = 32
 BATCH_DIRECT = 1
 
@@ -94,6 +96,8 @@ SIGNATURES = {
     "prh_operator_csr": ([_P, _P, _P, _P], ctypes.c_int),
     "prh_operator_reciprocity": ([_P, _P], ctypes.c_int),
     "prh_operator_stats": ([_P, _P, _I32], ctypes.c_int),
+    "prh_probe_gather": ([_P, _I32, _I32, _P], ctypes.c_int),
+    "prh_probe_tiles": ([_P, _I32, _I32, _P, _I32], ctypes.c_int),
     "prh_batch_stats": ([_P, _P, _I32], ctypes.c_int),
     "prh_operator_apply": ([_P, _I32, _P, _P], ctypes.c_int),
     "prh_radiosity": ([_P, _I32, _P, _P, _I32, _F64, _I32, _I32, _P, _P, _P], ctypes.c_int),
@@ -610,6 +614,22 @@ class Operator:
         err = ctypes.c_double()
         check(self._lib.prh_operator_reciprocity(self.handle, ctypes.byref(err)), "prh_operator_reciprocity")
         return float(err.value)
+
+    def probe_gather(self, fmt: int, reps: int = 20) -> float:
+        """Device ms of one F_p^T sweep over every slot in slot format ``fmt`` (a measurement probe)."""
+
+        ms = ctypes.c_double()
+        check(self._lib.prh_probe_gather(self.handle, int(fmt), int(reps), ctypes.byref(ms)), "prh_probe_gather")
+        return float(ms.value)
+
+    def probe_tiles(self, min_fill: int, reps: int = 20) -> dict:
+        """Dense 16 x 4 MFMA tiles (blocks with >= min_fill nonzeros) + CSR remainder vs the plain gather."""
+
+        out = np.zeros(7)
+        check(self._lib.prh_probe_tiles(self.handle, int(min_fill), int(reps), ptr(out), 7), "prh_probe_tiles")
+        return {"plain_ms": float(out[0]), "tiles_ms": float(out[1]), "remainder_ms": float(out[2]),
+                "max_rel_difference": float(out[3]), "tiles": int(out[4]), "nnz_fraction_in_tiles": float(out[5]),
+                "remainder_nnz": int(out[6])}
 
     def csr(self):
         """(rowptr, col, val) of the symmetric FP64 matrix S held on the device."""
