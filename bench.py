@@ -196,7 +196,7 @@ def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0, with_cpu=True):
     replies = srv.run(reqs, real_time=True)
     served = serving.latency_summary(replies, time.perf_counter() - t)
     served.update({"arrival_rate_per_s": rate, "requests": n_requests, "geometries": list(geoms),
-                   "precision": "fp32 chebyshev sweeps + fp64 refinement, rtol 1e-9", "sweeps": srv.sweeps})
+                   "precision": "chebyshev sweeps (fp16 gathered directions, fp32 recursions) + fp64 refinement, rtol 1e-9", "sweeps": srv.sweeps})
     ticks0 = srv.engine_stats()
     t = time.perf_counter()
     sat = srv.run(make_requests(n_requests, 8), real_time=False)
@@ -268,16 +268,16 @@ def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0, with_cpu=True):
     }
 
 
-def precision_decomposition(geometry, n_emissions=32, rtol=1e-9):
+def precision_decomposition(geometry, n_emissions=64, rtol=1e-9):
     """The same emissions solved four ways, to separate batching from precision.
 
-    32 prescribed-field emissions of one geometry (T_hot 500..900 K, eps
+    64 prescribed-field emissions of one geometry (T_hot 500..900 K, eps
     0.3..0.9) to ||flight||_1 <= 1e-9 ||E||_1: the FP64 series one emission
     at a time, the FP64 series 16 and 32 emissions per pass over the operator
-    (FP64 slots interleaved per patch), and the mixed-precision engine (32
-    FP32 slots with FP64 refinement).  Solves/s of each, the reflections or
-    sweeps they made, and the largest flux error against the one-at-a-time
-    FP64 series."""
+    (FP64 slots interleaved per patch), and the mixed-precision engine (64
+    slots: FP16 gathered directions, FP64 refinement).  Solves/s of each, the
+    reflections or sweeps they made, and the largest flux error against the
+    one-at-a-time FP64 series."""
 
     from porerad import exchange, serving
 
@@ -297,8 +297,8 @@ def precision_decomposition(geometry, n_emissions=32, rtol=1e-9):
         return out
 
     legs = {"fp64, one per pass": ("fp64", 1), "fp64, 16 per pass": ("fp64_batched", 16),
-            "fp64, 32 per pass": ("fp64_batched", 32), "mixed engine, 32 slots": ("mixed", 32)}
-    solve("mixed", 32)  # engine allocation outside the timing
+            "fp64, 32 per pass": ("fp64_batched", 32), "mixed engine, 64 slots": ("mixed", 64)}
+    solve("mixed", 64)  # engine allocation outside the timing
     results, rows = {}, {}
     for name, (precision, group) in legs.items():
         before = op.device.stats()
@@ -565,16 +565,19 @@ def engine_roofline(before, after) -> dict:
     """HBM roofline of the served engine's ticks over a run.
 
     Algorithmic bytes of one tick with k running right-hand sides:
-    nnz(S) (4 B column + s_v B value) + 8 (n + 1) row pointers + 16 n k for
-    the FP32 slot vectors (4 vectors x 4 B per patch and slot); the time is
-    the ticks' device time from HIP events around each tick."""
+    nnz(S) (4 B column + s_v B value) + 8 (n + 1) row pointers + 128 n for
+    the gathered FP16 vector of 64 slots read once + 24 n k for the running
+    slots' own state (x and r read and written, rho read: 20 B in FP32; own
+    z entry read and the next written: 4 B in FP16); the time is the ticks'
+    device time from HIP events around each tick."""
 
     total_bytes = total_ms = ticks = 0.0
     for key, a in after.items():
         b = before.get(key, {"ticks": 0, "ms": 0.0, "slot_ticks": 0})
         n_t = a["ticks"] - b["ticks"]
         k_t = a["slot_ticks"] - b["slot_ticks"]
-        total_bytes += n_t * (a["nnz"] * (4 + a["value_bytes"]) + 8 * (a["n"] + 1)) + 16 * a["n"] * k_t
+        total_bytes += (n_t * (a["nnz"] * (4 + a["value_bytes"]) + 8 * (a["n"] + 1) + 128 * a["n"])
+                        + 24 * a["n"] * k_t)
         total_ms += a["ms"] - b["ms"]
         ticks += n_t
     achieved = total_bytes / (total_ms * 1e-3) / 1e9 if total_ms > 0 else 0.0
@@ -707,7 +710,7 @@ def run_sweep_probes(R, reps=20) -> dict:
     """What one batched sweep of the foam operator costs per slot format, and
     what dense MFMA tiles would take off it.
 
-    Formats: FP32 x 32 slots (the engine's), FP16 x 64 and FP64 x 16 (the
+    Formats: FP32 x 32 slots, FP16 x 64 (the engine's) and FP64 x 16 (the
     same 128 B per gathered column), FP16 x 32 (64 B) and FP64 x 32 (256 B);
     the slot-sweep rate is slots / ms.  Tiles: 16 x 4 blocks of K holding at
     least 4, 8 or 16 nonzeros multiplied by v_mfma_f32_16x16x4f32, the rest

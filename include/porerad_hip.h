@@ -69,7 +69,7 @@ extern "C" {
 
 /* ---- radiosity precisions (prh_radiosity) ---------------------------- */
 #define PRH_FP64 0  /* plain FP64 reflection series */
-#define PRH_MIXED 1 /* batched FP32 Chebyshev sweeps + FP64 residual refinement */
+#define PRH_MIXED 1 /* batched Chebyshev sweeps (FP16 gathered directions) + FP64 refinement */
 #define PRH_FP64_BATCHED 2 /* the FP64 reflection series of up to 32 emissions per pass over the
                               operator (FP64 slot vectors interleaved per patch: 16 slots = one
                               128 B line per gathered column, 32 slots = two); sums in another
@@ -440,9 +440,10 @@ int prh_radiosity(prh_operator *op, int32_t n_rhs, const double *emitted, const 
 void prh_operator_destroy(prh_operator *op);
 
 /* ---- batched radiosity engine (continuous batching) -------------------
- * Up to 32 emissions ("slots") are solved together on one operator.  Each
- * slot alternates epochs of FP32 Chebyshev sweeps with one FP64 refinement
- * sweep (iterative refinement) and finishes when
+ * Up to 64 emissions ("slots") are solved together on one operator.  Each
+ * slot alternates epochs of Chebyshev sweeps (FP16 gathered directions,
+ * FP32 recursions, FP64 products) with one FP64 refinement sweep (iterative
+ * refinement) and finishes when
  * ||E - (I - diag(1-eps) F_p^T) J||_1 <= relative_tolerance ||E||_1; its
  * result is the absorbed power eps * F_p^T J.  Every sweep serves all
  * running slots; slots are admitted and retired at any sweep while the
@@ -459,8 +460,8 @@ int prh_batch_bytes(const prh_batch *b, int64_t *bytes);
  * bytes per matrix value read by a tick (2: integer K, 4: FP32 S).  No
  * counterpart in the reference (measurement). */
 int prh_batch_stats(const prh_batch *b, double *out, int32_t n);
-/* Bit q set = slot q free. */
-int prh_batch_free_slots(const prh_batch *b, uint32_t *free_mask);
+/* Bit q set = slot q free (64 slots). */
+int prh_batch_free_slots(const prh_batch *b, uint64_t *free_mask);
 /* Describe the patches for linear-field requests: class per patch (0 hot
  * wall, 1 cold wall, 2 internal), the x index of each internal patch's cell
  * (ignored for walls), the number of cells along x and the cross-section
@@ -479,7 +480,7 @@ int prh_batch_admit_linear(prh_batch *b, double T_hot_K, double T_cold_K, double
                            double relative_tolerance, int32_t *slot);
 /* Run up to max_ticks sweeps, returning early once a slot has finished;
  * done_mask: finished slots waiting for prh_batch_channels/result/release. */
-int prh_batch_step(prh_batch *b, int32_t max_ticks, uint32_t *done_mask, uint32_t *active_mask);
+int prh_batch_step(prh_batch *b, int32_t max_ticks, uint64_t *done_mask, uint64_t *active_mask);
 /* Channels q_h, q_c, q_i (W m^-2, as prescribed_field_channels), power still
  * in flight and sweep count of a finished slot; keeps the slot.
  * PRH_ERR_SERIES if it did not converge. */

@@ -1,6 +1,6 @@
 // Batched mixed-precision radiosity engine with continuous batching.
 //
-// Solves (I - diag(rho) F_p^T) J = E, rho = 1 - eps, for up to 32 emissions
+// Solves (I - diag(rho) F_p^T) J = E, rho = 1 - eps, for up to 64 emissions
 // ("slots") at once on one device-resident exchange operator, and returns the
 // absorbed power eps * (F_p^T J) -- the quantity the reference's reflection
 // series sums reflection by reflection (porerad/exchange.py:217-245).
@@ -8,25 +8,32 @@
 // Method.  F_p^T x = d * S(d x / A) is similar to a symmetric matrix of
 // spectral radius 1 (F_p is row-stochastic and A F_p symmetric), so the
 // eigenvalues of I - diag(rho) F_p^T lie in [1 - rho_max, 1 + rho_max].  A
-// slot alternates epochs of m FP32 Chebyshev iterations on that interval
+// slot alternates epochs of m Chebyshev iterations on that interval
 // (residual form) with one FP64 refinement step that adds the epoch's
 // correction c to the FP64 solution x and recomputes the FP64 residual
 // r = E - (x - rho * F_p^T x), until ||r||_1 <= rtol ||E||_1.  m is the
 // Chebyshev count whose error bound covers what is left to reach the
-// tolerance, at most a 1e-6 reduction (what FP32 iterations resolve).
+// tolerance, at most a 1e-3 reduction per epoch.
 //
-// Every sweep ("tick") reads the operator once for all 32 slots and
-// accumulates the products in FP64 from FP32 slot vectors: K = C + C^T is
-// stored exactly as uint16 when all patches share area and ray count
-// (S = A/(2R) K + A delta I), else S in FP64.  So the refinement step of a
-// slot is just another tick: the slot's gathered vector is its correction,
-// whose FP64 product F_p^T c is exact up to FP64 rounding; it updates
-// x += s c and y = F_p^T x += s F_p^T c and then starts the next epoch in
-// the same pass.  Each slot runs its own schedule (Chebyshev tick, last
-// tick, refinement tick, idle) independently of the others, and a slot may
-// be admitted or retired at any tick.  Slot vectors are interleaved
-// [row][slot] so one 128 B line serves all 32 slots; workgroups are mapped
-// so that each XCD sweeps a contiguous range of rows.
+// Every sweep ("tick") reads the operator once for all 64 slots.  The
+// vector a tick gathers (z, the scaled search direction) is held in FP16:
+// a sweep costs one random cache-line request per nonzero, and 64 FP16
+// slots fill the 128 B line that 32 FP32 slots filled, so a tick serves
+// twice the slots in about the same time.  The products are accumulated in
+// FP64 from the FP16 inputs: K = C + C^T is stored exactly as uint16 when
+// all patches share area and ray count (S = A/(2R) K + A delta I), else S in
+// FP64.  Each tick takes the direction it actually applied, z / d, as the
+// step of x and of the residual recursion (FP32), so rounding z to FP16
+// changes the search direction but never makes x and r disagree; FP16
+// limits what one epoch can resolve to about 1e-3, hence the epoch target.
+// The refinement step of a slot is just another tick: the slot's gathered
+// vector is its correction c, whose FP64 product F_p^T c is exact up to FP64
+// rounding; it updates x += s c and y = F_p^T x += s F_p^T c and then starts
+// the next epoch in the same pass.  Each slot runs its own schedule
+// (Chebyshev tick, last tick, refinement tick, idle) independently of the
+// others, and a slot may be admitted or retired at any tick.  Slot vectors
+// are interleaved [row][slot] so one 128 B line serves all 64 slots;
+// workgroups are mapped so that each XCD sweeps a contiguous range of rows.
 //
 // Host scheduling is pipelined: the host launches tick t and only then
 // waits for tick t-1, so the device always has the next tick queued.  A
@@ -45,7 +52,10 @@
 namespace prh {
 namespace {
 
-constexpr int SLOTS = 32;
+constexpr int SLOTS = 64;
+constexpr int PER = 8;                 // FP16 slots per 16 B gather: lane h serves slots 8h .. 8h+7
+constexpr double ENGINE_REDUCTION = 1.0e-3;  // finest reduction one epoch aims for (FP16 directions)
+constexpr float Z_MAX = 65504.f;       // largest finite FP16
 constexpr int BT = 256;
 constexpr int ROWS = BT / 64;
 constexpr int RING = 8;
@@ -72,11 +82,14 @@ struct MatView {
     const double *d64;
     const double *area;
     double inv_2R, delta;
+    // gathered form of a slot vector v: z = d v (equal areas) or d v a_ref / A
+    double aref, inv_aref;
+    float aref32;
     uint32_t blocks_per_xcd;
 };
 
 struct SlotVecs {
-    float *x32, *r32, *dir32, *rho32;
+    float *x32, *r32, *rho32;
     double *x64, *y64, *r64, *E64, *eps64;
 };
 
@@ -89,37 +102,51 @@ __device__ __forceinline__ int64_t first_row(const MatView &M)
     return ((int64_t)(b % 8u) * M.blocks_per_xcd + (b / 8u)) * ROWS;
 }
 
-// The gathered form of a slot vector v: d v (equal areas) or d v / A.
-__device__ __forceinline__ float scaled(float di, float v, float ai, bool uniform)
+// The gathered form of a slot vector v in FP16: d v (equal areas) or
+// d v a_ref / A, saturated to the finite FP16 range (a saturated entry only
+// changes the search direction: every tick steps by what it gathered).
+__device__ __forceinline__ _Float16 scaled(float di, float v, float ai, const MatView &M, bool uniform)
 {
-    return uniform ? di * v : di * v / ai;
+    const float z = uniform ? di * v : di * v * M.aref32 / ai;
+    return (_Float16)fminf(fmaxf(z, -Z_MAX), Z_MAX);
 }
 
-// Slots 4h .. 4h+3 of row j of an interleaved slot vector.  The byte offset
+// The slot vector a gathered entry stands for: z / d or z A / (d a_ref).
+__device__ __forceinline__ float unscaled(float di, float z, float ai, const MatView &M, bool uniform)
+{
+    return uniform ? z / di : z * ai / (di * M.aref32);
+}
+
+struct alignas(16) Z8 {
+    _Float16 v[PER];
+};
+
+// Slots 8h .. 8h+7 of row j of an interleaved slot vector.  The byte offset
 // j * 128 + 16 h stays below 2^32 (at most 2^25 patches), so the load takes a
 // scalar base and one 32-bit offset register.
-__device__ __forceinline__ float4 gather4(const float *z, int32_t j, int h)
+__device__ __forceinline__ Z8 gather8(const _Float16 *z, int32_t j, int h)
 {
-    const uint32_t off = (uint32_t)j * (uint32_t)(SLOTS * sizeof(float)) + 16u * (uint32_t)h;
-    return *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(z) + off);
+    const uint32_t off = (uint32_t)j * (uint32_t)(SLOTS * sizeof(_Float16)) + 16u * (uint32_t)h;
+    return *reinterpret_cast<const Z8 *>(reinterpret_cast<const char *>(z) + off);
 }
 
 struct TickArgs {
     MatView M;
     SlotVecs V;
-    const float *z_in;
-    float *z_out;
+    const _Float16 *z_in;
+    _Float16 *z_out;
     const SlotParam *param;       // [SLOTS]
     double *part_abs, *part_sum;  // [SLOTS][n_blocks], written when a slot refines
     int64_t n_blocks;
-    uint32_t group_mask;  // bit h: one of the slots 4h..4h+3 is not idle
+    uint32_t group_mask;  // bit h: one of the slots 8h..8h+7 is not idle
     int32_t any_refine;
 };
 
-// One tick: y = F_p^T z for every slot (FP64 accumulation of FP32 inputs),
+// One tick: y = F_p^T z for every slot (FP64 accumulation of FP16 inputs),
 // then each slot's own update (Chebyshev step, last step, or refinement).
+// Eight waves per SIMD (64 VGPRs, no spills) keep the most gathers in flight.
 template <bool UNIFORM>
-__global__ __launch_bounds__(BT) void k_tick(TickArgs a)
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_tick(TickArgs a)
 {
     __shared__ double s_abs[ROWS][SLOTS], s_sum[ROWS][SLOTS];
     const MatView &M = a.M;
@@ -127,13 +154,15 @@ __global__ __launch_bounds__(BT) void k_tick(TickArgs a)
     // one row per wavefront: the row (and its CSR range) is wave-uniform
     const int64_t row = (int64_t)__builtin_amdgcn_readfirstlane((int)(first_row(M) + wave));
     const int lane = threadIdx.x & 63;
-    const int h = lane & 7;   // gather: slots 4h .. 4h+3
+    const int h = lane & 7;   // gather: slots 8h .. 8h+7
     const int g = lane >> 3;  // nonzero group
-    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
+    double acc[PER];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) acc[p] = 0.0;
     if (row < M.n && a.group_mask) {
-        // A tick makes one random cache-line request per nonzero: the 32 slots
+        // A tick makes one random cache-line request per nonzero: the 64 slots
         // of a row are one 128 B line, fetched by the 8 lanes of a group, 16 B
-        // (slots 4h .. 4h+3) each.  Every lane loads a distinct column and
+        // (slots 8h .. 8h+7) each.  Every lane loads a distinct column and
         // value (64 nonzeros per load), the lanes exchange them through the LDS
         // crossbar, and each lane then has 8 gathers in flight.  Entries past
         // the row's end repeat its last entry with weight 0, and idle slot
@@ -158,41 +187,38 @@ __global__ __launch_bounds__(BT) void k_tick(TickArgs a)
                 j[u] = __shfl(jl, g + 8 * u);
                 v[u] = __shfl(vl, g + 8 * u);
             }
-            float4 z[8];
+            Z8 z[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) z[u] = gather4(a.z_in, j[u], h);
+            for (int u = 0; u < 8; ++u) z[u] = gather8(a.z_in, j[u], h);
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                acc0 = fma(v[u], (double)z[u].x, acc0);
-                acc1 = fma(v[u], (double)z[u].y, acc1);
-                acc2 = fma(v[u], (double)z[u].z, acc2);
-                acc3 = fma(v[u], (double)z[u].w, acc3);
-            }
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int p = 0; p < PER; ++p) acc[p] = fma(v[u], (double)(float)z[u].v[p], acc[p]);
         }
     }
-    for (int off = 8; off < 64; off <<= 1) {
-        acc0 += __shfl_xor(acc0, off);
-        acc1 += __shfl_xor(acc1, off);
-        acc2 += __shfl_xor(acc2, off);
-        acc3 += __shfl_xor(acc3, off);
-    }
-    // Every lane now holds the sums of its 4 slots; lanes g < 4 update slot 4h + g.
+#pragma unroll
+    for (int off = 8; off < 64; off <<= 1)
+#pragma unroll
+        for (int p = 0; p < PER; ++p) acc[p] += __shfl_xor(acc[p], off);
+    // Every lane now holds the sums of its 8 slots; lane (h, g) updates slot 8h + g.
+    double wsum = acc[0];
+#pragma unroll
+    for (int p = 1; p < PER; ++p) wsum = g == p ? acc[p] : wsum;
     double r_abs = 0.0, r_val = 0.0;
-    const int q = 4 * h + (g & 3);
-    if (g < 4 && row < M.n) {
+    const int q = PER * h + g;
+    if (row < M.n) {
         const SlotParam P = a.param[q];
         if (P.mode != MODE_IDLE) {
-            const double w = g == 0 ? acc0 : g == 1 ? acc1 : g == 2 ? acc2 : acc3;
             const SlotVecs &V = a.V;
             const size_t at = (size_t)row * SLOTS + q;
-            const float zi = a.z_in[at];
+            const float zi = (float)a.z_in[at];
             const double di = M.d64[row];
-            const double y = UNIFORM ? di * (w * M.inv_2R + M.delta * (double)zi) : di * w;
+            const double y = UNIFORM ? di * (wsum * M.inv_2R + M.delta * (double)zi) : di * wsum * M.inv_aref;
             const float di32 = M.d32[row];
             const float ai32 = UNIFORM ? 1.f : (float)M.area[row];
             if (P.mode == MODE_REFINE) {
                 const double ai = UNIFORM ? 1.0 : M.area[row];
-                const double c = UNIFORM ? (double)zi / di : (double)zi * ai / di;
+                const double c = UNIFORM ? (double)zi / di : (double)zi * ai / (di * M.aref);
                 const double x = V.x64[at] + P.s * c;
                 const double yy = V.y64[at] + P.s * y;
                 const double r = V.E64[at] - (x - (1.0 - V.eps64[at]) * yy);
@@ -205,10 +231,10 @@ __global__ __launch_bounds__(BT) void k_tick(TickArgs a)
                 const float rn = (float)(r * P.inv_s_next);
                 V.x32[at] = 0.f;
                 V.r32[at] = rn;
-                V.dir32[at] = rn;
-                a.z_out[at] = scaled(di32, rn, ai32, UNIFORM);
+                a.z_out[at] = scaled(di32, rn, ai32, M, UNIFORM);
             } else {
-                const float dk = V.dir32[at];
+                // the direction this tick applied, as gathered
+                const float dk = unscaled(di32, zi, ai32, M, UNIFORM);
                 const float Ad = dk - V.rho32[at] * (float)y;
                 const float rn = V.r32[at] - Ad;
                 const float x = V.x32[at] + dk;
@@ -216,19 +242,16 @@ __global__ __launch_bounds__(BT) void k_tick(TickArgs a)
                 if (P.mode == MODE_CHEB) {
                     const float dn = P.alpha * dk + P.beta * rn;
                     V.r32[at] = rn;
-                    V.dir32[at] = dn;
-                    a.z_out[at] = scaled(di32, dn, ai32, UNIFORM);
+                    a.z_out[at] = scaled(di32, dn, ai32, M, UNIFORM);
                 } else {  // MODE_LAST: the epoch's correction is gathered next
-                    a.z_out[at] = scaled(di32, x, ai32, UNIFORM);
+                    a.z_out[at] = scaled(di32, x, ai32, M, UNIFORM);
                 }
             }
         }
     }
     if (!a.any_refine) return;  // the same for every thread of the grid
-    if (g < 4) {
-        s_abs[wave][q] = r_abs;
-        s_sum[wave][q] = r_val;
-    }
+    s_abs[wave][q] = r_abs;
+    s_sum[wave][q] = r_val;
     __syncthreads();
     if (threadIdx.x < SLOTS) {
         double sa = 0.0, ss = 0.0;
@@ -272,7 +295,7 @@ __device__ __forceinline__ void block_sum_to(double v, double *sh, double *part_
 
 // Admission state of slot q in row i: x = y = 0, r = E, and the first
 // epoch's FP32 state r32 = dir = E / s with its gathered vector.
-__device__ __forceinline__ void admit_row(const MatView &M, const SlotVecs &V, float *z_next, int64_t i, int q,
+__device__ __forceinline__ void admit_row(const MatView &M, const SlotVecs &V, _Float16 *z_next, int64_t i, int q,
                                           double E, double e, double inv_s, bool uniform)
 {
     const size_t at = (size_t)i * SLOTS + q;
@@ -285,11 +308,10 @@ __device__ __forceinline__ void admit_row(const MatView &M, const SlotVecs &V, f
     const float r = (float)(E * inv_s);
     V.x32[at] = 0.f;
     V.r32[at] = r;
-    V.dir32[at] = r;
-    z_next[at] = scaled(M.d32[i], r, uniform ? 1.f : (float)M.area[i], uniform);
+    z_next[at] = scaled(M.d32[i], r, uniform ? 1.f : (float)M.area[i], M, uniform);
 }
 
-__global__ __launch_bounds__(256) void k_admit(MatView M, SlotVecs V, float *z_next, int q, const double *E,
+__global__ __launch_bounds__(256) void k_admit(MatView M, SlotVecs V, _Float16 *z_next, int q, const double *E,
                                                const double *eps, int eps_per_patch, double inv_s, int uniform)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -308,7 +330,7 @@ struct FieldView {
 // (T_m + dT/2) - (i + 1/2) dT / nx, minus the isothermal state at T_m
 // (porerad/coupled.py:138-143, 180-184; porerad/exchange.py:265-270).
 // Also the block sums of |E|.
-__global__ __launch_bounds__(256) void k_admit_linear(MatView M, SlotVecs V, FieldView F, float *z_next, int q,
+__global__ __launch_bounds__(256) void k_admit_linear(MatView M, SlotVecs V, FieldView F, _Float16 *z_next, int q,
                                                       double eps, double eps_sigma, double T_m, double dT,
                                                       double inv_s, int uniform, double *part)
 {
@@ -351,23 +373,6 @@ __global__ __launch_bounds__(256) void k_finish(int64_t n, int q, FieldView F, S
     block_sum_to(s1, sh, part + n_part + blockIdx.x);
     block_sum_to(s2, sh, part + 2 * n_part + blockIdx.x);
     block_sum_to(s3, sh, part + 3 * n_part + blockIdx.x);
-}
-
-// Admission of a device-resident emission with one emissivity.
-__global__ __launch_bounds__(256) void k_admit_uniform(MatView M, SlotVecs V, float *z_next, int q, const double *E,
-                                                       double eps, double inv_s, int uniform)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M.n) return;
-    admit_row(M, V, z_next, i, q, E[i], eps, inv_s, uniform != 0);
-}
-
-// Per-block sums of |x|.
-__global__ __launch_bounds__(256) void k_abs_part(const double *x, int64_t n, double *part)
-{
-    __shared__ double sh[256];
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    block_sum_to(i < n ? fabs(x[i]) : 0.0, sh, part + blockIdx.x);
 }
 
 // Absorbed power eps * y of slot q into a packed vector.
@@ -418,14 +423,16 @@ struct prh_batch {
     int device = 0;
     int64_t n = 0, n_blocks = 0;
     uint32_t blocks_per_xcd = 0;
-    float *x32 = nullptr, *r32 = nullptr, *dir32 = nullptr, *rho32 = nullptr, *zbuf[2] = {nullptr, nullptr};
+    float *x32 = nullptr, *r32 = nullptr, *rho32 = nullptr;
+    _Float16 *zbuf[2] = {nullptr, nullptr};
     double *x64 = nullptr, *y64 = nullptr, *r64 = nullptr, *E64 = nullptr, *eps64 = nullptr;
     double *part = nullptr, *tot = nullptr, *stage = nullptr;
     SlotParam *param_dev = nullptr;
     int8_t *cls = nullptr;
     int32_t *cell_x = nullptr;
     int32_t field_nx = 0;
-    double cross_section = 0.0, mean_area = 0.0;
+    double cross_section = 0.0;
+    double mean_area = 0.0, max_area = 0.0;  // a_ref of the gathered form (unequal areas), largest area
     Pinned *host = nullptr;
     hipEvent_t ev[RING] = {};
     bool ev_used[RING] = {};
@@ -456,21 +463,25 @@ MatView view_of(const prh_batch *b)
     M.area = op->area;
     M.inv_2R = op->inv_2R;
     M.delta = op->delta;
+    M.aref = op->uniform ? 1.0 : b->mean_area;
+    M.inv_aref = 1.0 / M.aref;
+    M.aref32 = (float)M.aref;
     M.blocks_per_xcd = b->blocks_per_xcd;
     return M;
 }
 
 SlotVecs vecs_of(const prh_batch *b)
 {
-    return SlotVecs{b->x32, b->r32, b->dir32, b->rho32, b->x64, b->y64, b->r64, b->E64, b->eps64};
+    return SlotVecs{b->x32, b->r32, b->rho32, b->x64, b->y64, b->r64, b->E64, b->eps64};
 }
 
 FieldView field_of(const prh_batch *b) { return FieldView{b->cls, b->cell_x, (double)b->field_nx}; }
 
 void free_batch(prh_batch *b)
 {
-    void *ptrs[] = {b->x32, b->r32,   b->dir32, b->rho32, b->zbuf[0], b->zbuf[1], b->x64,       b->y64, b->r64,
-                    b->E64, b->eps64, b->part,  b->tot,   b->stage,   b->cls,     b->cell_x,    b->param_dev};
+    void *ptrs[] = {b->x32, b->r32,  b->rho32, b->zbuf[0], b->zbuf[1], b->x64,    b->y64,
+                    b->r64, b->E64,  b->eps64, b->part,    b->tot,     b->stage,  b->cls,
+                    b->cell_x, b->param_dev};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->host) (void)hipHostFree(b->host);
@@ -482,7 +493,7 @@ void free_batch(prh_batch *b)
 }
 
 // The z buffer the next tick gathers.
-float *z_next(prh_batch *b) { return b->zbuf[b->ticks & 1]; }
+_Float16 *z_next(prh_batch *b) { return b->zbuf[b->ticks & 1]; }
 
 int wait_step(prh_batch *b, int64_t s)
 {
@@ -526,7 +537,7 @@ void read_step(prh_batch *b, int64_t s)
             S.state = FINISHING;  // finish kernels are queued by the caller
         } else {
             // length of the epoch that began with the refinement
-            S.tau = epoch_target(S.rtol, S.e_norm, S.r_norm);
+            S.tau = epoch_target(S.rtol, S.e_norm, S.r_norm, ENGINE_REDUCTION);
             S.m = ticks_for(S.rho_max, S.tau);
         }
     }
@@ -584,7 +595,7 @@ int batch_step(prh_batch *b, hipStream_t st, bool *launched)
             S.ticks += 1;
             S.refining = last;
         }
-        group_mask |= 1u << (q / 4);
+        group_mask |= 1u << (q / PER);
         any = 1;
         running += 1;
     }
@@ -645,12 +656,12 @@ int batch_drain(prh_batch *b)
     return PRH_OK;
 }
 
-void report(const prh_batch *b, uint32_t *done_mask, uint32_t *active_mask)
+void report(const prh_batch *b, uint64_t *done_mask, uint64_t *active_mask)
 {
-    uint32_t done = 0, active = 0;
+    uint64_t done = 0, active = 0;
     for (int q = 0; q < SLOTS; ++q) {
-        if (b->slot[q].state != FREE) active |= 1u << q;
-        if (b->slot[q].state == DONE) done |= 1u << q;
+        if (b->slot[q].state != FREE) active |= uint64_t(1) << q;
+        if (b->slot[q].state == DONE) done |= uint64_t(1) << q;
     }
     if (done_mask) *done_mask = done;
     if (active_mask) *active_mask = active;
@@ -658,74 +669,6 @@ void report(const prh_batch *b, uint32_t *done_mask, uint32_t *active_mask)
 
 }  // namespace
 
-namespace prh {
-
-int batch_solve_device(prh_batch *b, const double *E, double emissivity, double rtol, hipStream_t st,
-                       double *absorbed, int32_t *iterations)
-{
-    PRH_CHECK_ARG(b && E && absorbed, "batch_solve_device: null pointer");
-    PRH_CHECK_ARG(emissivity > 0.0 && emissivity <= 1.0 && rtol > 0.0, "batch_solve_device: bad emissivity or tolerance");
-    const int q = first_free(b);
-    if (q < 0) {
-        set_error("batch_solve_device: all %d slots are busy", SLOTS);
-        return PRH_ERR_CAPACITY;
-    }
-    const int64_t n = b->n;
-    const unsigned grid = grid_of(n, 256);
-    // ||E||_1 sets the scale of the first epoch (as prh_batch_admit does on the host)
-    double *part = b->part + (size_t)2 * SLOTS * b->n_blocks - grid;
-    k_abs_part<<<grid, 256, 0, st>>>(E, n, part);
-    k_column_totals<<<1, 256, 0, st>>>(part, (int64_t)grid, b->tot + 2 * SLOTS - 1);
-    PRH_HIP(hipGetLastError());
-    PRH_HIP(hipMemcpyAsync(&b->host->e_norm[q], b->tot + 2 * SLOTS - 1, sizeof(double), hipMemcpyDeviceToHost, st));
-    PRH_HIP(hipStreamSynchronize(st));
-    const double e_norm = b->host->e_norm[q];
-    const unsigned ngrid = grid_of(n, 256);
-    if (!(e_norm > 0.0)) {
-        // the zero emission is absorbed as zero
-        PRH_HIP(hipMemsetAsync(absorbed, 0, (size_t)n * 8, st));
-        if (iterations) *iterations = 0;
-        return PRH_OK;
-    }
-    const double s0 = e_norm / (double)n;
-    k_admit_uniform<<<ngrid, 256, 0, st>>>(view_of(b), vecs_of(b), z_next(b), q, E, emissivity, 1.0 / s0,
-                                           b->op->uniform ? 1 : 0);
-    PRH_HIP(hipGetLastError());
-    SlotHost &S = b->slot[q];
-    S = SlotHost();
-    S.state = RUNNING;
-    S.rtol = rtol;
-    S.rho_max = 1.0 - emissivity;
-    S.e_norm = e_norm;
-    S.r_norm = e_norm;
-    S.s = s0;
-    S.tau = epoch_target(S.rtol, 1.0, 1.0);
-    S.m = ticks_for(S.rho_max, S.tau);
-    int rc = PRH_OK;
-    while (S.state != DONE) {
-        bool launched = false;
-        if ((rc = batch_step(b, st, &launched))) return rc;
-        if (!launched) {
-            if ((rc = batch_drain(b))) return rc;
-            if (S.state != DONE && S.state != FINISHING) {
-                set_error("batch_solve_device: the solve stopped without finishing");
-                return PRH_ERR_HIP;
-            }
-        }
-    }
-    if (S.failed) {
-        S = SlotHost();
-        set_error("radiosity reflection series did not converge");
-        return PRH_ERR_SERIES;
-    }
-    k_collect<<<ngrid, 256, 0, st>>>(n, q, b->eps64, b->y64, absorbed);
-    PRH_HIP(hipGetLastError());
-    if (iterations) *iterations = S.ticks + S.epochs;
-    S = SlotHost();
-    return PRH_OK;
-}
-
-}  // namespace prh
 
 extern "C" {
 
@@ -754,10 +697,9 @@ int prh_batch_create(prh_operator *op, prh_batch **out)
     };
     alloc((void **)&b->x32, ns * 4);
     alloc((void **)&b->r32, ns * 4);
-    alloc((void **)&b->dir32, ns * 4);
     alloc((void **)&b->rho32, ns * 4);
-    alloc((void **)&b->zbuf[0], ns * 4);
-    alloc((void **)&b->zbuf[1], ns * 4);
+    alloc((void **)&b->zbuf[0], ns * 2);
+    alloc((void **)&b->zbuf[1], ns * 2);
     alloc((void **)&b->x64, ns * 8);
     alloc((void **)&b->y64, ns * 8);
     alloc((void **)&b->r64, ns * 8);
@@ -782,9 +724,21 @@ int prh_batch_create(prh_operator *op, prh_batch **out)
         return PRH_ERR_HIP;
     }
     // Zero the slot state so idle slots carry finite values through every sweep.
-    for (void *p : {(void *)b->x32, (void *)b->r32, (void *)b->dir32, (void *)b->rho32, (void *)b->zbuf[0],
-                    (void *)b->zbuf[1]})
-        PRH_HIP(hipMemset(p, 0, ns * 4));
+    for (void *p : {(void *)b->x32, (void *)b->r32, (void *)b->rho32}) PRH_HIP(hipMemset(p, 0, ns * 4));
+    for (void *p : {(void *)b->zbuf[0], (void *)b->zbuf[1]}) PRH_HIP(hipMemset(p, 0, ns * 2));
+    // a_ref of the gathered form (mean area) and the largest area
+    {
+        std::vector<double> area(op->n);
+        PRH_HIP(hipMemcpy(area.data(), op->area, op->n * 8, hipMemcpyDeviceToHost));
+        double sum = 0.0, largest = 0.0;
+        for (double a : area) {
+            sum += a;
+            largest = std::max(largest, a);
+        }
+        b->mean_area = op->n ? sum / (double)op->n : 1.0;
+        b->max_area = largest;
+        if (!(b->mean_area > 0.0)) b->mean_area = 1.0;
+    }
     for (void *p : {(void *)b->x64, (void *)b->y64, (void *)b->r64, (void *)b->E64, (void *)b->eps64})
         PRH_HIP(hipMemset(p, 0, ns * 8));
     *out = b;
@@ -807,12 +761,12 @@ int prh_batch_stats(const prh_batch *b, double *out, int32_t n)
     return PRH_OK;
 }
 
-int prh_batch_free_slots(const prh_batch *b, uint32_t *free_mask)
+int prh_batch_free_slots(const prh_batch *b, uint64_t *free_mask)
 {
     PRH_CHECK_ARG(b && free_mask, "prh_batch_free_slots: null pointer");
-    uint32_t m = 0;
+    uint64_t m = 0;
     for (int q = 0; q < SLOTS; ++q)
-        if (b->slot[q].state == FREE) m |= 1u << q;
+        if (b->slot[q].state == FREE) m |= uint64_t(1) << q;
     *free_mask = m;
     return PRH_OK;
 }
@@ -840,12 +794,7 @@ int prh_batch_set_field(prh_batch *b, const int8_t *patch_class, const int32_t *
     }
     PRH_HIP(hipMemcpyAsync(b->cls, patch_class, n, hipMemcpyHostToDevice, st));
     PRH_HIP(hipMemcpyAsync(b->cell_x, cell_x, n * 4, hipMemcpyHostToDevice, st));
-    std::vector<double> area(n);
-    PRH_HIP(hipMemcpyAsync(area.data(), b->op->area, n * 8, hipMemcpyDeviceToHost, st));
     PRH_HIP(hipStreamSynchronize(st));
-    double sum = 0.0;
-    for (double a : area) sum += a;
-    b->mean_area = n ? sum / (double)n : 0.0;
     b->field_nx = nx;
     b->cross_section = cross_section_m2;
     return PRH_OK;
@@ -862,9 +811,10 @@ int prh_batch_admit(prh_batch *b, const double *emitted, const double *emissivit
         return PRH_ERR_CAPACITY;
     }
     const int64_t n = b->n;
-    double e_norm = 0.0, rho_max = 0.0;
+    double e_norm = 0.0, e_max = 0.0, rho_max = 0.0;
     for (int64_t i = 0; i < n; ++i) {
         e_norm += std::fabs(emitted[i]);
+        e_max = std::max(e_max, std::fabs(emitted[i]));
         const double e = eps_per_patch ? emissivity[i] : emissivity[0];
         PRH_CHECK_ARG(e >= 0.0 && e <= 1.0, "prh_batch_admit: emissivity must lie in [0, 1]");
         rho_max = std::max(rho_max, 1.0 - e);
@@ -874,7 +824,8 @@ int prh_batch_admit(prh_batch *b, const double *emitted, const double *emissivit
     hipStream_t st;
     int rc = current_stream(&st);
     if (rc) return rc;
-    const double s0 = e_norm > 0.0 ? e_norm / (double)n : 1.0;
+    // the first epoch's scale: |E| / s <= 1 keeps the FP16 gathered vector in range
+    const double s0 = e_max > 0.0 ? e_max : 1.0;
     PRH_HIP(hipMemcpyAsync(b->stage, emitted, n * 8, hipMemcpyHostToDevice, st));
     PRH_HIP(hipMemcpyAsync(b->stage + n, emissivity, (eps_per_patch ? n : 1) * 8, hipMemcpyHostToDevice, st));
     k_admit<<<grid_of(n, 256), 256, 0, st>>>(view_of(b), vecs_of(b), z_next(b), q, b->stage, b->stage + n,
@@ -890,7 +841,7 @@ int prh_batch_admit(prh_batch *b, const double *emitted, const double *emissivit
     S.e_norm = e_norm;
     S.r_norm = e_norm;
     S.s = s0;
-    S.tau = epoch_target(S.rtol, 1.0, 1.0);
+    S.tau = epoch_target(S.rtol, 1.0, 1.0, ENGINE_REDUCTION);
     S.m = ticks_for(rho_max, S.tau);
     if (e_norm == 0.0) {
         // nothing to iterate: the zero solution is exact
@@ -921,11 +872,11 @@ int prh_batch_admit_linear(prh_batch *b, double T_hot_K, double T_cold_K, double
     if (rc) return rc;
     const double T_m = 0.5 * (T_hot_K + T_cold_K);
     const double dT = T_hot_K - T_cold_K;
-    // scale of the first epoch: the mean |E| of the linear profile is at most
-    // half of its largest value
+    // scale of the first epoch: the largest |E| of the linear profile (walls
+    // at T_m +/- dT/2, interior in between), so |E| / s <= 1
     const double Th4 = std::fabs(std::pow(T_hot_K, 4) - std::pow(T_m, 4));
     const double Tc4 = std::fabs(std::pow(T_cold_K, 4) - std::pow(T_m, 4));
-    double s0 = 0.5 * emissivity * SIGMA * b->mean_area * std::max(Th4, Tc4);
+    double s0 = emissivity * SIGMA * b->max_area * std::max(Th4, Tc4);
     if (!(s0 > 0.0) || !std::isfinite(s0)) s0 = 1.0;
     const int64_t n = b->n;
     const unsigned grid = grid_of(n, 256);
@@ -944,13 +895,13 @@ int prh_batch_admit_linear(prh_batch *b, double T_hot_K, double T_cold_K, double
     S.rho_max = 1.0 - emissivity;
     S.e_norm_step = b->step;  // the next step's completion covers this admission
     S.s = s0;
-    S.tau = epoch_target(S.rtol, 1.0, 1.0);
+    S.tau = epoch_target(S.rtol, 1.0, 1.0, ENGINE_REDUCTION);
     S.m = ticks_for(S.rho_max, S.tau);
     *slot_out = q;
     return PRH_OK;
 }
 
-int prh_batch_step(prh_batch *b, int32_t max_ticks, uint32_t *done_mask, uint32_t *active_mask)
+int prh_batch_step(prh_batch *b, int32_t max_ticks, uint64_t *done_mask, uint64_t *active_mask)
 {
     PRH_CHECK_ARG(b, "prh_batch_step: null batch");
     PRH_CHECK_ARG(max_ticks >= 1, "prh_batch_step: max_ticks must be positive");
@@ -969,7 +920,7 @@ int prh_batch_step(prh_batch *b, int32_t max_ticks, uint32_t *done_mask, uint32_
             if (!finishing) break;
             continue;
         }
-        uint32_t done = 0;
+        uint64_t done = 0;
         report(b, &done, nullptr);
         if (done) break;
     }
@@ -1045,7 +996,7 @@ int prh_radiosity_mixed(prh_operator *op, int32_t n_rhs, const double *emitted, 
     for (int q = 0; q < SLOTS; ++q) rhs_of_slot[q] = -1;
     int next = 0, finished = 0, status = PRH_OK;
     while (finished < n_rhs) {
-        uint32_t free_mask = 0;
+        uint64_t free_mask = 0;
         int rc = prh_batch_free_slots(b, &free_mask);
         if (rc) return rc;
         while (next < n_rhs && free_mask) {
@@ -1053,9 +1004,9 @@ int prh_radiosity_mixed(prh_operator *op, int32_t n_rhs, const double *emitted, 
             const double *eps = emissivity + (emissivity_per_rhs ? (int64_t)next * n : 0);
             if ((rc = prh_batch_admit(b, emitted + (int64_t)next * n, eps, 1, relative_tolerance, &q))) return rc;
             rhs_of_slot[q] = next++;
-            free_mask &= ~(1u << q);
+            free_mask &= ~(uint64_t(1) << q);
         }
-        uint32_t done = 0, active = 0;
+        uint64_t done = 0, active = 0;
         if ((rc = prh_batch_step(b, MAX_TICKS * MAX_EPOCHS, &done, &active))) return rc;
         for (int q = 0; q < SLOTS; ++q) {
             if (!((done >> q) & 1u)) continue;

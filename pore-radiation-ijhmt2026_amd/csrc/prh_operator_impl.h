@@ -27,11 +27,12 @@ inline int ticks_for(double rho_max, double tau)
 }
 
 // Reduction an epoch aims for: what is left to reach the tolerance, with a
-// factor 10 of margin, but no finer than FP32 iterations resolve.
-inline double epoch_target(double rtol, double e_norm, double r_norm)
+// factor 10 of margin, but no finer than the iterations resolve (inner:
+// INNER_REDUCTION with FP32 directions).
+inline double epoch_target(double rtol, double e_norm, double r_norm, double inner = INNER_REDUCTION)
 {
     const double left = 0.1 * rtol * e_norm / std::max(r_norm, 1e-300);
-    return std::min(std::max(left, INNER_REDUCTION), 0.5);
+    return std::min(std::max(left, inner), 0.5);
 }
 
 // alpha_k, beta_k of Chebyshev acceleration on [1 - delta, 1 + delta]:

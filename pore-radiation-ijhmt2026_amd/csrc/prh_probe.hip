@@ -186,7 +186,7 @@ __global__ __launch_bounds__(PT) void k_probe_tiles(TileView T, const float *z, 
 }
 
 // y64[row][s] = (K z)[row][s] over a CSR with uint16 values (FP64 accumulation),
-// the engine's gather: 8 lanes x 16 B per column, 8 gathers in flight per lane.
+// the 32-slot FP32 gather: 8 lanes x 16 B per column, 8 gathers in flight per lane.
 __global__ __launch_bounds__(PT) void k_probe_csr(ProbeView M, const float *z, double *y64)
 {
     const int64_t row = probe_row(M.blocks_per_xcd);

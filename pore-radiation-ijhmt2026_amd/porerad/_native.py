@@ -33,7 +33,7 @@ APPLY_SYMMETRIC, APPLY_MATVEC, APPLY_RMATVEC, APPLY_RAW, APPLY_RAW_T = 0, 1, 2, 
 FP64, MIXED, FP64_BATCHED = 0, 1, 2
 # slot formats of prh_probe_gather
 PROBE_FORMATS = {"fp32x32": 0, "fp16x64": 1, "fp16x32": 2, "fp64x16": 3, "fp64x32": 4}
-BATCH_SLOTS = 32
+BATCH_SLOTS = 64
 BATCH_DIRECT = 1
 
 # Every function the C ABI exports: name -> (argtypes, restype).
@@ -735,9 +735,14 @@ class Batch:
         handle = ctypes.c_void_p()
         check(self._lib.prh_batch_create(operator.handle, ctypes.byref(handle)), "prh_batch_create")
         self.handle = handle
+
+    @property
+    def bytes(self) -> int:
+        """Device bytes the engine holds (slot vectors, partial sums, staging, field)."""
+
         b = ctypes.c_int64()
-        check(self._lib.prh_batch_bytes(handle, ctypes.byref(b)), "prh_batch_bytes")
-        self.bytes = int(b.value)
+        check(self._lib.prh_batch_bytes(self.handle, ctypes.byref(b)), "prh_batch_bytes")
+        return int(b.value)
 
     def stats(self) -> dict:
         """Device time of the engine's ticks (HIP events around each tick)."""
@@ -748,7 +753,7 @@ class Batch:
                 "n": int(out[4]), "value_bytes": int(out[5])}
 
     def free_slots(self) -> int:
-        m = ctypes.c_uint32()
+        m = ctypes.c_uint64()
         check(self._lib.prh_batch_free_slots(self.handle, ctypes.byref(m)), "prh_batch_free_slots")
         return int(m.value)
 
@@ -781,7 +786,7 @@ class Batch:
     def step(self, max_ticks: int = 1) -> tuple[int, int]:
         """Run up to max_ticks sweeps (returns early when a slot finishes): (done mask, active mask)."""
 
-        done, active = ctypes.c_uint32(), ctypes.c_uint32()
+        done, active = ctypes.c_uint64(), ctypes.c_uint64()
         check(self._lib.prh_batch_step(self.handle, int(max_ticks), ctypes.byref(done), ctypes.byref(active)),
               "prh_batch_step")
         return int(done.value), int(active.value)

@@ -263,8 +263,9 @@ def reflection_series_batch(
     ``reflection_series``, one emission at a time; ``"fp64_batched"`` sums
     the same FP64 series for up to 32 emissions per pass over the operator
     (each stops on its own tolerance; sums in another order, so equal to
-    rounding); ``"mixed"`` solves the same radiosity balance with
-    batched FP32 Chebyshev sweeps and FP64 residual refinement until
+    rounding); ``"mixed"`` solves the same radiosity balance on the
+    batched engine (64 slots; Chebyshev sweeps with FP16 gathered
+    directions and FP32 recursions, FP64 residual refinement) until
     ||E - (I - (1-eps) F_p^T) J||_1 <= relative_tolerance ||E||_1 (the
     iteration count is then the number of sweeps).
     """

@@ -4,12 +4,12 @@ A request names a resident geometry (its ``SurfacePatches`` and
 ``ExchangeOperator``), the two wall temperatures and the emissivity; it asks
 for the radiation channels of the prescribed linear field between the walls
 (porerad/coupled.py:170-194), solved to a relative tolerance with the
-batched mixed-precision engine (``prh_batch_*``: FP32 Chebyshev sweeps over
-32 slots per operator, FP64 refinement).  The emission is formed and the
+batched mixed-precision engine (``prh_batch_*``: Chebyshev sweeps over 64
+slots per operator with FP16 gathered directions, FP64 refinement).  The emission is formed and the
 channels are summed on the device, so a request moves a few scalars over
 PCIe; the patch powers come back only when a request asks for them.
 
-Scheduling.  Every geometry has one engine with 32 slots.  Each scheduler
+Scheduling.  Every geometry has one engine with 64 slots.  Each scheduler
 step (1) admits queued requests, oldest first, into free slots of their
 geometry's engine, creating the engine first if its device memory fits the
 HBM budget (idle engines are released to make room); (2) advances every
@@ -44,7 +44,8 @@ import numpy as np
 from .constants import SIGMA
 from .view_factors import COLD, HOT, INTERNAL
 
-SLOTS = 32
+SLOTS = 64
+RING = 8  # sweeps the engine keeps in flight (parameter sets held on the device)
 
 
 @dataclasses.dataclass
@@ -191,11 +192,13 @@ def device_coupled_bytes(geometry: Geometry) -> int:
 
 
 def device_engine_bytes(geometry: Geometry) -> int:
-    """Device bytes of one engine: 32 slots x (6 FP32 + 5 FP64 vectors) per patch, partial sums, field."""
+    """Device bytes of one engine: 64 slots x (3 FP32 + 2 FP16 + 5 FP64 vectors) per patch, partial
+    sums and totals, staging (2 FP64 per patch), field (5 B per patch), the per-sweep slot parameters."""
 
     n = geometry.n
     blocks = ((n + 3) // 4 + 7) // 8 * 8
-    return n * SLOTS * (6 * 4 + 5 * 8) + max(2 * SLOTS * blocks, 4 * ((n + 255) // 256)) * 8 + n * 21 + 8192
+    return (n * SLOTS * (3 * 4 + 2 * 2 + 5 * 8) + max(2 * SLOTS * blocks, 4 * ((n + 255) // 256)) * 8
+            + 2 * SLOTS * 8 + n * 21 + RING * SLOTS * 32)
 
 
 def device_tracer_bytes() -> int:

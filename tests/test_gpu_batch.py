@@ -95,6 +95,46 @@ def test_alone_equals_in_mix_bitwise(mods):
         batch.close()
 
 
+@pytest.mark.parametrize("nonuniform", [False, True])
+def test_concentrated_and_wide_range_emissions(mods, nonuniform):
+    """Emission on one patch, and emissions spanning 30 decades across patches:
+    the FP16 gathered directions stay in range (the first epoch is scaled by
+    max|E|) and the solves meet the same 1e-8 bar against the oracle."""
+
+    _native, _, ex, _ = mods
+    Q, op, ref = _pair(12, nonuniform=nonuniform)
+    n = Q.n_patches
+    rng = np.random.default_rng(13)
+    single = np.zeros(n)
+    single[n // 3] = 5.0
+    wide = rng.normal(size=n) * 10.0 ** rng.uniform(-20, 10, n)
+    E = np.stack([single, wide])
+    eps = np.stack([np.full(n, 0.4), rng.uniform(0.2, 0.95, n)])
+    got = ex.reflection_series_batch(op, E, eps, relative_tolerance=1e-9, precision="mixed")
+    for q in range(2):
+        absorbed, _, _ = X.reflection_series(ref, E[q], eps[q], relative_tolerance=1e-13)
+        assert np.max(np.abs(got[q].absorbed_power_W - absorbed)) <= 1e-8 * np.max(np.abs(absorbed)), q
+        assert abs(got[q].global_energy_residual_W) <= 1e-8 * np.sum(np.abs(E[q])), q
+
+
+def test_engine_bytes_match_serving_estimate(mods):
+    """The server's HBM admission counts exactly what an engine allocates."""
+
+    _native, vf, _, _ = mods
+    from porerad import serving
+
+    Q, op, _ = _pair(14, shape=(12, 10, 10))
+    batch = _native.Batch(op.device)
+    try:
+        P = vf.build_surface_patches((np.random.default_rng(14).random((12, 10, 10)) < 0.14).astype(np.uint8), 0.01)
+        g = serving.Geometry("g", P, op)
+        assert P.n_patches == Q.n_patches
+        batch.set_field(P.patch_class, g.cell_x, 12, g.cross_section)
+        assert batch.bytes == serving.device_engine_bytes(g)
+    finally:
+        batch.close()
+
+
 def test_more_requests_than_slots(mods):
     _native, _, ex, _ = mods
     Q, op, ref = _pair(7, shape=(12, 10, 10))

@@ -1,6 +1,6 @@
 """Scheduling logic of the radiosity server, on the CPU with a stand-in engine.
 
-The stand-in keeps the engine contract of prh_batch (32 slots,
+The stand-in keeps the engine contract of prh_batch (64 slots,
 admit_linear / step / channels / result / release, device bytes) and
 finishes a slot after a fixed number of sweeps, with absorbed power = E / 4,
 so admission order, continuous batching and HBM-budget eviction can be
