@@ -120,9 +120,11 @@ def build_operator(P, R, seed):
 
     edges = view_factors.first_hit_edges(P, R, [seed])
     try:
-        return exchange.ExchangeOperator.from_edges(edges, np.full(P.n_patches, R, np.int64), P.patch_area_m2)
+        op = exchange.ExchangeOperator.from_edges(edges, np.full(P.n_patches, R, np.int64), P.patch_area_m2)
     finally:
         edges.close()
+    op.set_sweep_order(P.sweep_order())  # the batched engine sweeps rows along a Morton curve
+    return op
 
 
 def run_radiosity(foam_P, R, seed, n_requests=600, rate=200.0, with_cpu=True):
@@ -729,6 +731,12 @@ def run_sweep_probes(R, reps=20) -> dict:
     base = formats["fp32x32"]["slot_sweeps_per_ms"]
     for row in formats.values():
         row["vs_fp32x32"] = row["slot_sweeps_per_ms"] / base
+    orders = {}
+    for name, order in patch_orders(P).items():
+        o = op.device.probe_order(order, reps)
+        orders[name] = o
+        log(f"patch order {name}: stored {o['stored_ms']:.3f} ms (streamed matrix {o['stored_streamed_ms']:.3f}), "
+            f"reordered {o['reordered_ms']:.3f} ms (streamed {o['reordered_streamed_ms']:.3f})")
     tiles = {}
     for fill in (4, 8, 16):
         t = op.device.probe_tiles(fill, reps)
@@ -739,7 +747,15 @@ def run_sweep_probes(R, reps=20) -> dict:
             f"{t['plain_ms']:.3f} ms vs tiles {t['tiles_ms']:.3f} + remainder {t['remainder_ms']:.3f} ms")
     op.device.close()
     return {"workload": f"foam N=128, {P.n_patches} patches, {R} rays/patch (seed 3)", "reps": reps,
-            "formats": formats, "mfma_tiles": tiles}
+            "formats": formats, "patch_orders": orders, "mfma_tiles": tiles}
+
+
+def patch_orders(P) -> dict:
+    """Renumberings of the patches for the locality probe: Morton (Z-order) and
+    z-major lexicographic order of the patch origins on the half-voxel grid."""
+
+    ijk = np.floor(np.asarray(P.patch_origin) * 2.0).astype(np.int64)
+    return {"morton": P.sweep_order(), "z-major": np.lexsort((ijk[:, 0], ijk[:, 1], ijk[:, 2])).astype(np.int32)}
 
 
 def run_monte_carlo(phases, with_cpu: bool) -> dict:

@@ -373,6 +373,14 @@ int prh_operator_edges(const prh_operator *op, int32_t *origin, int32_t *receive
 /* Diagonal balancing factor d (n), used by modes MATVEC/RMATVEC and by
  * prh_radiosity.  Defaults to ones. */
 int prh_operator_set_scale(prh_operator *op, const double *scale);
+/* Order in which the batched engine (prh_batch_*) sweeps the rows: a
+ * permutation of the n patches (order[p] = the patch swept p-th), or NULL
+ * for the stored order.  Each XCD sweeps a contiguous run of positions, so a
+ * spatially coherent order (patches sorted along a space-filling curve)
+ * lets the rows an XCD sweeps together share the L2 lines they gather.
+ * Results change only in the rounding of the engine's norm sums.  No
+ * counterpart in the reference (performance). */
+int prh_operator_set_row_order(prh_operator *op, const int32_t *order);
 /* Balance the operator on the device and set d: solve d (S d) = A, as
  * ExchangeOperator._balance, porerad/exchange.py:153-197 (with
  * _closure_error, :142-151): no balancing when d = 1 already closes the rows
@@ -426,6 +434,14 @@ int prh_probe_gather(const prh_operator *op, int32_t format, int32_t reps, doubl
  * product and tiles + remainder relative to the largest product, tiles,
  * fraction of the nonzeros in tiles, remainder nonzeros. */
 int prh_probe_tiles(const prh_operator *op, int32_t min_fill, int32_t reps, double *out, int32_t n_out);
+/* prh_probe_order: the FP16 x 64 sweep of prh_probe_gather over S in its
+ * stored patch order and with patches renumbered by `order` (n patch
+ * indices, a permutation: position p takes patch order[p]), each with plain
+ * and with non-temporal (streamed) loads of the matrix.  out[0..n_out) = ms
+ * stored/plain, stored/streamed, reordered/plain, reordered/streamed, and the
+ * largest difference between the reordered product and the stored one
+ * reordered, relative to the largest product (FP16 outputs). */
+int prh_probe_order(const prh_operator *op, const int32_t *order, int32_t reps, double *out, int32_t n_out);
 
 /* Reflection series J = E + (1 - eps) F_p^T J for n_rhs emissions.
  * emitted: n_rhs x n (row-major, one emission per row); emissivity:

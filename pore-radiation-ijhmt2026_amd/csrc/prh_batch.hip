@@ -85,6 +85,7 @@ struct MatView {
     // gathered form of a slot vector v: z = d v (equal areas) or d v a_ref / A
     double aref, inv_aref;
     float aref32;
+    const int32_t *row_order;  // sweep position -> row, or null for the stored order
     uint32_t blocks_per_xcd;
 };
 
@@ -152,7 +153,11 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     const MatView &M = a.M;
     const int wave = threadIdx.x >> 6;
     // one row per wavefront: the row (and its CSR range) is wave-uniform
-    const int64_t row = (int64_t)__builtin_amdgcn_readfirstlane((int)(first_row(M) + wave));
+    // the wave's sweep position, and its row (row_order keeps the rows an XCD
+    // sweeps together close in space, so their gathers share L2 lines)
+    const int64_t pos = first_row(M) + wave;
+    const int64_t row = (int64_t)__builtin_amdgcn_readfirstlane(
+        (int)(pos < M.n && M.row_order ? (int64_t)M.row_order[pos] : pos));
     const int lane = threadIdx.x & 63;
     const int h = lane & 7;   // gather: slots 8h .. 8h+7
     const int g = lane >> 3;  // nonzero group
@@ -466,6 +471,7 @@ MatView view_of(const prh_batch *b)
     M.aref = op->uniform ? 1.0 : b->mean_area;
     M.inv_aref = 1.0 / M.aref;
     M.aref32 = (float)M.aref;
+    M.row_order = op->row_order;
     M.blocks_per_xcd = b->blocks_per_xcd;
     return M;
 }

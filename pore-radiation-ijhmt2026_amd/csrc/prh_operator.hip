@@ -390,13 +390,14 @@ struct SeriesSlots {
 // One wave per row: FS / 2 lanes cover a column's slots with 16 B loads, the
 // wave's 64 / (FS / 2) lane groups take interleaved nonzeros, every lane has
 // FS / 2 gathers in flight.  Workgroups are dealt round-robin to the 8 XCDs;
-// XCD x sweeps the logical blocks [x * blocks_per_xcd, (x + 1) * blocks_per_xcd)
-// so its L2 serves a contiguous range of patches.
+// XCD x sweeps the positions [x * blocks_per_xcd, (x + 1) * blocks_per_xcd)
+// (in blocks) of the row order, so its L2 serves a contiguous range of it.
 template <int FS, bool UNIFORM>
 __global__ __launch_bounds__(ROW_THREADS) void k_series_batch(SView M, int64_t n, const double *d, const double *area,
                                                               const double *eps, const double *w, double *w_next,
                                                               double *absorbed, double *flight, double *part_abs,
-                                                              uint32_t blocks_per_xcd, const SeriesSlots *S)
+                                                              uint32_t blocks_per_xcd, const int32_t *row_order,
+                                                              const SeriesSlots *S)
 {
     constexpr int HL = FS / 2;   // lanes per column
     constexpr int NG = 64 / HL;  // nonzeros in flight per wave
@@ -404,8 +405,10 @@ __global__ __launch_bounds__(ROW_THREADS) void k_series_batch(SView M, int64_t n
     __shared__ double sa[ROWS_PER_BLOCK][FS];
     const int wave = threadIdx.x >> 6;
     const uint32_t b = blockIdx.x;
-    const int64_t first = ((int64_t)(b % 8u) * blocks_per_xcd + (b / 8u)) * ROWS_PER_BLOCK;
-    const int64_t row = (int64_t)__builtin_amdgcn_readfirstlane((int)(first + wave));
+    const int64_t pos = ((int64_t)(b % 8u) * blocks_per_xcd + (b / 8u)) * ROWS_PER_BLOCK + wave;
+    // the engine's sweep order when the operator has one (prh_operator_set_row_order)
+    const int64_t row =
+        (int64_t)__builtin_amdgcn_readfirstlane((int)(pos < n && row_order ? (int64_t)row_order[pos] : pos));
     const int lane = threadIdx.x & 63;
     const int h = lane % HL;  // slots 2h, 2h + 1
     const int g = lane / HL;  // nonzero group
@@ -561,7 +564,8 @@ void free_operator(prh_operator *op)
     void *ptrs[] = {op->rowptr, op->col, op->val, op->c_rowptr, op->t_rowptr, op->c_col, op->c_cnt,
                     op->t_col, op->t_cnt, op->rays, op->area, op->scale, op->vx, op->vy, op->vt,
                     op->flight, op->absorbed, op->w0, op->w1, op->eps, op->emit, op->part_abs,
-                    op->part_sum, op->state, op->kval, op->val32, op->scale32, op->mix32, op->mix64};
+                    op->part_sum, op->state, op->kval, op->val32, op->scale32, op->mix32, op->mix64,
+                    op->row_order};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : op->series_ev)
@@ -1008,6 +1012,26 @@ int prh_operator_set_scale(prh_operator *op, const double *scale)
     return PRH_OK;
 }
 
+int prh_operator_set_row_order(prh_operator *op, const int32_t *order)
+{
+    PRH_CHECK_ARG(op, "prh_operator_set_row_order: null operator");
+    PRH_HIP(hipSetDevice(op->device));
+    if (!order) {
+        if (op->row_order) (void)hipFree(op->row_order);
+        op->row_order = nullptr;
+        return PRH_OK;
+    }
+    std::vector<uint8_t> seen(op->n, 0);
+    for (int64_t p = 0; p < op->n; ++p) {
+        PRH_CHECK_ARG(order[p] >= 0 && order[p] < op->n && !seen[order[p]],
+                      "prh_operator_set_row_order: not a permutation of the patches");
+        seen[order[p]] = 1;
+    }
+    if (!op->row_order) PRH_HIP(hipMalloc((void **)&op->row_order, sizeof(int32_t) * std::max<int64_t>(op->n, 1)));
+    PRH_HIP(hipMemcpy(op->row_order, order, sizeof(int32_t) * op->n, hipMemcpyHostToDevice));
+    return PRH_OK;
+}
+
 int prh_operator_reciprocity(prh_operator *op, double *error)
 {
     PRH_CHECK_ARG(op && error, "prh_operator_reciprocity: null pointer");
@@ -1128,7 +1152,7 @@ int radiosity_fp64_batched(prh_operator *op, int32_t n_rhs, const double *emitte
                 auto step = FS == 16 ? (op->uniform ? k_series_batch<16, true> : k_series_batch<16, false>)
                                      : (op->uniform ? k_series_batch<32, true> : k_series_batch<32, false>);
                 step<<<rb, ROW_THREADS, 0, st>>>(M, n, op->scale, op->area, eps_d, w_in, w_out, abs_d, fl_d, part,
-                                                 bpx, S);
+                                                 bpx, op->row_order, S);
                 k_series_batch_check<<<FS, RED_THREADS, 0, st>>>(part, (int64_t)rb, rtol, active, S);
                 std::swap(w_in, w_out);
             }

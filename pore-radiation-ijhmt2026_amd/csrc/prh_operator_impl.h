@@ -102,6 +102,9 @@ struct prh_operator {
     uint16_t *kval = nullptr;  // K = C + C^T on the sparsity of S (uniform only)
     float *val32 = nullptr;    // FP32 S (non-uniform only)
     float *scale32 = nullptr;
+    // order in which the batched engine sweeps the rows (a permutation of the
+    // patches, spatially coherent), or null for the stored order
+    int32_t *row_order = nullptr;
     prh_batch *batch = nullptr;  // engine of the mixed-precision path, created on first use
     // single-emission mixed-precision solve (mixed_solve_single), allocated on first use:
     // FP32 x, r, dir, z[2] (n each) and FP64 x, y, r (n each) + one total

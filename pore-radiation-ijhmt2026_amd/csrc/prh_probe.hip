@@ -67,8 +67,10 @@ __device__ __forceinline__ int64_t probe_row(uint32_t blocks_per_xcd)
 }
 
 // z_out[row][s] = (F_p^T z_in)[row][s] in the slot format T (bounded: F_p^T
-// has spectral radius 1), as the engine's tick forms it.
-template <typename T, int SL>
+// has spectral radius 1), as the engine's tick forms it.  STREAM: the matrix
+// (columns, values) is read with non-temporal loads, so its stream does not
+// displace lines of the gathered vector from the L2.
+template <typename T, int SL, bool STREAM = false>
 __global__ __launch_bounds__(PT) void k_probe_gather(ProbeView M, const T *z_in, T *z_out)
 {
     constexpr int PER = 16 / (int)sizeof(T);  // slots per 16 B load
@@ -92,8 +94,9 @@ __global__ __launch_bounds__(PT) void k_probe_gather(ProbeView M, const T *z_in,
             const uint32_t e = base + (uint32_t)lane;
             const bool ok = e < len;
             const uint32_t ec = ok ? e : len - 1;
-            const int32_t jl = col[ec];
-            const double vl = ok ? (double)kv[ec] : 0.0;
+            const int32_t jl = STREAM ? __builtin_nontemporal_load(col + ec) : col[ec];
+            const uint16_t kl = STREAM ? __builtin_nontemporal_load(kv + ec) : kv[ec];
+            const double vl = ok ? (double)kl : 0.0;
             int32_t j[U];
             double v[U];
 #pragma unroll
@@ -290,6 +293,44 @@ int time_gather(const prh_operator *op, int32_t reps, hipStream_t st, double *ms
     return PRH_OK;
 }
 
+// Device ms per sweep of the FP16 x 64 gather over the CSR (rowptr, col, kval)
+// with balancing factor d, plain (out[0]) and with streamed matrix loads
+// (out[1]); the product of one sweep from z0 is left in y.
+int time_fp16x64(int64_t n, const int64_t *rowptr, const int32_t *col, const uint16_t *kval, const double *d,
+                 double inv_2R, double delta, const _Float16 *z0, _Float16 *y, _Float16 *z_work, int32_t reps,
+                 hipStream_t st, double *out)
+{
+    const uint32_t bpx = xcd_blocks(n);
+    const ProbeView M{n, rowptr, col, kval, d, inv_2R, delta, bpx};
+    hipEvent_t ev[3];
+    for (auto &e : ev) PRH_HIP(hipEventCreate(&e));
+    k_probe_gather<_Float16, 64, false><<<8u * bpx, PT, 0, st>>>(M, z0, y);  // warm-up and the product
+    PRH_HIP(hipEventRecord(ev[0], st));
+    for (int r = 0; r < reps; ++r) k_probe_gather<_Float16, 64, false><<<8u * bpx, PT, 0, st>>>(M, z0, z_work);
+    PRH_HIP(hipEventRecord(ev[1], st));
+    for (int r = 0; r < reps; ++r) k_probe_gather<_Float16, 64, true><<<8u * bpx, PT, 0, st>>>(M, z0, z_work);
+    PRH_HIP(hipEventRecord(ev[2], st));
+    PRH_HIP(hipGetLastError());
+    PRH_HIP(hipEventSynchronize(ev[2]));
+    for (int k = 0; k < 2; ++k) {
+        float ms = 0.f;
+        PRH_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+        out[k] = ms / reps;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return PRH_OK;
+}
+
+// Values of order one in every slot, as a function of the patch's original index.
+__global__ void k_probe_fill_perm(_Float16 *z, const int32_t *order, int64_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * 64) return;
+    const size_t p = i / 64, s = i % 64;
+    const size_t src = order ? (size_t)order[p] * 64 + s : i;
+    z[i] = from_f64<_Float16>(0.25 + (double)((uint32_t)(src * 2654435761u) >> 8) * 0x1p-25);
+}
+
 }  // namespace
 }  // namespace prh
 
@@ -316,6 +357,85 @@ int prh_probe_gather(const prh_operator *op, int32_t format, int32_t reps, doubl
     }
     set_error("prh_probe_gather: unknown slot format %d", format);
     return PRH_ERR_ARG;
+}
+
+int prh_probe_order(const prh_operator *op, const int32_t *order, int32_t reps, double *out, int32_t n_out)
+{
+    PRH_CHECK_ARG(op && order && out, "prh_probe_order: null pointer");
+    PRH_CHECK_ARG(op->uniform && op->kval, "prh_probe_order: the operator has no integer form K");
+    PRH_CHECK_ARG(reps >= 1 && reps <= 10000, "prh_probe_order: reps must be in [1, 10000]");
+    const int64_t n = op->n, nnz = op->nnz_s;
+    PRH_CHECK_ARG((uint64_t)n * 128 < ((uint64_t)1 << 32), "prh_probe_order: slot vectors beyond 4 GiB");
+    std::vector<int32_t> inv(n, -1);
+    for (int64_t p = 0; p < n; ++p) {
+        PRH_CHECK_ARG(order[p] >= 0 && order[p] < n && inv[order[p]] < 0, "prh_probe_order: not a permutation");
+        inv[order[p]] = (int32_t)p;
+    }
+    PRH_HIP(hipSetDevice(op->device));
+    hipStream_t st;
+    int rc = current_stream(&st);
+    if (rc) return rc;
+    std::vector<int64_t> rowptr(n + 1), prow(n + 1, 0);
+    std::vector<int32_t> col(nnz), pcol(nnz);
+    std::vector<uint16_t> kval(nnz), pval(nnz);
+    std::vector<double> d(n), pd(n);
+    PRH_HIP(hipMemcpy(rowptr.data(), op->rowptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
+    PRH_HIP(hipMemcpy(col.data(), op->col, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+    PRH_HIP(hipMemcpy(kval.data(), op->kval, sizeof(uint16_t) * nnz, hipMemcpyDeviceToHost));
+    PRH_HIP(hipMemcpy(d.data(), op->scale, sizeof(double) * n, hipMemcpyDeviceToHost));
+    // row p of the reordered matrix is row order[p], columns renumbered by inv and sorted
+    std::vector<std::pair<int32_t, uint16_t>> row;
+    for (int64_t p = 0; p < n; ++p) {
+        const int64_t i = order[p];
+        row.clear();
+        for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e) row.push_back({inv[col[e]], kval[e]});
+        std::sort(row.begin(), row.end());
+        prow[p + 1] = prow[p] + (int64_t)row.size();
+        for (size_t k = 0; k < row.size(); ++k) {
+            pcol[prow[p] + (int64_t)k] = row[k].first;
+            pval[prow[p] + (int64_t)k] = row[k].second;
+        }
+        pd[p] = d[i];
+    }
+    DeviceBuffers buf;
+    int64_t *d_rowptr = nullptr;
+    int32_t *d_col = nullptr, *d_order = nullptr;
+    uint16_t *d_val = nullptr;
+    double *d_d = nullptr;
+    _Float16 *z = nullptr, *zp = nullptr, *y = nullptr, *yp = nullptr, *work = nullptr;
+    const size_t vec = (size_t)n * 64;
+    if ((rc = buf.get(&d_rowptr, n + 1)) || (rc = buf.get(&d_col, nnz)) || (rc = buf.get(&d_val, nnz)) ||
+        (rc = buf.get(&d_d, n)) || (rc = buf.get(&d_order, n)) || (rc = buf.get(&z, vec)) ||
+        (rc = buf.get(&zp, vec)) || (rc = buf.get(&y, vec)) || (rc = buf.get(&yp, vec)) || (rc = buf.get(&work, vec)))
+        return rc;
+    PRH_HIP(hipMemcpy(d_rowptr, prow.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
+    PRH_HIP(hipMemcpy(d_col, pcol.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+    PRH_HIP(hipMemcpy(d_val, pval.data(), sizeof(uint16_t) * nnz, hipMemcpyHostToDevice));
+    PRH_HIP(hipMemcpy(d_d, pd.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    PRH_HIP(hipMemcpy(d_order, order, sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    const unsigned fill_grid = (unsigned)((vec + 255) / 256);
+    k_probe_fill_perm<<<fill_grid, 256, 0, st>>>(z, nullptr, n);
+    k_probe_fill_perm<<<fill_grid, 256, 0, st>>>(zp, d_order, n);
+    double t[4];
+    if ((rc = time_fp16x64(n, op->rowptr, op->col, op->kval, op->scale, op->inv_2R, op->delta, z, y, work, reps, st,
+                           t)))
+        return rc;
+    if ((rc = time_fp16x64(n, d_rowptr, d_col, d_val, d_d, op->inv_2R, op->delta, zp, yp, work, reps, st, t + 2)))
+        return rc;
+    // the reordered product is the original one, reordered (up to FP16 rounding of the outputs)
+    std::vector<_Float16> hy(vec), hyp(vec);
+    PRH_HIP(hipMemcpy(hy.data(), y, sizeof(_Float16) * vec, hipMemcpyDeviceToHost));
+    PRH_HIP(hipMemcpy(hyp.data(), yp, sizeof(_Float16) * vec, hipMemcpyDeviceToHost));
+    double worst = 0.0, largest = 0.0;
+    for (int64_t p = 0; p < n; ++p)
+        for (int s = 0; s < 64; ++s) {
+            const double a = (double)(float)hy[(size_t)order[p] * 64 + s], b = (double)(float)hyp[(size_t)p * 64 + s];
+            worst = std::max(worst, std::fabs(a - b));
+            largest = std::max(largest, std::fabs(a));
+        }
+    const double v[] = {t[0], t[1], t[2], t[3], largest > 0.0 ? worst / largest : worst};
+    for (int k = 0; k < n_out && k < 5; ++k) out[k] = v[k];
+    return PRH_OK;
 }
 
 int prh_probe_tiles(const prh_operator *op, int32_t min_fill, int32_t reps, double *out, int32_t n_out)

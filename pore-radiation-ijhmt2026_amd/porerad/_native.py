@@ -96,8 +96,10 @@ SIGNATURES = {
     "prh_operator_csr": ([_P, _P, _P, _P], ctypes.c_int),
     "prh_operator_reciprocity": ([_P, _P], ctypes.c_int),
     "prh_operator_stats": ([_P, _P, _I32], ctypes.c_int),
+    "prh_operator_set_row_order": ([_P, _P], ctypes.c_int),
     "prh_probe_gather": ([_P, _I32, _I32, _P], ctypes.c_int),
     "prh_probe_tiles": ([_P, _I32, _I32, _P, _I32], ctypes.c_int),
+    "prh_probe_order": ([_P, _P, _I32, _P, _I32], ctypes.c_int),
     "prh_batch_stats": ([_P, _P, _I32], ctypes.c_int),
     "prh_operator_apply": ([_P, _I32, _P, _P], ctypes.c_int),
     "prh_radiosity": ([_P, _I32, _P, _P, _I32, _F64, _I32, _I32, _P, _P, _P], ctypes.c_int),
@@ -615,6 +617,17 @@ class Operator:
         check(self._lib.prh_operator_reciprocity(self.handle, ctypes.byref(err)), "prh_operator_reciprocity")
         return float(err.value)
 
+    def set_row_order(self, order: np.ndarray | None) -> None:
+        """Sweep order of the batched engine's rows (a permutation of the patches), or None."""
+
+        if order is None:
+            check(self._lib.prh_operator_set_row_order(self.handle, None), "prh_operator_set_row_order")
+            return
+        order = np.ascontiguousarray(order, dtype=np.int32)
+        if order.shape != (self.n,):
+            raise ValueError("order must hold one patch index per patch")
+        check(self._lib.prh_operator_set_row_order(self.handle, ptr(order)), "prh_operator_set_row_order")
+
     def probe_gather(self, fmt: int, reps: int = 20) -> float:
         """Device ms of one F_p^T sweep over every slot in slot format ``fmt`` (a measurement probe)."""
 
@@ -630,6 +643,17 @@ class Operator:
         return {"plain_ms": float(out[0]), "tiles_ms": float(out[1]), "remainder_ms": float(out[2]),
                 "max_rel_difference": float(out[3]), "tiles": int(out[4]), "nnz_fraction_in_tiles": float(out[5]),
                 "remainder_nnz": int(out[6])}
+
+    def probe_order(self, order: np.ndarray, reps: int = 20) -> dict:
+        """FP16 x 64 sweep in the stored patch order and renumbered by ``order``, plain and streamed."""
+
+        order = np.ascontiguousarray(order, dtype=np.int32)
+        if order.shape != (self.n,):
+            raise ValueError("order must hold one patch index per patch")
+        out = np.zeros(5)
+        check(self._lib.prh_probe_order(self.handle, ptr(order), int(reps), ptr(out), 5), "prh_probe_order")
+        return {"stored_ms": float(out[0]), "stored_streamed_ms": float(out[1]), "reordered_ms": float(out[2]),
+                "reordered_streamed_ms": float(out[3]), "max_rel_difference": float(out[4])}
 
     def csr(self):
         """(rowptr, col, val) of the symmetric FP64 matrix S held on the device."""

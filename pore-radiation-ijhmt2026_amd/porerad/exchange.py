@@ -177,6 +177,15 @@ class ExchangeOperator:
 
         return self.device.apply(_native.APPLY_RMATVEC, x)
 
+    def set_sweep_order(self, order: np.ndarray | None) -> None:
+        """Order in which the batched engine sweeps this operator's rows
+        (``SurfacePatches.sweep_order()``, or None for the stored order).
+        Rows swept together by one XCD then gather from nearby patches and
+        share L2 lines; the engine's results change only in the rounding of
+        its norm sums."""
+
+        self.device.set_row_order(order)
+
     def reciprocity_error(self) -> float:
         """Largest relative defect of A_i F_p,ij = A_j F_p,ji over the count table's pairs (exchange.py:124-140).
 

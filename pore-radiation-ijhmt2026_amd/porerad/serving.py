@@ -156,8 +156,12 @@ class Geometry:
 
 
 def device_engine(geometry: Geometry):
+    """A 64-slot engine over the geometry's operator, sweeping its rows in the
+    Morton order of the patches (rows an XCD sweeps together share L2 lines)."""
+
     from . import _native
 
+    geometry.operator.set_sweep_order(geometry.patches.sweep_order())
     engine = _native.Batch(geometry.operator.device)
     engine.set_field(geometry.patches.patch_class, geometry.cell_x, geometry.nx, geometry.cross_section)
     return engine

@@ -64,6 +64,20 @@ class SurfacePatches:
     def n_patches(self) -> int:
         return int(self.patch_class.size)
 
+    def sweep_order(self) -> np.ndarray:
+        """The patches along a Morton (Z-order) curve of their origins on the
+        half-voxel grid: an order in which consecutive patches are close in
+        space, for the batched engine's row sweep (see
+        ``ExchangeOperator.set_sweep_order``)."""
+
+        ijk = np.floor(np.asarray(self.patch_origin) * 2.0).astype(np.int64)
+        ijk -= ijk.min(axis=0)
+        key = np.zeros(self.n_patches, np.int64)
+        for b in range(20):
+            key |= (((ijk[:, 0] >> b) & 1) << (3 * b + 2)) | (((ijk[:, 1] >> b) & 1) << (3 * b + 1)) | \
+                   (((ijk[:, 2] >> b) & 1) << (3 * b))
+        return np.argsort(key, kind="stable").astype(np.int32)
+
     def internal_owners(self) -> tuple[np.ndarray, np.ndarray]:
         """(mask of the internal patches, flat index of the solid cell owning each), built once."""
 

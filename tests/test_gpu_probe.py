@@ -49,3 +49,13 @@ def test_every_slot_format_sweeps(operator):
         assert operator.device.probe_gather(fmt, reps=2) > 0.0, name
     with pytest.raises(Exception):
         operator.device.probe_gather(99, reps=1)
+
+
+def test_reordered_sweep_equals_stored_sweep_reordered(operator):
+    rng = np.random.default_rng(21)
+    order = rng.permutation(operator.n_patches).astype(np.int32)
+    got = operator.device.probe_order(order, reps=2)
+    assert got["max_rel_difference"] <= 2e-3, got  # FP16 outputs, sums in another order
+    assert min(got["stored_ms"], got["reordered_ms"], got["stored_streamed_ms"]) > 0.0
+    with pytest.raises(Exception):
+        operator.device.probe_order(np.zeros(operator.n_patches, np.int32), reps=1)
