@@ -473,7 +473,7 @@ int prh_batch_bytes(const prh_batch *b, int64_t *bytes);
 /* Device time of the engine's ticks (HIP events around every tick, read as
  * the pipeline retires it): out[0..n) = ticks timed, their milliseconds,
  * slot-ticks (running slots summed over the ticks), nonzeros of S, patches,
- * bytes per matrix value read by a tick (2: integer K, 4: FP32 S).  No
+ * bytes per matrix value read by a tick (2: integer K, 8: FP64 S).  No
  * counterpart in the reference (measurement). */
 int prh_batch_stats(const prh_batch *b, double *out, int32_t n);
 /* Bit q set = slot q free (64 slots). */

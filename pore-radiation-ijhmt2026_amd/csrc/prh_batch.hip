@@ -131,16 +131,39 @@ __device__ __forceinline__ Z8 gather8(const _Float16 *z, int32_t j, int h)
     return *reinterpret_cast<const Z8 *>(reinterpret_cast<const char *>(z) + off);
 }
 
+enum : int32_t { FREE = 0, RUNNING = 1, FINISHING = 2, DONE = 3 };
+
+// Schedule of one slot, kept on the device and moved on by k_schedule before
+// every tick, so that ticks can be replayed from graphs without the host.
+struct DevSlot {
+    int32_t state;     // RUNNING until the device finds it converged (FINISHING)
+    int32_t refining;  // the next tick is the slot's refinement
+    int32_t refined;   // the tick just run was its refinement: its norm is in the totals
+    int32_t k, m;      // tick index within the epoch, epoch length
+    int32_t epochs;    // refinements made
+    int32_t ticks;     // Chebyshev ticks run
+    int32_t failed;    // stopped at MAX_EPOCHS without converging
+    double rtol, rho_max, e_norm, s, tau;
+};
+
+// What the next tick runs, written by k_schedule.
+struct TickCtl {
+    uint32_t group_mask;  // bit h: one of the slots 8h .. 8h+7 runs
+    int32_t any_refine;   // a slot refines (the tick writes norm partials)
+    int32_t running;      // slots that run
+    int32_t pad;
+    unsigned long long ticks, slot_ticks;  // ticks that ran a slot, and their running slots summed
+};
+
 struct TickArgs {
     MatView M;
     SlotVecs V;
     const _Float16 *z_in;
     _Float16 *z_out;
-    const SlotParam *param;       // [SLOTS]
-    double *part_abs, *part_sum;  // [SLOTS][n_blocks], written when a slot refines
+    const SlotParam *param;  // [SLOTS]
+    double *part_abs;        // [SLOTS][n_blocks], written when a slot refines
     int64_t n_blocks;
-    uint32_t group_mask;  // bit h: one of the slots 8h..8h+7 is not idle
-    int32_t any_refine;
+    const TickCtl *ctl;
 };
 
 // One tick: y = F_p^T z for every slot (FP64 accumulation of FP16 inputs),
@@ -149,8 +172,9 @@ struct TickArgs {
 template <bool UNIFORM>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_tick(TickArgs a)
 {
-    __shared__ double s_abs[ROWS][SLOTS], s_sum[ROWS][SLOTS];
+    __shared__ double s_abs[ROWS][SLOTS];
     const MatView &M = a.M;
+    const uint32_t group_mask = a.ctl->group_mask;
     const int wave = threadIdx.x >> 6;
     // one row per wavefront: the row (and its CSR range) is wave-uniform
     // the wave's sweep position, and its row (row_order keeps the rows an XCD
@@ -164,7 +188,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     double acc[PER];
 #pragma unroll
     for (int p = 0; p < PER; ++p) acc[p] = 0.0;
-    if (row < M.n && a.group_mask) {
+    if (row < M.n && group_mask) {
         // A tick makes one random cache-line request per nonzero: the 64 slots
         // of a row are one 128 B line, fetched by the 8 lanes of a group, 16 B
         // (slots 8h .. 8h+7) each.  Every lane loads a distinct column and
@@ -209,7 +233,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     double wsum = acc[0];
 #pragma unroll
     for (int p = 1; p < PER; ++p) wsum = g == p ? acc[p] : wsum;
-    double r_abs = 0.0, r_val = 0.0;
+    double r_abs = 0.0;
     const int q = PER * h + g;
     if (row < M.n) {
         const SlotParam P = a.param[q];
@@ -231,7 +255,6 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
                 V.y64[at] = yy;
                 V.r64[at] = r;
                 r_abs = fabs(r);
-                r_val = r;
                 // next epoch: x32 = 0, r32 = dir = r / s
                 const float rn = (float)(r * P.inv_s_next);
                 V.x32[at] = 0.f;
@@ -254,24 +277,104 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             }
         }
     }
-    if (!a.any_refine) return;  // the same for every thread of the grid
+    if (!a.ctl->any_refine) return;  // the same for every thread of the grid
     s_abs[wave][q] = r_abs;
-    s_sum[wave][q] = r_val;
     __syncthreads();
     if (threadIdx.x < SLOTS) {
-        double sa = 0.0, ss = 0.0;
-        for (int w = 0; w < ROWS; ++w) {
-            sa += s_abs[w][threadIdx.x];
-            ss += s_sum[w][threadIdx.x];
-        }
+        double sa = 0.0;
+        for (int w = 0; w < ROWS; ++w) sa += s_abs[w][threadIdx.x];
         a.part_abs[(size_t)threadIdx.x * a.n_blocks + blockIdx.x] = sa;
-        a.part_sum[(size_t)threadIdx.x * a.n_blocks + blockIdx.x] = ss;
+    }
+}
+
+// Before every tick, one thread per slot: take in the norm of a refinement
+// the last tick made (stop when ||r||_1 <= rtol ||E||_1, else size the next
+// epoch), then write the slot's instructions for the next tick.  A slot's
+// decisions read only its own state and norms.
+__global__ __launch_bounds__(SLOTS) void k_schedule(DevSlot *S, const double *tot, SlotParam *param, TickCtl *ctl)
+{
+    __shared__ uint32_t gm;
+    __shared__ int32_t ar, run;
+    if (threadIdx.x == 0) {
+        gm = 0u;
+        ar = 0;
+        run = 0;
+    }
+    __syncthreads();
+    const int q = threadIdx.x;
+    DevSlot st = S[q];
+    SlotParam p{MODE_IDLE, 0.f, 1.f, 0, 0.0, 0.0};
+    if (st.state == RUNNING && st.refined) {
+        const double r_norm = tot[q];
+        st.refined = 0;
+        st.epochs += 1;
+        const bool converged = r_norm <= st.rtol * st.e_norm;
+        if (converged || st.epochs >= MAX_EPOCHS) {
+            st.failed = converged ? 0 : 1;
+            st.state = FINISHING;
+        } else {
+            st.tau = epoch_target(st.rtol, st.e_norm, r_norm, ENGINE_REDUCTION);
+            st.m = ticks_for(st.rho_max, st.tau);
+        }
+    }
+    if (st.state == RUNNING) {
+        if (st.refining) {
+            // the epoch's correction is gathered now: refine, then the next epoch
+            const double s_next = st.s * st.tau;
+            p.mode = MODE_REFINE;
+            p.s = st.s;
+            st.s = s_next > 0.0 ? s_next : 1.0;
+            p.inv_s_next = 1.0 / st.s;
+            st.refining = 0;
+            st.refined = 1;
+            st.k = 0;
+            atomicOr(&ar, 1);
+        } else {
+            const bool last = st.m > 0 && st.k >= st.m - 1;
+            p.mode = last ? MODE_LAST : MODE_CHEB;
+            if (!last) cheb_coefficient(st.rho_max, st.k, &p.alpha, &p.beta);
+            st.k += 1;
+            st.ticks += 1;
+            st.refining = last ? 1 : 0;
+        }
+        atomicOr(&gm, 1u << (q / PER));
+        atomicAdd(&run, 1);
+    }
+    param[q] = p;
+    S[q] = st;
+    __syncthreads();
+    if (q == 0) {
+        ctl->group_mask = gm;
+        ctl->any_refine = ar;
+        ctl->running = run;
+        if (run) {
+            ctl->ticks += 1;
+            ctl->slot_ticks += (unsigned long long)run;
+        }
     }
 }
 
 // Fixed-order totals of columns of partial sums: out[c] = sum_b part[c][b].
 __global__ __launch_bounds__(256) void k_column_totals(const double *part, int64_t n_blocks, double *out)
 {
+    __shared__ double sh[256];
+    const int c = blockIdx.x;
+    double a = 0.0;
+    for (int64_t b = threadIdx.x; b < n_blocks; b += 256) a += part[(size_t)c * n_blocks + b];
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = sh[0];
+}
+
+// The refinement norms of a tick (column totals of its partials), when a slot refined.
+__global__ __launch_bounds__(256) void k_refine_totals(const double *part, int64_t n_blocks, const TickCtl *ctl,
+                                                       double *out)
+{
+    if (!ctl->any_refine) return;
     __shared__ double sh[256];
     const int c = blockIdx.x;
     double a = 0.0;
@@ -391,32 +494,25 @@ __global__ void k_collect(int64_t n, int q, const double *eps64, const double *y
 
 unsigned grid_of(int64_t n, int b) { return (unsigned)std::max<int64_t>(1, (n + b - 1) / b); }
 
-enum SlotState { FREE = 0, RUNNING, FINISHING, DONE };
-
+// Host view of a slot.  The host owns admission and retirement (FREE ->
+// RUNNING, FINISHING -> DONE -> FREE); the device moves RUNNING to FINISHING.
 struct SlotHost {
-    SlotState state = FREE;
-    bool failed = false;
+    int32_t state = FREE;
     bool field = false;
-    bool refining = false;     // the slot's next tick is its refinement
-    double rtol = 0, rho_max = 0;
-    double e_norm = 0;         // ||E||_1 (once known)
-    int64_t e_norm_step = -1;  // step whose completion brings the device ||E||_1
-    double r_norm = 0;         // ||r||_1 after the last refinement read
-    int epochs = 0;            // refinements read
-    int ticks = 0;             // Chebyshev ticks run
-    int k = 0;                 // tick index within the current epoch
-    int m = 0;                 // length of the current epoch (0 = not known yet)
-    double s = 1.0, tau = 0.5; // scale and target reduction of the current epoch
-    int64_t refine_step = -1;  // step of a refinement not yet read
-    int64_t finish_step = -1;  // step whose completion brings the results
+    bool failed = false;
+    int32_t iterations = 0;
+    int64_t admit_launch = 0;   // first launch that runs the slot
+    bool finish_queued = false; // k_finish queued, slot_ev recorded after it
 };
 
 struct Pinned {
-    SlotParam param[RING][SLOTS];
-    double tot[RING][2 * SLOTS];
-    double e_norm[SLOTS];
+    DevSlot init[SLOTS];          // admission states, copied to the device
+    DevSlot mirror[RING][SLOTS];  // device states after a launch
+    TickCtl ctl[RING];
     double fin[SLOTS][4];
 };
+
+constexpr int GRAPH_TICKS = 8;  // ticks of a long replay (even: keeps the z-buffer parity)
 
 }  // namespace
 }  // namespace prh
@@ -433,21 +529,30 @@ struct prh_batch {
     double *x64 = nullptr, *y64 = nullptr, *r64 = nullptr, *E64 = nullptr, *eps64 = nullptr;
     double *part = nullptr, *tot = nullptr, *stage = nullptr;
     SlotParam *param_dev = nullptr;
+    DevSlot *dslots = nullptr;
+    TickCtl *ctl = nullptr;
     int8_t *cls = nullptr;
     int32_t *cell_x = nullptr;
     int32_t field_nx = 0;
     double cross_section = 0.0;
     double mean_area = 0.0, max_area = 0.0;  // a_ref of the gathered form (unequal areas), largest area
     Pinned *host = nullptr;
+    // replays: [long][parity] = GRAPH_TICKS or 1 tick starting on z buffer `parity`
+    hipGraph_t graph[2][2] = {};
+    hipGraphExec_t exec[2][2] = {};
+    const int32_t *graph_order = nullptr;  // row order the graphs were captured with
+    bool graphs_ready = false;
+    // launches (replays): completion events, their tick counts and device time
     hipEvent_t ev[RING] = {};
-    bool ev_used[RING] = {};
-    // device time of the ticks: events around each tick, read one step later
     hipEvent_t tick_ev[RING][2] = {};
-    int tick_slots[RING] = {};  // running slots of the step's tick (0: none launched)
-    int64_t timed_ticks = 0, slot_ticks = 0;
+    int launch_ticks[RING] = {};
+    int64_t launches = 0;  // replays launched
+    int64_t read = 0;      // replays whose results have been read
+    hipEvent_t slot_ev[SLOTS] = {};
+    int64_t timed_ticks = 0;
+    unsigned long long slot_ticks = 0;
     double tick_ms = 0.0;
-    int64_t step = 0;   // steps taken
-    int64_t ticks = 0;  // ticks launched
+    int64_t ticks = 0;  // ticks launched (their parity selects the z buffer)
     SlotHost slot[SLOTS];
     int64_t bytes = 0;
 };
@@ -483,11 +588,24 @@ SlotVecs vecs_of(const prh_batch *b)
 
 FieldView field_of(const prh_batch *b) { return FieldView{b->cls, b->cell_x, (double)b->field_nx}; }
 
+void free_graphs(prh_batch *b)
+{
+    for (int l = 0; l < 2; ++l)
+        for (int p = 0; p < 2; ++p) {
+            if (b->exec[l][p]) (void)hipGraphExecDestroy(b->exec[l][p]);
+            if (b->graph[l][p]) (void)hipGraphDestroy(b->graph[l][p]);
+            b->exec[l][p] = nullptr;
+            b->graph[l][p] = nullptr;
+        }
+    b->graphs_ready = false;
+}
+
 void free_batch(prh_batch *b)
 {
-    void *ptrs[] = {b->x32, b->r32,  b->rho32, b->zbuf[0], b->zbuf[1], b->x64,    b->y64,
-                    b->r64, b->E64,  b->eps64, b->part,    b->tot,     b->stage,  b->cls,
-                    b->cell_x, b->param_dev};
+    free_graphs(b);
+    void *ptrs[] = {b->x32,    b->r32,   b->rho32,  b->zbuf[0], b->zbuf[1], b->x64,
+                    b->y64,    b->r64,   b->E64,    b->eps64,   b->part,    b->tot,
+                    b->stage,  b->cls,   b->cell_x, b->param_dev, b->dslots, b->ctl};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->host) (void)hipHostFree(b->host);
@@ -496,68 +614,152 @@ void free_batch(prh_batch *b)
         for (hipEvent_t e : b->tick_ev[k])
             if (e) (void)hipEventDestroy(e);
     }
+    for (hipEvent_t e : b->slot_ev)
+        if (e) (void)hipEventDestroy(e);
 }
 
 // The z buffer the next tick gathers.
 _Float16 *z_next(prh_batch *b) { return b->zbuf[b->ticks & 1]; }
 
-int wait_step(prh_batch *b, int64_t s)
+// Enqueue `count` ticks whose first gathers z buffer `parity`: per tick the
+// schedule, the sweep and the refinement norms.
+int enqueue_ticks(prh_batch *b, int count, int parity, hipStream_t st)
 {
-    if (s < 0 || s >= b->step) return PRH_OK;
-    const int k = (int)(s % RING);
-    if (!b->ev_used[k]) return PRH_OK;
-    PRH_HIP(hipEventSynchronize(b->ev[k]));
-    if (b->tick_slots[k] > 0) {
-        float ms = 0.f;
-        PRH_HIP(hipEventElapsedTime(&ms, b->tick_ev[k][0], b->tick_ev[k][1]));
-        b->tick_ms += ms;
-        b->timed_ticks += 1;
-        b->slot_ticks += b->tick_slots[k];
-        b->tick_slots[k] = 0;
+    TickArgs a;
+    a.M = view_of(b);
+    a.V = vecs_of(b);
+    a.param = b->param_dev;
+    a.part_abs = b->part;
+    a.n_blocks = b->n_blocks;
+    a.ctl = b->ctl;
+    for (int t = 0; t < count; ++t) {
+        a.z_in = b->zbuf[(parity + t) & 1];
+        a.z_out = b->zbuf[(parity + t + 1) & 1];
+        k_schedule<<<1, SLOTS, 0, st>>>(b->dslots, b->tot, b->param_dev, b->ctl);
+        if (b->op->uniform) k_tick<true><<<(unsigned)b->n_blocks, BT, 0, st>>>(a);
+        else k_tick<false><<<(unsigned)b->n_blocks, BT, 0, st>>>(a);
+        k_refine_totals<<<SLOTS, 256, 0, st>>>(b->part, b->n_blocks, b->ctl, b->tot);
     }
+    PRH_HIP(hipGetLastError());
     return PRH_OK;
 }
 
-// Everything queued up to the end of step s has completed on the device:
-// read what it produced and move the slot schedules on.
-void read_step(prh_batch *b, int64_t s)
+// Capture the four replays (GRAPH_TICKS or 1 tick, from either z buffer).
+int capture_graphs(prh_batch *b, hipStream_t st)
 {
-    for (int q = 0; q < SLOTS; ++q) {
-        SlotHost &S = b->slot[q];
-        if (S.state == FREE) continue;
-        if (S.e_norm_step >= 0 && S.e_norm_step <= s) {
-            S.e_norm = b->host->e_norm[q];
-            S.r_norm = S.e_norm;
-            S.e_norm_step = -1;
+    free_graphs(b);
+    PRH_HIP(hipStreamSynchronize(st));
+    for (int l = 0; l < 2; ++l)
+        for (int p = 0; p < 2; ++p) {
+            if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                set_error("prh_batch: stream capture did not start");
+                return PRH_ERR_HIP;
+            }
+            const int rc = enqueue_ticks(b, l ? GRAPH_TICKS : 1, p, st);
+            hipGraph_t g = nullptr;
+            const hipError_t end = hipStreamEndCapture(st, &g);
+            b->graph[l][p] = g;
+            if (rc) return rc;
+            if (end != hipSuccess || !g) {
+                set_error("prh_batch: graph capture failed: %s", hipGetErrorString(end));
+                return PRH_ERR_HIP;
+            }
+            if (hipGraphInstantiate(&b->exec[l][p], g, nullptr, nullptr, 0) != hipSuccess) {
+                set_error("prh_batch: graph instantiation failed");
+                return PRH_ERR_HIP;
+            }
         }
-        if (S.state == FINISHING && S.finish_step >= 0 && S.finish_step <= s) S.state = DONE;
-        if (S.state != RUNNING || S.refine_step < 0 || S.refine_step > s) continue;
-        S.r_norm = b->host->tot[S.refine_step % RING][q];
-        S.refine_step = -1;
-        S.epochs += 1;
-        const bool converged = S.r_norm <= S.rtol * S.e_norm;
-        if (converged || S.epochs >= MAX_EPOCHS) {
-            // the epoch that began with the refinement is abandoned
-            S.ticks -= S.k;
-            S.failed = !converged;
-            S.state = FINISHING;  // finish kernels are queued by the caller
-        } else {
-            // length of the epoch that began with the refinement
-            S.tau = epoch_target(S.rtol, S.e_norm, S.r_norm, ENGINE_REDUCTION);
-            S.m = ticks_for(S.rho_max, S.tau);
-        }
+    b->graph_order = b->op->row_order;
+    b->graphs_ready = true;
+    return PRH_OK;
+}
+
+// Replay `count` (GRAPH_TICKS or 1) ticks, then copy the slot states back.
+int launch(prh_batch *b, int count, hipStream_t st)
+{
+    if (!b->graphs_ready || b->graph_order != b->op->row_order) {
+        int rc = capture_graphs(b, st);
+        if (rc) return rc;
     }
+    const int r = (int)(b->launches % RING);
+    PRH_HIP(hipEventRecord(b->tick_ev[r][0], st));
+    PRH_HIP(hipGraphLaunch(b->exec[count == GRAPH_TICKS ? 1 : 0][b->ticks & 1], st));
+    PRH_HIP(hipEventRecord(b->tick_ev[r][1], st));
+    PRH_HIP(hipMemcpyAsync(b->host->mirror[r], b->dslots, sizeof(DevSlot) * SLOTS, hipMemcpyDeviceToHost, st));
+    PRH_HIP(hipMemcpyAsync(&b->host->ctl[r], b->ctl, sizeof(TickCtl), hipMemcpyDeviceToHost, st));
+    PRH_HIP(hipEventRecord(b->ev[r], st));
+    b->launch_ticks[r] = count;
+    b->ticks += count;
+    b->launches += 1;
+    return PRH_OK;
 }
 
 int queue_finish(prh_batch *b, int q, hipStream_t st)
 {
     const int64_t n = b->n;
     const unsigned grid = grid_of(n, 256);
+    // tot[0, SLOTS) holds refinement norms a later schedule still reads: the sums go above
     k_finish<<<grid, 256, 0, st>>>(n, q, field_of(b), vecs_of(b), (int64_t)grid, b->part);
-    k_column_totals<<<4, 256, 0, st>>>(b->part, (int64_t)grid, b->tot);
+    k_column_totals<<<4, 256, 0, st>>>(b->part, (int64_t)grid, b->tot + SLOTS);
     PRH_HIP(hipGetLastError());
-    PRH_HIP(hipMemcpyAsync(b->host->fin[q], b->tot, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+    PRH_HIP(hipMemcpyAsync(b->host->fin[q], b->tot + SLOTS, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+    PRH_HIP(hipEventRecord(b->slot_ev[q], st));
+    b->slot[q].finish_queued = true;
     return PRH_OK;
+}
+
+// Read the results of launch L (in order): device time, and the slots the
+// device found converged, whose results are then summed.
+int read_launch(prh_batch *b, int64_t L, hipStream_t st)
+{
+    const int r = (int)(L % RING);
+    PRH_HIP(hipEventSynchronize(b->ev[r]));
+    float ms = 0.f;
+    PRH_HIP(hipEventElapsedTime(&ms, b->tick_ev[r][0], b->tick_ev[r][1]));
+    b->tick_ms += ms;
+    b->timed_ticks += b->launch_ticks[r];
+    b->slot_ticks = b->host->ctl[r].slot_ticks;
+    for (int q = 0; q < SLOTS; ++q) {
+        SlotHost &S = b->slot[q];
+        if (S.state != RUNNING || S.admit_launch > L) continue;
+        const DevSlot &D = b->host->mirror[r][q];
+        if (D.state != FINISHING) continue;
+        S.state = FINISHING;
+        S.failed = D.failed != 0;
+        S.iterations = D.ticks + D.epochs;
+        int rc = queue_finish(b, q, st);
+        if (rc) return rc;
+    }
+    b->read = L + 1;
+    return PRH_OK;
+}
+
+// Finished slots whose sums have arrived become DONE (waiting for them if `wait`).
+int settle(prh_batch *b, bool wait)
+{
+    for (int q = 0; q < SLOTS; ++q) {
+        SlotHost &S = b->slot[q];
+        if (S.state != FINISHING || !S.finish_queued) continue;
+        if (wait) PRH_HIP(hipEventSynchronize(b->slot_ev[q]));
+        const hipError_t e = hipEventQuery(b->slot_ev[q]);
+        if (e == hipSuccess) {
+            S.state = DONE;
+        } else if (e != hipErrorNotReady) {
+            set_error("prh_batch: %s", hipGetErrorString(e));
+            return PRH_ERR_HIP;
+        }
+    }
+    return PRH_OK;
+}
+
+// Wait for everything launched so far and read it.
+int batch_drain(prh_batch *b, hipStream_t st)
+{
+    for (int64_t L = b->read; L < b->launches; ++L) {
+        int rc = read_launch(b, L, st);
+        if (rc) return rc;
+    }
+    return settle(b, true);
 }
 
 int first_free(const prh_batch *b)
@@ -565,101 +767,6 @@ int first_free(const prh_batch *b)
     for (int k = 0; k < SLOTS; ++k)
         if (b->slot[k].state == FREE) return k;
     return -1;
-}
-
-// One pipelined step: launch the next tick of every running slot, then wait
-// for the previous step and read its results.
-int batch_step(prh_batch *b, hipStream_t st, bool *launched)
-{
-    const int64_t s = b->step;
-    SlotParam *P = b->host->param[s % RING];
-    uint32_t group_mask = 0;
-    int any = 0, any_refine = 0, running = 0;
-    for (int q = 0; q < SLOTS; ++q) {
-        SlotHost &S = b->slot[q];
-        SlotParam &p = P[q];
-        p = SlotParam{MODE_IDLE, 0.f, 1.f, 0, 0.0, 0.0};
-        if (S.state != RUNNING) continue;
-        if (S.refining) {
-            // the epoch's correction is gathered now: refine, then the next epoch
-            const double s_next = S.s * S.tau;
-            p.mode = MODE_REFINE;
-            p.s = S.s;
-            S.s = s_next > 0.0 ? s_next : 1.0;
-            p.inv_s_next = 1.0 / S.s;
-            S.refining = false;
-            S.refine_step = s;
-            S.k = 0;
-            S.m = 0;  // known once this refinement is read, one step later
-            any_refine = 1;
-        } else {
-            // With the one-step lag the epoch length is always known by k = 1.
-            const bool last = S.m > 0 && S.k >= S.m - 1;
-            p.mode = last ? MODE_LAST : MODE_CHEB;
-            if (!last) cheb_coefficient(S.rho_max, S.k, &p.alpha, &p.beta);
-            S.k += 1;
-            S.ticks += 1;
-            S.refining = last;
-        }
-        group_mask |= 1u << (q / PER);
-        any = 1;
-        running += 1;
-    }
-    *launched = any != 0;
-    if (any) {
-        SlotParam *dev = b->param_dev + (s % RING) * SLOTS;
-        PRH_HIP(hipMemcpyAsync(dev, P, SLOTS * sizeof(SlotParam), hipMemcpyHostToDevice, st));
-        TickArgs a;
-        a.M = view_of(b);
-        a.V = vecs_of(b);
-        a.z_in = b->zbuf[b->ticks & 1];
-        a.z_out = b->zbuf[(b->ticks + 1) & 1];
-        a.param = dev;
-        a.part_abs = b->part;
-        a.part_sum = b->part + (size_t)SLOTS * b->n_blocks;
-        a.n_blocks = b->n_blocks;
-        a.group_mask = group_mask;
-        a.any_refine = any_refine;
-        hipEvent_t *tev = b->tick_ev[s % RING];
-        PRH_HIP(hipEventRecord(tev[0], st));
-        if (b->op->uniform) k_tick<true><<<(unsigned)b->n_blocks, BT, 0, st>>>(a);
-        else k_tick<false><<<(unsigned)b->n_blocks, BT, 0, st>>>(a);
-        PRH_HIP(hipGetLastError());
-        PRH_HIP(hipEventRecord(tev[1], st));
-        b->tick_slots[s % RING] = running;
-        b->ticks += 1;
-        if (any_refine) {
-            k_column_totals<<<2 * SLOTS, 256, 0, st>>>(b->part, b->n_blocks, b->tot);
-            PRH_HIP(hipGetLastError());
-            PRH_HIP(hipMemcpyAsync(b->host->tot[s % RING], b->tot, 2 * SLOTS * sizeof(double),
-                                   hipMemcpyDeviceToHost, st));
-        }
-    }
-    // this step's tick is queued behind the previous one: read the previous one
-    int rc = wait_step(b, s - 1);
-    if (rc) return rc;
-    read_step(b, s - 1);
-    for (int q = 0; q < SLOTS; ++q) {
-        SlotHost &S = b->slot[q];
-        if (S.state == FINISHING && S.finish_step < 0) {
-            if ((rc = queue_finish(b, q, st))) return rc;
-            S.finish_step = s;
-        }
-    }
-    PRH_HIP(hipEventRecord(b->ev[s % RING], st));
-    b->ev_used[s % RING] = true;
-    b->step = s + 1;
-    return PRH_OK;
-}
-
-// Wait for everything queued so far and read it.
-int batch_drain(prh_batch *b)
-{
-    if (b->step == 0) return PRH_OK;
-    int rc = wait_step(b, b->step - 1);
-    if (rc) return rc;
-    read_step(b, b->step - 1);
-    return PRH_OK;
 }
 
 void report(const prh_batch *b, uint64_t *done_mask, uint64_t *active_mask)
@@ -673,8 +780,33 @@ void report(const prh_batch *b, uint64_t *done_mask, uint64_t *active_mask)
     if (active_mask) *active_mask = active;
 }
 
-}  // namespace
+// Admission state of a slot: RUNNING from the next launch on.
+void admit_slot(prh_batch *b, int q, double rtol, double rho_max, double e_norm, double s0, bool field,
+                hipStream_t st, int *rc)
+{
+    DevSlot &D = b->host->init[q];
+    D = DevSlot{};
+    D.state = RUNNING;
+    D.rtol = rtol;
+    D.rho_max = rho_max;
+    D.e_norm = e_norm;
+    D.s = s0;
+    D.tau = epoch_target(rtol, 1.0, 1.0, ENGINE_REDUCTION);
+    D.m = ticks_for(rho_max, D.tau);
+    *rc = PRH_OK;
+    if (hipMemcpyAsync(&b->dslots[q], &D, sizeof(DevSlot), hipMemcpyHostToDevice, st) != hipSuccess) {
+        set_error("prh_batch: slot admission copy failed");
+        *rc = PRH_ERR_HIP;
+        return;
+    }
+    SlotHost &S = b->slot[q];
+    S = SlotHost();
+    S.state = RUNNING;
+    S.field = field;
+    S.admit_launch = b->launches;
+}
 
+}  // namespace
 
 extern "C" {
 
@@ -711,18 +843,22 @@ int prh_batch_create(prh_operator *op, prh_batch **out)
     alloc((void **)&b->r64, ns * 8);
     alloc((void **)&b->E64, ns * 8);
     alloc((void **)&b->eps64, ns * 8);
-    // partial sums: refinement norms [2 SLOTS][n_blocks], and the row-block
+    // partial sums: refinement norms [SLOTS][n_blocks], and the row-block
     // sums of admissions (|E|, tail of the array) and finishes ([4][row_blocks])
-    alloc((void **)&b->part, (size_t)std::max<int64_t>(2 * SLOTS * b->n_blocks, 4 * row_blocks) * 8);
+    alloc((void **)&b->part, (size_t)std::max<int64_t>(SLOTS * b->n_blocks, 4 * row_blocks) * 8);
     alloc((void **)&b->tot, 2 * SLOTS * 8);
     alloc((void **)&b->stage, (size_t)op->n * 2 * 8);
-    alloc((void **)&b->param_dev, (size_t)RING * SLOTS * sizeof(SlotParam));
+    alloc((void **)&b->param_dev, SLOTS * sizeof(SlotParam));
+    alloc((void **)&b->dslots, SLOTS * sizeof(DevSlot));
+    alloc((void **)&b->ctl, sizeof(TickCtl));
     if (ok && hipHostMalloc((void **)&b->host, sizeof(Pinned), hipHostMallocDefault) != hipSuccess) ok = false;
     for (int k = 0; ok && k < RING; ++k) {
         if (hipEventCreateWithFlags(&b->ev[k], hipEventDisableTiming) != hipSuccess) ok = false;
         for (hipEvent_t &e : b->tick_ev[k])
             if (ok && hipEventCreate(&e) != hipSuccess) ok = false;
     }
+    for (int q = 0; ok && q < SLOTS; ++q)
+        if (hipEventCreateWithFlags(&b->slot_ev[q], hipEventDisableTiming) != hipSuccess) ok = false;
     if (!ok) {
         free_batch(b);
         delete b;
@@ -732,6 +868,9 @@ int prh_batch_create(prh_operator *op, prh_batch **out)
     // Zero the slot state so idle slots carry finite values through every sweep.
     for (void *p : {(void *)b->x32, (void *)b->r32, (void *)b->rho32}) PRH_HIP(hipMemset(p, 0, ns * 4));
     for (void *p : {(void *)b->zbuf[0], (void *)b->zbuf[1]}) PRH_HIP(hipMemset(p, 0, ns * 2));
+    PRH_HIP(hipMemset(b->dslots, 0, SLOTS * sizeof(DevSlot)));  // every slot FREE
+    PRH_HIP(hipMemset(b->ctl, 0, sizeof(TickCtl)));
+    PRH_HIP(hipMemset(b->tot, 0, 2 * SLOTS * 8));
     // a_ref of the gathered form (mean area) and the largest area
     {
         std::vector<double> area(op->n);
@@ -745,8 +884,6 @@ int prh_batch_create(prh_operator *op, prh_batch **out)
         b->max_area = largest;
         if (!(b->mean_area > 0.0)) b->mean_area = 1.0;
     }
-    for (void *p : {(void *)b->x64, (void *)b->y64, (void *)b->r64, (void *)b->E64, (void *)b->eps64})
-        PRH_HIP(hipMemset(p, 0, ns * 8));
     *out = b;
     return PRH_OK;
 }
@@ -762,7 +899,7 @@ int prh_batch_stats(const prh_batch *b, double *out, int32_t n)
 {
     PRH_CHECK_ARG(b && out && n >= 0, "prh_batch_stats: null pointer");
     const double v[] = {(double)b->timed_ticks, b->tick_ms, (double)b->slot_ticks, (double)b->op->nnz_s,
-                        (double)b->n, b->op->uniform ? 2.0 : 4.0};
+                        (double)b->n, b->op->uniform ? 2.0 : 8.0};
     for (int i = 0; i < n && i < (int)(sizeof(v) / sizeof(v[0])); ++i) out[i] = v[i];
     return PRH_OK;
 }
@@ -792,7 +929,7 @@ int prh_batch_set_field(prh_batch *b, const int8_t *patch_class, const int32_t *
     hipStream_t st;
     int rc = current_stream(&st);
     if (rc) return rc;
-    if ((rc = batch_drain(b))) return rc;
+    if ((rc = batch_drain(b, st))) return rc;
     if (!b->cls) {
         PRH_HIP(hipMalloc((void **)&b->cls, n));
         PRH_HIP(hipMalloc((void **)&b->cell_x, n * 4));
@@ -837,23 +974,18 @@ int prh_batch_admit(prh_batch *b, const double *emitted, const double *emissivit
     k_admit<<<grid_of(n, 256), 256, 0, st>>>(view_of(b), vecs_of(b), z_next(b), q, b->stage, b->stage + n,
                                              eps_per_patch, 1.0 / s0, b->op->uniform ? 1 : 0);
     PRH_HIP(hipGetLastError());
-    // the caller's arrays and the staging buffer are free again once this returns
-    PRH_HIP(hipStreamSynchronize(st));
-    SlotHost &S = b->slot[q];
-    S = SlotHost();
-    S.state = RUNNING;
-    S.rtol = relative_tolerance;
-    S.rho_max = rho_max;
-    S.e_norm = e_norm;
-    S.r_norm = e_norm;
-    S.s = s0;
-    S.tau = epoch_target(S.rtol, 1.0, 1.0, ENGINE_REDUCTION);
-    S.m = ticks_for(rho_max, S.tau);
     if (e_norm == 0.0) {
-        // nothing to iterate: the zero solution is exact
+        // nothing to iterate: the zero solution is exact (the slot never runs on the device)
+        SlotHost &S = b->slot[q];
+        S = SlotHost();
         S.state = DONE;
         for (int c = 0; c < 4; ++c) b->host->fin[q][c] = 0.0;
+    } else {
+        admit_slot(b, q, relative_tolerance, rho_max, e_norm, s0, false, st, &rc);
+        if (rc) return rc;
     }
+    // the caller's arrays and the staging buffer are free again once this returns
+    PRH_HIP(hipStreamSynchronize(st));
     *slot_out = q;
     return PRH_OK;
 }
@@ -886,23 +1018,14 @@ int prh_batch_admit_linear(prh_batch *b, double T_hot_K, double T_cold_K, double
     if (!(s0 > 0.0) || !std::isfinite(s0)) s0 = 1.0;
     const int64_t n = b->n;
     const unsigned grid = grid_of(n, 256);
-    double *part = b->part + (size_t)2 * SLOTS * b->n_blocks - grid;
+    // the slot's schedule first; ||E||_1, summed on the device, then lands in it
+    admit_slot(b, q, relative_tolerance, 1.0 - emissivity, 0.0, s0, true, st, &rc);
+    if (rc) return rc;
+    double *part = b->part + (size_t)SLOTS * b->n_blocks - grid;
     k_admit_linear<<<grid, 256, 0, st>>>(view_of(b), vecs_of(b), field_of(b), z_next(b), q, emissivity,
                                          emissivity * SIGMA, T_m, dT, 1.0 / s0, b->op->uniform ? 1 : 0, part);
-    k_column_totals<<<1, 256, 0, st>>>(part, (int64_t)grid, b->tot + 2 * SLOTS - 1);
+    k_column_totals<<<1, 256, 0, st>>>(part, (int64_t)grid, &b->dslots[q].e_norm);
     PRH_HIP(hipGetLastError());
-    PRH_HIP(hipMemcpyAsync(&b->host->e_norm[q], b->tot + 2 * SLOTS - 1, sizeof(double), hipMemcpyDeviceToHost,
-                           st));
-    SlotHost &S = b->slot[q];
-    S = SlotHost();
-    S.state = RUNNING;
-    S.field = true;
-    S.rtol = relative_tolerance;
-    S.rho_max = 1.0 - emissivity;
-    S.e_norm_step = b->step;  // the next step's completion covers this admission
-    S.s = s0;
-    S.tau = epoch_target(S.rtol, 1.0, 1.0, ENGINE_REDUCTION);
-    S.m = ticks_for(S.rho_max, S.tau);
     *slot_out = q;
     return PRH_OK;
 }
@@ -915,17 +1038,24 @@ int prh_batch_step(prh_batch *b, int32_t max_ticks, uint64_t *done_mask, uint64_
     hipStream_t st;
     int rc = current_stream(&st);
     if (rc) return rc;
-    for (int t = 0; t < max_ticks; ++t) {
-        bool launched = false;
-        if ((rc = batch_step(b, st, &launched))) return rc;
-        if (!launched) {
-            // nothing runs: let the queued finish work complete
-            if ((rc = batch_drain(b))) return rc;
-            bool finishing = false;
-            for (int q = 0; q < SLOTS; ++q) finishing |= b->slot[q].state == FINISHING;
-            if (!finishing) break;
-            continue;
+    // Replays are pipelined: replay L+1 is queued before the results of
+    // replay L are read, so the device always has the next ticks.
+    int32_t t = 0;
+    for (;;) {
+        bool running = false;
+        for (int q = 0; q < SLOTS; ++q) running |= b->slot[q].state == RUNNING;
+        if (!running) {
+            // nothing runs: read everything launched and let the finishes land
+            if ((rc = batch_drain(b, st))) return rc;
+            break;
         }
+        if (t >= max_ticks) break;  // the last replay stays in flight
+        const int count = max_ticks - t >= GRAPH_TICKS ? GRAPH_TICKS : 1;
+        if ((rc = launch(b, count, st))) return rc;
+        t += count;
+        for (int64_t L = b->read; L < b->launches - 1; ++L)
+            if ((rc = read_launch(b, L, st))) return rc;
+        if ((rc = settle(b, false))) return rc;
         uint64_t done = 0;
         report(b, &done, nullptr);
         if (done) break;
@@ -948,7 +1078,7 @@ int prh_batch_channels(prh_batch *b, int32_t slot, double *q_W_m2, double *resid
     const double *f = b->host->fin[slot];
     for (int c = 0; c < 3; ++c) q_W_m2[c] = -f[c] / b->cross_section;
     *residual = f[3];
-    *iterations = S.ticks + S.epochs;
+    *iterations = S.iterations;
     return PRH_OK;
 }
 
@@ -972,7 +1102,7 @@ int prh_batch_result(prh_batch *b, int32_t slot, double *absorbed, double *resid
     PRH_HIP(hipMemcpyAsync(absorbed, b->stage, b->n * 8, hipMemcpyDeviceToHost, st));
     PRH_HIP(hipStreamSynchronize(st));
     *residual = b->host->fin[slot][3];
-    *iterations = S.ticks + S.epochs;
+    *iterations = S.iterations;
     S = SlotHost();
     return PRH_OK;
 }

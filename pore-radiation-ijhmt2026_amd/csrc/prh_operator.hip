@@ -1016,6 +1016,13 @@ int prh_operator_set_row_order(prh_operator *op, const int32_t *order)
 {
     PRH_CHECK_ARG(op, "prh_operator_set_row_order: null operator");
     PRH_HIP(hipSetDevice(op->device));
+    {
+        // sweeps already queued read the current order: let them finish first
+        hipStream_t st;
+        int rc = current_stream(&st);
+        if (rc) return rc;
+        PRH_HIP(hipStreamSynchronize(st));
+    }
     if (!order) {
         if (op->row_order) (void)hipFree(op->row_order);
         op->row_order = nullptr;

@@ -16,29 +16,32 @@ constexpr int MAX_EPOCHS = 40;
 constexpr double INNER_REDUCTION = 1.0e-6;
 
 // Chebyshev iterations that shrink the error bound 2 c^k / (1 + c^2k) below
-// tau (at least 2: the epoch length is decided one tick after it starts).
-inline int ticks_for(double rho_max, double tau)
+// tau (at least 2).  Host and device (the batched engine schedules its slots
+// on the device).
+__host__ __device__ inline int ticks_for(double rho_max, double tau)
 {
     if (rho_max <= 0.0) return 2;
     const double kappa = (1.0 + rho_max) / (1.0 - rho_max);
-    const double c = (std::sqrt(kappa) - 1.0) / (std::sqrt(kappa) + 1.0);
-    const int m = (int)std::ceil(std::log(tau / 2.0) / std::log(c));
-    return std::min(std::max(m, 2), MAX_TICKS);
+    const double c = (sqrt(kappa) - 1.0) / (sqrt(kappa) + 1.0);
+    const int m = (int)ceil(log(tau / 2.0) / log(c));
+    return m < 2 ? 2 : (m > MAX_TICKS ? MAX_TICKS : m);
 }
 
 // Reduction an epoch aims for: what is left to reach the tolerance, with a
 // factor 10 of margin, but no finer than the iterations resolve (inner:
 // INNER_REDUCTION with FP32 directions).
-inline double epoch_target(double rtol, double e_norm, double r_norm, double inner = INNER_REDUCTION)
+__host__ __device__ inline double epoch_target(double rtol, double e_norm, double r_norm,
+                                               double inner = INNER_REDUCTION)
 {
-    const double left = 0.1 * rtol * e_norm / std::max(r_norm, 1e-300);
-    return std::min(std::max(left, inner), 0.5);
+    const double left = 0.1 * rtol * e_norm / (r_norm > 1e-300 ? r_norm : 1e-300);
+    const double t = left > inner ? left : inner;
+    return t < 0.5 ? t : 0.5;
 }
 
 // alpha_k, beta_k of Chebyshev acceleration on [1 - delta, 1 + delta]:
 // rho_0 = delta, rho_{k+1} = 1 / (2/delta - rho_k),
 // dir_{k+1} = rho_{k+1} rho_k dir_k + (2 rho_{k+1} / delta) r_{k+1}.
-inline void cheb_coefficient(double delta, int k, float *alpha, float *beta)
+__host__ __device__ inline void cheb_coefficient(double delta, int k, float *alpha, float *beta)
 {
     if (delta <= 0.0) {
         *alpha = 0.f;

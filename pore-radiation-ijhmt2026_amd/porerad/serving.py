@@ -45,7 +45,7 @@ from .constants import SIGMA
 from .view_factors import COLD, HOT, INTERNAL
 
 SLOTS = 64
-RING = 8  # sweeps the engine keeps in flight (parameter sets held on the device)
+DEV_SLOT_BYTES = 72  # device schedule state of one slot
 
 
 @dataclasses.dataclass
@@ -197,12 +197,13 @@ def device_coupled_bytes(geometry: Geometry) -> int:
 
 def device_engine_bytes(geometry: Geometry) -> int:
     """Device bytes of one engine: 64 slots x (3 FP32 + 2 FP16 + 5 FP64 vectors) per patch, partial
-    sums and totals, staging (2 FP64 per patch), field (5 B per patch), the per-sweep slot parameters."""
+    sums and totals, staging (2 FP64 per patch), field (5 B per patch), the slots' parameters and
+    schedule states, and the sweep control word."""
 
     n = geometry.n
     blocks = ((n + 3) // 4 + 7) // 8 * 8
-    return (n * SLOTS * (3 * 4 + 2 * 2 + 5 * 8) + max(2 * SLOTS * blocks, 4 * ((n + 255) // 256)) * 8
-            + 2 * SLOTS * 8 + n * 21 + RING * SLOTS * 32)
+    return (n * SLOTS * (3 * 4 + 2 * 2 + 5 * 8) + max(SLOTS * blocks, 4 * ((n + 255) // 256)) * 8
+            + 2 * SLOTS * 8 + n * 21 + SLOTS * 32 + SLOTS * DEV_SLOT_BYTES + 32)
 
 
 def device_tracer_bytes() -> int:
