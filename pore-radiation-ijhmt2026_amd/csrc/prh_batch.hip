@@ -1142,8 +1142,10 @@ int prh_radiosity_mixed(prh_operator *op, int32_t n_rhs, const double *emitted, 
             rhs_of_slot[q] = next++;
             free_mask &= ~(uint64_t(1) << q);
         }
+        // one tick per step while emissions wait for slots (a finished slot is
+        // refilled at once); long replays once every emission is in
         uint64_t done = 0, active = 0;
-        if ((rc = prh_batch_step(b, MAX_TICKS * MAX_EPOCHS, &done, &active))) return rc;
+        if ((rc = prh_batch_step(b, next < n_rhs ? 1 : MAX_TICKS * MAX_EPOCHS, &done, &active))) return rc;
         for (int q = 0; q < SLOTS; ++q) {
             if (!((done >> q) & 1u)) continue;
             const int k = rhs_of_slot[q];

@@ -223,10 +223,16 @@ class RadiosityServer:
                  clock: Callable[[], float] = time.perf_counter,
                  coupled_runner: Callable[[Geometry, CoupledRequest], tuple] = device_coupled_runner,
                  coupled_bytes: Callable[[Geometry], int] = device_coupled_bytes,
-                 other_bytes: Callable[[], int] = device_tracer_bytes) -> None:
+                 other_bytes: Callable[[], int] = device_tracer_bytes,
+                 sweeps_per_step: int = 1) -> None:
         """``other_bytes`` reports device memory held outside the server that counts against the budget
-        (by default the tracer's workspace)."""
+        (by default the tracer's workspace).  ``sweeps_per_step``: sweeps each busy engine runs per
+        scheduler step (one replay of that many ticks); 1 admits and retires at every sweep, more
+        saves host round trips at the cost of that granularity."""
 
+        if sweeps_per_step < 1:
+            raise ValueError("sweeps_per_step must be at least 1")
+        self.sweeps_per_step = int(sweeps_per_step)
         self.budget = hbm_budget_bytes
         self.other_size = other_bytes
         self.engine_factory = engine_factory
@@ -365,8 +371,8 @@ class RadiosityServer:
             busy = self.slots[key]
             if not busy:
                 continue
-            done, _ = engine.step(1)
-            self.sweeps += 1
+            done, _ = engine.step(self.sweeps_per_step)
+            self.sweeps += self.sweeps_per_step
             g = self.geometries[key]
             for slot in list(busy):
                 if not (done >> slot) & 1:

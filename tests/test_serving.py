@@ -24,6 +24,7 @@ class StandInEngine:
         self.epochs_needed = epochs_needed
         self.closed = False
         self.log = []
+        self.calls = []
 
     def admit_linear(self, T_hot, T_cold, eps, rtol):
         free = [q for q in range(serving.SLOTS) if q not in self.busy]
@@ -36,11 +37,16 @@ class StandInEngine:
         return q
 
     def step(self, max_ticks=1):
-        done = 0
-        for q, rec in self.busy.items():
-            rec[1] += 1
-            if rec[1] >= self.epochs_needed:
-                done |= 1 << q
+        # up to max_ticks sweeps, returning early once a slot has finished (the engine's contract)
+        self.calls.append(max_ticks)
+        for _ in range(max_ticks):
+            done = 0
+            for q, rec in self.busy.items():
+                rec[1] += 1
+                if rec[1] >= self.epochs_needed:
+                    done |= 1 << q
+            if done:
+                break
         return done, sum(1 << q for q in self.busy)
 
     def channels(self, q):
@@ -261,3 +267,20 @@ def test_memory_held_outside_the_server_counts_against_the_budget():
     while srv.pending():
         srv.step()
     assert len(srv.replies) == 2
+
+
+def test_sweeps_per_step_reaches_the_engine():
+    P = _patches()
+    engines = []
+
+    def factory(g):
+        engines.append(StandInEngine(g, epochs_needed=5))
+        return engines[-1]
+
+    srv = serving.RadiosityServer(engine_factory=factory, clock=Clock(), sweeps_per_step=3)
+    srv.add_geometry("a", P, operator=None)
+    replies = srv.run([serving.RadiosityRequest("a", 700.0, 300.0, 0.5)], real_time=False)
+    assert len(replies) == 1 and set(engines[0].calls) == {3}
+    assert srv.sweeps == 3 * len(engines[0].calls)
+    with pytest.raises(ValueError):
+        serving.RadiosityServer(sweeps_per_step=0)
