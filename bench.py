@@ -737,6 +737,12 @@ def run_sweep_probes(R, reps=20) -> dict:
         orders[name] = o
         log(f"patch order {name}: stored {o['stored_ms']:.3f} ms (streamed matrix {o['stored_streamed_ms']:.3f}), "
             f"reordered {o['reordered_ms']:.3f} ms (streamed {o['reordered_streamed_ms']:.3f})")
+    panels = {}
+    for k in (4, 8, 16):
+        pn = op.device.probe_panels(k, reps)
+        panels[f"panels_{k}"] = pn
+        log(f"column panels {k}: plain {pn['plain_ms']:.3f} ms, panels {pn['panels_ms']:.3f} ms "
+            f"(sweep order {pn['panels_sweep_order_ms']:.3f} ms)")
     tiles = {}
     for fill in (4, 8, 16):
         t = op.device.probe_tiles(fill, reps)
@@ -747,7 +753,7 @@ def run_sweep_probes(R, reps=20) -> dict:
             f"{t['plain_ms']:.3f} ms vs tiles {t['tiles_ms']:.3f} + remainder {t['remainder_ms']:.3f} ms")
     op.device.close()
     return {"workload": f"foam N=128, {P.n_patches} patches, {R} rays/patch (seed 3)", "reps": reps,
-            "formats": formats, "patch_orders": orders, "mfma_tiles": tiles}
+            "formats": formats, "patch_orders": orders, "column_panels": panels, "mfma_tiles": tiles}
 
 
 def patch_orders(P) -> dict:

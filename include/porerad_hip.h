@@ -442,6 +442,14 @@ int prh_probe_tiles(const prh_operator *op, int32_t min_fill, int32_t reps, doub
  * largest difference between the reordered product and the stored one
  * reordered, relative to the largest product (FP16 outputs). */
 int prh_probe_order(const prh_operator *op, const int32_t *order, int32_t reps, double *out, int32_t n_out);
+/* prh_probe_panels: the FP16 x 64 sweep in n_panels phases, phase k
+ * gathering only the columns [k W, (k + 1) W) (W = ceil(n / n_panels)) with
+ * FP64 accumulators kept in HBM between phases.  out[0..n_out) = ms of the
+ * plain sweep, of the panel sweep in the stored row order and in the
+ * operator's row order (prh_operator_set_row_order; the stored order when
+ * none is set), and the largest difference between the panel and the plain
+ * product relative to the largest (FP16 outputs). */
+int prh_probe_panels(const prh_operator *op, int32_t n_panels, int32_t reps, double *out, int32_t n_out);
 
 /* Reflection series J = E + (1 - eps) F_p^T J for n_rhs emissions.
  * emitted: n_rhs x n (row-major, one emission per row); emissivity:

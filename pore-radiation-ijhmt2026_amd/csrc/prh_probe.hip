@@ -321,6 +321,72 @@ int time_fp16x64(int64_t n, const int64_t *rowptr, const int32_t *col, const uin
     return PRH_OK;
 }
 
+// Column-panel sweep, phase `panel` of n_panels: every row gathers only its
+// entries with columns in the panel (pp[row][panel] .. pp[row][panel + 1]),
+// adding to FP64 accumulators kept in HBM between phases; the last phase
+// writes the FP16 product.  All rows of a phase read one slab of the gathered
+// vector, small enough for each XCD's L2.  Rows are taken in `order` (null:
+// stored order).
+__global__ __launch_bounds__(PT) void k_probe_panel(ProbeView M, const int64_t *pp, int n_panels, int panel,
+                                                    const int32_t *order, const _Float16 *z_in, double *acc,
+                                                    _Float16 *z_out)
+{
+    const uint32_t b = blockIdx.x;
+    const int64_t pos = ((int64_t)(b % 8u) * M.blocks_per_xcd + (b / 8u)) * PROWS + (threadIdx.x >> 6);
+    const int64_t row =
+        (int64_t)__builtin_amdgcn_readfirstlane((int)(pos < M.n && order ? (int64_t)order[pos] : pos));
+    const int lane = threadIdx.x & 63;
+    const int h = lane & 7;
+    const int g = lane >> 3;
+    double a[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) a[p] = 0.0;
+    if (row < M.n) {
+        const int64_t e0 = pp[row * (n_panels + 1) + panel];
+        const uint32_t len = (uint32_t)(pp[row * (n_panels + 1) + panel + 1] - e0);
+        for (uint32_t base = 0; base < len; base += 64u) {
+            const uint32_t e = base + (uint32_t)lane;
+            const bool ok = e < len;
+            const uint32_t ec = ok ? e : len - 1;
+            const int32_t jl = M.col[e0 + ec];
+            const double vl = ok ? (double)M.kval[e0 + ec] : 0.0;
+            int32_t j[8];
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                j[u] = __shfl(jl, g + 8 * u);
+                v[u] = __shfl(vl, g + 8 * u);
+            }
+            Pack<_Float16, 8> z[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint32_t off = (uint32_t)j[u] * 128u + 16u * (uint32_t)h;
+                z[u] = *reinterpret_cast<const Pack<_Float16, 8> *>(reinterpret_cast<const char *>(z_in) + off);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int p = 0; p < 8; ++p) a[p] = fma(v[u], (double)(float)z[u].v[p], a[p]);
+        }
+    }
+#pragma unroll
+    for (int off = 8; off < 64; off <<= 1)
+#pragma unroll
+        for (int p = 0; p < 8; ++p) a[p] += __shfl_xor(a[p], off);
+    if (row >= M.n) return;
+    double w = a[0];
+#pragma unroll
+    for (int p = 1; p < 8; ++p) w = g == p ? a[p] : w;
+    const size_t at = (size_t)row * 64 + (size_t)(8 * h + g);
+    const double total = (panel == 0 ? 0.0 : acc[at]) + w;
+    if (panel + 1 < n_panels) {
+        acc[at] = total;
+    } else {
+        const double di = M.d[row];
+        z_out[at] = from_f64<_Float16>(di * (total * M.inv_2R + M.delta * (double)(float)z_in[at]));
+    }
+}
+
 // Values of order one in every slot, as a function of the patch's original index.
 __global__ void k_probe_fill_perm(_Float16 *z, const int32_t *order, int64_t n)
 {
@@ -435,6 +501,77 @@ int prh_probe_order(const prh_operator *op, const int32_t *order, int32_t reps, 
         }
     const double v[] = {t[0], t[1], t[2], t[3], largest > 0.0 ? worst / largest : worst};
     for (int k = 0; k < n_out && k < 5; ++k) out[k] = v[k];
+    return PRH_OK;
+}
+
+int prh_probe_panels(const prh_operator *op, int32_t n_panels, int32_t reps, double *out, int32_t n_out)
+{
+    PRH_CHECK_ARG(op && out, "prh_probe_panels: null pointer");
+    PRH_CHECK_ARG(op->uniform && op->kval, "prh_probe_panels: the operator has no integer form K");
+    PRH_CHECK_ARG(n_panels >= 1 && n_panels <= 64, "prh_probe_panels: n_panels must be in [1, 64]");
+    PRH_CHECK_ARG(reps >= 1 && reps <= 10000, "prh_probe_panels: reps must be in [1, 10000]");
+    const int64_t n = op->n, nnz = op->nnz_s;
+    PRH_CHECK_ARG((uint64_t)n * 128 < ((uint64_t)1 << 32), "prh_probe_panels: slot vectors beyond 4 GiB");
+    PRH_HIP(hipSetDevice(op->device));
+    hipStream_t st;
+    int rc = current_stream(&st);
+    if (rc) return rc;
+    std::vector<int64_t> rowptr(n + 1);
+    std::vector<int32_t> col(nnz);
+    PRH_HIP(hipMemcpy(rowptr.data(), op->rowptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
+    PRH_HIP(hipMemcpy(col.data(), op->col, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+    // panel k holds the columns [k W, (k + 1) W); rows' columns are sorted
+    const int64_t W = (n + n_panels - 1) / n_panels;
+    std::vector<int64_t> pp((size_t)n * (n_panels + 1));
+    for (int64_t r = 0; r < n; ++r) {
+        int64_t e = rowptr[r];
+        for (int k = 0; k <= n_panels; ++k) {
+            const int64_t lim = std::min<int64_t>((int64_t)k * W, n);
+            while (k > 0 && e < rowptr[r + 1] && col[e] < lim) ++e;
+            pp[(size_t)r * (n_panels + 1) + k] = k == 0 ? rowptr[r] : (k == n_panels ? rowptr[r + 1] : e);
+        }
+    }
+    DeviceBuffers buf;
+    int64_t *d_pp = nullptr;
+    double *acc = nullptr;
+    _Float16 *z = nullptr, *y_plain = nullptr, *y_panel = nullptr;
+    const size_t vec = (size_t)n * 64;
+    if ((rc = buf.get(&d_pp, pp.size())) || (rc = buf.get(&acc, vec)) || (rc = buf.get(&z, vec)) ||
+        (rc = buf.get(&y_plain, vec)) || (rc = buf.get(&y_panel, vec)))
+        return rc;
+    PRH_HIP(hipMemcpy(d_pp, pp.data(), sizeof(int64_t) * pp.size(), hipMemcpyHostToDevice));
+    k_probe_fill_perm<<<(unsigned)((vec + 255) / 256), 256, 0, st>>>(z, nullptr, n);
+    const uint32_t bpx = xcd_blocks(n);
+    const ProbeView M{n, op->rowptr, op->col, op->kval, op->scale, op->inv_2R, op->delta, bpx};
+    hipEvent_t ev[5];
+    for (auto &e : ev) PRH_HIP(hipEventCreate(&e));
+    // plain sweep, stored and in the operator's row order; panel sweeps, the same two orders
+    const int32_t *orders[2] = {nullptr, op->row_order};
+    k_probe_gather<_Float16, 64, false><<<8u * bpx, PT, 0, st>>>(M, z, y_plain);  // warm-up and the product
+    PRH_HIP(hipEventRecord(ev[0], st));
+    for (int r = 0; r < reps; ++r) k_probe_gather<_Float16, 64, false><<<8u * bpx, PT, 0, st>>>(M, z, y_plain);
+    PRH_HIP(hipEventRecord(ev[1], st));
+    for (int o = 0; o < 2; ++o) {
+        for (int r = 0; r < reps; ++r)
+            for (int k = 0; k < n_panels; ++k)
+                k_probe_panel<<<8u * bpx, PT, 0, st>>>(M, d_pp, n_panels, k, orders[o], z, acc, y_panel);
+        PRH_HIP(hipEventRecord(ev[2 + o], st));
+    }
+    PRH_HIP(hipGetLastError());
+    PRH_HIP(hipEventSynchronize(ev[3]));
+    float ms[3];
+    for (int k = 0; k < 3; ++k) PRH_HIP(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    std::vector<_Float16> a(vec), bb(vec);
+    PRH_HIP(hipMemcpy(a.data(), y_plain, sizeof(_Float16) * vec, hipMemcpyDeviceToHost));
+    PRH_HIP(hipMemcpy(bb.data(), y_panel, sizeof(_Float16) * vec, hipMemcpyDeviceToHost));
+    double worst = 0.0, largest = 0.0;
+    for (size_t q = 0; q < vec; ++q) {
+        worst = std::max(worst, std::fabs((double)(float)a[q] - (double)(float)bb[q]));
+        largest = std::max(largest, std::fabs((double)(float)a[q]));
+    }
+    const double v[] = {ms[0] / reps, ms[1] / reps, ms[2] / reps, largest > 0.0 ? worst / largest : worst};
+    for (int k = 0; k < n_out && k < 4; ++k) out[k] = v[k];
     return PRH_OK;
 }
 

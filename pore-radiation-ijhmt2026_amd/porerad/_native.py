@@ -100,6 +100,7 @@ SIGNATURES = {
     "prh_probe_gather": ([_P, _I32, _I32, _P], ctypes.c_int),
     "prh_probe_tiles": ([_P, _I32, _I32, _P, _I32], ctypes.c_int),
     "prh_probe_order": ([_P, _P, _I32, _P, _I32], ctypes.c_int),
+    "prh_probe_panels": ([_P, _I32, _I32, _P, _I32], ctypes.c_int),
     "prh_batch_stats": ([_P, _P, _I32], ctypes.c_int),
     "prh_operator_apply": ([_P, _I32, _P, _P], ctypes.c_int),
     "prh_radiosity": ([_P, _I32, _P, _P, _I32, _F64, _I32, _I32, _P, _P, _P], ctypes.c_int),
@@ -654,6 +655,14 @@ class Operator:
         check(self._lib.prh_probe_order(self.handle, ptr(order), int(reps), ptr(out), 5), "prh_probe_order")
         return {"stored_ms": float(out[0]), "stored_streamed_ms": float(out[1]), "reordered_ms": float(out[2]),
                 "reordered_streamed_ms": float(out[3]), "max_rel_difference": float(out[4])}
+
+    def probe_panels(self, n_panels: int, reps: int = 20) -> dict:
+        """FP16 x 64 sweep in column panels against the plain sweep (a measurement probe)."""
+
+        out = np.zeros(4)
+        check(self._lib.prh_probe_panels(self.handle, int(n_panels), int(reps), ptr(out), 4), "prh_probe_panels")
+        return {"plain_ms": float(out[0]), "panels_ms": float(out[1]), "panels_sweep_order_ms": float(out[2]),
+                "max_rel_difference": float(out[3])}
 
     def csr(self):
         """(rowptr, col, val) of the symmetric FP64 matrix S held on the device."""

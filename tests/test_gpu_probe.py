@@ -59,3 +59,12 @@ def test_reordered_sweep_equals_stored_sweep_reordered(operator):
     assert min(got["stored_ms"], got["reordered_ms"], got["stored_streamed_ms"]) > 0.0
     with pytest.raises(Exception):
         operator.device.probe_order(np.zeros(operator.n_patches, np.int32), reps=1)
+
+
+@pytest.mark.parametrize("n_panels", [1, 3])
+def test_panel_sweep_equals_plain_sweep(operator, n_panels):
+    operator.set_sweep_order(np.random.default_rng(22).permutation(operator.n_patches).astype(np.int32))
+    got = operator.device.probe_panels(n_panels, reps=2)
+    operator.set_sweep_order(None)
+    assert got["max_rel_difference"] <= 2e-3, got  # FP16 outputs, sums in another order
+    assert got["plain_ms"] > 0 and got["panels_ms"] > 0 and got["panels_sweep_order_ms"] > 0
