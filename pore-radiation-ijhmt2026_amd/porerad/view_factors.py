@@ -38,6 +38,39 @@ def orient_along_x(array: np.ndarray, axis: str) -> np.ndarray:
     return np.ascontiguousarray(np.moveaxis(np.asarray(array), _AXIS_INDEX[axis], 0))
 
 
+def _hilbert_keys(ijk: np.ndarray, bits: int) -> np.ndarray:
+    """Index along the 3-D Hilbert curve of order `bits` of each integer point
+    (n x 3, coordinates below 2**bits): Skilling's transform from axes to the
+    transposed index, then the bits interleaved."""
+
+    X = np.array(ijk, dtype=np.int64, copy=True)
+    top = 1 << (bits - 1)
+    q = top
+    while q > 1:
+        low = q - 1
+        for i in range(3):
+            up = (X[:, i] & q) != 0
+            X[up, 0] ^= low
+            down = ~up
+            t = (X[down, 0] ^ X[down, i]) & low
+            X[down, 0] ^= t
+            X[down, i] ^= t
+        q >>= 1
+    X[:, 1] ^= X[:, 0]
+    X[:, 2] ^= X[:, 1]
+    t = np.zeros(X.shape[0], np.int64)
+    q = top
+    while q > 1:
+        t[(X[:, 2] & q) != 0] ^= q - 1
+        q >>= 1
+    X ^= t[:, None]
+    key = np.zeros(X.shape[0], np.int64)
+    for b in range(bits - 1, -1, -1):
+        for i in range(3):
+            key = (key << 1) | ((X[:, i] >> b) & 1)
+    return key
+
+
 @dataclass
 class SurfacePatches:
     """Patch table of one x-oriented microstructure (view_factors.py:43-67).
@@ -64,14 +97,19 @@ class SurfacePatches:
     def n_patches(self) -> int:
         return int(self.patch_class.size)
 
-    def sweep_order(self) -> np.ndarray:
-        """The patches along a Morton (Z-order) curve of their origins on the
-        half-voxel grid: an order in which consecutive patches are close in
-        space, for the batched engine's row sweep (see
-        ``ExchangeOperator.set_sweep_order``)."""
+    def sweep_order(self, curve: str = "morton") -> np.ndarray:
+        """The patches along a space-filling curve of their origins on the
+        half-voxel grid ("morton": Z-order; "hilbert"): an order in which
+        consecutive patches are close in space, for the batched engine's row
+        sweep (see ``ExchangeOperator.set_sweep_order``)."""
 
         ijk = np.floor(np.asarray(self.patch_origin) * 2.0).astype(np.int64)
         ijk -= ijk.min(axis=0)
+        if curve == "hilbert":
+            bits = max(1, int(ijk.max()).bit_length())
+            return np.argsort(_hilbert_keys(ijk, bits), kind="stable").astype(np.int32)
+        if curve != "morton":
+            raise ValueError(f"unknown curve {curve!r}")
         key = np.zeros(self.n_patches, np.int64)
         for b in range(20):
             key |= (((ijk[:, 0] >> b) & 1) << (3 * b + 2)) | (((ijk[:, 1] >> b) & 1) << (3 * b + 1)) | \

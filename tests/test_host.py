@@ -129,3 +129,34 @@ def test_case_driver_helpers(tmp_path):
     assert [(r["case_id"], r["q_h_W_m2"]) for r in rows] == [("b", "2.0"), ("a", "3.0")]
     with pytest.raises(ValueError, match="differs from the one listed"):
         C._check_phases(np.zeros((2, 2, 2), np.uint8), "0" * 64)
+
+
+@pytest.mark.parametrize("bits", [1, 2, 4])
+def test_hilbert_keys_walk_the_grid_one_cell_at_a_time(bits):
+    """The sweep order's Hilbert keys number a full 2^b cube once each, and
+    consecutive keys are face neighbours (what makes the curve local)."""
+
+    from porerad.view_factors import _hilbert_keys
+
+    m = 1 << bits
+    grid = np.stack(np.meshgrid(*(np.arange(m),) * 3, indexing="ij"), -1).reshape(-1, 3)
+    key = _hilbert_keys(grid, bits)
+    assert np.array_equal(np.sort(key), np.arange(m ** 3))
+    walk = grid[np.argsort(key)]
+    if m > 1:
+        assert np.abs(np.diff(walk, axis=0)).sum(axis=1).max() == 1
+
+
+def test_sweep_orders_are_permutations():
+    from porerad import view_factors as vf
+
+    rng = np.random.default_rng(3)
+    phases = (rng.random((10, 8, 9)) < 0.2).astype(np.uint8)
+    Q = O.build_surface_patches(phases, 0.01)
+    P = vf.SurfacePatches(Q.solid, Q.patch_class, Q.patch_cell, Q.patch_face, Q.patch_origin, Q.patch_normal,
+                          Q.patch_area_m2, Q.cell_face_patch, Q.hot_wall_patch, Q.cold_wall_patch, Q.H_m)
+    for curve in ("morton", "hilbert"):
+        order = P.sweep_order(curve)
+        assert order.dtype == np.int32 and np.array_equal(np.sort(order), np.arange(P.n_patches))
+    with pytest.raises(ValueError):
+        P.sweep_order("peano")
