@@ -19,10 +19,11 @@ Every row the driver writes is compared with its archived row:
   or 9 reflections is decided by the 1e-11 closure noise of the balancing,
   which no two balancing runs share bit for bit.  A converged field is then
   a root of one of two radiation maps a factor 10 apart in truncation error
-  (about 5e-12 W per cell), so eps = 0.9 rows are held to the bound of one
-  reflection: fluxes within 1e-5 of the total flux, temperatures within
-  5e-6 K (the one row where the stop counts differ is 4.3e-6 off in q_cond_hot
-  and 2.1e-7 K in T_min; the other eps = 0.9 rows agree to 1e-9).
+  (about 5e-12 W per cell).  The four eps = 0.9 rows are therefore held to
+  the bound of one reflection: fluxes within 1e-5 of the total flux,
+  temperatures within 5e-6 K.  Measured: q_cond_hot differs from the archive
+  by 1.2e-7 (OF), 3.7e-7 (FA), 1.2e-6 (FI) and 4.3e-6 (TL) of the total flux,
+  T_min by up to 2.1e-7 K; every other row agrees to 1e-9 and 1e-8 K.
 
 The 8 resolution cases (canonical microstructures at N = 160 and 192) use
 microstructures the reference does not ship: they are generated on the device
@@ -48,7 +49,8 @@ pytestmark = pytest.mark.gpu
 
 FULL = os.environ.get("PORERAD_FULL") == "1"
 PRESCRIBED_DEFAULT = ["pf_TL_core_phi0.10_s0_y", "pf_FA_matched_pair1_x", "pf_OF_core_phi0.15_s1_z"]
-COUPLED_DEFAULT = ["coupled_FI_matched_pair1_y", "coupled_OF_x_H20", "coupled_TL_x_T700_eps0.6_N160"]
+COUPLED_DEFAULT = ["coupled_FI_matched_pair1_y", "coupled_OF_x_H20", "coupled_TL_x_T700_eps0.6_N160",
+                   "coupled_TL_x_T700_eps0.9_N128"]
 
 
 def _pick(table, default):
@@ -101,6 +103,7 @@ def test_coupled_case_reproduces_archive(drivers, case):
     # the archive leaves some columns empty for the matched and thickness groups
     want = {k: v for k, v in case["archived"].items() if v != ""}
     assert row["newton_updates"] == int(want["newton_updates"])
+    # eps = 0.9: the Newton series' stop count rests on balancing noise (module docstring)
     flux_rel, temp_abs = (1e-5, 5e-6) if case["epsilon"] == 0.9 else (1e-9, 1e-8)
     scale = max(abs(want[k]) for k in ("q_total_hot_W_m2", "q_total_cold_W_m2") if k in want)
     for key in ("q_cond_hot_W_m2", "q_rad_hot_W_m2", "q_total_hot_W_m2", "q_cond_cold_W_m2", "q_rad_cold_W_m2",
