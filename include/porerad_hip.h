@@ -471,7 +471,10 @@ void prh_operator_destroy(prh_operator *op);
  * ||E - (I - diag(1-eps) F_p^T) J||_1 <= relative_tolerance ||E||_1; its
  * result is the absorbed power eps * F_p^T J.  Every sweep serves all
  * running slots; slots are admitted and retired at any sweep while the
- * others run, and a solve returns the same bits alone or in any mix.  Same
+ * others run, and a solve returns the same bits alone or in any mix.  The
+ * slots' schedules live on the device (a schedule kernel before every
+ * sweep) and sweeps are replayed from HIP graphs captured per batch: one
+ * sweep per replay, eight when prh_batch_step may run eight or more.  Same
  * contract as reflection_series (porerad/exchange.py:217-245); the serving
  * loop is built on it.  The batch is bound to its operator and must be
  * destroyed before it. */
@@ -502,8 +505,10 @@ int prh_batch_admit(prh_batch *b, const double *emitted, const double *emissivit
  * Needs prh_batch_set_field. */
 int prh_batch_admit_linear(prh_batch *b, double T_hot_K, double T_cold_K, double emissivity,
                            double relative_tolerance, int32_t *slot);
-/* Run up to max_ticks sweeps, returning early once a slot has finished;
- * done_mask: finished slots waiting for prh_batch_channels/result/release. */
+/* Queue up to max_ticks sweeps, returning early once a finished slot's
+ * results have arrived; replays are pipelined, so the last one may still
+ * run on the device when this returns (later calls read it).  done_mask:
+ * finished slots waiting for prh_batch_channels/result/release. */
 int prh_batch_step(prh_batch *b, int32_t max_ticks, uint64_t *done_mask, uint64_t *active_mask);
 /* Channels q_h, q_c, q_i (W m^-2, as prescribed_field_channels), power still
  * in flight and sweep count of a finished slot; keeps the slot.
