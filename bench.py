@@ -761,8 +761,9 @@ def patch_orders(P) -> dict:
     z-major lexicographic order of the patch origins on the half-voxel grid."""
 
     ijk = np.floor(np.asarray(P.patch_origin) * 2.0).astype(np.int64)
-    return {"morton": P.sweep_order(), "hilbert": P.sweep_order("hilbert"),
-            "z-major": np.lexsort((ijk[:, 0], ijk[:, 1], ijk[:, 2])).astype(np.int32)}
+    out = {"morton": P.sweep_order(), "hilbert": P.sweep_order("hilbert"),
+           "z-major": np.lexsort((ijk[:, 0], ijk[:, 1], ijk[:, 2])).astype(np.int32)}
+    return out
 
 
 def run_monte_carlo(phases, with_cpu: bool) -> dict:
