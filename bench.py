@@ -45,14 +45,24 @@ HBM_PEAK_GBPS = 8000.0
 METRIC = "first-hit rays/s at N=128; served radiosity solves/s, p50/p99 at 1e-8 rel err"
 
 
-TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "k_trace_stream_foam128_hbm_traffic.md")
+def traffic_profile(kernel: str) -> str:
+    return os.path.join(REPO, "profiles", f"{kernel}_foam128_hbm_traffic.md")
 
 
-def measured_traffic() -> float | None:
+def traversal_kernel() -> str:
+    """Name of the traversal kernel the current tracer variant launches on the foam."""
+
+    from porerad import _native
+
+    current = next(v for v in _native.tracer_variants() if v["current"])
+    return "k_trace_lean" if current["occupancy"] == 3 else "k_trace_stream"
+
+
+def measured_traffic(kernel: str) -> float | None:
     """HBM bytes per traversal launch at the default workload, from the committed counter summary."""
 
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(traffic_profile(kernel)) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return float(line.split(":")[1])
@@ -86,6 +96,21 @@ def run_tracer(P, R, seed, steps, warmup):
         reads, nnz = e.occupancy_reads, e.nnz
         e.close()
     return np.array(wall), np.array(parts), reads, nnz
+
+
+def lane_share(P, R, seed, reads):
+    """Share of wavefront lanes that step in one traced pass of the lean traversal
+    (occupancy reads over 64 x wave-steps, from the kernel's own counters)."""
+
+    from porerad import _native, view_factors
+
+    _native.tracer_stats(reset=True)
+    view_factors.first_hit_edges(P, R, [seed]).close()
+    st = _native.tracer_stats(reset=True)
+    if st["wave_steps"] <= 0:
+        return None
+    return {"wave_steps": st["wave_steps"], "refills": st["refills"],
+            "share": float(reads) / (64.0 * st["wave_steps"])}
 
 
 def count_parity(P, R, seed):
@@ -925,6 +950,8 @@ def main() -> None:
     log(f"counts equal to the CPU oracle's (count_sha256 {parity.get('count_sha256', '-')[:16]}...)"
         if parity["checked"] else "count parity not checked: " + parity["reason"])
     kernel = trace_s
+    kernel_name = traversal_kernel()
+    lanes = lane_share(P, R, seed, reads) if kernel_name == "k_trace_lean" else None
     algo_bytes = 4.0 * n_rays + float(reads)
     achieved = algo_bytes / kernel / 1e9
     record = {
@@ -953,9 +980,11 @@ def main() -> None:
             # FETCH_SIZE x 2 + WRITE_SIZE x 1 (copy-calibrated) of this kernel at
             # the default workload, from the rocprofv3 --pmc passes summarised in
             # traffic_source; null for other ray counts
-            "traffic": measured_traffic() if R == 1000 else None,
-            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, REPO) + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
-            "kernel": "k_trace_stream",
+            "traffic": measured_traffic(kernel_name) if R == 1000 else None,
+            "traffic_source": os.path.relpath(traffic_profile(kernel_name), REPO)
+            + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
+            "kernel": kernel_name,
+            "lane_share": lanes,
             "kernel_ms": kernel * 1e3,
             "algorithmic_bytes_per_launch": algo_bytes,
             "occupancy_reads_per_ray": reads / n_rays,

@@ -97,12 +97,19 @@ int prh_last_timings(double *ms, int32_t n);
 /* Tuning knob of the traversal kernel: variant index in [0, n_variants).
  * Each variant fixes the refill threshold (idle lanes of a wavefront before
  * it loads new rays) and the occupancy test: 0 the global bitset, 1 the LDS
- * brick bitset first, 2 the global bitset word loaded ahead of the step.  No
+ * brick bitset first, 2 the global bitset word loaded ahead of the step, 3 the
+ * lean step (walled bitset in 4x4x4 blocks, the current block's word held per
+ * lane; geometries without a walled bitset run the brick test instead).  No
  * counterpart in the reference (a launch parameter of this implementation);
  * results do not depend on it.  prh_tracer_variants fills the tables when
  * the pointers are non-NULL (arrays of n_variants entries). */
 int prh_tracer_set_variant(int32_t variant);
 int prh_tracer_variants(int32_t *n_variants, int32_t *current, int32_t *refill_min, int32_t *occupancy_mode);
+/* Wave-steps and refills of the lean traversal on the current device since
+ * the last reset (after the library stream has drained); reset != 0 clears
+ * them.  Diagnostic with no counterpart in the reference: with the occupancy
+ * reads, occupancy_reads / (64 wave_steps) is the share of lanes that step. */
+int prh_tracer_stats(int64_t *wave_steps, int64_t *refills, int32_t reset);
 /* Tuning knob of the traversal kernel: records a wavefront takes from the
  * launch's shared work queue at a time (64 .. 2^20, default 512).  No
  * counterpart in the reference; results do not depend on it. */

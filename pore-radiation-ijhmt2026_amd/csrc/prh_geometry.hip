@@ -12,6 +12,8 @@
 // Layout in HBM (see prh_internal.h): a flat occupancy bitset for the
 // traversal, one 32-bit code per cell (first patch id << 6 | face mask) to
 // turn a hit face into a patch id, and per-patch origin/face/class arrays.
+#include <algorithm>
+
 #include "prh_internal.h"
 #include "prh_device.h"
 
@@ -89,6 +91,40 @@ __global__ __launch_bounds__(GEO_THREADS) void k_pack_coarse(const uint8_t *soli
         }
         coarse[w] = bits;
     }
+}
+
+// Walled bitset (Geometry::wocc): word w holds the 4x4x4 block of cells whose
+// index P has P >> 6 == w; bit P & 63 is set for a solid cell and for every
+// cell of the planes i = -1 and i = nx.  Indices beyond ny, nz (the padding
+// up to a power of two) stay clear.
+__global__ __launch_bounds__(GEO_THREADS) void k_pack_walled(const uint8_t *solid, int nx, int ny, int nz, int lx,
+                                                              int ly, int lz, int64_t n_words, uint64_t *wocc)
+{
+    const LeanLayout L = lean_layout(lx, ly, lz);
+    for (int64_t w = (int64_t)blockIdx.x * GEO_THREADS + threadIdx.x; w < n_words;
+         w += (int64_t)gridDim.x * GEO_THREADS) {
+        uint64_t bits = 0;
+        for (uint32_t t = 0; t < 64; ++t) {
+            const uint32_t P = (uint32_t)w << 6 | t;
+            const uint32_t ip = lean_compact(L, 0, P & L.mask[0]);
+            const uint32_t j = lean_compact(L, 1, P & L.mask[1]);
+            const uint32_t k = lean_compact(L, 2, P & L.mask[2]);
+            bool set = false;
+            if (j < (uint32_t)ny && k < (uint32_t)nz) {
+                if (ip == 0u || ip == (uint32_t)nx + 1u) set = true;
+                else if (ip <= (uint32_t)nx) set = solid[(((int64_t)ip - 1) * ny + j) * nz + k] != 0;
+            }
+            if (set) bits |= 1ull << t;
+        }
+        wocc[w] = bits;
+    }
+}
+
+int ceil_log2(int n)
+{
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
 }
 
 __global__ __launch_bounds__(GEO_THREADS) void k_wall_counts(const uint8_t *solid, int nx, int ny, int nz,
@@ -175,9 +211,10 @@ unsigned grid_for(int64_t n)
 void free_geometry(Geometry &g)
 {
     void *ptrs[] = {g.occ, g.coarse, g.cell_code, g.patch_origin, g.patch_face, g.patch_cell, g.patch_class, g.hot_map,
-                    g.cold_map};
+                    g.cold_map, g.wocc};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    g.wocc = nullptr;
     g.occ = nullptr;
     g.coarse = nullptr;
     g.cell_code = nullptr;
@@ -211,6 +248,14 @@ int build(const uint8_t *solid_host, int nx, int ny, int nz, double H_m, Geometr
     g.H_m = H_m;
     const int64_t n_cells = (int64_t)nx * ny * nz;
     const int64_t n_lat = (int64_t)ny * nz;
+    // The walled bitset exists when its index fits 31 bits and both lateral
+    // sizes are at least 3 (each axis has two interleaved bits, and a lateral
+    // step always changes j or k).
+    g.wlx = std::max(2, ceil_log2(nx + 2));
+    g.wly = std::max(2, ceil_log2(ny));
+    g.wlz = std::max(2, ceil_log2(nz));
+    const int walled_bits = g.wlx + g.wly + g.wlz;
+    g.wocc_words = (ny >= 3 && nz >= 3 && walled_bits <= 31) ? (int64_t(1) << (walled_bits - 6)) : 0;
 
     uint8_t *solid = nullptr, *mask = nullptr;
     int32_t *face_count = nullptr, *wall_count = nullptr;
@@ -239,12 +284,16 @@ int build(const uint8_t *solid_host, int nx, int ny, int nz, double H_m, Geometr
     PRH_HIP(hipMalloc((void **)&g.cell_code, n_cells * sizeof(uint32_t)));
     PRH_HIP(hipMalloc((void **)&g.hot_map, n_lat * sizeof(int32_t)));
     PRH_HIP(hipMalloc((void **)&g.cold_map, n_lat * sizeof(int32_t)));
+    if (g.wocc_words) PRH_HIP(hipMalloc((void **)&g.wocc, (size_t)g.wocc_words * sizeof(uint64_t)));
     PRH_HIP(hipMemcpyAsync(solid, solid_host, n_cells, hipMemcpyHostToDevice, st));
 
     k_cell_masks<<<grid_for(n_cells), GEO_THREADS, 0, st>>>(solid, nx, ny, nz, mask, face_count);
     k_pack_occupancy<<<grid_for(g.occ_words), GEO_THREADS, 0, st>>>(solid, n_cells, g.occ);
     k_pack_coarse<<<grid_for(g.coarse_words), GEO_THREADS, 0, st>>>(solid, nx, ny, nz, g.coarse_shift, g.cnx, g.cny,
                                                                     g.cnz, g.coarse);
+    if (g.wocc)
+        k_pack_walled<<<grid_for(g.wocc_words), GEO_THREADS, 0, st>>>(solid, nx, ny, nz, g.wlx, g.wly, g.wlz,
+                                                                      g.wocc_words, g.wocc);
     k_wall_counts<<<grid_for(n_lat), GEO_THREADS, 0, st>>>(solid, nx, ny, nz, wall_count);
     PRH_HIP(hipGetLastError());
     int64_t n_internal = 0, n_wall = 0;

@@ -91,6 +91,13 @@ void host_seed_words(const uint32_t *words, int n_words, uint64_t out[4]);
 //   cell_code  per cell: (first patch id of the cell << 6) | exposed-face mask
 //   patch_*    per patch: origin (x,y,z) FP64, face 0..5 (walls: 0 hot, 1 cold)
 //   hot_map / cold_map  wall patch id per (j, k), -1 where the boundary cell is solid
+//   wocc       walled occupancy bitset in 4x4x4 blocks, one 64-bit word per block: bit P
+//              for the cell (i + 1, j, k), i in [-1, nx], where P interleaves the two low
+//              bits of i + 1, j, k (bits k0 j0 i0 k1 j1 i1 = the bit within the word) and
+//              holds the rest of k, j, i + 1 in fields above (lean_layout); the planes
+//              i = -1 and i = nx (the hot and cold walls) are set, so one bit test ends
+//              a ray at a solid cell or a wall (null when P would need more than 31 bits
+//              or ny, nz < 3)
 struct Geometry {
     int32_t nx, ny, nz;
     int64_t occ_words;
@@ -107,7 +114,46 @@ struct Geometry {
     int32_t *patch_cell;   // n_patches (-1 for walls)
     int8_t *patch_class;   // n_patches
     int32_t *hot_map, *cold_map;
+    uint64_t *wocc;
+    int64_t wocc_words;
+    int32_t wlx, wly, wlz;  // bit widths of nx + 2, ny, nz (each at least 2)
 };
+
+// Bit layout of the walled bitset's index P (Geometry::wocc): axis a's value v
+// sits at P bits lo_a (v bit 0), lo_a + 3 (v bit 1) and field_a.. (v >> 2),
+// with lo = 2, 1, 0 and the fields z, y, x upward from bit 6.  The set of an
+// axis's bits is its mask; P is the OR of the three dilated values.
+struct LeanLayout {
+    uint32_t mask[3];   // x, y, z
+    uint32_t field[3];  // first bit of each axis's field
+};
+
+__host__ __device__ inline LeanLayout lean_layout(int lx, int ly, int lz)
+{
+    LeanLayout L;
+    L.field[2] = 6;
+    L.field[1] = 6 + (uint32_t)(lz - 2);
+    L.field[0] = L.field[1] + (uint32_t)(ly - 2);
+    const int bits[3] = {lx, ly, lz};
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t lo = (uint32_t)(2 - a);
+        L.mask[a] = (1u << lo) | (1u << (lo + 3)) | (((1u << (bits[a] - 2)) - 1u) << L.field[a]);
+    }
+    return L;
+}
+
+// Value v of axis a (0 x, 1 y, 2 z) spread to its bits of P, and back.
+__host__ __device__ inline uint32_t lean_dilate(const LeanLayout &L, int a, uint32_t v)
+{
+    const uint32_t lo = (uint32_t)(2 - a);
+    return (v & 1u) << lo | ((v >> 1) & 1u) << (lo + 3) | (v >> 2) << L.field[a];
+}
+
+__host__ __device__ inline uint32_t lean_compact(const LeanLayout &L, int a, uint32_t d)
+{
+    const uint32_t lo = (uint32_t)(2 - a);
+    return ((d >> lo) & 1u) | ((d >> (lo + 2)) & 2u) | ((d & L.mask[a]) >> L.field[a]) << 2;
+}
 
 // Device exclusive scan helpers (prh_scan.hip).
 int exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *total, hipStream_t stream);

@@ -31,6 +31,12 @@ namespace prh {
 
 __constant__ JumpTables c_jump;
 
+// Wave-steps (stepping iterations x steps per iteration, counted per
+// wavefront) and refills of the lean traversal since the last reset
+// (prh_tracer_stats): with the occupancy reads they give the share of lanes
+// that step.  Concurrent launches on one device add into the same counters.
+__device__ unsigned long long g_trace_stats[2];
+
 namespace {
 
 constexpr int TRACE_THREADS = 256;
@@ -51,13 +57,18 @@ struct GeoView {
     int coarse_shift, cny, cnz;
     uint32_t coarse_words;
     double inv_ny, inv_nz;
+    const uint64_t *wocc;  // walled bitset (Geometry::wocc), or null
+    LeanLayout wl;         // its index layout
+    uint32_t wl_cold;      // dilated i + 1 of the cold wall plane (nx + 1)
 };
 
 GeoView view_of(const Geometry &g)
 {
     return GeoView{g.occ,    g.coarse, g.cell_code, g.hot_map, g.cold_map, g.patch_origin, g.patch_face,
                    g.patch_class, g.nx, g.ny,     g.nz,      g.coarse_shift, g.cny,    g.cnz,
-                   (uint32_t)g.coarse_words,     1.0 / (double)g.ny,   1.0 / (double)g.nz};
+                   (uint32_t)g.coarse_words,     1.0 / (double)g.ny,   1.0 / (double)g.nz,
+                   g.wocc,   g.wocc ? lean_layout(g.wlx, g.wly, g.wlz) : LeanLayout{},
+                   g.wocc ? lean_dilate(lean_layout(g.wlx, g.wly, g.wlz), 0, (uint32_t)g.nx + 1u) : 0u};
 }
 
 // DDA start state of one ray (the set-up of _trace_kernel,
@@ -68,7 +79,7 @@ struct alignas(16) DdaRec {
     uint32_t ij;   // (i + 1) | jj << 16
     uint32_t ks;   // kk | (sx + 1) << 16 | (sy + 1) << 18 | (sz + 1) << 20
     uint32_t cap;  // step cap; 0 for a ray with dx == 0 (it fails)
-    uint32_t pad;
+    uint32_t P;    // the start cell's walled-bitset index (0 without a walled bitset)
 };
 
 // What a wall receiver needs (view_factors.py:226-231), read only for rays
@@ -79,7 +90,10 @@ struct alignas(16) WallRec {
 
 // Final state of a traversed ray, 8 B: x = cell index (solid) or the wall
 // cell's j (walls); y = kind << 28 | face (solid) or the wall cell's k.
-enum : uint32_t { FIN_FAILED = 0, FIN_SOLID = 1, FIN_HOT = 2, FIN_COLD = 3 };
+// FIN_SOLID_P: a solid cell given by its walled-bitset index (x = P, see
+// Geometry::wocc), written by the lean traversal and turned into the cell
+// index when the receiver is resolved.
+enum : uint32_t { FIN_FAILED = 0, FIN_SOLID = 1, FIN_HOT = 2, FIN_COLD = 3, FIN_SOLID_P = 4 };
 
 // Per-lane traversal state (the loop variables of _trace_kernel).  The face
 // of the last crossing is kept as its axis (priority x > y > z) and turned
@@ -163,7 +177,7 @@ __device__ __forceinline__ void make_dda(const double o[3], const double d[3], c
     w.dz = d[2];
     if (d[0] == 0.0) {
         rec.tx = rec.ty = rec.tz = rec.ddx = rec.ddy = rec.ddz = 0.0;
-        rec.ij = rec.ks = rec.cap = rec.pad = 0u;
+        rec.ij = rec.ks = rec.cap = rec.P = 0u;
         return;
     }
     const double i = initial_cell(o[0], d[0]);
@@ -185,7 +199,10 @@ __device__ __forceinline__ void make_dda(const double o[3], const double d[3], c
     uint32_t cap = C.min_cap;
     if (fabs(d[0]) < C.grazing_dx) cap = max_steps_of(d[0], d[1], d[2], G.nx, G.ny, G.nz);
     rec.cap = cap;
-    rec.pad = 0u;
+    // the start cell's walled-bitset index, for the lean traversal
+    rec.P = G.wocc ? lean_dilate(G.wl, 0, (uint32_t)(ic + 1)) | lean_dilate(G.wl, 1, (uint32_t)jj) |
+                           lean_dilate(G.wl, 2, (uint32_t)kk)
+                     : 0u;
 }
 
 // Traversal state of ray g from its record; false for a ray that fails at once.
@@ -221,6 +238,7 @@ enum OccMode : int {
                      // for cells in bricks that hold solid
     OCC_AHEAD = 2,   // the global bitset word loaded ahead: the step is computed
                      // while the load is in flight and undone on a solid cell
+    OCC_LEAN = 3,    // the walled bitset indexed by one packed cell index (lean_step)
 };
 
 // Flag added to an occupancy mode when ny and nz are powers of two: the
@@ -365,6 +383,18 @@ __device__ __forceinline__ int resolve_receiver(uint2 f, const GeoView &G)
         const uint32_t face = f.y & 7u;
         if ((uint64_t)f.x >= (uint64_t)G.nx * (uint64_t)G.ny * (uint64_t)G.nz || face > 5u) return -1;
         const uint32_t code = G.cell_code[f.x];
+        const uint32_t m = code & 63u;
+        if (!((m >> face) & 1u)) return -1;
+        return (int)(code >> 6) + __popc(m & ((1u << face) - 1u));
+    }
+    if (kind == FIN_SOLID_P) {
+        const uint32_t face = f.y & 7u;
+        if (!G.wocc || face > 5u) return -1;
+        const uint32_t ip = lean_compact(G.wl, 0, f.x & G.wl.mask[0]);
+        const uint32_t j = lean_compact(G.wl, 1, f.x & G.wl.mask[1]);
+        const uint32_t k = lean_compact(G.wl, 2, f.x & G.wl.mask[2]);
+        if (ip < 1u || ip > (uint32_t)G.nx || j >= (uint32_t)G.ny || k >= (uint32_t)G.nz) return -1;
+        const uint32_t code = G.cell_code[((size_t)(ip - 1u) * (uint32_t)G.ny + j) * (uint32_t)G.nz + k];
         const uint32_t m = code & 63u;
         if (!((m >> face) & 1u)) return -1;
         return (int)(code >> 6) + __popc(m & ((1u << face) - 1u));
@@ -696,6 +726,243 @@ void k_trace_batch(const Segment *segs, int n_seg, const DdaRec *dda, const Wall
     }
     trace_body<REFILL_MIN, OCC>(G, segs[k].gen.offset, segs[k].gen.total, segs[k].grain, segs[k].queue, dda, walls,
                                 fin, segs[k].reads, lds_coarse);
+}
+
+// ------------------------------------------------------------------ lean traversal
+// The same traversal over the walled bitset (Geometry::wocc), which holds the
+// cells in 4x4x4 blocks of one 64-bit word and sets the two wall planes, so
+// one bit test ends a ray at a solid cell or at either wall.  Each lane keeps
+// the word of its current block in registers and loads the next one only when
+// the ray leaves the block (about one step in four), which keeps the requests
+// to L2 near those of the brick test without its index arithmetic.  The cell
+// is held as three dilated values (each axis's bits of the index P, see
+// LeanLayout): a crossing is (d - s) & mask with s = mask for +1 and 1 for -1,
+// and P is their OR.  A lateral size that is not a power of two wraps by one
+// compare and select.  No face is kept per step: the dilated steps of the
+// last crossing are (x and y; z crossed when neither did), so the face
+// follows from them when the ray ends.
+struct LeanRay {
+    double tx, ty, tz, ddx, ddy, ddz, time;
+    uint32_t xd, yd, zd;     // dilated i + 1, j, k
+    uint32_t sx, sy, sz;     // dilated steps: the axis mask (+1) or 1 (-1); 0 for no step
+    uint32_t last_x, last_y; // the x and y steps of the last crossing (0: not crossed)
+    uint32_t from_y, to_y, from_z, to_z;  // lateral wraps (sizes that are not powers of two)
+    uint32_t P;              // xd | yd | zd: the cell's bit in the walled bitset
+    uint64_t blk;            // the bitset word of the cell's block
+    uint32_t blk_at;         // its byte offset
+    uint32_t left, cap;
+};
+
+template <bool POW2>
+__device__ __forceinline__ bool lean_load(LeanRay &R, const DdaRec *recs, uint32_t g, const GeoView &G)
+{
+    const double2 *p = reinterpret_cast<const double2 *>(recs + g);
+    const double2 a = p[0], b = p[1], c = p[2];
+    const uint4 w = reinterpret_cast<const uint4 *>(recs + g)[3];
+    R.tx = a.x;
+    R.ty = a.y;
+    R.tz = b.x;
+    R.ddx = b.y;
+    R.ddy = c.x;
+    R.ddz = c.y;
+    const LeanLayout &L = G.wl;
+    // the record holds the start cell's index P and the step signs
+    R.P = w.w;
+    R.xd = w.w & L.mask[0];
+    R.yd = w.w & L.mask[1];
+    R.zd = w.w & L.mask[2];
+    const uint32_t ux = (w.y >> 16) & 3u, uy = (w.y >> 18) & 3u, uz = (w.y >> 20) & 3u;  // step + 1
+    R.sx = ux == 2u ? L.mask[0] : (ux == 0u ? 1u : 0u);
+    R.sy = uy == 2u ? L.mask[1] : (uy == 0u ? 1u : 0u);
+    R.sz = uz == 2u ? L.mask[2] : (uz == 0u ? 1u : 0u);
+    if (!POW2) {
+        // +1 from n - 1 reaches n, which wraps to 0; -1 from 0 leaves all the
+        // axis's bits set, which wraps to n - 1
+        R.from_y = uy == 2u ? lean_dilate(L, 1, (uint32_t)G.ny) : L.mask[1];
+        R.to_y = uy == 2u ? 0u : lean_dilate(L, 1, (uint32_t)G.ny - 1u);
+        R.from_z = uz == 2u ? lean_dilate(L, 2, (uint32_t)G.nz) : L.mask[2];
+        R.to_z = uz == 2u ? 0u : lean_dilate(L, 2, (uint32_t)G.nz - 1u);
+    }
+    R.last_x = R.last_y = 0u;
+    R.time = 0.0;
+    R.cap = w.z;
+    R.left = w.z;
+    // the block of the first cell, read by the first step
+    R.blk_at = (R.P >> 3) & ~7u;
+    R.blk = *reinterpret_cast<const uint64_t *>(reinterpret_cast<const char *>(G.wocc) + R.blk_at);
+    return w.z != 0u;
+}
+
+// One iteration of view_factors.py:225-255 on the walled bitset; returns
+// whether the ray goes on.  The crossing is worked out before the cell's
+// bitset word is tested (it does not depend on it), and taken only if the
+// cell is empty: the masks of the crossed axes are cleared for a solid cell,
+// so the commit needs no branch.  The word of the next cell's block is
+// requested right after the crossing and first tested by the next step, so
+// its latency overlaps that step's crossing work.
+template <bool POW2>
+__device__ __forceinline__ bool lean_step(LeanRay &R, const GeoView &G)
+{
+    const double nt = vmin_f64(vmin_f64(R.tx, R.ty), R.tz);
+    bool cx = fabs(R.tx - nt) <= TIE_TOLERANCE;
+    bool cy = fabs(R.ty - nt) <= TIE_TOLERANCE;
+    bool cz = fabs(R.tz - nt) <= TIE_TOLERANCE;
+    __builtin_amdgcn_sched_barrier(0);
+    const bool go = !((uint32_t)(R.blk >> (R.P & 63u)) & 1u);  // not a solid cell or wall
+    cx = cx && go;
+    cy = cy && go;
+    cz = cz && go;
+    R.tx = fma_f64(cx ? 1.0 : 0.0, R.ddx, R.tx);
+    R.ty = fma_f64(cy ? 1.0 : 0.0, R.ddy, R.ty);
+    R.tz = fma_f64(cz ? 1.0 : 0.0, R.ddz, R.tz);
+    const uint32_t dx = cx ? R.sx : 0u, dy = cy ? R.sy : 0u, dz = cz ? R.sz : 0u;
+    R.xd = (R.xd - dx) & G.wl.mask[0];
+    R.yd = (R.yd - dy) & G.wl.mask[1];
+    R.zd = (R.zd - dz) & G.wl.mask[2];
+    if (!POW2) {
+        R.yd = R.yd == R.from_y ? R.to_y : R.yd;
+        R.zd = R.zd == R.from_z ? R.to_z : R.zd;
+    }
+    R.last_x = go ? dx : R.last_x;
+    R.last_y = go ? dy : R.last_y;
+    R.time = go ? nt : R.time;
+    R.left -= go ? 1u : 0u;
+    R.P = R.xd | R.yd | R.zd;
+    const uint32_t at = (R.P >> 3) & ~7u;  // byte offset of the next cell's block word
+    if (at != R.blk_at) {
+        R.blk = *reinterpret_cast<const uint64_t *>(reinterpret_cast<const char *>(G.wocc) + at);
+        R.blk_at = at;
+    }
+    return go && R.left != 0u;
+}
+
+// Whether a finished lean ray stands on a wall plane (dilated i + 1 of 0 or nx + 1).
+__device__ __forceinline__ bool lean_at_wall(const LeanRay &R, const GeoView &G)
+{
+    return R.xd == 0u || R.xd == G.wl_cold;
+}
+
+// Occupancy reads of a finished lean traversal (as ray_reads): the steps,
+// plus the read that found a solid cell (a wall is not read).
+__device__ __forceinline__ uint32_t lean_reads(const LeanRay &R, const GeoView &G)
+{
+    const bool at_solid = R.left != 0u && !lean_at_wall(R, G);
+    return (R.cap - R.left) + (at_solid ? 1u : 0u);
+}
+
+// Final state of a finished lean traversal: FIN_FAILED, a wall cell as
+// ray_final gives it, or FIN_SOLID_P with the cell's index P.
+__device__ __forceinline__ uint2 lean_final(const LeanRay &R, const GeoView &G, const WallRec *walls, uint32_t g)
+{
+    if (R.left == 0u) return make_uint2(0u, FIN_FAILED << 28);
+    if (lean_at_wall(R, G)) {
+        const double2 *p = reinterpret_cast<const double2 *>(walls + g);
+        const double2 o = p[0], d = p[1];
+        const double yw = o.x + d.x * R.time;
+        const double zw = o.y + d.y * R.time;
+        const uint32_t jw = wrap_floor(yw, G.ny, G.inv_ny);
+        const uint32_t kw = wrap_floor(zw, G.nz, G.inv_nz);
+        return make_uint2(jw, (R.xd == 0u ? FIN_HOT : FIN_COLD) << 28 | kw);
+    }
+    if (R.left == R.cap) return make_uint2(0u, FIN_FAILED << 28);  // solid from the start: no face
+    // the face of the last crossing: the first axis crossed in x > y > z order;
+    // a +1 step is the axis mask, a -1 step is 1
+    const uint32_t axis = R.last_x != 0u ? 0u : (R.last_y != 0u ? 1u : 2u);
+    const uint32_t step = axis == 0u ? R.last_x : (axis == 1u ? R.last_y : R.sz);
+    const uint32_t face = 2u * axis + (step != 1u ? 1u : 0u);
+    return make_uint2(R.P, FIN_SOLID_P << 28 | face);
+}
+
+// trace_body with the lean step: the same queue, refill and final-state order,
+// in one loop whose refill part runs only when REFILL_MIN lanes are idle (or
+// none is active), so the stepping is not a loop of its own.
+template <int REFILL_MIN, bool POW2>
+__device__ __forceinline__ void lean_body(const GeoView &G, uint32_t base, uint32_t total, uint32_t grain,
+                                          unsigned long long *queue, const DdaRec *dda, const WallRec *walls,
+                                          uint2 *fin, unsigned long long *reads_out)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t cursor = 0, w_end = 0;  // the wavefront's grain [cursor, w_end)
+    bool more = true;                // the queue may still hold records
+
+    LeanRay R;
+    R.xd = R.yd = R.zd = 0u;
+    R.left = 1;
+    bool active = false, pending = false;
+    uint32_t g_cur = 0;
+    uint64_t reads = 0;
+    int n_active = 0;       // lanes of the wavefront that are tracing
+    bool finished = false;  // every record of the queue is traced and written
+    uint32_t wave_steps = 0, refills = 0;
+
+    do {
+        if (n_active == 0 || ((more || cursor < w_end) && n_active <= 64 - REFILL_MIN)) {
+            ++refills;
+            if (!active && pending) {
+                reads += lean_reads(R, G);
+                fin[g_cur] = lean_final(R, G, walls, g_cur);
+                pending = false;
+            }
+            if (more && cursor >= w_end) {
+                unsigned long long first = 0;
+                if (lane == 0) first = atomicAdd(queue, (unsigned long long)grain);
+                first = __shfl(first, 0);
+                if (first >= total) {
+                    more = false;
+                } else {
+                    cursor = base + (uint32_t)first;
+                    const unsigned long long last = first + grain;
+                    w_end = base + (last < total ? (uint32_t)last : total);
+                }
+            }
+            if (cursor < w_end) {
+                const uint64_t idle = __ballot(!active);
+                if (!active) {
+                    const uint32_t g = cursor + (uint32_t)__popcll(idle & below);
+                    if (g < w_end) {
+                        if (lean_load<POW2>(R, dda, g, G)) {
+                            active = true;
+                            pending = true;
+                            g_cur = g;
+                        } else {
+                            fin[g] = make_uint2(0u, FIN_FAILED << 28);
+                        }
+                    }
+                }
+                cursor += (uint32_t)__popcll(idle);
+            }
+            n_active = __popcll(__ballot(active));
+            finished = n_active == 0 && !more && cursor >= w_end;
+        }
+        if (n_active > 0) wave_steps += 2;
+        if (active) active = lean_step<POW2>(R, G);
+        if (active) active = lean_step<POW2>(R, G);
+        n_active = __popcll(__ballot(active));
+    } while (!finished);
+    reads = wave_sum_u64(reads);
+    if (lane == 0 && reads) atomicAdd(reads_out, (unsigned long long)reads);
+    if (lane == 0) {
+        atomicAdd(&g_trace_stats[0], (unsigned long long)wave_steps);
+        atomicAdd(&g_trace_stats[1], (unsigned long long)refills);
+    }
+}
+
+template <int REFILL_MIN, bool POW2>
+__global__ __launch_bounds__(STREAM_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
+void k_trace_lean(TraceArgs a)
+{
+    lean_body<REFILL_MIN, POW2>(a.G, 0u, a.total, a.grain, a.queue, a.dda, a.walls, a.fin, a.reads);
+}
+
+template <int REFILL_MIN, bool POW2>
+__global__ __launch_bounds__(STREAM_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
+void k_trace_batch_lean(const Segment *segs, int n_seg, const DdaRec *dda, const WallRec *walls, uint2 *fin)
+{
+    const int k = segment_of(segs, n_seg, blockIdx.x, true);
+    const GeoView G = segs[k].gen.G;
+    lean_body<REFILL_MIN, POW2>(G, segs[k].gen.offset, segs[k].gen.total, segs[k].grain, segs[k].queue, dda, walls,
+                                fin, segs[k].reads);
 }
 
 __global__ void k_zero_u64(unsigned long long *p, int n)
@@ -1035,7 +1302,8 @@ const TraceVariant TRACE_VARIANTS[] = {
     {k_trace_stream<48, OCC_BRICK>, 48, OCC_BRICK},   {k_trace_stream<32, OCC_GLOBAL>, 32, OCC_GLOBAL},
     {k_trace_stream<16, OCC_GLOBAL>, 16, OCC_GLOBAL}, {k_trace_stream<24, OCC_GLOBAL>, 24, OCC_GLOBAL},
     {k_trace_stream<16, OCC_AHEAD>, 16, OCC_AHEAD},   {k_trace_stream<24, OCC_AHEAD>, 24, OCC_AHEAD},
-    {k_trace_stream<32, OCC_AHEAD>, 32, OCC_AHEAD},
+    {k_trace_stream<32, OCC_AHEAD>, 32, OCC_AHEAD},   {k_trace_lean<16, false>, 16, OCC_LEAN},
+    {k_trace_lean<24, false>, 24, OCC_LEAN},          {k_trace_lean<32, false>, 32, OCC_LEAN},
 };
 constexpr int N_TRACE_VARIANTS = sizeof(TRACE_VARIANTS) / sizeof(TRACE_VARIANTS[0]);
 // The same variants for geometries whose ny and nz are powers of two.
@@ -1051,19 +1319,32 @@ const TraceVariant TRACE_VARIANTS_POW2[] = {
     {k_trace_stream<16, OCC_AHEAD | OCC_POW2>, 16, OCC_AHEAD},
     {k_trace_stream<24, OCC_AHEAD | OCC_POW2>, 24, OCC_AHEAD},
     {k_trace_stream<32, OCC_AHEAD | OCC_POW2>, 32, OCC_AHEAD},
+    {k_trace_lean<16, true>, 16, OCC_LEAN},
+    {k_trace_lean<24, true>, 24, OCC_LEAN},
+    {k_trace_lean<32, true>, 32, OCC_LEAN},
 };
 static_assert(sizeof(TRACE_VARIANTS_POW2) == sizeof(TRACE_VARIANTS), "one power-of-two kernel per variant");
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 // The traversal kernel of a variant for geometry g.
+// A lean variant needs the walled bitset; without it (an index wider than 31
+// bits, or ny or nz below 3) the brick variant 1 runs instead.
+constexpr int BRICK_VARIANT = 1;
+
+int usable_variant(const Geometry &g, int variant)
+{
+    return TRACE_VARIANTS[variant].occ == OCC_LEAN && !g.wocc ? BRICK_VARIANT : variant;
+}
+
 const TraceVariant &trace_variant_for(const Geometry &g, int variant)
 {
+    variant = usable_variant(g, variant);
     return is_pow2(g.ny) && is_pow2(g.nz) ? TRACE_VARIANTS_POW2[variant] : TRACE_VARIANTS[variant];
 }
-// Default: refill at 16 idle lanes with the LDS brick test, the fastest on
-// the foam and fibre media measured so far.
-int g_trace_variant = 1;
+// Default: the lean step refilling at 24 idle lanes, the fastest on the foam
+// measured so far (traversal 18.9 ms against 20.5 ms for the brick test).
+int g_trace_variant = 12;
 
 // Batch traversal kernels, one per entry of TRACE_VARIANTS (same index).
 using BatchKernel = void (*)(const Segment *, int, const DdaRec *, const WallRec *, uint2 *);
@@ -1077,7 +1358,8 @@ const BatchVariant BATCH_VARIANTS[] = {
     {k_trace_batch<48, OCC_BRICK>, 48, OCC_BRICK},   {k_trace_batch<32, OCC_GLOBAL>, 32, OCC_GLOBAL},
     {k_trace_batch<16, OCC_GLOBAL>, 16, OCC_GLOBAL}, {k_trace_batch<24, OCC_GLOBAL>, 24, OCC_GLOBAL},
     {k_trace_batch<16, OCC_AHEAD>, 16, OCC_AHEAD},   {k_trace_batch<24, OCC_AHEAD>, 24, OCC_AHEAD},
-    {k_trace_batch<32, OCC_AHEAD>, 32, OCC_AHEAD},
+    {k_trace_batch<32, OCC_AHEAD>, 32, OCC_AHEAD},   {k_trace_batch_lean<16, false>, 16, OCC_LEAN},
+    {k_trace_batch_lean<24, false>, 24, OCC_LEAN},   {k_trace_batch_lean<32, false>, 32, OCC_LEAN},
 };
 static_assert(sizeof(BATCH_VARIANTS) / sizeof(BATCH_VARIANTS[0]) == N_TRACE_VARIANTS,
               "one batch kernel per traversal variant");
@@ -1093,6 +1375,9 @@ const BatchVariant BATCH_VARIANTS_POW2[] = {
     {k_trace_batch<16, OCC_AHEAD | OCC_POW2>, 16, OCC_AHEAD},
     {k_trace_batch<24, OCC_AHEAD | OCC_POW2>, 24, OCC_AHEAD},
     {k_trace_batch<32, OCC_AHEAD | OCC_POW2>, 32, OCC_AHEAD},
+    {k_trace_batch_lean<16, true>, 16, OCC_LEAN},
+    {k_trace_batch_lean<24, true>, 24, OCC_LEAN},
+    {k_trace_batch_lean<32, true>, 32, OCC_LEAN},
 };
 
 // pow2: every geometry of the launch has power-of-two ny and nz.
@@ -1412,6 +1697,23 @@ int prh_tracer_grain(int32_t *records)
 {
     PRH_CHECK_ARG(records, "prh_tracer_grain: null pointer");
     *records = g_trace_grain;
+    return PRH_OK;
+}
+
+int prh_tracer_stats(int64_t *wave_steps, int64_t *refills, int32_t reset)
+{
+    hipStream_t st;
+    int rc = current_stream(&st);
+    if (rc) return rc;
+    PRH_HIP(hipStreamSynchronize(st));
+    unsigned long long v[2] = {0, 0};
+    PRH_HIP(hipMemcpyFromSymbol(v, HIP_SYMBOL(g_trace_stats), sizeof(v)));
+    if (wave_steps) *wave_steps = (int64_t)v[0];
+    if (refills) *refills = (int64_t)v[1];
+    if (reset) {
+        const unsigned long long z[2] = {0, 0};
+        PRH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(g_trace_stats), z, sizeof(z)));
+    }
     return PRH_OK;
 }
 
@@ -2030,6 +2332,7 @@ int prh_trace_batch_create(int32_t n_segments, const prh_geometry *const *geomet
     b->rays = total;
     b->use_graph = (flags & PRH_BATCH_DIRECT) == 0;
     b->variant = g_trace_variant;
+    for (int k = 0; k < n_segments; ++k) b->variant = usable_variant(geometries[k]->g, b->variant);
     b->edge_cap = edge_capacity > 0 ? edge_capacity : std::max<int64_t>(worst_edges, 1);
     for (int k = 0; k < n_segments; ++k) {
         b->geos.push_back(geometries[k]);

@@ -49,6 +49,7 @@ SIGNATURES = {
     "prh_libm_sincos": ([_P, _I64, _I32, _P, _P], ctypes.c_int),
     "prh_tracer_set_variant": ([_I32], ctypes.c_int),
     "prh_tracer_variants": ([_P, _P, _P, _P], ctypes.c_int),
+    "prh_tracer_stats": ([_P, _P, _I32], ctypes.c_int),
     "prh_tracer_set_grain": ([_I32], ctypes.c_int),
     "prh_tracer_grain": ([_P], ctypes.c_int),
     "prh_tracer_trim": ([], ctypes.c_int),
@@ -245,6 +246,15 @@ def tracer_variants() -> list[dict]:
     check(lib().prh_tracer_variants(None, None, ptr(refill), ptr(occ)), "prh_tracer_variants")
     return [{"variant": v, "refill_min": int(refill[v]), "occupancy": int(occ[v]), "current": v == cur.value}
             for v in range(n.value)]
+
+
+def tracer_stats(reset: bool = False) -> dict:
+    """Wave-steps and refills of the lean traversal since the last reset."""
+
+    steps = ctypes.c_int64()
+    refills = ctypes.c_int64()
+    check(lib().prh_tracer_stats(ctypes.byref(steps), ctypes.byref(refills), 1 if reset else 0), "prh_tracer_stats")
+    return {"wave_steps": int(steps.value), "refills": int(refills.value)}
 
 
 def set_tracer_variant(variant: int) -> None:
