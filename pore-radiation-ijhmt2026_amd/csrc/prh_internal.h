@@ -92,9 +92,8 @@ void host_seed_words(const uint32_t *words, int n_words, uint64_t out[4]);
 //   patch_*    per patch: origin (x,y,z) FP64, face 0..5 (walls: 0 hot, 1 cold)
 //   hot_map / cold_map  wall patch id per (j, k), -1 where the boundary cell is solid
 //   wocc       walled occupancy bitset in 4x4x4 blocks, one 64-bit word per block: bit P
-//              for the cell (i + 1, j, k), i in [-1, nx], where P interleaves the two low
-//              bits of i + 1, j, k (bits k0 j0 i0 k1 j1 i1 = the bit within the word) and
-//              holds the rest of k, j, i + 1 in fields above (lean_layout); the planes
+//              for the cell (i + 1, j, k), i in [-1, nx], where P is the Morton index of
+//              (i + 1, j, k) (lean_layout; bits 0-5 are the cell within the block); the planes
 //              i = -1 and i = nx (the hot and cold walls) are set, so one bit test ends
 //              a ray at a solid cell or a wall (null when P would need more than 31 bits
 //              or ny, nz < 3)
@@ -119,40 +118,66 @@ struct Geometry {
     int32_t wlx, wly, wlz;  // bit widths of nx + 2, ny, nz (each at least 2)
 };
 
-// Bit layout of the walled bitset's index P (Geometry::wocc): axis a's value v
-// sits at P bits lo_a (v bit 0), lo_a + 3 (v bit 1) and field_a.. (v >> 2),
-// with lo = 2, 1, 0 and the fields z, y, x upward from bit 6.  The set of an
-// axis's bits is its mask; P is the OR of the three dilated values.
+// Bit layout of the walled bitset's index P (Geometry::wocc): a Morton
+// order.  The low m bits of each axis's value (m = the smallest of the three
+// widths, at most 10) are interleaved as z, y, x, z, y, x, ... from bit 0, so
+// bits 0-5 (the bit within a block's word) hold the block's 4x4x4 cells and a
+// 128 B line holds 8x8x16 cells; the bits above m of each axis follow in
+// fields (z, then y, then x).  The set of an axis's bits is its mask; P is
+// the OR of the three dilated values.
 struct LeanLayout {
     uint32_t mask[3];   // x, y, z
     uint32_t field[3];  // first bit of each axis's field
+    uint32_t m;         // interleaved bits per axis
 };
+
+__host__ __device__ inline uint32_t morton_spread(uint32_t v)  // bit b -> bit 3b, b < 10
+{
+    v &= 0x000003ffu;
+    v = (v ^ (v << 16)) & 0xff0000ffu;
+    v = (v ^ (v << 8)) & 0x0300f00fu;
+    v = (v ^ (v << 4)) & 0x030c30c3u;
+    v = (v ^ (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__host__ __device__ inline uint32_t morton_gather(uint32_t v)  // bit 3b -> bit b
+{
+    v &= 0x09249249u;
+    v = (v ^ (v >> 2)) & 0x030c30c3u;
+    v = (v ^ (v >> 4)) & 0x0300f00fu;
+    v = (v ^ (v >> 8)) & 0xff0000ffu;
+    v = (v ^ (v >> 16)) & 0x000003ffu;
+    return v;
+}
 
 __host__ __device__ inline LeanLayout lean_layout(int lx, int ly, int lz)
 {
     LeanLayout L;
-    L.field[2] = 6;
-    L.field[1] = 6 + (uint32_t)(lz - 2);
-    L.field[0] = L.field[1] + (uint32_t)(ly - 2);
+    int m = lx < ly ? lx : ly;
+    m = m < lz ? m : lz;
+    m = m < 10 ? m : 10;
+    L.m = (uint32_t)m;
+    L.field[2] = 3u * L.m;
+    L.field[1] = L.field[2] + (uint32_t)(lz - m);
+    L.field[0] = L.field[1] + (uint32_t)(ly - m);
     const int bits[3] = {lx, ly, lz};
-    for (int a = 0; a < 3; ++a) {
-        const uint32_t lo = (uint32_t)(2 - a);
-        L.mask[a] = (1u << lo) | (1u << (lo + 3)) | (((1u << (bits[a] - 2)) - 1u) << L.field[a]);
-    }
+    const uint32_t inter = morton_spread((1u << L.m) - 1u);
+    for (int a = 0; a < 3; ++a)
+        L.mask[a] = inter << (2 - a) | (((1u << (bits[a] - m)) - 1u) << L.field[a]);
     return L;
 }
 
 // Value v of axis a (0 x, 1 y, 2 z) spread to its bits of P, and back.
 __host__ __device__ inline uint32_t lean_dilate(const LeanLayout &L, int a, uint32_t v)
 {
-    const uint32_t lo = (uint32_t)(2 - a);
-    return (v & 1u) << lo | ((v >> 1) & 1u) << (lo + 3) | (v >> 2) << L.field[a];
+    return morton_spread(v & ((1u << L.m) - 1u)) << (2 - a) | (v >> L.m) << L.field[a];
 }
 
 __host__ __device__ inline uint32_t lean_compact(const LeanLayout &L, int a, uint32_t d)
 {
-    const uint32_t lo = (uint32_t)(2 - a);
-    return ((d >> lo) & 1u) | ((d >> (lo + 2)) & 2u) | ((d & L.mask[a]) >> L.field[a]) << 2;
+    const uint32_t inter = morton_spread((1u << L.m) - 1u) << (2 - a);
+    return morton_gather((d & inter) >> (2 - a)) | ((d & L.mask[a] & ~inter) >> L.field[a]) << L.m;
 }
 
 // Device exclusive scan helpers (prh_scan.hip).

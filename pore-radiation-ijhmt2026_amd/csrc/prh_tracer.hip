@@ -88,12 +88,23 @@ struct alignas(16) WallRec {
     double oy, oz, dy, dz;
 };
 
-// Final state of a traversed ray, 8 B: x = cell index (solid) or the wall
-// cell's j (walls); y = kind << 28 | face (solid) or the wall cell's k.
-// FIN_SOLID_P: a solid cell given by its walled-bitset index (x = P, see
-// Geometry::wocc), written by the lean traversal and turned into the cell
-// index when the receiver is resolved.
+// Final state of a traversed ray, 8 B.  A ray that the lean traversal ends
+// on a wall keeps the time of its last crossing as a double, negated for the
+// cold wall; the wall cell is worked out from it (and the ray's wall record)
+// when the receiver is resolved.  Every other final state is a NaN: high word
+// FIN_BOX | kind << 16 | low 16 bits, low word a 32-bit payload:
+//   FIN_FAILED    nothing;
+//   FIN_SOLID     payload the cell index, low bits the face;
+//   FIN_HOT/COLD  payload the wall cell's j, low bits its k;
+//   FIN_SOLID_P   payload the cell's walled-bitset index P (Geometry::wocc),
+//                 low bits the face.
 enum : uint32_t { FIN_FAILED = 0, FIN_SOLID = 1, FIN_HOT = 2, FIN_COLD = 3, FIN_SOLID_P = 4 };
+constexpr uint32_t FIN_BOX = 0x7FF00000u;
+
+__device__ __forceinline__ uint2 fin_box(uint32_t kind, uint32_t payload, uint32_t low16)
+{
+    return make_uint2(payload, FIN_BOX | kind << 16 | low16);
+}
 
 // Per-lane traversal state (the loop variables of _trace_kernel).  The face
 // of the last crossing is kept as its axis (priority x > y > z) and turned
@@ -355,7 +366,7 @@ __device__ __forceinline__ uint32_t wrap_floor(double v, int n, double inv_n)
 // Final state of a finished traversal (see FIN_*).
 __device__ __forceinline__ uint2 ray_final(const Ray &R, const GeoView &G, const WallRec *walls, uint32_t g)
 {
-    if (R.left == 0u) return make_uint2(0u, FIN_FAILED << 28);
+    if (R.left == 0u) return fin_box(FIN_FAILED, 0u, 0u);
     if ((unsigned)R.i >= (unsigned)G.nx) {
         const double2 *p = reinterpret_cast<const double2 *>(walls + g);
         const double2 o = p[0], d = p[1];
@@ -363,49 +374,74 @@ __device__ __forceinline__ uint2 ray_final(const Ray &R, const GeoView &G, const
         const double zw = o.y + d.y * R.time;
         const uint32_t jw = wrap_floor(yw, G.ny, G.inv_ny);
         const uint32_t kw = wrap_floor(zw, G.nz, G.inv_nz);
-        return make_uint2(jw, (R.i < 0 ? FIN_HOT : FIN_COLD) << 28 | kw);
+        return fin_box(R.i < 0 ? FIN_HOT : FIN_COLD, jw, kw);
     }
-    if (R.axis < 0) return make_uint2(0u, FIN_FAILED << 28);
+    if (R.axis < 0) return fin_box(FIN_FAILED, 0u, 0u);
     // Direction bits as values (a select between struct fields would be
     // lowered to an indexed stack access).
     const uint32_t up = (R.sx > 0 ? 1u : 0u) | (R.sy > 0 ? 2u : 0u) | (R.sz > 0 ? 4u : 0u);
     const uint32_t face = 2u * (uint32_t)R.axis + ((up >> R.axis) & 1u);
-    return make_uint2(cell_index(R, G), FIN_SOLID << 28 | face);
+    return fin_box(FIN_SOLID, cell_index(R, G), face);
 }
 
 // Receiving patch of a final state (view_factors.py:226-236), -1 on failure:
-// the wall maps, or the patch of (cell, face) from the cell code (first
-// patch id of the cell << 6 | mask of its exposed faces).
-__device__ __forceinline__ int resolve_receiver(uint2 f, const GeoView &G)
+// the wall maps (the wall cell given, or worked out from the crossing time
+// as ray_final does), or the patch of (cell, face) from the cell code (first
+// patch id of the cell << 6 | mask of its exposed faces).  w is the ray's
+// wall record.
+__device__ __forceinline__ int patch_of_face(uint32_t cell, uint32_t face, const GeoView &G)
 {
-    const uint32_t kind = f.y >> 28;
+    const uint32_t code = G.cell_code[cell];
+    const uint32_t m = code & 63u;
+    if (!((m >> face) & 1u)) return -1;
+    return (int)(code >> 6) + __popc(m & ((1u << face) - 1u));
+}
+
+__device__ __forceinline__ bool fin_is_time(uint2 f)
+{
+    return (f.y & FIN_BOX) != FIN_BOX;
+}
+
+// The wall patch of a crossing time: floor(o + d t) mod n as ray_final forms it.
+__device__ __forceinline__ int resolve_wall_time(uint2 f, const GeoView &G, const WallRec &w)
+{
+    const double time = __hiloint2double((int)(f.y & 0x7FFFFFFFu), (int)f.x);
+    const double yw = w.oy + w.dy * time;
+    const double zw = w.oz + w.dz * time;
+    const uint32_t jw = wrap_floor(yw, G.ny, G.inv_ny);
+    const uint32_t kw = wrap_floor(zw, G.nz, G.inv_nz);
+    const size_t c = (size_t)jw * (size_t)G.nz + kw;
+    return (f.y >> 31) ? G.cold_map[c] : G.hot_map[c];
+}
+
+// The receiver of a boxed final state (any but a crossing time).
+__device__ __forceinline__ int resolve_boxed(uint2 f, const GeoView &G)
+{
+    const uint32_t kind = (f.y >> 16) & 15u;
+    const uint32_t low = f.y & 0xFFFFu;
     if (kind == FIN_SOLID) {
-        const uint32_t face = f.y & 7u;
-        if ((uint64_t)f.x >= (uint64_t)G.nx * (uint64_t)G.ny * (uint64_t)G.nz || face > 5u) return -1;
-        const uint32_t code = G.cell_code[f.x];
-        const uint32_t m = code & 63u;
-        if (!((m >> face) & 1u)) return -1;
-        return (int)(code >> 6) + __popc(m & ((1u << face) - 1u));
+        if ((uint64_t)f.x >= (uint64_t)G.nx * (uint64_t)G.ny * (uint64_t)G.nz || low > 5u) return -1;
+        return patch_of_face(f.x, low, G);
     }
     if (kind == FIN_SOLID_P) {
-        const uint32_t face = f.y & 7u;
-        if (!G.wocc || face > 5u) return -1;
+        if (!G.wocc || low > 5u) return -1;
         const uint32_t ip = lean_compact(G.wl, 0, f.x & G.wl.mask[0]);
         const uint32_t j = lean_compact(G.wl, 1, f.x & G.wl.mask[1]);
         const uint32_t k = lean_compact(G.wl, 2, f.x & G.wl.mask[2]);
         if (ip < 1u || ip > (uint32_t)G.nx || j >= (uint32_t)G.ny || k >= (uint32_t)G.nz) return -1;
-        const uint32_t code = G.cell_code[((size_t)(ip - 1u) * (uint32_t)G.ny + j) * (uint32_t)G.nz + k];
-        const uint32_t m = code & 63u;
-        if (!((m >> face) & 1u)) return -1;
-        return (int)(code >> 6) + __popc(m & ((1u << face) - 1u));
+        return patch_of_face(((ip - 1u) * (uint32_t)G.ny + j) * (uint32_t)G.nz + k, low, G);
     }
     if (kind == FIN_HOT || kind == FIN_COLD) {
-        const uint32_t kw = f.y & 0x0FFFFFFFu;
-        if (f.x >= (uint32_t)G.ny || kw >= (uint32_t)G.nz) return -1;
-        const size_t c = (size_t)f.x * (size_t)G.nz + kw;
+        if (f.x >= (uint32_t)G.ny || low >= (uint32_t)G.nz) return -1;
+        const size_t c = (size_t)f.x * (size_t)G.nz + low;
         return kind == FIN_HOT ? G.hot_map[c] : G.cold_map[c];
     }
     return -1;
+}
+
+__device__ __forceinline__ int resolve_receiver(uint2 f, const GeoView &G, const WallRec &w)
+{
+    return fin_is_time(f) ? resolve_wall_time(f, G, w) : resolve_boxed(f, G);
 }
 
 __device__ __forceinline__ void load_seed(const uint64_t *seeds, int s, u128 &st, u128 &inc)
@@ -672,7 +708,7 @@ __device__ __forceinline__ void trace_body(const GeoView &G, uint32_t base, uint
                         pending = true;
                         g_cur = g;
                     } else {
-                        fin[g] = make_uint2(0u, FIN_FAILED << 28);
+                        fin[g] = fin_box(FIN_FAILED, 0u, 0u);
                     }
                 }
             }
@@ -794,46 +830,41 @@ __device__ __forceinline__ bool lean_load(LeanRay &R, const DdaRec *recs, uint32
 }
 
 // One iteration of view_factors.py:225-255 on the walled bitset; returns
-// whether the ray goes on.  The crossing is worked out before the cell's
-// bitset word is tested (it does not depend on it), and taken only if the
-// cell is empty: the masks of the crossed axes are cleared for a solid cell,
-// so the commit needs no branch.  The word of the next cell's block is
-// requested right after the crossing and first tested by the next step, so
-// its latency overlaps that step's crossing work.
+// whether the ray goes on.  The crossing times are compared before the
+// cell's bitset word is tested (they do not depend on it), so they overlap
+// the word's load; the crossing is taken only by the lanes whose cell is
+// empty.  The word of the next cell's block is requested right after the
+// crossing and first tested by the next step, so its latency overlaps that
+// step's comparisons.
 template <bool POW2>
 __device__ __forceinline__ bool lean_step(LeanRay &R, const GeoView &G)
 {
     const double nt = vmin_f64(vmin_f64(R.tx, R.ty), R.tz);
-    bool cx = fabs(R.tx - nt) <= TIE_TOLERANCE;
-    bool cy = fabs(R.ty - nt) <= TIE_TOLERANCE;
-    bool cz = fabs(R.tz - nt) <= TIE_TOLERANCE;
+    const bool cx = fabs(R.tx - nt) <= TIE_TOLERANCE;
+    const bool cy = fabs(R.ty - nt) <= TIE_TOLERANCE;
+    const bool cz = fabs(R.tz - nt) <= TIE_TOLERANCE;
     __builtin_amdgcn_sched_barrier(0);
-    const bool go = !((uint32_t)(R.blk >> (R.P & 63u)) & 1u);  // not a solid cell or wall
-    cx = cx && go;
-    cy = cy && go;
-    cz = cz && go;
+    if ((uint32_t)(R.blk >> (R.P & 63u)) & 1u) return false;  // a solid cell or a wall: the ray ends here
     R.tx = fma_f64(cx ? 1.0 : 0.0, R.ddx, R.tx);
     R.ty = fma_f64(cy ? 1.0 : 0.0, R.ddy, R.ty);
     R.tz = fma_f64(cz ? 1.0 : 0.0, R.ddz, R.tz);
-    const uint32_t dx = cx ? R.sx : 0u, dy = cy ? R.sy : 0u, dz = cz ? R.sz : 0u;
-    R.xd = (R.xd - dx) & G.wl.mask[0];
-    R.yd = (R.yd - dy) & G.wl.mask[1];
-    R.zd = (R.zd - dz) & G.wl.mask[2];
+    R.last_x = cx ? R.sx : 0u;
+    R.last_y = cy ? R.sy : 0u;
+    R.xd = (R.xd - R.last_x) & G.wl.mask[0];
+    R.yd = (R.yd - R.last_y) & G.wl.mask[1];
+    R.zd = (R.zd - (cz ? R.sz : 0u)) & G.wl.mask[2];
     if (!POW2) {
         R.yd = R.yd == R.from_y ? R.to_y : R.yd;
         R.zd = R.zd == R.from_z ? R.to_z : R.zd;
     }
-    R.last_x = go ? dx : R.last_x;
-    R.last_y = go ? dy : R.last_y;
-    R.time = go ? nt : R.time;
-    R.left -= go ? 1u : 0u;
+    R.time = nt;
     R.P = R.xd | R.yd | R.zd;
     const uint32_t at = (R.P >> 3) & ~7u;  // byte offset of the next cell's block word
     if (at != R.blk_at) {
         R.blk = *reinterpret_cast<const uint64_t *>(reinterpret_cast<const char *>(G.wocc) + at);
         R.blk_at = at;
     }
-    return go && R.left != 0u;
+    return --R.left != 0u;
 }
 
 // Whether a finished lean ray stands on a wall plane (dilated i + 1 of 0 or nx + 1).
@@ -850,32 +881,29 @@ __device__ __forceinline__ uint32_t lean_reads(const LeanRay &R, const GeoView &
     return (R.cap - R.left) + (at_solid ? 1u : 0u);
 }
 
-// Final state of a finished lean traversal: FIN_FAILED, a wall cell as
-// ray_final gives it, or FIN_SOLID_P with the cell's index P.
-__device__ __forceinline__ uint2 lean_final(const LeanRay &R, const GeoView &G, const WallRec *walls, uint32_t g)
+// Final state of a finished lean traversal: FIN_FAILED, the crossing time of
+// a wall (negated for the cold wall), or FIN_SOLID_P with the cell's index P.
+__device__ __forceinline__ uint2 lean_final(const LeanRay &R, const GeoView &G)
 {
-    if (R.left == 0u) return make_uint2(0u, FIN_FAILED << 28);
+    if (R.left == 0u) return fin_box(FIN_FAILED, 0u, 0u);
     if (lean_at_wall(R, G)) {
-        const double2 *p = reinterpret_cast<const double2 *>(walls + g);
-        const double2 o = p[0], d = p[1];
-        const double yw = o.x + d.x * R.time;
-        const double zw = o.y + d.y * R.time;
-        const uint32_t jw = wrap_floor(yw, G.ny, G.inv_ny);
-        const uint32_t kw = wrap_floor(zw, G.nz, G.inv_nz);
-        return make_uint2(jw, (R.xd == 0u ? FIN_HOT : FIN_COLD) << 28 | kw);
+        const unsigned long long t = (unsigned long long)__double_as_longlong(R.time);
+        return make_uint2((uint32_t)t, (uint32_t)(t >> 32) | (R.xd == 0u ? 0u : 0x80000000u));
     }
-    if (R.left == R.cap) return make_uint2(0u, FIN_FAILED << 28);  // solid from the start: no face
+    if (R.left == R.cap) return fin_box(FIN_FAILED, 0u, 0u);  // solid from the start: no face
     // the face of the last crossing: the first axis crossed in x > y > z order;
     // a +1 step is the axis mask, a -1 step is 1
     const uint32_t axis = R.last_x != 0u ? 0u : (R.last_y != 0u ? 1u : 2u);
     const uint32_t step = axis == 0u ? R.last_x : (axis == 1u ? R.last_y : R.sz);
     const uint32_t face = 2u * axis + (step != 1u ? 1u : 0u);
-    return make_uint2(R.P, FIN_SOLID_P << 28 | face);
+    return fin_box(FIN_SOLID_P, R.P, face);
 }
 
 // trace_body with the lean step: the same queue, refill and final-state order,
 // in one loop whose refill part runs only when REFILL_MIN lanes are idle (or
 // none is active), so the stepping is not a loop of its own.
+constexpr int LEAN_STEPS = 4;  // steps between the wavefront's refill checks
+
 template <int REFILL_MIN, bool POW2>
 __device__ __forceinline__ void lean_body(const GeoView &G, uint32_t base, uint32_t total, uint32_t grain,
                                           unsigned long long *queue, const DdaRec *dda, const WallRec *walls,
@@ -901,7 +929,7 @@ __device__ __forceinline__ void lean_body(const GeoView &G, uint32_t base, uint3
             ++refills;
             if (!active && pending) {
                 reads += lean_reads(R, G);
-                fin[g_cur] = lean_final(R, G, walls, g_cur);
+                fin[g_cur] = lean_final(R, G);
                 pending = false;
             }
             if (more && cursor >= w_end) {
@@ -926,7 +954,7 @@ __device__ __forceinline__ void lean_body(const GeoView &G, uint32_t base, uint3
                             pending = true;
                             g_cur = g;
                         } else {
-                            fin[g] = make_uint2(0u, FIN_FAILED << 28);
+                            fin[g] = fin_box(FIN_FAILED, 0u, 0u);
                         }
                     }
                 }
@@ -935,9 +963,10 @@ __device__ __forceinline__ void lean_body(const GeoView &G, uint32_t base, uint3
             n_active = __popcll(__ballot(active));
             finished = n_active == 0 && !more && cursor >= w_end;
         }
-        if (n_active > 0) wave_steps += 2;
-        if (active) active = lean_step<POW2>(R, G);
-        if (active) active = lean_step<POW2>(R, G);
+        if (n_active > 0) wave_steps += LEAN_STEPS;
+#pragma unroll
+        for (int k = 0; k < LEAN_STEPS; ++k)
+            if (active) active = lean_step<POW2>(R, G);
         n_active = __popcll(__ballot(active));
     } while (!finished);
     reads = wave_sum_u64(reads);
@@ -972,13 +1001,14 @@ __global__ void k_zero_u64(unsigned long long *p, int n)
 }
 
 // Receivers of final states (prh_trace_rays), with the count of failures.
-__global__ __launch_bounds__(TRACE_THREADS) void k_resolve(const uint2 *fin, GeoView G, uint32_t total,
-                                                           int32_t *receiver, unsigned long long *failed)
+__global__ __launch_bounds__(TRACE_THREADS) void k_resolve(const uint2 *fin, const WallRec *walls, GeoView G,
+                                                           uint32_t total, int32_t *receiver,
+                                                           unsigned long long *failed)
 {
     const uint32_t g = blockIdx.x * TRACE_THREADS + threadIdx.x;
     int rec = 0;
     if (g < total) {
-        rec = resolve_receiver(fin[g], G);
+        rec = resolve_receiver(fin[g], G, walls[g]);
         receiver[g] = rec;
     }
     const uint64_t f = wave_sum_u64(g < total && rec < 0 ? 1ull : 0ull);
@@ -990,7 +1020,8 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_resolve(const uint2 *fin, Geo
 // pool_counts does, view_factors.py:290-300).  One workgroup per patch: the
 // receivers of the patch's S*R final states are resolved on load (failures
 // counted), then bitonic-sorted in LDS.
-__global__ __launch_bounds__(COUNT_THREADS) void k_count_patches(const uint2 *fin, GeoView G, int64_t n_p, int R, int S,
+__global__ __launch_bounds__(COUNT_THREADS) void k_count_patches(const uint2 *fin, const WallRec *walls, GeoView G,
+                                                                 int64_t n_p, int R, int S,
                                                                  int SRp2, int32_t *uniq_recv, int32_t *uniq_cnt,
                                                                  int32_t *n_unique, unsigned long long *failed)
 {
@@ -1005,7 +1036,8 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_patches(const uint2 *fi
             int32_t v = INT_MAX;
             if (t < SR) {
                 const int s = t / R, r = t - s * R;
-                v = resolve_receiver(fin[(int64_t)s * n_p * R + q * R + r], G);
+                const int64_t at = (int64_t)s * n_p * R + q * R + r;
+                v = resolve_receiver(fin[at], G, walls[at]);
                 if (v < 0) {
                     ++bad;
                     v = INT_MAX;
@@ -1070,7 +1102,8 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_patches(const uint2 *fi
 // neighbour shuffle and a wave prefix sum; run starts go through LDS to
 // give each run its length.
 template <int EPL>
-__global__ __launch_bounds__(COUNT_THREADS) void k_count_wave(const uint2 *fin, GeoView G, int64_t n_p, int R, int S,
+__global__ __launch_bounds__(COUNT_THREADS) void k_count_wave(const uint2 *fin, const WallRec *walls, GeoView G,
+                                                              int64_t n_p, int R, int S,
                                                               int32_t *uniq_recv, int32_t *uniq_cnt,
                                                               int32_t *n_unique, unsigned long long *failed)
 {
@@ -1083,9 +1116,11 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_wave(const uint2 *fin, 
     int32_t *pos = s_pos[wave];
     int32_t *ukey = s_key[wave];
     const int SR = S * R;
+    const uint64_t below = (1ull << lane) - 1ull;
     for (int64_t q = (int64_t)blockIdx.x * WAVES + wave; q < n_p; q += (int64_t)gridDim.x * WAVES) {
         int32_t key[EPL];
         uint32_t bad = 0;
+        uint32_t timed = 0;  // bit r: key r is a wall crossing time, resolved below
         // key r of a lane is the patch's ray lane + 64 r, so each load
         // instruction reads 64 consecutive final states (the sort below does
         // not depend on where a key starts)
@@ -1095,13 +1130,61 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_wave(const uint2 *fin, 
             int32_t v = INT_MAX;
             if (t < SR) {
                 const int s = S == 1 ? 0 : t / R, u = t - s * R;
-                v = resolve_receiver(fin[(int64_t)s * n_p * R + q * R + u], G);
-                if (v < 0) {
-                    ++bad;
-                    v = INT_MAX;
+                const uint2 f = fin[(int64_t)s * n_p * R + q * R + u];
+                if (fin_is_time(f)) {
+                    timed |= 1u << r;
+                } else {
+                    v = resolve_boxed(f, G);
+                    if (v < 0) {
+                        ++bad;
+                        v = INT_MAX;
+                    }
                 }
             }
             key[r] = v;
+        }
+        // Wall crossing times: listed in LDS and resolved one per lane, so the
+        // wall arithmetic and its record loads run once per 64 such rays.
+        if (__ballot(timed != 0u)) {
+            int32_t *list = pos, *res = ukey;
+            int n = 0;
+#pragma unroll
+            for (int r = 0; r < EPL; ++r) {
+                const bool w = (timed >> r) & 1u;
+                const uint64_t m = __ballot(w);
+                if (w) list[n + __popcll(m & below)] = lane + 64 * r;
+                n += __popcll(m);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            for (int i = lane; i < n; i += 64) {
+                const int t = list[i];
+                const int s = S == 1 ? 0 : t / R, u = t - s * R;
+                const int64_t at = (int64_t)s * n_p * R + q * R + u;
+                res[i] = resolve_wall_time(fin[at], G, walls[at]);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            n = 0;
+#pragma unroll
+            for (int r = 0; r < EPL; ++r) {
+                const bool w = (timed >> r) & 1u;
+                const uint64_t m = __ballot(w);
+                if (w) {
+                    int32_t v = res[n + __popcll(m & below)];
+                    if (v < 0) {
+                        ++bad;
+                        v = INT_MAX;
+                    }
+                    key[r] = v;
+                }
+                n += __popcll(m);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         }
         bad = (uint32_t)wave_sum_u64(bad);
         if (lane == 0 && bad) atomicAdd(failed, (unsigned long long)bad);
@@ -1304,6 +1387,7 @@ const TraceVariant TRACE_VARIANTS[] = {
     {k_trace_stream<16, OCC_AHEAD>, 16, OCC_AHEAD},   {k_trace_stream<24, OCC_AHEAD>, 24, OCC_AHEAD},
     {k_trace_stream<32, OCC_AHEAD>, 32, OCC_AHEAD},   {k_trace_lean<16, false>, 16, OCC_LEAN},
     {k_trace_lean<24, false>, 24, OCC_LEAN},          {k_trace_lean<32, false>, 32, OCC_LEAN},
+    {k_trace_lean<8, false>, 8, OCC_LEAN},            {k_trace_lean<12, false>, 12, OCC_LEAN},
 };
 constexpr int N_TRACE_VARIANTS = sizeof(TRACE_VARIANTS) / sizeof(TRACE_VARIANTS[0]);
 // The same variants for geometries whose ny and nz are powers of two.
@@ -1322,6 +1406,8 @@ const TraceVariant TRACE_VARIANTS_POW2[] = {
     {k_trace_lean<16, true>, 16, OCC_LEAN},
     {k_trace_lean<24, true>, 24, OCC_LEAN},
     {k_trace_lean<32, true>, 32, OCC_LEAN},
+    {k_trace_lean<8, true>, 8, OCC_LEAN},
+    {k_trace_lean<12, true>, 12, OCC_LEAN},
 };
 static_assert(sizeof(TRACE_VARIANTS_POW2) == sizeof(TRACE_VARIANTS), "one power-of-two kernel per variant");
 
@@ -1342,9 +1428,9 @@ const TraceVariant &trace_variant_for(const Geometry &g, int variant)
     variant = usable_variant(g, variant);
     return is_pow2(g.ny) && is_pow2(g.nz) ? TRACE_VARIANTS_POW2[variant] : TRACE_VARIANTS[variant];
 }
-// Default: the lean step refilling at 24 idle lanes, the fastest on the foam
-// measured so far (traversal 18.9 ms against 20.5 ms for the brick test).
-int g_trace_variant = 12;
+// Default: the lean step refilling at 16 idle lanes, the fastest on the foam
+// measured so far (traversal 18.0 ms against 20.5 ms for the brick test).
+int g_trace_variant = 11;
 
 // Batch traversal kernels, one per entry of TRACE_VARIANTS (same index).
 using BatchKernel = void (*)(const Segment *, int, const DdaRec *, const WallRec *, uint2 *);
@@ -1360,6 +1446,7 @@ const BatchVariant BATCH_VARIANTS[] = {
     {k_trace_batch<16, OCC_AHEAD>, 16, OCC_AHEAD},   {k_trace_batch<24, OCC_AHEAD>, 24, OCC_AHEAD},
     {k_trace_batch<32, OCC_AHEAD>, 32, OCC_AHEAD},   {k_trace_batch_lean<16, false>, 16, OCC_LEAN},
     {k_trace_batch_lean<24, false>, 24, OCC_LEAN},   {k_trace_batch_lean<32, false>, 32, OCC_LEAN},
+    {k_trace_batch_lean<8, false>, 8, OCC_LEAN},     {k_trace_batch_lean<12, false>, 12, OCC_LEAN},
 };
 static_assert(sizeof(BATCH_VARIANTS) / sizeof(BATCH_VARIANTS[0]) == N_TRACE_VARIANTS,
               "one batch kernel per traversal variant");
@@ -1378,6 +1465,8 @@ const BatchVariant BATCH_VARIANTS_POW2[] = {
     {k_trace_batch_lean<16, true>, 16, OCC_LEAN},
     {k_trace_batch_lean<24, true>, 24, OCC_LEAN},
     {k_trace_batch_lean<32, true>, 32, OCC_LEAN},
+    {k_trace_batch_lean<8, true>, 8, OCC_LEAN},
+    {k_trace_batch_lean<12, true>, 12, OCC_LEAN},
 };
 
 // pow2: every geometry of the launch has power-of-two ny and nz.
@@ -1503,11 +1592,13 @@ int trace_pass(const Geometry &g, Workspace &W, int64_t p0, int64_t n_p, int R, 
                     : epl == 4  ? k_count_wave<4>
                     : epl == 8  ? k_count_wave<8>
                                 : k_count_wave<16>;
-        kern<<<wgrid, COUNT_THREADS, 0, st>>>(W.fin.as<uint2>(), view_of(g), n_p, R, S, W.uniq_recv.as<int32_t>(),
+        kern<<<wgrid, COUNT_THREADS, 0, st>>>(W.fin.as<uint2>(), W.walls.as<WallRec>(), view_of(g), n_p, R, S,
+                                              W.uniq_recv.as<int32_t>(),
                                               W.uniq_cnt.as<int32_t>(), W.n_unique.as<int32_t>(), ctr + 1);
     } else {
         const size_t lds = sizeof(int32_t) * (SRp2 + COUNT_THREADS);
-        k_count_patches<<<cgrid, COUNT_THREADS, lds, st>>>(W.fin.as<uint2>(), view_of(g), n_p, R, S, SRp2,
+        k_count_patches<<<cgrid, COUNT_THREADS, lds, st>>>(W.fin.as<uint2>(), W.walls.as<WallRec>(), view_of(g), n_p,
+                                                           R, S, SRp2,
                                                            W.uniq_recv.as<int32_t>(), W.uniq_cnt.as<int32_t>(),
                                                            W.n_unique.as<int32_t>(), ctr + 1);
     }
@@ -1807,8 +1898,8 @@ int prh_trace_rays(const prh_geometry *geo, const double *origins, const double 
         if ((rc = launch_trace(g, W.dda.as<DdaRec>(), W.walls.as<WallRec>(), total, W.fin.as<uint2>(), ctr, ctr + 2,
                                st)))
             return rc;
-        k_resolve<<<grid, TRACE_THREADS, 0, st>>>(W.fin.as<uint2>(), view_of(g), total, W.receiver.as<int32_t>(),
-                                                  ctr + 1);
+        k_resolve<<<grid, TRACE_THREADS, 0, st>>>(W.fin.as<uint2>(), W.walls.as<WallRec>(), view_of(g), total,
+                                                  W.receiver.as<int32_t>(), ctr + 1);
         PRH_HIP(hipGetLastError());
         unsigned long long counters[2] = {0, 0};
         PRH_HIP(hipMemcpyAsync(receiver + r0, W.receiver.p, k * sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -1988,10 +2079,10 @@ int prh_trace_counts_into(const prh_geometry *geo, int64_t p0, int64_t p1, int32
                         : epl == 8  ? k_count_wave<8>
                                     : k_count_wave<16>;
             kern<<<(unsigned)std::min<int64_t>(blocks, 65536), COUNT_THREADS, 0, st>>>(
-                fin, view_of(g), n_p, R, S, uniq_recv, uniq_cnt, n_unique, ctr + 1);
+                fin, walls, view_of(g), n_p, R, S, uniq_recv, uniq_cnt, n_unique, ctr + 1);
         } else {
             k_count_patches<<<(unsigned)std::min<int64_t>(n_p, 65536), COUNT_THREADS,
-                              sizeof(int32_t) * (SRp2 + COUNT_THREADS), st>>>(fin, view_of(g), n_p, R, S, SRp2,
+                              sizeof(int32_t) * (SRp2 + COUNT_THREADS), st>>>(fin, walls, view_of(g), n_p, R, S, SRp2,
                                                                                uniq_recv, uniq_cnt, n_unique, ctr + 1);
         }
         PRH_HIP(hipGetLastError());
@@ -2144,6 +2235,7 @@ void count_launch(const prh_trace_batch *b, const BatchBuffers &B, const BatchPi
     int SRp2 = 1;
     while (SRp2 < R * S) SRp2 <<= 1;
     const uint2 *fin = B.fin + P.ray_off;
+    const WallRec *walls = B.walls + P.ray_off;
     int32_t *ur = B.uniq_recv + P.ray_off, *uc = B.uniq_cnt + P.ray_off, *nu = B.n_unique + P.patch_off;
     unsigned long long *failed = b->counters + b->n_seg;
     if (SRp2 <= 1024) {
@@ -2152,11 +2244,12 @@ void count_launch(const prh_trace_batch *b, const BatchBuffers &B, const BatchPi
         const unsigned wgrid = (unsigned)(blocks < 65536 ? blocks : 65536);
         auto kern = epl == 1 ? k_count_wave<1> : epl == 2 ? k_count_wave<2> : epl == 4 ? k_count_wave<4>
                     : epl == 8 ? k_count_wave<8> : k_count_wave<16>;
-        kern<<<wgrid, COUNT_THREADS, 0, st>>>(fin, view_of(g), np, R, S, ur, uc, nu, failed);
+        kern<<<wgrid, COUNT_THREADS, 0, st>>>(fin, walls, view_of(g), np, R, S, ur, uc, nu, failed);
     } else {
         const unsigned cgrid = (unsigned)(np < 65536 ? np : 65536);
         const size_t lds = sizeof(int32_t) * (SRp2 + COUNT_THREADS);
-        k_count_patches<<<cgrid, COUNT_THREADS, lds, st>>>(fin, view_of(g), np, R, S, SRp2, ur, uc, nu, failed);
+        k_count_patches<<<cgrid, COUNT_THREADS, lds, st>>>(fin, walls, view_of(g), np, R, S, SRp2, ur, uc, nu,
+                                                           failed);
     }
 }
 
