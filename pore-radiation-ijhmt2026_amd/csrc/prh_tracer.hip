@@ -778,7 +778,13 @@ void k_trace_batch(const Segment *segs, int n_seg, const DdaRec *dda, const Wall
 // last crossing are (x and y; z crossed when neither did), so the face
 // follows from them when the ray ends.
 struct LeanRay {
-    double tx, ty, tz, ddx, ddy, ddz, time;
+    double tx, ty, tz, time;
+    // The crossing increments of t: low words of ddx, ddy, ddz with their
+    // high words set for the axes that cross (0 otherwise: the value is then
+    // a subnormal or zero, and t, which is always a positive normal number,
+    // does not change when it is added).
+    double incx, incy, incz;
+    uint32_t hix, hiy, hiz;  // high words of ddx, ddy, ddz
     uint32_t xd, yd, zd;     // dilated i + 1, j, k
     uint32_t sx, sy, sz;     // dilated steps: the axis mask (+1) or 1 (-1); 0 for no step
     uint32_t last_x, last_y; // the x and y steps of the last crossing (0: not crossed)
@@ -786,7 +792,8 @@ struct LeanRay {
     uint32_t P;              // xd | yd | zd: the cell's bit in the walled bitset
     uint64_t blk;            // the bitset word of the cell's block
     uint32_t blk_at;         // its byte offset
-    uint32_t left, cap;
+    uint32_t left1;          // iterations left before the step cap, minus one (wraps when the cap is reached)
+    uint32_t cap;
 };
 
 template <bool POW2>
@@ -798,9 +805,12 @@ __device__ __forceinline__ bool lean_load(LeanRay &R, const DdaRec *recs, uint32
     R.tx = a.x;
     R.ty = a.y;
     R.tz = b.x;
-    R.ddx = b.y;
-    R.ddy = c.x;
-    R.ddz = c.y;
+    R.incx = b.y;
+    R.incy = c.x;
+    R.incz = c.y;
+    R.hix = (uint32_t)__double2hiint(b.y);
+    R.hiy = (uint32_t)__double2hiint(c.x);
+    R.hiz = (uint32_t)__double2hiint(c.y);
     const LeanLayout &L = G.wl;
     // the record holds the start cell's index P and the step signs
     R.P = w.w;
@@ -822,7 +832,7 @@ __device__ __forceinline__ bool lean_load(LeanRay &R, const DdaRec *recs, uint32
     R.last_x = R.last_y = 0u;
     R.time = 0.0;
     R.cap = w.z;
-    R.left = w.z;
+    R.left1 = w.z - 1u;
     // the block of the first cell, read by the first step
     R.blk_at = (R.P >> 3) & ~7u;
     R.blk = *reinterpret_cast<const uint64_t *>(reinterpret_cast<const char *>(G.wocc) + R.blk_at);
@@ -845,9 +855,14 @@ __device__ __forceinline__ bool lean_step(LeanRay &R, const GeoView &G)
     const bool cz = fabs(R.tz - nt) <= TIE_TOLERANCE;
     __builtin_amdgcn_sched_barrier(0);
     if ((uint32_t)(R.blk >> (R.P & 63u)) & 1u) return false;  // a solid cell or a wall: the ray ends here
-    R.tx = fma_f64(cx ? 1.0 : 0.0, R.ddx, R.tx);
-    R.ty = fma_f64(cy ? 1.0 : 0.0, R.ddy, R.ty);
-    R.tz = fma_f64(cz ? 1.0 : 0.0, R.ddz, R.tz);
+    // t += dd for a crossed axis (the reference's addition); for the others
+    // t plus a subnormal, which rounds back to t
+    R.incx = __hiloint2double(cx ? (int)R.hix : 0, __double2loint(R.incx));
+    R.incy = __hiloint2double(cy ? (int)R.hiy : 0, __double2loint(R.incy));
+    R.incz = __hiloint2double(cz ? (int)R.hiz : 0, __double2loint(R.incz));
+    R.tx = R.tx + R.incx;
+    R.ty = R.ty + R.incy;
+    R.tz = R.tz + R.incz;
     R.last_x = cx ? R.sx : 0u;
     R.last_y = cy ? R.sy : 0u;
     R.xd = (R.xd - R.last_x) & G.wl.mask[0];
@@ -864,7 +879,10 @@ __device__ __forceinline__ bool lean_step(LeanRay &R, const GeoView &G)
         R.blk = *reinterpret_cast<const uint64_t *>(reinterpret_cast<const char *>(G.wocc) + at);
         R.blk_at = at;
     }
-    return --R.left != 0u;
+    uint32_t left1;
+    const bool capped = __builtin_usub_overflow(R.left1, 1u, &left1);  // no iteration left
+    R.left1 = left1;
+    return !capped;
 }
 
 // Whether a finished lean ray stands on a wall plane (dilated i + 1 of 0 or nx + 1).
@@ -877,20 +895,22 @@ __device__ __forceinline__ bool lean_at_wall(const LeanRay &R, const GeoView &G)
 // plus the read that found a solid cell (a wall is not read).
 __device__ __forceinline__ uint32_t lean_reads(const LeanRay &R, const GeoView &G)
 {
-    const bool at_solid = R.left != 0u && !lean_at_wall(R, G);
-    return (R.cap - R.left) + (at_solid ? 1u : 0u);
+    const uint32_t left = R.left1 + 1u;
+    const bool at_solid = left != 0u && !lean_at_wall(R, G);
+    return (R.cap - left) + (at_solid ? 1u : 0u);
 }
 
 // Final state of a finished lean traversal: FIN_FAILED, the crossing time of
 // a wall (negated for the cold wall), or FIN_SOLID_P with the cell's index P.
 __device__ __forceinline__ uint2 lean_final(const LeanRay &R, const GeoView &G)
 {
-    if (R.left == 0u) return fin_box(FIN_FAILED, 0u, 0u);
+    const uint32_t left = R.left1 + 1u;
+    if (left == 0u) return fin_box(FIN_FAILED, 0u, 0u);
     if (lean_at_wall(R, G)) {
         const unsigned long long t = (unsigned long long)__double_as_longlong(R.time);
         return make_uint2((uint32_t)t, (uint32_t)(t >> 32) | (R.xd == 0u ? 0u : 0x80000000u));
     }
-    if (R.left == R.cap) return fin_box(FIN_FAILED, 0u, 0u);  // solid from the start: no face
+    if (left == R.cap) return fin_box(FIN_FAILED, 0u, 0u);  // solid from the start: no face
     // the face of the last crossing: the first axis crossed in x > y > z order;
     // a +1 step is the axis mask, a -1 step is 1
     const uint32_t axis = R.last_x != 0u ? 0u : (R.last_y != 0u ? 1u : 2u);
@@ -916,7 +936,7 @@ __device__ __forceinline__ void lean_body(const GeoView &G, uint32_t base, uint3
 
     LeanRay R;
     R.xd = R.yd = R.zd = 0u;
-    R.left = 1;
+    R.left1 = 0u;
     bool active = false, pending = false;
     uint32_t g_cur = 0;
     uint64_t reads = 0;
