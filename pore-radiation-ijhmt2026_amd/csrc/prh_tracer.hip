@@ -796,12 +796,13 @@ struct LeanRay {
     uint32_t cap;
 };
 
+// The ray of start record *rec (staged in LDS); false for a ray that fails at once.
 template <bool POW2>
-__device__ __forceinline__ bool lean_load(LeanRay &R, const DdaRec *recs, uint32_t g, const GeoView &G)
+__device__ __forceinline__ bool lean_load(LeanRay &R, const DdaRec *rec, const GeoView &G)
 {
-    const double2 *p = reinterpret_cast<const double2 *>(recs + g);
+    const double2 *p = reinterpret_cast<const double2 *>(rec);
     const double2 a = p[0], b = p[1], c = p[2];
-    const uint4 w = reinterpret_cast<const uint4 *>(recs + g)[3];
+    const uint4 w = reinterpret_cast<const uint4 *>(rec)[3];
     R.tx = a.x;
     R.ty = a.y;
     R.tz = b.x;
@@ -919,20 +920,43 @@ __device__ __forceinline__ uint2 lean_final(const LeanRay &R, const GeoView &G)
     return fin_box(FIN_SOLID_P, R.P, face);
 }
 
-// trace_body with the lean step: the same queue, refill and final-state order,
-// in one loop whose refill part runs only when REFILL_MIN lanes are idle (or
-// none is active), so the stepping is not a loop of its own.
+// Start records staged per wavefront: the wavefront copies up to 64
+// consecutive records of its grain (4 KB) from HBM straight into its LDS
+// slice with four direct loads of 16 B per lane, and waits for them once;
+// refills then read the records from LDS.  The wait on HBM comes once per 64
+// records instead of at every refill.
+constexpr uint32_t LEAN_STAGE = 64;  // records per wavefront slice
+
+__device__ __forceinline__ void stage_records(DdaRec *slice, const DdaRec *src, uint32_t cnt, uint32_t lane)
+{
+    const uint4 *from = reinterpret_cast<const uint4 *>(src);
+    uint4 *to = reinterpret_cast<uint4 *>(slice);
+#pragma unroll
+    for (uint32_t m = 0; m < 4; ++m) {
+        const uint32_t piece = 64u * m + lane;  // 16 B pieces of the records, in order
+        if (piece < 4u * cnt)
+            __builtin_amdgcn_global_load_lds(from + piece, (__attribute__((address_space(3))) void *)(to + 64u * m),
+                                             16, 0, 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the records are in LDS
+}
+
+// trace_body with the lean step: the same queue and final-state order, in one
+// loop whose refill part runs only when REFILL_MIN lanes are idle (or none is
+// active), so the stepping is not a loop of its own.  Idle lanes take the
+// staged records in lane order; the slice is refilled when it runs out.
 constexpr int LEAN_STEPS = 4;  // steps between the wavefront's refill checks
 
 template <int REFILL_MIN, bool POW2>
 __device__ __forceinline__ void lean_body(const GeoView &G, uint32_t base, uint32_t total, uint32_t grain,
-                                          unsigned long long *queue, const DdaRec *dda, const WallRec *walls,
-                                          uint2 *fin, unsigned long long *reads_out)
+                                          unsigned long long *queue, const DdaRec *dda, uint2 *fin,
+                                          unsigned long long *reads_out, DdaRec *slice)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
-    uint32_t cursor = 0, w_end = 0;  // the wavefront's grain [cursor, w_end)
+    uint32_t cursor = 0, w_end = 0;  // the wavefront's grain [cursor, w_end), beyond the staged records
     bool more = true;                // the queue may still hold records
+    uint32_t s_first = 0, s_n = 0, s_taken = 0;  // staged records: [s_first, s_first + s_n), s_taken handed out
 
     LeanRay R;
     R.xd = R.yd = R.zd = 0u;
@@ -945,43 +969,58 @@ __device__ __forceinline__ void lean_body(const GeoView &G, uint32_t base, uint3
     uint32_t wave_steps = 0, refills = 0;
 
     do {
-        if (n_active == 0 || ((more || cursor < w_end) && n_active <= 64 - REFILL_MIN)) {
+        const bool waiting = more || cursor < w_end || s_taken < s_n;  // records not yet handed out
+        if (n_active == 0 || (waiting && n_active <= 64 - REFILL_MIN)) {
             ++refills;
             if (!active && pending) {
                 reads += lean_reads(R, G);
                 fin[g_cur] = lean_final(R, G);
                 pending = false;
             }
-            if (more && cursor >= w_end) {
-                unsigned long long first = 0;
-                if (lane == 0) first = atomicAdd(queue, (unsigned long long)grain);
-                first = __shfl(first, 0);
-                if (first >= total) {
-                    more = false;
-                } else {
-                    cursor = base + (uint32_t)first;
-                    const unsigned long long last = first + grain;
-                    w_end = base + (last < total ? (uint32_t)last : total);
+            for (int round = 0; round < 2; ++round) {
+                if (s_taken == s_n) {
+                    if (more && cursor >= w_end) {
+                        unsigned long long first = 0;
+                        if (lane == 0) first = atomicAdd(queue, (unsigned long long)grain);
+                        first = __shfl(first, 0);
+                        if (first >= total) {
+                            more = false;
+                        } else {
+                            cursor = base + (uint32_t)first;
+                            const unsigned long long last = first + grain;
+                            w_end = base + (last < total ? (uint32_t)last : total);
+                        }
+                    }
+                    if (cursor < w_end) {
+                        const uint32_t cnt = w_end - cursor < LEAN_STAGE ? w_end - cursor : LEAN_STAGE;
+                        stage_records(slice, dda + cursor, cnt, lane);
+                        s_first = cursor;
+                        s_n = cnt;
+                        s_taken = 0;
+                        cursor += cnt;
+                    }
                 }
-            }
-            if (cursor < w_end) {
+                const uint32_t avail = s_n - s_taken;
                 const uint64_t idle = __ballot(!active);
+                const uint32_t n_idle = (uint32_t)__popcll(idle);
+                if (avail == 0u || n_idle == 0u) break;
                 if (!active) {
-                    const uint32_t g = cursor + (uint32_t)__popcll(idle & below);
-                    if (g < w_end) {
-                        if (lean_load<POW2>(R, dda, g, G)) {
+                    const uint32_t rank = (uint32_t)__popcll(idle & below);
+                    if (rank < avail) {
+                        const uint32_t e = s_taken + rank;
+                        if (lean_load<POW2>(R, slice + e, G)) {
                             active = true;
                             pending = true;
-                            g_cur = g;
+                            g_cur = s_first + e;
                         } else {
-                            fin[g] = fin_box(FIN_FAILED, 0u, 0u);
+                            fin[s_first + e] = fin_box(FIN_FAILED, 0u, 0u);
                         }
                     }
                 }
-                cursor += (uint32_t)__popcll(idle);
+                s_taken += n_idle < avail ? n_idle : avail;
             }
             n_active = __popcll(__ballot(active));
-            finished = n_active == 0 && !more && cursor >= w_end;
+            finished = n_active == 0 && !more && cursor >= w_end && s_taken == s_n;
         }
         if (n_active > 0) wave_steps += LEAN_STEPS;
 #pragma unroll
@@ -1001,17 +1040,20 @@ template <int REFILL_MIN, bool POW2>
 __global__ __launch_bounds__(STREAM_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
 void k_trace_lean(TraceArgs a)
 {
-    lean_body<REFILL_MIN, POW2>(a.G, 0u, a.total, a.grain, a.queue, a.dda, a.walls, a.fin, a.reads);
+    __shared__ DdaRec s_stage[STREAM_THREADS / 64][LEAN_STAGE];
+    lean_body<REFILL_MIN, POW2>(a.G, 0u, a.total, a.grain, a.queue, a.dda, a.fin, a.reads,
+                                s_stage[threadIdx.x >> 6]);
 }
 
 template <int REFILL_MIN, bool POW2>
 __global__ __launch_bounds__(STREAM_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
 void k_trace_batch_lean(const Segment *segs, int n_seg, const DdaRec *dda, const WallRec *walls, uint2 *fin)
 {
+    __shared__ DdaRec s_stage[STREAM_THREADS / 64][LEAN_STAGE];
     const int k = segment_of(segs, n_seg, blockIdx.x, true);
     const GeoView G = segs[k].gen.G;
-    lean_body<REFILL_MIN, POW2>(G, segs[k].gen.offset, segs[k].gen.total, segs[k].grain, segs[k].queue, dda, walls,
-                                fin, segs[k].reads);
+    lean_body<REFILL_MIN, POW2>(G, segs[k].gen.offset, segs[k].gen.total, segs[k].grain, segs[k].queue, dda, fin,
+                                segs[k].reads, s_stage[threadIdx.x >> 6]);
 }
 
 __global__ void k_zero_u64(unsigned long long *p, int n)
@@ -1448,9 +1490,9 @@ const TraceVariant &trace_variant_for(const Geometry &g, int variant)
     variant = usable_variant(g, variant);
     return is_pow2(g.ny) && is_pow2(g.nz) ? TRACE_VARIANTS_POW2[variant] : TRACE_VARIANTS[variant];
 }
-// Default: the lean step refilling at 16 idle lanes, the fastest on the foam
-// measured so far (traversal 18.0 ms against 20.5 ms for the brick test).
-int g_trace_variant = 11;
+// Default: the lean step refilling at 12 idle lanes, the fastest on the foam
+// measured so far (traversal 15.8 ms against 20.5 ms for the brick test).
+int g_trace_variant = 15;
 
 // Batch traversal kernels, one per entry of TRACE_VARIANTS (same index).
 using BatchKernel = void (*)(const Segment *, int, const DdaRec *, const WallRec *, uint2 *);
