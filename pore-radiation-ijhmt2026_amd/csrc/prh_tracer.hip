@@ -903,21 +903,23 @@ __device__ __forceinline__ uint32_t lean_reads(const LeanRay &R, const GeoView &
 
 // Final state of a finished lean traversal: FIN_FAILED, the crossing time of
 // a wall (negated for the cold wall), or FIN_SOLID_P with the cell's index P.
+// Formed with selects, so a refill runs one path for every finishing lane.
 __device__ __forceinline__ uint2 lean_final(const LeanRay &R, const GeoView &G)
 {
     const uint32_t left = R.left1 + 1u;
-    if (left == 0u) return fin_box(FIN_FAILED, 0u, 0u);
-    if (lean_at_wall(R, G)) {
-        const unsigned long long t = (unsigned long long)__double_as_longlong(R.time);
-        return make_uint2((uint32_t)t, (uint32_t)(t >> 32) | (R.xd == 0u ? 0u : 0x80000000u));
-    }
-    if (left == R.cap) return fin_box(FIN_FAILED, 0u, 0u);  // solid from the start: no face
+    const bool wall = lean_at_wall(R, G);
+    // failed: the step cap was reached, or the ray started in a solid cell (no face)
+    const bool failed = left == 0u || (!wall && left == R.cap);
     // the face of the last crossing: the first axis crossed in x > y > z order;
     // a +1 step is the axis mask, a -1 step is 1
     const uint32_t axis = R.last_x != 0u ? 0u : (R.last_y != 0u ? 1u : 2u);
-    const uint32_t step = axis == 0u ? R.last_x : (axis == 1u ? R.last_y : R.sz);
+    const uint32_t step = R.last_x != 0u ? R.last_x : (R.last_y != 0u ? R.last_y : R.sz);
     const uint32_t face = 2u * axis + (step != 1u ? 1u : 0u);
-    return fin_box(FIN_SOLID_P, R.P, face);
+    const unsigned long long t = (unsigned long long)__double_as_longlong(R.time);
+    const uint32_t x = wall ? (uint32_t)t : R.P;
+    const uint32_t y = wall ? ((uint32_t)(t >> 32) | (R.xd == 0u ? 0u : 0x80000000u))
+                            : (FIN_BOX | FIN_SOLID_P << 16 | face);
+    return make_uint2(failed ? 0u : x, failed ? (FIN_BOX | FIN_FAILED << 16) : y);
 }
 
 // Start records staged per wavefront: the wavefront copies up to 64
@@ -982,7 +984,10 @@ __device__ __forceinline__ void lean_body(const GeoView &G, uint32_t base, uint3
                     if (more && cursor >= w_end) {
                         unsigned long long first = 0;
                         if (lane == 0) first = atomicAdd(queue, (unsigned long long)grain);
-                        first = __shfl(first, 0);
+                        // lane 0's value, held as a wave-uniform (scalar) value
+                        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)first);
+                        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(first >> 32));
+                        first = (unsigned long long)hi << 32 | lo;
                         if (first >= total) {
                             more = false;
                         } else {
