@@ -94,8 +94,8 @@ __global__ __launch_bounds__(GEO_THREADS) void k_pack_coarse(const uint8_t *soli
 }
 
 // Walled bitset (Geometry::wocc): word w holds the 4x4x4 block of cells whose
-// index P has P >> 6 == w; bit P & 63 is set for a solid cell and for every
-// cell of the planes i = -1 and i = nx.  Indices beyond ny, nz (the padding
+// index P has P >> 6 == w; bit 63 - (P & 63) is set for a solid cell and for
+// every cell of the planes i = -1 and i = nx.  Indices beyond ny, nz (the padding
 // up to a power of two) stay clear.
 __global__ __launch_bounds__(GEO_THREADS) void k_pack_walled(const uint8_t *solid, int nx, int ny, int nz, int lx,
                                                               int ly, int lz, int64_t n_words, uint64_t *wocc)
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(GEO_THREADS) void k_pack_walled(const uint8_t *soli
                 if (ip == 0u || ip == (uint32_t)nx + 1u) set = true;
                 else if (ip <= (uint32_t)nx) set = solid[(((int64_t)ip - 1) * ny + j) * nz + k] != 0;
             }
-            if (set) bits |= 1ull << t;
+            if (set) bits |= 1ull << (63u - t);  // mirrored: a left shift by t brings the cell to bit 63
         }
         wocc[w] = bits;
     }

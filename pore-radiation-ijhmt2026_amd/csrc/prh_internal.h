@@ -91,9 +91,10 @@ void host_seed_words(const uint32_t *words, int n_words, uint64_t out[4]);
 //   cell_code  per cell: (first patch id of the cell << 6) | exposed-face mask
 //   patch_*    per patch: origin (x,y,z) FP64, face 0..5 (walls: 0 hot, 1 cold)
 //   hot_map / cold_map  wall patch id per (j, k), -1 where the boundary cell is solid
-//   wocc       walled occupancy bitset in 4x4x4 blocks, one 64-bit word per block: bit P
-//              for the cell (i + 1, j, k), i in [-1, nx], where P is the Morton index of
-//              (i + 1, j, k) (lean_layout; bits 0-5 are the cell within the block); the planes
+//   wocc       walled occupancy bitset in 4x4x4 blocks, one 64-bit word per block (P >> 6)
+//              holding the cell (i + 1, j, k), i in [-1, nx], at bit 63 - (P & 63), where P
+//              is the Morton index of (i + 1, j, k) (lean_layout; bits 0-5 of P are the cell
+//              within the block); the planes
 //              i = -1 and i = nx (the hot and cold walls) are set, so one bit test ends
 //              a ray at a solid cell or a wall (null when P would need more than 31 bits
 //              or ny, nz < 3)

@@ -855,7 +855,8 @@ __device__ __forceinline__ bool lean_step(LeanRay &R, const GeoView &G)
     const bool cy = fabs(R.ty - nt) <= TIE_TOLERANCE;
     const bool cz = fabs(R.tz - nt) <= TIE_TOLERANCE;
     __builtin_amdgcn_sched_barrier(0);
-    if ((uint32_t)(R.blk >> (R.P & 63u)) & 1u) return false;  // a solid cell or a wall: the ray ends here
+    // the cell's bit, shifted to bit 63: a solid cell or a wall ends the ray here
+    if ((int64_t)(R.blk << (R.P & 63u)) < 0) return false;
     // t += dd for a crossed axis (the reference's addition); for the others
     // t plus a subnormal, which rounds back to t
     R.incx = __hiloint2double(cx ? (int)R.hix : 0, __double2loint(R.incx));
