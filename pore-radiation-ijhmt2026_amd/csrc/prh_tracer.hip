@@ -590,7 +590,7 @@ __device__ __forceinline__ void gen_rays_body(const GenArgs &a, uint64_t base)
 }
 
 template <bool RECORDS>
-__global__ __launch_bounds__(TRACE_THREADS) void k_gen_rays(GenArgs a)
+__global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_gen_rays(GenArgs a)
 {
     const uint64_t wave = ((uint64_t)blockIdx.x * TRACE_THREADS + threadIdx.x) >> 6;
     gen_rays_body<RECORDS>(a, wave * (64ull * GEN_ROUNDS) + (threadIdx.x & 63u));
@@ -617,7 +617,8 @@ __device__ __forceinline__ int segment_of(const Segment *segs, int n_seg, uint32
 
 // Records of every segment of a batch in one launch (each wavefront works
 // inside one segment).
-__global__ __launch_bounds__(TRACE_THREADS) void k_gen_rays_batch(const Segment *segs, int n_seg)
+__global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_gen_rays_batch(
+    const Segment *segs, int n_seg)
 {
     // wave-uniform, so the segment's arguments load into scalar registers
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * TRACE_THREADS + threadIdx.x) >> 6);
