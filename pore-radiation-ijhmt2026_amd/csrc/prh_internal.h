@@ -53,11 +53,19 @@ struct u128 {
 
 __host__ __device__ inline u128 make_u128(uint64_t hi, uint64_t lo) { return u128{lo, hi}; }
 
+// a * b mod 2^128.  The full product of the low halves is formed from four
+// 32 x 32 -> 64 products (one v_mad_u64_u32 each, every partial sum below
+// 2^64), the cross terms only mod 2^64.
 __device__ inline u128 mul128(u128 a, u128 b)
 {
+    const uint32_t a0 = (uint32_t)a.lo, a1 = (uint32_t)(a.lo >> 32);
+    const uint32_t b0 = (uint32_t)b.lo, b1 = (uint32_t)(b.lo >> 32);
+    const uint64_t p00 = (uint64_t)a0 * b0;
+    const uint64_t t = (uint64_t)a0 * b1 + (p00 >> 32);
+    const uint64_t u = (uint64_t)a1 * b0 + (uint32_t)t;
     u128 r;
-    r.lo = a.lo * b.lo;
-    r.hi = __umul64hi(a.lo, b.lo) + a.lo * b.hi + a.hi * b.lo;
+    r.lo = u << 32 | (uint32_t)p00;
+    r.hi = (uint64_t)a1 * b1 + (t >> 32) + (u >> 32) + a.lo * b.hi + a.hi * b.lo;
     return r;
 }
 
