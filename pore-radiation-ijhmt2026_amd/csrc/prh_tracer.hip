@@ -779,7 +779,12 @@ void k_trace_batch(const Segment *segs, int n_seg, const DdaRec *dda, const Wall
 // last crossing are (x and y; z crossed when neither did), so the face
 // follows from them when the ray ends.
 struct LeanRay {
-    double tx, ty, tz, time;
+    double tx, ty, tz;
+    // The smallest crossing time of each step, written by even steps of the
+    // ray (counted from its first) to nt0 and by odd steps to nt1, before the
+    // step finds out whether the ray goes on: the time of the last crossing
+    // is in the register of the step before the one that stopped the ray.
+    double nt0, nt1;
     // The crossing increments of t: low words of ddx, ddy, ddz with their
     // high words set for the axes that cross (0 otherwise: the value is then
     // a subnormal or zero, and t, which is always a positive normal number,
@@ -787,19 +792,31 @@ struct LeanRay {
     double incx, incy, incz;
     uint32_t hix, hiy, hiz;  // high words of ddx, ddy, ddz
     uint32_t xd, yd, zd;     // dilated i + 1, j, k
-    uint32_t sx, sy, sz;     // dilated steps: the axis mask (+1) or 1 (-1); 0 for no step
+    uint32_t sx, sy, sz;     // dilated steps: the axis mask (+1) or its lowest bit (-1)
     uint32_t last_x, last_y; // the x and y steps of the last crossing (0: not crossed)
     uint32_t from_y, to_y, from_z, to_z;  // lateral wraps (sizes that are not powers of two)
     uint32_t P;              // xd | yd | zd: the cell's bit in the walled bitset
     uint64_t blk;            // the bitset word of the cell's block
     uint32_t blk_at;         // its byte offset
     uint32_t left1;          // iterations left before the step cap, minus one (wraps when the cap is reached)
-    uint32_t cap;
+    uint32_t cap1;           // left1 at the start: the ray has not crossed a face while left1 == cap1
 };
 
-// The ray of start record *rec (staged in LDS); false for a ray that fails at once.
+// The ray of start record *rec (staged in LDS).  A ray with dx == 0 (cap 0,
+// P 0: a hot-wall cell) is loaded like any other: its first step finds the
+// wall bit, and with its cap already spent it ends as FIN_FAILED.
+// last_x and last_y are not reset: both are written by every crossing, and a
+// ray's final state reads them only after it has crossed a face.
+//
+// Dilated steps: subtracting the axis mask adds one to the axis's value (the
+// borrow runs through the gaps, which the mask then clears); subtracting the
+// mask's lowest bit takes one away.  With at least two interleaved bits per
+// axis the lowest bits are 4 (x), 2 (y) and 1 (z), and every +1 step has a
+// bit above bit 2 set.  A step sign is taken from one bit of its code (+1 or
+// not): an axis whose direction component is 0 never crosses, so its step
+// value is never used.
 template <bool POW2>
-__device__ __forceinline__ bool lean_load(LeanRay &R, const DdaRec *rec, const GeoView &G)
+__device__ __forceinline__ void lean_load(LeanRay &R, const DdaRec *rec, const GeoView &G)
 {
     const double2 *p = reinterpret_cast<const double2 *>(rec);
     const double2 a = p[0], b = p[1], c = p[2];
@@ -819,39 +836,41 @@ __device__ __forceinline__ bool lean_load(LeanRay &R, const DdaRec *rec, const G
     R.xd = w.w & L.mask[0];
     R.yd = w.w & L.mask[1];
     R.zd = w.w & L.mask[2];
-    const uint32_t ux = (w.y >> 16) & 3u, uy = (w.y >> 18) & 3u, uz = (w.y >> 20) & 3u;  // step + 1
-    R.sx = ux == 2u ? L.mask[0] : (ux == 0u ? 1u : 0u);
-    R.sy = uy == 2u ? L.mask[1] : (uy == 0u ? 1u : 0u);
-    R.sz = uz == 2u ? L.mask[2] : (uz == 0u ? 1u : 0u);
+    // (mask & (all ones for a +1 step, the high bit of the 2-bit code step + 1))
+    // | lowest bit, two ALU operations each (the compiler otherwise turns the
+    // expression into a compare and a select)
+    asm("v_bfe_i32 %0, %1, 17, 1\n\tv_and_or_b32 %0, %0, %2, 4" : "=&v"(R.sx) : "v"(w.y), "s"(L.mask[0]));
+    asm("v_bfe_i32 %0, %1, 19, 1\n\tv_and_or_b32 %0, %0, %2, 2" : "=&v"(R.sy) : "v"(w.y), "s"(L.mask[1]));
+    asm("v_bfe_i32 %0, %1, 21, 1\n\tv_and_or_b32 %0, %0, %2, 1" : "=&v"(R.sz) : "v"(w.y), "s"(L.mask[2]));
     if (!POW2) {
+        const bool py = (w.y >> 19) & 1u, pz = (w.y >> 21) & 1u;  // +1 steps
         // +1 from n - 1 reaches n, which wraps to 0; -1 from 0 leaves all the
         // axis's bits set, which wraps to n - 1
-        R.from_y = uy == 2u ? lean_dilate(L, 1, (uint32_t)G.ny) : L.mask[1];
-        R.to_y = uy == 2u ? 0u : lean_dilate(L, 1, (uint32_t)G.ny - 1u);
-        R.from_z = uz == 2u ? lean_dilate(L, 2, (uint32_t)G.nz) : L.mask[2];
-        R.to_z = uz == 2u ? 0u : lean_dilate(L, 2, (uint32_t)G.nz - 1u);
+        R.from_y = py ? lean_dilate(L, 1, (uint32_t)G.ny) : L.mask[1];
+        R.to_y = py ? 0u : lean_dilate(L, 1, (uint32_t)G.ny - 1u);
+        R.from_z = pz ? lean_dilate(L, 2, (uint32_t)G.nz) : L.mask[2];
+        R.to_z = pz ? 0u : lean_dilate(L, 2, (uint32_t)G.nz - 1u);
     }
-    R.last_x = R.last_y = 0u;
-    R.time = 0.0;
-    R.cap = w.z;
     R.left1 = w.z - 1u;
+    R.cap1 = R.left1;
+    R.nt1 = 0.0;  // a ray given with x outside [0, nx) is on a wall before any crossing
     // the block of the first cell, read by the first step
     R.blk_at = (R.P >> 3) & ~7u;
     R.blk = *reinterpret_cast<const uint64_t *>(reinterpret_cast<const char *>(G.wocc) + R.blk_at);
-    return w.z != 0u;
 }
 
 // One iteration of view_factors.py:225-255 on the walled bitset; returns
-// whether the ray goes on.  The crossing times are compared before the
+// whether the ray goes on.  ODD: the step's parity within the ray.  The crossing times are compared before the
 // cell's bitset word is tested (they do not depend on it), so they overlap
 // the word's load; the crossing is taken only by the lanes whose cell is
 // empty.  The word of the next cell's block is requested right after the
 // crossing and first tested by the next step, so its latency overlaps that
 // step's comparisons.
-template <bool POW2>
+template <bool POW2, bool ODD>
 __device__ __forceinline__ bool lean_step(LeanRay &R, const GeoView &G)
 {
-    const double nt = vmin_f64(vmin_f64(R.tx, R.ty), R.tz);
+    double &nt = ODD ? R.nt1 : R.nt0;
+    nt = vmin_f64(vmin_f64(R.tx, R.ty), R.tz);
     const bool cx = fabs(R.tx - nt) <= TIE_TOLERANCE;
     const bool cy = fabs(R.ty - nt) <= TIE_TOLERANCE;
     const bool cz = fabs(R.tz - nt) <= TIE_TOLERANCE;
@@ -875,7 +894,6 @@ __device__ __forceinline__ bool lean_step(LeanRay &R, const GeoView &G)
         R.yd = R.yd == R.from_y ? R.to_y : R.yd;
         R.zd = R.zd == R.from_z ? R.to_z : R.zd;
     }
-    R.time = nt;
     R.P = R.xd | R.yd | R.zd;
     const uint32_t at = (R.P >> 3) & ~7u;  // byte offset of the next cell's block word
     if (at != R.blk_at) {
@@ -894,30 +912,33 @@ __device__ __forceinline__ bool lean_at_wall(const LeanRay &R, const GeoView &G)
     return R.xd == 0u || R.xd == G.wl_cold;
 }
 
-// Occupancy reads of a finished lean traversal (as ray_reads): the steps,
-// plus the read that found a solid cell (a wall is not read).
-__device__ __forceinline__ uint32_t lean_reads(const LeanRay &R, const GeoView &G)
+// Whether a finished lean ray stopped at its step cap.
+__device__ __forceinline__ bool lean_capped(const LeanRay &R) { return R.left1 == 0xFFFFFFFFu; }
+
+// Occupancy reads of a finished lean traversal (as ray_reads): one per
+// crossing, plus the read that found a solid cell (a wall is not read, and a
+// ray stopped by its cap read before each of its crossings).
+__device__ __forceinline__ uint32_t lean_reads(const LeanRay &R, bool wall, bool capped)
 {
-    const uint32_t left = R.left1 + 1u;
-    const bool at_solid = left != 0u && !lean_at_wall(R, G);
-    return (R.cap - left) + (at_solid ? 1u : 0u);
+    return (R.cap1 - R.left1) + (!capped && !wall ? 1u : 0u);
 }
 
 // Final state of a finished lean traversal: FIN_FAILED, the crossing time of
 // a wall (negated for the cold wall), or FIN_SOLID_P with the cell's index P.
 // Formed with selects, so a refill runs one path for every finishing lane.
-__device__ __forceinline__ uint2 lean_final(const LeanRay &R, const GeoView &G)
+__device__ __forceinline__ uint2 lean_final(const LeanRay &R, bool wall, bool capped)
 {
-    const uint32_t left = R.left1 + 1u;
-    const bool wall = lean_at_wall(R, G);
     // failed: the step cap was reached, or the ray started in a solid cell (no face)
-    const bool failed = left == 0u || (!wall && left == R.cap);
+    const bool failed = capped || (!wall && R.left1 == R.cap1);
     // the face of the last crossing: the first axis crossed in x > y > z order;
-    // a +1 step is the axis mask, a -1 step is 1
+    // a +1 step has a bit above bit 2 set, a -1 step is 4, 2 or 1
     const uint32_t axis = R.last_x != 0u ? 0u : (R.last_y != 0u ? 1u : 2u);
     const uint32_t step = R.last_x != 0u ? R.last_x : (R.last_y != 0u ? R.last_y : R.sz);
-    const uint32_t face = 2u * axis + (step != 1u ? 1u : 0u);
-    const unsigned long long t = (unsigned long long)__double_as_longlong(R.time);
+    const uint32_t face = 2u * axis + (step > 7u ? 1u : 0u);
+    // the crossings made; the last one was made by step crossings - 1
+    const uint32_t crossings = R.cap1 - R.left1;
+    const double time = (crossings & 1u) ? R.nt0 : R.nt1;
+    const unsigned long long t = (unsigned long long)__double_as_longlong(time);
     const uint32_t x = wall ? (uint32_t)t : R.P;
     const uint32_t y = wall ? ((uint32_t)(t >> 32) | (R.xd == 0u ? 0u : 0x80000000u))
                             : (FIN_BOX | FIN_SOLID_P << 16 | face);
@@ -964,7 +985,9 @@ __device__ __forceinline__ void lean_body(const GeoView &G, uint32_t base, uint3
 
     LeanRay R;
     R.xd = R.yd = R.zd = 0u;
-    R.left1 = 0u;
+    R.left1 = R.cap1 = 0u;
+    R.last_x = R.last_y = 0u;
+    R.nt0 = R.nt1 = 0.0;
     bool active = false, pending = false;
     uint32_t g_cur = 0;
     uint64_t reads = 0;
@@ -977,8 +1000,9 @@ __device__ __forceinline__ void lean_body(const GeoView &G, uint32_t base, uint3
         if (n_active == 0 || (waiting && n_active <= 64 - REFILL_MIN)) {
             ++refills;
             if (!active && pending) {
-                reads += lean_reads(R, G);
-                fin[g_cur] = lean_final(R, G);
+                const bool wall = lean_at_wall(R, G), capped = lean_capped(R);
+                reads += lean_reads(R, wall, capped);
+                fin[g_cur] = lean_final(R, wall, capped);
                 pending = false;
             }
             for (int round = 0; round < 2; ++round) {
@@ -1015,13 +1039,10 @@ __device__ __forceinline__ void lean_body(const GeoView &G, uint32_t base, uint3
                     const uint32_t rank = (uint32_t)__popcll(idle & below);
                     if (rank < avail) {
                         const uint32_t e = s_taken + rank;
-                        if (lean_load<POW2>(R, slice + e, G)) {
-                            active = true;
-                            pending = true;
-                            g_cur = s_first + e;
-                        } else {
-                            fin[s_first + e] = fin_box(FIN_FAILED, 0u, 0u);
-                        }
+                        lean_load<POW2>(R, slice + e, G);
+                        active = true;
+                        pending = true;
+                        g_cur = s_first + e;
                     }
                 }
                 s_taken += n_idle < avail ? n_idle : avail;
@@ -1030,9 +1051,12 @@ __device__ __forceinline__ void lean_body(const GeoView &G, uint32_t base, uint3
             finished = n_active == 0 && !more && cursor >= w_end && s_taken == s_n;
         }
         if (n_active > 0) wave_steps += LEAN_STEPS;
+        static_assert(LEAN_STEPS % 2 == 0, "a ray's first step is an even step");
 #pragma unroll
-        for (int k = 0; k < LEAN_STEPS; ++k)
-            if (active) active = lean_step<POW2>(R, G);
+        for (int k = 0; k < LEAN_STEPS; k += 2) {
+            if (active) active = lean_step<POW2, false>(R, G);
+            if (active) active = lean_step<POW2, true>(R, G);
+        }
         n_active = __popcll(__ballot(active));
     } while (!finished);
     reads = wave_sum_u64(reads);
