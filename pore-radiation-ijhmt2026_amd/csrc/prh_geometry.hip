@@ -105,7 +105,7 @@ __global__ __launch_bounds__(GEO_THREADS) void k_pack_walled(const uint8_t *soli
          w += (int64_t)gridDim.x * GEO_THREADS) {
         uint64_t bits = 0;
         for (uint32_t t = 0; t < 64; ++t) {
-            const uint32_t P = (uint32_t)w << 6 | t;
+            const uint32_t P = (uint32_t)w << (6 + LEAN_GAP) | t;
             const uint32_t ip = lean_compact(L, 0, P & L.mask[0]);
             const uint32_t j = lean_compact(L, 1, P & L.mask[1]);
             const uint32_t k = lean_compact(L, 2, P & L.mask[2]);
@@ -248,14 +248,15 @@ int build(const uint8_t *solid_host, int nx, int ny, int nz, double H_m, Geometr
     g.H_m = H_m;
     const int64_t n_cells = (int64_t)nx * ny * nz;
     const int64_t n_lat = (int64_t)ny * nz;
-    // The walled bitset exists when its index fits 31 bits and both lateral
-    // sizes are at least 3 (each axis has two interleaved bits, and a lateral
-    // step always changes j or k).
+    // The walled bitset exists when its index (with the LEAN_GAP zero bits)
+    // fits 31 bits and both lateral sizes are at least 3 (each axis has two
+    // interleaved bits, and a lateral step always changes j or k).
     g.wlx = std::max(2, ceil_log2(nx + 2));
     g.wly = std::max(2, ceil_log2(ny));
     g.wlz = std::max(2, ceil_log2(nz));
     const int walled_bits = g.wlx + g.wly + g.wlz;
-    g.wocc_words = (ny >= 3 && nz >= 3 && walled_bits <= 31) ? (int64_t(1) << (walled_bits - 6)) : 0;
+    g.wocc_words =
+        (ny >= 3 && nz >= 3 && walled_bits + (int)LEAN_GAP <= 31) ? (int64_t(1) << (walled_bits - 6)) : 0;
 
     uint8_t *solid = nullptr, *mask = nullptr;
     int32_t *face_count = nullptr, *wall_count = nullptr;

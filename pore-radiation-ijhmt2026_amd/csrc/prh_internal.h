@@ -99,13 +99,13 @@ void host_seed_words(const uint32_t *words, int n_words, uint64_t out[4]);
 //   cell_code  per cell: (first patch id of the cell << 6) | exposed-face mask
 //   patch_*    per patch: origin (x,y,z) FP64, face 0..5 (walls: 0 hot, 1 cold)
 //   hot_map / cold_map  wall patch id per (j, k), -1 where the boundary cell is solid
-//   wocc       walled occupancy bitset in 4x4x4 blocks, one 64-bit word per block (P >> 6)
-//              holding the cell (i + 1, j, k), i in [-1, nx], at bit 63 - (P & 63), where P
-//              is the Morton index of (i + 1, j, k) (lean_layout; bits 0-5 of P are the cell
-//              within the block); the planes
-//              i = -1 and i = nx (the hot and cold walls) are set, so one bit test ends
-//              a ray at a solid cell or a wall (null when P would need more than 31 bits
-//              or ny, nz < 3)
+//   wocc       walled occupancy bitset in 4x4x4 blocks, one 64-bit word per block (word
+//              P >> 9, at byte offset P >> 6) holding the cell (i + 1, j, k), i in [-1, nx],
+//              at bit 63 - (P & 63), where P is the Morton index of (i + 1, j, k)
+//              (lean_layout; bits 0-5 of P are the cell within the block, bits 6-8 are
+//              zero); the planes i = -1 and i = nx (the hot and cold walls) are set, so
+//              one bit test ends a ray at a solid cell or a wall (null when P would need
+//              more than 31 bits or ny, nz < 3)
 struct Geometry {
     int32_t nx, ny, nz;
     int64_t occ_words;
@@ -129,11 +129,16 @@ struct Geometry {
 
 // Bit layout of the walled bitset's index P (Geometry::wocc): a Morton
 // order.  The low m bits of each axis's value (m = the smallest of the three
-// widths, at most 10) are interleaved as z, y, x, z, y, x, ... from bit 0, so
-// bits 0-5 (the bit within a block's word) hold the block's 4x4x4 cells and a
-// 128 B line holds 8x8x16 cells; the bits above m of each axis follow in
+// widths, at least 2 and at most 10) are interleaved as z, y, x, z, y, x, ...
+// so bits 0-5 (the bit within a block's word) hold the block's 4x4x4 cells;
+// the interleaved bits above those start at bit 9, leaving bits 6-8 zero, so
+// P >> 6 is the byte offset of the block's word and the word index is P >> 9;
+// a 128 B line holds 8x8x16 cells.  The bits above m of each axis follow in
 // fields (z, then y, then x).  The set of an axis's bits is its mask; P is
-// the OR of the three dilated values.
+// the OR of the three dilated values.  The gap bits belong to no axis, so
+// adding or taking one from an axis's value (subtracting its mask or its
+// mask's lowest bit, then masking) carries or borrows across them.
+constexpr uint32_t LEAN_GAP = 3;  // zero bits between the cell within the block and the word index
 struct LeanLayout {
     uint32_t mask[3];   // x, y, z
     uint32_t field[3];  // first bit of each axis's field
@@ -160,6 +165,9 @@ __host__ __device__ inline uint32_t morton_gather(uint32_t v)  // bit 3b -> bit 
     return v;
 }
 
+// Interleaved bits 0-5 stay; the ones above move up past the gap.
+__host__ __device__ inline uint32_t lean_open_gap(uint32_t v) { return (v & 63u) | (v & ~63u) << LEAN_GAP; }
+
 __host__ __device__ inline LeanLayout lean_layout(int lx, int ly, int lz)
 {
     LeanLayout L;
@@ -167,26 +175,28 @@ __host__ __device__ inline LeanLayout lean_layout(int lx, int ly, int lz)
     m = m < lz ? m : lz;
     m = m < 10 ? m : 10;
     L.m = (uint32_t)m;
-    L.field[2] = 3u * L.m;
+    L.field[2] = 3u * L.m + LEAN_GAP;
     L.field[1] = L.field[2] + (uint32_t)(lz - m);
     L.field[0] = L.field[1] + (uint32_t)(ly - m);
     const int bits[3] = {lx, ly, lz};
-    const uint32_t inter = morton_spread((1u << L.m) - 1u);
     for (int a = 0; a < 3; ++a)
-        L.mask[a] = inter << (2 - a) | (((1u << (bits[a] - m)) - 1u) << L.field[a]);
+        L.mask[a] = lean_open_gap(morton_spread((1u << L.m) - 1u) << (2 - a)) |
+                    (((1u << (bits[a] - m)) - 1u) << L.field[a]);
     return L;
 }
 
 // Value v of axis a (0 x, 1 y, 2 z) spread to its bits of P, and back.
 __host__ __device__ inline uint32_t lean_dilate(const LeanLayout &L, int a, uint32_t v)
 {
-    return morton_spread(v & ((1u << L.m) - 1u)) << (2 - a) | (v >> L.m) << L.field[a];
+    return lean_open_gap(morton_spread(v & ((1u << L.m) - 1u)) << (2 - a)) | (v >> L.m) << L.field[a];
 }
 
 __host__ __device__ inline uint32_t lean_compact(const LeanLayout &L, int a, uint32_t d)
 {
-    const uint32_t inter = morton_spread((1u << L.m) - 1u) << (2 - a);
-    return morton_gather((d & inter) >> (2 - a)) | ((d & L.mask[a] & ~inter) >> L.field[a]) << L.m;
+    const uint32_t inter = lean_open_gap(morton_spread((1u << L.m) - 1u) << (2 - a));
+    const uint32_t low = d & inter;  // the interleaved bits, gap opened
+    return morton_gather(((low & 63u) | (low & ~63u) >> LEAN_GAP) >> (2 - a)) |
+           ((d & L.mask[a] & ~inter) >> L.field[a]) << L.m;
 }
 
 // Device exclusive scan helpers (prh_scan.hip).
