@@ -797,7 +797,11 @@ struct LeanRay {
     uint32_t from_y, to_y, from_z, to_z;  // lateral wraps (sizes that are not powers of two)
     uint32_t P;              // xd | yd | zd: the cell's bit in the walled bitset
     uint64_t blk;            // the bitset word of the cell's block
-    uint32_t blk_at;         // its byte offset, P >> 6
+    // P >> 6 (the byte offset of the block's word) after each step, written
+    // by even steps to at0 and by odd steps to at1, so a step compares its
+    // block with the one before without a copy; the word in blk is always
+    // that of the block the last step reached
+    uint32_t at0, at1;
     uint32_t left1;          // iterations left before the step cap, minus one (wraps when the cap is reached)
     uint32_t cap1;           // left1 at the start: the ray has not crossed a face while left1 == cap1
 };
@@ -854,9 +858,10 @@ __device__ __forceinline__ void lean_load(LeanRay &R, const DdaRec *rec, const G
     R.left1 = w.z - 1u;
     R.cap1 = R.left1;
     R.nt1 = 0.0;  // a ray given with x outside [0, nx) is on a wall before any crossing
-    // the block of the first cell, read by the first step
-    R.blk_at = R.P >> 6;
-    R.blk = *reinterpret_cast<const uint64_t *>(reinterpret_cast<const char *>(G.wocc) + R.blk_at);
+    // the block of the first cell, read by the first step (an even step,
+    // which compares with at1)
+    R.at1 = R.P >> 6;
+    R.blk = *reinterpret_cast<const uint64_t *>(reinterpret_cast<const char *>(G.wocc) + R.at1);
 }
 
 // One iteration of view_factors.py:225-255 on the walled bitset; returns
@@ -895,11 +900,11 @@ __device__ __forceinline__ bool lean_step(LeanRay &R, const GeoView &G)
         R.zd = R.zd == R.from_z ? R.to_z : R.zd;
     }
     R.P = R.xd | R.yd | R.zd;
-    const uint32_t at = R.P >> 6;  // byte offset of the next cell's block word (bits 6-8 of P are zero)
-    if (at != R.blk_at) {
+    // byte offset of the next cell's block word (bits 6-8 of P are zero)
+    uint32_t &at = ODD ? R.at1 : R.at0;
+    at = R.P >> 6;
+    if (at != (ODD ? R.at0 : R.at1))
         R.blk = *reinterpret_cast<const uint64_t *>(reinterpret_cast<const char *>(G.wocc) + at);
-        R.blk_at = at;
-    }
     uint32_t left1;
     const bool capped = __builtin_usub_overflow(R.left1, 1u, &left1);  // no iteration left
     R.left1 = left1;
