@@ -120,6 +120,25 @@ __global__ __launch_bounds__(GEO_THREADS) void k_pack_walled(const uint8_t *soli
     }
 }
 
+// Geometry::wcode: entry c is the cell_code of the cell whose walled-bitset
+// index is P = (c >> 6) << 9 | (c & 63), or 0 for the wall planes and the
+// padding beyond nx, ny, nz.
+__global__ __launch_bounds__(GEO_THREADS) void k_pack_wcode(const uint32_t *cell_code, int nx, int ny, int nz, int lx,
+                                                             int ly, int lz, int64_t n, uint32_t *wcode)
+{
+    const LeanLayout L = lean_layout(lx, ly, lz);
+    for (int64_t c = (int64_t)blockIdx.x * GEO_THREADS + threadIdx.x; c < n; c += (int64_t)gridDim.x * GEO_THREADS) {
+        const uint32_t P = (uint32_t)(c >> 6) << (6 + LEAN_GAP) | (uint32_t)(c & 63);
+        const uint32_t ip = lean_compact(L, 0, P & L.mask[0]);
+        const uint32_t j = lean_compact(L, 1, P & L.mask[1]);
+        const uint32_t k = lean_compact(L, 2, P & L.mask[2]);
+        uint32_t code = 0u;
+        if (ip >= 1u && ip <= (uint32_t)nx && j < (uint32_t)ny && k < (uint32_t)nz)
+            code = cell_code[(((int64_t)ip - 1) * ny + j) * nz + k];
+        wcode[c] = code;
+    }
+}
+
 int ceil_log2(int n)
 {
     int l = 0;
@@ -211,10 +230,11 @@ unsigned grid_for(int64_t n)
 void free_geometry(Geometry &g)
 {
     void *ptrs[] = {g.occ, g.coarse, g.cell_code, g.patch_origin, g.patch_face, g.patch_cell, g.patch_class, g.hot_map,
-                    g.cold_map, g.wocc};
+                    g.cold_map, g.wocc, g.wcode};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     g.wocc = nullptr;
+    g.wcode = nullptr;
     g.occ = nullptr;
     g.coarse = nullptr;
     g.cell_code = nullptr;
@@ -286,6 +306,8 @@ int build(const uint8_t *solid_host, int nx, int ny, int nz, double H_m, Geometr
     PRH_HIP(hipMalloc((void **)&g.hot_map, n_lat * sizeof(int32_t)));
     PRH_HIP(hipMalloc((void **)&g.cold_map, n_lat * sizeof(int32_t)));
     if (g.wocc_words) PRH_HIP(hipMalloc((void **)&g.wocc, (size_t)g.wocc_words * sizeof(uint64_t)));
+    const int64_t wcode_n = g.wocc_words * 64;
+    if (g.wocc_words) PRH_HIP(hipMalloc((void **)&g.wcode, (size_t)wcode_n * sizeof(uint32_t)));
     PRH_HIP(hipMemcpyAsync(solid, solid_host, n_cells, hipMemcpyHostToDevice, st));
 
     k_cell_masks<<<grid_for(n_cells), GEO_THREADS, 0, st>>>(solid, nx, ny, nz, mask, face_count);
@@ -318,6 +340,9 @@ int build(const uint8_t *solid_host, int nx, int ny, int nz, double H_m, Geometr
     k_fill_internal<<<grid_for(n_cells), GEO_THREADS, 0, st>>>(mask, face_base, nx, ny, nz, n_wall,
                                                                g.patch_origin, g.patch_face, g.patch_cell,
                                                                g.patch_class, g.cell_code);
+    if (g.wcode)
+        k_pack_wcode<<<grid_for(wcode_n), GEO_THREADS, 0, st>>>(g.cell_code, nx, ny, nz, g.wlx, g.wly, g.wlz, wcode_n,
+                                                                 g.wcode);
     PRH_HIP(hipGetLastError());
     PRH_HIP(hipStreamSynchronize(st));
     return PRH_OK;

@@ -125,6 +125,11 @@ struct Geometry {
     uint64_t *wocc;
     int64_t wocc_words;
     int32_t wlx, wly, wlz;  // bit widths of nx + 2, ny, nz (each at least 2)
+    // cell_code in walled-bitset order: entry (P >> 9) << 6 | (P & 63) is the
+    // code of the cell whose index is P (0 for wall and padding cells), so a
+    // final state holding P finds its patch with one load; present exactly
+    // when wocc is (at most 2^28 entries, 1 GiB)
+    uint32_t *wcode;
 };
 
 // Bit layout of the walled bitset's index P (Geometry::wocc): a Morton

@@ -60,6 +60,8 @@ struct GeoView {
     const uint64_t *wocc;  // walled bitset (Geometry::wocc), or null
     LeanLayout wl;         // its index layout
     uint32_t wl_cold;      // dilated i + 1 of the cold wall plane (nx + 1)
+    const uint32_t *wcode;  // cell codes in walled-bitset order (Geometry::wcode), or null
+    uint32_t wcode_n;       // its entries
 };
 
 GeoView view_of(const Geometry &g)
@@ -68,7 +70,8 @@ GeoView view_of(const Geometry &g)
                    g.patch_class, g.nx, g.ny,     g.nz,      g.coarse_shift, g.cny,    g.cnz,
                    (uint32_t)g.coarse_words,     1.0 / (double)g.ny,   1.0 / (double)g.nz,
                    g.wocc,   g.wocc ? lean_layout(g.wlx, g.wly, g.wlz) : LeanLayout{},
-                   g.wocc ? lean_dilate(lean_layout(g.wlx, g.wly, g.wlz), 0, (uint32_t)g.nx + 1u) : 0u};
+                   g.wocc ? lean_dilate(lean_layout(g.wlx, g.wly, g.wlz), 0, (uint32_t)g.nx + 1u) : 0u,
+                   g.wcode,  g.wcode ? (uint32_t)(g.wocc_words * 64) : 0u};
 }
 
 // DDA start state of one ray (the set-up of _trace_kernel,
@@ -389,6 +392,13 @@ __device__ __forceinline__ uint2 ray_final(const Ray &R, const GeoView &G, const
 // as ray_final does), or the patch of (cell, face) from the cell code (first
 // patch id of the cell << 6 | mask of its exposed faces).  w is the ray's
 // wall record.
+__device__ __forceinline__ int patch_of_code(uint32_t code, uint32_t face)
+{
+    const uint32_t m = code & 63u;
+    if (!((m >> face) & 1u)) return -1;
+    return (int)(code >> 6) + __popc(m & ((1u << face) - 1u));
+}
+
 __device__ __forceinline__ int patch_of_face(uint32_t cell, uint32_t face, const GeoView &G)
 {
     const uint32_t code = G.cell_code[cell];
@@ -415,6 +425,27 @@ __device__ __forceinline__ int resolve_wall_time(uint2 f, const GeoView &G, cons
 }
 
 // The receiver of a boxed final state (any but a crossing time).
+__device__ __forceinline__ uint32_t fin_kind(uint2 f) { return (f.y >> 16) & 15u; }
+
+// Entry of Geometry::wcode for a FIN_SOLID_P final state whose index is in
+// range (bits 6-8 of P zero, entry below wcode_n), else 0.
+__device__ __forceinline__ uint32_t fin_wcode_index(uint2 f, const GeoView &G)
+{
+    const uint32_t c = (f.x >> (6 + LEAN_GAP)) << 6 | (f.x & 63u);
+    const bool ok = !fin_is_time(f) && fin_kind(f) == FIN_SOLID_P && ((f.x >> 6) & ((1u << LEAN_GAP) - 1u)) == 0u &&
+                    c < G.wcode_n;
+    return ok ? c : 0u;
+}
+
+// Receiver of a FIN_SOLID_P final state given its cell code (loaded from
+// Geometry::wcode at fin_wcode_index), or -1.
+__device__ __forceinline__ int resolve_solid_p(uint2 f, uint32_t code, const GeoView &G)
+{
+    const uint32_t low = f.y & 0xFFFFu;
+    if (low > 5u || fin_wcode_index(f, G) == 0u) return -1;
+    return patch_of_code(code, low);
+}
+
 __device__ __forceinline__ int resolve_boxed(uint2 f, const GeoView &G)
 {
     const uint32_t kind = (f.y >> 16) & 15u;
@@ -424,12 +455,10 @@ __device__ __forceinline__ int resolve_boxed(uint2 f, const GeoView &G)
         return patch_of_face(f.x, low, G);
     }
     if (kind == FIN_SOLID_P) {
-        if (!G.wocc || low > 5u) return -1;
-        const uint32_t ip = lean_compact(G.wl, 0, f.x & G.wl.mask[0]);
-        const uint32_t j = lean_compact(G.wl, 1, f.x & G.wl.mask[1]);
-        const uint32_t k = lean_compact(G.wl, 2, f.x & G.wl.mask[2]);
-        if (ip < 1u || ip > (uint32_t)G.nx || j >= (uint32_t)G.ny || k >= (uint32_t)G.nz) return -1;
-        return patch_of_face(((ip - 1u) * (uint32_t)G.ny + j) * (uint32_t)G.nz + k, low, G);
+        // bits 6-8 of P are zero; the table is dense in the other bits
+        const uint32_t c = (f.x >> (6 + LEAN_GAP)) << 6 | (f.x & 63u);
+        if (!G.wcode || low > 5u || (f.x >> 6) & ((1u << LEAN_GAP) - 1u) || c >= G.wcode_n) return -1;
+        return patch_of_code(G.wcode[c], low);
     }
     if (kind == FIN_HOT || kind == FIN_COLD) {
         if (f.x >= (uint32_t)G.ny || low >= (uint32_t)G.nz) return -1;
@@ -1199,7 +1228,103 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_patches(const uint2 *fi
 // the others one __shfl_xor per key.  Runs are then found with one
 // neighbour shuffle and a wave prefix sum; run starts go through LDS to
 // give each run its length.
+// Value of lane (lane ^ M) for a compile-time M: DPP within a quad, a
+// swizzle within 32 lanes, a permute across the halves.
+template <int M>
+__device__ __forceinline__ int32_t xor_lanes(int32_t v)
+{
+    if constexpr (M == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
+    if constexpr (M == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+    if constexpr (M == 3) return __builtin_amdgcn_mov_dpp(v, 0x1B, 0xF, 0xF, false);  // quad_perm 3,2,1,0
+    if constexpr (M < 32) return __builtin_amdgcn_ds_swizzle(v, 0x1F | M << 10);     // and 31, xor M
+    return __shfl_xor(v, M);
+}
+
+// One cross-lane compare-exchange of register r with lane (lane ^ M) at
+// register r ^ FLIP: the lane whose bit M is clear keeps the smaller key.
+template <int M, int FLIP, int EPL>
+__device__ __forceinline__ void exchange_lanes(int32_t (&key)[EPL], bool lower)
+{
+    if constexpr (FLIP == 0) {
+#pragma unroll
+        for (int r = 0; r < EPL; ++r) {
+            const int32_t o = xor_lanes<M>(key[r]);
+            key[r] = lower ? (key[r] < o ? key[r] : o) : (key[r] < o ? o : key[r]);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < EPL / 2; ++r) {
+            const int32_t a = xor_lanes<M>(key[r ^ FLIP]), b = xor_lanes<M>(key[r]);
+            const int32_t x = key[r], y = key[r ^ FLIP];
+            key[r] = lower ? (x < a ? x : a) : (x < a ? a : x);
+            key[r ^ FLIP] = lower ? (y < b ? y : b) : (y < b ? b : y);
+        }
+    }
+}
+
+// Half-cleaners J, J / 2, ..., 1 of the sort below: position i against i + J
+// in each block of 2J, the smaller key to the lower position.
+template <int J, int EPL>
+__device__ __forceinline__ void sort_clean(int32_t (&key)[EPL], int lane)
+{
+    if constexpr (J >= EPL) {
+        exchange_lanes<J / EPL, 0>(key, (lane & (J / EPL)) == 0);
+    } else {
+#pragma unroll
+        for (int r = 0; r < EPL; ++r)
+            if ((r & J) == 0) {
+                const int32_t a = key[r], b = key[r | J];
+                key[r] = a < b ? a : b;
+                key[r | J] = a < b ? b : a;
+            }
+    }
+    if constexpr (J > 1) sort_clean<J / 2, EPL>(key, lane);
+}
+
+// Ascending sort of the 64 * EPL keys of a wavefront, key r of lane l at
+// position l * EPL + r: a bitonic network in the form whose merges start by
+// comparing mirrored positions (i with k - 1 - i in each block of k), so
+// every compare-exchange puts the smaller key at the lower position and no
+// direction is selected per block.
+template <int K, int EPL>
+__device__ __forceinline__ void sort_merge_level(int32_t (&key)[EPL], int lane)
+{
+    // the mirrored compare of blocks of K
+    if constexpr (K <= EPL) {
+#pragma unroll
+        for (int r = 0; r < EPL; ++r)
+            if ((r & (K / 2)) == 0) {
+                const int q = r ^ (K - 1);
+                const int32_t a = key[r], b = key[q];
+                key[r] = a < b ? a : b;
+                key[q] = a < b ? b : a;
+            }
+    } else {
+        constexpr int M = (K - 1) / EPL;
+        exchange_lanes<M, EPL - 1>(key, (lane & (K / 2 / EPL)) == 0);
+    }
+    // then half-cleaners of blocks of J * 2, J = K / 4 ... 1
+    if constexpr (K >= 4) sort_clean<K / 4, EPL>(key, lane);
+}
+
 template <int EPL>
+__device__ __forceinline__ void sort_keys(int32_t (&key)[EPL], int lane)
+{
+    sort_merge_level<2, EPL>(key, lane);
+    if constexpr (64 * EPL >= 4) sort_merge_level<4, EPL>(key, lane);
+    if constexpr (64 * EPL >= 8) sort_merge_level<8, EPL>(key, lane);
+    if constexpr (64 * EPL >= 16) sort_merge_level<16, EPL>(key, lane);
+    if constexpr (64 * EPL >= 32) sort_merge_level<32, EPL>(key, lane);
+    if constexpr (64 * EPL >= 64) sort_merge_level<64, EPL>(key, lane);
+    if constexpr (64 * EPL >= 128) sort_merge_level<128, EPL>(key, lane);
+    if constexpr (64 * EPL >= 256) sort_merge_level<256, EPL>(key, lane);
+    if constexpr (64 * EPL >= 512) sort_merge_level<512, EPL>(key, lane);
+    if constexpr (64 * EPL >= 1024) sort_merge_level<1024, EPL>(key, lane);
+    if constexpr (64 * EPL >= 2048) sort_merge_level<2048, EPL>(key, lane);
+    static_assert(64 * EPL <= 2048, "at most 2048 keys per wavefront");
+}
+
+template <int EPL, bool ONE_SEED>
 __global__ __launch_bounds__(COUNT_THREADS) void k_count_wave(const uint2 *fin, const WallRec *walls, GeoView G,
                                                               int64_t n_p, int R, int S,
                                                               int32_t *uniq_recv, int32_t *uniq_cnt,
@@ -1213,26 +1338,49 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_wave(const uint2 *fin, 
     const int wave = threadIdx.x >> 6;
     int32_t *pos = s_pos[wave];
     int32_t *ukey = s_key[wave];
-    const int SR = S * R;
+    const int SR0 = S * R;
     const uint64_t below = (1ull << lane) - 1ull;
     for (int64_t q = (int64_t)blockIdx.x * WAVES + wave; q < n_p; q += (int64_t)gridDim.x * WAVES) {
+        // S * R, opaque to the compiler in each iteration: the per-key
+        // comparisons with it are then made where they are used instead of
+        // being hoisted out of the loop as lane masks that do not fit the
+        // scalar registers
+        int SR = SR0;
+        asm volatile("" : "+s"(SR));
         int32_t key[EPL];
         uint32_t bad = 0;
         uint32_t timed = 0;  // bit r: key r is a wall crossing time, resolved below
         // key r of a lane is the patch's ray lane + 64 r, so each load
         // instruction reads 64 consecutive final states (the sort below does
-        // not depend on where a key starts)
+        // not depend on where a key starts).  All final states are loaded
+        // first, then the walled-bitset cell codes of the FIN_SOLID_P ones
+        // (entry 0 for the others), so the loads of a patch are in flight
+        // together; then the receivers are worked out.
+        uint2 f[EPL];
+#pragma unroll
+        for (int r = 0; r < EPL; ++r) {
+            const int t = lane + 64 * r;
+            f[r] = make_uint2(0u, FIN_BOX | FIN_FAILED << 16);
+            if (t < SR) {
+                const int s = ONE_SEED ? 0 : t / R, u = t - s * R;
+                f[r] = fin[(int64_t)s * n_p * R + q * R + u];
+            }
+        }
+        uint32_t code[EPL];
+        if (G.wcode) {
+#pragma unroll
+            for (int r = 0; r < EPL; ++r) code[r] = G.wcode[fin_wcode_index(f[r], G)];
+        }
 #pragma unroll
         for (int r = 0; r < EPL; ++r) {
             const int t = lane + 64 * r;
             int32_t v = INT_MAX;
             if (t < SR) {
-                const int s = S == 1 ? 0 : t / R, u = t - s * R;
-                const uint2 f = fin[(int64_t)s * n_p * R + q * R + u];
-                if (fin_is_time(f)) {
+                if (fin_is_time(f[r])) {
                     timed |= 1u << r;
                 } else {
-                    v = resolve_boxed(f, G);
+                    v = fin_kind(f[r]) == FIN_SOLID_P && G.wcode ? resolve_solid_p(f[r], code[r], G)
+                                                                   : resolve_boxed(f[r], G);
                     if (v < 0) {
                         ++bad;
                         v = INT_MAX;
@@ -1258,7 +1406,7 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_wave(const uint2 *fin, 
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
             for (int i = lane; i < n; i += 64) {
                 const int t = list[i];
-                const int s = S == 1 ? 0 : t / R, u = t - s * R;
+                const int s = ONE_SEED ? 0 : t / R, u = t - s * R;
                 const int64_t at = (int64_t)s * n_p * R + q * R + u;
                 res[i] = resolve_wall_time(fin[at], G, walls[at]);
             }
@@ -1286,36 +1434,7 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_wave(const uint2 *fin, 
         }
         bad = (uint32_t)wave_sum_u64(bad);
         if (lane == 0 && bad) atomicAdd(failed, (unsigned long long)bad);
-#pragma unroll
-        for (int k = 2; k <= N; k <<= 1) {
-#pragma unroll
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                if (j < EPL) {
-#pragma unroll
-                    for (int r = 0; r < EPL; ++r) {
-                        if (r & j) continue;
-                        const int e = lane * EPL + r;
-                        const bool up = (e & k) == 0;
-                        const int32_t a = key[r], b = key[r | j];
-                        const int32_t lo = a < b ? a : b, hi = a < b ? b : a;
-                        key[r] = up ? lo : hi;
-                        key[r | j] = up ? hi : lo;
-                    }
-                } else {
-                    const int lj = j / EPL;
-                    const bool lower = (lane & lj) == 0;
-#pragma unroll
-                    for (int r = 0; r < EPL; ++r) {
-                        const int e = lane * EPL + r;
-                        const bool up = (e & k) == 0;
-                        const int32_t other = __shfl_xor(key[r], lj);
-                        const int32_t lo = key[r] < other ? key[r] : other;
-                        const int32_t hi = key[r] < other ? other : key[r];
-                        key[r] = (up == lower) ? lo : hi;
-                    }
-                }
-            }
-        }
+        sort_keys<EPL>(key, lane);
         // run starts among the first SR elements
         const int32_t prev_last = __shfl_up(key[EPL - 1], 1);
         int starts = 0;
@@ -1358,6 +1477,24 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_wave(const uint2 *fin, 
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
+}
+
+// The k_count_wave instance for EPL keys per lane and S seeds per call.
+using CountWaveKernel = void (*)(const uint2 *, const WallRec *, GeoView, int64_t, int, int, int32_t *, int32_t *,
+                                 int32_t *, unsigned long long *);
+CountWaveKernel count_wave_kernel(int epl, int S)
+{
+    if (S == 1)
+        return epl == 1 ? k_count_wave<1, true>
+             : epl == 2 ? k_count_wave<2, true>
+             : epl == 4 ? k_count_wave<4, true>
+             : epl == 8 ? k_count_wave<8, true>
+                        : k_count_wave<16, true>;
+    return epl == 1 ? k_count_wave<1, false>
+         : epl == 2 ? k_count_wave<2, false>
+         : epl == 4 ? k_count_wave<4, false>
+         : epl == 8 ? k_count_wave<8, false>
+                    : k_count_wave<16, false>;
 }
 
 __global__ __launch_bounds__(COUNT_THREADS) void k_compact_edges(const int32_t *uniq_recv, const int32_t *uniq_cnt,
@@ -1685,11 +1822,7 @@ int trace_pass(const Geometry &g, Workspace &W, int64_t p0, int64_t n_p, int R, 
         const int epl = SRp2 <= 64 ? 1 : SRp2 / 64;
         const int64_t blocks = (n_p + COUNT_THREADS / 64 - 1) / (COUNT_THREADS / 64);
         const unsigned wgrid = (unsigned)(blocks < 65536 ? blocks : 65536);
-        auto kern = epl == 1    ? k_count_wave<1>
-                    : epl == 2  ? k_count_wave<2>
-                    : epl == 4  ? k_count_wave<4>
-                    : epl == 8  ? k_count_wave<8>
-                                : k_count_wave<16>;
+        auto kern = count_wave_kernel(epl, S);
         kern<<<wgrid, COUNT_THREADS, 0, st>>>(W.fin.as<uint2>(), W.walls.as<WallRec>(), view_of(g), n_p, R, S,
                                               W.uniq_recv.as<int32_t>(),
                                               W.uniq_cnt.as<int32_t>(), W.n_unique.as<int32_t>(), ctr + 1);
@@ -2171,11 +2304,7 @@ int prh_trace_counts_into(const prh_geometry *geo, int64_t p0, int64_t p1, int32
         const int64_t blocks = (n_p + COUNT_THREADS / 64 - 1) / (COUNT_THREADS / 64);
         if (SRp2 <= 1024) {
             const int epl = SRp2 <= 64 ? 1 : SRp2 / 64;
-            auto kern = epl == 1    ? k_count_wave<1>
-                        : epl == 2  ? k_count_wave<2>
-                        : epl == 4  ? k_count_wave<4>
-                        : epl == 8  ? k_count_wave<8>
-                                    : k_count_wave<16>;
+            auto kern = count_wave_kernel(epl, S);
             kern<<<(unsigned)std::min<int64_t>(blocks, 65536), COUNT_THREADS, 0, st>>>(
                 fin, walls, view_of(g), n_p, R, S, uniq_recv, uniq_cnt, n_unique, ctr + 1);
         } else {
@@ -2340,8 +2469,7 @@ void count_launch(const prh_trace_batch *b, const BatchBuffers &B, const BatchPi
         const int epl = SRp2 <= 64 ? 1 : SRp2 / 64;
         const int64_t blocks = (np + COUNT_THREADS / 64 - 1) / (COUNT_THREADS / 64);
         const unsigned wgrid = (unsigned)(blocks < 65536 ? blocks : 65536);
-        auto kern = epl == 1 ? k_count_wave<1> : epl == 2 ? k_count_wave<2> : epl == 4 ? k_count_wave<4>
-                    : epl == 8 ? k_count_wave<8> : k_count_wave<16>;
+        auto kern = count_wave_kernel(epl, S);
         kern<<<wgrid, COUNT_THREADS, 0, st>>>(fin, walls, view_of(g), np, R, S, ur, uc, nu, failed);
     } else {
         const unsigned cgrid = (unsigned)(np < 65536 ? np : 65536);
