@@ -116,14 +116,16 @@ int prh_tracer_stats(int64_t *wave_steps, int64_t *refills, int32_t reset);
 int prh_tracer_set_grain(int32_t records);
 /* Free the current device's tracer workspace (ray records of the largest
  * prh_trace_counts / prh_trace_rays pass so far; it grows on demand and is
- * otherwise kept).  No counterpart in the reference. */
+ * otherwise kept) and the kept block of the last destroyed edge table (see
+ * prh_edges_destroy).  No counterpart in the reference. */
 int prh_tracer_trim(void);
 /* Bound of the tracer workspace (default 32 GiB, at least 1 MiB).  A call
  * also keeps the workspace within half of the device memory that is free or
  * already held by it; larger inputs are traced in several passes (patch
  * ranges, seed blocks), with the same results.  prh_tracer_workspace reports
- * the bytes the current device holds and the bound (either pointer may be
- * NULL), e.g. for an HBM budget that admits other work.  The workspace is
+ * the bytes the current device holds (the workspace and the kept edge-table
+ * block) and the bound (either pointer may be NULL), e.g. for an HBM budget
+ * that admits other work.  The workspace is
  * the bounded counterpart of the reference's patch_chunk
  * (porerad/view_factors.py:273-287), which bounds its host arrays. */
 int prh_tracer_set_workspace_limit(int64_t bytes);
@@ -355,6 +357,11 @@ void prh_trace_batch_destroy(prh_trace_batch *b);
 int prh_edges_equal(const prh_edges *a, const prh_edges *b, int32_t *equal);
 /* Copy a device edge table to host arrays of nnz entries. */
 int prh_edges_copy(const prh_edges *e, int32_t *origin, int32_t *receiver, int64_t *count);
+/* Free an edge table.  The single block that holds a counted table's three
+ * arrays is kept (one per device, the largest) for the next counted table
+ * that fits in it, so repeated counting neither allocates nor frees device
+ * memory; prh_tracer_trim releases it.  Waits for the device first, as
+ * freeing device memory does. */
 void prh_edges_destroy(prh_edges *e);
 
 /* Assemble the symmetric exchange matrix

@@ -20,6 +20,10 @@ struct prh_edges {
     int32_t *origin;
     int32_t *receiver;
     int32_t *count;
+    // the one allocation holding the three arrays (edge tables of the
+    // tracer's counting), or null when each array is an allocation of its own
+    void *block;
+    size_t block_bytes;
 };
 
 namespace prh {

@@ -1497,6 +1497,48 @@ CountWaveKernel count_wave_kernel(int epl, int S)
                     : k_count_wave<16, false>;
 }
 
+// Compaction by output ranges, for a pass whose edge total is known: block
+// r writes edges [r * COMPACT_RANGE, (r + 1) * COMPACT_RANGE) of the table,
+// so every store instruction of the three arrays writes 1 KB in whole lines.
+// range_q[r] is the patch holding the range's first edge (k_range_starts);
+// the range's later edges reach at most a few patches further.
+constexpr int COMPACT_RANGE = 1024;
+
+__global__ __launch_bounds__(256) void k_range_starts(const int64_t *offsets, int64_t n_p, int64_t nnz,
+                                                      int32_t *range_q)
+{
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n_p; q += (int64_t)gridDim.x * 256) {
+        const int64_t lo = offsets[q], hi = q + 1 < n_p ? offsets[q + 1] : nnz;
+        for (int64_t r = (lo + COMPACT_RANGE - 1) / COMPACT_RANGE; r * COMPACT_RANGE < hi; ++r) range_q[r] = (int32_t)q;
+    }
+}
+
+__global__ __launch_bounds__(COUNT_THREADS) void k_compact_ranges(const int32_t *uniq_recv, const int32_t *uniq_cnt,
+                                                                   const int64_t *offsets, const int32_t *range_q,
+                                                                   int64_t n_p, int SR, int64_t p0, int64_t nnz,
+                                                                   int32_t *origin, int32_t *receiver, int32_t *count)
+{
+    static_assert(COMPACT_RANGE % COUNT_THREADS == 0, "whole rounds per range");
+    for (int64_t r = blockIdx.x; r * COMPACT_RANGE < nnz; r += gridDim.x) {
+        int64_t q = range_q[r];
+        int64_t base = offsets[q], next = q + 1 < n_p ? offsets[q + 1] : nnz;
+#pragma unroll
+        for (int i = 0; i < COMPACT_RANGE / COUNT_THREADS; ++i) {
+            const int64_t e = r * COMPACT_RANGE + i * COUNT_THREADS + threadIdx.x;
+            if (e >= nnz) break;
+            while (e >= next) {  // the next patch with edges
+                ++q;
+                base = next;
+                next = q + 1 < n_p ? offsets[q + 1] : nnz;
+            }
+            const int64_t at = q * SR + (e - base);
+            origin[e] = (int32_t)(p0 + q);
+            receiver[e] = uniq_recv[at];
+            count[e] = uniq_cnt[at];
+        }
+    }
+}
+
 __global__ __launch_bounds__(COUNT_THREADS) void k_compact_edges(const int32_t *uniq_recv, const int32_t *uniq_cnt,
                                                                  const int32_t *n_unique, const int64_t *offsets,
                                                                  int64_t n_p, int SR, int64_t p0, int32_t *origin,
@@ -1583,6 +1625,99 @@ int workspace_free(Workspace &W)
         b->cap = 0;
     }
     return PRH_OK;
+}
+
+// Edge tables of the counting hold their three arrays in one allocation.
+// A destroyed table's block is kept, one per device, for the next table that
+// fits in it (at most twice its size), so a traced step neither allocates
+// nor frees device memory; prh_tracer_trim releases it and
+// prh_tracer_workspace counts it as held.
+struct EdgeBlockCache {
+    std::mutex m;
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+EdgeBlockCache g_edge_cache[64];
+
+constexpr size_t edge_array_bytes(int64_t nnz) { return ((size_t)(nnz > 0 ? nnz : 1) * sizeof(int32_t) + 255) & ~size_t(255); }
+
+// A new table of nnz edges (arrays uninitialised) on device dev, current.
+int new_edge_table(int dev, int64_t nnz, prh_edges **out)
+{
+    const size_t a = edge_array_bytes(nnz), need = 3 * a;
+    void *p = nullptr;
+    size_t bytes = 0;
+    {
+        EdgeBlockCache &c = g_edge_cache[dev];
+        std::lock_guard<std::mutex> lock(c.m);
+        if (c.p && c.bytes >= need && c.bytes <= 2 * need) {
+            p = c.p;
+            bytes = c.bytes;
+            c.p = nullptr;
+            c.bytes = 0;
+        }
+    }
+    if (!p) {
+        if (hipMalloc(&p, need) != hipSuccess) {
+            (void)hipGetLastError();
+            // the kept block may be what stands in the way
+            {
+                EdgeBlockCache &c = g_edge_cache[dev];
+                std::lock_guard<std::mutex> lock(c.m);
+                if (c.p) (void)hipFree(c.p);
+                c.p = nullptr;
+                c.bytes = 0;
+            }
+            if (hipMalloc(&p, need) != hipSuccess) {
+                (void)hipGetLastError();
+                set_error("prh_trace_counts: out of device memory for %lld edges", (long long)nnz);
+                return PRH_ERR_HIP;
+            }
+        }
+        bytes = need;
+    }
+    char *b = static_cast<char *>(p);
+    *out = new prh_edges{dev, nnz, reinterpret_cast<int32_t *>(b), reinterpret_cast<int32_t *>(b + a),
+                         reinterpret_cast<int32_t *>(b + 2 * a), p, bytes};
+    return PRH_OK;
+}
+
+// Keeps or frees the block of a destroyed table (device dev current).
+void release_edge_block(int dev, void *p, size_t bytes)
+{
+    // work that may still read the table (on any stream) ends first, as
+    // hipFree would wait for it
+    (void)hipDeviceSynchronize();
+    if (dev < 0 || dev >= 64) {
+        (void)hipFree(p);
+        return;
+    }
+    EdgeBlockCache &c = g_edge_cache[dev];
+    std::lock_guard<std::mutex> lock(c.m);
+    // kept only within the workspace bound, with the workspace
+    if ((c.p && c.bytes >= bytes) || (int64_t)bytes + workspace_held(g_ws[dev]) > g_ws_limit) {
+        (void)hipFree(p);
+        return;
+    }
+    if (c.p) (void)hipFree(c.p);
+    c.p = p;
+    c.bytes = bytes;
+}
+
+int64_t edge_cache_held(int dev)
+{
+    EdgeBlockCache &c = g_edge_cache[dev];
+    std::lock_guard<std::mutex> lock(c.m);
+    return (int64_t)c.bytes;
+}
+
+void edge_cache_free(int dev)
+{
+    EdgeBlockCache &c = g_edge_cache[dev];
+    std::lock_guard<std::mutex> lock(c.m);
+    if (c.p) (void)hipFree(c.p);
+    c.p = nullptr;
+    c.bytes = 0;
 }
 
 // Workspace bytes a call may use on the current device.
@@ -1835,30 +1970,29 @@ int trace_pass(const Geometry &g, Workspace &W, int64_t p0, int64_t n_p, int R, 
     }
     PRH_HIP(hipGetLastError());
     PRH_HIP(hipEventRecord(W.ev[3], st));
+    // the counters come back with the edge total (one wait for both)
     unsigned long long counters[2] = {0, 0};
     PRH_HIP(hipMemcpyAsync(counters, W.counters.p, sizeof(counters), hipMemcpyDeviceToHost, st));
-    PRH_HIP(hipStreamSynchronize(st));
+    int64_t nnz = 0;
+    if ((rc = exclusive_scan_i32_to_i64(W.n_unique.as<int32_t>(), W.offsets.as<int64_t>(), n_p, &nnz, st)))
+        return rc;
     *reads += counters[0];
     if (counters[1]) {
         set_error("%llu rays did not resolve a receiving patch", counters[1]);
         return PRH_ERR_UNRESOLVED;
     }
-    int64_t nnz = 0;
-    if ((rc = exclusive_scan_i32_to_i64(W.n_unique.as<int32_t>(), W.offsets.as<int64_t>(), n_p, &nnz, st)))
-        return rc;
-    prh_edges *e = new prh_edges{dev, nnz, nullptr, nullptr, nullptr};
-    const int64_t alloc = nnz > 0 ? nnz : 1;
-    if (hipMalloc((void **)&e->origin, alloc * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc((void **)&e->receiver, alloc * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc((void **)&e->count, alloc * sizeof(int32_t)) != hipSuccess) {
-        prh_edges_destroy(e);
-        (void)hipGetLastError();
-        set_error("prh_trace_counts: out of device memory for %lld edges", (long long)nnz);
-        return PRH_ERR_HIP;
+    prh_edges *e = nullptr;
+    if ((rc = new_edge_table(dev, nnz, &e))) return rc;
+    if (nnz > 0) {
+        // the wall records are not read after counting: their space holds the range starts
+        const int64_t n_ranges = (nnz + COMPACT_RANGE - 1) / COMPACT_RANGE;
+        int32_t *range_q = W.walls.as<int32_t>();
+        k_range_starts<<<(unsigned)std::min<int64_t>((n_p + 255) / 256, 65536), 256, 0, st>>>(
+            W.offsets.as<int64_t>(), n_p, nnz, range_q);
+        k_compact_ranges<<<(unsigned)std::min<int64_t>(n_ranges, 65536), COUNT_THREADS, 0, st>>>(
+            W.uniq_recv.as<int32_t>(), W.uniq_cnt.as<int32_t>(), W.offsets.as<int64_t>(), range_q, n_p, SR, p0, nnz,
+            e->origin, e->receiver, e->count);
     }
-    k_compact_edges<<<cgrid, COUNT_THREADS, 0, st>>>(W.uniq_recv.as<int32_t>(), W.uniq_cnt.as<int32_t>(),
-                                                     W.n_unique.as<int32_t>(), W.offsets.as<int64_t>(), n_p, SR, p0,
-                                                     e->origin, e->receiver, e->count, nnz);
     hipError_t err = hipGetLastError();
     if (err == hipSuccess) err = hipEventRecord(W.ev[4], st);
     if (err == hipSuccess) err = hipStreamSynchronize(st);
@@ -1883,16 +2017,9 @@ int concatenate_edges(const std::vector<prh_edges *> &parts, int dev, hipStream_
 {
     int64_t nnz = 0;
     for (const prh_edges *e : parts) nnz += e->nnz;
-    prh_edges *res = new prh_edges{dev, nnz, nullptr, nullptr, nullptr};
-    const int64_t alloc = nnz > 0 ? nnz : 1;
-    if (hipMalloc((void **)&res->origin, alloc * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc((void **)&res->receiver, alloc * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc((void **)&res->count, alloc * sizeof(int32_t)) != hipSuccess) {
-        prh_edges_destroy(res);
-        (void)hipGetLastError();
-        set_error("prh_trace_counts: out of device memory for %lld edges", (long long)nnz);
-        return PRH_ERR_HIP;
-    }
+    prh_edges *res = nullptr;
+    int rc = new_edge_table(dev, nnz, &res);
+    if (rc) return rc;
     int64_t at = 0;
     hipError_t err = hipSuccess;
     for (const prh_edges *e : parts) {
@@ -2004,6 +2131,7 @@ int prh_tracer_trim(void)
     int rc = current_stream(&st);
     if (rc) return rc;
     PRH_HIP(hipStreamSynchronize(st));
+    edge_cache_free(dev);
     return workspace_free(g_ws[dev]);
 }
 
@@ -2104,6 +2232,7 @@ int prh_trace_rays(const prh_geometry *geo, const double *origins, const double 
     int64_t budget = 0;
     if ((rc = workspace_budget(W, &budget))) return rc;
     if (workspace_held(W) > budget && (rc = workspace_free(W))) return rc;
+    if (workspace_held(W) + edge_cache_held(g.device) > budget) edge_cache_free(g.device);
     // rays in chunks that fit the workspace budget (and 2^31 records)
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(budget / GIVEN_BYTES_PER_RAY, int64_t(1) << 31));
     const int64_t m = std::min(chunk, n_rays);
@@ -2175,6 +2304,7 @@ int prh_trace_counts(const prh_geometry *geo, int64_t p0, int64_t p1, int32_t ra
     int64_t budget = 0;
     if ((rc = workspace_budget(W, &budget))) return rc;
     if (workspace_held(W) > budget && (rc = workspace_free(W))) return rc;
+    if (workspace_held(W) + edge_cache_held(g.device) > budget) edge_cache_free(g.device);
     if ((rc = grow(W.seeds, sizeof(uint64_t) * 4 * S))) return rc;
     PRH_HIP(hipMemcpyAsync(W.seeds.p, state_inc, sizeof(uint64_t) * 4 * S, hipMemcpyHostToDevice, st));
 
@@ -2351,7 +2481,7 @@ int prh_tracer_workspace(int64_t *held_bytes, int64_t *limit_bytes)
     int dev = 0;
     PRH_HIP(hipGetDevice(&dev));
     PRH_CHECK_ARG(dev >= 0 && dev < 64, "prh_tracer_workspace: device %d out of range", dev);
-    if (held_bytes) *held_bytes = workspace_held(g_ws[dev]);
+    if (held_bytes) *held_bytes = workspace_held(g_ws[dev]) + edge_cache_held(dev);
     if (limit_bytes) *limit_bytes = g_ws_limit;
     return PRH_OK;
 }
@@ -2964,9 +3094,13 @@ void prh_edges_destroy(prh_edges *e)
 {
     if (!e) return;
     (void)hipSetDevice(e->device);
-    if (e->origin) (void)hipFree(e->origin);
-    if (e->receiver) (void)hipFree(e->receiver);
-    if (e->count) (void)hipFree(e->count);
+    if (e->block) {
+        release_edge_block(e->device, e->block, e->block_bytes);
+    } else {
+        if (e->origin) (void)hipFree(e->origin);
+        if (e->receiver) (void)hipFree(e->receiver);
+        if (e->count) (void)hipFree(e->count);
+    }
     delete e;
 }
 
