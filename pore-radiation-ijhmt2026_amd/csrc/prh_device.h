@@ -137,19 +137,17 @@ __device__ __forceinline__ double initial_cell(double position, double direction
 __device__ __forceinline__ void axis_state(double position, double direction, double index, int &step,
                                            double &t_max, double &t_delta)
 {
-    if (direction > 0.0) {
-        step = 1;
-        t_max = (index + 1.0 - position) / direction;
-        t_delta = 1.0 / direction;
-    } else if (direction < 0.0) {
-        step = -1;
-        t_max = (index - position) / direction;
-        t_delta = -1.0 / direction;
-    } else {
-        step = 0;
-        t_max = INF_T;
-        t_delta = INF_T;
-    }
+    // Branch-free: a branch per sign would run both division pairs in a
+    // wavefront whose rays point both ways.  The numerator is the
+    // reference's for the sign, and 1 / |d| is its -1.0 / d for d < 0
+    // (division rounds symmetrically in sign).
+    const bool up = direction > 0.0;
+    const double num = up ? (index + 1.0) - position : index - position;
+    const double tm = num / direction;
+    const double td = 1.0 / fabs(direction);
+    step = up ? 1 : (direction < 0.0 ? -1 : 0);
+    t_max = direction != 0.0 ? tm : INF_T;
+    t_delta = direction != 0.0 ? td : INF_T;
 }
 
 // Python's f % n for an integral double f with |f| < 2^52, inv_n = 1 / n
