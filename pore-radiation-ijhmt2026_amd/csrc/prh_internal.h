@@ -200,6 +200,15 @@ __host__ __device__ inline uint32_t lean_dilate(const LeanLayout &L, int a, uint
     return lean_open_gap(morton_spread(v & ((1u << L.m) - 1u)) << (2 - a)) | (v >> L.m) << L.field[a];
 }
 
+// The same with the interleaved part read from a table of
+// lean_open_gap(morton_spread(v)), v < 1024 (LEAN_SPREAD_ENTRIES): opening
+// the gap commutes with the axis's shift of 0-2 bits.
+constexpr int LEAN_SPREAD_ENTRIES = 1024;
+__host__ __device__ inline uint32_t lean_dilate_table(const LeanLayout &L, int a, uint32_t v, const uint32_t *spread)
+{
+    return spread[v & ((1u << L.m) - 1u)] << (2 - a) | (v >> L.m) << L.field[a];
+}
+
 __host__ __device__ inline uint32_t lean_compact(const LeanLayout &L, int a, uint32_t d)
 {
     const uint32_t inter = lean_open_gap(morton_spread((1u << L.m) - 1u) << (2 - a));

@@ -182,8 +182,10 @@ TraversalConst traversal_const(const Geometry &g)
 // (view_factors.py:211-224): initial cell (_initial_cell), per-axis step,
 // crossing time and increment (_axis_state), the periodic wrap of the
 // lateral indices and the step cap.  A ray with dx == 0 gets cap 0.
+// spread: a table for lean_dilate_table (in LDS), or null.
 __device__ __forceinline__ void make_dda(const double o[3], const double d[3], const GeoView &G,
-                                         const TraversalConst &C, DdaRec &rec, WallRec &w)
+                                         const TraversalConst &C, DdaRec &rec, WallRec &w,
+                                         const uint32_t *spread = nullptr)
 {
     w.oy = o[1];
     w.oz = o[2];
@@ -214,9 +216,12 @@ __device__ __forceinline__ void make_dda(const double o[3], const double d[3], c
     if (fabs(d[0]) < C.grazing_dx) cap = max_steps_of(d[0], d[1], d[2], G.nx, G.ny, G.nz);
     rec.cap = cap;
     // the start cell's walled-bitset index, for the lean traversal
-    rec.P = G.wocc ? lean_dilate(G.wl, 0, (uint32_t)(ic + 1)) | lean_dilate(G.wl, 1, (uint32_t)jj) |
-                           lean_dilate(G.wl, 2, (uint32_t)kk)
-                     : 0u;
+    rec.P = !G.wocc ? 0u
+            : spread ? lean_dilate_table(G.wl, 0, (uint32_t)(ic + 1), spread) |
+                           lean_dilate_table(G.wl, 1, (uint32_t)jj, spread) |
+                           lean_dilate_table(G.wl, 2, (uint32_t)kk, spread)
+                     : lean_dilate(G.wl, 0, (uint32_t)(ic + 1)) | lean_dilate(G.wl, 1, (uint32_t)jj) |
+                           lean_dilate(G.wl, 2, (uint32_t)kk);
 }
 
 // Traversal state of ray g from its record; false for a ray that fails at once.
@@ -516,8 +521,16 @@ struct GenArgs {
 // RECORDS: write the traversal records (DDA start state, wall inputs)
 // instead of the origins and directions, so the divisions of the set-up run
 // here in converged waves rather than in the traversal's refills.
+// The table of lean_dilate_table in the workgroup's LDS (every thread calls
+// this before any returns).
+__device__ __forceinline__ void fill_spread_table(uint32_t *t)
+{
+    for (int i = threadIdx.x; i < LEAN_SPREAD_ENTRIES; i += blockDim.x) t[i] = lean_open_gap(morton_spread((uint32_t)i));
+    __syncthreads();
+}
+
 template <bool RECORDS>
-__device__ __forceinline__ void gen_rays_body(const GenArgs &a, uint64_t base)
+__device__ __forceinline__ void gen_rays_body(const GenArgs &a, uint64_t base, const uint32_t *spread)
 {
     const GeoView &G = a.G;
     const uint32_t seg = a.n_p * a.R;
@@ -573,7 +586,7 @@ __device__ __forceinline__ void gen_rays_body(const GenArgs &a, uint64_t base)
             const double *po = G.patch_origin + 3 * (size_t)p;
             make_ray(st, inc, po[0], po[1], po[2], normal_code_of(G, p), fny, fnz, o, d);
             st = next;
-            if (RECORDS) make_dda(o, d, G, a.C, rec, w);
+            if (RECORDS) make_dda(o, d, G, a.C, rec, w, spread);
         }
         if (RECORDS) {
             if (valid) {
@@ -621,8 +634,10 @@ __device__ __forceinline__ void gen_rays_body(const GenArgs &a, uint64_t base)
 template <bool RECORDS>
 __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_gen_rays(GenArgs a)
 {
+    __shared__ uint32_t s_spread[LEAN_SPREAD_ENTRIES];
+    fill_spread_table(s_spread);
     const uint64_t wave = ((uint64_t)blockIdx.x * TRACE_THREADS + threadIdx.x) >> 6;
-    gen_rays_body<RECORDS>(a, wave * (64ull * GEN_ROUNDS) + (threadIdx.x & 63u));
+    gen_rays_body<RECORDS>(a, wave * (64ull * GEN_ROUNDS) + (threadIdx.x & 63u), s_spread);
 }
 
 // One segment of a multi-geometry batch: a patch range of one geometry,
@@ -649,11 +664,14 @@ __device__ __forceinline__ int segment_of(const Segment *segs, int n_seg, uint32
 __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_gen_rays_batch(
     const Segment *segs, int n_seg)
 {
+    __shared__ uint32_t s_spread[LEAN_SPREAD_ENTRIES];
+    fill_spread_table(s_spread);
     // wave-uniform, so the segment's arguments load into scalar registers
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * TRACE_THREADS + threadIdx.x) >> 6);
     const int k = segment_of(segs, n_seg, wave, false);
     const GenArgs a = segs[k].gen;
-    gen_rays_body<true>(a, (uint64_t)(wave - segs[k].gen_wave_begin) * (64ull * GEN_ROUNDS) + (threadIdx.x & 63u));
+    gen_rays_body<true>(a, (uint64_t)(wave - segs[k].gen_wave_begin) * (64ull * GEN_ROUNDS) + (threadIdx.x & 63u),
+                        s_spread);
 }
 
 // Traversal records of given rays (prh_trace_rays).
