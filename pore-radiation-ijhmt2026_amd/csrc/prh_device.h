@@ -70,6 +70,43 @@ __device__ __forceinline__ double clip_uniform(double u)
     return u < lo ? lo : (u > hi ? hi : u);
 }
 
+// a / b and sqrt(x), correctly rounded, for operands that need none of the
+// exponent scaling, class tests and fix-ups of the compiler's IEEE
+// expansions: the expansions' own core sequences (reciprocal or reciprocal
+// square root, Newton steps, one fused correction), which give the
+// correctly rounded result whenever no intermediate leaves the normal
+// range.  div_rn_plain: b finite, 2^-900 < |b| < 2^900, a == 0 or
+// 2^-900 < |a| < 2^900, and |a / b| within the same range; sqrt_rn_plain:
+// 2^-767 < x < 2^1000.  Generated rays stay far inside both: unit-vector
+// components are 0 or above 1e-25 in magnitude, and a set-up numerator
+// (a cell face minus an origin coordinate) is 0 or at least the smaller of
+// 2^-52 and the face-edge inset.
+__device__ __forceinline__ double div_rn_plain(double a, double b)
+{
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    const double q = a * r;
+    const double res = fma(-b, q, a);
+    return fma(res, r, q);
+}
+
+__device__ __forceinline__ double sqrt_rn_plain(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    double d = fma(-g, g, x);
+    h = fma(h, r, h);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+
 // One Lambertian ray of a patch (generate_rays, view_factors.py:147-181).
 // normal_code: 0..5 = +x, -x, +y, -y, +z, -z (walls: hot 0, cold 1).
 __device__ __forceinline__ void make_ray(u128 &s, const u128 inc, const double po0, const double po1, const double po2,
@@ -97,13 +134,13 @@ __device__ __forceinline__ void make_ray(u128 &s, const u128 inc, const double p
     o[0] = x;
     o[1] = py_mod(y, fny);
     o[2] = py_mod(z, fnz);
-    const double radius = sqrt(u2);
+    const double radius = u2 > 0.0 ? sqrt_rn_plain(u2) : 0.0;  // u2 is 0 or at least 2^-53
     const double phi = TWO_PI * u3;
     double sphi, cphi;
     libm::sincos(phi, sphi, cphi);  // the C library's bits, as NumPy's cos/sin give them
     const double la = radius * cphi;
     const double lb = radius * sphi;
-    const double ct = sqrt(1.0 - u2);
+    const double ct = sqrt_rn_plain(1.0 - u2);  // 1 - u2 >= 2^-53
     double d0, d1, d2;
     if (axis == 0) {
         d0 = sign * ct;
@@ -118,10 +155,10 @@ __device__ __forceinline__ void make_ray(u128 &s, const u128 inc, const double p
         d0 = la;
         d1 = sign * lb;
     }
-    const double norm = sqrt((d0 * d0 + d1 * d1) + d2 * d2);
-    d[0] = d0 / norm;
-    d[1] = d1 / norm;
-    d[2] = d2 / norm;
+    const double norm = sqrt_rn_plain((d0 * d0 + d1 * d1) + d2 * d2);  // about 1
+    d[0] = div_rn_plain(d0, norm);
+    d[1] = div_rn_plain(d1, norm);
+    d[2] = div_rn_plain(d2, norm);
 }
 
 // _initial_cell, view_factors.py:185-190 (round() = round-half-even), as an
@@ -134,6 +171,8 @@ __device__ __forceinline__ double initial_cell(double position, double direction
 }
 
 // _axis_state, view_factors.py:193-199 (index + 1.0 is exact: index is integral).
+// PLAIN: the operands are a generated ray's (div_rn_plain applies).
+template <bool PLAIN = false>
 __device__ __forceinline__ void axis_state(double position, double direction, double index, int &step,
                                            double &t_max, double &t_delta)
 {
@@ -143,8 +182,8 @@ __device__ __forceinline__ void axis_state(double position, double direction, do
     // (division rounds symmetrically in sign).
     const bool up = direction > 0.0;
     const double num = up ? (index + 1.0) - position : index - position;
-    const double tm = num / direction;
-    const double td = 1.0 / fabs(direction);
+    const double tm = PLAIN ? div_rn_plain(num, direction) : num / direction;
+    const double td = PLAIN ? div_rn_plain(1.0, fabs(direction)) : 1.0 / fabs(direction);
     step = up ? 1 : (direction < 0.0 ? -1 : 0);
     t_max = direction != 0.0 ? tm : INF_T;
     t_delta = direction != 0.0 ? td : INF_T;

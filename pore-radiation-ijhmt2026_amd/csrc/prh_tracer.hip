@@ -182,7 +182,10 @@ TraversalConst traversal_const(const Geometry &g)
 // (view_factors.py:211-224): initial cell (_initial_cell), per-axis step,
 // crossing time and increment (_axis_state), the periodic wrap of the
 // lateral indices and the step cap.  A ray with dx == 0 gets cap 0.
-// spread: a table for lean_dilate_table (in LDS), or null.
+// spread: a table for lean_dilate_table (in LDS), or null.  GENERATED: the
+// ray comes from make_ray (unit direction, origin a multiple of 2^-53), so
+// the set-up's divisions need no IEEE special-case handling.
+template <bool GENERATED = false>
 __device__ __forceinline__ void make_dda(const double o[3], const double d[3], const GeoView &G,
                                          const TraversalConst &C, DdaRec &rec, WallRec &w,
                                          const uint32_t *spread = nullptr)
@@ -200,9 +203,9 @@ __device__ __forceinline__ void make_dda(const double o[3], const double d[3], c
     const double j = initial_cell(o[1], d[1]);
     const double k = initial_cell(o[2], d[2]);
     int sx, sy, sz;
-    axis_state(o[0], d[0], i, sx, rec.tx, rec.ddx);
-    axis_state(o[1], d[1], j, sy, rec.ty, rec.ddy);
-    axis_state(o[2], d[2], k, sz, rec.tz, rec.ddz);
+    axis_state<GENERATED>(o[0], d[0], i, sx, rec.tx, rec.ddx);
+    axis_state<GENERATED>(o[1], d[1], j, sy, rec.ty, rec.ddy);
+    axis_state<GENERATED>(o[2], d[2], k, sz, rec.tz, rec.ddz);
     // The reference keeps j, k unbounded and reads j % ny, k % nz; the
     // traversal carries the wrapped indices from the start.  A ray whose
     // x index starts outside [0, nx) ends at once on the wall of that side
@@ -586,7 +589,7 @@ __device__ __forceinline__ void gen_rays_body(const GenArgs &a, uint64_t base, c
             const double *po = G.patch_origin + 3 * (size_t)p;
             make_ray(st, inc, po[0], po[1], po[2], normal_code_of(G, p), fny, fnz, o, d);
             st = next;
-            if (RECORDS) make_dda(o, d, G, a.C, rec, w, spread);
+            if (RECORDS) make_dda<true>(o, d, G, a.C, rec, w, spread);
         }
         if (RECORDS) {
             if (valid) {
@@ -605,15 +608,23 @@ __device__ __forceinline__ void gen_rays_body(const GenArgs &a, uint64_t base, c
             const uint64_t n_round = a.total - round0 < 64 ? a.total - round0 : 64;  // records of this round
             uint4 *dd = reinterpret_cast<uint4 *>(a.dda + (uint32_t)round0 + a.offset);
             uint4 *wd = reinterpret_cast<uint4 *>(a.walls + (uint32_t)round0 + a.offset);
+            if (n_round == 64) {  // every round but a launch's last: no per-piece checks
 #pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const uint32_t piece = lane + 64u * m;
-                if ((piece >> 2) < n_round) dd[piece] = sd[piece];
-            }
+                for (int m = 0; m < 4; ++m) dd[lane + 64u * m] = sd[lane + 64u * m];
 #pragma unroll
-            for (int m = 0; m < 2; ++m) {
-                const uint32_t piece = lane + 64u * m;
-                if ((piece >> 1) < n_round) wd[piece] = sw[piece];
+                for (int m = 0; m < 2; ++m) wd[lane + 64u * m] = sw[lane + 64u * m];
+            } else {
+                const uint32_t nr = (uint32_t)n_round;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const uint32_t piece = lane + 64u * m;
+                    if ((piece >> 2) < nr) dd[piece] = sd[piece];
+                }
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                    const uint32_t piece = lane + 64u * m;
+                    if ((piece >> 1) < nr) wd[piece] = sw[piece];
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             __builtin_amdgcn_wave_barrier();
