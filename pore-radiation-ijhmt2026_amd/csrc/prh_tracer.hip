@@ -1269,24 +1269,31 @@ __device__ __forceinline__ int32_t xor_lanes(int32_t v)
     return __shfl_xor(v, M);
 }
 
+// The middle of a, b and c: with c = INT_MIN the smaller of a and b, with
+// c = INT_MAX the larger, so one instruction keeps either side of an
+// exchange.
+__device__ __forceinline__ int32_t med3_i32(int32_t a, int32_t b, int32_t c)
+{
+    int32_t r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 // One cross-lane compare-exchange of register r with lane (lane ^ M) at
 // register r ^ FLIP: the lane whose bit M is clear keeps the smaller key.
 template <int M, int FLIP, int EPL>
 __device__ __forceinline__ void exchange_lanes(int32_t (&key)[EPL], bool lower)
 {
+    const int32_t keep = lower ? INT_MIN : INT_MAX;  // med3 with it keeps this lane's side
     if constexpr (FLIP == 0) {
 #pragma unroll
-        for (int r = 0; r < EPL; ++r) {
-            const int32_t o = xor_lanes<M>(key[r]);
-            key[r] = lower ? (key[r] < o ? key[r] : o) : (key[r] < o ? o : key[r]);
-        }
+        for (int r = 0; r < EPL; ++r) key[r] = med3_i32(key[r], xor_lanes<M>(key[r]), keep);
     } else {
 #pragma unroll
         for (int r = 0; r < EPL / 2; ++r) {
             const int32_t a = xor_lanes<M>(key[r ^ FLIP]), b = xor_lanes<M>(key[r]);
-            const int32_t x = key[r], y = key[r ^ FLIP];
-            key[r] = lower ? (x < a ? x : a) : (x < a ? a : x);
-            key[r ^ FLIP] = lower ? (y < b ? y : b) : (y < b ? b : y);
+            key[r] = med3_i32(key[r], a, keep);
+            key[r ^ FLIP] = med3_i32(key[r ^ FLIP], b, keep);
         }
     }
 }
@@ -1354,7 +1361,7 @@ __device__ __forceinline__ void sort_keys(int32_t (&key)[EPL], int lane)
 }
 
 template <int EPL, bool ONE_SEED>
-__global__ __launch_bounds__(COUNT_THREADS) void k_count_wave(const uint2 *fin, const WallRec *walls, GeoView G,
+__global__ __launch_bounds__(COUNT_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_count_wave(const uint2 *fin, const WallRec *walls, GeoView G,
                                                               int64_t n_p, int R, int S,
                                                               int32_t *uniq_recv, int32_t *uniq_cnt,
                                                               int32_t *n_unique, unsigned long long *failed)
