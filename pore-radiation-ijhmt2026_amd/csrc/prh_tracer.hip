@@ -42,6 +42,9 @@ namespace {
 constexpr int TRACE_THREADS = 256;
 constexpr int STREAM_THREADS = 1024;  // traversal: 16 waves share one LDS brick bitset
 constexpr int COUNT_THREADS = 256;
+// Rays per patch (and call) that one wavefront counts with its keys in
+// registers (32 per lane); beyond, a workgroup sorts them in LDS.
+constexpr int COUNT_WAVE_MAX_KEYS = 2048;
 constexpr int MAX_RAYS_PER_PATCH_CALL = 16384;
 
 struct GeoView {
@@ -1361,7 +1364,7 @@ __device__ __forceinline__ void sort_keys(int32_t (&key)[EPL], int lane)
 }
 
 template <int EPL, bool ONE_SEED>
-__global__ __launch_bounds__(COUNT_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_count_wave(const uint2 *fin, const WallRec *walls, GeoView G,
+__device__ __forceinline__ void count_wave_body(const uint2 *fin, const WallRec *walls, GeoView G,
                                                               int64_t n_p, int R, int S,
                                                               int32_t *uniq_recv, int32_t *uniq_cnt,
                                                               int32_t *n_unique, unsigned long long *failed)
@@ -1385,54 +1388,71 @@ __global__ __launch_bounds__(COUNT_THREADS) __attribute__((amdgpu_waves_per_eu(4
         asm volatile("" : "+s"(SR));
         int32_t key[EPL];
         uint32_t bad = 0;
-        uint32_t timed = 0;  // bit r: key r is a wall crossing time, resolved below
+        uint64_t timed = 0;  // bit r: key r is a wall crossing time, resolved below
         // key r of a lane is the patch's ray lane + 64 r, so each load
         // instruction reads 64 consecutive final states (the sort below does
         // not depend on where a key starts).  All final states are loaded
         // first, then the walled-bitset cell codes of the FIN_SOLID_P ones
         // (entry 0 for the others), so the loads of a patch are in flight
         // together; then the receivers are worked out.
-        uint2 f[EPL];
-#pragma unroll
-        for (int r = 0; r < EPL; ++r) {
-            const int t = lane + 64 * r;
-            f[r] = make_uint2(0u, FIN_BOX | FIN_FAILED << 16);
-            if (t < SR) {
-                const int s = ONE_SEED ? 0 : t / R, u = t - s * R;
-                f[r] = fin[(int64_t)s * n_p * R + q * R + u];
+        // (in batches of at most 16 keys, which bounds the registers)
+        constexpr int BATCH = EPL < 16 ? EPL : 16;
+        // key t = lane + 64 r is ray u of seed s (t = s R + u); with several
+        // seeds (s, u) advance by 64 keys at a time instead of a division per key
+        int ks = 0, ku = lane;
+        if (!ONE_SEED)
+            while (ku >= R) {
+                ku -= R;
+                ++ks;
             }
-        }
-        uint32_t code[EPL];
-        if (G.wcode) {
 #pragma unroll
-            for (int r = 0; r < EPL; ++r) code[r] = G.wcode[fin_wcode_index(f[r], G)];
-        }
+        for (int b0 = 0; b0 < EPL; b0 += BATCH) {
+            uint2 f[BATCH];
 #pragma unroll
-        for (int r = 0; r < EPL; ++r) {
-            const int t = lane + 64 * r;
-            int32_t v = INT_MAX;
-            if (t < SR) {
-                if (fin_is_time(f[r])) {
-                    timed |= 1u << r;
-                } else {
-                    v = fin_kind(f[r]) == FIN_SOLID_P && G.wcode ? resolve_solid_p(f[r], code[r], G)
-                                                                   : resolve_boxed(f[r], G);
-                    if (v < 0) {
-                        ++bad;
-                        v = INT_MAX;
+            for (int i = 0; i < BATCH; ++i) {
+                const int t = lane + 64 * (b0 + i);
+                f[i] = make_uint2(0u, FIN_BOX | FIN_FAILED << 16);
+                if (t < SR) f[i] = fin[(int64_t)ks * n_p * R + q * R + (ONE_SEED ? t : ku)];
+                if (!ONE_SEED) {
+                    ku += 64;
+                    while (ku >= R) {
+                        ku -= R;
+                        ++ks;
                     }
                 }
             }
-            key[r] = v;
+            uint32_t code[BATCH];
+            if (G.wcode) {
+#pragma unroll
+                for (int i = 0; i < BATCH; ++i) code[i] = G.wcode[fin_wcode_index(f[i], G)];
+            }
+#pragma unroll
+            for (int i = 0; i < BATCH; ++i) {
+                const int r = b0 + i, t = lane + 64 * r;
+                int32_t v = INT_MAX;
+                if (t < SR) {
+                    if (fin_is_time(f[i])) {
+                        timed |= 1ull << r;
+                    } else {
+                        v = fin_kind(f[i]) == FIN_SOLID_P && G.wcode ? resolve_solid_p(f[i], code[i], G)
+                                                                       : resolve_boxed(f[i], G);
+                        if (v < 0) {
+                            ++bad;
+                            v = INT_MAX;
+                        }
+                    }
+                }
+                key[r] = v;
+            }
         }
         // Wall crossing times: listed in LDS and resolved one per lane, so the
         // wall arithmetic and its record loads run once per 64 such rays.
-        if (__ballot(timed != 0u)) {
+        if (__ballot(timed != 0ull)) {
             int32_t *list = pos, *res = ukey;
             int n = 0;
 #pragma unroll
             for (int r = 0; r < EPL; ++r) {
-                const bool w = (timed >> r) & 1u;
+                const bool w = (timed >> r) & 1ull;
                 const uint64_t m = __ballot(w);
                 if (w) list[n + __popcll(m & below)] = lane + 64 * r;
                 n += __popcll(m);
@@ -1452,7 +1472,7 @@ __global__ __launch_bounds__(COUNT_THREADS) __attribute__((amdgpu_waves_per_eu(4
             n = 0;
 #pragma unroll
             for (int r = 0; r < EPL; ++r) {
-                const bool w = (timed >> r) & 1u;
+                const bool w = (timed >> r) & 1ull;
                 const uint64_t m = __ballot(w);
                 if (w) {
                     int32_t v = res[n + __popcll(m & below)];
@@ -1515,22 +1535,53 @@ __global__ __launch_bounds__(COUNT_THREADS) __attribute__((amdgpu_waves_per_eu(4
     }
 }
 
+// One-seed calls (the common case) hold the sort at four waves per SIMD;
+// several seeds keep the compiler's register count.
+template <int EPL>
+__global__ __launch_bounds__(COUNT_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_count_wave(
+    const uint2 *fin, const WallRec *walls, GeoView G, int64_t n_p, int R, int S, int32_t *uniq_recv,
+    int32_t *uniq_cnt, int32_t *n_unique, unsigned long long *failed)
+{
+    count_wave_body<EPL, true>(fin, walls, G, n_p, R, S, uniq_recv, uniq_cnt, n_unique, failed);
+}
+
+// 32 keys per lane hold about 200 VGPRs (two waves per SIMD) either way.
+template <int EPL>
+__global__ __launch_bounds__(COUNT_THREADS) void k_count_wave_wide(const uint2 *fin, const WallRec *walls, GeoView G,
+                                                                   int64_t n_p, int R, int S, int32_t *uniq_recv,
+                                                                   int32_t *uniq_cnt, int32_t *n_unique,
+                                                                   unsigned long long *failed)
+{
+    count_wave_body<EPL, true>(fin, walls, G, n_p, R, S, uniq_recv, uniq_cnt, n_unique, failed);
+}
+
+template <int EPL>
+__global__ __launch_bounds__(COUNT_THREADS) void k_count_wave_seeds(const uint2 *fin, const WallRec *walls, GeoView G,
+                                                                    int64_t n_p, int R, int S, int32_t *uniq_recv,
+                                                                    int32_t *uniq_cnt, int32_t *n_unique,
+                                                                    unsigned long long *failed)
+{
+    count_wave_body<EPL, false>(fin, walls, G, n_p, R, S, uniq_recv, uniq_cnt, n_unique, failed);
+}
+
 // The k_count_wave instance for EPL keys per lane and S seeds per call.
 using CountWaveKernel = void (*)(const uint2 *, const WallRec *, GeoView, int64_t, int, int, int32_t *, int32_t *,
                                  int32_t *, unsigned long long *);
 CountWaveKernel count_wave_kernel(int epl, int S)
 {
     if (S == 1)
-        return epl == 1 ? k_count_wave<1, true>
-             : epl == 2 ? k_count_wave<2, true>
-             : epl == 4 ? k_count_wave<4, true>
-             : epl == 8 ? k_count_wave<8, true>
-                        : k_count_wave<16, true>;
-    return epl == 1 ? k_count_wave<1, false>
-         : epl == 2 ? k_count_wave<2, false>
-         : epl == 4 ? k_count_wave<4, false>
-         : epl == 8 ? k_count_wave<8, false>
-                    : k_count_wave<16, false>;
+        return epl == 1 ? k_count_wave<1>
+             : epl == 2 ? k_count_wave<2>
+             : epl == 4 ? k_count_wave<4>
+             : epl == 8 ? k_count_wave<8>
+             : epl == 16 ? k_count_wave<16>
+                        : k_count_wave_wide<32>;
+    return epl == 1 ? k_count_wave_seeds<1>
+         : epl == 2 ? k_count_wave_seeds<2>
+         : epl == 4 ? k_count_wave_seeds<4>
+         : epl == 8 ? k_count_wave_seeds<8>
+         : epl == 16 ? k_count_wave_seeds<16>
+                    : k_count_wave_seeds<32>;
 }
 
 // Compaction by output ranges, for a pass whose edge total is known: block
@@ -1988,7 +2039,7 @@ int trace_pass(const Geometry &g, Workspace &W, int64_t p0, int64_t n_p, int R, 
         return rc;
     PRH_HIP(hipEventRecord(W.ev[2], st));
     const unsigned cgrid = (unsigned)(n_p < 65536 ? n_p : 65536);
-    if (SRp2 <= 1024) {
+    if (SRp2 <= COUNT_WAVE_MAX_KEYS) {
         // one wavefront per patch, keys in registers
         const int epl = SRp2 <= 64 ? 1 : SRp2 / 64;
         const int64_t blocks = (n_p + COUNT_THREADS / 64 - 1) / (COUNT_THREADS / 64);
@@ -2468,7 +2519,7 @@ int prh_trace_counts_into(const prh_geometry *geo, int64_t p0, int64_t p1, int32
             (rc = launch_trace(g, dda, walls, total, fin, ctr, ctr + 2, st)))
             return rc;
         const int64_t blocks = (n_p + COUNT_THREADS / 64 - 1) / (COUNT_THREADS / 64);
-        if (SRp2 <= 1024) {
+        if (SRp2 <= COUNT_WAVE_MAX_KEYS) {
             const int epl = SRp2 <= 64 ? 1 : SRp2 / 64;
             auto kern = count_wave_kernel(epl, S);
             kern<<<(unsigned)std::min<int64_t>(blocks, 65536), COUNT_THREADS, 0, st>>>(
@@ -2631,7 +2682,7 @@ void count_launch(const prh_trace_batch *b, const BatchBuffers &B, const BatchPi
     const WallRec *walls = B.walls + P.ray_off;
     int32_t *ur = B.uniq_recv + P.ray_off, *uc = B.uniq_cnt + P.ray_off, *nu = B.n_unique + P.patch_off;
     unsigned long long *failed = b->counters + b->n_seg;
-    if (SRp2 <= 1024) {
+    if (SRp2 <= COUNT_WAVE_MAX_KEYS) {
         const int epl = SRp2 <= 64 ? 1 : SRp2 / 64;
         const int64_t blocks = (np + COUNT_THREADS / 64 - 1) / (COUNT_THREADS / 64);
         const unsigned wgrid = (unsigned)(blocks < 65536 ? blocks : 65536);

@@ -222,7 +222,7 @@ def test_single_seed_api_equals_pooled_singles(vf):
         assert np.array_equal(a, c) and np.array_equal(b, c)
 
 
-@pytest.mark.parametrize("R", [1, 7, 64, 1000, 4096])
+@pytest.mark.parametrize("R", [1, 7, 64, 1000, 2000, 4096])
 def test_partial_ranges_and_ray_counts(vf, R):
     phases = _small_random(4, (16, 9, 13), 0.2)
     P, Q = _both(vf, ("small4", "x"), phases)
