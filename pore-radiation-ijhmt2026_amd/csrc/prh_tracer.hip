@@ -1032,6 +1032,25 @@ __device__ __forceinline__ void stage_records(DdaRec *slice, const DdaRec *src, 
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the records are in LDS
 }
 
+// Final states leave through LDS: each wavefront holds two buffers of
+// LEAN_STAGE entries, one for the staged block of start records that refills
+// take from and one for the block before it.  A ray that ends writes its
+// final state into its block's buffer (or, when its block is older than
+// both, straight to memory); when a new block is staged the older buffer is
+// written out, its finished entries by one store instruction of 512 B in
+// whole lines, and becomes the new block's buffer.
+constexpr uint32_t FIN_EMPTY = 0xFFFFFFFFu;  // high word of an empty entry (no final state has it)
+
+__device__ __forceinline__ void flush_finals(uint2 *buf, uint32_t first, uint32_t n, uint2 *fin, uint32_t lane)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const uint2 v = buf[lane];
+    if (lane < n && v.y != FIN_EMPTY) fin[first + lane] = v;
+    buf[lane] = make_uint2(0u, FIN_EMPTY);
+}
+
 // trace_body with the lean step: the same queue and final-state order, in one
 // loop whose refill part runs only when REFILL_MIN lanes are idle (or none is
 // active), so the stepping is not a loop of its own.  Idle lanes take the
@@ -1041,13 +1060,17 @@ constexpr int LEAN_STEPS = 4;  // steps between the wavefront's refill checks
 template <int REFILL_MIN, bool POW2>
 __device__ __forceinline__ void lean_body(const GeoView &G, uint32_t base, uint32_t total, uint32_t grain,
                                           unsigned long long *queue, const DdaRec *dda, uint2 *fin,
-                                          unsigned long long *reads_out, DdaRec *slice)
+                                          unsigned long long *reads_out, DdaRec *slice, uint2 *fbuf)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
     uint32_t cursor = 0, w_end = 0;  // the wavefront's grain [cursor, w_end), beyond the staged records
     bool more = true;                // the queue may still hold records
     uint32_t s_first = 0, s_n = 0, s_taken = 0;  // staged records: [s_first, s_first + s_n), s_taken handed out
+    uint32_t p_first = 0, p_n = 0;               // the block staged before them
+    uint32_t cur = 0;                            // buffer of the staged block (fbuf + LEAN_STAGE * cur)
+    fbuf[lane] = make_uint2(0u, FIN_EMPTY);
+    fbuf[LEAN_STAGE + lane] = make_uint2(0u, FIN_EMPTY);
 
     LeanRay R;
     R.xd = R.yd = R.zd = 0u;
@@ -1068,7 +1091,14 @@ __device__ __forceinline__ void lean_body(const GeoView &G, uint32_t base, uint3
             if (!active && pending) {
                 const bool wall = lean_at_wall(R, G), capped = lean_capped(R);
                 reads += lean_reads(R, wall, capped);
-                fin[g_cur] = lean_final(R, wall, capped);
+                const uint2 v = lean_final(R, wall, capped);
+                const uint32_t dc = g_cur - s_first, dp = g_cur - p_first;
+                if (dc < s_n)
+                    fbuf[LEAN_STAGE * cur + dc] = v;
+                else if (dp < p_n)
+                    fbuf[LEAN_STAGE * (cur ^ 1u) + dp] = v;
+                else
+                    fin[g_cur] = v;
                 pending = false;
             }
             for (int round = 0; round < 2; ++round) {
@@ -1090,6 +1120,11 @@ __device__ __forceinline__ void lean_body(const GeoView &G, uint32_t base, uint3
                     }
                     if (cursor < w_end) {
                         const uint32_t cnt = w_end - cursor < LEAN_STAGE ? w_end - cursor : LEAN_STAGE;
+                        // the older block's finals go out; its buffer takes the new block's
+                        flush_finals(fbuf + LEAN_STAGE * (cur ^ 1u), p_first, p_n, fin, lane);
+                        p_first = s_first;
+                        p_n = s_n;
+                        cur ^= 1u;
                         stage_records(slice, dda + cursor, cnt, lane);
                         s_first = cursor;
                         s_n = cnt;
@@ -1125,6 +1160,8 @@ __device__ __forceinline__ void lean_body(const GeoView &G, uint32_t base, uint3
         }
         n_active = __popcll(__ballot(active));
     } while (!finished);
+    flush_finals(fbuf + LEAN_STAGE * (cur ^ 1u), p_first, p_n, fin, lane);
+    flush_finals(fbuf + LEAN_STAGE * cur, s_first, s_n, fin, lane);
     reads = wave_sum_u64(reads);
     if (lane == 0 && reads) atomicAdd(reads_out, (unsigned long long)reads);
     if (lane == 0) {
@@ -1138,8 +1175,9 @@ __global__ __launch_bounds__(STREAM_THREADS) __attribute__((amdgpu_waves_per_eu(
 void k_trace_lean(TraceArgs a)
 {
     __shared__ DdaRec s_stage[STREAM_THREADS / 64][LEAN_STAGE];
+    __shared__ uint2 s_fin[STREAM_THREADS / 64][2 * LEAN_STAGE];
     lean_body<REFILL_MIN, POW2>(a.G, 0u, a.total, a.grain, a.queue, a.dda, a.fin, a.reads,
-                                s_stage[threadIdx.x >> 6]);
+                                s_stage[threadIdx.x >> 6], s_fin[threadIdx.x >> 6]);
 }
 
 template <int REFILL_MIN, bool POW2>
@@ -1147,10 +1185,11 @@ __global__ __launch_bounds__(STREAM_THREADS) __attribute__((amdgpu_waves_per_eu(
 void k_trace_batch_lean(const Segment *segs, int n_seg, const DdaRec *dda, const WallRec *walls, uint2 *fin)
 {
     __shared__ DdaRec s_stage[STREAM_THREADS / 64][LEAN_STAGE];
+    __shared__ uint2 s_fin[STREAM_THREADS / 64][2 * LEAN_STAGE];
     const int k = segment_of(segs, n_seg, blockIdx.x, true);
     const GeoView G = segs[k].gen.G;
     lean_body<REFILL_MIN, POW2>(G, segs[k].gen.offset, segs[k].gen.total, segs[k].grain, segs[k].queue, dda, fin,
-                                segs[k].reads, s_stage[threadIdx.x >> 6]);
+                                segs[k].reads, s_stage[threadIdx.x >> 6], s_fin[threadIdx.x >> 6]);
 }
 
 __global__ void k_zero_u64(unsigned long long *p, int n)

@@ -233,6 +233,17 @@ def test_partial_ranges_and_ray_counts(vf, R):
         assert all(np.array_equal(x, y) for x, y in zip(got, (o, r, c))), (p0, p1)
 
 
+@pytest.mark.parametrize("R,seeds", [(1000, [11, 12]), (500, [1, 2, 3, 4])])
+def test_pooled_seeds_up_to_2048_rays_per_patch_match_oracle(vf, R, seeds):
+    # 1025..2048 rays per patch and call: one wavefront per patch, 32 keys per lane
+    phases = _small_random(6, (12, 10, 11), 0.2)
+    P, Q = _both(vf, ("small6", "x"), phases)
+    got = vf.first_hit_counts_pooled(P, R, seeds)
+    ref = O.pool_counts([O.first_hit_counts(Q, R, s) for s in seeds], Q.n_patches)
+    assert _equal_tables(got, ref)
+    assert int(got[2].sum()) == P.n_patches * R * len(seeds)
+
+
 def test_counts_are_deterministic(vf):
     phases = _small_random(5, (20, 16, 16), 0.12)
     P, _ = _both(vf, ("small5", "x"), phases)
