@@ -926,12 +926,12 @@ __device__ __forceinline__ void lean_load(LeanRay &R, const DdaRec *rec, const G
 }
 
 // One iteration of view_factors.py:225-255 on the walled bitset; returns
-// whether the ray goes on.  ODD: the step's parity within the ray.  The crossing times are compared before the
-// cell's bitset word is tested (they do not depend on it), so they overlap
-// the word's load; the crossing is taken only by the lanes whose cell is
-// empty.  The word of the next cell's block is requested right after the
-// crossing and first tested by the next step, so its latency overlaps that
-// step's comparisons.
+// whether the ray goes on.  ODD: the step's parity within the ray.  The
+// crossing times are compared and advanced before the cell's bitset word is
+// tested (they do not depend on it), so that work overlaps the word's load;
+// the cell index moves only for the lanes whose cell is empty.  The word of
+// the next cell's block is requested right after the crossing and first
+// tested by the next step, so its latency overlaps that step's FP64 work.
 template <bool POW2, bool ODD>
 __device__ __forceinline__ bool lean_step(LeanRay &R, const GeoView &G)
 {
@@ -940,22 +940,28 @@ __device__ __forceinline__ bool lean_step(LeanRay &R, const GeoView &G)
     const bool cx = fabs(R.tx - nt) <= TIE_TOLERANCE;
     const bool cy = fabs(R.ty - nt) <= TIE_TOLERANCE;
     const bool cz = fabs(R.tz - nt) <= TIE_TOLERANCE;
-    __builtin_amdgcn_sched_barrier(0);
-    // the cell's bit, shifted to bit 63: a solid cell or a wall ends the ray here
-    if ((int64_t)(R.blk << (R.P & 63u)) < 0) return false;
     // t += dd for a crossed axis (the reference's addition); for the others
-    // t plus a subnormal, which rounds back to t
+    // t plus a subnormal, which rounds back to t.  Made before the cell's
+    // bit is known: a ray that stops here never reads its crossing times
+    // again (its final state is formed from the cell, the last crossing and
+    // the step's minimum), so this work overlaps the wait for the block word
+    // (the compiler would otherwise sink it behind the test).
     R.incx = __hiloint2double(cx ? (int)R.hix : 0, __double2loint(R.incx));
     R.incy = __hiloint2double(cy ? (int)R.hiy : 0, __double2loint(R.incy));
     R.incz = __hiloint2double(cz ? (int)R.hiz : 0, __double2loint(R.incz));
     R.tx = R.tx + R.incx;
     R.ty = R.ty + R.incy;
     R.tz = R.tz + R.incz;
+    const uint32_t step_z = cz ? R.sz : 0u;
+    asm volatile("" : "+v"(R.tx), "+v"(R.ty), "+v"(R.tz), "+v"(R.incx), "+v"(R.incy), "+v"(R.incz));
+    __builtin_amdgcn_sched_barrier(0);
+    // the cell's bit, shifted to bit 63: a solid cell or a wall ends the ray here
+    if ((int64_t)(R.blk << (R.P & 63u)) < 0) return false;
     R.last_x = cx ? R.sx : 0u;
     R.last_y = cy ? R.sy : 0u;
     R.xd = (R.xd - R.last_x) & G.wl.mask[0];
     R.yd = (R.yd - R.last_y) & G.wl.mask[1];
-    R.zd = (R.zd - (cz ? R.sz : 0u)) & G.wl.mask[2];
+    R.zd = (R.zd - step_z) & G.wl.mask[2];
     if (!POW2) {
         R.yd = R.yd == R.from_y ? R.to_y : R.yd;
         R.zd = R.zd == R.from_z ? R.to_z : R.zd;
