@@ -76,8 +76,8 @@ __device__ inline u128 mul128(u128 a, u128 b)
 __device__ inline u128 add128(u128 a, u128 b)
 {
     u128 r;
-    r.lo = a.lo + b.lo;
-    r.hi = a.hi + b.hi + (r.lo < a.lo ? 1u : 0u);
+    const bool carry = __builtin_add_overflow(a.lo, b.lo, &r.lo);
+    r.hi = a.hi + b.hi + (carry ? 1u : 0u);
     return r;
 }
 
