@@ -1061,7 +1061,7 @@ __device__ __forceinline__ void flush_finals(uint2 *buf, uint32_t first, uint32_
 // loop whose refill part runs only when REFILL_MIN lanes are idle (or none is
 // active), so the stepping is not a loop of its own.  Idle lanes take the
 // staged records in lane order; the slice is refilled when it runs out.
-constexpr int LEAN_STEPS = 4;  // steps between the wavefront's refill checks
+constexpr int LEAN_STEPS = 6;  // steps between the wavefront's refill checks
 
 template <int REFILL_MIN, bool POW2>
 __device__ __forceinline__ void lean_body(const GeoView &G, uint32_t base, uint32_t total, uint32_t grain,
@@ -1099,12 +1099,16 @@ __device__ __forceinline__ void lean_body(const GeoView &G, uint32_t base, uint3
                 reads += lean_reads(R, wall, capped);
                 const uint2 v = lean_final(R, wall, capped);
                 const uint32_t dc = g_cur - s_first, dp = g_cur - p_first;
-                if (dc < s_n)
-                    fbuf[LEAN_STAGE * cur + dc] = v;
-                else if (dp < p_n)
-                    fbuf[LEAN_STAGE * (cur ^ 1u) + dp] = v;
-                else
+                if (dc < s_n || dp < p_n) {
+                    fbuf[dc < s_n ? LEAN_STAGE * cur + dc : LEAN_STAGE * (cur ^ 1u) + dp] = v;
+                } else {
                     fin[g_cur] = v;
+                    // kept apart from the LDS store above (the empty asm ends
+                    // this branch): one store through a selected address would
+                    // be a flat store, which the next step's wait for its
+                    // block word would wait for too
+                    asm volatile("" ::: "memory");
+                }
                 pending = false;
             }
             for (int round = 0; round < 2; ++round) {
