@@ -13,14 +13,20 @@
 //                    32 B wall record per ray in HBM; fully converged waves,
 //                    each thread jumps its PCG64 stream once and then only
 //                    draws;
-//   k_trace_stream   FP64 DDA over the bit-packed occupancy grid; each wave
-//                    streams through its chunk of rays and refills idle lanes
-//                    in batches, so lanes do not wait for the longest ray.
-//                    A refill is four coalesced loads and a finished ray
-//                    writes an 8 B final state (cell and face, or wall cell);
-//   k_count_patches  per patch: receivers resolved from the final states
-//                    (cell code or wall maps), LDS sort -> unique counts;
-//   k_compact_edges  prefix-summed edge table (origin, receiver, count).
+//   k_trace_lean     FP64 DDA, one ray per lane, over the walled bitset in
+//                    4x4x4-block words (the current block's word held per
+//                    lane); wavefronts take 512-record grains from a queue,
+//                    stage 64 start records at a time in LDS and refill idle
+//                    lanes from them; final states (a solid cell's index and
+//                    face, or a wall crossing time) leave through LDS in
+//                    512 B stores.  k_trace_stream is the earlier variant over
+//                    the flat bitset with an LDS brick test;
+//   k_count_wave     one wavefront per patch: receivers resolved from the
+//                    final states, keys sorted in registers -> unique counts
+//                    (k_count_patches: the LDS sort for more keys);
+//   k_compact_ranges prefix-summed edge table (origin, receiver, count) in
+//                    1 KB output ranges (k_compact_edges where the total stays
+//                    on the device).
 #include <climits>
 #include <mutex>
 
