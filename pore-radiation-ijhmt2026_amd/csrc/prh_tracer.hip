@@ -1013,9 +1013,13 @@ __device__ __forceinline__ uint2 lean_final(const LeanRay &R, bool wall, bool ca
     const uint32_t axis = R.last_x != 0u ? 0u : (R.last_y != 0u ? 1u : 2u);
     const uint32_t step = R.last_x != 0u ? R.last_x : (R.last_y != 0u ? R.last_y : R.sz);
     const uint32_t face = 2u * axis + (step > 7u ? 1u : 0u);
-    // the crossings made; the last one was made by step crossings - 1
-    const uint32_t crossings = R.cap1 - R.left1;
-    const double time = (crossings & 1u) ? R.nt0 : R.nt1;
+    // the time of the last crossing: the two registers hold it and the
+    // smallest crossing time of the step that stopped the ray, which is
+    // larger (each step's smallest time exceeds the one before: every axis
+    // within the tie tolerance of it crosses and its time grows by its
+    // increment, the others are beyond the tolerance); a ray that stopped
+    // before any crossing has nt1 = 0
+    const double time = vmin_f64(R.nt0, R.nt1);
     const unsigned long long t = (unsigned long long)__double_as_longlong(time);
     const uint32_t x = wall ? (uint32_t)t : R.P;
     const uint32_t y = wall ? ((uint32_t)(t >> 32) | (R.xd == 0u ? 0u : 0x80000000u))
