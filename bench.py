@@ -58,17 +58,27 @@ def traversal_kernel() -> str:
     return "k_trace_lean" if current["occupancy"] == 3 else "k_trace_stream"
 
 
-def measured_traffic(kernel: str) -> float | None:
-    """HBM bytes per traversal launch at the default workload, from the committed counter summary."""
+def counters_profile(kernel: str) -> str:
+    return os.path.join(REPO, "profiles", f"{kernel}_foam128_counters.md")
+
+
+def profile_value(path: str, key: str) -> float | None:
+    """The value of a `key: value` line of a committed counter summary (None if absent)."""
 
     try:
-        with open(traffic_profile(kernel)) as f:
+        with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key + ":"):
                     return float(line.split(":")[1])
     except OSError:
         pass
     return None
+
+
+def measured_traffic(kernel: str) -> float | None:
+    """HBM bytes per traversal launch at the default workload, from the committed counter summary."""
+
+    return profile_value(traffic_profile(kernel), "traffic_bytes_per_launch")
 
 
 def log(msg: str) -> None:
@@ -984,6 +994,12 @@ def main() -> None:
             "traffic_source": os.path.relpath(traffic_profile(kernel_name), REPO)
             + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
             "kernel": kernel_name,
+            # the kernel's actual limit: VALU issue (SQ_INSTS_VALU x 4 cycles over
+            # the SIMDs' busy cycles) at the default workload, from the
+            # rocprofv3 --pmc pass summarised in valu_issue_source
+            "valu_issue_frac": profile_value(counters_profile(kernel_name), "valu_issue_frac") if R == 1000 else None,
+            "valu_issue_source": os.path.relpath(counters_profile(kernel_name), REPO)
+            + " (rocprofv3 --pmc SQ_INSTS_VALU, SQ_BUSY_CYCLES)",
             "lane_share": lanes,
             "kernel_ms": kernel * 1e3,
             "algorithmic_bytes_per_launch": algo_bytes,
